@@ -1,0 +1,117 @@
+"""ctypes binding of the C ABI declared in include/gpis_hip.h (libgpis_hip.so).
+
+The product path: there is no CPU fallback.  If the HIP library is missing or
+cannot be loaded, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libgpis_hip.so")
+
+GPIS_OK, GPIS_INVALID_ARGUMENT, GPIS_DEVICE_ERROR, GPIS_UNSUPPORTED, GPIS_OUT_OF_MEMORY = range(5)
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "DEVICE_ERROR", 3: "UNSUPPORTED", 4: "OUT_OF_MEMORY"}
+
+COV_SE, COV_PROC_NS = 0, 1
+MEAN_SPHERE, MEAN_BOX, MEAN_PLANE = 0, 1, 2
+CTX_GLOBAL, CTX_RENEWAL, CTX_RENEWAL_PLUS, CTX_NONE = 0, 1, 2, 3
+SCHEME_UNI, SCHEME_NEE, SCHEME_MIS = 0, 1, 2
+LS_CONST, LS_SINCOS = 0, 1
+EXIT, HIT, FAIL, INVALID = 0, 1, 2, 3
+FAIL_NONE, FAIL_OUTWARD, FAIL_ZERO_GRAD, FAIL_NONFINITE = 0, 1, 2, 3
+
+#: every symbol include/gpis_hip.h declares
+EXPORTS = (
+    "gpis_create", "gpis_destroy", "gpis_last_error", "gpis_set_domain", "gpis_impulse_table",
+    "gpis_eval_points", "gpis_trace_flights", "gpis_trace_paths", "gpis_get_counters",
+    "gpis_reset_counters", "gpis_set_launch",
+)
+
+
+class FieldDesc(C.Structure):
+    _fields_ = [
+        ("cov_kind", C.c_int32), ("sigma", C.c_float), ("length_scale", C.c_float),
+        ("kernel_scale", C.c_float), ("aniso", C.c_float * 3),
+        ("ns_multires", C.c_int32), ("ns_ls_field", C.c_int32), ("ns_ls_params", C.c_float * 4),
+        ("ns_ls_max", C.c_float), ("ns_var", C.c_float),
+        ("mean_kind", C.c_int32), ("mean_radius", C.c_float), ("mean_center", C.c_double * 3),
+        ("mean_half", C.c_double * 3), ("plane_normal", C.c_double * 3), ("plane_offset", C.c_double),
+        ("global_seed", C.c_uint32), ("impulse_density", C.c_float), ("single_realization", C.c_int32),
+        ("ctxt", C.c_int32), ("one_d", C.c_int32), ("scheme", C.c_int32), ("correlation_xy", C.c_int32),
+        ("step_size", C.c_float), ("min_step", C.c_uint32), ("max_bounce", C.c_int32),
+    ]
+
+
+_vp = C.c_void_p
+
+
+class RaySoA(C.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "ox", "oy", "oz", "dx", "dy", "dz", "near_t", "far_t", "jitter", "noise_seed", "normal_seed",
+        "flags", "prev_value", "prev_gx", "prev_gy", "prev_gz", "length_scale")]
+
+
+class ReplySoA(C.Structure):
+    _fields_ = [(n, _vp) for n in ("t", "value", "gx", "gy", "gz", "outcome", "evals")]
+
+
+class PathReply(C.Structure):
+    _fields_ = [(n, _vp) for n in ("t0", "gx0", "gy0", "gz0", "outcome0", "flights", "hits", "evals")]
+
+
+class Counters(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "flights", "evals", "steps", "hits", "exits", "fails", "invalid", "kernel_launches")]
+
+
+_lib = None
+
+
+class GpisError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libgpis_hip.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GpisError(f"HIP extension missing: {LIB_PATH}; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    st = C.c_int
+    L.gpis_create.restype = st
+    L.gpis_create.argtypes = [C.POINTER(FieldDesc), C.POINTER(_vp)]
+    L.gpis_destroy.restype = st
+    L.gpis_destroy.argtypes = [_vp]
+    L.gpis_last_error.restype = C.c_char_p
+    L.gpis_last_error.argtypes = [_vp]
+    L.gpis_set_domain.restype = st
+    L.gpis_set_domain.argtypes = [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32]
+    L.gpis_impulse_table.restype = st
+    L.gpis_impulse_table.argtypes = [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_uint32, _vp, _vp]
+    L.gpis_eval_points.restype = st
+    L.gpis_eval_points.argtypes = [_vp, _vp, _vp, _vp, C.c_int64, C.c_uint32, _vp, _vp, _vp, _vp, _vp]
+    L.gpis_trace_flights.restype = st
+    L.gpis_trace_flights.argtypes = [_vp, C.POINTER(RaySoA), C.c_int64, C.POINTER(ReplySoA), _vp]
+    L.gpis_trace_paths.restype = st
+    L.gpis_trace_paths.argtypes = [_vp, C.POINTER(RaySoA), C.c_int64, C.c_int32, C.c_uint32, C.c_float,
+                                   C.POINTER(PathReply), _vp]
+    L.gpis_get_counters.restype = st
+    L.gpis_get_counters.argtypes = [_vp, C.POINTER(Counters)]
+    L.gpis_reset_counters.restype = st
+    L.gpis_reset_counters.argtypes = [_vp]
+    L.gpis_set_launch.restype = st
+    L.gpis_set_launch.argtypes = [_vp, C.c_int32, C.c_int32]
+    _lib = L
+    return L
+
+
+def check(status: int, ctx=None, what: str = "") -> None:
+    if status != GPIS_OK:
+        msg = ""
+        if ctx:
+            msg = lib().gpis_last_error(ctx).decode(errors="replace")
+        raise GpisError(f"{what}: {STATUS_NAMES.get(status, status)} {msg}".strip())

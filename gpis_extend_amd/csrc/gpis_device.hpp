@@ -1,0 +1,121 @@
+// gpis_device.hpp — device-side building blocks of the gfx950 GPIS flight engine.
+//
+// Integer parts (hash, PCG, unit float, Bernoulli, next3D order) are bit-exact
+// with the reference; the noise arithmetic is FP32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gpis {
+
+// ---------------------------------------------------------------- leaves --
+// xxhash32 on 1, 2 and 4 words: src/core/math/MathUtil.hpp:179-224.
+#define GPIS_P2 2246822519U
+#define GPIS_P3 3266489917U
+#define GPIS_P4 668265263U
+#define GPIS_P5 374761393U
+__host__ __device__ __forceinline__ uint32_t rotl17(uint32_t h) { return (h << 17) | (h >> 15); }
+__host__ __device__ __forceinline__ uint32_t xx_final(uint32_t h) {
+    h = GPIS_P2 * (h ^ (h >> 15));
+    h = GPIS_P3 * (h ^ (h >> 13));
+    return h ^ (h >> 16);
+}
+__host__ __device__ __forceinline__ uint32_t xxh1(uint32_t p) { return xx_final(GPIS_P4 * rotl17(p + GPIS_P5)); }
+__host__ __device__ __forceinline__ uint32_t xxh2(uint32_t x, uint32_t y) {
+    return xx_final(GPIS_P4 * rotl17(y + GPIS_P5 + x * GPIS_P3));
+}
+__host__ __device__ __forceinline__ uint32_t xxh4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) {
+    uint32_t h = GPIS_P4 * rotl17(w + GPIS_P5 + x * GPIS_P3);
+    h = GPIS_P4 * rotl17(h + y * GPIS_P3);
+    h = GPIS_P4 * rotl17(h + z * GPIS_P3);
+    return xx_final(h);
+}
+
+// PCG32 XSH-RR with increment 1 (UniformSampler.hpp:41-48).  64-bit state
+// arithmetic is done with the 32x32->64 mad the ISA provides.
+struct Pcg {
+    uint64_t s;
+    __host__ __device__ __forceinline__ uint32_t next() {
+        uint64_t old = s;
+        s = old * 6364136223846793005ULL + 1ULL;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+    // set_state(h): state = h, then next2D() discards two draws (UniformSampler.hpp:71-75).
+    __host__ __device__ __forceinline__ void set(uint64_t st) { s = st; next(); next(); }
+};
+// BitManip::normalizedUint (BitManip.hpp:47-50).
+__host__ __device__ __forceinline__ float u01(uint32_t i) {
+    union { uint32_t u; float f; } c;
+    c.u = (i >> 9u) | 0x3F800000u;
+    return c.f - 1.0f;
+}
+
+// ------------------------------------------------------------ parameters --
+struct FieldParams {
+    // covariance
+    int kind;            // 0 SE, 1 procedural nonstationary
+    int multires;        // _multiResolutionGrid
+    float sigma;         // covariance sigma
+    float amp_const;     // amplitude when it is spatially constant
+    float l_aniso[3], l_aniso_inv[3];
+    float l_max;         // max(l_aniso) (splattingKernelRadius)
+    float kernel_scale;  // _kernelScale (3)
+    int ls_field; float ls_params[4]; float ls_max; float ns_var;
+    int N; float Nf;
+    int seed_add_single;      // floor(log(kss)/log(2.5)) for the single-level grid
+    int level_seed_add[64];   // the same for multi-resolution level L, indexed L + 32
+    // single-level world-space constants (stationary SE)
+    float R, invR;
+    float c2[3];         // R^2 * m_i : exponent factor per axis in grid units
+    float gfac[3];       // -2 R m_i : world-gradient factor per axis
+    float nf_inv;        // 1 / sqrt(variance)
+    // mean
+    int mean_kind; float mc[3]; float mr; float mh[3]; float pn[3]; float poff;
+    // medium
+    uint32_t global_seed; int single; int ctxt; int activate; int one_d; int scheme1d; int corr_xy;
+    float step_size; float min_step_f; int max_bounce;
+    // precomputed impulse table (single realization)
+    const float4 *table; int tlo[3]; int tdim[3]; uint32_t tseed; int has_table;
+};
+
+struct Eval4 { float v, gx, gy, gz; };
+
+// ------------------------------------------------------------------ means --
+// SphericalMean (GPFunctions.hpp:952-996); box SDF and plane are defined in code.
+__device__ __forceinline__ float mean_value(const FieldParams &P, float x, float y, float z, float &gx, float &gy,
+                                            float &gz) {
+    if (P.mean_kind == 1) {
+        float px = x - P.mc[0], py = y - P.mc[1], pz = z - P.mc[2];
+        float qx = fabsf(px) - P.mh[0], qy = fabsf(py) - P.mh[1], qz = fabsf(pz) - P.mh[2];
+        float ox = fmaxf(qx, 0.f), oy = fmaxf(qy, 0.f), oz = fmaxf(qz, 0.f);
+        float o2 = ox * ox + oy * oy + oz * oz;
+        float mx = fmaxf(qx, fmaxf(qy, qz));
+        if (o2 > 0.f) {
+            float inv = rsqrtf(o2);
+            gx = copysignf(ox * inv, px); gy = copysignf(oy * inv, py); gz = copysignf(oz * inv, pz);
+        } else {
+            gx = gy = gz = 0.f;
+            int imax = (qy > qx) ? 1 : 0;
+            float qm = imax ? qy : qx;
+            if (qz > qm) imax = 2;
+            if (imax == 0) gx = px < 0.f ? -1.f : 1.f;
+            else if (imax == 1) gy = py < 0.f ? -1.f : 1.f;
+            else gz = pz < 0.f ? -1.f : 1.f;
+        }
+        return sqrtf(o2) + fminf(mx, 0.f);
+    }
+    if (P.mean_kind == 2) {
+        gx = P.pn[0]; gy = P.pn[1]; gz = P.pn[2];
+        return P.pn[0] * x + P.pn[1] * y + P.pn[2] * z - P.poff;
+    }
+    float px = x - P.mc[0], py = y - P.mc[1], pz = z - P.mc[2];
+    float r = sqrtf(px * px + py * py + pz * pz);
+    float inv = 1.0f / r;
+    gx = px * inv; gy = py * inv; gz = pz * inv;
+    return r - P.mr;
+}
+
+}  // namespace gpis

@@ -1,0 +1,210 @@
+// gpis_field.hpp — FP32 field evaluation on gfx950: sparse-convolution noise value
+// and world gradient summed over the 27 neighbour cells, multi-resolution blend,
+// and Renewal/Renewal+ conditioning splats.
+//
+// Reference behaviour restated (SparseConvolutionNoise.cpp):
+//   noise3D / cell3D                      :419-452
+//   evaluateNoise3DNormalized             :308-317
+//   evaluateNoise3D (world space)         :142-153
+//   kernelScaleLevelRatio                 :101-111
+//   evaluateValue / evaluateGradient      :114-140
+// SE kernels and normalisation: GPFunctions.cpp:555-580, 706-843.
+#pragma once
+
+#include "gpis_device.hpp"
+
+namespace gpis {
+
+// Per-flight realization state: the seed and the conditioning coefficients.
+struct RealState {
+    uint32_t seed;
+    int cond;            // conditioning splats active (non-zero coefficients)
+    float ox, oy, oz;    // conditioning origin (coeff_3D.ray_origin)
+    float a;             // coeff_3D.value_scale
+    float bx, by, bz;    // coeff_3D.gradient_scale
+};
+
+// One impulse-query pair in grid units: local point (lx,ly,lz), impulse (ix,iy,iz,w).
+// The support test |to|^2 < 1 is cell3D's (SparseConvolutionNoise.cpp:444-448);
+// the splat exp(-sum_i c_i to_i^2) is splattingKernel3DVal in grid units.
+__device__ __forceinline__ void pair3(float lx, float ly, float lz, float4 im, float c0, float c1, float c2,
+                                      float &v, float &ax, float &ay, float &az) {
+    float tx = lx - im.x, ty = ly - im.y, tz = lz - im.z;
+    float r2 = tx * tx + ty * ty + tz * tz;
+    float e = __expf(-(c0 * (tx * tx) + c1 * (ty * ty) + c2 * (tz * tz)));
+    float we = (r2 < 1.0f) ? im.w * e : 0.0f;
+    v += we;
+    ax += we * tx;
+    ay += we * ty;
+    az += we * tz;
+}
+
+// Impulses of one cell from the generator (cell3D's RNG sequence, bit-exact):
+// set_state(xxhash32(k, j, i, seed) + 1), then per impulse z, y, x, weight.
+template <int NI>
+__device__ __forceinline__ void cell_generated(int cx, int cy, int cz, int n, uint32_t seed, float lx, float ly,
+                                               float lz, float c0, float c1, float c2, float &v, float &ax, float &ay,
+                                               float &az) {
+    Pcg g;
+    g.set((uint64_t)(uint32_t)(xxh4((uint32_t)cz, (uint32_t)cy, (uint32_t)cx, seed) + 1u));
+    const int N = NI > 0 ? NI : n;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        float4 im;
+        im.z = u01(g.next());
+        im.y = u01(g.next());
+        im.x = u01(g.next());
+        im.w = u01(g.next()) < 0.5f ? -1.0f : 1.0f;
+        pair3(lx, ly, lz, im, c0, c1, c2, v, ax, ay, az);
+    }
+}
+
+// Raw 27-cell sum at point p for one grid level: returns (sum w*h, sum w*h*to).
+template <int NI>
+__device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, float py, float pz, uint32_t seed, float R,
+                                             float c0, float c1, float c2) {
+    float gx = px / R, gy = py / R, gz = pz / R;
+    float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+    float rx = gx - fx, ry = gy - fy, rz = gz - fz;
+    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    float v = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
+    const int N = NI > 0 ? NI : P.N;
+    bool tab = P.has_table && seed == P.tseed;
+    for (int dx = -1; dx <= 1; ++dx) {
+        for (int dy = -1; dy <= 1; ++dy) {
+            for (int dz = -1; dz <= 1; ++dz) {
+                int cx = ix + dx, cy = iy + dy, cz = iz + dz;
+                float lx = rx - (float)dx, ly = ry - (float)dy, lz = rz - (float)dz;
+                unsigned ux = (unsigned)(cx - P.tlo[0]), uy = (unsigned)(cy - P.tlo[1]), uz = (unsigned)(cz - P.tlo[2]);
+                if (tab && ux < (unsigned)P.tdim[0] && uy < (unsigned)P.tdim[1] && uz < (unsigned)P.tdim[2]) {
+                    const float4 *q = P.table + ((size_t)(uz * (unsigned)P.tdim[1] + uy) * (unsigned)P.tdim[0] + ux) * N;
+#pragma unroll
+                    for (int k = 0; k < N; ++k) pair3(lx, ly, lz, q[k], c0, c1, c2, v, ax, ay, az);
+                } else {
+                    cell_generated<NI>(cx, cy, cz, N, seed, lx, ly, lz, c0, c1, c2, v, ax, ay, az);
+                }
+            }
+        }
+    }
+    Eval4 r = {v, ax, ay, az};
+    return r;
+}
+
+// Grid constants of one level with spatial scale gs (multi-resolution level
+// scale, or 1 for a stationary kernel) and local scale ls:
+//   R = splattingKernelRadius(false, gs)         (GPFunctions.cpp:706-719)
+//   m_i = l_aniso_inv_i^2 / gs^2 / ls^2 * 0.5    (getInvCovMtx, :784-812)
+//   variance = N / R^3 * pi^1.5 * prod(l_aniso) * gs^3 * ls^3  (:751-764)
+struct Level {
+    float R, c0, c1, c2, g0, g1, g2, nf_inv;
+    uint32_t seed_add;
+};
+__device__ __forceinline__ Level make_level(const FieldParams &P, float gs, float ls, uint32_t seed_add) {
+    Level L;
+    L.R = P.kernel_scale * gs * P.l_max;
+    float s2 = gs * gs * ls * ls;
+    float m0 = P.l_aniso_inv[0] * P.l_aniso_inv[0] / s2 * 0.5f;
+    float m1 = P.l_aniso_inv[1] * P.l_aniso_inv[1] / s2 * 0.5f;
+    float m2 = P.l_aniso_inv[2] * P.l_aniso_inv[2] / s2 * 0.5f;
+    float R2 = L.R * L.R;
+    L.c0 = R2 * m0; L.c1 = R2 * m1; L.c2 = R2 * m2;
+    L.g0 = -2.f * L.R * m0; L.g1 = -2.f * L.R * m1; L.g2 = -2.f * L.R * m2;
+    float var = P.Nf / (L.R * L.R * L.R) * 5.568327996831708f * (P.l_aniso[0] * P.l_aniso[1] * P.l_aniso[2]) *
+                (gs * gs * gs) * (ls * ls * ls);
+    L.nf_inv = 1.0f / sqrtf(var);
+    L.seed_add = seed_add;
+    return L;
+}
+
+// Conditioning splats κ(p,o)·a + Σ b_i ∂_iκ(p,o) and their gradient for one level
+// (splattingKernel3D / splattingKernel3DGrad with isCov, GPFunctions.cpp:555-580, 829-843).
+__device__ __forceinline__ void cond_splat(const FieldParams &P, const RealState &rs, float px, float py, float pz,
+                                           float gs, float ls, Eval4 &n) {
+    float s2 = gs * gs * ls * ls;
+    float m0 = P.l_aniso_inv[0] * P.l_aniso_inv[0] / s2 * 0.25f;
+    float m1 = P.l_aniso_inv[1] * P.l_aniso_inv[1] / s2 * 0.25f;
+    float m2 = P.l_aniso_inv[2] * P.l_aniso_inv[2] / s2 * 0.25f;
+    float ax = px - rs.ox, ay = py - rs.oy, az = pz - rs.oz;
+    float k = __expf(-(ax * ax * m0 + ay * ay * m1 + az * az * m2));
+    float d0 = -2.f * ax * m0, d1 = -2.f * ay * m1, d2 = -2.f * az * m2;
+    float bd = rs.bx * d0 + rs.by * d1 + rs.bz * d2;
+    n.v += (rs.a + bd) * k;
+    n.gx += (rs.a * d0 + bd * d0 - 2.f * m0 * rs.bx) * k;
+    n.gy += (rs.a * d1 + bd * d1 - 2.f * m1 * rs.by) * k;
+    n.gz += (rs.a * d2 + bd * d2 - 2.f * m2 * rs.bz) * k;
+}
+
+// Procedural length-scale field of the nonstationary covariance.
+__device__ __forceinline__ float ls_field(const FieldParams &P, float x, float y, float z) {
+    (void)y;
+    if (P.ls_field == 1)
+        return P.ls_params[0] + P.ls_params[1] * (0.5f + 0.5f * sinf(P.ls_params[2] * x) * cosf(P.ls_params[2] * z));
+    return P.ls_params[0];
+}
+
+// Normalised noise value and world gradient (evaluateNoise3D, world space).
+template <int NI>
+__device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealState &rs, 
+                                            float px, float py, float pz, bool cond) {
+    if (!P.multires) {
+        Eval4 r;
+        float ls = 1.0f, gs = 1.0f;
+        Level L;
+        if (P.kind == 0) {
+            L.R = P.R; L.c0 = P.c2[0]; L.c1 = P.c2[1]; L.c2 = P.c2[2];
+            L.g0 = P.gfac[0]; L.g1 = P.gfac[1]; L.g2 = P.gfac[2]; L.nf_inv = P.nf_inv; L.seed_add = 0;
+        } else {
+            // brute-force nonstationary: gs = ls_max, ls = ls(p)/ls_max
+            gs = P.ls_max;
+            ls = ls_field(P, px, py, pz) / P.ls_max;
+            L = make_level(P, gs, ls, (uint32_t)P.seed_add_single);
+            L.R = P.kernel_scale * gs * P.l_max;
+        }
+        Eval4 n = noise3d_raw<NI>(P, px, py, pz, rs.seed + L.seed_add, L.R, L.c0, L.c1, L.c2);
+        r.v = n.v * L.nf_inv;
+        r.gx = n.gx * L.g0 * L.nf_inv;
+        r.gy = n.gy * L.g1 * L.nf_inv;
+        r.gz = n.gz * L.g2 * L.nf_inv;
+        if (cond && rs.cond) cond_splat(P, rs, px, py, pz, gs, ls, r);
+        return r;
+    }
+    // multi-resolution: two adjacent levels of the base-2.5 ladder (kernelScaleLevelRatio)
+    float lat = ls_field(P, px, py, pz);
+    float lvl = floorf(__logf(lat) / 0.91629073187415506f);
+    int li = (int)lvl;
+    float slo = __powf(2.5f, lvl), shi = __powf(2.5f, lvl + 1.0f);
+    float fr = (lat - slo) / (shi - slo);
+    float den = rsqrtf(1.0f - 2.0f * fr + 2.0f * fr * fr);
+    float wlo = (1.0f - fr) * den, whi = fr * den;
+    int ilo = li + 32, ihi = li + 33;
+    ilo = ilo < 0 ? 0 : (ilo > 63 ? 63 : ilo);
+    ihi = ihi < 0 ? 0 : (ihi > 63 ? 63 : ihi);
+    Level A = make_level(P, slo, 1.0f, (uint32_t)P.level_seed_add[ilo]);
+    Level B = make_level(P, shi, 1.0f, (uint32_t)P.level_seed_add[ihi]);
+    Eval4 na = noise3d_raw<NI>(P, px, py, pz, rs.seed + A.seed_add, A.R, A.c0, A.c1, A.c2);
+    Eval4 nb = noise3d_raw<NI>(P, px, py, pz, rs.seed + B.seed_add, B.R, B.c0, B.c1, B.c2);
+    Eval4 ra = {na.v * A.nf_inv, na.gx * A.g0 * A.nf_inv, na.gy * A.g1 * A.nf_inv, na.gz * A.g2 * A.nf_inv};
+    Eval4 rb = {nb.v * B.nf_inv, nb.gx * B.g0 * B.nf_inv, nb.gy * B.g1 * B.nf_inv, nb.gz * B.g2 * B.nf_inv};
+    if (cond && rs.cond) {
+        cond_splat(P, rs, px, py, pz, slo, 1.0f, ra);
+        cond_splat(P, rs, px, py, pz, shi, 1.0f, rb);
+    }
+    Eval4 r = {wlo * ra.v + whi * rb.v, wlo * ra.gx + whi * rb.gx, wlo * ra.gy + whi * rb.gy, wlo * ra.gz + whi * rb.gz};
+    return r;
+}
+
+__device__ __forceinline__ float amplitude(const FieldParams &P) { return P.amp_const; }
+
+// f(p) = amplitude * noise + mean, and its world gradient (evaluateValue/evaluateGradient).
+template <int NI>
+__device__ __forceinline__ Eval4 eval_field(const FieldParams &P, const RealState &rs,
+                                            float px, float py, float pz, bool cond) {
+    Eval4 n = eval_noise<NI>(P, rs, px, py, pz, cond);
+    float mgx, mgy, mgz;
+    float m = mean_value(P, px, py, pz, mgx, mgy, mgz);
+    float a = amplitude(P);
+    Eval4 r = {a * n.v + m, a * n.gx + mgx, a * n.gy + mgy, a * n.gz + mgz};
+    return r;
+}
+
+}  // namespace gpis

@@ -1,0 +1,753 @@
+// gpis_hip.hip — gfx950 kernels and the C ABI (include/gpis_hip.h) of the
+// sparse-convolution GPIS flight engine.
+//
+// Flight semantics restated from the reference:
+//   prepareGpisSegment / finishGpisSegment   src/core/media/SparseConvolutionNoiseMedium.cpp:199-347
+//   beginDistance / resumeDistance           src/core/media/GaussianProcessMedium.cpp:236-369
+//   conditioning3D                           src/core/math/SparseConvolutionNoise.cpp:498-574
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/gpis_hip.h"
+#include "gpis_field.hpp"
+
+using namespace gpis;
+
+// ------------------------------------------------------------ contexts --
+struct gpis_ctx {
+    gpis_field_desc desc;
+    FieldParams P;
+    float4 *table = nullptr;
+    size_t table_cells = 0;
+    unsigned long long *counters = nullptr;  // device: flights evals steps hits exits fails invalid launches
+    unsigned int *work = nullptr;            // device work counter of the persistent kernels
+    int device = 0;
+    int num_cus = 0;
+    int waves_per_cu = 0;
+    int block_size = 256;
+    std::string err;
+};
+
+static gpis_status fail(gpis_ctx *c, gpis_status s, const char *msg) {
+    if (c) c->err = msg;
+    return s;
+}
+static gpis_status hip_fail(gpis_ctx *c, hipError_t e, const char *where) {
+    if (c) c->err = std::string(where) + ": " + hipGetErrorString(e);
+    return GPIS_DEVICE_ERROR;
+}
+#define HIP_TRY(c, expr)                                  \
+    do {                                                  \
+        hipError_t _e = (expr);                           \
+        if (_e != hipSuccess) return hip_fail(c, _e, #expr); \
+    } while (0)
+
+// Host restatement of floor(log(s)/log(2.5)) in float (SparseConvolutionNoise.cpp:309).
+static int host_level_index(float s) { return (int)floorf(logf(s) / logf(2.5f)); }
+
+static gpis_status build_params(gpis_ctx *c) {
+    const gpis_field_desc &d = c->desc;
+    FieldParams &P = c->P;
+    memset(&P, 0, sizeof P);
+    if (!(d.impulse_density >= 1.0f) || d.impulse_density > 64.0f) return fail(c, GPIS_INVALID_ARGUMENT, "impulse_density must be in [1, 64]");
+    if (!(d.length_scale > 0.0f) || !std::isfinite(d.length_scale)) return fail(c, GPIS_INVALID_ARGUMENT, "lengthScale must be positive");
+    if (!(d.step_size > 0.0f) || d.min_step == 0) return fail(c, GPIS_INVALID_ARGUMENT, "step_size and min_step must be positive");
+    if (d.cov_kind != GPIS_COV_SE && d.cov_kind != GPIS_COV_PROC_NS) return fail(c, GPIS_UNSUPPORTED, "covariance kind");
+    if (d.one_d) return fail(c, GPIS_UNSUPPORTED, "1D sampling is served by the 1D entry points");
+    P.kind = d.cov_kind;
+    P.multires = (d.cov_kind == GPIS_COV_PROC_NS) && d.ns_multires;
+    P.sigma = d.sigma;
+    // SquaredExponentialCovariance::fromJson (GPFunctions.cpp:657-685)
+    float l_conv = d.length_scale * sqrtf(2.f) / 2;
+    for (int i = 0; i < 3; ++i) {
+        P.l_aniso[i] = l_conv * d.aniso[i];
+        float inv = 1.0f / P.l_aniso[i];
+        P.l_aniso_inv[i] = std::isfinite(inv) ? inv : 0.0f;
+    }
+    P.l_max = fmaxf(fmaxf(P.l_aniso[0], P.l_aniso[1]), P.l_aniso[2]);
+    P.kernel_scale = d.kernel_scale;
+    P.ls_field = d.ns_ls_field;
+    for (int i = 0; i < 4; ++i) P.ls_params[i] = d.ns_ls_params[i];
+    P.ls_max = d.cov_kind == GPIS_COV_SE ? 1.0f : d.ns_ls_max;
+    P.ns_var = d.ns_var;
+    P.amp_const = d.cov_kind == GPIS_COV_SE ? d.sigma : (float)((double)d.ns_var * d.sigma);
+    P.N = (int)(unsigned)d.impulse_density;
+    P.Nf = (float)P.N;
+    // single-level grid (stationary SE: kss = 1)
+    float kss = d.cov_kind == GPIS_COV_SE ? 1.0f : P.ls_max;
+    P.seed_add_single = host_level_index(kss);
+    for (int L = -32; L < 32; ++L) P.level_seed_add[L + 32] = host_level_index(powf(2.5f, (float)L));
+    P.R = P.kernel_scale * 1.0f * P.l_max;
+    P.invR = 1.0f / P.R;
+    for (int i = 0; i < 3; ++i) {
+        float m = P.l_aniso_inv[i] * P.l_aniso_inv[i] * 0.5f;
+        P.c2[i] = P.R * P.R * m;
+        P.gfac[i] = -2.f * P.R * m;
+    }
+    double var = (double)(P.Nf / (P.R * P.R * P.R)) * pow(M_PI, 1.5) *
+                 (double)(P.l_aniso[0] * P.l_aniso[1] * P.l_aniso[2]);
+    P.nf_inv = (float)(1.0 / sqrt((double)(float)var));
+    P.mean_kind = d.mean_kind;
+    for (int i = 0; i < 3; ++i) {
+        P.mc[i] = (float)d.mean_center[i];
+        P.mh[i] = (float)d.mean_half[i];
+        P.pn[i] = (float)d.plane_normal[i];
+    }
+    P.mr = d.mean_radius;
+    P.poff = (float)d.plane_offset;
+    P.global_seed = d.global_seed;
+    P.single = d.single_realization;
+    P.ctxt = d.ctxt;
+    P.activate = !d.single_realization && (d.ctxt == GPIS_CTX_RENEWAL || d.ctxt == GPIS_CTX_RENEWAL_PLUS);
+    P.one_d = 0;
+    P.corr_xy = d.correlation_xy;
+    P.step_size = d.step_size;
+    P.min_step_f = (float)d.min_step;
+    P.max_bounce = d.max_bounce;
+    P.table = c->table;
+    P.has_table = 0;
+    return GPIS_OK;
+}
+
+// ------------------------------------------------------------- kernels --
+// Impulses of every cell of a box (bit-exact cell3D RNG). One thread per cell.
+__global__ void impulse_table_kernel(int lox, int loy, int loz, int dx, int dy, int dz, uint32_t seed, int n,
+                                     float4 *out) {
+    long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long total = (long long)dx * dy * dz;
+    if (cell >= total) return;
+    int x = (int)(cell % dx);
+    int y = (int)((cell / dx) % dy);
+    int z = (int)(cell / ((long long)dx * dy));
+    int cx = lox + x, cy = loy + y, cz = loz + z;
+    Pcg g;
+    g.set((uint64_t)(uint32_t)(xxh4((uint32_t)cz, (uint32_t)cy, (uint32_t)cx, seed) + 1u));
+    float4 *q = out + cell * n;
+    for (int k = 0; k < n; ++k) {
+        float4 im;
+        im.z = u01(g.next());
+        im.y = u01(g.next());
+        im.x = u01(g.next());
+        im.w = u01(g.next()) < 0.5f ? -1.0f : 1.0f;
+        q[k] = im;
+    }
+}
+
+template <int NI>
+__global__ __launch_bounds__(256) void eval_points_kernel(FieldParams P, const float *__restrict__ x,
+                                                          const float *__restrict__ y, const float *__restrict__ z,
+                                                          long long n, uint32_t seed, float *__restrict__ v,
+                                                          float *__restrict__ gx, float *__restrict__ gy,
+                                                          float *__restrict__ gz) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    RealState rs = {};
+    rs.seed = seed;
+    Eval4 e = eval_field<NI>(P, rs, x[i], y[i], z[i], false);
+    v[i] = e.v;
+    gx[i] = e.gx;
+    gy[i] = e.gy;
+    gz[i] = e.gz;
+}
+
+// Lane phases of the persistent flight state machine.  Every loop iteration a
+// live lane performs exactly one field evaluation at (qx,qy,qz).
+enum Phase : int {
+    PH_NEED = 0,   // wants a new ray
+    PH_COND = 1,   // unconditioned value+gradient at the origin (conditioning)
+    PH_F0 = 2,     // f(nearT)
+    PH_MARCH = 3,  // lattice step
+    PH_SECANT = 4, // secant candidate with 0.9 shrink
+    PH_FAR = 5,    // f(farT) on exit
+    PH_GRAD = 6,   // gradient at the endpoint
+    PH_DEAD = 7    // no more work
+};
+
+// Lattice steps after which a flight is abandoned and reported invalid.
+constexpr int kMaxSteps = 1 << 22;
+
+struct RayIn {
+    const float *ox, *oy, *oz, *dx, *dy, *dz, *near_t, *far_t, *jitter;
+    const uint32_t *noise_seed, *normal_seed, *flags;
+    const float *pv, *pgx, *pgy, *pgz;
+};
+struct ReplyOut {
+    float *t, *value, *gx, *gy, *gz;
+    uint32_t *outcome, *evals;
+};
+struct PathOut {
+    float *t0, *gx0, *gy0, *gz0;
+    uint32_t *outcome0, *flights, *hits, *evals;
+};
+
+__device__ __forceinline__ unsigned lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+__device__ __forceinline__ unsigned mask_rank(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// Exit distance of the ray o + t d from the box [-b, b]^3 (slab method).
+__device__ __forceinline__ float box_exit(float ox, float oy, float oz, float dx, float dy, float dz, float b) {
+    float tx = dx != 0.f ? fmaxf((b - ox) / dx, (-b - ox) / dx) : 3.4e38f;
+    float ty = dy != 0.f ? fmaxf((b - oy) / dy, (-b - oy) / dy) : 3.4e38f;
+    float tz = dz != 0.f ? fmaxf((b - oz) / dz, (-b - oz) / dz) : 3.4e38f;
+    return fmaxf(fminf(tx, fminf(ty, tz)), 0.f);
+}
+
+template <int NI, bool PATHS>
+__global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, ReplyOut out, PathOut pout,
+                                                     unsigned int n, unsigned int *work,
+                                                     unsigned long long *stats, int max_bounces,
+                                                     uint32_t bounce_seed, float box) {
+    const unsigned lane = lane_id();
+    int phase = PH_NEED;
+    unsigned ray = 0;
+    // flight state
+    float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, near_t = 0, far_t = 0, step = 0, t0 = 0;
+    float qx = 0, qy = 0, qz = 0;
+    int first = 1, sign0 = 1, k = 0;
+    float pf = 0.f;
+    double sa = 0, sb = 0, fac = 0, tp = 0, tt = 0;
+    float t_end = 0, val_end = 0;
+    int hit = 0;
+    float prev_v = 0, prev_gx = 0, prev_gy = 0, prev_gz = 0;
+    RealState rs = {};
+    unsigned evals_flight = 0;
+    // path state
+    int bounce = 0;
+    unsigned path_flights = 0, path_hits = 0, path_evals = 0;
+    // statistics
+    unsigned s_flights = 0, s_evals = 0, s_steps = 0, s_hits = 0, s_exits = 0, s_fails = 0, s_invalid = 0;
+
+    for (;;) {
+        // ---- refill: lanes that finished take the next rays (ballot + prefix sum)
+        unsigned long long need = __ballot(phase == PH_NEED);
+        if (need) {
+            unsigned cnt = (unsigned)__popcll(need);
+            unsigned leader = (unsigned)(__ffsll((long long)need) - 1);
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(work, cnt);
+            base = __shfl(base, leader);
+            if (phase == PH_NEED) {
+                unsigned r = base + mask_rank(need);
+                if (r >= n) {
+                    phase = PH_DEAD;
+                } else {
+                    ray = r;
+                    ox = in.ox[r]; oy = in.oy[r]; oz = in.oz[r];
+                    dx = in.dx[r]; dy = in.dy[r]; dz = in.dz[r];
+                    near_t = in.near_t[r]; far_t = in.far_t[r];
+                    float jit = in.jitter[r];
+                    rs.seed = in.noise_seed[r];
+                    rs.cond = 0;
+                    first = (in.flags[r] & 1u) ? 1 : 0;
+                    if (!first) { prev_v = in.pv[r]; prev_gx = in.pgx[r]; prev_gy = in.pgy[r]; prev_gz = in.pgz[r]; }
+                    bounce = 0; path_flights = 0; path_hits = 0; path_evals = 0;
+                    evals_flight = 0;
+                    // beginDistance (GaussianProcessMedium.cpp:244-265)
+                    if (!isfinite(far_t)) far_t = near_t + 2000.f;
+                    bool bad = !isfinite(ox) || !isfinite(oy) || !isfinite(oz) || !isfinite(dx) || !isfinite(dy) ||
+                               !isfinite(dz) || !isfinite(near_t) || !(jit >= 0.f && jit < 1.f);
+                    if (bad) {
+                        out.t[r] = far_t; out.value[r] = 0.f; out.gx[r] = out.gy[r] = out.gz[r] = 0.f;
+                        out.outcome[r] = GPIS_INVALID;
+                        if (out.evals) out.evals[r] = 0;
+                        if (PATHS) { pout.flights[r] = 0; pout.hits[r] = 0; pout.outcome0[r] = GPIS_INVALID; pout.t0[r] = far_t; }
+                        ++s_invalid;
+                        phase = PH_NEED;
+                    } else if (far_t <= near_t) {
+                        // exited without a flight: no RNG draw, no evaluation
+                        if (!PATHS) {
+                            out.t[r] = far_t; out.value[r] = first ? 0.f : prev_v;
+                            out.gx[r] = out.gy[r] = out.gz[r] = 0.f;
+                            out.outcome[r] = GPIS_EXIT;
+                            if (out.evals) out.evals[r] = 0;
+                        } else {
+                            pout.t0[r] = far_t; pout.outcome0[r] = GPIS_EXIT; pout.flights[r] = 0; pout.hits[r] = 0;
+                            pout.gx0[r] = pout.gy0[r] = pout.gz0[r] = 0.f;
+                            if (pout.evals) pout.evals[r] = 0;
+                        }
+                        ++s_exits;
+                        phase = PH_NEED;
+                    } else {
+                        // finishGpisSegment step rule (SparseConvolutionNoiseMedium.cpp:289-291)
+                        step = (far_t - near_t) / P.min_step_f;
+                        if (P.step_size < step) step = P.step_size;
+                        t0 = near_t + step * jit;
+                        if (P.activate && !first) {
+                            phase = PH_COND;
+                            qx = ox; qy = oy; qz = oz;
+                        } else {
+                            phase = PH_F0;
+                            qx = fmaf(near_t, dx, ox); qy = fmaf(near_t, dy, oy); qz = fmaf(near_t, dz, oz);
+                        }
+                    }
+                }
+            }
+        }
+        if (!__ballot(phase != PH_DEAD)) break;
+        if (phase == PH_NEED || phase == PH_DEAD) continue;
+
+        // ---- one evaluation per live lane
+        Eval4 e = eval_field<NI>(P, rs, qx, qy, qz, phase != PH_COND);
+        ++evals_flight;
+        float fv = e.v;
+
+        switch (phase) {
+        case PH_COND: {
+            // conditioning3D (SparseConvolutionNoise.cpp:498-574) in FP64: the value and
+            // gradient observation at one point decouple (the covariance gradient vanishes
+            // at zero lag), so the Schur complement is the diagonal second-derivative block.
+            float mgx, mgy, mgz;
+            double mean = mean_value(P, ox, oy, oz, mgx, mgy, mgz);
+            double amp = P.amp_const;
+            double cur_v = ((double)e.v - mean) / amp;
+            double a = ((double)prev_v - mean) / amp - cur_v;
+            rs.ox = ox; rs.oy = oy; rs.oz = oz;
+            rs.a = (float)a; rs.bx = rs.by = rs.bz = 0.f;
+            if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
+                double ngx = ((double)e.gx - mgx) / amp, ngy = ((double)e.gy - mgy) / amp, ngz = ((double)e.gz - mgz) / amp;
+                double del0 = ((double)prev_gx - mgx) / amp - ngx;
+                double del1 = ((double)prev_gy - mgy) / amp - ngy;
+                double del2 = ((double)prev_gz - mgz) / amp - ngz;
+                // H^-1 of the covariance second derivative: -1 / (2 m_i) with m_i = l_aniso_inv_i^2 / 4
+                double h0 = -2.0 / ((double)P.l_aniso_inv[0] * P.l_aniso_inv[0]);
+                double h1 = -2.0 / ((double)P.l_aniso_inv[1] * P.l_aniso_inv[1]);
+                double h2 = -2.0 / ((double)P.l_aniso_inv[2] * P.l_aniso_inv[2]);
+                double b0 = h0 * del0, b1 = h1 * del1, b2 = h2 * del2;
+                if (P.multires) {
+                    // kernelScaleLevelRatio at the origin: value /= wlo + whi, gradient /= wlo/slo^2 + whi/shi^2
+                    double lat = ls_field(P, ox, oy, oz);
+                    double lvl = floor(log(lat) / log(2.5));
+                    double slo = pow(2.5, lvl), shi = pow(2.5, lvl + 1.0);
+                    double fr = (lat - slo) / (shi - slo);
+                    double den = sqrt(1.0 - 2.0 * fr + 2.0 * fr * fr);
+                    double wlo = (1.0 - fr) / den, whi = fr / den;
+                    a /= (wlo + whi);
+                    double g = wlo / (slo * slo) + whi / (shi * shi);
+                    b0 /= g; b1 /= g; b2 /= g;
+                    rs.a = (float)a;
+                }
+                rs.bx = (float)b0; rs.by = (float)b1; rs.bz = (float)b2;
+            } else if (P.multires) {
+                double lat = ls_field(P, ox, oy, oz);
+                double lvl = floor(log(lat) / log(2.5));
+                double slo = pow(2.5, lvl), shi = pow(2.5, lvl + 1.0);
+                double fr = (lat - slo) / (shi - slo);
+                double den = sqrt(1.0 - 2.0 * fr + 2.0 * fr * fr);
+                rs.a = (float)(a / ((1.0 - fr) / den + fr / den));
+            }
+            rs.cond = 1;
+            phase = PH_F0;
+            qx = fmaf(near_t, dx, ox); qy = fmaf(near_t, dy, oy); qz = fmaf(near_t, dz, oz);
+            break;
+        }
+        case PH_F0: {
+            sign0 = fv < 0.f ? -1 : 1;
+            pf = fv;
+            k = 0;
+            if (t0 < far_t) {
+                phase = PH_MARCH;
+                qx = fmaf(t0, dx, ox); qy = fmaf(t0, dy, oy); qz = fmaf(t0, dz, oz);
+            } else {
+                phase = PH_FAR;
+                qx = fmaf(far_t, dx, ox); qy = fmaf(far_t, dy, oy); qz = fmaf(far_t, dz, oz);
+            }
+            break;
+        }
+        case PH_MARCH: {
+            ++k;
+            ++s_steps;
+            int sc = fv < 0.f ? -1 : 1;
+            float tk = fmaf((float)(k - 1), step, t0);
+            if (!first && k == 1) {
+                sign0 = sc;
+            } else if (sc != sign0) {
+                // secant with 0.9 shrink (SparseConvolutionNoiseMedium.cpp:309-329), in FP64
+                fac = (double)pf / ((double)pf - (double)fv);
+                sa = (double)tk - (double)step;
+                sb = (double)tk;
+                tp = sa * (1.0 - fac) + sb * fac;
+                tt = tp;
+                phase = PH_SECANT;
+                qx = (float)((double)ox + tt * (double)dx);
+                qy = (float)((double)oy + tt * (double)dy);
+                qz = (float)((double)oz + tt * (double)dz);
+                break;
+            }
+            pf = fv;
+            float tn = fmaf((float)k, step, t0);
+            if (k >= kMaxSteps) {
+                // step limit (RaySegmentStatus::StepLimit, RaySegmentBatch.cu:281-284): reported invalid
+                ++s_invalid;
+                if (!PATHS) {
+                    out.t[ray] = tn; out.value[ray] = fv; out.gx[ray] = out.gy[ray] = out.gz[ray] = 0.f;
+                    out.outcome[ray] = GPIS_INVALID;
+                    if (out.evals) out.evals[ray] = evals_flight;
+                } else {
+                    if (bounce == 0) { pout.t0[ray] = tn; pout.outcome0[ray] = GPIS_INVALID; }
+                    pout.flights[ray] = path_flights; pout.hits[ray] = path_hits;
+                    if (pout.evals) pout.evals[ray] = path_evals + evals_flight;
+                }
+                evals_flight = 0;
+                phase = PH_NEED;
+                break;
+            }
+            if (tn < far_t) {
+                qx = fmaf(tn, dx, ox); qy = fmaf(tn, dy, oy); qz = fmaf(tn, dz, oz);
+            } else {
+                phase = PH_FAR;
+                qx = fmaf(far_t, dx, ox); qy = fmaf(far_t, dy, oy); qz = fmaf(far_t, dz, oz);
+            }
+            break;
+        }
+        case PH_SECANT: {
+            int st = fv < 0.f ? -1 : 1;
+            bool done = false;
+            if (st == sign0) {
+                done = true;
+            } else {
+                fac *= 0.9;
+                if (fac <= 0.01) {
+                    tp = 0.0;
+                    done = true;
+                } else {
+                    tp = tt;
+                    tt = sa * (1.0 - fac) + sb * fac;
+                    qx = (float)((double)ox + tt * (double)dx);
+                    qy = (float)((double)oy + tt * (double)dy);
+                    qz = (float)((double)oz + tt * (double)dz);
+                }
+            }
+            if (done) {
+                hit = 1;
+                t_end = (float)tp;
+                val_end = 0.f;
+                phase = PH_GRAD;
+                qx = (float)((double)ox + tp * (double)dx);
+                qy = (float)((double)oy + tp * (double)dy);
+                qz = (float)((double)oz + tp * (double)dz);
+            }
+            break;
+        }
+        case PH_FAR: {
+            hit = 0;
+            t_end = far_t;
+            val_end = fv;
+            phase = PH_GRAD;
+            qx = fmaf(far_t, dx, ox); qy = fmaf(far_t, dy, oy); qz = fmaf(far_t, dz, oz);
+            break;
+        }
+        case PH_GRAD: {
+            // resumeDistance outcome rules (GaussianProcessMedium.cpp:305-348)
+            uint32_t outcome;
+            float gxv = e.gx, gyv = e.gy, gzv = e.gz;
+            if (hit) {
+                float avg = (gxv + gyv + gzv);
+                if (!isfinite(avg)) outcome = GPIS_FAIL | (GPIS_FAIL_NONFINITE << 8);
+                else if (gxv * dx + gyv * dy + gzv * dz > 0.f) outcome = GPIS_FAIL | (GPIS_FAIL_OUTWARD << 8);
+                else if (gxv * gxv + gyv * gyv + gzv * gzv < 1e-7f) outcome = GPIS_FAIL | (GPIS_FAIL_ZERO_GRAD << 8);
+                else outcome = GPIS_HIT;
+            } else {
+                outcome = GPIS_EXIT;
+            }
+            ++s_flights;
+            s_evals += evals_flight;
+            unsigned oc = outcome & 0xFFu;
+            if (oc == GPIS_HIT) ++s_hits; else if (oc == GPIS_EXIT) ++s_exits; else ++s_fails;
+            if (!PATHS) {
+                out.t[ray] = t_end; out.value[ray] = val_end;
+                out.gx[ray] = gxv; out.gy[ray] = gyv; out.gz[ray] = gzv;
+                out.outcome[ray] = outcome;
+                if (out.evals) out.evals[ray] = evals_flight;
+                phase = PH_NEED;
+            } else {
+                if (bounce == 0) {
+                    pout.t0[ray] = t_end; pout.outcome0[ray] = outcome;
+                    pout.gx0[ray] = gxv; pout.gy0[ray] = gyv; pout.gz0[ray] = gzv;
+                }
+                ++path_flights;
+                path_evals += evals_flight;
+                evals_flight = 0;
+                if (oc == GPIS_HIT) ++path_hits;
+                if (oc == GPIS_HIT && bounce < max_bounces) {
+                    // diffuse bounce: cosine-weighted about the normalised gradient
+                    ++bounce;
+                    float hx = fmaf(t_end, dx, ox), hy = fmaf(t_end, dy, oy), hz = fmaf(t_end, dz, oz);
+                    float il = rsqrtf(gxv * gxv + gyv * gyv + gzv * gzv);
+                    float nx = gxv * il, ny = gyv * il, nz = gzv * il;
+                    Pcg g;
+                    g.set((uint64_t)(uint32_t)(xxh4(ray, (uint32_t)bounce, bounce_seed, 0u) + 1u));
+                    float u1 = u01(g.next()), u2 = u01(g.next()), jit = u01(g.next());
+                    float rr = sqrtf(u1), ph = 6.2831853071795864f * u2;
+                    float lx = rr * cosf(ph), ly = rr * sinf(ph), lz = sqrtf(fmaxf(0.f, 1.f - u1));
+                    float sgn = copysignf(1.0f, nz);
+                    float aa = -1.0f / (sgn + nz), bb = nx * ny * aa;
+                    float t0x = 1.0f + sgn * nx * nx * aa, t0y = sgn * bb, t0z = -sgn * nx;
+                    float b0x = bb, b0y = sgn + ny * ny * aa, b0z = -ny;
+                    float ndx = t0x * lx + b0x * ly + nx * lz;
+                    float ndy = t0y * lx + b0y * ly + ny * lz;
+                    float ndz = t0z * lx + b0z * ly + nz * lz;
+                    float inl = rsqrtf(ndx * ndx + ndy * ndy + ndz * ndz);
+                    ndx *= inl; ndy *= inl; ndz *= inl;
+                    ox = hx; oy = hy; oz = hz; dx = ndx; dy = ndy; dz = ndz;
+                    near_t = 0.f;
+                    far_t = box_exit(ox, oy, oz, dx, dy, dz, box);
+                    prev_v = 0.f; prev_gx = gxv; prev_gy = gyv; prev_gz = gzv;
+                    first = 0;
+                    if (far_t <= near_t) {
+                        ++s_exits; ++path_flights;
+                        phase = PH_NEED;
+                    } else {
+                        step = (far_t - near_t) / P.min_step_f;
+                        if (P.step_size < step) step = P.step_size;
+                        t0 = near_t + step * jit;
+                        if (P.activate) { phase = PH_COND; qx = ox; qy = oy; qz = oz; }
+                        else { phase = PH_F0; qx = ox; qy = oy; qz = oz; }
+                        rs.cond = 0;
+                    }
+                } else {
+                    phase = PH_NEED;
+                }
+                if (phase == PH_NEED) {
+                    pout.flights[ray] = path_flights;
+                    pout.hits[ray] = path_hits;
+                    if (pout.evals) pout.evals[ray] = path_evals;
+                }
+            }
+            if (!PATHS) evals_flight = 0;
+            break;
+        }
+        default:
+            break;
+        }
+    }
+    // ---- statistics: one atomic per counter per wave
+    unsigned vals[7] = {s_flights, s_evals, s_steps, s_hits, s_exits, s_fails, s_invalid};
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+        unsigned v = vals[i];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0 && v) atomicAdd(&stats[i], (unsigned long long)v);
+    }
+}
+
+// ---------------------------------------------------------------- C ABI --
+extern "C" {
+
+gpis_status gpis_create(const gpis_field_desc *desc, gpis_ctx **out) {
+    if (!desc || !out) return GPIS_INVALID_ARGUMENT;
+    *out = nullptr;
+    gpis_ctx *c = new (std::nothrow) gpis_ctx();
+    if (!c) return GPIS_OUT_OF_MEMORY;
+    c->desc = *desc;
+    gpis_status s = build_params(c);
+    if (s != GPIS_OK) {
+        fprintf(stderr, "gpis_create: %s\n", c->err.c_str());
+        delete c;
+        return s;
+    }
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->counters, 16 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->counters, 0, 16 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->work, 64);
+    if (e != hipSuccess) {
+        fprintf(stderr, "gpis_create: %s\n", hipGetErrorString(e));
+        if (c->counters) (void)hipFree(c->counters);
+        if (c->work) (void)hipFree(c->work);
+        delete c;
+        return GPIS_DEVICE_ERROR;
+    }
+    *out = c;
+    return GPIS_OK;
+}
+
+gpis_status gpis_destroy(gpis_ctx *c) {
+    if (!c) return GPIS_INVALID_ARGUMENT;
+    (void)hipDeviceSynchronize();
+    if (c->table) (void)hipFree(c->table);
+    if (c->counters) (void)hipFree(c->counters);
+    if (c->work) (void)hipFree(c->work);
+    delete c;
+    return GPIS_OK;
+}
+
+const char *gpis_last_error(const gpis_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+gpis_status gpis_set_domain(gpis_ctx *c, const float lo[3], const float hi[3], uint32_t seed) {
+    if (!c || !lo || !hi) return fail(c, GPIS_INVALID_ARGUMENT, "null argument");
+    if (c->P.multires) return fail(c, GPIS_UNSUPPORTED, "impulse tables cover the single-level grid");
+    if (c->table) { HIP_TRY(c, hipFree(c->table)); c->table = nullptr; }
+    c->P.has_table = 0;
+    c->P.table = nullptr;
+    float R = c->P.R;
+    int clo[3], dims[3];
+    size_t cells = 1;
+    for (int i = 0; i < 3; ++i) {
+        if (!(hi[i] > lo[i])) return fail(c, GPIS_INVALID_ARGUMENT, "empty domain");
+        clo[i] = (int)floorf(lo[i] / R) - 1;
+        int chi = (int)floorf(hi[i] / R) + 1;
+        dims[i] = chi - clo[i] + 1;
+        cells *= (size_t)dims[i];
+    }
+    if (cells * c->P.N * sizeof(float4) > (size_t)8 << 30) return fail(c, GPIS_INVALID_ARGUMENT, "domain too large for a table");
+    HIP_TRY(c, hipMalloc((void **)&c->table, cells * c->P.N * sizeof(float4)));
+    int threads = 256;
+    unsigned blocks = (unsigned)((cells + threads - 1) / threads);
+    hipLaunchKernelGGL(impulse_table_kernel, dim3(blocks), dim3(threads), 0, 0, clo[0], clo[1], clo[2], dims[0],
+                       dims[1], dims[2], seed, c->P.N, c->table);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipDeviceSynchronize());
+    c->table_cells = cells;
+    c->P.table = c->table;
+    for (int i = 0; i < 3; ++i) { c->P.tlo[i] = clo[i]; c->P.tdim[i] = dims[i]; }
+    c->P.tseed = seed;
+    c->P.has_table = 1;
+    return GPIS_OK;
+}
+
+gpis_status gpis_impulse_table(gpis_ctx *c, const int32_t lo[3], const int32_t dims[3], uint32_t seed, float *out,
+                               void *stream) {
+    if (!c || !lo || !dims || !out) return fail(c, GPIS_INVALID_ARGUMENT, "null argument");
+    long long total = 1;
+    for (int i = 0; i < 3; ++i) {
+        if (dims[i] <= 0) return fail(c, GPIS_INVALID_ARGUMENT, "dims must be positive");
+        total *= dims[i];
+    }
+    if (total > (1LL << 31)) return fail(c, GPIS_INVALID_ARGUMENT, "too many cells");
+    int threads = 256;
+    unsigned blocks = (unsigned)((total + threads - 1) / threads);
+    hipLaunchKernelGGL(impulse_table_kernel, dim3(blocks), dim3(threads), 0, (hipStream_t)stream, lo[0], lo[1], lo[2],
+                       dims[0], dims[1], dims[2], seed, c->P.N, (float4 *)out);
+    HIP_TRY(c, hipGetLastError());
+    return GPIS_OK;
+}
+
+gpis_status gpis_eval_points(gpis_ctx *c, const float *x, const float *y, const float *z, int64_t n, uint32_t seed,
+                             float *v, float *gx, float *gy, float *gz, void *stream) {
+    if (!c || n < 0) return fail(c, GPIS_INVALID_ARGUMENT, "bad argument");
+    if (n == 0) return GPIS_OK;
+    if (!x || !y || !z || !v || !gx || !gy || !gz) return fail(c, GPIS_INVALID_ARGUMENT, "null pointer");
+    int threads = 256;
+    unsigned blocks = (unsigned)((n + threads - 1) / threads);
+    if (c->P.N == 8)
+        hipLaunchKernelGGL(eval_points_kernel<8>, dim3(blocks), dim3(threads), 0, (hipStream_t)stream, c->P, x, y, z,
+                           (long long)n, seed, v, gx, gy, gz);
+    else
+        hipLaunchKernelGGL(eval_points_kernel<0>, dim3(blocks), dim3(threads), 0, (hipStream_t)stream, c->P, x, y, z,
+                           (long long)n, seed, v, gx, gy, gz);
+    HIP_TRY(c, hipGetLastError());
+    return GPIS_OK;
+}
+
+static gpis_status check_rays(gpis_ctx *c, const gpis_ray_soa *r, int64_t n) {
+    if (!r) return fail(c, GPIS_INVALID_ARGUMENT, "null rays");
+    if (n < 0 || n >= (1LL << 31)) return fail(c, GPIS_INVALID_ARGUMENT, "ray count out of range");
+    if (!r->ox || !r->oy || !r->oz || !r->dx || !r->dy || !r->dz || !r->near_t || !r->far_t || !r->jitter ||
+        !r->noise_seed || !r->flags)
+        return fail(c, GPIS_INVALID_ARGUMENT, "null ray field");
+    if (!r->prev_value || !r->prev_gx || !r->prev_gy || !r->prev_gz)
+        return fail(c, GPIS_INVALID_ARGUMENT, "null previous-state field");
+    if (r->length_scale) return fail(c, GPIS_UNSUPPORTED, "per-ray length scale is a 1D feature");
+    return GPIS_OK;
+}
+
+static void launch_shape(gpis_ctx *c, int64_t n, unsigned &blocks, unsigned &threads) {
+    threads = (unsigned)(c->block_size > 0 ? c->block_size : 256);
+    int wpc = c->waves_per_cu > 0 ? c->waves_per_cu : 16;
+    long long want = (long long)c->num_cus * wpc * 64 / threads;
+    long long need = (n + threads - 1) / threads;
+    if (want < 1) want = 1;
+    blocks = (unsigned)(need < want ? need : want);
+    if (blocks < 1) blocks = 1;
+}
+
+static RayIn make_in(const gpis_ray_soa *r) {
+    RayIn in = {r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, r->near_t, r->far_t, r->jitter,
+                r->noise_seed, r->normal_seed, r->flags, r->prev_value, r->prev_gx, r->prev_gy, r->prev_gz};
+    return in;
+}
+
+gpis_status gpis_trace_flights(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, const gpis_reply_soa *reply,
+                               void *stream) {
+    if (!c) return GPIS_INVALID_ARGUMENT;
+    gpis_status s = check_rays(c, rays, n);
+    if (s != GPIS_OK) return s;
+    if (!reply || !reply->t || !reply->value || !reply->gx || !reply->gy || !reply->gz || !reply->outcome)
+        return fail(c, GPIS_INVALID_ARGUMENT, "null reply field");
+    if (n == 0) return GPIS_OK;
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(c, hipMemsetAsync(c->work, 0, sizeof(unsigned), st));
+    unsigned blocks, threads;
+    launch_shape(c, n, blocks, threads);
+    ReplyOut out = {reply->t, reply->value, reply->gx, reply->gy, reply->gz, reply->outcome, reply->evals};
+    PathOut pout = {};
+    if (c->P.N == 8)
+        hipLaunchKernelGGL((flight_kernel<8, false>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays), out,
+                           pout, (unsigned)n, c->work, c->counters, 0, 0u, 0.f);
+    else
+        hipLaunchKernelGGL((flight_kernel<0, false>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays), out,
+                           pout, (unsigned)n, c->work, c->counters, 0, 0u, 0.f);
+    HIP_TRY(c, hipGetLastError());
+    return GPIS_OK;
+}
+
+gpis_status gpis_trace_paths(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, int32_t max_bounces,
+                             uint32_t bounce_seed, float box, const gpis_path_reply *reply, void *stream) {
+    if (!c) return GPIS_INVALID_ARGUMENT;
+    gpis_status s = check_rays(c, rays, n);
+    if (s != GPIS_OK) return s;
+    if (!reply || !reply->t0 || !reply->gx0 || !reply->gy0 || !reply->gz0 || !reply->outcome0 || !reply->flights ||
+        !reply->hits)
+        return fail(c, GPIS_INVALID_ARGUMENT, "null reply field");
+    if (max_bounces < 0 || max_bounces > 64) return fail(c, GPIS_INVALID_ARGUMENT, "max_bounces out of range");
+    if (!(box > 0.f)) return fail(c, GPIS_INVALID_ARGUMENT, "box must be positive");
+    if (n == 0) return GPIS_OK;
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(c, hipMemsetAsync(c->work, 0, sizeof(unsigned), st));
+    unsigned blocks, threads;
+    launch_shape(c, n, blocks, threads);
+    ReplyOut out = {};
+    PathOut pout = {reply->t0, reply->gx0, reply->gy0, reply->gz0, reply->outcome0, reply->flights, reply->hits,
+                    reply->evals};
+    if (c->P.N == 8)
+        hipLaunchKernelGGL((flight_kernel<8, true>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays), out, pout,
+                           (unsigned)n, c->work, c->counters, max_bounces, bounce_seed, box);
+    else
+        hipLaunchKernelGGL((flight_kernel<0, true>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays), out, pout,
+                           (unsigned)n, c->work, c->counters, max_bounces, bounce_seed, box);
+    HIP_TRY(c, hipGetLastError());
+    return GPIS_OK;
+}
+
+gpis_status gpis_get_counters(gpis_ctx *c, gpis_counters *o) {
+    if (!c || !o) return fail(c, GPIS_INVALID_ARGUMENT, "null argument");
+    unsigned long long h[16];
+    HIP_TRY(c, hipDeviceSynchronize());
+    HIP_TRY(c, hipMemcpy(h, c->counters, sizeof h, hipMemcpyDeviceToHost));
+    o->flights = h[0]; o->evals = h[1]; o->steps = h[2]; o->hits = h[3];
+    o->exits = h[4]; o->fails = h[5]; o->invalid = h[6]; o->kernel_launches = h[7];
+    return GPIS_OK;
+}
+
+gpis_status gpis_reset_counters(gpis_ctx *c) {
+    if (!c) return GPIS_INVALID_ARGUMENT;
+    HIP_TRY(c, hipMemset(c->counters, 0, 16 * sizeof(unsigned long long)));
+    return GPIS_OK;
+}
+
+gpis_status gpis_set_launch(gpis_ctx *c, int32_t waves_per_cu, int32_t block_size) {
+    if (!c) return GPIS_INVALID_ARGUMENT;
+    if (waves_per_cu < 0 || waves_per_cu > 32) return fail(c, GPIS_INVALID_ARGUMENT, "waves_per_cu out of range");
+    if (block_size != 0 && (block_size < 64 || block_size > 256 || block_size % 64))
+        return fail(c, GPIS_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
+    c->waves_per_cu = waves_per_cu;
+    c->block_size = block_size ? block_size : 256;
+    return GPIS_OK;
+}
+
+}  // extern "C"

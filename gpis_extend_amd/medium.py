@@ -1,0 +1,345 @@
+"""Host-side mirror of Tungsten's ``sparse_conv_noise`` medium over the HIP C ABI.
+
+``SparseConvolutionNoiseMedium.from_json`` reads the same JSON parameters as the
+reference (src/core/media/SparseConvolutionNoiseMedium.cpp:102-118,
+src/core/media/GaussianProcessMedium.cpp:98-127, src/core/math/GPFunctions.cpp:657-685,
+src/core/math/GaussianProcess.cpp:173-191) and fails on the same invalid values.
+``GpisDevice`` owns one gpis_ctx and exposes the wavefront entry points on torch
+device tensors: ``sample_distance`` (GaussianProcessMedium::sampleDistance, :222-234)
+and ``transmittance`` (:424-440) semantics, batched.
+
+torch is used only for device memory and the current HIP stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import _capi as A
+from .hashing import xxhash32_1, xxhash32_4
+
+
+class MediumConfigError(ValueError):
+    pass
+
+
+_CTX = {"global": A.CTX_GLOBAL, "renewal+": A.CTX_RENEWAL_PLUS, "renewal": A.CTX_RENEWAL, "none": A.CTX_NONE}
+_SCHEME = {"uni": A.SCHEME_UNI, "UNI": A.SCHEME_UNI, "nee": A.SCHEME_NEE, "NEE": A.SCHEME_NEE,
+           "mis": A.SCHEME_MIS, "MIS": A.SCHEME_MIS}
+
+
+@dataclass
+class SparseConvolutionNoiseMedium:
+    """Parameters of one sparse_conv_noise medium.  Defaults are the reference
+    constructor's (SparseConvolutionNoiseMedium.cpp:62-79, Medium.cpp:24,
+    GaussianProcessMedium.cpp:87-96)."""
+    step_size: float = 0.01
+    min_step: int = 8
+    seed: int = 0
+    impulse_density: float = 3.0
+    single_realization: bool = False
+    isotropic_3D_sampling: bool = False
+    one_d: bool = False
+    scheme: int = A.SCHEME_UNI
+    correlation_xy: bool = False
+    correlation_context: int = A.CTX_RENEWAL_PLUS
+    max_bounces: int = 1024
+    # gaussian_process.covariance
+    cov_kind: int = A.COV_SE
+    sigma: float = 1.0
+    length_scale: float = 1.0
+    local_scale: float = 3.0
+    aniso: tuple = (1.0, 1.0, 1.0)
+    ns_multires: bool = False
+    ns_ls_field: int = A.LS_CONST
+    ns_ls_params: tuple = (1.0, 0.0, 0.0, 0.0)
+    ns_ls_max: float = 1.0
+    ns_var: float = 1.0
+    # gaussian_process.mean
+    mean_kind: int = A.MEAN_SPHERE
+    mean_center: tuple = (0.0, 0.0, 0.0)
+    mean_radius: float = 1.0
+    mean_half: tuple = (0.5, 0.5, 0.5)
+    plane_normal: tuple = (1.0, 0.0, 0.0)
+    plane_offset: float = 0.0
+    extra: Dict = field(default_factory=dict)
+
+    @classmethod
+    def from_json(cls, js: dict) -> "SparseConvolutionNoiseMedium":
+        if js.get("type", "sparse_conv_noise") != "sparse_conv_noise":
+            raise MediumConfigError(f"not a sparse_conv_noise medium: {js.get('type')!r}")
+        m = cls()
+        m.step_size = float(js.get("step_size", m.step_size))
+        m.min_step = int(js.get("min_step", m.min_step))
+        m.seed = int(js.get("seed", m.seed)) & 0xFFFFFFFF
+        m.impulse_density = float(js.get("impulse_density", m.impulse_density))
+        m.single_realization = bool(js.get("single_realization", m.single_realization))
+        m.isotropic_3D_sampling = bool(js.get("isotropic_3D_sampling", False))
+        m.one_d = bool(js.get("1D_sampling", False))
+        scheme = js.get("1D_sampling_scheme", "uni")
+        if scheme not in _SCHEME:
+            raise MediumConfigError(f"Invalid sparse conv sampling scheme: '{scheme}'")
+        m.scheme = _SCHEME[scheme]
+        m.correlation_xy = bool(js.get("1D_gradient_correlationXY", False))
+        # the reference parses a placeholder and FAILs when the key is absent (GaussianProcessMedium.cpp:105-107)
+        ctx = js.get("correlation_context", "goldfish")
+        if ctx not in _CTX:
+            raise MediumConfigError(f"Invalid correlation context: '{ctx}'")
+        m.correlation_context = _CTX[ctx]
+        if js.get("intersect_method", "gp_discrete") != "gp_discrete":
+            raise MediumConfigError("only intersect_method 'gp_discrete' is a sparse-convolution flight")
+        m.max_bounces = int(js.get("max_bounces", m.max_bounces))
+        gp = js.get("gaussian_process", {})
+        mean = gp.get("mean", {"type": "spherical"})
+        mt = mean.get("type")
+        if mt == "spherical":
+            m.mean_kind = A.MEAN_SPHERE
+            m.mean_center = tuple(float(x) for x in mean.get("center", (0.0, 0.0, 0.0)))
+            m.mean_radius = float(mean.get("radius", 1.0))
+        elif mt == "box":
+            m.mean_kind = A.MEAN_BOX
+            m.mean_center = tuple(float(x) for x in mean.get("center", (0.0, 0.0, 0.0)))
+            h = mean.get("half_extent", 0.5)
+            m.mean_half = tuple(float(x) for x in (h if isinstance(h, (list, tuple)) else (h, h, h)))
+        elif mt == "plane":
+            m.mean_kind = A.MEAN_PLANE
+            m.plane_normal = tuple(float(x) for x in mean.get("normal", (1.0, 0.0, 0.0)))
+            m.plane_offset = float(mean.get("offset", 0.0))
+        else:
+            raise MediumConfigError(f"mean type {mt!r} is not served by the GPU flight path")
+        cov = gp.get("covariance", {"type": "squared_exponential"})
+        ct = cov.get("type")
+        if ct == "proc_nonstationary":
+            m.cov_kind = A.COV_PROC_NS
+            m.ns_multires = bool(cov.get("multiResolutionGrid", False))
+            ls = cov.get("ls", {})
+            if ls.get("type") == "sincos":
+                m.ns_ls_field = A.LS_SINCOS
+                m.ns_ls_params = (float(ls["base"]), float(ls["amplitude"]), float(ls["frequency"]), 0.0)
+                m.ns_ls_max = float(ls["base"]) + float(ls["amplitude"])
+            else:
+                v = float(ls.get("value", 1.0))
+                m.ns_ls_field, m.ns_ls_params, m.ns_ls_max = A.LS_CONST, (v, 0.0, 0.0, 0.0), v
+            m.ns_var = float(cov.get("var", 1.0))
+            cov = cov.get("cov", {"type": "squared_exponential"})
+            ct = cov.get("type")
+        if ct != "squared_exponential":
+            raise MediumConfigError(f"covariance {ct!r} is not served by the GPU flight path")
+        m.sigma = float(cov.get("sigma", 1.0))
+        m.length_scale = float(cov.get("lengthScale", 1.0))
+        m.local_scale = float(cov.get("localScale", 3.0))
+        a = cov.get("aniso", (1.0, 1.0, 1.0))
+        m.aniso = tuple(float(x) for x in (a if isinstance(a, (list, tuple)) else (a, a, a)))
+        if cov.get("useAnisoMtx", False):
+            raise MediumConfigError("useAnisoMtx is not served by the GPU flight path")
+        if m.isotropic_3D_sampling and not m.one_d:
+            raise MediumConfigError("isotropic_3D_sampling is not served by the GPU flight path")
+        return m
+
+    def to_desc(self) -> A.FieldDesc:
+        d = A.FieldDesc()
+        d.cov_kind = self.cov_kind
+        d.sigma, d.length_scale, d.kernel_scale = self.sigma, self.length_scale, self.local_scale
+        for i in range(3):
+            d.aniso[i] = self.aniso[i]
+            d.mean_center[i] = self.mean_center[i]
+            d.mean_half[i] = self.mean_half[i]
+            d.plane_normal[i] = self.plane_normal[i]
+        d.ns_multires = int(self.ns_multires)
+        d.ns_ls_field = self.ns_ls_field
+        for i in range(4):
+            d.ns_ls_params[i] = self.ns_ls_params[i]
+        d.ns_ls_max, d.ns_var = self.ns_ls_max, self.ns_var
+        d.mean_kind, d.mean_radius, d.plane_offset = self.mean_kind, self.mean_radius, self.plane_offset
+        d.global_seed = self.seed
+        d.impulse_density = self.impulse_density
+        d.single_realization = int(self.single_realization)
+        d.ctxt = self.correlation_context
+        d.one_d = int(self.one_d)
+        d.scheme = self.scheme
+        d.correlation_xy = int(self.correlation_xy)
+        d.step_size, d.min_step, d.max_bounce = self.step_size, self.min_step, self.max_bounces
+        return d
+
+    @property
+    def kernel_radius(self) -> float:
+        """World cell size of the 3D grid, splattingKernelRadius(false, 1) (GPFunctions.cpp:706-719)."""
+        f = np.float32
+        l_conv = f(self.length_scale) * np.sqrt(f(2.0)) / f(2)
+        la = [f(l_conv) * f(a) for a in self.aniso]
+        ls = f(1.0) if self.cov_kind == A.COV_SE or self.ns_multires else f(self.ns_ls_max)
+        return float(f(self.local_scale) * ls * max(la))
+
+    def compute_seed(self, pss: np.ndarray, scene_seed) -> np.ndarray:
+        """SparseConvolutionNoiseRealization::computeSeed (SparseConvolutionNoise.cpp:57-66), vectorised."""
+        pss = np.asarray(pss, np.uint32).reshape(-1, 4)
+        n = pss.shape[0]
+        if self.single_realization:
+            return np.full(n, self.seed, np.uint32)
+        w = np.zeros(n, np.uint32) if self.correlation_context == A.CTX_GLOBAL else pss[:, 3]
+        with np.errstate(over="ignore"):
+            return (np.uint32(self.seed) + xxhash32_4(pss[:, 0], pss[:, 1], pss[:, 2], w)
+                    + xxhash32_1(np.broadcast_to(np.asarray(scene_seed, np.uint32), (n,)))).astype(np.uint32)
+
+    @staticmethod
+    def normal_seed(pss: np.ndarray, scene_seed) -> np.ndarray:
+        """xxhash32(pixelSampleSegment) + xxhash32(sceneSeed) + 1 (SparseConvolutionNoiseMedium.cpp:237-238)."""
+        pss = np.asarray(pss, np.uint32).reshape(-1, 4)
+        n = pss.shape[0]
+        with np.errstate(over="ignore"):
+            return (xxhash32_4(pss[:, 0], pss[:, 1], pss[:, 2], pss[:, 3])
+                    + xxhash32_1(np.broadcast_to(np.asarray(scene_seed, np.uint32), (n,))) + np.uint32(1)).astype(np.uint32)
+
+
+RAY_FIELDS = ("ox", "oy", "oz", "dx", "dy", "dz", "near_t", "far_t", "jitter", "noise_seed", "normal_seed",
+              "flags", "prev_value", "prev_gx", "prev_gy", "prev_gz")
+
+
+class RayBatch:
+    """Structure-of-arrays wavefront of flight requests resident in device memory."""
+
+    def __init__(self, tensors: Dict[str, "object"]):
+        self.t = tensors
+        self.n = int(tensors["ox"].shape[0])
+
+    @classmethod
+    def from_numpy(cls, arrays: Dict[str, np.ndarray], device="cuda"):
+        import torch
+        out = {}
+        for k in RAY_FIELDS:
+            a = np.ascontiguousarray(arrays[k])
+            if a.dtype == np.uint32:
+                a = a.view(np.int32)
+            out[k] = torch.from_numpy(a).to(device)
+        return cls(out)
+
+    def soa(self) -> A.RaySoA:
+        s = A.RaySoA()
+        for k in RAY_FIELDS:
+            setattr(s, k, self.t[k].data_ptr())
+        s.length_scale = None
+        return s
+
+
+class GpisDevice:
+    """One gpis_ctx on the current HIP device."""
+
+    def __init__(self, medium: SparseConvolutionNoiseMedium):
+        import torch
+        if not torch.cuda.is_available():
+            raise A.GpisError("no HIP device visible")
+        self.medium = medium
+        self.L = A.lib()
+        self.desc = medium.to_desc()
+        h = C.c_void_p()
+        A.check(self.L.gpis_create(C.byref(self.desc), C.byref(h)), None, "gpis_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gpis_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _stream():
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def set_domain(self, lo, hi, seed: Optional[int] = None):
+        lo3 = (C.c_float * 3)(*lo)
+        hi3 = (C.c_float * 3)(*hi)
+        s = self.medium.seed if seed is None else seed
+        A.check(self.L.gpis_set_domain(self.h, lo3, hi3, s & 0xFFFFFFFF), self.h, "gpis_set_domain")
+
+    def set_launch(self, waves_per_cu=0, block_size=0):
+        A.check(self.L.gpis_set_launch(self.h, waves_per_cu, block_size), self.h, "gpis_set_launch")
+
+    def impulse_table(self, lo, dims, seed):
+        import torch
+        n = int(self.medium.impulse_density)
+        cells = int(dims[0]) * int(dims[1]) * int(dims[2])
+        out = torch.empty(cells * n * 4, dtype=torch.float32, device="cuda")
+        lo3 = (C.c_int32 * 3)(*[int(x) for x in lo])
+        d3 = (C.c_int32 * 3)(*[int(x) for x in dims])
+        A.check(self.L.gpis_impulse_table(self.h, lo3, d3, seed & 0xFFFFFFFF, out.data_ptr(), self._stream()),
+                self.h, "gpis_impulse_table")
+        return out.view(cells, n, 4)
+
+    def eval_points(self, x, y, z, seed):
+        import torch
+        n = int(x.shape[0])
+        v = torch.empty(n, dtype=torch.float32, device=x.device)
+        gx, gy, gz = torch.empty_like(v), torch.empty_like(v), torch.empty_like(v)
+        A.check(self.L.gpis_eval_points(self.h, x.data_ptr(), y.data_ptr(), z.data_ptr(), n, seed & 0xFFFFFFFF,
+                                        v.data_ptr(), gx.data_ptr(), gy.data_ptr(), gz.data_ptr(), self._stream()),
+                self.h, "gpis_eval_points")
+        return v, gx, gy, gz
+
+    def alloc_reply(self, n, device="cuda"):
+        import torch
+        f = dict(dtype=torch.float32, device=device)
+        return {"t": torch.empty(n, **f), "value": torch.empty(n, **f), "gx": torch.empty(n, **f),
+                "gy": torch.empty(n, **f), "gz": torch.empty(n, **f),
+                "outcome": torch.empty(n, dtype=torch.int32, device=device),
+                "evals": torch.empty(n, dtype=torch.int32, device=device)}
+
+    def trace_flights(self, rays: RayBatch, reply=None):
+        if reply is None:
+            reply = self.alloc_reply(rays.n)
+        r = A.ReplySoA()
+        for k in ("t", "value", "gx", "gy", "gz", "outcome", "evals"):
+            setattr(r, k, reply[k].data_ptr())
+        soa = rays.soa()
+        A.check(self.L.gpis_trace_flights(self.h, C.byref(soa), rays.n, C.byref(r), self._stream()),
+                self.h, "gpis_trace_flights")
+        return reply
+
+    def alloc_path_reply(self, n, device="cuda"):
+        import torch
+        f = dict(dtype=torch.float32, device=device)
+        i = dict(dtype=torch.int32, device=device)
+        return {"t0": torch.empty(n, **f), "gx0": torch.empty(n, **f), "gy0": torch.empty(n, **f),
+                "gz0": torch.empty(n, **f), "outcome0": torch.empty(n, **i), "flights": torch.empty(n, **i),
+                "hits": torch.empty(n, **i), "evals": torch.empty(n, **i)}
+
+    def trace_paths(self, rays: RayBatch, max_bounces: int, bounce_seed: int, box: float, reply=None):
+        if reply is None:
+            reply = self.alloc_path_reply(rays.n)
+        r = A.PathReply()
+        for k in ("t0", "gx0", "gy0", "gz0", "outcome0", "flights", "hits", "evals"):
+            setattr(r, k, reply[k].data_ptr())
+        soa = rays.soa()
+        A.check(self.L.gpis_trace_paths(self.h, C.byref(soa), rays.n, max_bounces, bounce_seed & 0xFFFFFFFF,
+                                        box, C.byref(r), self._stream()), self.h, "gpis_trace_paths")
+        return reply
+
+    def counters(self) -> Dict[str, int]:
+        c = A.Counters()
+        A.check(self.L.gpis_get_counters(self.h, C.byref(c)), self.h, "gpis_get_counters")
+        return {k: int(getattr(c, k)) for k, _ in A.Counters._fields_}
+
+    def reset_counters(self):
+        A.check(self.L.gpis_reset_counters(self.h), self.h, "gpis_reset_counters")
+
+    # ---- Medium-shaped views of a reply ---------------------------------
+    @staticmethod
+    def sample_distance(reply) -> Dict[str, "object"]:
+        """sampleDistance semantics per flight: ok (not Failed), exited (no hit), t, aniso."""
+        oc = reply["outcome"] & 0xFF
+        return {"ok": (oc == A.HIT) | (oc == A.EXIT), "exited": oc == A.EXIT, "t": reply["t"],
+                "aniso": (reply["gx"], reply["gy"], reply["gz"])}
+
+    @staticmethod
+    def transmittance(reply):
+        """transmittance semantics: 1 when the flight exits, 0 at any crossing (GaussianProcessMedium.cpp:394-421)."""
+        oc = reply["outcome"] & 0xFF
+        return (oc == A.EXIT).float()

@@ -1,0 +1,167 @@
+/*
+ * gpis_hip.h — C ABI of the MI355X (gfx950) sparse-convolution GPIS flight engine.
+ *
+ * This boundary replaces, for one GPU, the reference's batch boundary between a
+ * renderer and the `sparse_conv_noise` medium:
+ *   GpRayBatchBackend::trace / submitBatch        src/core/media/GpRayBatchBackend.hpp:43-55
+ *   GpRaySegmentRequest / GpRaySegmentResult       src/core/media/GpRayBatchBackend.hpp:15-35
+ *   launchSe1DRaySegments (CUDA device API)        src/gpu/cuda/RaySegmentBatch.hpp:83-93
+ *   SparseConvolutionNoiseMedium::intersectGP +    src/core/media/SparseConvolutionNoiseMedium.cpp:188-347
+ *     GaussianProcessMedium::resumeDistance rules  src/core/media/GaussianProcessMedium.cpp:289-369
+ *
+ * Conventions: every call returns a gpis_status and never throws; all array
+ * arguments are DEVICE pointers owned by the caller; work is enqueued on the
+ * given HIP stream (NULL = default stream) and is asynchronous unless stated.
+ * One context may be used from one host thread at a time.
+ */
+#ifndef GPIS_HIP_H_
+#define GPIS_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gpis_status {
+    GPIS_OK = 0,
+    GPIS_INVALID_ARGUMENT = 1,
+    GPIS_DEVICE_ERROR = 2,
+    GPIS_UNSUPPORTED = 3,
+    GPIS_OUT_OF_MEMORY = 4
+} gpis_status;
+
+/* Covariance kinds: SquaredExponentialCovariance (GPFunctions.hpp:1880-1990) and
+ * ProceduralNonstationaryCovariance (GPFunctions.hpp:2459-2505). */
+enum { GPIS_COV_SE = 0, GPIS_COV_PROC_NS = 1 };
+/* Mean kinds: SphericalMean (GPFunctions.hpp:952-996); box SDF and plane in code. */
+enum { GPIS_MEAN_SPHERE = 0, GPIS_MEAN_BOX = 1, GPIS_MEAN_PLANE = 2 };
+/* GPCorrelationContext (GaussianProcessMedium.cpp:20-42). */
+enum { GPIS_CTX_GLOBAL = 0, GPIS_CTX_RENEWAL = 1, GPIS_CTX_RENEWAL_PLUS = 2, GPIS_CTX_NONE = 3 };
+/* SparseConv1DSamplingScheme (Medium.hpp:42-46). */
+enum { GPIS_SCHEME_UNI = 0, GPIS_SCHEME_NEE = 1, GPIS_SCHEME_MIS = 2 };
+/* Procedural length-scale fields of the nonstationary covariance. */
+enum { GPIS_LS_CONST = 0, GPIS_LS_SINCOS = 1 };
+
+/* The sparse_conv_noise medium's parameters (SparseConvolutionNoiseMedium.cpp:102-118,
+ * GaussianProcessMedium.cpp:98-127, GPFunctions.cpp:657-685). */
+typedef struct gpis_field_desc {
+    int32_t cov_kind;
+    float sigma, length_scale, kernel_scale;   /* "sigma", "lengthScale", "localScale" */
+    float aniso[3];                            /* "aniso" */
+    int32_t ns_multires;                       /* "multiResolutionGrid" */
+    int32_t ns_ls_field;
+    float ns_ls_params[4];                     /* SINCOS: base, amplitude, frequency */
+    float ns_ls_max;
+    float ns_var;
+    int32_t mean_kind;
+    float mean_radius;
+    double mean_center[3];
+    double mean_half[3];
+    double plane_normal[3];
+    double plane_offset;
+    uint32_t global_seed;                      /* "seed" */
+    float impulse_density;                     /* "impulse_density" */
+    int32_t single_realization;                /* "single_realization" */
+    int32_t ctxt;                              /* "correlation_context" */
+    int32_t one_d;                             /* "1D_sampling" */
+    int32_t scheme;                            /* "1D_sampling_scheme" */
+    int32_t correlation_xy;                    /* "1D_gradient_correlationXY" */
+    float step_size;                           /* "step_size" */
+    uint32_t min_step;                         /* "min_step" */
+    int32_t max_bounce;                        /* Medium "max_bounces" */
+} gpis_field_desc;
+
+/* A wavefront of flight requests in structure-of-arrays form
+ * (GpRaySegmentRequest fields, GpRayBatchBackend.hpp:15-29). */
+typedef struct gpis_ray_soa {
+    const float *ox, *oy, *oz;        /* origin */
+    const float *dx, *dy, *dz;        /* unit direction */
+    const float *near_t, *far_t;
+    const float *jitter;              /* the one PathSampleGenerator draw, [0,1) */
+    const uint32_t *noise_seed;       /* SparseConvolutionNoiseRealization::computeSeed */
+    const uint32_t *normal_seed;      /* xxhash32(pss) + xxhash32(sceneSeed) + 1 */
+    const uint32_t *flags;            /* bit 0: firstScatter */
+    const float *prev_value;          /* MediumState::lastVal   (read when !firstScatter) */
+    const float *prev_gx, *prev_gy, *prev_gz; /* MediumState::lastAniso (idem) */
+    const float *length_scale;        /* optional per-ray SE length scale (NULL: the medium's) */
+} gpis_ray_soa;
+
+/* Outcomes of one flight (DistanceStatus and sample.exited, GaussianProcessMedium.cpp:289-369). */
+enum { GPIS_EXIT = 0, GPIS_HIT = 1, GPIS_FAIL = 2, GPIS_INVALID = 3 };
+enum { GPIS_FAIL_NONE = 0, GPIS_FAIL_OUTWARD = 1, GPIS_FAIL_ZERO_GRAD = 2, GPIS_FAIL_NONFINITE = 3 };
+
+typedef struct gpis_reply_soa {
+    float *t;                         /* sample distance; farT on exit */
+    float *value;                     /* new MediumState::lastVal (0 at a hit, f(farT) on exit) */
+    float *gx, *gy, *gz;              /* sample.aniso: gradient at the hit, or at farT on exit */
+    uint32_t *outcome;                /* low byte GPIS_EXIT/HIT/FAIL/INVALID, next byte fail reason */
+    uint32_t *evals;                  /* field evaluations spent on this flight (may be NULL) */
+} gpis_reply_soa;
+
+typedef struct gpis_counters {
+    uint64_t flights, evals, steps, hits, exits, fails, invalid;
+    uint64_t kernel_launches;
+} gpis_counters;
+
+typedef struct gpis_ctx gpis_ctx;
+
+/* Create a context for one medium on the current HIP device.
+ * Replaces SparseConvolutionNoiseMedium::fromJson + the realization constructor
+ * (SparseConvolutionNoiseMedium.cpp:102-118, SparseConvolutionNoise.cpp:25-54). */
+gpis_status gpis_create(const gpis_field_desc *desc, gpis_ctx **out);
+gpis_status gpis_destroy(gpis_ctx *ctx);
+/* Text of the last error on this context (never NULL). */
+const char *gpis_last_error(const gpis_ctx *ctx);
+
+/* Precompute the impulse tables of every cell in [lo, lo+dims) for `seed`
+ * (used when single_realization makes the realization a pure function of the
+ * cell).  Cells outside the box are generated on the fly.  Synchronous.
+ * The table is a pure function of SparseConvolutionNoise.cpp:432-452's cell3D RNG. */
+gpis_status gpis_set_domain(gpis_ctx *ctx, const float lo[3], const float hi[3], uint32_t seed);
+
+/* Write the impulses of the cells [lo, lo+dims) for `seed`: out[(cell*N + k)*4 + {x,y,z,w}],
+ * cells ordered x fastest.  Bit-exact with cell3D (SparseConvolutionNoise.cpp:432-452). */
+gpis_status gpis_impulse_table(gpis_ctx *ctx, const int32_t lo[3], const int32_t dims[3],
+                               uint32_t seed, float *out, void *stream);
+
+/* Value and world gradient of the unconditioned field at n points (SoA in/out):
+ * evaluateValue/evaluateGradient (SparseConvolutionNoise.cpp:114-140) with seed. */
+gpis_status gpis_eval_points(gpis_ctx *ctx, const float *x, const float *y, const float *z,
+                             int64_t n, uint32_t seed, float *value, float *gx, float *gy,
+                             float *gz, void *stream);
+
+/* Trace n independent flights, one per request: prepareGpisSegment + finishGpisSegment
+ * + resumeDistance outcome rules, in one persistent launch. */
+gpis_status gpis_trace_flights(gpis_ctx *ctx, const gpis_ray_soa *rays, int64_t n,
+                               const gpis_reply_soa *reply, void *stream);
+
+/* Trace n paths of up to 1 + max_bounces flights each in one persistent launch:
+ * each hit continues as a cosine-weighted diffuse bounce about the normalised
+ * gradient (secondary flights use the step-1 sign rule), bounded by the box
+ * [-box, box]^3.  Per path: number of flights, flights per outcome, and the
+ * first flight's reply.  Bounce directions use PCG32 seeded by
+ * xxhash32(ray index, bounce, bounce_seed). */
+typedef struct gpis_path_reply {
+    float *t0, *gx0, *gy0, *gz0;      /* first flight: t and gradient */
+    uint32_t *outcome0;
+    uint32_t *flights;                /* flights traced for this path */
+    uint32_t *hits;                   /* hit flights on this path */
+    uint32_t *evals;                  /* field evaluations on this path (may be NULL) */
+} gpis_path_reply;
+gpis_status gpis_trace_paths(gpis_ctx *ctx, const gpis_ray_soa *rays, int64_t n,
+                             int32_t max_bounces, uint32_t bounce_seed, float box,
+                             const gpis_path_reply *reply, void *stream);
+
+/* Accumulated device counters (synchronises the context's stream). */
+gpis_status gpis_get_counters(gpis_ctx *ctx, gpis_counters *out);
+gpis_status gpis_reset_counters(gpis_ctx *ctx);
+
+/* Launch-shape control (the autotuner's knobs): waves per CU of the persistent
+ * flight kernel, and rays per lane batch.  0 = built-in default. */
+gpis_status gpis_set_launch(gpis_ctx *ctx, int32_t waves_per_cu, int32_t block_size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

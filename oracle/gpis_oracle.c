@@ -1,0 +1,1052 @@
+/*
+ * gpis_oracle.c — CPU restatement of the sparse-convolution GPIS ray flight.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product): loaded by tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg.
+ *
+ * Build: plain C11, -O2 -ffp-contract=off, no fast-math, so that every float
+ * and double operation rounds as written.  Types follow the reference:
+ * a float in the reference is a float here, a double a double.  Where the
+ * reference calls an unqualified libm function on a float, the float overload
+ * is used (logf, powf, floorf); exp on float kernel arguments is evaluated in
+ * double and rounded, which is within 1 ulp of either overload.
+ *
+ * Citations are file:line in the reference tree.
+ */
+#include "gpis_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI_F 3.1415926536f /* src/core/math/Angle.hpp:8 (float) */
+
+typedef struct { float x, y, z; } v3f;
+typedef struct { double x, y, z; } v3d;
+typedef struct { float x, y, z, w; } v4f;
+typedef struct { float x, y; } v2f;
+typedef struct { float m[3][3]; } m3f; /* m[row][col] */
+
+static inline v3f V3(float x, float y, float z) { v3f r = {x, y, z}; return r; }
+static inline v3f vadd(v3f a, v3f b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3f vsub(v3f a, v3f b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3f vmul(v3f a, v3f b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline v3f vscale(float s, v3f a) { return V3(s * a.x, s * a.y, s * a.z); }
+static inline v3f vdivs(v3f a, float s) { return V3(a.x / s, a.y / s, a.z / s); }
+/* Vec::dot: sum = a0*b0; sum += a1*b1; sum += a2*b2 (Vec.hpp:200-206) */
+static inline float vdot(v3f a, v3f b) { float s = a.x * b.x; s += a.y * b.y; s += a.z * b.z; return s; }
+/* Vec::lengthSq starts from 0 (Vec.hpp:134-140) */
+static inline float vlen2(v3f a) { float s = 0.f; s += a.x * a.x; s += a.y * a.y; s += a.z * a.z; return s; }
+static inline float vlen(v3f a) { return sqrtf(vlen2(a)); }
+/* Vec::normalized multiplies by 1/length (Vec.hpp:191-198) */
+static inline v3f vnorm(v3f a) { float inv = 1.0f / vlen(a); return V3(a.x * inv, a.y * inv, a.z * inv); }
+static inline v3f vcross(v3f a, v3f b) {
+    return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline v4f V4(float x, float y, float z, float w) { v4f r = {x, y, z, w}; return r; }
+static inline v4f v4add(v4f a, v4f b) { return V4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+static inline v4f v4scale(float s, v4f a) { return V4(s * a.x, s * a.y, s * a.z, s * a.w); }
+static inline v4f v4divs(v4f a, float s) { return V4(a.x / s, a.y / s, a.z / s, a.w / s); }
+static inline v3d D3(double x, double y, double z) { v3d r = {x, y, z}; return r; }
+static inline v3d d_of(v3f a) { return D3(a.x, a.y, a.z); }
+static inline v3f f_of(v3d a) { return V3((float)a.x, (float)a.y, (float)a.z); }
+static inline double ddot(v3d a, v3d b) { double s = a.x * b.x; s += a.y * b.y; s += a.z * b.z; return s; }
+static inline double dlen2(v3d a) { double s = 0.0; s += a.x * a.x; s += a.y * a.y; s += a.z * a.z; return s; }
+static inline v3d dnorm(v3d a) { double inv = 1.0 / sqrt(dlen2(a)); return D3(a.x * inv, a.y * inv, a.z * inv); }
+static inline float sqrf(float a) { return a * a; }
+
+/* ===================================================================== */
+/* Leaves                                                                */
+/* ===================================================================== */
+
+#define P2 2246822519U
+#define P3 3266489917U
+#define P4 668265263U
+#define P5 374761393U
+static inline uint32_t rotl17(uint32_t h) { return (h << 17) | (h >> 15); }
+static inline uint32_t xxfinal(uint32_t h) {
+    h = P2 * (h ^ (h >> 15));
+    h = P3 * (h ^ (h >> 13));
+    return h ^ (h >> 16);
+}
+/* MathUtil.hpp:179-187 */
+uint32_t orc_xxhash32_1(uint32_t p) { uint32_t h = p + P5; h = P4 * rotl17(h); return xxfinal(h); }
+/* MathUtil.hpp:190-198: key (x, y) = (p.x, p.y) */
+uint32_t orc_xxhash32_2(uint32_t x, uint32_t y) {
+    uint32_t h = y + P5 + x * P3;
+    h = P4 * rotl17(h);
+    return xxfinal(h);
+}
+/* MathUtil.hpp:212-224 */
+uint32_t orc_xxhash32_4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) {
+    uint32_t h = w + P5 + x * P3;
+    h = P4 * rotl17(h);
+    h += y * P3; h = P4 * rotl17(h);
+    h += z * P3; h = P4 * rotl17(h);
+    return xxfinal(h);
+}
+
+/* UniformSampler (UniformSampler.hpp:17-75), sequence 0 => increment 1. */
+typedef struct { uint64_t s; } pcg_t;
+static inline uint32_t pcg_next(pcg_t *g) {
+    uint64_t old = g->s;
+    g->s = old * 6364136223846793005ULL + 1ULL;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((uint32_t)(-(int32_t)rot) & 31u));
+}
+/* BitManip::normalizedUint (BitManip.hpp:47-50) */
+static inline float u01(uint32_t i) {
+    union { uint32_t u; float f; } c; c.u = (i >> 9u) | 0x3F800000u; return c.f - 1.0f;
+}
+static inline float pcg_1d(pcg_t *g) { return u01(pcg_next(g)); }
+/* set_state: state = seed, then next2D() discards two draws (UniformSampler.hpp:71-75) */
+static inline void pcg_set(pcg_t *g, uint64_t st) { g->s = st; (void)pcg_next(g); (void)pcg_next(g); }
+/* Default constructor seeds 0xBA5EBA11 (UniformSampler.hpp:17-27). */
+static inline void pcg_default(pcg_t *g) { pcg_set(g, 0xBA5EBA11ULL); }
+
+/* next3D() = Vec3f(next1D(), next1D(), next1D()) (UniformSampler.hpp:62-64).
+ * GCC evaluates these constructor arguments right to left: the first draw is z
+ * and the third is x (pinned against the reference headers built by g++). */
+static inline v3f pcg_3d(pcg_t *g) { float z = pcg_1d(g); float y = pcg_1d(g); float x = pcg_1d(g); return V3(x, y, z); }
+
+void orc_cell3d_impulses(uint32_t i, uint32_t j, uint32_t k, uint32_t seed, uint32_t n, float *out) {
+    pcg_t g;
+    /* cell3D: set_state(xxhash32(Vec4u(ijk.z, ijk.y, ijk.x, seed)) + 1u) (SparseConvolutionNoise.cpp:433) */
+    pcg_set(&g, (uint64_t)(uint32_t)(orc_xxhash32_4(k, j, i, seed) + 1u));
+    for (uint32_t q = 0; q < n; ++q) {
+        v3f p = pcg_3d(&g);
+        float w = pcg_1d(&g) < 0.5f ? -1.f : 1.f; /* Bernoulli (MathUtil.hpp:226-228) */
+        out[4 * q + 0] = p.x; out[4 * q + 1] = p.y; out[4 * q + 2] = p.z; out[4 * q + 3] = w;
+    }
+}
+
+void orc_cell1d_impulses(uint32_t i, uint32_t seed, uint32_t n, float *out) {
+    pcg_t g;
+    pcg_set(&g, (uint64_t)(uint32_t)(orc_xxhash32_2(i, seed) + 1u)); /* :466 */
+    for (uint32_t q = 0; q < n; ++q) {
+        float t = pcg_1d(&g);
+        float w = pcg_1d(&g) < 0.5f ? -1.f : 1.f;
+        out[2 * q] = t; out[2 * q + 1] = w;
+    }
+}
+
+/* rand_normal_2 (Gaussian.cpp:22-34): 2*PI is float, the rest double. */
+static void box_muller(pcg_t *g, double *z1, double *z2) {
+    double a = pcg_1d(g);
+    double b = pcg_1d(g);
+    double r = sqrt(-2 * log(1. - a));
+    double two_pi = (double)(2 * PI_F);
+    double x = cos(two_pi * b);
+    double y = sin(two_pi * b);
+    *z1 = r * x; *z2 = r * y;
+}
+void orc_standard_normals(uint64_t state, int npairs, double *out) {
+    pcg_t g; pcg_set(&g, state);
+    for (int m = 0; m < npairs; ++m) box_muller(&g, &out[2 * m], &out[2 * m + 1]);
+}
+
+/* TangentFrame(n) (TangentFrame.hpp:74-85) */
+typedef struct { v3f normal, tangent, bitangent; } frame_t;
+static frame_t make_frame(v3f n) {
+    frame_t F;
+    F.normal = vnorm(n);
+    float sign = copysignf(1.0f, F.normal.z);
+    const float a = -1.0f / (sign + F.normal.z);
+    const float b = F.normal.x * F.normal.y * a;
+    F.tangent = V3(1.0f + sign * F.normal.x * F.normal.x * a, sign * b, -sign * F.normal.x);
+    F.bitangent = V3(b, sign + F.normal.y * F.normal.y * a, -F.normal.y);
+    F.tangent = vnorm(F.tangent);
+    F.bitangent = vnorm(F.bitangent);
+    return F;
+}
+static inline v3f to_local(const frame_t *F, v3f p) { return V3(vdot(F->tangent, p), vdot(F->bitangent, p), vdot(F->normal, p)); }
+static inline v3f to_global(const frame_t *F, v3f p) {
+    return vadd(vadd(vscale(p.x, F->tangent), vscale(p.y, F->bitangent)), vscale(p.z, F->normal));
+}
+void orc_tangent_frame(const float n[3], float out9[9]) {
+    frame_t F = make_frame(V3(n[0], n[1], n[2]));
+    out9[0] = F.tangent.x; out9[1] = F.tangent.y; out9[2] = F.tangent.z;
+    out9[3] = F.bitangent.x; out9[4] = F.bitangent.y; out9[5] = F.bitangent.z;
+    out9[6] = F.normal.x; out9[7] = F.normal.y; out9[8] = F.normal.z;
+}
+
+/* 3x3 float inverse by cofactors and 1/det (Eigen compute_inverse size 3). */
+static m3f m3_inverse(m3f A) {
+    float (*m)[3] = A.m;
+    float c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+    float c10 = m[0][2] * m[2][1] - m[0][1] * m[2][2];
+    float c20 = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+    float det = c00 * m[0][0] + c10 * m[1][0] + c20 * m[2][0];
+    float inv = 1.0f / det;
+    m3f R;
+    R.m[0][0] = c00 * inv; R.m[0][1] = c10 * inv; R.m[0][2] = c20 * inv;
+    R.m[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) * inv;
+    R.m[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * inv;
+    R.m[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * inv;
+    R.m[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * inv;
+    R.m[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * inv;
+    R.m[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * inv;
+    return R;
+}
+static v3f m3_mul(const m3f *A, v3f v) {
+    return V3(A->m[0][0] * v.x + A->m[0][1] * v.y + A->m[0][2] * v.z,
+              A->m[1][0] * v.x + A->m[1][1] * v.y + A->m[1][2] * v.z,
+              A->m[2][0] * v.x + A->m[2][1] * v.y + A->m[2][2] * v.z);
+}
+
+/* ===================================================================== */
+/* Field: covariance, mean, realization                                  */
+/* ===================================================================== */
+
+struct orc_field {
+    orc_medium_desc d;
+    float l_conv;          /* _l_conv = _l*sqrt(2.f)/2 (GPFunctions.cpp:661) */
+    v3f l_aniso, l_aniso_inv;
+    int multires;          /* SparseConvolutionNoise.cpp:47 */
+    int activate;          /* :38 */
+    int scheme1d;          /* :40-43 */
+    float base;            /* :49 */
+};
+
+typedef struct {
+    float value_scale; v3f grad_scale; v3f origin;                 /* coeff_3D */
+    float v1; v3f g1; v3f o1; v3f target_grad; int has_grad;     /* coeff_1D */
+} cond_t;
+
+typedef struct {
+    const orc_field *f;
+    uint32_t seed;         /* computeSeed(info) */
+    uint32_t normal_seed;  /* xxh(pss) + xxh(sceneSeed) + 1 */
+    cond_t c;
+} real_t;
+
+static int is_se(const orc_field *f) { return f->d.cov_kind == ORC_COV_SE; }
+
+/* ---- procedural nonstationary fields (D3: defined in code) ---------- */
+static double ns_ls_value(const orc_field *f, v3d p) {
+    const float *q = f->d.ns_ls_params;
+    switch (f->d.ns_ls_field) {
+    case ORC_LS_SINCOS: {
+        double fr = q[2];
+        return (double)q[0] + (double)q[1] * (0.5 + 0.5 * sin(fr * p.x) * cos(fr * p.z));
+    }
+    default:
+        return q[0];
+    }
+}
+
+/* ---- CovarianceFunction virtuals (GPFunctions.cpp:610-875, 1225-1345, 1600-1760) */
+static float cov_amplitude(const orc_field *f, v3f p) {
+    (void)p;
+    if (is_se(f)) return f->d.sigma;                       /* :721 */
+    return (float)((double)f->d.ns_var * f->d.sigma);       /* :1241-1243 */
+}
+static float cov_lateral(const orc_field *f, v3f p) {
+    if (is_se(f)) return 1.0f;                               /* :610-612 */
+    float l = (float)ns_ls_value(f, d_of(p));                /* getKernelScale :1734-1740 */
+    return fmaxf(fmaxf(l, l), l);
+}
+static float cov_ls_max(const orc_field *f) { return is_se(f) ? 1.f : f->d.ns_ls_max; }
+static float cov_splat_scale(const orc_field *f, v3f p) {  /* nonStationarySplattingKernelScale */
+    if (is_se(f) || f->multires) return 1.f;
+    return cov_lateral(f, p) / cov_ls_max(f);
+}
+static float cov_world_sampling_scale(const orc_field *f) { return cov_ls_max(f); }
+static float se_kernel_radius(const orc_field *f, int identity, float local_scale) { /* :706-719 */
+    float s = f->d.kernel_scale;
+    if (identity) return s;
+    float mf = fmaxf(fmaxf(f->l_aniso.x, f->l_aniso.y), f->l_aniso.z);
+    return s * local_scale * mf;
+}
+static float cov_kernel_radius(const orc_field *f, int identity, float local_scale) {
+    if (!is_se(f)) local_scale *= f->multires ? 1.0f : cov_ls_max(f);   /* :1251-1255 */
+    return se_kernel_radius(f, identity, local_scale);
+}
+static float se_var3d(const orc_field *f, float N, float R, int identity, float gs, float ls) { /* :751-764 */
+    double dens = N / (R * R * R);
+    double det = 1.0;
+    if (!identity) {
+        det = f->l_aniso.x * f->l_aniso.y * f->l_aniso.z;
+        det *= pow(gs, 3);
+    }
+    det *= pow(ls, 3);
+    double integral = pow(M_PI, 1.5) * det;
+    return (float)(dens * integral);
+}
+static float cov_var3d(const orc_field *f, v3f p, float N, float R, int identity, float gs) {
+    if (is_se(f)) return se_var3d(f, N, R, identity, gs, 1.0f);
+    float ls = f->multires ? 1.0f : cov_splat_scale(f, p);
+    return se_var3d(f, N, R, identity, gs, ls);
+}
+static float se_var1d(float N, float R, float ls) { /* :766-770 */
+    double dens = N / R;
+    double integral = sqrt(M_PI) * ls;
+    return (float)(dens * integral);
+}
+static float cov_var1d(const orc_field *f, v3f p, float N, float R) {
+    if (is_se(f)) return se_var1d(N, R, 1.0f);
+    return se_var1d(N, R, cov_splat_scale(f, p));
+}
+/* transforms (:634-644; nonstationary :1267-1285) */
+static float ns_scale(const orc_field *f, float gs) { return (is_se(f) || f->multires) ? gs : cov_ls_max(f); }
+static v3f w2l(const orc_field *f, v3f v, float gs) {
+    float s = ns_scale(f, gs);
+    return vdivs(vmul(f->l_aniso_inv, v), s);
+}
+static v3f grad_l2w(const orc_field *f, v3f g, float gs) {
+    float s = ns_scale(f, gs);
+    return vdivs(vmul(f->l_aniso_inv, g), s);
+}
+static v3f grad_w2l(const orc_field *f, v3f g, float gs) {
+    float s = ns_scale(f, gs);
+    v3f r = vmul(f->l_aniso, g);
+    return V3(r.x * s, r.y * s, r.z * s);
+}
+
+/* getInvCovMtx diagonal (GPFunctions.cpp:784-812), no anisotropy field. */
+static v3f inv_cov_diag(const orc_field *f, int is_cov, int is_iso, float gs, float ls) {
+    v3f m;
+    if (!is_iso) {
+        m = vmul(f->l_aniso_inv, f->l_aniso_inv);
+        float g2 = sqrf(gs);
+        m = V3(m.x / g2, m.y / g2, m.z / g2);
+    } else {
+        m = V3(1.f, 1.f, 1.f);
+    }
+    if (is_cov) m = vscale(0.5f, m);
+    float l2 = sqrf(ls);
+    m = V3(m.x / l2, m.y / l2, m.z / l2);
+    return vscale(0.5f, m);
+}
+/* CovarianceFunction::splattingKernel3D (GPFunctions.cpp:555-565, 814-827) */
+static v4f splat3d(const orc_field *f, v3f pa, v3f pb, int is_cov, int is_iso, float gs, v3f p_world) {
+    float ls = cov_splat_scale(f, p_world);
+    v3f m = inv_cov_diag(f, is_cov, is_iso, gs, ls);
+    v3f ab = vsub(pa, pb);
+    float absq = ab.x * m.x * ab.x + ab.y * m.y * ab.y + ab.z * m.z * ab.z;
+    float v = (float)exp(-(double)absq);
+    return V4(v, -2.f * (ab.x * m.x) * v, -2.f * (ab.y * m.y) * v, -2.f * (ab.z * m.z) * v);
+}
+/* CovarianceFunction::splattingKernel3DGrad (:567-580, 829-843) */
+static v4f splat3d_grad(const orc_field *f, v3f pa, v3f pb, v3f coeff, int is_cov, int is_iso, float gs, v3f p_world) {
+    float ls = cov_splat_scale(f, p_world);
+    v3f m = inv_cov_diag(f, is_cov, is_iso, gs, ls);
+    v3f ab = vsub(pa, pb);
+    float absq = ab.x * m.x * ab.x + ab.y * m.y * ab.y + ab.z * m.z * ab.z;
+    float v = (float)exp(-(double)absq);
+    float dx = -2.f * (ab.x * m.x), dy = -2.f * (ab.y * m.y), dz = -2.f * (ab.z * m.z);
+    float g1x = dx * v, g1y = dy * v, g1z = dz * v;
+    float h00 = (dx * dx - 2.f * m.x) * v, h11 = (dy * dy - 2.f * m.y) * v, h22 = (dz * dz - 2.f * m.z) * v;
+    float h01 = (dx * dy) * v, h02 = (dx * dz) * v, h12 = (dy * dz) * v;
+    v4f vx = V4(g1x, h00, h01, h02), vy = V4(g1y, h01, h11, h12), vz = V4(g1z, h02, h12, h22);
+    return v4add(v4add(v4scale(coeff.x, vx), v4scale(coeff.y, vy)), v4scale(coeff.z, vz));
+}
+/* sparseConvNoiseOneOverSecondDerivative (:723-749), no anisotropy field */
+static m3f one_over_second(const orc_field *f, int is_iso) {
+    m3f M; memset(&M, 0, sizeof M);
+    v3f d = is_iso ? V3(1.f, 1.f, 1.f) : vmul(f->l_aniso_inv, f->l_aniso_inv);
+    M.m[0][0] = d.x * 0.25f * -2.f; M.m[1][1] = d.y * 0.25f * -2.f; M.m[2][2] = d.z * 0.25f * -2.f;
+    return m3_inverse(M);
+}
+/* 1D kernels (GPFunctions.cpp:845-875) */
+static float se_splat1d(float ab, float ls) { float den = (float)(2.0 * sqrf(ls)); return (float)exp(-(double)(ab * ab / den)); }
+static float se_splat1d_g(float ab, float ls) {
+    float fv = se_splat1d(ab, ls);
+    float den = (float)(2.0 * sqrf(ls));
+    return (float)(-2.0 * ab / den * fv);
+}
+static float sqrt2ls(float ls) { return (float)(ls * sqrt(2.0)); }
+static float se_cov1d(float ab, float ls) { return se_splat1d(ab, sqrt2ls(ls)); }
+static float se_cov1d_g(float ab, float ls) { return se_splat1d_g(ab, sqrt2ls(ls)); }
+static float se_cov1d_gg(float ab, float ls) {
+    float fv = se_cov1d(ab, ls);
+    float den = (float)(4.0 * sqrf(ls));
+    float g1 = (float)(-2.0 * ab / den);
+    return (float)((g1 * g1 - 2.0 / den) * fv);
+}
+static float se_cov1d_gg3(float ab, float ls) {
+    float fv = se_cov1d(ab, ls);
+    float den = (float)(4.0 * sqrf(ls));
+    return (float)(-2.0 / den * fv);
+}
+/* getNonstationaryCovSplatCov1D (:1720-1732), no anisotropy field */
+static float ns_cov_splat_cov1d(const orc_field *f, v3f pq, v3f pc) {
+    if (is_se(f)) return 1.0f;
+    float s = (float)(0.5 * (sqrf(cov_splat_scale(f, pq)) + sqrf(cov_splat_scale(f, pc))));
+    return sqrtf(s);
+}
+/* CovarianceFunction::splattingKernel1D (:582-588) */
+static v2f splat1d(const orc_field *f, float pq, float pc, v3f pcw) {
+    float ls = cov_splat_scale(f, pcw);
+    ls *= sqrtf(1.0f);
+    v2f r;
+    if (is_se(f)) { r.x = se_splat1d(pq - pc, 1.0f); r.y = se_splat1d_g(pq - pc, 1.0f); }
+    else { r.x = se_splat1d(pq - pc, ls); r.y = se_splat1d_g(pq - pc, ls); }
+    return r;
+}
+static v2f cov1d(const orc_field *f, float pq, float pc, v3f pqw, v3f pcw) {
+    float ls = ns_cov_splat_cov1d(f, pqw, pcw);
+    v2f r = { se_cov1d(pq - pc, ls), se_cov1d_g(pq - pc, ls) };
+    return r;
+}
+static v2f cov1d_grad(const orc_field *f, float pq, float pc, v3f pqw, v3f pcw) {
+    float ls = ns_cov_splat_cov1d(f, pqw, pcw);
+    v2f r = { se_cov1d_g(pq - pc, ls), se_cov1d_gg(pq - pc, ls) };
+    return r;
+}
+static float cov1d_grad3n(const orc_field *f, float pq, float pc, v3f pqw, v3f pcw) {
+    return se_cov1d_gg3(pq - pc, ns_cov_splat_cov1d(f, pqw, pcw));
+}
+
+/* ---- means (GPFunctions.hpp:952-996 and code-defined box/plane) ----- */
+static double mean_value(const orc_field *f, v3d p) {
+    const orc_medium_desc *d = &f->d;
+    switch (d->mean_kind) {
+    case ORC_MEAN_BOX: {
+        double q[3], o2 = 0.0, mx = -1e300;
+        double pp[3] = {p.x - d->mean_center[0], p.y - d->mean_center[1], p.z - d->mean_center[2]};
+        for (int i = 0; i < 3; ++i) {
+            q[i] = fabs(pp[i]) - d->mean_half[i];
+            double c = q[i] > 0 ? q[i] : 0.0;
+            o2 += c * c;
+            if (q[i] > mx) mx = q[i];
+        }
+        return sqrt(o2) + (mx < 0 ? mx : 0.0);
+    }
+    case ORC_MEAN_PLANE:
+        return d->plane_normal[0] * p.x + d->plane_normal[1] * p.y + d->plane_normal[2] * p.z - d->plane_offset;
+    default: {
+        v3d c = D3(p.x - d->mean_center[0], p.y - d->mean_center[1], p.z - d->mean_center[2]);
+        return sqrt(dlen2(c)) - (double)d->mean_radius;
+    }
+    }
+}
+static v3d mean_grad(const orc_field *f, v3d p) {
+    const orc_medium_desc *d = &f->d;
+    switch (d->mean_kind) {
+    case ORC_MEAN_BOX: {
+        double pp[3] = {p.x - d->mean_center[0], p.y - d->mean_center[1], p.z - d->mean_center[2]};
+        double q[3], g[3] = {0, 0, 0}, o2 = 0.0;
+        int imax = 0;
+        for (int i = 0; i < 3; ++i) {
+            q[i] = fabs(pp[i]) - d->mean_half[i];
+            if (q[i] > q[imax]) imax = i;
+            if (q[i] > 0) o2 += q[i] * q[i];
+        }
+        if (o2 > 0) {
+            double inv = 1.0 / sqrt(o2);
+            for (int i = 0; i < 3; ++i) g[i] = (q[i] > 0 ? q[i] : 0.0) * inv * (pp[i] < 0 ? -1.0 : 1.0);
+        } else {
+            g[imax] = pp[imax] < 0 ? -1.0 : 1.0;
+        }
+        return D3(g[0], g[1], g[2]);
+    }
+    case ORC_MEAN_PLANE:
+        return D3(d->plane_normal[0], d->plane_normal[1], d->plane_normal[2]);
+    default:
+        return dnorm(D3(p.x - d->mean_center[0], p.y - d->mean_center[1], p.z - d->mean_center[2]));
+    }
+}
+
+/* ---- realization (SparseConvolutionNoise.cpp) ------------------------ */
+static int ilog_level(const orc_field *f, float s) { return (int)floorf(logf(s) / logf(f->base)); }
+
+/* kernelScaleLevelRatio (:101-111) */
+static v4f level_ratio(const orc_field *f, v3f p) {
+    float lat = cov_lateral(f, p);
+    float lo = floorf(logf(lat) / logf(f->base));
+    float hi = lo + 1;
+    float slo = powf(f->base, lo);
+    float shi = powf(f->base, hi);
+    float fr = (lat - slo) / (shi - slo);
+    float rlo = (float)((1.0 - fr) / sqrt(1.0 - 2.0 * fr + 2.0 * fr * fr));
+    float rhi = (float)(fr / sqrt(1.0 - 2.0 * fr + 2.0 * fr * fr));
+    return V4(slo, shi, rlo, rhi);
+}
+
+/* cell3D (:432-452) */
+static v4f cell3d(const orc_field *f, v3f pw, uint32_t ci, uint32_t cj, uint32_t ck, v3f p,
+                  uint32_t seed, float N, float R, float kss, int *pairs) {
+    pcg_t g;
+    pcg_set(&g, (uint64_t)(uint32_t)(orc_xxhash32_4(ck, cj, ci, seed) + 1u));
+    uint32_t n = (uint32_t)N;
+    v4f sum = V4(0.f, 0.f, 0.f, 0.f);
+    for (uint32_t k = 0; k < n; ++k) {
+        v3f pi = pcg_3d(&g);
+        float w = pcg_1d(&g) < 0.5f ? -1.f : 1.f;
+        v3f to = vsub(p, pi);
+        if (vlen2(to) < 1.0f) {
+            sum = v4add(sum, v4scale(w, splat3d(f, vscale(R, p), vscale(R, pi), 0, 0, kss, pw)));
+        }
+    }
+    if (pairs) *pairs += (int)n;
+    return sum;
+}
+/* noise3D (:419-430) */
+static v4f noise3d(const orc_field *f, v3f pw, v3f p, uint32_t seed, float N, float R, float kss) {
+    v3f pg = vdivs(p, R);
+    v3f fl = V3(floorf(pg.x), floorf(pg.y), floorf(pg.z));
+    v3f fr = vsub(pg, fl);
+    int ix = (int)fl.x, iy = (int)fl.y, iz = (int)fl.z;
+    v4f sum = V4(0.f, 0.f, 0.f, 0.f);
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz)
+                sum = v4add(sum, cell3d(f, pw, (uint32_t)(ix + dx), (uint32_t)(iy + dy), (uint32_t)(iz + dz),
+                                        vsub(fr, V3((float)dx, (float)dy, (float)dz)), seed, N, R, kss, NULL));
+    return sum;
+}
+/* evaluateNoise3DNormalized (:308-317) */
+static v4f noise3d_normalized(const real_t *r, v3f p, float N, float R, float kss, int cond) {
+    const orc_field *f = r->f;
+    int addl = ilog_level(f, kss);
+    v4f nz = noise3d(f, p, p, r->seed + (uint32_t)addl, N, R, kss);
+    float nf = sqrtf(cov_var3d(f, p, N, R, 0, kss));
+    nz = v4divs(nz, nf);
+    if (f->activate && cond) {
+        v4f a = v4scale(r->c.value_scale, splat3d(f, p, r->c.origin, 1, 0, kss, p));
+        v4f b = splat3d_grad(f, p, r->c.origin, r->c.grad_scale, 1, 0, kss, p);
+        nz = v4add(nz, v4add(a, b));
+    }
+    return nz;
+}
+/* evaluateNoise3D, world space (:142-153) */
+static v4f eval_noise3d(const real_t *r, v3f p, int cond) {
+    const orc_field *f = r->f;
+    float N = f->d.impulse_density;
+    if (!f->multires) {
+        float kss = cov_world_sampling_scale(f);
+        return noise3d_normalized(r, p, N, cov_kernel_radius(f, 0, 1.0f), kss, cond);
+    }
+    v4f info = level_ratio(f, p);
+    v4f lo = noise3d_normalized(r, p, N, cov_kernel_radius(f, 0, info.x), info.x, cond);
+    v4f hi = noise3d_normalized(r, p, N, cov_kernel_radius(f, 0, info.y), info.y, cond);
+    return v4add(v4scale(info.z, lo), v4scale(info.w, hi));
+}
+
+/* cell1D / noise1D (:454-485) */
+static v2f cell1d(const orc_field *f, v3f pw, uint32_t i, float t, uint32_t seed, float N, float R) {
+    pcg_t g;
+    pcg_set(&g, (uint64_t)(uint32_t)(orc_xxhash32_2(i, seed) + 1u));
+    uint32_t n = (uint32_t)N;
+    v2f sum = {0.f, 0.f};
+    for (uint32_t k = 0; k < n; ++k) {
+        float ti = pcg_1d(&g);
+        float w = pcg_1d(&g) < 0.5f ? -1.f : 1.f;
+        float to = t - ti;
+        if (to * to < 1.0f) {
+            v2f s = splat1d(f, R * t, R * ti, pw);
+            sum.x += w * s.x; sum.y += w * s.y;
+        }
+    }
+    return sum;
+}
+static v2f noise1d(const orc_field *f, v3f pw, float t, uint32_t seed, float N, float R) {
+    float tg = t / R;
+    float fl = floorf(tg);
+    float fr = tg - fl;
+    int i = (int)fl;
+    v2f sum = {0.f, 0.f};
+    for (int dx = -1; dx <= 1; ++dx) {
+        v2f c = cell1d(f, pw, (uint32_t)(i + dx), fr - (float)dx, seed, N, R);
+        sum.x += c.x; sum.y += c.y;
+    }
+    return sum;
+}
+/* evaluateNoise1DNormalized (:379-417) */
+static v4f noise1d_normalized(const real_t *r, v3f p, v3f dir, float N, float R, float kss, int cond, int low) {
+    const orc_field *f = r->f;
+    v3f dir_iso = vnorm(w2l(f, dir, 1.0f));
+    frame_t F = make_frame(dir_iso);
+    v3f piso = w2l(f, p, kss);
+    v3f pir = to_local(&F, piso);
+    int addl = ilog_level(f, kss);
+    v2f nz = noise1d(f, p, pir.z, r->seed + (uint32_t)addl, N, R);
+    float nf = sqrtf(cov_var1d(f, p, N, R));
+    nz.x /= nf; nz.y /= nf;
+    float gx = 0.f, gy = 0.f;
+    if (f->activate && cond) {
+        float osf = 1.0f;
+        if (f->multires) { v4f oi = level_ratio(f, r->c.o1); osf = low ? oi.z : oi.w; }
+        v3f oiso = w2l(f, r->c.o1, kss);
+        v3f oir = to_local(&F, oiso);
+        v2f kv = cov1d(f, pir.z, oir.z, p, r->c.o1);
+        v2f kg = cov1d_grad(f, pir.z, oir.z, p, r->c.o1);
+        float gz = kss * r->c.g1.z;
+        v2f vs = {r->c.v1 * kv.x, r->c.v1 * kv.y};
+        v2f gs = {gz * kg.x, gz * kg.y};
+        nz.x += osf * (vs.x + gs.x);
+        nz.y += osf * (vs.y + gs.y);
+        if (f->d.correlation_xy) {
+            gx = osf * kss * r->c.g1.x * cov1d_grad3n(f, pir.z, oir.z, p, r->c.o1);
+            gy = osf * kss * r->c.g1.y * cov1d_grad3n(f, pir.z, oir.z, p, r->c.o1);
+        }
+    }
+    return V4(nz.x, gx, gy, nz.y);
+}
+static float eval_value_noise1d(const real_t *r, v3f p, v3f dir, int cond) { /* :167-176 */
+    const orc_field *f = r->f;
+    float N = f->d.impulse_density;
+    float R = cov_kernel_radius(f, 1, 1.0f);
+    if (!f->multires) return noise1d_normalized(r, p, dir, N, R, 1.0f, cond, 0).x;
+    v4f info = level_ratio(f, p);
+    float lo = noise1d_normalized(r, p, dir, N, R, info.x, cond, 1).x;
+    float hi = noise1d_normalized(r, p, dir, N, R, info.y, cond, 0).x;
+    return info.z * lo + info.w * hi;
+}
+/* correlatedGradientMoments (:13-22) */
+static void corr_moments(float distance, double *factor, double *variance) {
+    const double x = 0.5 * (double)distance * (double)distance;
+    *factor = 0.5 * exp(-0.5 * x) * (1.0 - x);
+    *variance = x < 1.0 ? 0.5 * (-expm1(-x) + exp(-x) * x * (2.0 - x)) : 0.5 - 2.0 * (*factor) * (*factor);
+}
+/* seTransverseGradientMoments (:78-99) */
+static int transverse_moments(const real_t *r, v3f p, v3f dir, int cond, double mean[2], double *var) {
+    const orc_field *f = r->f;
+    if (!f->d.one_d || !f->d.correlation_xy || f->d.ctxt != ORC_CTX_RENEWAL_PLUS || f->multires || !is_se(f))
+        return 0;
+    mean[0] = mean[1] = 0.0;
+    *var = 0.5;
+    if (cond && r->c.has_grad) {
+        v3f delta = w2l(f, vsub(p, r->c.o1), 1.f);
+        double q2 = 0.5 * dlen2(d_of(delta));
+        double rho = exp(-0.5 * q2);
+        *var = -0.5 * expm1(-q2);
+        frame_t F = make_frame(vnorm(w2l(f, dir, 1.f)));
+        v3f obs = to_local(&F, grad_w2l(f, r->c.target_grad, 1.f));
+        mean[0] = rho * obs.x; mean[1] = rho * obs.y;
+    }
+    return 1;
+}
+/* evaluateGradientNoise1D (:178-296) */
+static v3f eval_grad_noise1d(const real_t *r, v3f p, float tseg, v3f dir, int cond) {
+    const orc_field *f = r->f;
+    float N = f->d.impulse_density;
+    float R = cov_kernel_radius(f, 1, 1.0f);
+    v3f dir_iso = vnorm(w2l(f, dir, 1.0f));
+    frame_t F = make_frame(dir_iso);
+    double tm[2], tv;
+    const double s2f = (double)sqrtf(2.f);
+    if (transverse_moments(r, p, dir, cond, tm, &tv)) {
+        v4f base = noise1d_normalized(r, p, dir, N, R, 1.f, cond, 0);
+        double z[2];
+        orc_standard_normals((uint64_t)r->normal_seed, 1, z);
+        double s = sqrt(tv);
+        v3f g = V3((float)(tm[0] + s * z[0]), (float)(tm[1] + s * z[1]), base.w);
+        return grad_l2w(f, to_global(&F, g), 1.f);
+    }
+    float sx = ns_cov_splat_cov1d(f, p, p), sy = ns_cov_splat_cov1d(f, p, p);
+    double xs0 = 1.0 / sx, xs1 = 1.0 / sy;
+    int corr = (f->d.ctxt == ORC_CTX_RENEWAL_PLUS && f->d.correlation_xy);
+    if (!f->multires) {
+        v4f b4 = noise1d_normalized(r, p, dir, N, R, 1.0f, cond, 0);
+        v3f base = V3(b4.y, b4.z, b4.w);
+        pcg_t g; pcg_set(&g, (uint64_t)r->normal_seed);
+        double a0, a1; box_muller(&g, &a0, &a1);
+        a0 /= s2f; a1 /= s2f;
+        v3f gir;
+        if (tseg == 0) {
+            a0 *= xs0; a1 *= xs1;
+            gir = vadd(V3((float)a0, (float)a1, 0.f), base);
+        } else if (!corr) {
+            double c0, c1; box_muller(&g, &c0, &c1);
+            c0 = c0 / s2f * xs0; c1 = c1 / s2f * xs1;
+            gir = vadd(V3((float)c0, (float)c1, 0.f), base);
+        } else {
+            double fac, cv; corr_moments(tseg, &fac, &cv);
+            double m0 = fac * 2.0 * a0, m1 = fac * 2.0 * a1;
+            double c0, c1; box_muller(&g, &c0, &c1);
+            double sc = sqrt(cv);
+            gir = vadd(V3((float)(c0 * sc + m0), (float)(c1 * sc + m1), 0.f), base);
+        }
+        return grad_l2w(f, to_global(&F, gir), 1.0f);
+    }
+    /* multi-resolution (:244-295) */
+    v4f info = level_ratio(f, p);
+    v4f bl4 = noise1d_normalized(r, p, dir, N, R, info.x, cond, 1);
+    v4f bh4 = noise1d_normalized(r, p, dir, N, R, info.y, cond, 0);
+    v3f bl = V3(bl4.y, bl4.z, bl4.w), bh = V3(bh4.y, bh4.z, bh4.w);
+    pcg_t g; pcg_set(&g, (uint64_t)r->normal_seed);
+    double l0, l1, h0, h1;
+    box_muller(&g, &l0, &l1); l0 /= s2f; l1 /= s2f;
+    box_muller(&g, &h0, &h1); h0 /= s2f; h1 /= s2f;
+    v3f gl, gh;
+    if (tseg == 0) {
+        gl = vadd(V3((float)(l0 * xs0), (float)(l1 * xs1), 0.f), bl);
+        gh = vadd(V3((float)(h0 * xs0), (float)(h1 * xs1), 0.f), bh);
+    } else if (!corr) {
+        double a, b, c, e;
+        box_muller(&g, &a, &b); a = a / s2f * xs0; b = b / s2f * xs1;
+        box_muller(&g, &c, &e); c = c / s2f * xs0; e = e / s2f * xs1;
+        gl = vadd(V3((float)a, (float)b, 0.f), bl);
+        gh = vadd(V3((float)c, (float)e, 0.f), bh);
+    } else {
+        double fac, cv; corr_moments(tseg, &fac, &cv);
+        double a, b, c, e; double sc = sqrt(cv);
+        box_muller(&g, &a, &b); box_muller(&g, &c, &e);
+        gl = vadd(V3((float)(a * sc + fac * 2.0 * l0), (float)(b * sc + fac * 2.0 * l1), 0.f), bl);
+        gh = vadd(V3((float)(c * sc + fac * 2.0 * h0), (float)(e * sc + fac * 2.0 * h1), 0.f), bh);
+    }
+    frame_t Fl = make_frame(vnorm(w2l(f, dir, info.x)));
+    frame_t Fh = make_frame(vnorm(w2l(f, dir, info.y)));
+    v3f wl = grad_l2w(f, to_global(&Fl, gl), info.x);
+    v3f wh = grad_l2w(f, to_global(&Fh, gh), info.y);
+    return vadd(vscale(info.z, wl), vscale(info.w, wh));
+}
+
+/* evaluateValue / evaluateGradient (:114-140) */
+static float eval_value(const real_t *r, v3f p, v3f dir) {
+    const orc_field *f = r->f;
+    float amp = cov_amplitude(f, p);
+    float nv = f->d.one_d ? eval_value_noise1d(r, p, dir, 1) : eval_noise3d(r, p, 1).x;
+    double m = mean_value(f, d_of(p));
+    return (float)((double)(amp * nv) + m);
+}
+static v3f eval_gradient(const real_t *r, v3f p, float t, v3f dir) {
+    const orc_field *f = r->f;
+    float amp = cov_amplitude(f, p);
+    v3f ng;
+    if (f->d.one_d) ng = eval_grad_noise1d(r, p, t, dir, 1);
+    else { v4f n4 = eval_noise3d(r, p, 1); ng = V3(n4.y, n4.z, n4.w); }
+    v3f mg = f_of(mean_grad(f, d_of(p)));
+    return vadd(vscale(amp, ng), mg);
+}
+
+/* conditioning3D (:498-574), world space */
+static void conditioning3d(real_t *r, v3f p, v3f dir, float tv, v3f tg) {
+    const orc_field *f = r->f;
+    (void)dir;
+    r->c.value_scale = 0.f; r->c.grad_scale = V3(0.f, 0.f, 0.f); r->c.origin = p;
+    v4f ksi = V4(0, 0, 0, 0);
+    if (f->multires) ksi = level_ratio(f, p);
+    float amp = 0.f;
+    if (f->d.ctxt == ORC_CTX_RENEWAL || f->d.ctxt == ORC_CTX_RENEWAL_PLUS) {
+        amp = cov_amplitude(f, p);
+        if (amp == 0) return;
+        float mean = (float)mean_value(f, d_of(p));
+        float cur = eval_noise3d(r, p, 0).x;
+        r->c.value_scale = (tv - mean) / amp - cur;
+        if (f->multires) r->c.value_scale /= ksi.z + ksi.w;
+    }
+    if (f->d.ctxt == ORC_CTX_RENEWAL_PLUS) {
+        v3f mg = f_of(mean_grad(f, d_of(p)));
+        v4f n4 = eval_noise3d(r, p, 0);
+        v3f cur = V3(n4.y, n4.z, n4.w);
+        v3f delta = vsub(vdivs(vsub(tg, mg), amp), cur);
+        m3f inv = one_over_second(f, 0);
+        r->c.grad_scale = m3_mul(&inv, delta);
+        if (f->multires) {
+            float den = ksi.z / (ksi.x * ksi.x) + ksi.w / (ksi.y * ksi.y);
+            r->c.grad_scale = vdivs(r->c.grad_scale, den);
+        } else {
+            float s = sqrf(cov_lateral(f, p));
+            r->c.grad_scale = V3(r->c.grad_scale.x * s, r->c.grad_scale.y * s, r->c.grad_scale.z * s);
+        }
+    }
+}
+/* conditioning1D (:576-656) */
+static void conditioning1d(real_t *r, v3f p, v3f dir, float tv, v3f tg) {
+    const orc_field *f = r->f;
+    r->c.v1 = 0.f; r->c.g1 = V3(0.f, 0.f, 0.f); r->c.o1 = p; r->c.has_grad = 0;
+    v4f ksi = V4(0, 0, 0, 0);
+    if (f->multires) ksi = level_ratio(f, p);
+    float amp = 0.f;
+    if (f->d.ctxt == ORC_CTX_RENEWAL || f->d.ctxt == ORC_CTX_RENEWAL_PLUS) {
+        amp = cov_amplitude(f, p);
+        if (amp == 0) return;
+        float mean = (float)mean_value(f, d_of(p));
+        float cur = eval_value_noise1d(r, p, dir, 0);
+        r->c.v1 = (tv - mean) / amp - cur;
+        if (f->multires) r->c.v1 /= sqrf(ksi.z) + sqrf(ksi.w);
+    }
+    if (f->d.ctxt == ORC_CTX_RENEWAL_PLUS) {
+        v3f mg = f_of(mean_grad(f, d_of(p)));
+        v3f cur = eval_grad_noise1d(r, p, 0.f, dir, 0);
+        v3f delta = vsub(vdivs(vsub(tg, mg), amp), cur);
+        v3f dir_iso = vnorm(w2l(f, dir, 1.0f));
+        frame_t F = make_frame(dir_iso);
+        v3f gsi = to_local(&F, grad_w2l(f, delta, 1.0f));
+        float sz = ns_cov_splat_cov1d(f, p, p), sx = sz, sy = sz;
+        r->c.g1 = vmul(vscale(-2.f, gsi), V3(sqrf(sx), sqrf(sy), sqrf(sz)));
+        if (f->multires) r->c.g1 = vdivs(r->c.g1, sqrf(ksi.z) + sqrf(ksi.w));
+        if (!f->d.correlation_xy) { r->c.g1.x = 0; r->c.g1.y = 0; }
+        r->c.target_grad = vdivs(vsub(tg, mg), amp);
+        r->c.has_grad = 1;
+    }
+}
+static void conditioning(real_t *r, v3f p, v3f dir, float tv, v3f tg) { /* :488-496 */
+    if (!r->f->activate) return;
+    if (r->f->d.one_d) conditioning1d(r, p, dir, tv, tg);
+    else conditioning3d(r, p, dir, tv, tg);
+}
+/* samplingScheme (:68-76) */
+static int sampling_scheme(const real_t *r, v3f p) {
+    const orc_field *f = r->f;
+    float amp = cov_amplitude(f, p);
+    if (amp < 1e-6f || (f->d.correlation_xy && r->c.has_grad && p.x == r->c.o1.x && p.y == r->c.o1.y &&
+                        p.z == r->c.o1.z && is_se(f)))
+        return ORC_SCHEME_UNI;
+    return f->scheme1d;
+}
+
+/* ===================================================================== */
+/* Public field API                                                      */
+/* ===================================================================== */
+
+orc_field *orc_field_create(const orc_medium_desc *desc) {
+    orc_field *f = (orc_field *)calloc(1, sizeof(orc_field));
+    f->d = *desc;
+    /* SquaredExponentialCovariance::fromJson (GPFunctions.cpp:657-685) */
+    f->l_conv = f->d.length_scale * sqrtf(2.f) / 2;
+    f->l_aniso = vscale(f->l_conv, V3(f->d.aniso[0], f->d.aniso[1], f->d.aniso[2]));
+    f->l_aniso_inv = V3(1.0f / f->l_aniso.x, 1.0f / f->l_aniso.y, 1.0f / f->l_aniso.z);
+    if (isinf(f->l_aniso_inv.x) || isnan(f->l_aniso_inv.x)) f->l_aniso_inv.x = 0;
+    if (isinf(f->l_aniso_inv.y) || isnan(f->l_aniso_inv.y)) f->l_aniso_inv.y = 0;
+    if (isinf(f->l_aniso_inv.z) || isnan(f->l_aniso_inv.z)) f->l_aniso_inv.z = 0;
+    /* SparseConvolutionNoiseRealization ctor (SparseConvolutionNoise.cpp:25-54) */
+    f->activate = !f->d.single_realization && (f->d.ctxt == ORC_CTX_RENEWAL || f->d.ctxt == ORC_CTX_RENEWAL_PLUS);
+    f->scheme1d = ORC_SCHEME_UNI;
+    if (!f->d.single_realization && f->d.one_d) f->scheme1d = f->d.scheme;
+    f->multires = !is_se(f) && f->d.ns_multires;
+    f->base = 2.5f;
+    return f;
+}
+void orc_field_destroy(orc_field *f) { free(f); }
+float orc_kernel_radius(const orc_field *f) { return cov_kernel_radius(f, 0, 1.0f); }
+
+/* computeSeed (:57-66) */
+uint32_t orc_compute_seed(const orc_field *f, const uint32_t pss[4], uint32_t scene_seed) {
+    uint32_t seed = f->d.global_seed;
+    if (!f->d.single_realization) {
+        uint32_t w = f->d.ctxt == ORC_CTX_GLOBAL ? 0u : pss[3];
+        seed += orc_xxhash32_4(pss[0], pss[1], pss[2], w) + orc_xxhash32_1(scene_seed);
+    }
+    return seed;
+}
+static void real_init(real_t *r, const orc_field *f, const uint32_t pss[4], uint32_t scene_seed) {
+    memset(r, 0, sizeof *r);
+    r->f = f;
+    r->seed = orc_compute_seed(f, pss, scene_seed);
+    r->normal_seed = orc_xxhash32_4(pss[0], pss[1], pss[2], pss[3]) + orc_xxhash32_1(scene_seed) + 1u;
+}
+
+void orc_noise3d(const orc_field *f, const float *p, uint32_t seed, float out4[4]) {
+    real_t r; memset(&r, 0, sizeof r); r.f = f; r.seed = seed;
+    v4f n = eval_noise3d(&r, V3(p[0], p[1], p[2]), 0);
+    out4[0] = n.x; out4[1] = n.y; out4[2] = n.z; out4[3] = n.w;
+}
+
+typedef struct {
+    const orc_field *f; const float *xyz; float *out; int64_t n0, n1; uint32_t seed;
+    const orc_ray *rays; orc_flight *fl;
+} job_t;
+
+static void eval_point(const orc_field *f, uint32_t seed, const float *q, float *o) {
+    real_t r; memset(&r, 0, sizeof r); r.f = f; r.seed = seed;
+    v3f p = V3(q[0], q[1], q[2]);
+    o[0] = eval_value(&r, p, V3(0.f, 0.f, 1.f));
+    v3f g = eval_gradient(&r, p, 1.0f, V3(0.f, 0.f, 1.f));
+    o[1] = g.x; o[2] = g.y; o[3] = g.z;
+}
+static void *eval_worker(void *arg) {
+    job_t *j = (job_t *)arg;
+    for (int64_t i = j->n0; i < j->n1; ++i) eval_point(j->f, j->seed, j->xyz + 3 * i, j->out + 4 * i);
+    return NULL;
+}
+void orc_eval_points(const orc_field *f, const float *xyz, int64_t n, uint32_t seed, float *out4, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256]; job_t jb[256];
+    for (int t = 0; t < nthreads; ++t) {
+        jb[t].f = f; jb[t].xyz = xyz; jb[t].out = out4; jb[t].seed = seed;
+        jb[t].n0 = n * t / nthreads; jb[t].n1 = n * (t + 1) / nthreads;
+        pthread_create(&th[t], NULL, eval_worker, &jb[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+}
+
+/* prepareGpisSegment (SparseConvolutionNoiseMedium.cpp:199-263) */
+static void prepare(real_t *r, const orc_field *f, const orc_ray *ray) {
+    real_init(r, f, ray->pss, ray->scene_seed);
+    v3f p = f_of(d_of(V3(ray->origin[0], ray->origin[1], ray->origin[2])));
+    v3f dir = V3(ray->dir[0], ray->dir[1], ray->dir[2]);
+    if (!ray->first_scatter) {
+        v3f tg = V3((float)ray->last_aniso[0], (float)ray->last_aniso[1], (float)ray->last_aniso[2]);
+        conditioning(r, p, dir, (float)(double)ray->last_val, tg);
+    }
+}
+
+void orc_trace_flight(const orc_field *f, const orc_ray *ray, orc_flight *out) {
+    memset(out, 0, sizeof *out);
+    int evals = 0;
+    /* beginDistance (GaussianProcessMedium.cpp:236-287) */
+    float near_t = ray->near_t, far_t = ray->far_t;
+    if (!isfinite(far_t)) far_t = near_t + 2000;
+    float maxT = far_t;
+    out->value = ray->last_val;
+    if (ray->bounce >= f->d.max_bounce) {
+        out->outcome = ORC_FAIL; out->fail_reason = ORC_FAIL_BOUNCE_LIMIT; out->t = maxT;
+        return;
+    }
+    if (maxT <= near_t) {
+        out->outcome = ORC_EXIT; out->t = maxT; out->scheme = ORC_SCHEME_UNI;
+        return;
+    }
+    real_t r;
+    prepare(&r, f, ray);
+    v3f dirf = V3(ray->dir[0], ray->dir[1], ray->dir[2]);
+    v3d p = D3(ray->origin[0], ray->origin[1], ray->origin[2]);
+    v3d rd = d_of(dirf);
+    /* finishGpisSegment, CPU march (SparseConvolutionNoiseMedium.cpp:287-346) */
+    float step_size = (far_t - near_t) / (float)f->d.min_step;
+    if (f->d.step_size < step_size) step_size = f->d.step_size;
+    double t = near_t;
+    double f0 = eval_value(&r, f_of(D3(p.x + t * rd.x, p.y + t * rd.y, p.z + t * rd.z)), dirf); ++evals;
+    int sign0 = f0 < 0 ? -1 : 1;
+    double pf = f0;
+    t = near_t + step_size * ray->jitter;
+    int step = 0, hit = 0;
+    while (t < far_t) {
+        step++;
+        double fc = eval_value(&r, f_of(D3(p.x + t * rd.x, p.y + t * rd.y, p.z + t * rd.z)), dirf); ++evals;
+        int sc = fc < 0 ? -1 : 1;
+        if (!ray->first_scatter && step == 1) {
+            sign0 = sc;
+        } else if (sc != sign0) {
+            double fac = pf / (pf - fc);
+            double a = t - step_size, b = t;
+            double tp = a * (1.0 - fac) + b * fac, tt;
+            for (;;) {
+                tt = a * (1.0 - fac) + b * fac;
+                double ft = eval_value(&r, f_of(D3(p.x + tt * rd.x, p.y + tt * rd.y, p.z + tt * rd.z)), dirf); ++evals;
+                int st = ft < 0 ? -1 : 1;
+                if (st == sign0) break;
+                fac *= 0.9;
+                if (fac <= 0.01) { tp = tt = 0; break; }
+                tp = tt;
+            }
+            t = tp;
+            hit = 1;
+            break;
+        }
+        pf = fc;
+        t += step_size;
+    }
+    out->steps = step;
+    if (hit) {
+        out->value = 0.0f;
+    } else {
+        t = far_t;
+        out->value = eval_value(&r, f_of(D3(p.x + t * rd.x, p.y + t * rd.y, p.z + t * rd.z)), dirf); ++evals;
+    }
+    out->t = t;
+    /* resumeDistance (GaussianProcessMedium.cpp:289-369): gradient, rejection rules */
+    v3d ro = p, rdn = dnorm(rd);
+    v3f ip = f_of(D3(ro.x + rdn.x * t, ro.y + rdn.y * t, ro.z + rdn.z * t));
+    v3f g = eval_gradient(&r, ip, (float)t, dirf); ++evals;
+    out->grad[0] = g.x; out->grad[1] = g.y; out->grad[2] = g.z;
+    out->evals = evals;
+    if (hit) {
+        out->scheme = sampling_scheme(&r, f_of(D3(p.x + t * rd.x, p.y + t * rd.y, p.z + t * rd.z)));
+        v3d gd = d_of(g);
+        double avg = (gd.x + gd.y + gd.z) * (1.0 / 3.0);
+        if (!isfinite(avg)) { out->outcome = ORC_FAIL; out->fail_reason = ORC_FAIL_NONFINITE; return; }
+        if (ddot(gd, d_of(dirf)) > 0) { out->outcome = ORC_FAIL; out->fail_reason = ORC_FAIL_OUTWARD; return; }
+        if (dlen2(gd) < (double)0.0000001f) { out->outcome = ORC_FAIL; out->fail_reason = ORC_FAIL_ZERO_GRAD; return; }
+        out->outcome = ORC_HIT;
+    } else {
+        out->scheme = ORC_SCHEME_UNI;
+        out->outcome = ORC_EXIT;
+    }
+}
+
+static void *flight_worker(void *arg) {
+    job_t *j = (job_t *)arg;
+    for (int64_t i = j->n0; i < j->n1; ++i) orc_trace_flight(j->f, &j->rays[i], &j->fl[i]);
+    return NULL;
+}
+void orc_trace_flights(const orc_field *f, const orc_ray *rays, int64_t n, orc_flight *out, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256]; job_t jb[256];
+    for (int t = 0; t < nthreads; ++t) {
+        jb[t].f = f; jb[t].rays = rays; jb[t].fl = out;
+        jb[t].n0 = n * t / nthreads; jb[t].n1 = n * (t + 1) / nthreads;
+        pthread_create(&th[t], NULL, flight_worker, &jb[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+}
+
+void orc_conditioned_eval(const orc_field *f, const orc_ray *ray, const float q[3], float t_segment, float out4[4]) {
+    real_t r;
+    prepare(&r, f, ray);
+    v3f dir = V3(ray->dir[0], ray->dir[1], ray->dir[2]);
+    v3f p = V3(q[0], q[1], q[2]);
+    out4[0] = eval_value(&r, p, dir);
+    v3f g = eval_gradient(&r, p, t_segment, dir);
+    out4[1] = g.x; out4[2] = g.y; out4[3] = g.z;
+}
+
+/* neeShared / neeGrad / neePDF (SparseConvolutionNoise.cpp:662-821), single-level SE */
+void orc_nee(const orc_field *f, const orc_ray *ray, const float normal_in[3], const float p_in[3],
+             float t_segment, float grad_out[3], float *pdf_out) {
+    real_t r;
+    prepare(&r, f, ray);
+    v3f dir = V3(ray->dir[0], ray->dir[1], ray->dir[2]);
+    v3f normal = V3(normal_in[0], normal_in[1], normal_in[2]);
+    v3f p = V3(p_in[0], p_in[1], p_in[2]);
+    float amp = cov_amplitude(f, p);
+    v3f mg = f_of(mean_grad(f, d_of(p)));
+    v3f dir_iso = vnorm(w2l(f, dir, 1.0f));
+    frame_t F = make_frame(dir_iso);
+    /* mtx_pt = amp * localToWorldInvTranspose * [t b n] */
+    m3f M;
+    v3f cols[3] = {F.tangent, F.bitangent, F.normal};
+    float ltwit[3] = {f->l_aniso_inv.x, f->l_aniso_inv.y, f->l_aniso_inv.z};
+    float sc = is_se(f) ? 1.0f : (f->multires ? 1.0f : cov_ls_max(f));
+    for (int c = 0; c < 3; ++c) {
+        float cv[3] = {cols[c].x, cols[c].y, cols[c].z};
+        for (int rr = 0; rr < 3; ++rr) M.m[rr][c] = (amp * (ltwit[rr] / sc)) * cv[rr];
+    }
+    m3f Minv = m3_inverse(M);
+    v4f gc4 = noise1d_normalized(&r, p, dir, f->d.impulse_density, cov_kernel_radius(f, 1, 1.0f), 1.0f, 1, 0);
+    v3f gc = V3(gc4.y, gc4.z, gc4.w);
+    v3f ppt = vadd(m3_mul(&M, V3(0.f, 0.f, gc.z)), mg);
+    v3f xa = m3_mul(&M, V3(1.f, 0.f, 0.f)), ya = m3_mul(&M, V3(0.f, 1.f, 0.f));
+    v3f cr = vcross(xa, ya);
+    v3f pn = vnorm(cr);
+    float stretch = vlen(cr);
+    float isect = vdot(ppt, pn) / vdot(normal, pn);
+    v3f g3w = vscale(isect, normal);
+    v3f g3 = m3_mul(&Minv, vsub(g3w, mg));
+    grad_out[0] = g3w.x; grad_out[1] = g3w.y; grad_out[2] = g3w.z;
+    if (isect < 0.0) { *pdf_out = 0.f; return; }
+    float sx = ns_cov_splat_cov1d(f, p, p), sy = sx;
+    float pdf_area;
+    double tm[2], tv;
+    if (transverse_moments(&r, p, dir, 1, tm, &tv)) {
+        if (!(tv > 0.)) { *pdf_out = 0.f; return; }
+        double dx = g3.x - tm[0], dy = g3.y - tm[1];
+        pdf_area = (float)(exp(-0.5 * (dx * dx + dy * dy) / tv) / (2.0 * M_PI * tv));
+    } else if (!(f->d.ctxt == ORC_CTX_RENEWAL_PLUS && f->d.correlation_xy) || t_segment == 0.0f) {
+        float smx = (float)((g3.x - gc.x) * sqrt(2.0) * sx);
+        float smy = (float)((g3.y - gc.y) * sqrt(2.0) * sy);
+        pdf_area = (float)(exp(-smx * smx / 2.0) * exp(-smy * smy / 2.0) / (2.0 * M_PI) * 2.0 * sx * sy);
+    } else {
+        double a0, a1;
+        pcg_t g; pcg_set(&g, (uint64_t)r.normal_seed);
+        box_muller(&g, &a0, &a1);
+        double s2f = (double)sqrtf(2.f);
+        a0 /= s2f; a1 /= s2f;
+        double fac, cv; corr_moments(t_segment, &fac, &cv);
+        double m0 = fac * 2.0 * a0, m1 = fac * 2.0 * a1;
+        float smx = (float)((g3.x - gc.x - m0) / sqrt(cv));
+        float smy = (float)((g3.y - gc.y - m1) / sqrt(cv));
+        pdf_area = (float)(exp(-smx * smx / 2.0) * exp(-smy * smy / 2.0) / (2.0 * M_PI) / cv);
+    }
+    float cosl = fabsf(vdot(normal, pn));
+    float pdf_n = pdf_area * (isect * isect) / cosl / stretch;
+    *pdf_out = (float)(pdf_n / (4.0 * vdot(normal, vscale(-1.f, dir))));
+}

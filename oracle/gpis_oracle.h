@@ -1,0 +1,138 @@
+/*
+ * gpis_oracle.h — CPU restatement of the sparse-convolution GPIS ray-flight path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path: only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+ * may load it.  The product (gpis_extend_amd, libgpis_hip.so) never calls it.
+ *
+ * It restates, function by function, the reference's CPU algorithm with the
+ * same float/double choices:
+ *   src/core/math/MathUtil.hpp:179-228            xxhash32, Bernoulli
+ *   src/core/sampling/UniformSampler.hpp:17-75     PCG32 (XSH-RR, inc 1)
+ *   src/core/math/BitManip.hpp:47-50               normalizedUint
+ *   src/core/math/TangentFrame.hpp:74-85           Duff et al. basis
+ *   src/core/sampling/Gaussian.cpp:22-34,89-104    Box-Muller
+ *   src/core/math/GPFunctions.cpp:555-875          SE splat / covariance kernels
+ *   src/core/math/GPFunctions.cpp:1225-1345,1600-1760  nonstationary covariance
+ *   src/core/math/GPFunctions.hpp:952-996          spherical mean
+ *   src/core/math/SparseConvolutionNoise.cpp       realization (all)
+ *   src/core/media/SparseConvolutionNoiseMedium.cpp:199-347  prepare/finish
+ *   src/core/media/GaussianProcessMedium.cpp:222-440         outcome rules
+ */
+#ifndef GPIS_ORACLE_H_
+#define GPIS_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- leaves ---------------------------------------------------------- */
+uint32_t orc_xxhash32_1(uint32_t p);
+uint32_t orc_xxhash32_2(uint32_t x, uint32_t y);
+uint32_t orc_xxhash32_4(uint32_t x, uint32_t y, uint32_t z, uint32_t w);
+/* Impulses of one 3D cell: out[4*k..4*k+3] = (x, y, z, weight).
+ * (i, j, k) are the cell's x, y, z indices cast to uint32. */
+void orc_cell3d_impulses(uint32_t i, uint32_t j, uint32_t k, uint32_t seed,
+                         uint32_t n, float *out);
+/* Impulses of one 1D cell: out[2*k..2*k+1] = (t, weight). */
+void orc_cell1d_impulses(uint32_t i, uint32_t seed, uint32_t n, float *out);
+/* Box-Muller pair after UniformSampler::set_state(state). out[2*m], m < npairs. */
+void orc_standard_normals(uint64_t state, int npairs, double *out);
+void orc_tangent_frame(const float n[3], float out9[9]); /* tangent, bitangent, normal */
+
+/* ---- medium description --------------------------------------------- */
+enum { ORC_COV_SE = 0, ORC_COV_PROC_NS = 1 };
+enum { ORC_MEAN_SPHERE = 0, ORC_MEAN_BOX = 1, ORC_MEAN_PLANE = 2 };
+enum { ORC_CTX_GLOBAL = 0, ORC_CTX_RENEWAL = 1, ORC_CTX_RENEWAL_PLUS = 2, ORC_CTX_NONE = 3 };
+enum { ORC_SCHEME_UNI = 0, ORC_SCHEME_NEE = 1, ORC_SCHEME_MIS = 2 };
+enum { ORC_LS_CONST = 0, ORC_LS_SINCOS = 1 };
+
+typedef struct orc_medium_desc {
+    /* covariance (SquaredExponentialCovariance JSON: sigma, lengthScale, localScale, aniso) */
+    int32_t cov_kind;
+    float sigma, length_scale, kernel_scale;
+    float aniso[3];
+    /* ProceduralNonstationaryCovariance: ls field, multiResolutionGrid */
+    int32_t ns_multires;
+    int32_t ns_ls_field;
+    float ns_ls_params[4];   /* SINCOS: base, amplitude, frequency (x0, x2) */
+    float ns_ls_max;         /* ProceduralVector::maxVal() of the ls field */
+    float ns_var;            /* constant variance field */
+    /* mean */
+    int32_t mean_kind;
+    float mean_radius;
+    double mean_center[3];
+    double mean_half[3];     /* BOX half extents */
+    double plane_normal[3];  /* PLANE: mean = dot(n, p) - offset */
+    double plane_offset;
+    /* SparseConvolutionNoiseMedium JSON */
+    uint32_t global_seed;
+    float impulse_density;
+    int32_t single_realization;
+    int32_t ctxt;
+    int32_t one_d;
+    int32_t scheme;
+    int32_t correlation_xy;
+    float step_size;
+    uint32_t min_step;
+    int32_t max_bounce;
+} orc_medium_desc;
+
+/* One flight request: the fields of GpRaySegmentRequest plus MediumState. */
+typedef struct orc_ray {
+    float origin[3], dir[3];
+    float near_t, far_t, jitter;
+    uint32_t pss[4];          /* RayInfo::pixelSampleSegment */
+    uint32_t scene_seed;
+    int32_t first_scatter;
+    float last_val;           /* MediumState::lastVal */
+    double last_aniso[3];     /* MediumState::lastAniso */
+    int32_t bounce;
+} orc_ray;
+
+enum { ORC_EXIT = 0, ORC_HIT = 1, ORC_FAIL = 2 };
+enum { ORC_FAIL_NONE = 0, ORC_FAIL_OUTWARD = 1, ORC_FAIL_ZERO_GRAD = 2,
+       ORC_FAIL_NONFINITE = 3, ORC_FAIL_BOUNCE_LIMIT = 4 };
+
+typedef struct orc_flight {
+    double t;            /* sampled distance (hit) or farT (exit) */
+    float value;         /* MediumState::lastVal after the flight */
+    float grad[3];       /* sample.aniso: gradient at the hit, or at farT on exit */
+    int32_t outcome;     /* ORC_EXIT / ORC_HIT / ORC_FAIL */
+    int32_t fail_reason;
+    int32_t steps;       /* lattice steps taken */
+    int32_t evals;       /* field evaluations performed (value or value+gradient) */
+    int32_t scheme;      /* sparseConv1DSamplingScheme at the endpoint */
+} orc_flight;
+
+/* ---- field API ------------------------------------------------------- */
+typedef struct orc_field orc_field;
+orc_field *orc_field_create(const orc_medium_desc *desc);
+void orc_field_destroy(orc_field *f);
+/* Realization seed for a RayInfo (computeSeed). */
+uint32_t orc_compute_seed(const orc_field *f, const uint32_t pss[4], uint32_t scene_seed);
+/* Unconditioned normalised noise + world gradient (evaluateNoise3D), per point. */
+void orc_noise3d(const orc_field *f, const float *p, uint32_t seed, float out4[4]);
+/* f(p) and grad f(p) as evaluateValue/evaluateGradient with no conditioning. */
+void orc_eval_points(const orc_field *f, const float *xyz, int64_t n,
+                     uint32_t seed, float *out4, int nthreads);
+/* World-space cell size used by the 3D grid (splattingKernelRadius(false, 1)). */
+float orc_kernel_radius(const orc_field *f);
+/* One full flight (prepare + finish + resumeDistance outcome rules). */
+void orc_trace_flight(const orc_field *f, const orc_ray *ray, orc_flight *out);
+void orc_trace_flights(const orc_field *f, const orc_ray *rays, int64_t n,
+                       orc_flight *out, int nthreads);
+/* Conditioning check: evaluate value and gradient of the conditioned realization
+ * at q, after conditioning at ray origin on (last_val, last_aniso). */
+void orc_conditioned_eval(const orc_field *f, const orc_ray *ray, const float q[3],
+                          float t_segment, float out4[4]);
+/* 1D NEE (neeGrad / neePDF) after conditioning on the ray's origin state. */
+void orc_nee(const orc_field *f, const orc_ray *ray, const float normal[3],
+             const float p[3], float t_segment, float grad_out[3], float *pdf_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

