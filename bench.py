@@ -1,0 +1,187 @@
+"""Flights/s of the sparse-convolution GPIS flight path on one MI355X.
+
+Workload (one "step"): the primary-ray-flights-at-scale configuration -- a
+stationary SE field (sigma 1, lengthScale 0.05, 8 Bernoulli impulses per cell,
+single realization, seed 1) around a sphere SDF of radius 0.5; 2048x2048
+pinhole camera rays (40 deg FOV, eye (0,0,3) looking at the origin) clipped to
+the box [-1,1]^3, each traced as a path of one primary flight plus up to 2
+cosine-weighted diffuse bounces (bounce seed 7), all in one persistent launch.
+Inputs are generated on the host, copied to HBM before timing, and stay there.
+
+Prints one JSON line (see DESIGN.md for the field definitions).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+# Algorithmic FP32 work per 3D value+gradient evaluation, N impulses per cell:
+# 27N*8 (pair test) + (4pi/3)N*24 (in-support kernel, gradient, accumulate) + 16.
+def flops_per_eval_3d(n_imp: int) -> float:
+    return 27 * n_imp * 8 + (4.0 * np.pi / 3.0) * n_imp * 24 + 16
+
+
+PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector peak (packed FMA issue)
+PEAK_FP32_SCALAR_TFLOPS = 78.6    # one FMA per lane per cycle, no packed issue
+
+
+def cpu_baseline(med, rays, max_seconds=12.0):
+    """The oracle (a CPU port of the reference algorithm) on a bounded sample of the
+    same workload: whole paths (primary flight + up to 2 bounces) of the first
+    camera rays, all host cores, flights/s."""
+    from oracle import pyoracle as po
+    from tests.helpers import oracle_field, oracle_rays
+    from gpis_extend_amd import workloads as W
+    from gpis_extend_amd.hashing import Pcg32, xxhash32_4
+
+    F = oracle_field(med)
+    cores = min(os.cpu_count() or 1, 64)
+    n_total = rays["ox"].shape[0]
+    order = np.random.default_rng(12345).permutation(n_total)  # uniform sample over the image
+    sample = 2048
+    flights = 0
+    spent = 0.0
+    done_rays = 0
+    start_idx = 0
+    while spent < max_seconds and start_idx < n_total:
+        idx = np.sort(order[start_idx:start_idx + sample])
+        start_idx += len(idx)
+        cur = W.rays_subset(rays, idx)
+        ray_id = idx.astype(np.uint32)
+        for bounce in range(3):
+            orr = oracle_rays(cur, med)
+            t0 = time.perf_counter()
+            res = F.trace(orr, nthreads=cores)
+            spent += time.perf_counter() - t0
+            flights += len(res)
+            hit = res["outcome"] == po.HIT
+            if bounce == 2 or not hit.any():
+                break
+            # cosine-weighted bounce about the normalised gradient (same rule as the GPU path kernel)
+            sel = np.nonzero(hit)[0]
+            g = res["grad"][sel].astype(np.float32)
+            nrm = g / np.linalg.norm(g, axis=1, keepdims=True)
+            h = xxhash32_4(ray_id[sel], np.full(len(sel), bounce + 1, np.uint32), np.full(len(sel), 7, np.uint32),
+                           np.zeros(len(sel), np.uint32)) + np.uint32(1)
+            pcg = Pcg32(h.astype(np.uint64))
+            u1, u2, jit = pcg.next_1d(), pcg.next_1d(), pcg.next_1d()
+            r_, ph = np.sqrt(u1), np.float32(2 * np.pi) * u2
+            loc = np.stack([r_ * np.cos(ph), r_ * np.sin(ph), np.sqrt(np.maximum(0, 1 - u1))], 1)
+            sgn = np.where(nrm[:, 2] < 0, -1.0, 1.0)
+            a = -1.0 / (sgn + nrm[:, 2]); b = nrm[:, 0] * nrm[:, 1] * a
+            tt = np.stack([1 + sgn * nrm[:, 0] ** 2 * a, sgn * b, -sgn * nrm[:, 0]], 1)
+            bb = np.stack([b, sgn + nrm[:, 1] ** 2 * a, -nrm[:, 1]], 1)
+            d = loc[:, :1] * tt + loc[:, 1:2] * bb + loc[:, 2:] * nrm
+            d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+            o = (np.stack([cur["ox"][sel], cur["oy"][sel], cur["oz"][sel]], 1)
+                 + res["t"][sel, None].astype(np.float32) * np.stack([cur["dx"][sel], cur["dy"][sel], cur["dz"][sel]], 1)).astype(np.float32)
+            _, tf = W.box_interval(o.astype(np.float64), d.astype(np.float64), 1.0)
+            nxt = W._empty_rays(len(sel))
+            nxt["ox"], nxt["oy"], nxt["oz"] = o[:, 0].copy(), o[:, 1].copy(), o[:, 2].copy()
+            nxt["dx"], nxt["dy"], nxt["dz"] = d[:, 0].copy(), d[:, 1].copy(), d[:, 2].copy()
+            nxt["far_t"] = tf.astype(np.float32)
+            nxt["jitter"] = jit.astype(np.float32)
+            nxt["flags"] = np.zeros(len(sel), np.uint32)
+            nxt["prev_gx"], nxt["prev_gy"], nxt["prev_gz"] = g[:, 0].copy(), g[:, 1].copy(), g[:, 2].copy()
+            nxt["pss"] = cur["pss"][sel]
+            nxt["noise_seed"] = cur["noise_seed"][sel]
+            nxt["normal_seed"] = cur["normal_seed"][sel]
+            cur = nxt
+            ray_id = ray_id[sel]
+        done_rays += len(idx)
+    return {"value": flights / spent, "unit": "flights/s", "cores": cores, "kind": "port",
+            "sample": f"{done_rays} camera-ray paths (primary + up to 2 bounces, {flights} flights) of the same "
+                      f"workload through the C oracle on {cores} host threads, {spent:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--width", type=int, default=2048)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--waves-per-cu", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(flights shard by ray index; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from gpis_extend_amd import GpisDevice, RayBatch, SparseConvolutionNoiseMedium, workloads as W
+
+    med = SparseConvolutionNoiseMedium.from_json(W.config1_json())
+    rays = W.camera_rays(med, width=args.width, height=args.width)
+    n = rays["ox"].shape[0]
+    dev = GpisDevice(med)
+    dev.set_domain((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+    if args.waves_per_cu:
+        dev.set_launch(args.waves_per_cu, 0)
+    batch = RayBatch.from_numpy(rays)
+    reply = dev.alloc_path_reply(n)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        dev.trace_paths(batch, 2, 7, 1.0, reply)
+    torch.cuda.synchronize()
+    dev.reset_counters()
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        dev.trace_paths(batch, 2, 7, 1.0, reply)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    c = dev.counters()
+    flights_per_step = c["flights"] / args.steps
+    evals_per_step = c["evals"] / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    value = flights_per_step / (ms_per_step / 1e3)
+    fpe = flops_per_eval_3d(int(med.impulse_density))
+    achieved = evals_per_step * fpe / (kernel_ms / 1e3) / 1e12
+    line = {
+        "metric": "GPIS flights/s on 1 GPU (plain method)",
+        "value": value,
+        "unit": "flights/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain primary-ray flights at scale: SE l=0.05, 8 impulses/cell, "
+                               f"{args.width}x{args.width} pinhole rays + 2 diffuse bounces",
+                   "rays": n, "flights_per_step": flights_per_step, "evals_per_step": evals_per_step,
+                   "hits_per_step": c["hits"] / args.steps, "fails_per_step": c["fails"] / args.steps},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac_scalar_issue_peak": achieved / PEAK_FP32_SCALAR_TFLOPS,
+                     "kernel": "flight_kernel<8,true>", "kernel_ms": kernel_ms,
+                     "flops_per_eval": fpe},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(med, rays)
+    print(json.dumps(line), flush=True)
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -12,6 +12,7 @@ OUT = os.path.join(PKG, "libgpis_hip.so")
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Wno-pass-failed",
 ]
 

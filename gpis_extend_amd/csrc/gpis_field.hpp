@@ -30,7 +30,10 @@ struct RealState {
 __device__ __forceinline__ void pair3(float lx, float ly, float lz, float4 im, float c0, float c1, float c2,
                                       float &v, float &ax, float &ay, float &az) {
     float tx = lx - im.x, ty = ly - im.y, tz = lz - im.z;
-    float r2 = tx * tx + ty * ty + tz * tz;
+    // The support test rounds exactly as Vec::lengthSq: ((x*x + y*y) + z*z), each
+    // operation rounded, no contraction -- so an impulse at the truncation radius
+    // falls on the same side as in the reference.
+    float r2 = __fadd_rn(__fadd_rn(__fmul_rn(tx, tx), __fmul_rn(ty, ty)), __fmul_rn(tz, tz));
     float e = __expf(-(c0 * (tx * tx) + c1 * (ty * ty) + c2 * (tz * tz)));
     float we = (r2 < 1.0f) ? im.w * e : 0.0f;
     v += we;

@@ -189,6 +189,31 @@ __device__ __forceinline__ unsigned mask_rank(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
+// Position p + t*d evaluated as the reference does (Vec3d arithmetic, then Vec3f):
+// one rounded double multiply and add per axis, no contraction.
+__device__ __forceinline__ void ray_point(float ox, float oy, float oz, float dx, float dy, float dz, double t,
+                                          float &qx, float &qy, float &qz) {
+    qx = (float)__dadd_rn((double)ox, __dmul_rn(t, (double)dx));
+    qy = (float)__dadd_rn((double)oy, __dmul_rn(t, (double)dy));
+    qz = (float)__dadd_rn((double)oz, __dmul_rn(t, (double)dz));
+}
+// Gradient position ro + rd*t with rd the double-normalised direction
+// (GaussianProcessMedium.cpp:281, :307).
+__device__ __forceinline__ void grad_point(float ox, float oy, float oz, float dx, float dy, float dz, double t,
+                                           float &qx, float &qy, float &qz) {
+    double l2 = __dadd_rn(__dadd_rn(__dmul_rn((double)dx, (double)dx), __dmul_rn((double)dy, (double)dy)),
+                          __dmul_rn((double)dz, (double)dz));
+    double inv = 1.0 / sqrt(l2);
+    double nx = __dmul_rn((double)dx, inv), ny = __dmul_rn((double)dy, inv), nz = __dmul_rn((double)dz, inv);
+    qx = (float)__dadd_rn((double)ox, __dmul_rn(nx, t));
+    qy = (float)__dadd_rn((double)oy, __dmul_rn(ny, t));
+    qz = (float)__dadd_rn((double)oz, __dmul_rn(nz, t));
+}
+// lerp(a, b, r) = a*(1 - r) + b*r in double (MathUtil.hpp:91-94).
+__device__ __forceinline__ double lerp_d(double a, double b, double r) {
+    return __dadd_rn(__dmul_rn(a, __dsub_rn(1.0, r)), __dmul_rn(b, r));
+}
+
 // Exit distance of the ray o + t d from the box [-b, b]^3 (slab method).
 __device__ __forceinline__ float box_exit(float ox, float oy, float oz, float dx, float dy, float dz, float b) {
     float tx = dx != 0.f ? fmaxf((b - ox) / dx, (-b - ox) / dx) : 3.4e38f;
@@ -276,13 +301,13 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
                         // finishGpisSegment step rule (SparseConvolutionNoiseMedium.cpp:289-291)
                         step = (far_t - near_t) / P.min_step_f;
                         if (P.step_size < step) step = P.step_size;
-                        t0 = near_t + step * jit;
+                        t0 = __fadd_rn(near_t, __fmul_rn(step, jit));
                         if (P.activate && !first) {
                             phase = PH_COND;
                             qx = ox; qy = oy; qz = oz;
                         } else {
                             phase = PH_F0;
-                            qx = fmaf(near_t, dx, ox); qy = fmaf(near_t, dy, oy); qz = fmaf(near_t, dz, oz);
+                            ray_point(ox, oy, oz, dx, dy, dz, (double)near_t, qx, qy, qz);
                         }
                     }
                 }
@@ -342,7 +367,7 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
             }
             rs.cond = 1;
             phase = PH_F0;
-            qx = fmaf(near_t, dx, ox); qy = fmaf(near_t, dy, oy); qz = fmaf(near_t, dz, oz);
+            ray_point(ox, oy, oz, dx, dy, dz, (double)near_t, qx, qy, qz);
             break;
         }
         case PH_F0: {
@@ -351,10 +376,10 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
             k = 0;
             if (t0 < far_t) {
                 phase = PH_MARCH;
-                qx = fmaf(t0, dx, ox); qy = fmaf(t0, dy, oy); qz = fmaf(t0, dz, oz);
+                ray_point(ox, oy, oz, dx, dy, dz, (double)t0, qx, qy, qz);
             } else {
                 phase = PH_FAR;
-                qx = fmaf(far_t, dx, ox); qy = fmaf(far_t, dy, oy); qz = fmaf(far_t, dz, oz);
+                ray_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
             }
             break;
         }
@@ -362,33 +387,32 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
             ++k;
             ++s_steps;
             int sc = fv < 0.f ? -1 : 1;
-            float tk = fmaf((float)(k - 1), step, t0);
+            // lattice t_k = t0 + k*step, exact in double as the reference's `t += step_size`
+            double tk = __dadd_rn((double)t0, __dmul_rn((double)(k - 1), (double)step));
             if (!first && k == 1) {
                 sign0 = sc;
             } else if (sc != sign0) {
                 // secant with 0.9 shrink (SparseConvolutionNoiseMedium.cpp:309-329), in FP64
-                fac = (double)pf / ((double)pf - (double)fv);
-                sa = (double)tk - (double)step;
-                sb = (double)tk;
-                tp = sa * (1.0 - fac) + sb * fac;
+                fac = (double)pf / __dsub_rn((double)pf, (double)fv);
+                sa = __dsub_rn(tk, (double)step);
+                sb = tk;
+                tp = lerp_d(sa, sb, fac);
                 tt = tp;
                 phase = PH_SECANT;
-                qx = (float)((double)ox + tt * (double)dx);
-                qy = (float)((double)oy + tt * (double)dy);
-                qz = (float)((double)oz + tt * (double)dz);
+                ray_point(ox, oy, oz, dx, dy, dz, tt, qx, qy, qz);
                 break;
             }
             pf = fv;
-            float tn = fmaf((float)k, step, t0);
+            double tn = __dadd_rn((double)t0, __dmul_rn((double)k, (double)step));
             if (k >= kMaxSteps) {
                 // step limit (RaySegmentStatus::StepLimit, RaySegmentBatch.cu:281-284): reported invalid
                 ++s_invalid;
                 if (!PATHS) {
-                    out.t[ray] = tn; out.value[ray] = fv; out.gx[ray] = out.gy[ray] = out.gz[ray] = 0.f;
+                    out.t[ray] = (float)tn; out.value[ray] = fv; out.gx[ray] = out.gy[ray] = out.gz[ray] = 0.f;
                     out.outcome[ray] = GPIS_INVALID;
                     if (out.evals) out.evals[ray] = evals_flight;
                 } else {
-                    if (bounce == 0) { pout.t0[ray] = tn; pout.outcome0[ray] = GPIS_INVALID; }
+                    if (bounce == 0) { pout.t0[ray] = (float)tn; pout.outcome0[ray] = GPIS_INVALID; }
                     pout.flights[ray] = path_flights; pout.hits[ray] = path_hits;
                     if (pout.evals) pout.evals[ray] = path_evals + evals_flight;
                 }
@@ -396,11 +420,11 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
                 phase = PH_NEED;
                 break;
             }
-            if (tn < far_t) {
-                qx = fmaf(tn, dx, ox); qy = fmaf(tn, dy, oy); qz = fmaf(tn, dz, oz);
+            if (tn < (double)far_t) {
+                ray_point(ox, oy, oz, dx, dy, dz, tn, qx, qy, qz);
             } else {
                 phase = PH_FAR;
-                qx = fmaf(far_t, dx, ox); qy = fmaf(far_t, dy, oy); qz = fmaf(far_t, dz, oz);
+                ray_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
             }
             break;
         }
@@ -416,10 +440,8 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
                     done = true;
                 } else {
                     tp = tt;
-                    tt = sa * (1.0 - fac) + sb * fac;
-                    qx = (float)((double)ox + tt * (double)dx);
-                    qy = (float)((double)oy + tt * (double)dy);
-                    qz = (float)((double)oz + tt * (double)dz);
+                    tt = lerp_d(sa, sb, fac);
+                    ray_point(ox, oy, oz, dx, dy, dz, tt, qx, qy, qz);
                 }
             }
             if (done) {
@@ -427,9 +449,7 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
                 t_end = (float)tp;
                 val_end = 0.f;
                 phase = PH_GRAD;
-                qx = (float)((double)ox + tp * (double)dx);
-                qy = (float)((double)oy + tp * (double)dy);
-                qz = (float)((double)oz + tp * (double)dz);
+                grad_point(ox, oy, oz, dx, dy, dz, tp, qx, qy, qz);
             }
             break;
         }
@@ -438,7 +458,7 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
             t_end = far_t;
             val_end = fv;
             phase = PH_GRAD;
-            qx = fmaf(far_t, dx, ox); qy = fmaf(far_t, dy, oy); qz = fmaf(far_t, dz, oz);
+            grad_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
             break;
         }
         case PH_GRAD: {
@@ -476,7 +496,9 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
                 if (oc == GPIS_HIT && bounce < max_bounces) {
                     // diffuse bounce: cosine-weighted about the normalised gradient
                     ++bounce;
-                    float hx = fmaf(t_end, dx, ox), hy = fmaf(t_end, dy, oy), hz = fmaf(t_end, dz, oz);
+                    float hx = __fadd_rn(ox, __fmul_rn(t_end, dx));
+                    float hy = __fadd_rn(oy, __fmul_rn(t_end, dy));
+                    float hz = __fadd_rn(oz, __fmul_rn(t_end, dz));
                     float il = rsqrtf(gxv * gxv + gyv * gyv + gzv * gzv);
                     float nx = gxv * il, ny = gyv * il, nz = gzv * il;
                     Pcg g;
@@ -499,14 +521,14 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
                     prev_v = 0.f; prev_gx = gxv; prev_gy = gyv; prev_gz = gzv;
                     first = 0;
                     if (far_t <= near_t) {
-                        ++s_exits; ++path_flights;
+                        ++s_flights; ++s_exits; ++path_flights;
                         phase = PH_NEED;
                     } else {
                         step = (far_t - near_t) / P.min_step_f;
                         if (P.step_size < step) step = P.step_size;
-                        t0 = near_t + step * jit;
+                        t0 = __fadd_rn(near_t, __fmul_rn(step, jit));
                         if (P.activate) { phase = PH_COND; qx = ox; qy = oy; qz = oz; }
-                        else { phase = PH_F0; qx = ox; qy = oy; qz = oz; }
+                        else { phase = PH_F0; ray_point(ox, oy, oz, dx, dy, dz, 0.0, qx, qy, qz); }
                         rs.cond = 0;
                     }
                 } else {

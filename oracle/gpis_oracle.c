@@ -904,6 +904,7 @@ void orc_trace_flight(const orc_field *f, const orc_ray *ray, orc_flight *out) {
     if (f->d.step_size < step_size) step_size = f->d.step_size;
     double t = near_t;
     double f0 = eval_value(&r, f_of(D3(p.x + t * rd.x, p.y + t * rd.y, p.z + t * rd.z)), dirf); ++evals;
+    float minf = ray->first_scatter ? (float)fabs(f0) : INFINITY;
     int sign0 = f0 < 0 ? -1 : 1;
     double pf = f0;
     t = near_t + step_size * ray->jitter;
@@ -912,16 +913,19 @@ void orc_trace_flight(const orc_field *f, const orc_ray *ray, orc_flight *out) {
         step++;
         double fc = eval_value(&r, f_of(D3(p.x + t * rd.x, p.y + t * rd.y, p.z + t * rd.z)), dirf); ++evals;
         int sc = fc < 0 ? -1 : 1;
+        if ((float)fabs(fc) < minf) minf = (float)fabs(fc);
         if (!ray->first_scatter && step == 1) {
             sign0 = sc;
         } else if (sc != sign0) {
             double fac = pf / (pf - fc);
+            out->bracket[0] = (float)pf; out->bracket[1] = (float)fc;
             double a = t - step_size, b = t;
             double tp = a * (1.0 - fac) + b * fac, tt;
             for (;;) {
                 tt = a * (1.0 - fac) + b * fac;
                 double ft = eval_value(&r, f_of(D3(p.x + tt * rd.x, p.y + tt * rd.y, p.z + tt * rd.z)), dirf); ++evals;
                 int st = ft < 0 ? -1 : 1;
+                if ((float)fabs(ft) < minf) minf = (float)fabs(ft);
                 if (st == sign0) break;
                 fac *= 0.9;
                 if (fac <= 0.01) { tp = tt = 0; break; }
@@ -935,6 +939,8 @@ void orc_trace_flight(const orc_field *f, const orc_ray *ray, orc_flight *out) {
         t += step_size;
     }
     out->steps = step;
+    out->min_abs_f = minf;
+    (void)0;
     if (hit) {
         out->value = 0.0f;
     } else {

@@ -105,6 +105,8 @@ typedef struct orc_flight {
     int32_t steps;       /* lattice steps taken */
     int32_t evals;       /* field evaluations performed (value or value+gradient) */
     int32_t scheme;      /* sparseConv1DSamplingScheme at the endpoint */
+    float min_abs_f;     /* smallest |f| of any evaluation whose sign decided the march */
+    float bracket[2];    /* f at the lattice points bracketing the crossing (pf, fc) */
 } orc_flight;
 
 /* ---- field API ------------------------------------------------------- */

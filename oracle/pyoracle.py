@@ -50,7 +50,8 @@ class Flight(C.Structure):
     _fields_ = [
         ("t", C.c_double), ("value", C.c_float), ("grad", C.c_float * 3),
         ("outcome", C.c_int32), ("fail_reason", C.c_int32), ("steps", C.c_int32),
-        ("evals", C.c_int32), ("scheme", C.c_int32),
+        ("evals", C.c_int32), ("scheme", C.c_int32), ("min_abs_f", C.c_float),
+        ("bracket", C.c_float * 2),
     ]
 
 
@@ -60,7 +61,8 @@ RAY_DTYPE = np.dtype([
     ("last_aniso", "<f8", 3), ("bounce", "<i4")], align=True)
 FLIGHT_DTYPE = np.dtype([
     ("t", "<f8"), ("value", "<f4"), ("grad", "<f4", 3), ("outcome", "<i4"), ("fail_reason", "<i4"),
-    ("steps", "<i4"), ("evals", "<i4"), ("scheme", "<i4")], align=True)
+    ("steps", "<i4"), ("evals", "<i4"), ("scheme", "<i4"), ("min_abs_f", "<f4"),
+    ("bracket", "<f4", 2)], align=True)
 assert RAY_DTYPE.itemsize == C.sizeof(Ray), (RAY_DTYPE.itemsize, C.sizeof(Ray))
 assert FLIGHT_DTYPE.itemsize == C.sizeof(Flight), (FLIGHT_DTYPE.itemsize, C.sizeof(Flight))
 

@@ -65,6 +65,9 @@ def test_eval_points_config0(table):
     assert np.max(dg / np.maximum(gmag, 1.0)) < 1e-4
 
 
+GRAZE = 1e-4  # |f| below which a sign decision is within FP32 evaluation noise (measured <= 2e-6)
+
+
 def _flight_compare(js, rays, domain, name):
     from gpis_extend_amd import RayBatch
     med, dev = _dev(js, domain)
@@ -73,35 +76,50 @@ def _flight_compare(js, rays, domain, name):
     got = {k: v.cpu().numpy() for k, v in rep.items()}
     ref = oracle_field(med).trace(oracle_rays(rays, med), nthreads=NTH)
     dev.close()
+    return got, ref
+
+
+def parity_gate(got, ref, name, min_agree=0.9999, rel_tol=1e-4):
+    """Outcome agreement >= min_agree; every disagreement (outcome, or distance
+    beyond rel_tol) must be grazing: the oracle based a sign decision on |f| < GRAZE."""
     oc = got["outcome"] & 0xFF
-    agree = oc == ref["outcome"]
-    both_hit = (oc == po.HIT) & (ref["outcome"] == po.HIT)
-    t_ref = ref["t"]
-    rel = np.abs(got["t"] - t_ref) / np.maximum(np.abs(t_ref), 1e-3)
+    same = oc == ref["outcome"]
+    crossing = same & (ref["outcome"] != po.EXIT)
+    # hit distance: relative error <= rel_tol, with an absolute floor of 1e-6 (about 4 ulp of a
+    # coordinate of magnitude 2) for crossings within a step of the ray origin
+    dt = np.abs(got["t"] - ref["t"])
+    rel = dt / np.maximum(np.abs(ref["t"]), 1e-30)
+    bad = ~same | (crossing & (dt > rel_tol * np.abs(ref["t"]) + 1e-6))
+    # Ill-conditioned crossings: the secant t = a + step*pf/(pf - fc) moves by
+    # step*(|pf| + |fc|)*eps/(pf - fc)^2 when f moves by eps; with eps = 1e-5 (five
+    # times the measured FP32 evaluation error) such a crossing cannot be held to
+    # the distance tolerance and counts as grazing too.
+    pf, fc = ref["bracket"][:, 0].astype(np.float64), ref["bracket"][:, 1].astype(np.float64)
+    step = 0.01
+    sens = step * (np.abs(pf) + np.abs(fc)) * 1e-5 / np.maximum((pf - fc) ** 2, 1e-30)
+    grazing = (ref["min_abs_f"] < GRAZE) | (sens > rel_tol * np.abs(ref["t"]) + 1e-6)
     n = len(oc)
-    print(f"{name}: n={n} outcome agreement={agree.mean():.6f} ({(~agree).sum()} differ); "
-          f"hits={both_hit.sum()} max rel t err={rel[both_hit].max() if both_hit.any() else 0:.3e}; "
+    print(f"{name}: n={n} agree={1 - bad.mean():.6f} ({bad.sum()} differ, {(bad & ~grazing).sum()} non-grazing); "
+          f"crossings={crossing.sum()} max rel t err (agreeing)={rel[crossing & ~bad].max() if (crossing & ~bad).any() else 0:.3e}; "
           f"evals gpu/oracle={got['evals'].sum() / max(ref['evals'].sum(), 1):.4f}")
-    return got, ref, agree, both_hit, rel
-
-
-def _grazing(rays, ref, got, idx, med):
-    """A disagreement is grazing when the oracle's field near the endpoint is flat
-    along the ray: |f| small at the lattice step where the two runs part."""
-    return True
+    for i in np.nonzero(bad)[0][:8]:
+        print(f"   ray {i}: gpu ({oc[i]}, t={got['t'][i]:.7f}) oracle ({ref['outcome'][i]}, t={ref['t'][i]:.7f}) "
+              f"min|f|={ref['min_abs_f'][i]:.2e} bracket=({pf[i]:.2e},{fc[i]:.2e}) sens={sens[i]:.1e}")
+    assert 1 - bad.mean() >= min_agree
+    assert not np.any(bad & ~grazing), "non-grazing disagreement"
+    ex = (oc == po.EXIT) & (ref["outcome"] == po.EXIT)
+    if ex.any():
+        assert np.max(np.abs(got["value"][ex] - ref["value"][ex])) < 1e-4
+    return bad
 
 
 def test_flights_config0_workload2():
     from gpis_extend_amd import workloads as W
     from gpis_extend_amd import SparseConvolutionNoiseMedium
     med = SparseConvolutionNoiseMedium.from_json(W.config0_json())
-    rays = W.sphere_rays(med, n=4096)
-    got, ref, agree, both_hit, rel = _flight_compare(W.config0_json(), rays, ((-2.1,) * 3, (2.1,) * 3), "C0W2")
-    assert agree.mean() >= 0.999
-    assert np.max(rel[both_hit]) <= 1e-4
-    # exit values and gradients
-    ex = ((got["outcome"] & 0xFF) == po.EXIT) & (ref["outcome"] == po.EXIT)
-    assert np.max(np.abs(got["value"][ex] - ref["value"][ex])) < 1e-4
+    rays = W.sphere_rays(med, n=65536)
+    got, ref = _flight_compare(W.config0_json(), rays, ((-2.1,) * 3, (2.1,) * 3), "C0W2")
+    parity_gate(got, ref, "C0W2 (65,536 flights)")
 
 
 def test_flights_config1_camera_subset():
@@ -111,9 +129,8 @@ def test_flights_config1_camera_subset():
     rays = W.camera_rays(med, width=256, height=256)
     idx = np.arange(0, rays["ox"].shape[0], 8)
     rays = W.rays_subset(rays, idx)
-    got, ref, agree, both_hit, rel = _flight_compare(W.config1_json(), rays, ((-1.0,) * 3, (1.0,) * 3), "C1")
-    assert agree.mean() >= 0.999
-    assert np.max(rel[both_hit]) <= 1e-4
+    got, ref = _flight_compare(W.config1_json(), rays, ((-1.0,) * 3, (1.0,) * 3), "C1")
+    parity_gate(got, ref, "C1 primary subset", min_agree=0.999)
 
 
 def test_flights_config2_conditioned_multires_subset():
@@ -121,9 +138,8 @@ def test_flights_config2_conditioned_multires_subset():
     from gpis_extend_amd import SparseConvolutionNoiseMedium
     med = SparseConvolutionNoiseMedium.from_json(W.config2_json())
     rays = W.box_surface_rays(med, n=512)
-    got, ref, agree, both_hit, rel = _flight_compare(W.config2_json(), rays, None, "C2")
-    assert agree.mean() >= 0.99
-    assert np.median(rel[both_hit]) <= 1e-4
+    got, ref = _flight_compare(W.config2_json(), rays, None, "C2")
+    parity_gate(got, ref, "C2 conditioned multires subset", min_agree=0.99)
 
 
 def test_paths_first_flight_equals_single_flights():
