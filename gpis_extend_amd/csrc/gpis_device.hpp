@@ -65,6 +65,7 @@ struct FieldParams {
     float kernel_scale;  // _kernelScale (3)
     int ls_field; float ls_params[4]; float ls_max; float ns_var;
     int N; float Nf;
+    int iso;                  // isotropic kernel: exponent = c * r2
     int seed_add_single;      // floor(log(kss)/log(2.5)) for the single-level grid
     int level_seed_add[64];   // the same for multi-resolution level L, indexed L + 32
     // single-level world-space constants (stationary SE)
@@ -78,7 +79,8 @@ struct FieldParams {
     uint32_t global_seed; int single; int ctxt; int activate; int one_d; int scheme1d; int corr_xy;
     float step_size; float min_step_f; int max_bounce;
     // precomputed impulse table (single realization)
-    const float4 *table; int tlo[3]; int tdim[3]; uint32_t tseed; int has_table;
+    // packed impulse table: per cell 3N floats (x with the weight in its sign bit, y, z)
+    const float *table; int tlo[3]; int tdim[3]; uint32_t tseed; int has_table;
 };
 
 struct Eval4 { float v, gx, gy, gz; };

@@ -13,6 +13,10 @@
 
 #include "gpis_device.hpp"
 
+#ifndef GPIS_PREFETCH_CELLS
+#define GPIS_PREFETCH_CELLS 0  // double-buffer the next cell's impulses (costs 6 float4 registers)
+#endif
+
 namespace gpis {
 
 // Per-flight realization state: the seed and the conditioning coefficients.
@@ -24,18 +28,48 @@ struct RealState {
     float bx, by, bz;    // coeff_3D.gradient_scale
 };
 
-// One impulse-query pair in grid units: local point (lx,ly,lz), impulse (ix,iy,iz,w).
-// The support test |to|^2 < 1 is cell3D's (SparseConvolutionNoise.cpp:444-448);
-// the splat exp(-sum_i c_i to_i^2) is splattingKernel3DVal in grid units.
+// The support test |to|^2 < 1 of cell3D (SparseConvolutionNoise.cpp:444-448), rounded
+// exactly as Vec::lengthSq: ((x*x + y*y) + z*z) with every operation rounded and no
+// contraction, so an impulse at the truncation radius falls on the same side as in
+// the reference.
+__device__ __forceinline__ float support_r2(float tx, float ty, float tz) {
+#pragma clang fp contract(off)
+    float xx = tx * tx;
+    float yy = ty * ty;
+    float zz = tz * tz;
+    return (xx + yy) + zz;
+}
+
+// One impulse-query pair in grid units: local point (lx,ly,lz), impulse (ix,iy,iz,w);
+// the splat exp(-sum_i c_i to_i^2) is splattingKernel3DVal in grid units.  With an
+// isotropic kernel the exponent is c * r2 (c pre-scaled by log2(e) in c0 for exp2).
+template <bool ISO>
 __device__ __forceinline__ void pair3(float lx, float ly, float lz, float4 im, float c0, float c1, float c2,
                                       float &v, float &ax, float &ay, float &az) {
     float tx = lx - im.x, ty = ly - im.y, tz = lz - im.z;
-    // The support test rounds exactly as Vec::lengthSq: ((x*x + y*y) + z*z), each
-    // operation rounded, no contraction -- so an impulse at the truncation radius
-    // falls on the same side as in the reference.
-    float r2 = __fadd_rn(__fadd_rn(__fmul_rn(tx, tx), __fmul_rn(ty, ty)), __fmul_rn(tz, tz));
-    float e = __expf(-(c0 * (tx * tx) + c1 * (ty * ty) + c2 * (tz * tz)));
+    float r2 = support_r2(tx, ty, tz);
+    float e;
+    if (ISO) e = __builtin_amdgcn_exp2f(c0 * r2);
+    else e = __builtin_amdgcn_exp2f(c0 * (tx * tx) + c1 * (ty * ty) + c2 * (tz * tz));
     float we = (r2 < 1.0f) ? im.w * e : 0.0f;
+    v += we;
+    ax += we * tx;
+    ay += we * ty;
+    az += we * tz;
+}
+
+// The same pair with an impulse from the packed table: (x with the weight in its
+// sign bit, y, z).  |x| is an operand modifier and the weight is applied with a
+// bit insert, so the packed form costs no extra instruction.
+template <bool ISO>
+__device__ __forceinline__ void pair3p(float lx, float ly, float lz, float xs, float iy, float iz, float c0, float c1,
+                                       float c2, float &v, float &ax, float &ay, float &az) {
+    float tx = lx - fabsf(xs), ty = ly - iy, tz = lz - iz;
+    float r2 = support_r2(tx, ty, tz);
+    float e;
+    if (ISO) e = __builtin_amdgcn_exp2f(c0 * r2);
+    else e = __builtin_amdgcn_exp2f(c0 * (tx * tx) + c1 * (ty * ty) + c2 * (tz * tz));
+    float we = (r2 < 1.0f) ? copysignf(e, xs) : 0.0f;
     v += we;
     ax += we * tx;
     ay += we * ty;
@@ -44,7 +78,7 @@ __device__ __forceinline__ void pair3(float lx, float ly, float lz, float4 im, f
 
 // Impulses of one cell from the generator (cell3D's RNG sequence, bit-exact):
 // set_state(xxhash32(k, j, i, seed) + 1), then per impulse z, y, x, weight.
-template <int NI>
+template <int NI, bool ISO>
 __device__ __forceinline__ void cell_generated(int cx, int cy, int cz, int n, uint32_t seed, float lx, float ly,
                                                float lz, float c0, float c1, float c2, float &v, float &ax, float &ay,
                                                float &az) {
@@ -58,12 +92,16 @@ __device__ __forceinline__ void cell_generated(int cx, int cy, int cz, int n, ui
         im.y = u01(g.next());
         im.x = u01(g.next());
         im.w = u01(g.next()) < 0.5f ? -1.0f : 1.0f;
-        pair3(lx, ly, lz, im, c0, c1, c2, v, ax, ay, az);
+        pair3<ISO>(lx, ly, lz, im, c0, c1, c2, v, ax, ay, az);
     }
 }
 
 // Raw 27-cell sum at point p for one grid level: returns (sum w*h, sum w*h*to).
-template <int NI>
+// c0..c2 are the per-axis exponent factors already scaled by -log2(e).
+// When all 27 cells lie in the precomputed table the loads of cell j+1 are issued
+// before cell j is summed, so one memory latency is exposed per evaluation rather
+// than one per cell; otherwise each cell comes from the table or the generator.
+template <int NI, bool ISO>
 __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, float py, float pz, uint32_t seed, float R,
                                              float c0, float c1, float c2) {
     float gx = px / R, gy = py / R, gz = pz / R;
@@ -73,6 +111,57 @@ __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, flo
     float v = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
     const int N = NI > 0 ? NI : P.N;
     bool tab = P.has_table && seed == P.tseed;
+    unsigned bx = (unsigned)(ix - 1 - P.tlo[0]), by = (unsigned)(iy - 1 - P.tlo[1]), bz = (unsigned)(iz - 1 - P.tlo[2]);
+    if (NI > 0 && (NI * 3) % 4 == 0 && tab && bx + 2 < (unsigned)P.tdim[0] && by + 2 < (unsigned)P.tdim[1] &&
+        bz + 2 < (unsigned)P.tdim[2]) {
+        // fast path: the whole 3x3x3 block is in the packed table; the loads of cell
+        // j+1 are in flight while cell j is summed
+        constexpr int Q = NI > 0 ? NI * 3 / 4 : 1;  // float4 per cell
+        const unsigned cs = NI * 3;                 // floats per cell
+        const unsigned sy = (unsigned)P.tdim[0] * cs, sz = (unsigned)P.tdim[0] * (unsigned)P.tdim[1] * cs;
+        const float *base = P.table + ((size_t)bz * sz + (size_t)by * sy + (size_t)bx * cs);
+#if GPIS_PREFETCH_CELLS
+        float4 cur[Q], nxt[Q];
+#pragma unroll
+        for (int k = 0; k < Q; ++k) cur[k] = reinterpret_cast<const float4 *>(base)[k];
+#pragma unroll 1
+        for (int j = 0; j < 27; ++j) {
+            int dxi = j / 9, dyi = (j / 3) % 3, dzi = j % 3;  // the reference's dx, dy, dz order
+            if (j < 26) {
+                int jn = j + 1;
+                const float4 *q = reinterpret_cast<const float4 *>(
+                    base + ((size_t)(jn % 3) * sz + (size_t)((jn / 3) % 3) * sy + (size_t)(jn / 9) * cs));
+#pragma unroll
+                for (int k = 0; k < Q; ++k) nxt[k] = q[k];
+            }
+            float lx = rx - (float)(dxi - 1), ly = ry - (float)(dyi - 1), lz = rz - (float)(dzi - 1);
+            const float *f = reinterpret_cast<const float *>(cur);
+#pragma unroll
+            for (int k = 0; k < (NI > 0 ? NI : 1); ++k)
+                pair3p<ISO>(lx, ly, lz, f[3 * k], f[3 * k + 1], f[3 * k + 2], c0, c1, c2, v, ax, ay, az);
+#pragma unroll
+            for (int k = 0; k < Q; ++k) cur[k] = nxt[k];
+        }
+#else
+        // the other resident waves hide each cell's load latency
+#pragma unroll 1
+        for (int j = 0; j < 27; ++j) {
+            int dxi = j / 9, dyi = (j / 3) % 3, dzi = j % 3;  // the reference's dx, dy, dz order
+            const float4 *q = reinterpret_cast<const float4 *>(
+                base + ((size_t)dzi * sz + (size_t)dyi * sy + (size_t)dxi * cs));
+            float4 cur[Q];
+#pragma unroll
+            for (int k = 0; k < Q; ++k) cur[k] = q[k];
+            float lx = rx - (float)(dxi - 1), ly = ry - (float)(dyi - 1), lz = rz - (float)(dzi - 1);
+            const float *f = reinterpret_cast<const float *>(cur);
+#pragma unroll
+            for (int k = 0; k < (NI > 0 ? NI : 1); ++k)
+                pair3p<ISO>(lx, ly, lz, f[3 * k], f[3 * k + 1], f[3 * k + 2], c0, c1, c2, v, ax, ay, az);
+        }
+#endif
+        Eval4 r = {v, ax, ay, az};
+        return r;
+    }
     for (int dx = -1; dx <= 1; ++dx) {
         for (int dy = -1; dy <= 1; ++dy) {
             for (int dz = -1; dz <= 1; ++dz) {
@@ -80,11 +169,13 @@ __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, flo
                 float lx = rx - (float)dx, ly = ry - (float)dy, lz = rz - (float)dz;
                 unsigned ux = (unsigned)(cx - P.tlo[0]), uy = (unsigned)(cy - P.tlo[1]), uz = (unsigned)(cz - P.tlo[2]);
                 if (tab && ux < (unsigned)P.tdim[0] && uy < (unsigned)P.tdim[1] && uz < (unsigned)P.tdim[2]) {
-                    const float4 *q = P.table + ((size_t)(uz * (unsigned)P.tdim[1] + uy) * (unsigned)P.tdim[0] + ux) * N;
+                    const float *q =
+                        P.table + ((size_t)(uz * (unsigned)P.tdim[1] + uy) * (unsigned)P.tdim[0] + ux) * (3 * N);
 #pragma unroll
-                    for (int k = 0; k < N; ++k) pair3(lx, ly, lz, q[k], c0, c1, c2, v, ax, ay, az);
+                    for (int k = 0; k < N; ++k)
+                        pair3p<ISO>(lx, ly, lz, q[3 * k], q[3 * k + 1], q[3 * k + 2], c0, c1, c2, v, ax, ay, az);
                 } else {
-                    cell_generated<NI>(cx, cy, cz, N, seed, lx, ly, lz, c0, c1, c2, v, ax, ay, az);
+                    cell_generated<NI, ISO>(cx, cy, cz, N, seed, lx, ly, lz, c0, c1, c2, v, ax, ay, az);
                 }
             }
         }
@@ -98,6 +189,8 @@ __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, flo
 //   R = splattingKernelRadius(false, gs)         (GPFunctions.cpp:706-719)
 //   m_i = l_aniso_inv_i^2 / gs^2 / ls^2 * 0.5    (getInvCovMtx, :784-812)
 //   variance = N / R^3 * pi^1.5 * prod(l_aniso) * gs^3 * ls^3  (:751-764)
+constexpr float kLog2e = 1.4426950408889634f;
+
 struct Level {
     float R, c0, c1, c2, g0, g1, g2, nf_inv;
     uint32_t seed_add;
@@ -110,7 +203,7 @@ __device__ __forceinline__ Level make_level(const FieldParams &P, float gs, floa
     float m1 = P.l_aniso_inv[1] * P.l_aniso_inv[1] / s2 * 0.5f;
     float m2 = P.l_aniso_inv[2] * P.l_aniso_inv[2] / s2 * 0.5f;
     float R2 = L.R * L.R;
-    L.c0 = R2 * m0; L.c1 = R2 * m1; L.c2 = R2 * m2;
+    L.c0 = -kLog2e * R2 * m0; L.c1 = -kLog2e * R2 * m1; L.c2 = -kLog2e * R2 * m2;
     L.g0 = -2.f * L.R * m0; L.g1 = -2.f * L.R * m1; L.g2 = -2.f * L.R * m2;
     float var = P.Nf / (L.R * L.R * L.R) * 5.568327996831708f * (P.l_aniso[0] * P.l_aniso[1] * P.l_aniso[2]) *
                 (gs * gs * gs) * (ls * ls * ls);
@@ -146,7 +239,7 @@ __device__ __forceinline__ float ls_field(const FieldParams &P, float x, float y
 }
 
 // Normalised noise value and world gradient (evaluateNoise3D, world space).
-template <int NI>
+template <int NI, bool ISO>
 __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealState &rs, 
                                             float px, float py, float pz, bool cond) {
     if (!P.multires) {
@@ -163,7 +256,7 @@ __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealStat
             L = make_level(P, gs, ls, (uint32_t)P.seed_add_single);
             L.R = P.kernel_scale * gs * P.l_max;
         }
-        Eval4 n = noise3d_raw<NI>(P, px, py, pz, rs.seed + L.seed_add, L.R, L.c0, L.c1, L.c2);
+        Eval4 n = noise3d_raw<NI, ISO>(P, px, py, pz, rs.seed + L.seed_add, L.R, L.c0, L.c1, L.c2);
         r.v = n.v * L.nf_inv;
         r.gx = n.gx * L.g0 * L.nf_inv;
         r.gy = n.gy * L.g1 * L.nf_inv;
@@ -184,8 +277,8 @@ __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealStat
     ihi = ihi < 0 ? 0 : (ihi > 63 ? 63 : ihi);
     Level A = make_level(P, slo, 1.0f, (uint32_t)P.level_seed_add[ilo]);
     Level B = make_level(P, shi, 1.0f, (uint32_t)P.level_seed_add[ihi]);
-    Eval4 na = noise3d_raw<NI>(P, px, py, pz, rs.seed + A.seed_add, A.R, A.c0, A.c1, A.c2);
-    Eval4 nb = noise3d_raw<NI>(P, px, py, pz, rs.seed + B.seed_add, B.R, B.c0, B.c1, B.c2);
+    Eval4 na = noise3d_raw<NI, ISO>(P, px, py, pz, rs.seed + A.seed_add, A.R, A.c0, A.c1, A.c2);
+    Eval4 nb = noise3d_raw<NI, ISO>(P, px, py, pz, rs.seed + B.seed_add, B.R, B.c0, B.c1, B.c2);
     Eval4 ra = {na.v * A.nf_inv, na.gx * A.g0 * A.nf_inv, na.gy * A.g1 * A.nf_inv, na.gz * A.g2 * A.nf_inv};
     Eval4 rb = {nb.v * B.nf_inv, nb.gx * B.g0 * B.nf_inv, nb.gy * B.g1 * B.nf_inv, nb.gz * B.g2 * B.nf_inv};
     if (cond && rs.cond) {
@@ -199,10 +292,10 @@ __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealStat
 __device__ __forceinline__ float amplitude(const FieldParams &P) { return P.amp_const; }
 
 // f(p) = amplitude * noise + mean, and its world gradient (evaluateValue/evaluateGradient).
-template <int NI>
+template <int NI, bool ISO>
 __device__ __forceinline__ Eval4 eval_field(const FieldParams &P, const RealState &rs,
                                             float px, float py, float pz, bool cond) {
-    Eval4 n = eval_noise<NI>(P, rs, px, py, pz, cond);
+    Eval4 n = eval_noise<NI, ISO>(P, rs, px, py, pz, cond);
     float mgx, mgy, mgz;
     float m = mean_value(P, px, py, pz, mgx, mgy, mgz);
     float a = amplitude(P);

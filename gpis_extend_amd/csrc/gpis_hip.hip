@@ -21,7 +21,7 @@ using namespace gpis;
 struct gpis_ctx {
     gpis_field_desc desc;
     FieldParams P;
-    float4 *table = nullptr;
+    float *table = nullptr;  // packed impulse table (3N floats per cell)
     size_t table_cells = 0;
     unsigned long long *counters = nullptr;  // device: flights evals steps hits exits fails invalid launches
     unsigned int *work = nullptr;            // device work counter of the persistent kernels
@@ -76,6 +76,7 @@ static gpis_status build_params(gpis_ctx *c) {
     P.ns_var = d.ns_var;
     P.amp_const = d.cov_kind == GPIS_COV_SE ? d.sigma : (float)((double)d.ns_var * d.sigma);
     P.N = (int)(unsigned)d.impulse_density;
+    P.iso = d.aniso[0] == d.aniso[1] && d.aniso[1] == d.aniso[2];
     P.Nf = (float)P.N;
     // single-level grid (stationary SE: kss = 1)
     float kss = d.cov_kind == GPIS_COV_SE ? 1.0f : P.ls_max;
@@ -85,7 +86,7 @@ static gpis_status build_params(gpis_ctx *c) {
     P.invR = 1.0f / P.R;
     for (int i = 0; i < 3; ++i) {
         float m = P.l_aniso_inv[i] * P.l_aniso_inv[i] * 0.5f;
-        P.c2[i] = P.R * P.R * m;
+        P.c2[i] = -1.4426950408889634f * (P.R * P.R * m);  // exp(-x) = exp2(-log2(e) x)
         P.gfac[i] = -2.f * P.R * m;
     }
     double var = (double)(P.Nf / (P.R * P.R * P.R)) * pow(M_PI, 1.5) *
@@ -137,7 +138,31 @@ __global__ void impulse_table_kernel(int lox, int loy, int loz, int dx, int dy, 
     }
 }
 
-template <int NI>
+// The packed impulse table of a box of cells: per cell N x (x with the weight in its
+// sign bit, y, z).  Bit-exact impulse positions; one thread per cell.
+__global__ void table_build_kernel(int lox, int loy, int loz, int dx, int dy, int dz, uint32_t seed, int n,
+                                   float *out) {
+    long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long total = (long long)dx * dy * dz;
+    if (cell >= total) return;
+    int x = (int)(cell % dx);
+    int y = (int)((cell / dx) % dy);
+    int z = (int)(cell / ((long long)dx * dy));
+    Pcg g;
+    g.set((uint64_t)(uint32_t)(xxh4((uint32_t)(loz + z), (uint32_t)(loy + y), (uint32_t)(lox + x), seed) + 1u));
+    float *q = out + cell * 3 * n;
+    for (int k = 0; k < n; ++k) {
+        float iz = u01(g.next());
+        float iy = u01(g.next());
+        float ix = u01(g.next());
+        float w = u01(g.next()) < 0.5f ? -1.0f : 1.0f;
+        q[3 * k] = copysignf(ix, w);
+        q[3 * k + 1] = iy;
+        q[3 * k + 2] = iz;
+    }
+}
+
+template <int NI, bool ISO>
 __global__ __launch_bounds__(256) void eval_points_kernel(FieldParams P, const float *__restrict__ x,
                                                           const float *__restrict__ y, const float *__restrict__ z,
                                                           long long n, uint32_t seed, float *__restrict__ v,
@@ -147,7 +172,7 @@ __global__ __launch_bounds__(256) void eval_points_kernel(FieldParams P, const f
     if (i >= n) return;
     RealState rs = {};
     rs.seed = seed;
-    Eval4 e = eval_field<NI>(P, rs, x[i], y[i], z[i], false);
+    Eval4 e = eval_field<NI, ISO>(P, rs, x[i], y[i], z[i], false);
     v[i] = e.v;
     gx[i] = e.gx;
     gy[i] = e.gy;
@@ -166,6 +191,10 @@ enum Phase : int {
     PH_GRAD = 6,   // gradient at the endpoint
     PH_DEAD = 7    // no more work
 };
+
+#ifndef GPIS_FLIGHT_MIN_WAVES
+#define GPIS_FLIGHT_MIN_WAVES 4  // waves per SIMD the register budget must allow
+#endif
 
 // Lattice steps after which a flight is abandoned and reported invalid.
 constexpr int kMaxSteps = 1 << 22;
@@ -189,29 +218,50 @@ __device__ __forceinline__ unsigned mask_rank(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
+// Exact-rounding helpers: each operation is rounded on its own, as in the reference
+// build (no fused multiply-add), so lattice points and positions match it bit for bit.
+__device__ __forceinline__ float mul_add_exact(float a, float b, float c) {
+#pragma clang fp contract(off)
+    float m = a * b;
+    return c + m;
+}
+
 // Position p + t*d evaluated as the reference does (Vec3d arithmetic, then Vec3f):
-// one rounded double multiply and add per axis, no contraction.
+// one rounded double multiply and add per axis.
 __device__ __forceinline__ void ray_point(float ox, float oy, float oz, float dx, float dy, float dz, double t,
                                           float &qx, float &qy, float &qz) {
-    qx = (float)__dadd_rn((double)ox, __dmul_rn(t, (double)dx));
-    qy = (float)__dadd_rn((double)oy, __dmul_rn(t, (double)dy));
-    qz = (float)__dadd_rn((double)oz, __dmul_rn(t, (double)dz));
+#pragma clang fp contract(off)
+    double mx = t * (double)dx, my = t * (double)dy, mz = t * (double)dz;
+    qx = (float)((double)ox + mx);
+    qy = (float)((double)oy + my);
+    qz = (float)((double)oz + mz);
 }
 // Gradient position ro + rd*t with rd the double-normalised direction
 // (GaussianProcessMedium.cpp:281, :307).
 __device__ __forceinline__ void grad_point(float ox, float oy, float oz, float dx, float dy, float dz, double t,
                                            float &qx, float &qy, float &qz) {
-    double l2 = __dadd_rn(__dadd_rn(__dmul_rn((double)dx, (double)dx), __dmul_rn((double)dy, (double)dy)),
-                          __dmul_rn((double)dz, (double)dz));
+#pragma clang fp contract(off)
+    double xx = (double)dx * (double)dx, yy = (double)dy * (double)dy, zz = (double)dz * (double)dz;
+    double l2 = (xx + yy) + zz;
     double inv = 1.0 / sqrt(l2);
-    double nx = __dmul_rn((double)dx, inv), ny = __dmul_rn((double)dy, inv), nz = __dmul_rn((double)dz, inv);
-    qx = (float)__dadd_rn((double)ox, __dmul_rn(nx, t));
-    qy = (float)__dadd_rn((double)oy, __dmul_rn(ny, t));
-    qz = (float)__dadd_rn((double)oz, __dmul_rn(nz, t));
+    double nx = (double)dx * inv, ny = (double)dy * inv, nz = (double)dz * inv;
+    double mx = nx * t, my = ny * t, mz = nz * t;
+    qx = (float)((double)ox + mx);
+    qy = (float)((double)oy + my);
+    qz = (float)((double)oz + mz);
 }
 // lerp(a, b, r) = a*(1 - r) + b*r in double (MathUtil.hpp:91-94).
 __device__ __forceinline__ double lerp_d(double a, double b, double r) {
-    return __dadd_rn(__dmul_rn(a, __dsub_rn(1.0, r)), __dmul_rn(b, r));
+#pragma clang fp contract(off)
+    double u = a * (1.0 - r);
+    double v = b * r;
+    return u + v;
+}
+// Lattice point t0 + k*step in double (exact: both terms fit in 53 bits).
+__device__ __forceinline__ double lattice_t(float t0, int k, float step) {
+#pragma clang fp contract(off)
+    double m = (double)k * (double)step;
+    return (double)t0 + m;
 }
 
 // Exit distance of the ray o + t d from the box [-b, b]^3 (slab method).
@@ -222,30 +272,97 @@ __device__ __forceinline__ float box_exit(float ox, float oy, float oz, float dx
     return fmaxf(fminf(tx, fminf(ty, tz)), 0.f);
 }
 
-template <int NI, bool PATHS>
-__global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, ReplyOut out, PathOut pout,
-                                                     unsigned int n, unsigned int *work,
-                                                     unsigned long long *stats, int max_bounces,
-                                                     uint32_t bounce_seed, float box) {
+// Conditioning coefficients at a flight origin (conditioning3D, SparseConvolutionNoise.cpp:498-574)
+// in FP64.  With one observation point the value and gradient observations decouple
+// (the covariance gradient vanishes at zero lag), so the Schur complement of the
+// value+gradient observation covariance is the diagonal second-derivative block.
+__device__ __noinline__ void condition_coefficients(const FieldParams &P, const Eval4 &e, float ox, float oy, float oz,
+                                                    float pv, float pgx, float pgy, float pgz, RealState &rs) {
+    float mgx, mgy, mgz;
+    double mean = mean_value(P, ox, oy, oz, mgx, mgy, mgz);
+    double amp = P.amp_const;
+    double cur_v = ((double)e.v - mean) / amp;
+    double a = ((double)pv - mean) / amp - cur_v;
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    double wsum = 1.0, gsum = 1.0;
+    if (P.multires) {
+        // kernelScaleLevelRatio at the origin: value /= wlo + whi, gradient /= wlo/slo^2 + whi/shi^2
+        double lat = ls_field(P, ox, oy, oz);
+        double lvl = floor(log(lat) / log(2.5));
+        double slo = pow(2.5, lvl), shi = pow(2.5, lvl + 1.0);
+        double fr = (lat - slo) / (shi - slo);
+        double den = sqrt(1.0 - 2.0 * fr + 2.0 * fr * fr);
+        double wlo = (1.0 - fr) / den, whi = fr / den;
+        wsum = wlo + whi;
+        gsum = wlo / (slo * slo) + whi / (shi * shi);
+    }
+    if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
+        double del0 = ((double)pgx - mgx) / amp - ((double)e.gx - mgx) / amp;
+        double del1 = ((double)pgy - mgy) / amp - ((double)e.gy - mgy) / amp;
+        double del2 = ((double)pgz - mgz) / amp - ((double)e.gz - mgz) / amp;
+        // inverse second derivative of the covariance at zero lag: -2 / l_aniso_inv_i^2
+        b0 = -2.0 / ((double)P.l_aniso_inv[0] * P.l_aniso_inv[0]) * del0 / gsum;
+        b1 = -2.0 / ((double)P.l_aniso_inv[1] * P.l_aniso_inv[1]) * del1 / gsum;
+        b2 = -2.0 / ((double)P.l_aniso_inv[2] * P.l_aniso_inv[2]) * del2 / gsum;
+    }
+    rs.ox = ox; rs.oy = oy; rs.oz = oz;
+    rs.a = (float)(a / wsum);
+    rs.bx = (float)b0; rs.by = (float)b1; rs.bz = (float)b2;
+    rs.cond = 1;
+}
+
+// Cosine-weighted diffuse direction about the unit normal n (TangentFrame basis).
+__device__ __noinline__ void diffuse_bounce(uint32_t ray, int bounce, uint32_t bounce_seed, float nx, float ny,
+                                            float nz, float &dx, float &dy, float &dz, float &jit) {
+    Pcg g;
+    g.set((uint64_t)(uint32_t)(xxh4(ray, (uint32_t)bounce, bounce_seed, 0u) + 1u));
+    float u1 = u01(g.next()), u2 = u01(g.next());
+    jit = u01(g.next());
+    float rr = sqrtf(u1), ph = 6.2831853071795864f * u2;
+    float lx = rr * cosf(ph), ly = rr * sinf(ph), lz = sqrtf(fmaxf(0.f, 1.f - u1));
+    float sgn = copysignf(1.0f, nz);
+    float aa = -1.0f / (sgn + nz), bb = nx * ny * aa;
+    float tx = 1.0f + sgn * nx * nx * aa, ty = sgn * bb, tz = -sgn * nx;
+    float bx = bb, by = sgn + ny * ny * aa, bz = -ny;
+    dx = tx * lx + bx * ly + nx * lz;
+    dy = ty * lx + by * ly + ny * lz;
+    dz = tz * lx + bz * ly + nz * lz;
+    float inl = rsqrtf(dx * dx + dy * dy + dz * dz);
+    dx *= inl; dy *= inl; dz *= inl;
+}
+
+// Block statistics kept in LDS and flushed once per block.
+enum { ST_FLIGHTS = 0, ST_EVALS, ST_STEPS, ST_HITS, ST_EXITS, ST_FAILS, ST_INVALID, ST_N };
+
+// The persistent flight kernel.  Lanes run independent per-flight state machines;
+// every iteration each live lane performs one field evaluation.  Lanes whose flight
+// (or path) finished take the next ray indices from a global counter: one atomic
+// per wave, ranks from the ballot mask.  COND: the medium conditions secondary
+// flights (Renewal/Renewal+); PATHS: hits continue as diffuse bounces.
+template <int NI, bool ISO, bool COND, bool PATHS>
+__global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(FieldParams P, RayIn in, ReplyOut out,
+                                                                            PathOut pout, unsigned int n,
+                                                                            unsigned int *work,
+                                                                            unsigned long long *stats,
+                                                                            int max_bounces, uint32_t bounce_seed,
+                                                                            float box) {
+    __shared__ unsigned int bstat[ST_N];
+    if (threadIdx.x < ST_N) bstat[threadIdx.x] = 0;
+    __syncthreads();
     const unsigned lane = lane_id();
     int phase = PH_NEED;
     unsigned ray = 0;
-    // flight state
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, near_t = 0, far_t = 0, step = 0, t0 = 0;
     float qx = 0, qy = 0, qz = 0;
-    int first = 1, sign0 = 1, k = 0;
-    float pf = 0.f;
-    double sa = 0, sb = 0, fac = 0, tp = 0, tt = 0;
-    float t_end = 0, val_end = 0;
-    int hit = 0;
+    float pf = 0.f, t_end = 0.f, val_end = 0.f;
+    int first = 1, sign0 = 1, k = 0, hit = 0;
+    double tk = 0, fac = 0, tp = 0, tt = 0;
     float prev_v = 0, prev_gx = 0, prev_gy = 0, prev_gz = 0;
     RealState rs = {};
     unsigned evals_flight = 0;
-    // path state
     int bounce = 0;
-    unsigned path_flights = 0, path_hits = 0, path_evals = 0;
-    // statistics
-    unsigned s_flights = 0, s_evals = 0, s_steps = 0, s_hits = 0, s_exits = 0, s_fails = 0, s_invalid = 0;
+    unsigned path_fh = 0;  // flights (low 16 bits) and hits (high 16 bits) of the current path
+    unsigned path_evals = 0;
 
     for (;;) {
         // ---- refill: lanes that finished take the next rays (ballot + prefix sum)
@@ -269,40 +386,46 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
                     rs.seed = in.noise_seed[r];
                     rs.cond = 0;
                     first = (in.flags[r] & 1u) ? 1 : 0;
-                    if (!first) { prev_v = in.pv[r]; prev_gx = in.pgx[r]; prev_gy = in.pgy[r]; prev_gz = in.pgz[r]; }
-                    bounce = 0; path_flights = 0; path_hits = 0; path_evals = 0;
-                    evals_flight = 0;
+                    if (COND && !first) { prev_v = in.pv[r]; prev_gx = in.pgx[r]; prev_gy = in.pgy[r]; prev_gz = in.pgz[r]; }
+                    bounce = 0; path_fh = 0; path_evals = 0; evals_flight = 0;
                     // beginDistance (GaussianProcessMedium.cpp:244-265)
                     if (!isfinite(far_t)) far_t = near_t + 2000.f;
                     bool bad = !isfinite(ox) || !isfinite(oy) || !isfinite(oz) || !isfinite(dx) || !isfinite(dy) ||
                                !isfinite(dz) || !isfinite(near_t) || !(jit >= 0.f && jit < 1.f);
                     if (bad) {
-                        out.t[r] = far_t; out.value[r] = 0.f; out.gx[r] = out.gy[r] = out.gz[r] = 0.f;
-                        out.outcome[r] = GPIS_INVALID;
-                        if (out.evals) out.evals[r] = 0;
-                        if (PATHS) { pout.flights[r] = 0; pout.hits[r] = 0; pout.outcome0[r] = GPIS_INVALID; pout.t0[r] = far_t; }
-                        ++s_invalid;
+                        if (!PATHS) {
+                            out.t[r] = far_t; out.value[r] = 0.f; out.gx[r] = out.gy[r] = out.gz[r] = 0.f;
+                            out.outcome[r] = GPIS_INVALID;
+                            if (out.evals) out.evals[r] = 0;
+                        } else {
+                            pout.t0[r] = far_t; pout.outcome0[r] = GPIS_INVALID;
+                            pout.gx0[r] = pout.gy0[r] = pout.gz0[r] = 0.f;
+                            pout.flights[r] = 0; pout.hits[r] = 0;
+                            if (pout.evals) pout.evals[r] = 0;
+                        }
+                        atomicAdd(&bstat[ST_INVALID], 1u);
                         phase = PH_NEED;
                     } else if (far_t <= near_t) {
                         // exited without a flight: no RNG draw, no evaluation
                         if (!PATHS) {
-                            out.t[r] = far_t; out.value[r] = first ? 0.f : prev_v;
+                            out.t[r] = far_t; out.value[r] = first ? 0.f : in.pv[r];
                             out.gx[r] = out.gy[r] = out.gz[r] = 0.f;
                             out.outcome[r] = GPIS_EXIT;
                             if (out.evals) out.evals[r] = 0;
                         } else {
-                            pout.t0[r] = far_t; pout.outcome0[r] = GPIS_EXIT; pout.flights[r] = 0; pout.hits[r] = 0;
+                            pout.t0[r] = far_t; pout.outcome0[r] = GPIS_EXIT;
                             pout.gx0[r] = pout.gy0[r] = pout.gz0[r] = 0.f;
+                            pout.flights[r] = 0; pout.hits[r] = 0;
                             if (pout.evals) pout.evals[r] = 0;
                         }
-                        ++s_exits;
+                        atomicAdd(&bstat[ST_EXITS], 1u);
                         phase = PH_NEED;
                     } else {
                         // finishGpisSegment step rule (SparseConvolutionNoiseMedium.cpp:289-291)
                         step = (far_t - near_t) / P.min_step_f;
                         if (P.step_size < step) step = P.step_size;
-                        t0 = __fadd_rn(near_t, __fmul_rn(step, jit));
-                        if (P.activate && !first) {
+                        t0 = mul_add_exact(step, jit, near_t);
+                        if (COND && P.activate && !first) {
                             phase = PH_COND;
                             qx = ox; qy = oy; qz = oz;
                         } else {
@@ -317,60 +440,15 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
         if (phase == PH_NEED || phase == PH_DEAD) continue;
 
         // ---- one evaluation per live lane
-        Eval4 e = eval_field<NI>(P, rs, qx, qy, qz, phase != PH_COND);
+        Eval4 e = eval_field<NI, ISO>(P, rs, qx, qy, qz, !COND || phase != PH_COND);
         ++evals_flight;
-        float fv = e.v;
+        const float fv = e.v;
 
-        switch (phase) {
-        case PH_COND: {
-            // conditioning3D (SparseConvolutionNoise.cpp:498-574) in FP64: the value and
-            // gradient observation at one point decouple (the covariance gradient vanishes
-            // at zero lag), so the Schur complement is the diagonal second-derivative block.
-            float mgx, mgy, mgz;
-            double mean = mean_value(P, ox, oy, oz, mgx, mgy, mgz);
-            double amp = P.amp_const;
-            double cur_v = ((double)e.v - mean) / amp;
-            double a = ((double)prev_v - mean) / amp - cur_v;
-            rs.ox = ox; rs.oy = oy; rs.oz = oz;
-            rs.a = (float)a; rs.bx = rs.by = rs.bz = 0.f;
-            if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
-                double ngx = ((double)e.gx - mgx) / amp, ngy = ((double)e.gy - mgy) / amp, ngz = ((double)e.gz - mgz) / amp;
-                double del0 = ((double)prev_gx - mgx) / amp - ngx;
-                double del1 = ((double)prev_gy - mgy) / amp - ngy;
-                double del2 = ((double)prev_gz - mgz) / amp - ngz;
-                // H^-1 of the covariance second derivative: -1 / (2 m_i) with m_i = l_aniso_inv_i^2 / 4
-                double h0 = -2.0 / ((double)P.l_aniso_inv[0] * P.l_aniso_inv[0]);
-                double h1 = -2.0 / ((double)P.l_aniso_inv[1] * P.l_aniso_inv[1]);
-                double h2 = -2.0 / ((double)P.l_aniso_inv[2] * P.l_aniso_inv[2]);
-                double b0 = h0 * del0, b1 = h1 * del1, b2 = h2 * del2;
-                if (P.multires) {
-                    // kernelScaleLevelRatio at the origin: value /= wlo + whi, gradient /= wlo/slo^2 + whi/shi^2
-                    double lat = ls_field(P, ox, oy, oz);
-                    double lvl = floor(log(lat) / log(2.5));
-                    double slo = pow(2.5, lvl), shi = pow(2.5, lvl + 1.0);
-                    double fr = (lat - slo) / (shi - slo);
-                    double den = sqrt(1.0 - 2.0 * fr + 2.0 * fr * fr);
-                    double wlo = (1.0 - fr) / den, whi = fr / den;
-                    a /= (wlo + whi);
-                    double g = wlo / (slo * slo) + whi / (shi * shi);
-                    b0 /= g; b1 /= g; b2 /= g;
-                    rs.a = (float)a;
-                }
-                rs.bx = (float)b0; rs.by = (float)b1; rs.bz = (float)b2;
-            } else if (P.multires) {
-                double lat = ls_field(P, ox, oy, oz);
-                double lvl = floor(log(lat) / log(2.5));
-                double slo = pow(2.5, lvl), shi = pow(2.5, lvl + 1.0);
-                double fr = (lat - slo) / (shi - slo);
-                double den = sqrt(1.0 - 2.0 * fr + 2.0 * fr * fr);
-                rs.a = (float)(a / ((1.0 - fr) / den + fr / den));
-            }
-            rs.cond = 1;
+        if (COND && phase == PH_COND) {
+            condition_coefficients(P, e, ox, oy, oz, prev_v, prev_gx, prev_gy, prev_gz, rs);
             phase = PH_F0;
             ray_point(ox, oy, oz, dx, dy, dz, (double)near_t, qx, qy, qz);
-            break;
-        }
-        case PH_F0: {
+        } else if (phase == PH_F0) {
             sign0 = fv < 0.f ? -1 : 1;
             pf = fv;
             k = 0;
@@ -381,57 +459,50 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
                 phase = PH_FAR;
                 ray_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
             }
-            break;
-        }
-        case PH_MARCH: {
+        } else if (phase == PH_MARCH) {
             ++k;
-            ++s_steps;
             int sc = fv < 0.f ? -1 : 1;
             // lattice t_k = t0 + k*step, exact in double as the reference's `t += step_size`
-            double tk = __dadd_rn((double)t0, __dmul_rn((double)(k - 1), (double)step));
+            tk = lattice_t(t0, k - 1, step);
             if (!first && k == 1) {
                 sign0 = sc;
             } else if (sc != sign0) {
                 // secant with 0.9 shrink (SparseConvolutionNoiseMedium.cpp:309-329), in FP64
-                fac = (double)pf / __dsub_rn((double)pf, (double)fv);
-                sa = __dsub_rn(tk, (double)step);
-                sb = tk;
-                tp = lerp_d(sa, sb, fac);
+                fac = (double)pf / ((double)pf - (double)fv);
+                tp = lerp_d(tk - (double)step, tk, fac);
                 tt = tp;
                 phase = PH_SECANT;
                 ray_point(ox, oy, oz, dx, dy, dz, tt, qx, qy, qz);
-                break;
             }
-            pf = fv;
-            double tn = __dadd_rn((double)t0, __dmul_rn((double)k, (double)step));
-            if (k >= kMaxSteps) {
-                // step limit (RaySegmentStatus::StepLimit, RaySegmentBatch.cu:281-284): reported invalid
-                ++s_invalid;
-                if (!PATHS) {
-                    out.t[ray] = (float)tn; out.value[ray] = fv; out.gx[ray] = out.gy[ray] = out.gz[ray] = 0.f;
-                    out.outcome[ray] = GPIS_INVALID;
-                    if (out.evals) out.evals[ray] = evals_flight;
+            if (phase == PH_MARCH) {
+                pf = fv;
+                double tn = lattice_t(t0, k, step);
+                if (k >= kMaxSteps) {
+                    // step limit (RaySegmentStatus::StepLimit, RaySegmentBatch.cu:281-284): reported invalid
+                    atomicAdd(&bstat[ST_INVALID], 1u);
+                    atomicAdd(&bstat[ST_STEPS], (unsigned)k);
+                    if (!PATHS) {
+                        out.t[ray] = (float)tn; out.value[ray] = fv;
+                        out.gx[ray] = out.gy[ray] = out.gz[ray] = 0.f;
+                        out.outcome[ray] = GPIS_INVALID;
+                        if (out.evals) out.evals[ray] = evals_flight;
+                    } else {
+                        if (bounce == 0) { pout.t0[ray] = (float)tn; pout.outcome0[ray] = GPIS_INVALID; }
+                        pout.flights[ray] = path_fh & 0xFFFFu; pout.hits[ray] = path_fh >> 16;
+                        if (pout.evals) pout.evals[ray] = path_evals + evals_flight;
+                    }
+                    evals_flight = 0;
+                    phase = PH_NEED;
+                } else if (tn < (double)far_t) {
+                    ray_point(ox, oy, oz, dx, dy, dz, tn, qx, qy, qz);
                 } else {
-                    if (bounce == 0) { pout.t0[ray] = (float)tn; pout.outcome0[ray] = GPIS_INVALID; }
-                    pout.flights[ray] = path_flights; pout.hits[ray] = path_hits;
-                    if (pout.evals) pout.evals[ray] = path_evals + evals_flight;
+                    phase = PH_FAR;
+                    ray_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
                 }
-                evals_flight = 0;
-                phase = PH_NEED;
-                break;
             }
-            if (tn < (double)far_t) {
-                ray_point(ox, oy, oz, dx, dy, dz, tn, qx, qy, qz);
-            } else {
-                phase = PH_FAR;
-                ray_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
-            }
-            break;
-        }
-        case PH_SECANT: {
-            int st = fv < 0.f ? -1 : 1;
+        } else if (phase == PH_SECANT) {
             bool done = false;
-            if (st == sign0) {
+            if ((fv < 0.f ? -1 : 1) == sign0) {
                 done = true;
             } else {
                 fac *= 0.9;
@@ -440,7 +511,7 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
                     done = true;
                 } else {
                     tp = tt;
-                    tt = lerp_d(sa, sb, fac);
+                    tt = lerp_d(tk - (double)step, tk, fac);
                     ray_point(ox, oy, oz, dx, dy, dz, tt, qx, qy, qz);
                 }
             }
@@ -451,110 +522,101 @@ __global__ __launch_bounds__(256) void flight_kernel(FieldParams P, RayIn in, Re
                 phase = PH_GRAD;
                 grad_point(ox, oy, oz, dx, dy, dz, tp, qx, qy, qz);
             }
-            break;
-        }
-        case PH_FAR: {
+        } else if (phase == PH_FAR) {
             hit = 0;
             t_end = far_t;
             val_end = fv;
             phase = PH_GRAD;
             grad_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
-            break;
-        }
-        case PH_GRAD: {
+        } else if (phase == PH_GRAD) {
             // resumeDistance outcome rules (GaussianProcessMedium.cpp:305-348)
             uint32_t outcome;
-            float gxv = e.gx, gyv = e.gy, gzv = e.gz;
+            const float gxv = e.gx, gyv = e.gy, gzv = e.gz;
             if (hit) {
-                float avg = (gxv + gyv + gzv);
-                if (!isfinite(avg)) outcome = GPIS_FAIL | (GPIS_FAIL_NONFINITE << 8);
+                if (!isfinite(gxv + gyv + gzv)) outcome = GPIS_FAIL | (GPIS_FAIL_NONFINITE << 8);
                 else if (gxv * dx + gyv * dy + gzv * dz > 0.f) outcome = GPIS_FAIL | (GPIS_FAIL_OUTWARD << 8);
                 else if (gxv * gxv + gyv * gyv + gzv * gzv < 1e-7f) outcome = GPIS_FAIL | (GPIS_FAIL_ZERO_GRAD << 8);
                 else outcome = GPIS_HIT;
             } else {
                 outcome = GPIS_EXIT;
             }
-            ++s_flights;
-            s_evals += evals_flight;
-            unsigned oc = outcome & 0xFFu;
-            if (oc == GPIS_HIT) ++s_hits; else if (oc == GPIS_EXIT) ++s_exits; else ++s_fails;
+            const unsigned oc = outcome & 0xFFu;
+            atomicAdd(&bstat[ST_FLIGHTS], 1u);
+            atomicAdd(&bstat[ST_EVALS], evals_flight);
+            atomicAdd(&bstat[ST_STEPS], (unsigned)k);
+            atomicAdd(&bstat[oc == GPIS_HIT ? ST_HITS : (oc == GPIS_EXIT ? ST_EXITS : ST_FAILS)], 1u);
             if (!PATHS) {
                 out.t[ray] = t_end; out.value[ray] = val_end;
                 out.gx[ray] = gxv; out.gy[ray] = gyv; out.gz[ray] = gzv;
                 out.outcome[ray] = outcome;
                 if (out.evals) out.evals[ray] = evals_flight;
+                evals_flight = 0;
                 phase = PH_NEED;
             } else {
                 if (bounce == 0) {
                     pout.t0[ray] = t_end; pout.outcome0[ray] = outcome;
                     pout.gx0[ray] = gxv; pout.gy0[ray] = gyv; pout.gz0[ray] = gzv;
                 }
-                ++path_flights;
+                path_fh += 1u + (oc == GPIS_HIT ? 0x10000u : 0u);
                 path_evals += evals_flight;
                 evals_flight = 0;
-                if (oc == GPIS_HIT) ++path_hits;
+                phase = PH_NEED;
                 if (oc == GPIS_HIT && bounce < max_bounces) {
-                    // diffuse bounce: cosine-weighted about the normalised gradient
                     ++bounce;
-                    float hx = __fadd_rn(ox, __fmul_rn(t_end, dx));
-                    float hy = __fadd_rn(oy, __fmul_rn(t_end, dy));
-                    float hz = __fadd_rn(oz, __fmul_rn(t_end, dz));
+                    // sample.p = ray.pos() + t * ray.dir() in float
+                    float hx = mul_add_exact(t_end, dx, ox);
+                    float hy = mul_add_exact(t_end, dy, oy);
+                    float hz = mul_add_exact(t_end, dz, oz);
                     float il = rsqrtf(gxv * gxv + gyv * gyv + gzv * gzv);
-                    float nx = gxv * il, ny = gyv * il, nz = gzv * il;
-                    Pcg g;
-                    g.set((uint64_t)(uint32_t)(xxh4(ray, (uint32_t)bounce, bounce_seed, 0u) + 1u));
-                    float u1 = u01(g.next()), u2 = u01(g.next()), jit = u01(g.next());
-                    float rr = sqrtf(u1), ph = 6.2831853071795864f * u2;
-                    float lx = rr * cosf(ph), ly = rr * sinf(ph), lz = sqrtf(fmaxf(0.f, 1.f - u1));
-                    float sgn = copysignf(1.0f, nz);
-                    float aa = -1.0f / (sgn + nz), bb = nx * ny * aa;
-                    float t0x = 1.0f + sgn * nx * nx * aa, t0y = sgn * bb, t0z = -sgn * nx;
-                    float b0x = bb, b0y = sgn + ny * ny * aa, b0z = -ny;
-                    float ndx = t0x * lx + b0x * ly + nx * lz;
-                    float ndy = t0y * lx + b0y * ly + ny * lz;
-                    float ndz = t0z * lx + b0z * ly + nz * lz;
-                    float inl = rsqrtf(ndx * ndx + ndy * ndy + ndz * ndz);
-                    ndx *= inl; ndy *= inl; ndz *= inl;
-                    ox = hx; oy = hy; oz = hz; dx = ndx; dy = ndy; dz = ndz;
+                    float jit;
+                    diffuse_bounce(ray, bounce, bounce_seed, gxv * il, gyv * il, gzv * il, dx, dy, dz, jit);
+                    ox = hx; oy = hy; oz = hz;
                     near_t = 0.f;
                     far_t = box_exit(ox, oy, oz, dx, dy, dz, box);
                     prev_v = 0.f; prev_gx = gxv; prev_gy = gyv; prev_gz = gzv;
                     first = 0;
+                    rs.cond = 0;
                     if (far_t <= near_t) {
-                        ++s_flights; ++s_exits; ++path_flights;
-                        phase = PH_NEED;
+                        atomicAdd(&bstat[ST_FLIGHTS], 1u);
+                        atomicAdd(&bstat[ST_EXITS], 1u);
+                        path_fh += 1u;
                     } else {
                         step = (far_t - near_t) / P.min_step_f;
                         if (P.step_size < step) step = P.step_size;
-                        t0 = __fadd_rn(near_t, __fmul_rn(step, jit));
-                        if (P.activate) { phase = PH_COND; qx = ox; qy = oy; qz = oz; }
+                        t0 = mul_add_exact(step, jit, near_t);
+                        if (COND && P.activate) { phase = PH_COND; qx = ox; qy = oy; qz = oz; }
                         else { phase = PH_F0; ray_point(ox, oy, oz, dx, dy, dz, 0.0, qx, qy, qz); }
-                        rs.cond = 0;
                     }
-                } else {
-                    phase = PH_NEED;
                 }
                 if (phase == PH_NEED) {
-                    pout.flights[ray] = path_flights;
-                    pout.hits[ray] = path_hits;
+                    pout.flights[ray] = path_fh & 0xFFFFu;
+                    pout.hits[ray] = path_fh >> 16;
                     if (pout.evals) pout.evals[ray] = path_evals;
                 }
             }
-            if (!PATHS) evals_flight = 0;
-            break;
-        }
-        default:
-            break;
         }
     }
-    // ---- statistics: one atomic per counter per wave
-    unsigned vals[7] = {s_flights, s_evals, s_steps, s_hits, s_exits, s_fails, s_invalid};
-#pragma unroll
-    for (int i = 0; i < 7; ++i) {
-        unsigned v = vals[i];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0 && v) atomicAdd(&stats[i], (unsigned long long)v);
-    }
+    // ---- statistics: LDS block totals, one global atomic per counter per block
+    __syncthreads();
+    if (threadIdx.x < ST_N && bstat[threadIdx.x]) atomicAdd(&stats[threadIdx.x], (unsigned long long)bstat[threadIdx.x]);
+}
+
+template <bool PATHS>
+static void launch_flights(gpis_ctx *c, unsigned blocks, unsigned threads, hipStream_t st, RayIn in, ReplyOut out,
+                           PathOut pout, unsigned n, int max_bounces, uint32_t bounce_seed, float box) {
+#define GPIS_FLIGHT_LAUNCH(NI, ISO, COND)                                                                         \
+    hipLaunchKernelGGL((flight_kernel<NI, ISO, COND, PATHS>), dim3(blocks), dim3(threads), 0, st, c->P, in, out, \
+                       pout, n, c->work, c->counters, max_bounces, bounce_seed, box)
+    const bool cond = c->P.activate != 0;
+    if (c->P.N == 8 && c->P.iso && !cond) GPIS_FLIGHT_LAUNCH(8, true, false);
+    else if (c->P.N == 8 && c->P.iso) GPIS_FLIGHT_LAUNCH(8, true, true);
+    else if (c->P.N == 8 && !cond) GPIS_FLIGHT_LAUNCH(8, false, false);
+    else if (c->P.N == 8) GPIS_FLIGHT_LAUNCH(8, false, true);
+    else if (c->P.iso && !cond) GPIS_FLIGHT_LAUNCH(0, true, false);
+    else if (c->P.iso) GPIS_FLIGHT_LAUNCH(0, true, true);
+    else if (!cond) GPIS_FLIGHT_LAUNCH(0, false, false);
+    else GPIS_FLIGHT_LAUNCH(0, false, true);
+#undef GPIS_FLIGHT_LAUNCH
 }
 
 // ---------------------------------------------------------------- C ABI --
@@ -616,11 +678,11 @@ gpis_status gpis_set_domain(gpis_ctx *c, const float lo[3], const float hi[3], u
         dims[i] = chi - clo[i] + 1;
         cells *= (size_t)dims[i];
     }
-    if (cells * c->P.N * sizeof(float4) > (size_t)8 << 30) return fail(c, GPIS_INVALID_ARGUMENT, "domain too large for a table");
-    HIP_TRY(c, hipMalloc((void **)&c->table, cells * c->P.N * sizeof(float4)));
+    if (cells * c->P.N * 3 * sizeof(float) > (size_t)8 << 30) return fail(c, GPIS_INVALID_ARGUMENT, "domain too large for a table");
+    HIP_TRY(c, hipMalloc((void **)&c->table, cells * c->P.N * 3 * sizeof(float)));
     int threads = 256;
     unsigned blocks = (unsigned)((cells + threads - 1) / threads);
-    hipLaunchKernelGGL(impulse_table_kernel, dim3(blocks), dim3(threads), 0, 0, clo[0], clo[1], clo[2], dims[0],
+    hipLaunchKernelGGL(table_build_kernel, dim3(blocks), dim3(threads), 0, 0, clo[0], clo[1], clo[2], dims[0],
                        dims[1], dims[2], seed, c->P.N, c->table);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipDeviceSynchronize());
@@ -656,12 +718,14 @@ gpis_status gpis_eval_points(gpis_ctx *c, const float *x, const float *y, const 
     if (!x || !y || !z || !v || !gx || !gy || !gz) return fail(c, GPIS_INVALID_ARGUMENT, "null pointer");
     int threads = 256;
     unsigned blocks = (unsigned)((n + threads - 1) / threads);
-    if (c->P.N == 8)
-        hipLaunchKernelGGL(eval_points_kernel<8>, dim3(blocks), dim3(threads), 0, (hipStream_t)stream, c->P, x, y, z,
-                           (long long)n, seed, v, gx, gy, gz);
-    else
-        hipLaunchKernelGGL(eval_points_kernel<0>, dim3(blocks), dim3(threads), 0, (hipStream_t)stream, c->P, x, y, z,
-                           (long long)n, seed, v, gx, gy, gz);
+#define GPIS_EVAL_LAUNCH(NI, ISO)                                                                              \
+    hipLaunchKernelGGL((eval_points_kernel<NI, ISO>), dim3(blocks), dim3(threads), 0, (hipStream_t)stream, c->P, x, \
+                       y, z, (long long)n, seed, v, gx, gy, gz)
+    if (c->P.N == 8 && c->P.iso) GPIS_EVAL_LAUNCH(8, true);
+    else if (c->P.N == 8) GPIS_EVAL_LAUNCH(8, false);
+    else if (c->P.iso) GPIS_EVAL_LAUNCH(0, true);
+    else GPIS_EVAL_LAUNCH(0, false);
+#undef GPIS_EVAL_LAUNCH
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }
@@ -708,12 +772,7 @@ gpis_status gpis_trace_flights(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n,
     launch_shape(c, n, blocks, threads);
     ReplyOut out = {reply->t, reply->value, reply->gx, reply->gy, reply->gz, reply->outcome, reply->evals};
     PathOut pout = {};
-    if (c->P.N == 8)
-        hipLaunchKernelGGL((flight_kernel<8, false>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays), out,
-                           pout, (unsigned)n, c->work, c->counters, 0, 0u, 0.f);
-    else
-        hipLaunchKernelGGL((flight_kernel<0, false>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays), out,
-                           pout, (unsigned)n, c->work, c->counters, 0, 0u, 0.f);
+    launch_flights<false>(c, blocks, threads, st, make_in(rays), out, pout, (unsigned)n, 0, 0u, 0.f);
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }
@@ -736,12 +795,7 @@ gpis_status gpis_trace_paths(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, i
     ReplyOut out = {};
     PathOut pout = {reply->t0, reply->gx0, reply->gy0, reply->gz0, reply->outcome0, reply->flights, reply->hits,
                     reply->evals};
-    if (c->P.N == 8)
-        hipLaunchKernelGGL((flight_kernel<8, true>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays), out, pout,
-                           (unsigned)n, c->work, c->counters, max_bounces, bounce_seed, box);
-    else
-        hipLaunchKernelGGL((flight_kernel<0, true>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays), out, pout,
-                           (unsigned)n, c->work, c->counters, max_bounces, bounce_seed, box);
+    launch_flights<true>(c, blocks, threads, st, make_in(rays), out, pout, (unsigned)n, max_bounces, bounce_seed, box);
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }

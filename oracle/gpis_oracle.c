@@ -220,6 +220,8 @@ typedef struct {
     uint32_t seed;         /* computeSeed(info) */
     uint32_t normal_seed;  /* xxh(pss) + xxh(sceneSeed) + 1 */
     cond_t c;
+    float perturb;         /* test hook: +-perturb added to each marched value (0 = off) */
+    uint32_t pstate;
 } real_t;
 
 static int is_se(const orc_field *f) { return f->d.cov_kind == ORC_COV_SE; }
@@ -714,6 +716,17 @@ static v3f eval_gradient(const real_t *r, v3f p, float t, v3f dir) {
     return vadd(vscale(amp, ng), mg);
 }
 
+/* A marched value, optionally perturbed by +-perturb with a hashed sign: the test
+ * hook that decides whether a result is stable under FP32-sized evaluation error. */
+static float march_value(real_t *r, v3f p, v3f dir) {
+    float v = eval_value(r, p, dir);
+    if (r->perturb != 0.f) {
+        r->pstate = orc_xxhash32_1(r->pstate);
+        v += (r->pstate & 1u) ? r->perturb : -r->perturb;
+    }
+    return v;
+}
+
 /* conditioning3D (:498-574), world space */
 static void conditioning3d(real_t *r, v3f p, v3f dir, float tv, v3f tg) {
     const orc_field *f = r->f;
@@ -878,7 +891,12 @@ static void prepare(real_t *r, const orc_field *f, const orc_ray *ray) {
     }
 }
 
-void orc_trace_flight(const orc_field *f, const orc_ray *ray, orc_flight *out) {
+static void trace_flight_impl(const orc_field *f, const orc_ray *ray, orc_flight *out, float perturb, uint32_t pseed);
+void orc_trace_flight(const orc_field *f, const orc_ray *ray, orc_flight *out) { trace_flight_impl(f, ray, out, 0.f, 0u); }
+void orc_trace_flight_perturbed(const orc_field *f, const orc_ray *ray, float eps, uint32_t pseed, orc_flight *out) {
+    trace_flight_impl(f, ray, out, eps, pseed);
+}
+static void trace_flight_impl(const orc_field *f, const orc_ray *ray, orc_flight *out, float perturb, uint32_t pseed) {
     memset(out, 0, sizeof *out);
     int evals = 0;
     /* beginDistance (GaussianProcessMedium.cpp:236-287) */
@@ -896,6 +914,8 @@ void orc_trace_flight(const orc_field *f, const orc_ray *ray, orc_flight *out) {
     }
     real_t r;
     prepare(&r, f, ray);
+    r.perturb = perturb;
+    r.pstate = pseed;
     v3f dirf = V3(ray->dir[0], ray->dir[1], ray->dir[2]);
     v3d p = D3(ray->origin[0], ray->origin[1], ray->origin[2]);
     v3d rd = d_of(dirf);
@@ -903,7 +923,7 @@ void orc_trace_flight(const orc_field *f, const orc_ray *ray, orc_flight *out) {
     float step_size = (far_t - near_t) / (float)f->d.min_step;
     if (f->d.step_size < step_size) step_size = f->d.step_size;
     double t = near_t;
-    double f0 = eval_value(&r, f_of(D3(p.x + t * rd.x, p.y + t * rd.y, p.z + t * rd.z)), dirf); ++evals;
+    double f0 = march_value(&r, f_of(D3(p.x + t * rd.x, p.y + t * rd.y, p.z + t * rd.z)), dirf); ++evals;
     float minf = ray->first_scatter ? (float)fabs(f0) : INFINITY;
     int sign0 = f0 < 0 ? -1 : 1;
     double pf = f0;
@@ -911,7 +931,7 @@ void orc_trace_flight(const orc_field *f, const orc_ray *ray, orc_flight *out) {
     int step = 0, hit = 0;
     while (t < far_t) {
         step++;
-        double fc = eval_value(&r, f_of(D3(p.x + t * rd.x, p.y + t * rd.y, p.z + t * rd.z)), dirf); ++evals;
+        double fc = march_value(&r, f_of(D3(p.x + t * rd.x, p.y + t * rd.y, p.z + t * rd.z)), dirf); ++evals;
         int sc = fc < 0 ? -1 : 1;
         if ((float)fabs(fc) < minf) minf = (float)fabs(fc);
         if (!ray->first_scatter && step == 1) {
@@ -923,7 +943,7 @@ void orc_trace_flight(const orc_field *f, const orc_ray *ray, orc_flight *out) {
             double tp = a * (1.0 - fac) + b * fac, tt;
             for (;;) {
                 tt = a * (1.0 - fac) + b * fac;
-                double ft = eval_value(&r, f_of(D3(p.x + tt * rd.x, p.y + tt * rd.y, p.z + tt * rd.z)), dirf); ++evals;
+                double ft = march_value(&r, f_of(D3(p.x + tt * rd.x, p.y + tt * rd.y, p.z + tt * rd.z)), dirf); ++evals;
                 int st = ft < 0 ? -1 : 1;
                 if ((float)fabs(ft) < minf) minf = (float)fabs(ft);
                 if (st == sign0) break;

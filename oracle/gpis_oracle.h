@@ -126,6 +126,11 @@ float orc_kernel_radius(const orc_field *f);
 void orc_trace_flight(const orc_field *f, const orc_ray *ray, orc_flight *out);
 void orc_trace_flights(const orc_field *f, const orc_ray *rays, int64_t n,
                        orc_flight *out, int nthreads);
+/* Test hook: the same flight with +-eps added to every marched value (hashed signs
+ * from pseed).  A result that changes under eps of the size of FP32 evaluation
+ * error is ill-conditioned (grazing); tests use it to classify disagreements. */
+void orc_trace_flight_perturbed(const orc_field *f, const orc_ray *ray, float eps, uint32_t pseed,
+                                orc_flight *out);
 /* Conditioning check: evaluate value and gradient of the conditioned realization
  * at q, after conditioning at ray origin on (last_val, last_aniso). */
 void orc_conditioned_eval(const orc_field *f, const orc_ray *ray, const float q[3],

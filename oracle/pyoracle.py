@@ -93,6 +93,7 @@ def lib():
         L.orc_kernel_radius.restype = C.c_float; L.orc_kernel_radius.argtypes = [vp]
         L.orc_trace_flight.argtypes = [vp, C.POINTER(Ray), C.POINTER(Flight)]
         L.orc_trace_flights.argtypes = [vp, vp, C.c_int64, vp, C.c_int]
+        L.orc_trace_flight_perturbed.argtypes = [vp, C.POINTER(Ray), C.c_float, C.c_uint32, C.POINTER(Flight)]
         L.orc_conditioned_eval.argtypes = [vp, C.POINTER(Ray), f32p, C.c_float, f32p]
         L.orc_nee.argtypes = [vp, C.POINTER(Ray), f32p, f32p, C.c_float, f32p, f32p]
         _lib = L
@@ -230,6 +231,12 @@ class Field:
         out = np.zeros(rays.shape[0], FLIGHT_DTYPE)
         lib().orc_trace_flights(self.h, rays.ctypes.data, rays.shape[0], out.ctypes.data, nthreads)
         return out
+
+    def trace_perturbed(self, ray_rec, eps, pseed):
+        r = Ray.from_buffer_copy(np.ascontiguousarray(ray_rec, RAY_DTYPE).tobytes())
+        out = np.zeros(1, FLIGHT_DTYPE)
+        lib().orc_trace_flight_perturbed(self.h, C.byref(r), eps, pseed, out.ctypes.data_as(C.POINTER(Flight)))
+        return out[0]
 
     def conditioned_eval(self, ray_rec, q, t_segment=1.0):
         r = Ray.from_buffer_copy(np.ascontiguousarray(ray_rec, RAY_DTYPE).tobytes())

@@ -74,12 +74,30 @@ def _flight_compare(js, rays, domain, name):
     rep = dev.trace_flights(RayBatch.from_numpy(rays))
     torch.cuda.synchronize()
     got = {k: v.cpu().numpy() for k, v in rep.items()}
-    ref = oracle_field(med).trace(oracle_rays(rays, med), nthreads=NTH)
+    F = oracle_field(med)
+    orays = oracle_rays(rays, med)
+    ref = F.trace(orays, nthreads=NTH)
     dev.close()
-    return got, ref
+    return got, ref, (F, orays)
 
 
-def parity_gate(got, ref, name, min_agree=0.9999, rel_tol=1e-4):
+EPS_F = 2e-6  # measured bound of |f_gpu - f_oracle| for one evaluation is 1.4e-6
+
+
+def fragile(F, ray_rec, ref_rec, rel_tol=1e-4, trials=16):
+    """True when the oracle's own answer moves beyond the gate under +-EPS_F
+    perturbations of its marched values: an ill-conditioned (grazing) flight."""
+    for s in range(trials):
+        for eps in (EPS_F, -EPS_F):
+            r = F.trace_perturbed(ray_rec, eps, 1000 + s)
+            if r["outcome"] != ref_rec["outcome"]:
+                return True
+            if abs(r["t"] - ref_rec["t"]) > rel_tol * abs(ref_rec["t"]) + 1e-6:
+                return True
+    return False
+
+
+def parity_gate(got, ref, name, min_agree=0.9999, rel_tol=1e-4, oracle=None):
     """Outcome agreement >= min_agree; every disagreement (outcome, or distance
     beyond rel_tol) must be grazing: the oracle based a sign decision on |f| < GRAZE."""
     oc = got["outcome"] & 0xFF
@@ -98,6 +116,10 @@ def parity_gate(got, ref, name, min_agree=0.9999, rel_tol=1e-4):
     step = 0.01
     sens = step * (np.abs(pf) + np.abs(fc)) * 1e-5 / np.maximum((pf - fc) ** 2, 1e-30)
     grazing = (ref["min_abs_f"] < GRAZE) | (sens > rel_tol * np.abs(ref["t"]) + 1e-6)
+    if oracle is not None:
+        F, orays = oracle
+        for i in np.nonzero(bad & ~grazing)[0]:
+            grazing[i] = fragile(F, orays[i], ref[i], rel_tol)
     n = len(oc)
     print(f"{name}: n={n} agree={1 - bad.mean():.6f} ({bad.sum()} differ, {(bad & ~grazing).sum()} non-grazing); "
           f"crossings={crossing.sum()} max rel t err (agreeing)={rel[crossing & ~bad].max() if (crossing & ~bad).any() else 0:.3e}; "
@@ -118,8 +140,8 @@ def test_flights_config0_workload2():
     from gpis_extend_amd import SparseConvolutionNoiseMedium
     med = SparseConvolutionNoiseMedium.from_json(W.config0_json())
     rays = W.sphere_rays(med, n=65536)
-    got, ref = _flight_compare(W.config0_json(), rays, ((-2.1,) * 3, (2.1,) * 3), "C0W2")
-    parity_gate(got, ref, "C0W2 (65,536 flights)")
+    got, ref, orc = _flight_compare(W.config0_json(), rays, ((-2.1,) * 3, (2.1,) * 3), "C0W2")
+    parity_gate(got, ref, "C0W2 (65,536 flights)", oracle=orc)
 
 
 def test_flights_config1_camera_subset():
@@ -129,8 +151,8 @@ def test_flights_config1_camera_subset():
     rays = W.camera_rays(med, width=256, height=256)
     idx = np.arange(0, rays["ox"].shape[0], 8)
     rays = W.rays_subset(rays, idx)
-    got, ref = _flight_compare(W.config1_json(), rays, ((-1.0,) * 3, (1.0,) * 3), "C1")
-    parity_gate(got, ref, "C1 primary subset", min_agree=0.999)
+    got, ref, orc = _flight_compare(W.config1_json(), rays, ((-1.0,) * 3, (1.0,) * 3), "C1")
+    parity_gate(got, ref, "C1 primary subset", min_agree=0.999, oracle=orc)
 
 
 def test_flights_config2_conditioned_multires_subset():
@@ -138,8 +160,8 @@ def test_flights_config2_conditioned_multires_subset():
     from gpis_extend_amd import SparseConvolutionNoiseMedium
     med = SparseConvolutionNoiseMedium.from_json(W.config2_json())
     rays = W.box_surface_rays(med, n=512)
-    got, ref = _flight_compare(W.config2_json(), rays, None, "C2")
-    parity_gate(got, ref, "C2 conditioned multires subset", min_agree=0.99)
+    got, ref, orc = _flight_compare(W.config2_json(), rays, None, "C2")
+    parity_gate(got, ref, "C2 conditioned multires subset", min_agree=0.99, oracle=orc)
 
 
 def test_paths_first_flight_equals_single_flights():
