@@ -111,6 +111,9 @@ def main():
     ap.add_argument("--width", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the timed region, copy this many MiB device-to-device once "
+                         "(a known-size kernel for calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -177,6 +180,13 @@ def main():
                      "kernel": "flight_kernel<8,true>", "kernel_ms": kernel_ms,
                      "flops_per_eval": fpe},
     }
+    if args.copy_probe_mib:
+        src = torch.empty(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(med, rays)
     print(json.dumps(line), flush=True)

@@ -76,6 +76,16 @@ __device__ __forceinline__ void pair3p(float lx, float ly, float lz, float xs, f
     az += we * tz;
 }
 
+// A cell at offset d in {-1,0,1} from the query's cell holds impulses whose |to| along
+// an axis is at least rx (d=-1), 0 (d=0) or 1-rx (d=+1).  When the sum of these squared
+// bounds exceeds 1 (with a margin far above the rounding of to), no impulse of the cell
+// can pass the support test, so the cell is skipped without changing the result.
+__device__ __forceinline__ float axis_bound2(float r, int d) {
+    float b = d == 0 ? r : (d == 1 ? 0.0f : 1.0f - r);
+    return b * b;
+}
+constexpr float kCullR2 = 1.0001f;
+
 // Impulses of one cell from the generator (cell3D's RNG sequence, bit-exact):
 // set_state(xxhash32(k, j, i, seed) + 1), then per impulse z, y, x, weight.
 template <int NI, bool ISO>
@@ -147,6 +157,7 @@ __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, flo
 #pragma unroll 1
         for (int j = 0; j < 27; ++j) {
             int dxi = j / 9, dyi = (j / 3) % 3, dzi = j % 3;  // the reference's dx, dy, dz order
+            if (axis_bound2(rx, dxi) + axis_bound2(ry, dyi) + axis_bound2(rz, dzi) >= kCullR2) continue;
             const float4 *q = reinterpret_cast<const float4 *>(
                 base + ((size_t)dzi * sz + (size_t)dyi * sy + (size_t)dxi * cs));
             float4 cur[Q];
@@ -165,6 +176,7 @@ __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, flo
     for (int dx = -1; dx <= 1; ++dx) {
         for (int dy = -1; dy <= 1; ++dy) {
             for (int dz = -1; dz <= 1; ++dz) {
+                if (axis_bound2(rx, dx + 1) + axis_bound2(ry, dy + 1) + axis_bound2(rz, dz + 1) >= kCullR2) continue;
                 int cx = ix + dx, cy = iy + dy, cz = iz + dz;
                 float lx = rx - (float)dx, ly = ry - (float)dy, lz = rz - (float)dz;
                 unsigned ux = (unsigned)(cx - P.tlo[0]), uy = (unsigned)(cy - P.tlo[1]), uz = (unsigned)(cz - P.tlo[2]);
