@@ -111,6 +111,8 @@ def main():
     ap.add_argument("--width", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the launch-shape sweep that runs before the warmup steps")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed region, copy this many MiB device-to-device once "
                          "(a known-size kernel for calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -128,11 +130,15 @@ def main():
     n = rays["ox"].shape[0]
     dev = GpisDevice(med)
     dev.set_domain((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
-    if args.waves_per_cu:
-        dev.set_launch(args.waves_per_cu, 0)
     batch = RayBatch.from_numpy(rays)
     reply = dev.alloc_path_reply(n)
     stream = torch.cuda.current_stream()
+    tuned = None
+    if args.waves_per_cu:
+        dev.set_launch(args.waves_per_cu, 0)
+    elif not args.no_autotune:
+        from gpis_extend_amd import autotune
+        tuned = autotune.tune(dev, "paths", n, lambda: dev.trace_paths(batch, 2, 7, 1.0, reply))
 
     for _ in range(args.warmup):
         dev.trace_paths(batch, 2, 7, 1.0, reply)
@@ -173,7 +179,8 @@ def main():
         "config": {"workload": "plain primary-ray flights at scale: SE l=0.05, 8 impulses/cell, "
                                f"{args.width}x{args.width} pinhole rays + 2 diffuse bounces",
                    "rays": n, "flights_per_step": flights_per_step, "evals_per_step": evals_per_step,
-                   "hits_per_step": c["hits"] / args.steps, "fails_per_step": c["fails"] / args.steps},
+                   "hits_per_step": c["hits"] / args.steps, "fails_per_step": c["fails"] / args.steps,
+                   "launch": {"refill": tuned[0], "waves_per_cu": tuned[1], "block": tuned[2]} if tuned else None},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "frac_scalar_issue_peak": achieved / PEAK_FP32_SCALAR_TFLOPS,

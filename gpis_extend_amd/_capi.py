@@ -26,7 +26,7 @@ FAIL_NONE, FAIL_OUTWARD, FAIL_ZERO_GRAD, FAIL_NONFINITE = 0, 1, 2, 3
 EXPORTS = (
     "gpis_create", "gpis_destroy", "gpis_last_error", "gpis_set_domain", "gpis_impulse_table",
     "gpis_eval_points", "gpis_trace_flights", "gpis_trace_paths", "gpis_get_counters",
-    "gpis_reset_counters", "gpis_set_launch",
+    "gpis_reset_counters", "gpis_set_launch", "gpis_set_refill",
 )
 
 
@@ -105,6 +105,8 @@ def lib():
     L.gpis_reset_counters.argtypes = [_vp]
     L.gpis_set_launch.restype = st
     L.gpis_set_launch.argtypes = [_vp, C.c_int32, C.c_int32]
+    L.gpis_set_refill.restype = st
+    L.gpis_set_refill.argtypes = [_vp, C.c_int32]
     _lib = L
     return L
 

@@ -24,6 +24,10 @@ def hipcc() -> str:
     return "hipcc"
 
 
+HOST_DIR = os.path.join(PKG, "host")
+SELFTEST = os.path.join(HOST_DIR, "backend_selftest")
+
+
 def build(verbose: bool = False) -> str:
     tmp = OUT + ".tmp"
     cmd = [hipcc(), *HIPCC_FLAGS, SRC, "-o", tmp]
@@ -31,6 +35,14 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, OUT)
+    # the C++ batch backend (host code over the C ABI) and its self-test driver
+    cmd = [hipcc(), "-O2", "-std=c++17", "-Wno-unused-result",
+           os.path.join(HOST_DIR, "gpis_backend.cpp"), os.path.join(HOST_DIR, "backend_selftest.cpp"),
+           "-L" + PKG, "-lgpis_hip", "-Wl,-rpath,$ORIGIN/..", "-lpthread", "-o", SELFTEST + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(SELFTEST + ".tmp", SELFTEST)
     return OUT
 
 

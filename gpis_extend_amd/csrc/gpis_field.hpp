@@ -153,21 +153,29 @@ __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, flo
             for (int k = 0; k < Q; ++k) cur[k] = nxt[k];
         }
 #else
-        // the other resident waves hide each cell's load latency
+        // the other resident waves hide each cell's load latency; the dz row is unrolled
+        const float bz0 = axis_bound2(rz, 0), bz2 = axis_bound2(rz, 2);
 #pragma unroll 1
-        for (int j = 0; j < 27; ++j) {
-            int dxi = j / 9, dyi = (j / 3) % 3, dzi = j % 3;  // the reference's dx, dy, dz order
-            if (axis_bound2(rx, dxi) + axis_bound2(ry, dyi) + axis_bound2(rz, dzi) >= kCullR2) continue;
-            const float4 *q = reinterpret_cast<const float4 *>(
-                base + ((size_t)dzi * sz + (size_t)dyi * sy + (size_t)dxi * cs));
-            float4 cur[Q];
+        for (int j = 0; j < 9; ++j) {
+            const int dxi = j / 3, dyi = j % 3;  // the reference's dx, dy order; dz innermost
+            const float bxy = axis_bound2(rx, dxi) + axis_bound2(ry, dyi);
+            if (bxy >= kCullR2) continue;
+            const float *row = base + ((size_t)dyi * sy + (size_t)dxi * cs);
+            const float lx = rx - (float)(dxi - 1), ly = ry - (float)(dyi - 1);
 #pragma unroll
-            for (int k = 0; k < Q; ++k) cur[k] = q[k];
-            float lx = rx - (float)(dxi - 1), ly = ry - (float)(dyi - 1), lz = rz - (float)(dzi - 1);
-            const float *f = reinterpret_cast<const float *>(cur);
+            for (int dzi = 0; dzi < 3; ++dzi) {
+                const float bzz = dzi == 0 ? bz0 : (dzi == 1 ? 0.0f : bz2);
+                if (bxy + bzz >= kCullR2) continue;
+                const float4 *q = reinterpret_cast<const float4 *>(row + (size_t)dzi * sz);
+                float4 cur[Q];
 #pragma unroll
-            for (int k = 0; k < (NI > 0 ? NI : 1); ++k)
-                pair3p<ISO>(lx, ly, lz, f[3 * k], f[3 * k + 1], f[3 * k + 2], c0, c1, c2, v, ax, ay, az);
+                for (int k = 0; k < Q; ++k) cur[k] = q[k];
+                const float lz = rz - (float)(dzi - 1);
+                const float *f = reinterpret_cast<const float *>(cur);
+#pragma unroll
+                for (int k = 0; k < (NI > 0 ? NI : 1); ++k)
+                    pair3p<ISO>(lx, ly, lz, f[3 * k], f[3 * k + 1], f[3 * k + 2], c0, c1, c2, v, ax, ay, az);
+            }
         }
 #endif
         Eval4 r = {v, ax, ay, az};

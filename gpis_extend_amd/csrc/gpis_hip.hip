@@ -29,6 +29,7 @@ struct gpis_ctx {
     int num_cus = 0;
     int waves_per_cu = 0;
     int block_size = 256;
+    int wave_refill = 0;
     std::string err;
 };
 
@@ -345,7 +346,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
                                                                             unsigned int *work,
                                                                             unsigned long long *stats,
                                                                             int max_bounces, uint32_t bounce_seed,
-                                                                            float box) {
+                                                                            float box, int wave_refill) {
     __shared__ unsigned int bstat[ST_N];
     if (threadIdx.x < ST_N) bstat[threadIdx.x] = 0;
     __syncthreads();
@@ -365,8 +366,11 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
     unsigned path_evals = 0;
 
     for (;;) {
-        // ---- refill: lanes that finished take the next rays (ballot + prefix sum)
+        // ---- refill: lanes that finished take the next rays (ballot + prefix sum).
+        // With wave_refill the wave refills only when every lane is done, so the 64
+        // consecutive rays of a tile march together (coherent cell accesses).
         unsigned long long need = __ballot(phase == PH_NEED);
+        if (wave_refill && __ballot(phase != PH_NEED && phase != PH_DEAD)) need = 0;
         if (need) {
             unsigned cnt = (unsigned)__popcll(need);
             unsigned leader = (unsigned)(__ffsll((long long)need) - 1);
@@ -606,7 +610,7 @@ static void launch_flights(gpis_ctx *c, unsigned blocks, unsigned threads, hipSt
                            PathOut pout, unsigned n, int max_bounces, uint32_t bounce_seed, float box) {
 #define GPIS_FLIGHT_LAUNCH(NI, ISO, COND)                                                                         \
     hipLaunchKernelGGL((flight_kernel<NI, ISO, COND, PATHS>), dim3(blocks), dim3(threads), 0, st, c->P, in, out, \
-                       pout, n, c->work, c->counters, max_bounces, bounce_seed, box)
+                       pout, n, c->work, c->counters, max_bounces, bounce_seed, box, c->wave_refill)
     const bool cond = c->P.activate != 0;
     if (c->P.N == 8 && c->P.iso && !cond) GPIS_FLIGHT_LAUNCH(8, true, false);
     else if (c->P.N == 8 && c->P.iso) GPIS_FLIGHT_LAUNCH(8, true, true);
@@ -797,6 +801,13 @@ gpis_status gpis_trace_paths(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, i
                     reply->evals};
     launch_flights<true>(c, blocks, threads, st, make_in(rays), out, pout, (unsigned)n, max_bounces, bounce_seed, box);
     HIP_TRY(c, hipGetLastError());
+    return GPIS_OK;
+}
+
+gpis_status gpis_set_refill(gpis_ctx *c, int32_t policy) {
+    if (!c) return GPIS_INVALID_ARGUMENT;
+    if (policy != 0 && policy != 1) return fail(c, GPIS_INVALID_ARGUMENT, "refill policy must be 0 (lane) or 1 (wave)");
+    c->wave_refill = policy;
     return GPIS_OK;
 }
 

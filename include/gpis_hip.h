@@ -160,6 +160,11 @@ gpis_status gpis_reset_counters(gpis_ctx *ctx);
 /* Launch-shape control (the autotuner's knobs): waves per CU of the persistent
  * flight kernel, and rays per lane batch.  0 = built-in default. */
 gpis_status gpis_set_launch(gpis_ctx *ctx, int32_t waves_per_cu, int32_t block_size);
+/* Refill policy of the persistent flight kernels: 0 = a lane takes the next ray as
+ * soon as its flight (or path) ends; 1 = a wave takes 64 new rays only when all of
+ * its lanes are done, so rays submitted in spatially coherent groups of 64 (such as
+ * 8x8 pixel tiles) march together.  No reference counterpart (a scheduling knob). */
+gpis_status gpis_set_refill(gpis_ctx *ctx, int32_t policy);
 
 #ifdef __cplusplus
 }
