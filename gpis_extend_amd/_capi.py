@@ -26,7 +26,7 @@ FAIL_NONE, FAIL_OUTWARD, FAIL_ZERO_GRAD, FAIL_NONFINITE = 0, 1, 2, 3
 EXPORTS = (
     "gpis_create", "gpis_destroy", "gpis_last_error", "gpis_set_domain", "gpis_impulse_table",
     "gpis_eval_points", "gpis_trace_flights", "gpis_trace_paths", "gpis_get_counters",
-    "gpis_reset_counters", "gpis_set_launch", "gpis_set_refill",
+    "gpis_reset_counters", "gpis_set_launch", "gpis_set_refill", "gpis_trace_flights_1d",
 )
 
 
@@ -59,6 +59,10 @@ class ReplySoA(C.Structure):
 
 class PathReply(C.Structure):
     _fields_ = [(n, _vp) for n in ("t0", "gx0", "gy0", "gz0", "outcome0", "flights", "hits", "evals")]
+
+
+class NeeOut(C.Structure):
+    _fields_ = [(n, _vp) for n in ("gx", "gy", "gz", "pdf")]
 
 
 class Counters(C.Structure):
@@ -99,6 +103,9 @@ def lib():
     L.gpis_trace_paths.restype = st
     L.gpis_trace_paths.argtypes = [_vp, C.POINTER(RaySoA), C.c_int64, C.c_int32, C.c_uint32, C.c_float,
                                    C.POINTER(PathReply), _vp]
+    L.gpis_trace_flights_1d.restype = st
+    L.gpis_trace_flights_1d.argtypes = [_vp, C.POINTER(RaySoA), C.c_int64, C.POINTER(ReplySoA),
+                                        C.POINTER(C.c_float), C.POINTER(NeeOut), _vp]
     L.gpis_get_counters.restype = st
     L.gpis_get_counters.argtypes = [_vp, C.POINTER(Counters)]
     L.gpis_reset_counters.restype = st

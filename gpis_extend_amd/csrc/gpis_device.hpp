@@ -61,6 +61,7 @@ struct FieldParams {
     float sigma;         // covariance sigma
     float amp_const;     // amplitude when it is spatially constant
     float l_aniso[3], l_aniso_inv[3];
+    float aniso[3];  // anisotropy factors of the SE length scale (1D rays rescale by their own l)
     float l_max;         // max(l_aniso) (splattingKernelRadius)
     float kernel_scale;  // _kernelScale (3)
     int ls_field; float ls_params[4]; float ls_max; float ns_var;
@@ -75,6 +76,7 @@ struct FieldParams {
     float nf_inv;        // 1 / sqrt(variance)
     // mean
     int mean_kind; float mc[3]; float mr; float mh[3]; float pn[3]; float poff;
+    double mcd[3], mhd[3], pnd[3], poffd, mrd;  // the same mean parameters in double (exact 1D path)
     // medium
     uint32_t global_seed; int single; int ctxt; int activate; int one_d; int scheme1d; int corr_xy;
     float step_size; float min_step_f; int max_bounce;

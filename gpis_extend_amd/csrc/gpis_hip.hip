@@ -13,6 +13,7 @@
 #include <string>
 
 #include "../../include/gpis_hip.h"
+#include "gpis_1d.hpp"
 #include "gpis_field.hpp"
 
 using namespace gpis;
@@ -30,6 +31,7 @@ struct gpis_ctx {
     int waves_per_cu = 0;
     int block_size = 256;
     int wave_refill = 0;
+    float nf1d = 1.0f;  // sqrt(sparseConvNoiseVariance1D) of the 1D grid
     std::string err;
 };
 
@@ -58,7 +60,10 @@ static gpis_status build_params(gpis_ctx *c) {
     if (!(d.length_scale > 0.0f) || !std::isfinite(d.length_scale)) return fail(c, GPIS_INVALID_ARGUMENT, "lengthScale must be positive");
     if (!(d.step_size > 0.0f) || d.min_step == 0) return fail(c, GPIS_INVALID_ARGUMENT, "step_size and min_step must be positive");
     if (d.cov_kind != GPIS_COV_SE && d.cov_kind != GPIS_COV_PROC_NS) return fail(c, GPIS_UNSUPPORTED, "covariance kind");
-    if (d.one_d) return fail(c, GPIS_UNSUPPORTED, "1D sampling is served by the 1D entry points");
+    if (d.one_d && d.cov_kind != GPIS_COV_SE)
+        return fail(c, GPIS_UNSUPPORTED, "1D sampling is served for the stationary SE kernel");
+    if (d.one_d && d.correlation_xy)
+        return fail(c, GPIS_UNSUPPORTED, "1D_gradient_correlationXY (Renewal+ transverse law) is not served");
     P.kind = d.cov_kind;
     P.multires = (d.cov_kind == GPIS_COV_PROC_NS) && d.ns_multires;
     P.sigma = d.sigma;
@@ -68,6 +73,7 @@ static gpis_status build_params(gpis_ctx *c) {
         P.l_aniso[i] = l_conv * d.aniso[i];
         float inv = 1.0f / P.l_aniso[i];
         P.l_aniso_inv[i] = std::isfinite(inv) ? inv : 0.0f;
+        P.aniso[i] = d.aniso[i];
     }
     P.l_max = fmaxf(fmaxf(P.l_aniso[0], P.l_aniso[1]), P.l_aniso[2]);
     P.kernel_scale = d.kernel_scale;
@@ -101,17 +107,26 @@ static gpis_status build_params(gpis_ctx *c) {
     }
     P.mr = d.mean_radius;
     P.poff = (float)d.plane_offset;
+    for (int i = 0; i < 3; ++i) {
+        P.mcd[i] = d.mean_center[i];
+        P.mhd[i] = d.mean_half[i];
+        P.pnd[i] = d.plane_normal[i];
+    }
+    P.poffd = d.plane_offset;
+    P.mrd = (double)d.mean_radius;
     P.global_seed = d.global_seed;
     P.single = d.single_realization;
     P.ctxt = d.ctxt;
     P.activate = !d.single_realization && (d.ctxt == GPIS_CTX_RENEWAL || d.ctxt == GPIS_CTX_RENEWAL_PLUS);
-    P.one_d = 0;
+    P.one_d = d.one_d;
     P.corr_xy = d.correlation_xy;
     P.step_size = d.step_size;
     P.min_step_f = (float)d.min_step;
     P.max_bounce = d.max_bounce;
     P.table = c->table;
     P.has_table = 0;
+    // sparseConvNoiseVariance1D(N, R = localScale, 1) (GPFunctions.cpp:766-770)
+    c->nf1d = sqrtf((float)((double)(P.Nf / d.kernel_scale) * sqrt(M_PI) * 1.0));
     return GPIS_OK;
 }
 
@@ -190,7 +205,8 @@ enum Phase : int {
     PH_SECANT = 4, // secant candidate with 0.9 shrink
     PH_FAR = 5,    // f(farT) on exit
     PH_GRAD = 6,   // gradient at the endpoint
-    PH_DEAD = 7    // no more work
+    PH_DEAD = 7,   // no more work
+    PH_NEE = 8     // next-event estimation at the scattering point (1D flights)
 };
 
 #ifndef GPIS_FLIGHT_MIN_WAVES
@@ -605,6 +621,347 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
     if (threadIdx.x < ST_N && bstat[threadIdx.x]) atomicAdd(&stats[threadIdx.x], (unsigned long long)bstat[threadIdx.x]);
 }
 
+
+// ---------------------------------------------------------------- 1D flights --
+struct Nee1DOut {
+    float *gx, *gy, *gz, *pdf;
+};
+
+// Per-lane 1D field evaluation at q: normalised noise value and derivative along the
+// ray (conditioned when `cond`), then f = (float)(amplitude * noise + mean in double),
+// the reference's evaluateValue rounding.
+template <int NI>
+__device__ __forceinline__ void eval1d(const FieldParams &P, const Ray1D &R, Cells1D<NI> &C, bool &cache_valid,
+                                       float nf, float qx, float qy, float qz, bool cond, float &f, float &nv,
+                                       float &ndv) {
+#pragma clang fp contract(off)
+    float u = ray_coord(R, qx, qy, qz);
+    float v, dv;
+    noise1d_raw<NI>(C, cache_valid, u, R.seed, v, dv);
+    v = v / nf;
+    dv = dv / nf;
+    if (cond && R.cond) cond_splat1d(R, u, v, dv);
+    nv = v;
+    ndv = dv;
+    f = (float)((double)(P.amp_const * v) + mean_value_d(P, qx, qy, qz));
+}
+
+// neeShared + neeGrad + neePDF (SparseConvolutionNoise.cpp:662-821) for the single-level
+// stationary kernel without the transverse law: the mirror normal `h`, the hit point p,
+// the conditioned derivative along the ray `dvp`; float arithmetic as the reference.
+__device__ __forceinline__ void nee1d(const FieldParams &P, const Ray1D &R, float hx, float hy, float hz, float px,
+                                   float py, float pz, float dvp, float dx, float dy, float dz, float &gx, float &gy,
+                                   float &gz, float &pdf) {
+#pragma clang fp contract(off)
+    float mgx, mgy, mgz;
+    mean_grad_d(P, px, py, pz, mgx, mgy, mgz);
+    const float amp = P.amp_const;
+    const float s0 = amp * R.ia[0], s1 = amp * R.ia[1], s2 = amp * R.ia[2];
+    // M = amp * diag(1/l_aniso) * [tangent bitangent normal]
+    float M[3][3] = {{s0 * R.F.tx, s0 * R.F.bx, s0 * R.F.nx},
+                     {s1 * R.F.ty, s1 * R.F.by, s1 * R.F.ny},
+                     {s2 * R.F.tz, s2 * R.F.bz, s2 * R.F.nz}};
+    // Eigen's 3x3 inverse: cofactors scaled by 1/det
+    float c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+    float c10 = M[0][2] * M[2][1] - M[0][1] * M[2][2];
+    float c20 = M[0][1] * M[1][2] - M[0][2] * M[1][1];
+    float det = c00 * M[0][0] + c10 * M[1][0] + c20 * M[2][0];
+    float idet = 1.0f / det;
+    float I[3][3];
+    I[0][0] = c00 * idet; I[0][1] = c10 * idet; I[0][2] = c20 * idet;
+    I[1][0] = (M[1][2] * M[2][0] - M[1][0] * M[2][2]) * idet;
+    I[1][1] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) * idet;
+    I[1][2] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) * idet;
+    I[2][0] = (M[1][0] * M[2][1] - M[1][1] * M[2][0]) * idet;
+    I[2][1] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) * idet;
+    I[2][2] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) * idet;
+    // the gradient plane: a point and the two transverse axes
+    float ppx = M[0][2] * dvp + mgx, ppy = M[1][2] * dvp + mgy, ppz = M[2][2] * dvp + mgz;
+    float ax = M[0][0], ay = M[1][0], az = M[2][0];
+    float bx = M[0][1], by = M[1][1], bz = M[2][1];
+    float cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+    float cl2 = cx * cx;
+    cl2 += cy * cy;
+    cl2 += cz * cz;
+    float stretch = sqrtf(cl2);
+    float il = 1.0f / stretch;
+    float nx = cx * il, ny = cy * il, nz = cz * il;
+    float isect = dot3(ppx, ppy, ppz, nx, ny, nz) / dot3(hx, hy, hz, nx, ny, nz);
+    gx = isect * hx; gy = isect * hy; gz = isect * hz;
+    pdf = 0.0f;
+    if (isect < 0.0f) return;
+    float rx = gx - mgx, ry = gy - mgy, rz = gz - mgz;
+    float g3x = I[0][0] * rx + I[0][1] * ry;
+    g3x += I[0][2] * rz;
+    float g3y = I[1][0] * rx + I[1][1] * ry;
+    g3y += I[1][2] * rz;
+    // transverse draw density with variance 1/2 per axis (no correlation with the origin)
+    float smx = (float)((double)g3x * 1.4142135623730951 * 1.0);
+    float smy = (float)((double)g3y * 1.4142135623730951 * 1.0);
+    float pdf_area = (float)(exp((double)(-smx * smx) / 2.0) * exp((double)(-smy * smy) / 2.0) / (2.0 * M_PI) * 2.0 *
+                             1.0 * 1.0);
+    float cosl = fabsf(dot3(hx, hy, hz, nx, ny, nz));
+    float pdf_n = pdf_area * (isect * isect) / cosl / stretch;
+    pdf = (float)(pdf_n / (4.0 * (double)dot3(hx, hy, hz, -dx, -dy, -dz)));
+}
+
+// The 1D persistent flight kernel: same lane state machine as flight_kernel, with the
+// 1D field (cells cached per lane), conditioning1D at the origin, the Box-Muller
+// transverse gradient at the endpoint, and optional NEE toward a point light.
+template <int NI>
+__global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn in, const float *__restrict__ lscale,
+                                                          ReplyOut out, unsigned int n, unsigned int *work,
+                                                          unsigned long long *stats, float nf, int use_light,
+                                                          float lx, float ly, float lz, Nee1DOut nee) {
+    __shared__ unsigned int bstat[ST_N];
+    if (threadIdx.x < ST_N) bstat[threadIdx.x] = 0;
+    __syncthreads();
+    const unsigned lane = lane_id();
+    int phase = PH_NEED;
+    unsigned ray = 0;
+    float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, near_t = 0, far_t = 0, step = 0, t0 = 0;
+    float qx = 0, qy = 0, qz = 0, pf = 0.f, t_end = 0.f, val_end = 0.f;
+    int first = 1, sign0 = 1, k = 0, hit = 0;
+    double tk = 0, fac = 0, tp = 0, tt = 0;
+    float prev_v = 0, prev_gx = 0, prev_gy = 0, prev_gz = 0;
+    uint32_t normal_seed = 0;
+    unsigned evals_flight = 0;
+    float gxv = 0, gyv = 0, gzv = 0;
+    uint32_t outcome = 0;
+    Ray1D R = {};
+    Cells1D<NI> C;
+    C.base = 0;
+    bool cache_valid = false;
+
+    for (;;) {
+        unsigned long long need = __ballot(phase == PH_NEED);
+        if (need) {
+            unsigned cnt = (unsigned)__popcll(need);
+            unsigned leader = (unsigned)(__ffsll((long long)need) - 1);
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(work, cnt);
+            base = __shfl(base, leader);
+            if (phase == PH_NEED) {
+                unsigned r = base + mask_rank(need);
+                if (r >= n) {
+                    phase = PH_DEAD;
+                } else {
+                    ray = r;
+                    ox = in.ox[r]; oy = in.oy[r]; oz = in.oz[r];
+                    dx = in.dx[r]; dy = in.dy[r]; dz = in.dz[r];
+                    near_t = in.near_t[r]; far_t = in.far_t[r];
+                    float jit = in.jitter[r];
+                    R.seed = in.noise_seed[r];
+                    normal_seed = in.normal_seed[r];
+                    R.cond = 0;
+                    first = (in.flags[r] & 1u) ? 1 : 0;
+                    if (!first) { prev_v = in.pv[r]; prev_gx = in.pgx[r]; prev_gy = in.pgy[r]; prev_gz = in.pgz[r]; }
+                    evals_flight = 0;
+                    cache_valid = false;
+                    ray_scales(P.aniso, P.l_aniso, P.l_aniso_inv, lscale, r, R.la, R.ia);
+                    float ix = R.ia[0] * dx, iy = R.ia[1] * dy, iz = R.ia[2] * dz;
+                    normalize3(ix, iy, iz);
+                    R.F = make_frame(ix, iy, iz);
+                    if (!isfinite(far_t)) far_t = near_t + 2000.f;
+                    bool bad = !isfinite(ox) || !isfinite(oy) || !isfinite(oz) || !isfinite(dx) || !isfinite(dy) ||
+                               !isfinite(dz) || !isfinite(near_t) || !(jit >= 0.f && jit < 1.f) ||
+                               !(R.la[0] > 0.f && R.la[1] > 0.f && R.la[2] > 0.f) || !isfinite(R.la[0] + R.la[1] + R.la[2]);
+                    if (bad || far_t <= near_t) {
+                        out.t[r] = far_t; out.value[r] = (bad || first) ? 0.f : prev_v;
+                        out.gx[r] = out.gy[r] = out.gz[r] = 0.f;
+                        out.outcome[r] = bad ? GPIS_INVALID : GPIS_EXIT;
+                        if (out.evals) out.evals[r] = 0;
+                        if (use_light) { nee.gx[r] = nee.gy[r] = nee.gz[r] = 0.f; nee.pdf[r] = 0.f; }
+                        atomicAdd(&bstat[bad ? ST_INVALID : ST_EXITS], 1u);
+                        phase = PH_NEED;
+                    } else {
+                        step = (far_t - near_t) / P.min_step_f;
+                        if (P.step_size < step) step = P.step_size;
+                        t0 = mul_add_exact(step, jit, near_t);
+                        if (P.activate && !first) {
+                            phase = PH_COND;
+                            qx = ox; qy = oy; qz = oz;
+                        } else {
+                            phase = PH_F0;
+                            ray_point(ox, oy, oz, dx, dy, dz, (double)near_t, qx, qy, qz);
+                        }
+                    }
+                }
+            }
+        }
+        if (!__ballot(phase != PH_DEAD)) break;
+        if (phase == PH_NEED || phase == PH_DEAD) continue;
+
+        float fv, nv, ndv;
+        eval1d<NI>(P, R, C, cache_valid, nf, qx, qy, qz, phase != PH_COND, fv, nv, ndv);
+        ++evals_flight;
+
+        if (phase == PH_COND) {
+            // conditioning1D (:576-656)
+#pragma clang fp contract(off)
+            const float amp = P.amp_const;
+            R.uo = ray_coord(R, ox, oy, oz);
+            R.v1 = 0.f;
+            R.g1z = 0.f;
+            if (amp != 0.f) {
+                const float mean = (float)mean_value_d(P, ox, oy, oz);
+                R.v1 = (prev_v - mean) / amp - nv;
+                if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
+                    float mgx, mgy, mgz;
+                    mean_grad_d(P, ox, oy, oz, mgx, mgy, mgz);
+                    double x1, y1;
+                    transverse_pair(normal_seed, 0, x1, y1);
+                    float cgx, cgy, cgz;
+                    ray_grad_to_world(R, (float)x1, (float)y1, ndv, cgx, cgy, cgz);
+                    float ddx = (prev_gx - mgx) / amp - cgx;
+                    float ddy = (prev_gy - mgy) / amp - cgy;
+                    float ddz = (prev_gz - mgz) / amp - cgz;
+                    // gradient_scale = -2 * frame.toLocal(l_aniso * delta) * scale^2; only z
+                    // survives without the xy correlation
+                    float gsz = dot3(R.F.nx, R.F.ny, R.F.nz, R.la[0] * ddx, R.la[1] * ddy, R.la[2] * ddz);
+                    R.g1z = -2.f * gsz * 1.0f;
+                }
+            }
+            R.cond = 1;
+            phase = PH_F0;
+            ray_point(ox, oy, oz, dx, dy, dz, (double)near_t, qx, qy, qz);
+        } else if (phase == PH_F0) {
+            sign0 = fv < 0.f ? -1 : 1;
+            pf = fv;
+            k = 0;
+            if (t0 < far_t) {
+                phase = PH_MARCH;
+                ray_point(ox, oy, oz, dx, dy, dz, (double)t0, qx, qy, qz);
+            } else {
+                phase = PH_FAR;
+                ray_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
+            }
+        } else if (phase == PH_MARCH) {
+            ++k;
+            int sc = fv < 0.f ? -1 : 1;
+            tk = lattice_t(t0, k - 1, step);
+            if (!first && k == 1) {
+                sign0 = sc;
+            } else if (sc != sign0) {
+                fac = (double)pf / ((double)pf - (double)fv);
+                tp = lerp_d(tk - (double)step, tk, fac);
+                tt = tp;
+                phase = PH_SECANT;
+                ray_point(ox, oy, oz, dx, dy, dz, tt, qx, qy, qz);
+            }
+            if (phase == PH_MARCH) {
+                pf = fv;
+                double tn = lattice_t(t0, k, step);
+                if (k >= kMaxSteps) {
+                    atomicAdd(&bstat[ST_INVALID], 1u);
+                    out.t[ray] = (float)tn; out.value[ray] = fv;
+                    out.gx[ray] = out.gy[ray] = out.gz[ray] = 0.f;
+                    out.outcome[ray] = GPIS_INVALID;
+                    if (out.evals) out.evals[ray] = evals_flight;
+                    if (use_light) { nee.gx[ray] = nee.gy[ray] = nee.gz[ray] = 0.f; nee.pdf[ray] = 0.f; }
+                    phase = PH_NEED;
+                } else if (tn < (double)far_t) {
+                    ray_point(ox, oy, oz, dx, dy, dz, tn, qx, qy, qz);
+                } else {
+                    phase = PH_FAR;
+                    ray_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
+                }
+            }
+        } else if (phase == PH_SECANT) {
+            bool done = false;
+            if ((fv < 0.f ? -1 : 1) == sign0) {
+                done = true;
+            } else {
+                fac *= 0.9;
+                if (fac <= 0.01) {
+                    tp = 0.0;
+                    done = true;
+                } else {
+                    tp = tt;
+                    tt = lerp_d(tk - (double)step, tk, fac);
+                    ray_point(ox, oy, oz, dx, dy, dz, tt, qx, qy, qz);
+                }
+            }
+            if (done) {
+                hit = 1;
+                t_end = (float)tp;
+                val_end = 0.f;
+                phase = PH_GRAD;
+                grad_point(ox, oy, oz, dx, dy, dz, tp, qx, qy, qz);
+            }
+        } else if (phase == PH_FAR) {
+            hit = 0;
+            t_end = far_t;
+            val_end = fv;
+            phase = PH_GRAD;
+            grad_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
+        } else if (phase == PH_GRAD || phase == PH_NEE) {
+            if (phase == PH_GRAD) {
+                // evaluateGradientNoise1D at the endpoint (:204-242): transverse pair 0 when
+                // tSegment == 0, otherwise an independent second pair
+                double x2, y2;
+                transverse_pair(normal_seed, t_end == 0.0f ? 0 : 1, x2, y2);
+                float wx, wy, wz;
+                ray_grad_to_world(R, (float)x2, (float)y2, ndv, wx, wy, wz);
+                float mgx, mgy, mgz;
+                mean_grad_d(P, qx, qy, qz, mgx, mgy, mgz);
+                const float amp = P.amp_const;
+                {
+#pragma clang fp contract(off)
+                    gxv = amp * wx + mgx; gyv = amp * wy + mgy; gzv = amp * wz + mgz;
+                }
+                if (hit) {
+                    // resumeDistance rejection rules, in double (GaussianProcessMedium.cpp:289-369)
+                    const double gx = gxv, gy = gyv, gz = gzv;
+                    double gd = gx * (double)dx;
+                    gd += gy * (double)dy;
+                    gd += gz * (double)dz;
+                    double g2 = 0.0;
+                    g2 += gx * gx;
+                    g2 += gy * gy;
+                    g2 += gz * gz;
+                    if (!isfinite((gx + gy + gz) * (1.0 / 3.0))) outcome = GPIS_FAIL | (GPIS_FAIL_NONFINITE << 8);
+                    else if (gd > 0.0) outcome = GPIS_FAIL | (GPIS_FAIL_OUTWARD << 8);
+                    else if (g2 < (double)1e-7f) outcome = GPIS_FAIL | (GPIS_FAIL_ZERO_GRAD << 8);
+                    else outcome = GPIS_HIT;
+                } else {
+                    outcome = GPIS_EXIT;
+                }
+                if (use_light && (outcome & 0xFFu) == GPIS_HIT) {
+                    // NEE at the scattering point sample.p = pos + t * dir (float)
+                    phase = PH_NEE;
+                    qx = mul_add_exact(t_end, dx, ox);
+                    qy = mul_add_exact(t_end, dy, oy);
+                    qz = mul_add_exact(t_end, dz, oz);
+                    continue;
+                }
+                if (use_light) { nee.gx[ray] = nee.gy[ray] = nee.gz[ray] = 0.f; nee.pdf[ray] = 0.f; }
+            } else {
+                // mirror normal reflecting the ray toward the light, then neeGrad / neePDF
+                float wlx = lx - qx, wly = ly - qy, wlz = lz - qz;
+                normalize3(wlx, wly, wlz);
+                float hx = wlx - dx, hy = wly - dy, hz = wlz - dz;
+                normalize3(hx, hy, hz);
+                float ngx, ngy, ngz, pdf;
+                nee1d(P, R, hx, hy, hz, qx, qy, qz, ndv, dx, dy, dz, ngx, ngy, ngz, pdf);
+                nee.gx[ray] = ngx; nee.gy[ray] = ngy; nee.gz[ray] = ngz; nee.pdf[ray] = pdf;
+            }
+            const unsigned oc = outcome & 0xFFu;
+            atomicAdd(&bstat[ST_FLIGHTS], 1u);
+            atomicAdd(&bstat[ST_EVALS], evals_flight);
+            atomicAdd(&bstat[ST_STEPS], (unsigned)k);
+            atomicAdd(&bstat[oc == GPIS_HIT ? ST_HITS : (oc == GPIS_EXIT ? ST_EXITS : ST_FAILS)], 1u);
+            out.t[ray] = t_end; out.value[ray] = val_end;
+            out.gx[ray] = gxv; out.gy[ray] = gyv; out.gz[ray] = gzv;
+            out.outcome[ray] = outcome;
+            if (out.evals) out.evals[ray] = evals_flight;
+            phase = PH_NEED;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < ST_N && bstat[threadIdx.x]) atomicAdd(&stats[threadIdx.x], (unsigned long long)bstat[threadIdx.x]);
+}
+
 template <bool PATHS>
 static void launch_flights(gpis_ctx *c, unsigned blocks, unsigned threads, hipStream_t st, RayIn in, ReplyOut out,
                            PathOut pout, unsigned n, int max_bounces, uint32_t bounce_seed, float box) {
@@ -735,6 +1092,7 @@ gpis_status gpis_eval_points(gpis_ctx *c, const float *x, const float *y, const 
 }
 
 static gpis_status check_rays(gpis_ctx *c, const gpis_ray_soa *r, int64_t n) {
+    if (c->P.one_d) return fail(c, GPIS_INVALID_ARGUMENT, "1D_sampling media use gpis_trace_flights_1d");
     if (!r) return fail(c, GPIS_INVALID_ARGUMENT, "null rays");
     if (n < 0 || n >= (1LL << 31)) return fail(c, GPIS_INVALID_ARGUMENT, "ray count out of range");
     if (!r->ox || !r->oy || !r->oz || !r->dx || !r->dy || !r->dz || !r->near_t || !r->far_t || !r->jitter ||
@@ -808,6 +1166,39 @@ gpis_status gpis_set_refill(gpis_ctx *c, int32_t policy) {
     if (!c) return GPIS_INVALID_ARGUMENT;
     if (policy != 0 && policy != 1) return fail(c, GPIS_INVALID_ARGUMENT, "refill policy must be 0 (lane) or 1 (wave)");
     c->wave_refill = policy;
+    return GPIS_OK;
+}
+
+gpis_status gpis_trace_flights_1d(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, const gpis_reply_soa *reply,
+                                  const float *light, const gpis_nee_out *nee, void *stream) {
+    if (!c) return GPIS_INVALID_ARGUMENT;
+    if (!c->P.one_d) return fail(c, GPIS_INVALID_ARGUMENT, "the medium is not 1D_sampling");
+    if (!rays) return fail(c, GPIS_INVALID_ARGUMENT, "null rays");
+    if (n < 0 || n >= (1LL << 31)) return fail(c, GPIS_INVALID_ARGUMENT, "ray count out of range");
+    if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz || !rays->near_t ||
+        !rays->far_t || !rays->jitter || !rays->noise_seed || !rays->normal_seed || !rays->flags ||
+        !rays->prev_value || !rays->prev_gx || !rays->prev_gy || !rays->prev_gz)
+        return fail(c, GPIS_INVALID_ARGUMENT, "null ray field");
+    if (!reply || !reply->t || !reply->value || !reply->gx || !reply->gy || !reply->gz || !reply->outcome)
+        return fail(c, GPIS_INVALID_ARGUMENT, "null reply field");
+    if (light && (!nee || !nee->gx || !nee->gy || !nee->gz || !nee->pdf))
+        return fail(c, GPIS_INVALID_ARGUMENT, "NEE needs all four output arrays");
+    if (c->P.N != 8) return fail(c, GPIS_UNSUPPORTED, "1D flights are compiled for 8 impulses per cell");
+    if (n == 0) return GPIS_OK;
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(c, hipMemsetAsync(c->work, 0, sizeof(unsigned), st));
+    unsigned threads = 256;
+    int wpc = c->waves_per_cu > 0 ? c->waves_per_cu : 8;
+    long long want = (long long)c->num_cus * wpc * 64 / threads;
+    long long need = (n + threads - 1) / threads;
+    unsigned blocks = (unsigned)(need < want ? need : (want < 1 ? 1 : want));
+    ReplyOut out = {reply->t, reply->value, reply->gx, reply->gy, reply->gz, reply->outcome, reply->evals};
+    Nee1DOut no = {};
+    if (light) no = {nee->gx, nee->gy, nee->gz, nee->pdf};
+    hipLaunchKernelGGL((flight1d_kernel<8>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+                       rays->length_scale, out, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
+                       light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, no);
+    HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }
 

@@ -207,9 +207,13 @@ class RayBatch:
 
     @classmethod
     def from_numpy(cls, arrays: Dict[str, np.ndarray], device="cuda"):
+        """`arrays` holds RAY_FIELDS and, for 1D media with a per-ray SE length
+        scale, an optional float32 "length_scale"."""
         import torch
         out = {}
-        for k in RAY_FIELDS:
+        for k in RAY_FIELDS + ("length_scale",):
+            if k not in arrays:
+                continue
             a = np.ascontiguousarray(arrays[k])
             if a.dtype == np.uint32:
                 a = a.view(np.int32)
@@ -220,7 +224,8 @@ class RayBatch:
         s = A.RaySoA()
         for k in RAY_FIELDS:
             setattr(s, k, self.t[k].data_ptr())
-        s.length_scale = None
+        ls = self.t.get("length_scale")
+        s.length_scale = None if ls is None else ls.data_ptr()
         return s
 
 
@@ -304,6 +309,30 @@ class GpisDevice:
         soa = rays.soa()
         A.check(self.L.gpis_trace_flights(self.h, C.byref(soa), rays.n, C.byref(r), self._stream()),
                 self.h, "gpis_trace_flights")
+        return reply
+
+    def trace_flights_1d(self, rays: RayBatch, light=None, reply=None):
+        """One 1D flight per ray (1D_sampling media).  With `light` = (x, y, z), each
+        hit also returns the NEE gradient and pdf toward that point light in
+        reply["nee_gx"/"nee_gy"/"nee_gz"/"nee_pdf"]."""
+        import torch
+        if reply is None:
+            reply = self.alloc_reply(rays.n)
+        r = A.ReplySoA()
+        for k in ("t", "value", "gx", "gy", "gz", "outcome", "evals"):
+            setattr(r, k, reply[k].data_ptr())
+        soa = rays.soa()
+        lp, nee = None, None
+        if light is not None:
+            for k in ("nee_gx", "nee_gy", "nee_gz", "nee_pdf"):
+                if k not in reply:
+                    reply[k] = torch.empty(rays.n, dtype=torch.float32, device=reply["t"].device)
+            nee = A.NeeOut(reply["nee_gx"].data_ptr(), reply["nee_gy"].data_ptr(), reply["nee_gz"].data_ptr(),
+                           reply["nee_pdf"].data_ptr())
+            lp = (C.c_float * 3)(*[float(x) for x in light])
+        A.check(self.L.gpis_trace_flights_1d(self.h, C.byref(soa), rays.n, C.byref(r), lp,
+                                             C.byref(nee) if nee is not None else None, self._stream()),
+                self.h, "gpis_trace_flights_1d")
         return reply
 
     def alloc_path_reply(self, n, device="cuda"):

@@ -17,6 +17,8 @@ SEED_POINTS = 0x5EED0000
 SEED_RAYS = 0x5EED0001
 SEED_CAMERA = 0x5EED0002
 SEED_BOX = 0x5EED0003
+SEED_RENEWAL1D = 0x5EED0004
+LIGHT_POS = (0.0, 2.0, 0.0)
 
 
 def config0_json(length_scale=0.1):
@@ -58,9 +60,34 @@ def config2_json():
     }
 
 
+def config3_json():
+    """1D along-ray SE process (sigma 1) over the sphere SDF mean of the stationary
+    field (r 0.5), Renewal+ without xy correlation, MIS scheme.  Each ray carries its
+    own SE length scale; the medium's lengthScale is the default for rays without one."""
+    return {
+        "type": "sparse_conv_noise", "max_bounces": 1024,
+        "gaussian_process": {
+            "type": "standard",
+            "mean": {"type": "spherical", "center": [0.0, 0.0, 0.0], "radius": 0.5},
+            "covariance": {"type": "squared_exponential", "sigma": 1.0, "lengthScale": 0.1},
+        },
+        "step_size": 0.01, "min_step": 8, "seed": 3, "impulse_density": 8,
+        "single_realization": False, "correlation_context": "renewal+", "intersect_method": "gp_discrete",
+        "isotropic_3D_sampling": False, "1D_sampling": True, "1D_sampling_scheme": "mis",
+        "1D_gradient_correlationXY": False,
+    }
+
+
 def points_workload(n=1 << 20, seed=SEED_POINTS):
     rng = np.random.default_rng(seed)
     return rng.uniform(-1.0, 1.0, size=(n, 3)).astype(np.float32)
+
+
+def _jitter(rng, n):
+    """Uniform floats in [0, 1) formed as UniformSampler::next1D forms them from a
+    32-bit draw (BitManip::normalizedUint): never 1.0 after rounding."""
+    x = rng.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)
+    return (((x >> np.uint32(9)) | np.uint32(0x3F800000)).view(np.float32) - np.float32(1.0)).astype(np.float32)
 
 
 def _empty_rays(n):
@@ -99,7 +126,7 @@ def sphere_rays(medium, n=65536, seed=SEED_RAYS, radius=2.0, scene_seed=0):
     rays["ox"], rays["oy"], rays["oz"] = o[:, 0].copy(), o[:, 1].copy(), o[:, 2].copy()
     rays["dx"], rays["dy"], rays["dz"] = d[:, 0].copy(), d[:, 1].copy(), d[:, 2].copy()
     rays["far_t"] = (-2.0 * (o.astype(np.float64) * d).sum(1)).astype(np.float32)
-    rays["jitter"] = rng.uniform(0, 1, size=n).astype(np.float32)
+    rays["jitter"] = _jitter(rng, n)
     pss = np.zeros((n, 4), np.uint32)
     pss[:, 0] = np.arange(n, dtype=np.uint32)
     _fill_seeds(rays, medium, pss, scene_seed)
@@ -151,7 +178,7 @@ def camera_rays(medium, width=2048, height=2048, fov_deg=40.0, eye=(0.0, 0.0, 3.
     rays["dx"], rays["dy"], rays["dz"] = d[:, 0].copy(), d[:, 1].copy(), d[:, 2].copy()
     rays["near_t"] = tn[keep].astype(np.float32)
     rays["far_t"] = tf[keep].astype(np.float32)
-    rays["jitter"] = rng.uniform(0, 1, size=n).astype(np.float32)
+    rays["jitter"] = _jitter(rng, n)
     pss = np.zeros((n, 4), np.uint32)
     pss[:, 0] = px[keep]
     pss[:, 1] = py[keep]
@@ -188,12 +215,49 @@ def box_surface_rays(medium, n=1 << 20, half=0.6, seed=SEED_BOX, scene_seed=2, b
     rays["dx"], rays["dy"], rays["dz"] = d[:, 0].copy(), d[:, 1].copy(), d[:, 2].copy()
     _, tf = box_interval(o.astype(np.float64), d.astype(np.float64), box)
     rays["far_t"] = tf.astype(np.float32)
-    rays["jitter"] = rng.uniform(0, 1, size=n).astype(np.float32)
+    rays["jitter"] = _jitter(rng, n)
     rays["flags"] = np.zeros(n, np.uint32)
     rays["prev_value"] = np.zeros(n, np.float32)
     rays["prev_gx"] = nrm[:, 0].astype(np.float32)
     rays["prev_gy"] = nrm[:, 1].astype(np.float32)
     rays["prev_gz"] = nrm[:, 2].astype(np.float32)
+    pss = np.zeros((n, 4), np.uint32)
+    pss[:, 0] = np.arange(n, dtype=np.uint32)
+    _fill_seeds(rays, medium, pss, scene_seed)
+    return rays
+
+
+def renewal1d_rays(medium, n=1 << 21, seed=SEED_RENEWAL1D, scene_seed=3, box=1.0, ls_range=(0.02, 0.2)):
+    """Secondary 1D flights: origins uniform in the shell 0.55 <= |x| <= 0.95 around the
+    sphere mean, directions toward uniform points of the ball of radius 0.5 (the
+    flight runs to the exit of [-box, box]^3).  Each ray is conditioned on a value
+    v0 ~ N(0.3, 0.2^2) and a gradient whose derivative along the ray is ~ N(0, 1)
+    (prev_grad = g_d * dir), with its own SE length scale l ~ U(ls_range)."""
+    rng = np.random.default_rng(seed)
+    o = rng.normal(size=(n, 3))
+    o /= np.linalg.norm(o, axis=1, keepdims=True)
+    rad = (0.55 ** 3 + rng.uniform(size=(n, 1)) * (0.95 ** 3 - 0.55 ** 3)) ** (1.0 / 3.0)
+    o *= rad
+    u = rng.normal(size=(n, 3))
+    u = u / np.linalg.norm(u, axis=1, keepdims=True) * 0.5 * rng.uniform(0, 1, size=(n, 1)) ** (1.0 / 3.0)
+    d = u - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = o.astype(np.float32)
+    d = d.astype(np.float32)
+    d = d / np.sqrt((d.astype(np.float64) ** 2).sum(1, keepdims=True)).astype(np.float32)
+    rays = _empty_rays(n)
+    rays["ox"], rays["oy"], rays["oz"] = o[:, 0].copy(), o[:, 1].copy(), o[:, 2].copy()
+    rays["dx"], rays["dy"], rays["dz"] = d[:, 0].copy(), d[:, 1].copy(), d[:, 2].copy()
+    _, tf = box_interval(o.astype(np.float64), d.astype(np.float64), box)
+    rays["far_t"] = tf.astype(np.float32)
+    rays["jitter"] = _jitter(rng, n)
+    rays["flags"] = np.zeros(n, np.uint32)
+    rays["prev_value"] = rng.normal(0.3, 0.2, size=n).astype(np.float32)
+    gd = rng.normal(0.0, 1.0, size=n).astype(np.float32)
+    rays["prev_gx"] = (gd * d[:, 0]).astype(np.float32)
+    rays["prev_gy"] = (gd * d[:, 1]).astype(np.float32)
+    rays["prev_gz"] = (gd * d[:, 2]).astype(np.float32)
+    rays["length_scale"] = rng.uniform(ls_range[0], ls_range[1], size=n).astype(np.float32)
     pss = np.zeros((n, 4), np.uint32)
     pss[:, 0] = np.arange(n, dtype=np.uint32)
     _fill_seeds(rays, medium, pss, scene_seed)

@@ -153,6 +153,23 @@ gpis_status gpis_trace_paths(gpis_ctx *ctx, const gpis_ray_soa *rays, int64_t n,
                              int32_t max_bounces, uint32_t bounce_seed, float box,
                              const gpis_path_reply *reply, void *stream);
 
+/* Next-event estimation outputs of a 1D flight (neeGrad and neePDF,
+ * SparseConvolutionNoise.cpp:662-821): the gradient implied by the mirror normal that
+ * reflects the ray toward the light, and the solid-angle pdf of sampling it. */
+typedef struct gpis_nee_out {
+    float *gx, *gy, *gz, *pdf;
+} gpis_nee_out;
+
+/* 1D_sampling media (evaluateNoise1DNormalized / conditioning1D / evaluateGradientNoise1D,
+ * SparseConvolutionNoise.cpp:178-243, 379-485, 576-656): one flight per request.
+ * rays->length_scale may give each ray its own SE length scale.  When `light` is
+ * non-NULL, every hit also computes NEE toward the point light at light[0..2]:
+ * the mirror normal h = normalize(normalize(light - p) - dir) at the scattering point
+ * p = origin + t * dir, then neeGrad(h) and neePDF(h) (zeros for other outcomes). */
+gpis_status gpis_trace_flights_1d(gpis_ctx *ctx, const gpis_ray_soa *rays, int64_t n,
+                                  const gpis_reply_soa *reply, const float *light,
+                                  const gpis_nee_out *nee, void *stream);
+
 /* Accumulated device counters (synchronises the context's stream). */
 gpis_status gpis_get_counters(gpis_ctx *ctx, gpis_counters *out);
 gpis_status gpis_reset_counters(gpis_ctx *ctx);

@@ -1076,3 +1076,51 @@ void orc_nee(const orc_field *f, const orc_ray *ray, const float normal_in[3], c
     float pdf_n = pdf_area * (isect * isect) / cosl / stretch;
     *pdf_out = (float)(pdf_n / (4.0 * vdot(normal, vscale(-1.f, dir))));
 }
+
+/* ---- per-ray length scale -------------------------------------------- */
+typedef struct {
+    const orc_medium_desc *d; const orc_ray *rays; const float *ls; orc_flight *fl; int64_t n0, n1;
+} ls_job_t;
+static void *ls_worker(void *arg) {
+    ls_job_t *j = (ls_job_t *)arg;
+    for (int64_t i = j->n0; i < j->n1; ++i) {
+        orc_medium_desc d = *j->d;
+        if (j->ls) d.length_scale = j->ls[i];
+        orc_field *f = orc_field_create(&d);
+        orc_trace_flight(f, &j->rays[i], &j->fl[i]);
+        orc_field_destroy(f);
+    }
+    return NULL;
+}
+void orc_trace_flights_ls(const orc_medium_desc *desc, const orc_ray *rays, const float *ls, int64_t n,
+                          orc_flight *out, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256]; ls_job_t jb[256];
+    for (int t = 0; t < nthreads; ++t) {
+        jb[t].d = desc; jb[t].rays = rays; jb[t].ls = ls; jb[t].fl = out;
+        jb[t].n0 = n * t / nthreads; jb[t].n1 = n * (t + 1) / nthreads;
+        pthread_create(&th[t], NULL, ls_worker, &jb[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+}
+void orc_nee_ls(const orc_medium_desc *desc, const orc_ray *rays, const float *ls, const float *normals,
+                const float *points, const float *t_segment, int64_t n, float *grads, float *pdfs) {
+    for (int64_t i = 0; i < n; ++i) {
+        orc_medium_desc d = *desc;
+        if (ls) d.length_scale = ls[i];
+        orc_field *f = orc_field_create(&d);
+        orc_nee(f, &rays[i], normals + 3 * i, points + 3 * i, t_segment[i], grads + 3 * i, pdfs + i);
+        orc_field_destroy(f);
+    }
+}
+void orc_conditioned_eval_ls(const orc_medium_desc *desc, const orc_ray *rays, const float *ls,
+                             const float *q, const float *t_segment, int64_t n, float *out4) {
+    for (int64_t i = 0; i < n; ++i) {
+        orc_medium_desc d = *desc;
+        if (ls) d.length_scale = ls[i];
+        orc_field *f = orc_field_create(&d);
+        orc_conditioned_eval(f, &rays[i], q + 3 * i, t_segment[i], out4 + 4 * i);
+        orc_field_destroy(f);
+    }
+}

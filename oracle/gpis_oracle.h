@@ -139,6 +139,16 @@ void orc_conditioned_eval(const orc_field *f, const orc_ray *ray, const float q[
 void orc_nee(const orc_field *f, const orc_ray *ray, const float normal[3],
              const float p[3], float t_segment, float grad_out[3], float *pdf_out);
 
+/* Per-ray SE length scale (1D media whose rays each draw their own l): the same
+ * flights and NEE as above, each ray on the field of `desc` with length_scale =
+ * ls[i].  `normals`/`points`/`t_segment` are per ray (3, 3, 1 values). */
+void orc_trace_flights_ls(const orc_medium_desc *desc, const orc_ray *rays, const float *ls, int64_t n,
+                          orc_flight *out, int nthreads);
+void orc_conditioned_eval_ls(const orc_medium_desc *desc, const orc_ray *rays, const float *ls,
+                             const float *q, const float *t_segment, int64_t n, float *out4);
+void orc_nee_ls(const orc_medium_desc *desc, const orc_ray *rays, const float *ls, const float *normals,
+                const float *points, const float *t_segment, int64_t n, float *grads, float *pdfs);
+
 #ifdef __cplusplus
 }
 #endif

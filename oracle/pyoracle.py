@@ -93,6 +93,9 @@ def lib():
         L.orc_kernel_radius.restype = C.c_float; L.orc_kernel_radius.argtypes = [vp]
         L.orc_trace_flight.argtypes = [vp, C.POINTER(Ray), C.POINTER(Flight)]
         L.orc_trace_flights.argtypes = [vp, vp, C.c_int64, vp, C.c_int]
+        L.orc_trace_flights_ls.argtypes = [vp, vp, vp, C.c_int64, vp, C.c_int]
+        L.orc_nee_ls.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int64, vp, vp]
+        L.orc_conditioned_eval_ls.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp]
         L.orc_trace_flight_perturbed.argtypes = [vp, C.POINTER(Ray), C.c_float, C.c_uint32, C.POINTER(Flight)]
         L.orc_conditioned_eval.argtypes = [vp, C.POINTER(Ray), f32p, C.c_float, f32p]
         L.orc_nee.argtypes = [vp, C.POINTER(Ray), f32p, f32p, C.c_float, f32p, f32p]
@@ -253,3 +256,41 @@ class Field:
         pdf = np.zeros(1, np.float32)
         lib().orc_nee(self.h, C.byref(r), _fp(nn), _fp(pp), t_segment, _fp(g), _fp(pdf))
         return g, float(pdf[0])
+
+
+def trace_ls(desc: MediumDesc, rays: np.ndarray, ls, nthreads=1) -> np.ndarray:
+    """Flights with a per-ray SE length scale `ls` (None: the medium's own)."""
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    out = np.zeros(rays.shape[0], FLIGHT_DTYPE)
+    lsa = None if ls is None else np.ascontiguousarray(ls, np.float32)
+    lib().orc_trace_flights_ls(C.byref(desc), rays.ctypes.data, None if lsa is None else lsa.ctypes.data,
+                               rays.shape[0], out.ctypes.data, nthreads)
+    return out
+
+
+def nee_ls(desc: MediumDesc, rays: np.ndarray, ls, normals, points, t_segment):
+    """neeGrad / neePDF per ray with a per-ray SE length scale."""
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    n = rays.shape[0]
+    lsa = None if ls is None else np.ascontiguousarray(ls, np.float32)
+    nn = np.ascontiguousarray(normals, np.float32).reshape(n, 3)
+    pp = np.ascontiguousarray(points, np.float32).reshape(n, 3)
+    ts = np.ascontiguousarray(t_segment, np.float32).reshape(n)
+    g = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    lib().orc_nee_ls(C.byref(desc), rays.ctypes.data, None if lsa is None else lsa.ctypes.data, nn.ctypes.data,
+                     pp.ctypes.data, ts.ctypes.data, n, g.ctypes.data, pdf.ctypes.data)
+    return g, pdf
+
+
+def conditioned_eval_ls(desc: MediumDesc, rays: np.ndarray, ls, q, t_segment):
+    """Value and gradient of each ray's conditioned realization at q[i] (per-ray l)."""
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    n = rays.shape[0]
+    lsa = None if ls is None else np.ascontiguousarray(ls, np.float32)
+    qq = np.ascontiguousarray(q, np.float32).reshape(n, 3)
+    ts = np.ascontiguousarray(t_segment, np.float32).reshape(n)
+    out = np.zeros((n, 4), np.float32)
+    lib().orc_conditioned_eval_ls(C.byref(desc), rays.ctypes.data, None if lsa is None else lsa.ctypes.data,
+                                  qq.ctypes.data, ts.ctypes.data, n, out.ctypes.data)
+    return out

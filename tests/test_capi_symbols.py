@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     txt = open(os.path.join(ROOT, "include", "gpis_hip.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(gpis_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(gpis_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_header_matches_binding_list():
