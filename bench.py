@@ -30,6 +30,12 @@ def flops_per_eval_3d(n_imp: int) -> float:
     return 27 * n_imp * 8 + (4.0 * np.pi / 3.0) * n_imp * 24 + 16
 
 
+# ... and per 1D evaluation along a ray: 3 cells x N pair tests (4 flops), 2N in-support
+# splats (10 flops: kernel, derivative, accumulate), 32 for setup, conditioning, mean.
+def flops_per_eval_1d(n_imp: int) -> float:
+    return 3 * n_imp * 4 + 2 * n_imp * 10 + 32
+
+
 PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector peak (packed FMA issue)
 PEAK_FP32_SCALAR_TFLOPS = 78.6    # one FMA per lane per cycle, no packed issue
 
@@ -103,12 +109,107 @@ def cpu_baseline(med, rays, max_seconds=12.0):
                       f"workload through the C oracle on {cores} host threads, {spent:.1f} s"}
 
 
+def cpu_baseline_1d(med, rays, light, max_seconds=12.0):
+    """The oracle on a bounded uniform sample of the 1D workload: conditioned 1D flights
+    with their per-ray length scales, then NEE toward the light at every hit."""
+    from oracle import pyoracle as po
+    from tests.helpers import oracle_desc, oracle_rays
+    from gpis_extend_amd import workloads as W
+
+    desc = oracle_desc(med)
+    cores = min(os.cpu_count() or 1, 64)
+    n_total = rays["ox"].shape[0]
+    order = np.random.default_rng(12345).permutation(n_total)
+    sample, flights, spent, start_idx = 8192, 0, 0.0, 0
+    while spent < max_seconds and start_idx < n_total:
+        idx = np.sort(order[start_idx:start_idx + sample])
+        start_idx += len(idx)
+        cur = W.rays_subset(rays, idx)
+        orr = oracle_rays(cur, med)
+        t0 = time.perf_counter()
+        res = po.trace_ls(desc, orr, cur["length_scale"], nthreads=cores)
+        hit = np.nonzero(res["outcome"] == po.HIT)[0]
+        t = res["t"][hit].astype(np.float32)
+        p = np.stack([cur["ox"][hit] + t * cur["dx"][hit], cur["oy"][hit] + t * cur["dy"][hit],
+                      cur["oz"][hit] + t * cur["dz"][hit]], 1).astype(np.float32)
+        wl = np.asarray(light, np.float32)[None, :] - p
+        wl = wl / np.linalg.norm(wl, axis=1, keepdims=True)
+        d = np.stack([cur["dx"][hit], cur["dy"][hit], cur["dz"][hit]], 1)
+        h = (wl - d) / np.linalg.norm(wl - d, axis=1, keepdims=True)
+        po.nee_ls(desc, orr[hit], cur["length_scale"][hit], h.astype(np.float32), p, t)
+        spent += time.perf_counter() - t0
+        flights += len(idx)
+    return {"value": flights / spent, "unit": "flights/s", "cores": cores, "kind": "port",
+            "sample": f"{flights} conditioned 1D flights (+ NEE at each hit) of the same workload through "
+                      f"the C oracle, flights on {cores} host threads and NEE on 1, {spent:.1f} s"}
+
+
+def run_1d(args):
+    """1D along-ray Renewal+ flights with NEE toward a point light (one step = one
+    launch over all rays)."""
+    import torch
+    from gpis_extend_amd import GpisDevice, RayBatch, SparseConvolutionNoiseMedium, workloads as W
+
+    med = SparseConvolutionNoiseMedium.from_json(W.config3_json())
+    rays = W.renewal1d_rays(med, n=args.rays_1d)
+    n = rays["ox"].shape[0]
+    dev = GpisDevice(med)
+    batch = RayBatch.from_numpy(rays)
+    reply = dev.alloc_reply(n)
+    stream = torch.cuda.current_stream()
+    if args.waves_per_cu:
+        dev.set_launch(args.waves_per_cu, 0)
+    for _ in range(args.warmup):
+        dev.trace_flights_1d(batch, light=W.LIGHT_POS, reply=reply)
+    torch.cuda.synchronize()
+    dev.reset_counters()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        dev.trace_flights_1d(batch, light=W.LIGHT_POS, reply=reply)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(starts, ends)]))
+    c = dev.counters()
+    flights_per_step = c["flights"] / args.steps
+    evals_per_step = c["evals"] / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    fpe = flops_per_eval_1d(int(med.impulse_density))
+    achieved = evals_per_step * fpe / (kernel_ms / 1e3) / 1e12
+    line = {
+        "metric": "GPIS 1D flights/s on 1 GPU (plain method, NEE at each hit)",
+        "value": flights_per_step / (ms_per_step / 1e3), "unit": "flights/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain 1D along-ray Renewal+ flights: SE sigma 1, l ~ U[0.02, 0.2] per ray, "
+                               "sphere SDF mean, v0 ~ N(0.3, 0.2^2), directional derivative ~ N(0, 1), "
+                               "NEE toward a point light at (0, 2, 0)",
+                   "rays": n, "flights_per_step": flights_per_step, "evals_per_step": evals_per_step,
+                   "hits_per_step": c["hits"] / args.steps, "fails_per_step": c["fails"] / args.steps},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac_scalar_issue_peak": achieved / PEAK_FP32_SCALAR_TFLOPS,
+                     "kernel": "flight1d_kernel<8>", "kernel_ms": kernel_ms, "flops_per_eval": fpe},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline_1d(med, rays, W.LIGHT_POS)
+    print(json.dumps(line), flush=True)
+    dev.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--width", type=int, default=2048)
+    ap.add_argument("--workload", choices=("c1", "c3"), default="c1",
+                    help="c1: camera-ray paths on the 3D field (default); c3: 1D Renewal+ flights with NEE")
+    ap.add_argument("--rays-1d", type=int, default=1 << 21)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true",
@@ -121,6 +222,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(flights shard by ray index; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "c3":
+        return run_1d(args)
 
     import torch
     from gpis_extend_amd import GpisDevice, RayBatch, SparseConvolutionNoiseMedium, workloads as W

@@ -30,7 +30,8 @@ bool supported(const RaySegmentRequest &r) {
 }
 }  // namespace
 
-HipRayBatchBackend::HipRayBatchBackend(const gpis_field_desc &desc, std::size_t capacity) : capacity_(capacity) {
+HipRayBatchBackend::HipRayBatchBackend(const gpis_field_desc &desc, std::size_t capacity)
+    : capacity_(capacity), oneD_(desc.one_d != 0) {
     if (gpis_create(&desc, &ctx_) != GPIS_OK) {
         error_ = "gpis_create failed";
         ctx_ = nullptr;
@@ -121,7 +122,9 @@ std::vector<RaySegmentReply> HipRayBatchBackend::process(const std::vector<RaySe
         auto OF = [&](int a) { return reinterpret_cast<float *>(dout + a * capacity_ * 4); };
         auto OU = [&](int a) { return reinterpret_cast<std::uint32_t *>(dout + a * capacity_ * 4); };
         gpis_reply_soa rep{OF(0), OF(1), OF(2), OF(3), OF(4), OU(5), OU(6)};
-        if (good) good = gpis_trace_flights(ctx_, &rays, (int64_t)n, &rep, st) == GPIS_OK;
+        if (good)
+            good = (oneD_ ? gpis_trace_flights_1d(ctx_, &rays, (int64_t)n, &rep, nullptr, nullptr, st)
+                          : gpis_trace_flights(ctx_, &rays, (int64_t)n, &rep, st)) == GPIS_OK;
         for (int a = 0; a < kOut && good; ++a)
             good = hipMemcpyAsync(static_cast<char *>(hostOut_) + a * capacity_ * 4, dout + a * capacity_ * 4, n * 4,
                                   hipMemcpyDeviceToHost, st) == hipSuccess;

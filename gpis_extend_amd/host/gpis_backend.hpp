@@ -96,6 +96,7 @@ private:
 
     gpis_ctx *ctx_ = nullptr;
     std::size_t capacity_;
+    bool oneD_;  // 1D_sampling medium: flights go to gpis_trace_flights_1d
     void *stream_ = nullptr;
     // pinned host staging and device buffers, SoA, `capacity_` entries each
     void *hostIn_ = nullptr, *hostOut_ = nullptr, *devIn_ = nullptr, *devOut_ = nullptr;
