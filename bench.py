@@ -8,6 +8,11 @@ the box [-1,1]^3, each traced as a path of one primary flight plus up to 2
 cosine-weighted diffuse bounces (bounce seed 7), all in one persistent launch.
 Inputs are generated on the host, copied to HBM before timing, and stay there.
 
+The same line carries, under "plain_1d" and "surrogate", the 1D Renewal+ flights
+with NEE (2,097,152 rays) and the learned surrogate against the plain method
+(trained on 2^24 plain flights, served on a 512x512 image at 16 spp), each with
+its own timing.
+
 Prints one JSON line (see DESIGN.md for the field definitions).
 """
 from __future__ import annotations
@@ -197,8 +202,121 @@ def run_1d(args):
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_1d(med, rays, W.LIGHT_POS)
-    print(json.dumps(line), flush=True)
     dev.close()
+    return line
+
+
+MLP_FLOPS_PER_ROW = 2 * (8 * 128 + 3 * 128 * 128 + 128 * 32)  # 108,544 (forward)
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+
+
+def run_surrogate(args):
+    """The learned flight sampler: train on plain-method flights of the renewal
+    workload (seed 11), then serve fresh rays arranged as a 512x512 image with 16
+    samples per pixel (seed 12), against the plain method on the same rays."""
+    import torch
+    from gpis_extend_amd import GpisDevice, RayBatch, SparseConvolutionNoiseMedium, workloads as W
+    from gpis_extend_amd import surrogate as S
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    med = SparseConvolutionNoiseMedium.from_json(W.config3_json())
+    dev = GpisDevice(med)
+    net = S.FlightSurrogate(seed=S.SEED_TRAIN)
+    t0 = time.perf_counter()
+    feats, s, ex, oc = S.flight_dataset(dev, net, med, args.train_flights, seed=S.SEED_TRAIN)
+    log(f"training set: {feats.shape[0]} plain flights in {time.perf_counter() - t0:.1f} s")
+    cal = S.calibrate(feats, s[~ex])
+    net.set_calibration(cal)
+    cls = S.classes_of(net, s, oc)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    losses = net.train(feats, cls, epochs=args.epochs, batch=65536, lr=1e-3, seed=S.SEED_TRAIN, log=log)
+    e1.record()
+    torch.cuda.synchronize()
+    train_s = e0.elapsed_time(e1) / 1e3
+    samples = (feats.shape[0] // 65536) * 65536 * args.epochs
+    # entropy bound: the 99.5th percentile of the trained network's entropy on training inputs
+    sub = feats[: min(feats.shape[0], 1 << 20)]
+    H = S.entropy_of(net.forward(sub))
+    cal.entropy_max = float(torch.quantile(H.float(), 0.995).item())
+    net.set_calibration(cal)
+    del feats, s, ex, oc, cls
+    torch.cuda.empty_cache()
+
+    rays = W.renewal1d_image_rays(med, width=512, spp=16, seed=S.SEED_INFER)
+    rb = RayBatch.from_numpy(rays)
+    n = rb.n
+    plain = dev.alloc_reply(n)
+    sur = dev.alloc_reply(n)
+    for _ in range(args.warmup):
+        dev.trace_flights_1d(rb, reply=plain)
+        net.flights(dev, rb, reply=sur, count_fallback=False)
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        a.record()
+        for _ in range(args.steps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - w0) / args.steps, a.elapsed_time(b) / args.steps
+
+    plain_wall, plain_ms = timed(lambda: dev.trace_flights_1d(rb, reply=plain))
+    sur_wall, sur_ms = timed(lambda: net.flights(dev, rb, reply=sur, count_fallback=False))
+    _, nfb = net.flights(dev, rb, reply=sur)
+    fe = net.features(dev, rb)
+    _, fwd_ms = timed(lambda: net.forward(fe))
+    torch.cuda.synchronize()
+
+    sp, exp_, _ = S.normalised_distance(rb, plain)
+    ss, _, _ = S.normalised_distance(rb, sur)
+    sp_np, ss_np = sp.cpu().numpy(), ss.cpu().numpy()
+    ks = S.ks_two_sample(sp_np, ss_np)
+    # PIT of the plain distances under the network's CDFs, on the rays it serves
+    served = (sur["evals"] == 0).cpu().numpy()
+    probs = net.forward(fe).cpu().numpy()
+    rng = np.random.default_rng(5)
+    pit = S.pit_ks(probs[served], cal.edges, sp_np[served], exp_.cpu().numpy()[served], rng)
+    # image error: per-pixel mean normalised distance over the 16 samples
+    img_p = sp_np.reshape(-1, 16).mean(1)
+    img_s = ss_np.reshape(-1, 16).mean(1)
+    rays2 = W.renewal1d_image_rays(med, width=512, spp=16, seed=S.SEED_INFER, scene_seed=4)
+    rb2 = RayBatch.from_numpy(rays2)
+    plain2 = dev.trace_flights_1d(rb2)
+    sp2, _, _ = S.normalised_distance(rb2, plain2)
+    img_p2 = sp2.cpu().numpy().reshape(-1, 16).mean(1)
+    rel = lambda a, b: float(np.mean(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)))
+    line = {
+        "metric": "GPIS 1D flights/s on 1 GPU, surrogate (plain fallback) vs plain",
+        "value": n / sur_wall, "unit": "flights/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sur_wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "renewal 1D flights (SE sigma 1, l ~ U[0.02, 0.2] per ray, sphere SDF mean, "
+                               "v0 ~ N(0.3, 0.2^2), derivative ~ N(0, 1)) as a 512x512 image at 16 spp; "
+                               "MLP 8-128x4-32 trained on plain flights",
+                   "rays": n, "train_flights": int(args.train_flights), "epochs": args.epochs, "batch": 65536},
+        "plain": {"value": n / plain_wall, "ms_per_step": plain_wall * 1e3, "kernel_ms": plain_ms},
+        "speedup_vs_plain": plain_wall / sur_wall,
+        "fallback_fraction": nfb / n,
+        "ks_vs_plain": ks, "pit_ks": pit,
+        "image_rel_error": rel(img_s, img_p), "image_rel_error_plain_vs_plain": rel(img_p2, img_p),
+        "train": {"seconds": train_s, "samples_per_s": samples / train_s, "loss_first_epoch": losses[0],
+                  "loss_last_epoch": losses[-1],
+                  "tflops": samples * 3 * MLP_FLOPS_PER_ROW / train_s / 1e12},
+        "roofline": {"bound": "mfma", "achieved": n * MLP_FLOPS_PER_ROW / (fwd_ms / 1e3) / 1e12,
+                     "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": n * MLP_FLOPS_PER_ROW / (fwd_ms / 1e3) / 1e12 / PEAK_BF16_TFLOPS,
+                     "traffic": None, "kernel": "infer_kernel (probabilities only)", "kernel_ms": fwd_ms,
+                     "flops_per_row": MLP_FLOPS_PER_ROW},
+    }
+    net.close()
+    dev.close()
+    return line
 
 
 def main():
@@ -207,8 +325,13 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--width", type=int, default=2048)
-    ap.add_argument("--workload", choices=("c1", "c3"), default="c1",
-                    help="c1: camera-ray paths on the 3D field (default); c3: 1D Renewal+ flights with NEE")
+    ap.add_argument("--workload", choices=("c1", "c3", "c4"), default="c1",
+                    help="c1: camera-ray paths on the 3D field (default); c3: 1D Renewal+ flights with NEE; "
+                         "c4: the learned surrogate against the plain method")
+    ap.add_argument("--train-flights", type=int, default=1 << 24)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="report only the camera-ray paths workload (skip the 1D and surrogate legs)")
     ap.add_argument("--rays-1d", type=int, default=1 << 21)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--waves-per-cu", type=int, default=0)
@@ -224,7 +347,11 @@ def main():
         sys.exit(2)
 
     if args.workload == "c3":
-        return run_1d(args)
+        print(json.dumps(run_1d(args)), flush=True)
+        return
+    if args.workload == "c4":
+        print(json.dumps(run_surrogate(args)), flush=True)
+        return
 
     import torch
     from gpis_extend_amd import GpisDevice, RayBatch, SparseConvolutionNoiseMedium, workloads as W
@@ -300,8 +427,17 @@ def main():
         torch.cuda.synchronize()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(med, rays)
-    print(json.dumps(line), flush=True)
     dev.close()
+    if not args.no_extras:
+        # the other two headline comparisons: plain 1D flights with NEE, and the learned
+        # surrogate against the plain method (its own timing, KS and image error)
+        keep = ("value", "unit", "ms_per_step", "config", "roofline", "cpu_baseline", "plain", "speedup_vs_plain",
+                "fallback_fraction", "ks_vs_plain", "pit_ks", "image_rel_error", "image_rel_error_plain_vs_plain",
+                "train", "metric", "dtype")
+        for key, fn in (("plain_1d", run_1d), ("surrogate", run_surrogate)):
+            sub = fn(args)
+            line[key] = {k: v for k, v in sub.items() if k in keep}
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -27,7 +27,13 @@ EXPORTS = (
     "gpis_create", "gpis_destroy", "gpis_last_error", "gpis_set_domain", "gpis_impulse_table",
     "gpis_eval_points", "gpis_trace_flights", "gpis_trace_paths", "gpis_get_counters",
     "gpis_reset_counters", "gpis_set_launch", "gpis_set_refill", "gpis_trace_flights_1d",
+    # include/gpis_surrogate.h
+    "gpis_net_create", "gpis_net_destroy", "gpis_net_last_error", "gpis_net_set_params", "gpis_net_get_params",
+    "gpis_net_set_calibration", "gpis_net_features_1d", "gpis_net_classes_1d", "gpis_net_gradient",
+    "gpis_net_train_step", "gpis_net_forward", "gpis_surrogate_flights_1d",
 )
+
+NET_FEATURES, NET_CLASSES, NET_PARAMS = 8, 32, 54816
 
 
 class FieldDesc(C.Structure):
@@ -63,6 +69,11 @@ class PathReply(C.Structure):
 
 class NeeOut(C.Structure):
     _fields_ = [(n, _vp) for n in ("gx", "gy", "gz", "pdf")]
+
+
+class NetCalibration(C.Structure):
+    _fields_ = [("feat_mean", C.c_float * 8), ("feat_inv_std", C.c_float * 8), ("hull_lo", C.c_float * 8),
+                ("hull_hi", C.c_float * 8), ("edges", C.c_float * 32), ("entropy_max", C.c_float)]
 
 
 class Counters(C.Structure):
@@ -114,13 +125,40 @@ def lib():
     L.gpis_set_launch.argtypes = [_vp, C.c_int32, C.c_int32]
     L.gpis_set_refill.restype = st
     L.gpis_set_refill.argtypes = [_vp, C.c_int32]
+    L.gpis_net_create.restype = st
+    L.gpis_net_create.argtypes = [C.c_uint64, C.POINTER(_vp)]
+    L.gpis_net_destroy.restype = st
+    L.gpis_net_destroy.argtypes = [_vp]
+    L.gpis_net_last_error.restype = C.c_char_p
+    L.gpis_net_last_error.argtypes = [_vp]
+    L.gpis_net_set_params.restype = st
+    L.gpis_net_set_params.argtypes = [_vp, C.POINTER(C.c_float), C.c_int64]
+    L.gpis_net_get_params.restype = st
+    L.gpis_net_get_params.argtypes = [_vp, C.POINTER(C.c_float), C.c_int64]
+    L.gpis_net_set_calibration.restype = st
+    L.gpis_net_set_calibration.argtypes = [_vp, C.POINTER(NetCalibration)]
+    L.gpis_net_features_1d.restype = st
+    L.gpis_net_features_1d.argtypes = [_vp, C.POINTER(RaySoA), C.c_int64, _vp, _vp]
+    L.gpis_net_classes_1d.restype = st
+    L.gpis_net_classes_1d.argtypes = [_vp, C.POINTER(RaySoA), C.POINTER(ReplySoA), C.c_int64, _vp, _vp]
+    L.gpis_net_gradient.restype = st
+    L.gpis_net_gradient.argtypes = [_vp, _vp, _vp, C.c_int64, _vp, _vp, _vp]
+    L.gpis_net_train_step.restype = st
+    L.gpis_net_train_step.argtypes = [_vp, _vp, _vp, C.c_int64, C.c_float, _vp, _vp]
+    L.gpis_net_forward.restype = st
+    L.gpis_net_forward.argtypes = [_vp, _vp, C.c_int64, _vp, _vp]
+    L.gpis_surrogate_flights_1d.restype = st
+    L.gpis_surrogate_flights_1d.argtypes = [_vp, _vp, C.POINTER(RaySoA), C.c_int64, C.POINTER(ReplySoA),
+                                            C.POINTER(C.c_uint32), _vp]
     _lib = L
     return L
 
 
-def check(status: int, ctx=None, what: str = "") -> None:
+def check(status: int, ctx=None, what: str = "", net=None) -> None:
     if status != GPIS_OK:
         msg = ""
-        if ctx:
+        if net:
+            msg = lib().gpis_net_last_error(net).decode(errors="replace")
+        if ctx and not msg:
             msg = lib().gpis_last_error(ctx).decode(errors="replace")
         raise GpisError(f"{what}: {STATUS_NAMES.get(status, status)} {msg}".strip())

@@ -62,6 +62,7 @@ struct FieldParams {
     float amp_const;     // amplitude when it is spatially constant
     float l_aniso[3], l_aniso_inv[3];
     float aniso[3];  // anisotropy factors of the SE length scale (1D rays rescale by their own l)
+    float length_scale;  // the medium's SE lengthScale
     float l_max;         // max(l_aniso) (splattingKernelRadius)
     float kernel_scale;  // _kernelScale (3)
     int ls_field; float ls_params[4]; float ls_max; float ns_var;

@@ -12,9 +12,14 @@
 #include <cstring>
 #include <string>
 
+#include <random>
+#include <vector>
+
 #include "../../include/gpis_hip.h"
+#include "../../include/gpis_surrogate.h"
 #include "gpis_1d.hpp"
 #include "gpis_field.hpp"
+#include "gpis_mlp.hpp"
 
 using namespace gpis;
 
@@ -75,6 +80,7 @@ static gpis_status build_params(gpis_ctx *c) {
         P.l_aniso_inv[i] = std::isfinite(inv) ? inv : 0.0f;
         P.aniso[i] = d.aniso[i];
     }
+    P.length_scale = d.length_scale;
     P.l_max = fmaxf(fmaxf(P.l_aniso[0], P.l_aniso[1]), P.l_aniso[2]);
     P.kernel_scale = d.kernel_scale;
     P.ls_field = d.ns_ls_field;
@@ -712,7 +718,11 @@ template <int NI>
 __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn in, const float *__restrict__ lscale,
                                                           ReplyOut out, unsigned int n, unsigned int *work,
                                                           unsigned long long *stats, float nf, int use_light,
-                                                          float lx, float ly, float lz, Nee1DOut nee) {
+                                                          float lx, float ly, float lz, Nee1DOut nee,
+                                                          const uint32_t *__restrict__ idx,
+                                                          const uint32_t *__restrict__ n_dev) {
+    // idx / n_dev: optional list of ray indices (and its device-side length) to trace
+    const unsigned nn = n_dev ? *n_dev : n;
     __shared__ unsigned int bstat[ST_N];
     if (threadIdx.x < ST_N) bstat[threadIdx.x] = 0;
     __syncthreads();
@@ -742,10 +752,11 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
             if (lane == leader) base = atomicAdd(work, cnt);
             base = __shfl(base, leader);
             if (phase == PH_NEED) {
-                unsigned r = base + mask_rank(need);
-                if (r >= n) {
+                const unsigned rl = base + mask_rank(need);
+                if (rl >= nn) {
                     phase = PH_DEAD;
                 } else {
+                    const unsigned r = idx ? idx[rl] : rl;
                     ray = r;
                     ox = in.ox[r]; oy = in.oy[r]; oz = in.oz[r];
                     dx = in.dx[r]; dy = in.dy[r]; dz = in.dz[r];
@@ -1169,6 +1180,24 @@ gpis_status gpis_set_refill(gpis_ctx *c, int32_t policy) {
     return GPIS_OK;
 }
 
+// Launch the persistent 1D kernel over n rays, or over the first *n_dev entries of the
+// index list idx (n is then their upper bound).
+static gpis_status launch_flight1d(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, const ReplyOut &out,
+                                   const float *light, const Nee1DOut &nee, const uint32_t *idx,
+                                   const uint32_t *n_dev, hipStream_t st) {
+    HIP_TRY(c, hipMemsetAsync(c->work, 0, sizeof(unsigned), st));
+    unsigned threads = 256;
+    int wpc = c->waves_per_cu > 0 ? c->waves_per_cu : 8;
+    long long want = (long long)c->num_cus * wpc * 64 / threads;
+    long long need = (n + threads - 1) / threads;
+    unsigned blocks = (unsigned)(need < want ? need : (want < 1 ? 1 : want));
+    hipLaunchKernelGGL((flight1d_kernel<8>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+                       rays->length_scale, out, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
+                       light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, nee, idx, n_dev);
+    HIP_TRY(c, hipGetLastError());
+    return GPIS_OK;
+}
+
 gpis_status gpis_trace_flights_1d(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, const gpis_reply_soa *reply,
                                   const float *light, const gpis_nee_out *nee, void *stream) {
     if (!c) return GPIS_INVALID_ARGUMENT;
@@ -1185,21 +1214,10 @@ gpis_status gpis_trace_flights_1d(gpis_ctx *c, const gpis_ray_soa *rays, int64_t
         return fail(c, GPIS_INVALID_ARGUMENT, "NEE needs all four output arrays");
     if (c->P.N != 8) return fail(c, GPIS_UNSUPPORTED, "1D flights are compiled for 8 impulses per cell");
     if (n == 0) return GPIS_OK;
-    hipStream_t st = (hipStream_t)stream;
-    HIP_TRY(c, hipMemsetAsync(c->work, 0, sizeof(unsigned), st));
-    unsigned threads = 256;
-    int wpc = c->waves_per_cu > 0 ? c->waves_per_cu : 8;
-    long long want = (long long)c->num_cus * wpc * 64 / threads;
-    long long need = (n + threads - 1) / threads;
-    unsigned blocks = (unsigned)(need < want ? need : (want < 1 ? 1 : want));
     ReplyOut out = {reply->t, reply->value, reply->gx, reply->gy, reply->gz, reply->outcome, reply->evals};
     Nee1DOut no = {};
     if (light) no = {nee->gx, nee->gy, nee->gz, nee->pdf};
-    hipLaunchKernelGGL((flight1d_kernel<8>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
-                       rays->length_scale, out, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
-                       light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, no);
-    HIP_TRY(c, hipGetLastError());
-    return GPIS_OK;
+    return launch_flight1d(c, rays, n, out, light, no, nullptr, nullptr, (hipStream_t)stream);
 }
 
 gpis_status gpis_get_counters(gpis_ctx *c, gpis_counters *o) {
@@ -1225,6 +1243,368 @@ gpis_status gpis_set_launch(gpis_ctx *c, int32_t waves_per_cu, int32_t block_siz
         return fail(c, GPIS_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
     c->waves_per_cu = waves_per_cu;
     c->block_size = block_size ? block_size : 256;
+    return GPIS_OK;
+}
+
+}  // extern "C"
+
+// ===================================================================== surrogate --
+// Features of a conditioned 1D flight (order documented in include/gpis_surrogate.h).
+__global__ void features1d_kernel(FieldParams P, RayIn in, const float *__restrict__ lscale, int n,
+                                  float *__restrict__ feats) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const float ox = in.ox[r], oy = in.oy[r], oz = in.oz[r];
+    const float dx = in.dx[r], dy = in.dy[r], dz = in.dz[r];
+    const float near_t = in.near_t[r];
+    float far_t = in.far_t[r];
+    if (!isfinite(far_t)) far_t = near_t + 2000.f;
+    float mgx, mgy, mgz;
+    mean_grad_d(P, ox, oy, oz, mgx, mgy, mgz);
+    float f[8];
+    f[0] = in.pv[r];
+    f[1] = dot3(in.pgx[r], in.pgy[r], in.pgz[r], dx, dy, dz);
+    f[2] = lscale ? lscale[r] : P.length_scale;
+    f[3] = P.amp_const * P.amp_const;
+    f[4] = far_t - near_t;
+    f[5] = (float)mean_value_d(P, ox, oy, oz);
+    f[6] = dot3(mgx, mgy, mgz, dx, dy, dz);
+    f[7] = P.kind == GPIS_COV_SE ? 1.0f : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) feats[(size_t)r * 8 + i] = f[i];
+}
+
+// Training class of a plain-method flight: bin of s = (t - near) / (far - near), 31 for
+// an exit, -1 for an invalid request.
+__global__ void classes1d_kernel(RayIn in, const float *__restrict__ t, const uint32_t *__restrict__ outcome, int n,
+                                 const mlp::Calibration *__restrict__ C, int32_t *__restrict__ cls) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t oc = outcome[r] & 0xFFu;
+    int32_t c;
+    if (oc == GPIS_INVALID) {
+        c = -1;
+    } else if (oc == GPIS_EXIT) {
+        c = mlp::kBins;
+    } else {
+        const float near_t = in.near_t[r];
+        float far_t = in.far_t[r];
+        if (!isfinite(far_t)) far_t = near_t + 2000.f;
+        float sv = (t[r] - near_t) / (far_t - near_t);
+        sv = fminf(fmaxf(sv, 0.0f), 0.99999994f);
+        // class 0 holds s <= 0 (an atom when edges[1] == 0: crossings at the origin);
+        // otherwise the last bin whose lower edge is <= s
+        c = 0;
+        if (sv > 0.0f)
+            for (int b = 1; b < mlp::kBins; ++b)
+                if (C->edges[b] <= sv) c = b;
+    }
+    cls[r] = c;
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, int n, uint32_t v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+struct gpis_net {
+    int device = 0;
+    int num_cus = 0;
+    float *theta = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+    float *part = nullptr;
+    int64_t part_chunks = 0;
+    mlp::bf16 *afwd = nullptr, *abwd = nullptr;
+    mlp::Calibration *calib = nullptr;
+    bool calibrated = false;
+    mlp::bf16 *xt = nullptr, *dzt = nullptr;  // dzt and zt live inside the xt allocation
+    float *zt = nullptr;
+    int64_t ws_rows = 0;
+    float *feats = nullptr;
+    int64_t feats_rows = 0;
+    uint32_t *fallback = nullptr, *n_fallback = nullptr;
+    int64_t fb_rows = 0;
+    float *loss = nullptr;
+    int64_t step = 0;
+    std::string err;
+};
+
+static gpis_status nfail(gpis_net *n, gpis_status s, const char *msg) {
+    if (n) n->err = msg;
+    return s;
+}
+#define NET_TRY(n, expr)                                                           \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess) {                                                    \
+            if (n) (n)->err = std::string(#expr) + ": " + hipGetErrorString(_e);   \
+            return GPIS_DEVICE_ERROR;                                              \
+        }                                                                          \
+    } while (0)
+
+static gpis_status net_pack(gpis_net *n, hipStream_t st) {
+    const int total = mlp::kFwdElems + mlp::kBwdElems;
+    hipLaunchKernelGGL(mlp::pack_kernel, dim3((total + 255) / 256), dim3(256), 0, st, n->theta, n->afwd, n->abwd);
+    NET_TRY(n, hipGetLastError());
+    return GPIS_OK;
+}
+
+template <typename T>
+static hipError_t grow(T **p, int64_t *have, int64_t want, int64_t per_row) {
+    if (*have >= want) return hipSuccess;
+    if (*p) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) return e;
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    hipError_t e = hipMalloc((void **)p, (size_t)want * per_row * sizeof(T));
+    if (e == hipSuccess) *have = want;
+    else *have = 0;
+    return e;
+}
+
+// Largest chunk of rows (<= 1024, a multiple of 64) dividing the batch.
+static int chunk_rows_for(int64_t batch) {
+    for (int c = 1024; c >= 64; c >>= 1)
+        if (batch % c == 0) return c;
+    return 0;
+}
+
+static gpis_status net_backprop(gpis_net *n, const float *feats, const int32_t *classes, int64_t batch,
+                                float *loss_sum, hipStream_t st, int *chunks_out) {
+    if (!n->calibrated) return nfail(n, GPIS_INVALID_ARGUMENT, "calibration not set");
+    if (!feats || !classes) return nfail(n, GPIS_INVALID_ARGUMENT, "null features or classes");
+    if (batch < 64 || batch % 64 != 0 || batch > (1LL << 24))
+        return nfail(n, GPIS_INVALID_ARGUMENT, "batch must be a positive multiple of 64, at most 2^24");
+    const int cr = chunk_rows_for(batch);
+    const int chunks = (int)(batch / cr);
+    // workspace per row: bf16 Xt and dZt, fp32 hidden pre-activations (counted in bf16 units)
+    NET_TRY(n, grow(&n->xt, &n->ws_rows, batch, (int64_t)mlp::kXtRows + mlp::kDztRows + 2 * 4 * mlp::kWidth));
+    n->dzt = n->xt + (size_t)mlp::kXtRows * batch;
+    n->zt = (float *)(n->dzt + (size_t)mlp::kDztRows * batch);
+    NET_TRY(n, grow(&n->part, &n->part_chunks, chunks, (int64_t)mlp::kParams));
+    float *ls = loss_sum ? loss_sum : n->loss;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(32), 0, st, (uint32_t *)ls, 1, 0u);
+    hipLaunchKernelGGL(mlp::train_fwd_bwd_kernel, dim3((unsigned)(batch / 64)), dim3(256), 0, st, feats, classes,
+                       (int)batch, n->calib, n->afwd, n->abwd, n->theta, n->xt, n->zt, n->dzt, ls);
+    NET_TRY(n, hipGetLastError());
+    hipLaunchKernelGGL(mlp::wgrad_kernel, dim3((unsigned)chunks, mlp::kLayers), dim3(256), 0, st, n->xt, n->dzt,
+                       (int)batch, cr, n->part);
+    NET_TRY(n, hipGetLastError());
+    *chunks_out = chunks;
+    return GPIS_OK;
+}
+
+extern "C" {
+
+gpis_status gpis_net_create(uint64_t seed, gpis_net **out) {
+    if (!out) return GPIS_INVALID_ARGUMENT;
+    *out = nullptr;
+    gpis_net *n = new gpis_net();
+    hipError_t e = hipGetDevice(&n->device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&n->num_cus, hipDeviceAttributeMultiprocessorCount, n->device);
+    const size_t pb = sizeof(float) * mlp::kParams;
+    if (e == hipSuccess) e = hipMalloc((void **)&n->theta, pb);
+    if (e == hipSuccess) e = hipMalloc((void **)&n->m, pb);
+    if (e == hipSuccess) e = hipMalloc((void **)&n->v, pb);
+    if (e == hipSuccess) e = hipMalloc((void **)&n->grad, pb);
+    if (e == hipSuccess) e = hipMalloc((void **)&n->afwd, sizeof(mlp::bf16) * mlp::kFwdElems);
+    if (e == hipSuccess) e = hipMalloc((void **)&n->abwd, sizeof(mlp::bf16) * mlp::kBwdElems);
+    if (e == hipSuccess) e = hipMalloc((void **)&n->calib, sizeof(mlp::Calibration));
+    if (e == hipSuccess) e = hipMalloc((void **)&n->loss, 64);
+    if (e == hipSuccess) e = hipMalloc((void **)&n->n_fallback, 64);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)mlp::infer_kernel,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)(mlp::kFwdElems * sizeof(mlp::bf16) +
+                                                       (4 * mlp::kWidth + mlp::kClasses) * sizeof(float)));
+    if (e != hipSuccess) {
+        fprintf(stderr, "gpis_net_create: %s\n", hipGetErrorString(e));
+        gpis_net_destroy(n);
+        return GPIS_DEVICE_ERROR;
+    }
+    // He-uniform weights, zero biases
+    std::vector<float> theta(mlp::kParams, 0.0f);
+    std::mt19937_64 rng(seed);
+    for (int l = 0; l < mlp::kLayers; ++l) {
+        const float lim = sqrtf(6.0f / (float)mlp::layer_in(l));
+        std::uniform_real_distribution<float> U(-lim, lim);
+        for (int i = 0; i < mlp::layer_in(l) * mlp::layer_out(l); ++i) theta[mlp::w_off(l) + i] = U(rng);
+    }
+    gpis_status s = gpis_net_set_params(n, theta.data(), mlp::kParams);
+    if (s != GPIS_OK) {
+        gpis_net_destroy(n);
+        return s;
+    }
+    *out = n;
+    return GPIS_OK;
+}
+
+gpis_status gpis_net_destroy(gpis_net *n) {
+    if (!n) return GPIS_INVALID_ARGUMENT;
+    (void)hipDeviceSynchronize();
+    void *ptrs[] = {n->theta, n->m, n->v, n->grad, n->part, n->afwd, n->abwd, n->calib,
+                    n->xt, n->feats, n->fallback, n->n_fallback, n->loss};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete n;
+    return GPIS_OK;
+}
+
+const char *gpis_net_last_error(const gpis_net *n) { return n ? n->err.c_str() : "null network"; }
+
+gpis_status gpis_net_set_params(gpis_net *n, const float *params, int64_t count) {
+    if (!n || !params) return nfail(n, GPIS_INVALID_ARGUMENT, "null argument");
+    if (count != mlp::kParams) return nfail(n, GPIS_INVALID_ARGUMENT, "parameter count must be 54816");
+    const size_t pb = sizeof(float) * mlp::kParams;
+    NET_TRY(n, hipMemcpy(n->theta, params, pb, hipMemcpyHostToDevice));
+    NET_TRY(n, hipMemset(n->m, 0, pb));
+    NET_TRY(n, hipMemset(n->v, 0, pb));
+    n->step = 0;
+    gpis_status s = net_pack(n, 0);
+    if (s != GPIS_OK) return s;
+    NET_TRY(n, hipDeviceSynchronize());
+    return GPIS_OK;
+}
+
+gpis_status gpis_net_get_params(gpis_net *n, float *params, int64_t count) {
+    if (!n || !params) return nfail(n, GPIS_INVALID_ARGUMENT, "null argument");
+    if (count != mlp::kParams) return nfail(n, GPIS_INVALID_ARGUMENT, "parameter count must be 54816");
+    NET_TRY(n, hipDeviceSynchronize());
+    NET_TRY(n, hipMemcpy(params, n->theta, sizeof(float) * mlp::kParams, hipMemcpyDeviceToHost));
+    return GPIS_OK;
+}
+
+gpis_status gpis_net_set_calibration(gpis_net *n, const gpis_net_calibration *cal) {
+    if (!n || !cal) return nfail(n, GPIS_INVALID_ARGUMENT, "null argument");
+    mlp::Calibration C;
+    for (int i = 0; i < mlp::kIn; ++i) {
+        C.mean[i] = cal->feat_mean[i];
+        C.inv_std[i] = cal->feat_inv_std[i];
+        C.lo[i] = cal->hull_lo[i];
+        C.hi[i] = cal->hull_hi[i];
+        if (!std::isfinite(C.mean[i]) || !std::isfinite(C.inv_std[i]) || !(C.lo[i] <= C.hi[i]))
+            return nfail(n, GPIS_INVALID_ARGUMENT, "non-finite standardisation or empty hull");
+    }
+    if (cal->edges[0] != 0.0f || cal->edges[mlp::kBins] != 1.0f)
+        return nfail(n, GPIS_INVALID_ARGUMENT, "bin edges must run from 0 to 1");
+    for (int b = 0; b <= mlp::kBins; ++b) {
+        C.edges[b] = cal->edges[b];
+        // strictly increasing, except that edges[1] == 0 makes class 0 an atom at s = 0
+        if (b > 0 && !(C.edges[b] > C.edges[b - 1]) && !(b == 1 && C.edges[1] == 0.0f))
+            return nfail(n, GPIS_INVALID_ARGUMENT, "bin edges must increase");
+    }
+    C.entropy_max = cal->entropy_max;
+    NET_TRY(n, hipMemcpy(n->calib, &C, sizeof C, hipMemcpyHostToDevice));
+    n->calibrated = true;
+    return GPIS_OK;
+}
+
+gpis_status gpis_net_features_1d(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, float *feats, void *stream) {
+    if (!c || !rays || !feats) return fail(c, GPIS_INVALID_ARGUMENT, "null argument");
+    if (!c->P.one_d) return fail(c, GPIS_INVALID_ARGUMENT, "the medium is not 1D_sampling");
+    if (n < 0 || n >= (1LL << 31)) return fail(c, GPIS_INVALID_ARGUMENT, "ray count out of range");
+    if (n == 0) return GPIS_OK;
+    hipLaunchKernelGGL(features1d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       c->P, make_in(rays), rays->length_scale, (int)n, feats);
+    HIP_TRY(c, hipGetLastError());
+    return GPIS_OK;
+}
+
+gpis_status gpis_net_classes_1d(gpis_net *net, const gpis_ray_soa *rays, const gpis_reply_soa *reply, int64_t n,
+                                int32_t *classes, void *stream) {
+    if (!net || !rays || !reply || !reply->t || !reply->outcome || !classes)
+        return nfail(net, GPIS_INVALID_ARGUMENT, "null argument");
+    if (!net->calibrated) return nfail(net, GPIS_INVALID_ARGUMENT, "calibration not set");
+    if (n < 0 || n >= (1LL << 31)) return nfail(net, GPIS_INVALID_ARGUMENT, "ray count out of range");
+    if (n == 0) return GPIS_OK;
+    hipLaunchKernelGGL(classes1d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       make_in(rays), reply->t, reply->outcome, (int)n, net->calib, classes);
+    NET_TRY(net, hipGetLastError());
+    return GPIS_OK;
+}
+
+gpis_status gpis_net_gradient(gpis_net *n, const float *feats, const int32_t *classes, int64_t batch, float *grad,
+                              float *loss_sum, void *stream) {
+    if (!n || !grad) return nfail(n, GPIS_INVALID_ARGUMENT, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    int chunks = 0;
+    gpis_status s = net_backprop(n, feats, classes, batch, loss_sum, st, &chunks);
+    if (s != GPIS_OK) return s;
+    hipLaunchKernelGGL(mlp::adam_kernel, dim3((mlp::kParams + 255) / 256), dim3(256), 0, st, n->part, chunks,
+                       1.0f / (float)batch, n->theta, n->m, n->v, grad, 0.0f, 1.0f, 1.0f, 0);
+    NET_TRY(n, hipGetLastError());
+    return GPIS_OK;
+}
+
+gpis_status gpis_net_train_step(gpis_net *n, const float *feats, const int32_t *classes, int64_t batch, float lr,
+                                float *loss_sum, void *stream) {
+    if (!n) return GPIS_INVALID_ARGUMENT;
+    if (!(lr > 0.0f) || !std::isfinite(lr)) return nfail(n, GPIS_INVALID_ARGUMENT, "learning rate must be positive");
+    hipStream_t st = (hipStream_t)stream;
+    int chunks = 0;
+    gpis_status s = net_backprop(n, feats, classes, batch, loss_sum, st, &chunks);
+    if (s != GPIS_OK) return s;
+    n->step += 1;
+    const float bc1 = (float)(1.0 / (1.0 - pow(0.9, (double)n->step)));
+    const float bc2 = (float)(1.0 / (1.0 - pow(0.999, (double)n->step)));
+    hipLaunchKernelGGL(mlp::adam_kernel, dim3((mlp::kParams + 255) / 256), dim3(256), 0, st, n->part, chunks,
+                       1.0f / (float)batch, n->theta, n->m, n->v, n->grad, lr, bc1, bc2, 1);
+    NET_TRY(n, hipGetLastError());
+    return net_pack(n, st);
+}
+
+static gpis_status net_infer(gpis_net *n, const float *feats, int64_t rows, const float *near_t, const float *far_t,
+                             const float *jitter, const uint32_t *flags, const mlp::InferOut &o, hipStream_t st) {
+    const size_t smem = mlp::kFwdElems * sizeof(mlp::bf16) + (4 * mlp::kWidth + mlp::kClasses) * sizeof(float);
+    const int64_t groups = (rows + 16 * mlp::kNT - 1) / (16 * mlp::kNT);
+    int64_t blocks = (groups + mlp::kInferWaves - 1) / mlp::kInferWaves;
+    if (blocks > n->num_cus) blocks = n->num_cus;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(mlp::infer_kernel, dim3((unsigned)blocks), dim3(mlp::kInferWaves * 64), smem, st, feats,
+                       near_t, far_t, jitter, flags, (int)rows, n->calib, n->afwd, n->theta, o);
+    NET_TRY(n, hipGetLastError());
+    return GPIS_OK;
+}
+
+gpis_status gpis_net_forward(gpis_net *n, const float *feats, int64_t rows, float *probs, void *stream) {
+    if (!n || !feats || !probs) return nfail(n, GPIS_INVALID_ARGUMENT, "null argument");
+    if (!n->calibrated) return nfail(n, GPIS_INVALID_ARGUMENT, "calibration not set");
+    if (rows < 0 || rows >= (1LL << 31)) return nfail(n, GPIS_INVALID_ARGUMENT, "row count out of range");
+    if (rows == 0) return GPIS_OK;
+    mlp::InferOut o = {};
+    o.probs = probs;
+    return net_infer(n, feats, rows, nullptr, nullptr, nullptr, nullptr, o, (hipStream_t)stream);
+}
+
+gpis_status gpis_surrogate_flights_1d(gpis_ctx *c, gpis_net *net, const gpis_ray_soa *rays, int64_t n,
+                                      const gpis_reply_soa *reply, uint32_t *n_fallback, void *stream) {
+    if (!c || !net) return GPIS_INVALID_ARGUMENT;
+    if (!net->calibrated) return nfail(net, GPIS_INVALID_ARGUMENT, "calibration not set");
+    if (!c->P.one_d) return fail(c, GPIS_INVALID_ARGUMENT, "the medium is not 1D_sampling");
+    if (!rays || !reply || !reply->t || !reply->value || !reply->gx || !reply->gy || !reply->gz || !reply->outcome)
+        return fail(c, GPIS_INVALID_ARGUMENT, "null ray or reply field");
+    if (n < 0 || n >= (1LL << 31)) return fail(c, GPIS_INVALID_ARGUMENT, "ray count out of range");
+    if (n_fallback) *n_fallback = 0;
+    if (n == 0) return GPIS_OK;
+    if (c->P.N != 8) return fail(c, GPIS_UNSUPPORTED, "1D flights are compiled for 8 impulses per cell");
+    hipStream_t st = (hipStream_t)stream;
+    NET_TRY(net, grow(&net->feats, &net->feats_rows, n, (int64_t)mlp::kIn));
+    NET_TRY(net, grow(&net->fallback, &net->fb_rows, n, (int64_t)1));
+    gpis_status s = gpis_net_features_1d(c, rays, n, net->feats, stream);
+    if (s != GPIS_OK) return s;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(32), 0, st, net->n_fallback, 1, 0u);
+    mlp::InferOut o = {reply->t, reply->value, reply->gx, reply->gy, reply->gz, reply->outcome, reply->evals,
+                       nullptr, net->fallback, net->n_fallback};
+    s = net_infer(net, net->feats, n, (const float *)rays->near_t, (const float *)rays->far_t,
+                  (const float *)rays->jitter, (const uint32_t *)rays->flags, o, st);
+    if (s != GPIS_OK) return s;
+    ReplyOut out = {reply->t, reply->value, reply->gx, reply->gy, reply->gz, reply->outcome, reply->evals};
+    s = launch_flight1d(c, rays, n, out, nullptr, Nee1DOut{}, net->fallback, net->n_fallback, st);
+    if (s != GPIS_OK) return s;
+    if (n_fallback) {
+        HIP_TRY(c, hipMemcpyAsync(n_fallback, net->n_fallback, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIP_TRY(c, hipStreamSynchronize(st));
+    }
     return GPIS_OK;
 }
 

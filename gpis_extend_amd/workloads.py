@@ -227,12 +227,14 @@ def box_surface_rays(medium, n=1 << 20, half=0.6, seed=SEED_BOX, scene_seed=2, b
     return rays
 
 
-def renewal1d_rays(medium, n=1 << 21, seed=SEED_RENEWAL1D, scene_seed=3, box=1.0, ls_range=(0.02, 0.2)):
+def renewal1d_rays(medium, n=1 << 21, seed=SEED_RENEWAL1D, scene_seed=3, box=1.0, ls_range=(0.02, 0.2),
+                   first_id=0):
     """Secondary 1D flights: origins uniform in the shell 0.55 <= |x| <= 0.95 around the
     sphere mean, directions toward uniform points of the ball of radius 0.5 (the
     flight runs to the exit of [-box, box]^3).  Each ray is conditioned on a value
     v0 ~ N(0.3, 0.2^2) and a gradient whose derivative along the ray is ~ N(0, 1)
-    (prev_grad = g_d * dir), with its own SE length scale l ~ U(ls_range)."""
+    (prev_grad = g_d * dir), with its own SE length scale l ~ U(ls_range).  `seed` is an
+    integer or a numpy SeedSequence; ray ids (the realization seeds) start at first_id."""
     rng = np.random.default_rng(seed)
     o = rng.normal(size=(n, 3))
     o /= np.linalg.norm(o, axis=1, keepdims=True)
@@ -259,7 +261,26 @@ def renewal1d_rays(medium, n=1 << 21, seed=SEED_RENEWAL1D, scene_seed=3, box=1.0
     rays["prev_gz"] = (gd * d[:, 2]).astype(np.float32)
     rays["length_scale"] = rng.uniform(ls_range[0], ls_range[1], size=n).astype(np.float32)
     pss = np.zeros((n, 4), np.uint32)
-    pss[:, 0] = np.arange(n, dtype=np.uint32)
+    pss[:, 0] = (first_id + np.arange(n, dtype=np.int64)).astype(np.uint32)
+    _fill_seeds(rays, medium, pss, scene_seed)
+    return rays
+
+
+def renewal1d_image_rays(medium, width=512, spp=16, seed=12, scene_seed=3):
+    """The renewal workload arranged as an image: width^2 pixels, each one conditioned
+    flight configuration of the renewal workload (origin, direction, bound, value,
+    derivative, length scale), traced spp times with independent realizations
+    (pss = (pixel, sample, 0, 0)) and jitters.  Ray index = pixel * spp + sample."""
+    ss = np.random.SeedSequence(seed)
+    s_cfg, s_jit = ss.spawn(2)
+    npx = width * width
+    base = renewal1d_rays(medium, n=npx, seed=s_cfg, scene_seed=scene_seed)
+    rays = {k: np.repeat(v, spp, axis=0) for k, v in base.items() if isinstance(v, np.ndarray) and k != "pss"}
+    n = npx * spp
+    rays["jitter"] = _jitter(np.random.default_rng(s_jit), n)
+    pss = np.zeros((n, 4), np.uint32)
+    pss[:, 0] = np.repeat(np.arange(npx, dtype=np.uint32), spp)
+    pss[:, 1] = np.tile(np.arange(spp, dtype=np.uint32), npx)
     _fill_seeds(rays, medium, pss, scene_seed)
     return rays
 

@@ -1,4 +1,4 @@
-"""The C-ABI library loads and exports every function include/gpis_hip.h declares
+"""The C-ABI library loads and exports every function include/*.h declares
 (no device calls)."""
 import ctypes
 import os
@@ -10,7 +10,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_functions():
-    txt = open(os.path.join(ROOT, "include", "gpis_hip.h")).read()
+    inc = os.path.join(ROOT, "include")
+    txt = "".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc)) if f.endswith(".h"))
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(gpis_[a-z0-9_]+)\s*\(", txt)))
 
