@@ -411,7 +411,7 @@ def main():
                                f"{args.width}x{args.width} pinhole rays + 2 diffuse bounces",
                    "rays": n, "flights_per_step": flights_per_step, "evals_per_step": evals_per_step,
                    "hits_per_step": c["hits"] / args.steps, "fails_per_step": c["fails"] / args.steps,
-                   "launch": {"refill": tuned[0], "waves_per_cu": tuned[1], "block": tuned[2]} if tuned else None},
+                   "launch": {"refill_batch": tuned[0], "waves_per_cu": tuned[1], "block": tuned[2]} if tuned else None},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "frac_scalar_issue_peak": achieved / PEAK_FP32_SCALAR_TFLOPS,

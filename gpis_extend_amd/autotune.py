@@ -2,9 +2,9 @@
 
 Knobs (all exposed by the C ABI): waves per CU of the persistent grid
 (occupancy of the launch), threads per block (waves sharing one block's LDS
-counters), and the refill policy (a lane takes a new ray as soon as it is done,
-or a wave takes 64 new rays only when all its lanes are done, which keeps
-spatially coherent ray groups together).  The tuner times every combination on
+counters), and the finish/refill batch (how many lanes of a wave wait for each
+other before the divergent finish, bounce and refill code runs once for all of
+them: 1 serves each lane at once, 64 keeps 64-ray tiles together).  The tuner times every combination on
 the caller's own batch with HIP events on the current stream and keeps the
 fastest; results are cached per (entry point, medium signature, batch-size
 bucket) for the life of the process.
@@ -19,7 +19,7 @@ _CACHE: Dict[Tuple, Tuple[int, int, int]] = {}
 
 WAVES_PER_CU = (8, 12, 16, 20, 24)
 BLOCK_SIZES = (64, 128, 256)
-REFILL = (0, 1)
+REFILL = (1, 8, 16, 32, 64)
 
 
 def _key(dev, mode: str, n: int):

@@ -35,7 +35,7 @@ struct gpis_ctx {
     int num_cus = 0;
     int waves_per_cu = 0;
     int block_size = 256;
-    int wave_refill = 0;
+    int batch_k = 1;  // finish/bounce/refill batch of the 3D kernel (lanes per wave)
     float nf1d = 1.0f;  // sqrt(sparseConvNoiseVariance1D) of the 1D grid
     std::string err;
 };
@@ -212,7 +212,8 @@ enum Phase : int {
     PH_FAR = 5,    // f(farT) on exit
     PH_GRAD = 6,   // gradient at the endpoint
     PH_DEAD = 7,   // no more work
-    PH_NEE = 8     // next-event estimation at the scattering point (1D flights)
+    PH_NEE = 8,    // next-event estimation at the scattering point (1D flights)
+    PH_DONE = 9    // flight finished, waiting for the batched finish/bounce/refill step
 };
 
 #ifndef GPIS_FLIGHT_MIN_WAVES
@@ -281,6 +282,23 @@ __device__ __forceinline__ double lerp_d(double a, double b, double r) {
     return u + v;
 }
 // Lattice point t0 + k*step in double (exact: both terms fit in 53 bits).
+// ray_point, or grad_point when `unit` (the one position computation of a loop
+// iteration; the normalisation runs only for the lanes that need it).
+__device__ __forceinline__ void point_at(float ox, float oy, float oz, float dx, float dy, float dz, double t, bool unit,
+                                         float &qx, float &qy, float &qz) {
+#pragma clang fp contract(off)
+    double ex = (double)dx, ey = (double)dy, ez = (double)dz;
+    if (unit) {
+        double xx = ex * ex, yy = ey * ey, zz = ez * ez;
+        double l2 = (xx + yy) + zz;
+        double inv = 1.0 / sqrt(l2);
+        ex = ex * inv; ey = ey * inv; ez = ez * inv;
+    }
+    double mx = ex * t, my = ey * t, mz = ez * t;
+    qx = (float)((double)ox + mx);
+    qy = (float)((double)oy + my);
+    qz = (float)((double)oz + mz);
+}
 __device__ __forceinline__ double lattice_t(float t0, int k, float step) {
 #pragma clang fp contract(off)
     double m = (double)k * (double)step;
@@ -368,7 +386,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
                                                                             unsigned int *work,
                                                                             unsigned long long *stats,
                                                                             int max_bounces, uint32_t bounce_seed,
-                                                                            float box, int wave_refill) {
+                                                                            float box, int batch_k) {
     __shared__ unsigned int bstat[ST_N];
     if (threadIdx.x < ST_N) bstat[threadIdx.x] = 0;
     __syncthreads();
@@ -386,104 +404,171 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
     int bounce = 0;
     unsigned path_fh = 0;  // flights (low 16 bits) and hits (high 16 bits) of the current path
     unsigned path_evals = 0;
+    uint32_t outcome = 0;     // of a finished flight (PH_DONE)
+    float gxv = 0.f, gyv = 0.f, gzv = 0.f;
 
     for (;;) {
-        // ---- refill: lanes that finished take the next rays (ballot + prefix sum).
-        // With wave_refill the wave refills only when every lane is done, so the 64
-        // consecutive rays of a tile march together (coherent cell accesses).
-        unsigned long long need = __ballot(phase == PH_NEED);
-        if (wave_refill && __ballot(phase != PH_NEED && phase != PH_DEAD)) need = 0;
-        if (need) {
-            unsigned cnt = (unsigned)__popcll(need);
-            unsigned leader = (unsigned)(__ffsll((long long)need) - 1);
-            unsigned base = 0;
-            if (lane == leader) base = atomicAdd(work, cnt);
-            base = __shfl(base, leader);
-            if (phase == PH_NEED) {
-                unsigned r = base + mask_rank(need);
-                if (r >= n) {
-                    phase = PH_DEAD;
+        // ---- finish, bounce and refill, batched.  A lane whose flight is done (PH_DONE)
+        // or that wants a ray (PH_NEED) waits until at least batch_k lanes of its wave
+        // do, or until no lane is marching; the wave then runs this divergent code once
+        // for all of them.  batch_k = 1 serves lanes at once, 64 keeps 64-ray tiles together.
+        const unsigned long long waiting = __ballot(phase == PH_DONE || phase == PH_NEED);
+        const unsigned long long running = __ballot(phase != PH_DONE && phase != PH_NEED && phase != PH_DEAD);
+        if (waiting && ((unsigned)__popcll(waiting) >= (unsigned)batch_k || !running)) {
+            if (phase == PH_DONE) {
+                const unsigned oc = outcome & 0xFFu;
+                atomicAdd(&bstat[ST_FLIGHTS], 1u);
+                atomicAdd(&bstat[ST_EVALS], evals_flight);
+                atomicAdd(&bstat[ST_STEPS], (unsigned)k);
+                atomicAdd(&bstat[oc == GPIS_HIT ? ST_HITS : (oc == GPIS_EXIT ? ST_EXITS : ST_FAILS)], 1u);
+                if (!PATHS) {
+                    out.t[ray] = t_end; out.value[ray] = val_end;
+                    out.gx[ray] = gxv; out.gy[ray] = gyv; out.gz[ray] = gzv;
+                    out.outcome[ray] = outcome;
+                    if (out.evals) out.evals[ray] = evals_flight;
+                    evals_flight = 0;
+                    phase = PH_NEED;
                 } else {
-                    ray = r;
-                    ox = in.ox[r]; oy = in.oy[r]; oz = in.oz[r];
-                    dx = in.dx[r]; dy = in.dy[r]; dz = in.dz[r];
-                    near_t = in.near_t[r]; far_t = in.far_t[r];
-                    float jit = in.jitter[r];
-                    rs.seed = in.noise_seed[r];
-                    rs.cond = 0;
-                    first = (in.flags[r] & 1u) ? 1 : 0;
-                    if (COND && !first) { prev_v = in.pv[r]; prev_gx = in.pgx[r]; prev_gy = in.pgy[r]; prev_gz = in.pgz[r]; }
-                    bounce = 0; path_fh = 0; path_evals = 0; evals_flight = 0;
-                    // beginDistance (GaussianProcessMedium.cpp:244-265)
-                    if (!isfinite(far_t)) far_t = near_t + 2000.f;
-                    bool bad = !isfinite(ox) || !isfinite(oy) || !isfinite(oz) || !isfinite(dx) || !isfinite(dy) ||
-                               !isfinite(dz) || !isfinite(near_t) || !(jit >= 0.f && jit < 1.f);
-                    if (bad) {
-                        if (!PATHS) {
-                            out.t[r] = far_t; out.value[r] = 0.f; out.gx[r] = out.gy[r] = out.gz[r] = 0.f;
-                            out.outcome[r] = GPIS_INVALID;
-                            if (out.evals) out.evals[r] = 0;
+                    if (bounce == 0) {
+                        pout.t0[ray] = t_end; pout.outcome0[ray] = outcome;
+                        pout.gx0[ray] = gxv; pout.gy0[ray] = gyv; pout.gz0[ray] = gzv;
+                    }
+                    path_fh += 1u + (oc == GPIS_HIT ? 0x10000u : 0u);
+                    path_evals += evals_flight;
+                    evals_flight = 0;
+                    phase = PH_NEED;
+                    if (oc == GPIS_HIT && bounce < max_bounces) {
+                        ++bounce;
+                        // sample.p = ray.pos() + t * ray.dir() in float
+                        float hx = mul_add_exact(t_end, dx, ox);
+                        float hy = mul_add_exact(t_end, dy, oy);
+                        float hz = mul_add_exact(t_end, dz, oz);
+                        float il = rsqrtf(gxv * gxv + gyv * gyv + gzv * gzv);
+                        float jit;
+                        diffuse_bounce(ray, bounce, bounce_seed, gxv * il, gyv * il, gzv * il, dx, dy, dz, jit);
+                        ox = hx; oy = hy; oz = hz;
+                        near_t = 0.f;
+                        far_t = box_exit(ox, oy, oz, dx, dy, dz, box);
+                        prev_v = 0.f; prev_gx = gxv; prev_gy = gyv; prev_gz = gzv;
+                        first = 0;
+                        rs.cond = 0;
+                        if (far_t <= near_t) {
+                            atomicAdd(&bstat[ST_FLIGHTS], 1u);
+                            atomicAdd(&bstat[ST_EXITS], 1u);
+                            path_fh += 1u;
                         } else {
-                            pout.t0[r] = far_t; pout.outcome0[r] = GPIS_INVALID;
-                            pout.gx0[r] = pout.gy0[r] = pout.gz0[r] = 0.f;
-                            pout.flights[r] = 0; pout.hits[r] = 0;
-                            if (pout.evals) pout.evals[r] = 0;
+                            step = (far_t - near_t) / P.min_step_f;
+                            if (P.step_size < step) step = P.step_size;
+                            t0 = mul_add_exact(step, jit, near_t);
+                            if (COND && P.activate) { phase = PH_COND; qx = ox; qy = oy; qz = oz; }
+                            else { phase = PH_F0; ray_point(ox, oy, oz, dx, dy, dz, 0.0, qx, qy, qz); }
                         }
-                        atomicAdd(&bstat[ST_INVALID], 1u);
-                        phase = PH_NEED;
-                    } else if (far_t <= near_t) {
-                        // exited without a flight: no RNG draw, no evaluation
-                        if (!PATHS) {
-                            out.t[r] = far_t; out.value[r] = first ? 0.f : in.pv[r];
-                            out.gx[r] = out.gy[r] = out.gz[r] = 0.f;
-                            out.outcome[r] = GPIS_EXIT;
-                            if (out.evals) out.evals[r] = 0;
-                        } else {
-                            pout.t0[r] = far_t; pout.outcome0[r] = GPIS_EXIT;
-                            pout.gx0[r] = pout.gy0[r] = pout.gz0[r] = 0.f;
-                            pout.flights[r] = 0; pout.hits[r] = 0;
-                            if (pout.evals) pout.evals[r] = 0;
-                        }
-                        atomicAdd(&bstat[ST_EXITS], 1u);
-                        phase = PH_NEED;
+                    }
+                    if (phase == PH_NEED) {
+                        pout.flights[ray] = path_fh & 0xFFFFu;
+                        pout.hits[ray] = path_fh >> 16;
+                        if (pout.evals) pout.evals[ray] = path_evals;
+                    }
+                }
+            }
+            // ---- refill: lanes that want a ray take the next indices (one atomic per
+            // wave, ranks from the ballot mask)
+            const unsigned long long need = __ballot(phase == PH_NEED);
+            if (need) {
+                unsigned cnt = (unsigned)__popcll(need);
+                unsigned leader = (unsigned)(__ffsll((long long)need) - 1);
+                unsigned base = 0;
+                if (lane == leader) base = atomicAdd(work, cnt);
+                base = __shfl(base, leader);
+                if (phase == PH_NEED) {
+                    unsigned r = base + mask_rank(need);
+                    if (r >= n) {
+                        phase = PH_DEAD;
                     } else {
-                        // finishGpisSegment step rule (SparseConvolutionNoiseMedium.cpp:289-291)
-                        step = (far_t - near_t) / P.min_step_f;
-                        if (P.step_size < step) step = P.step_size;
-                        t0 = mul_add_exact(step, jit, near_t);
-                        if (COND && P.activate && !first) {
-                            phase = PH_COND;
-                            qx = ox; qy = oy; qz = oz;
+                        ray = r;
+                        ox = in.ox[r]; oy = in.oy[r]; oz = in.oz[r];
+                        dx = in.dx[r]; dy = in.dy[r]; dz = in.dz[r];
+                        near_t = in.near_t[r]; far_t = in.far_t[r];
+                        float jit = in.jitter[r];
+                        rs.seed = in.noise_seed[r];
+                        rs.cond = 0;
+                        first = (in.flags[r] & 1u) ? 1 : 0;
+                        if (COND && !first) { prev_v = in.pv[r]; prev_gx = in.pgx[r]; prev_gy = in.pgy[r]; prev_gz = in.pgz[r]; }
+                        bounce = 0; path_fh = 0; path_evals = 0; evals_flight = 0;
+                        // beginDistance (GaussianProcessMedium.cpp:244-265)
+                        if (!isfinite(far_t)) far_t = near_t + 2000.f;
+                        bool bad = !isfinite(ox) || !isfinite(oy) || !isfinite(oz) || !isfinite(dx) || !isfinite(dy) ||
+                                   !isfinite(dz) || !isfinite(near_t) || !(jit >= 0.f && jit < 1.f);
+                        if (bad) {
+                            if (!PATHS) {
+                                out.t[r] = far_t; out.value[r] = 0.f; out.gx[r] = out.gy[r] = out.gz[r] = 0.f;
+                                out.outcome[r] = GPIS_INVALID;
+                                if (out.evals) out.evals[r] = 0;
+                            } else {
+                                pout.t0[r] = far_t; pout.outcome0[r] = GPIS_INVALID;
+                                pout.gx0[r] = pout.gy0[r] = pout.gz0[r] = 0.f;
+                                pout.flights[r] = 0; pout.hits[r] = 0;
+                                if (pout.evals) pout.evals[r] = 0;
+                            }
+                            atomicAdd(&bstat[ST_INVALID], 1u);
+                            phase = PH_NEED;
+                        } else if (far_t <= near_t) {
+                            // exited without a flight: no RNG draw, no evaluation
+                            if (!PATHS) {
+                                out.t[r] = far_t; out.value[r] = first ? 0.f : in.pv[r];
+                                out.gx[r] = out.gy[r] = out.gz[r] = 0.f;
+                                out.outcome[r] = GPIS_EXIT;
+                                if (out.evals) out.evals[r] = 0;
+                            } else {
+                                pout.t0[r] = far_t; pout.outcome0[r] = GPIS_EXIT;
+                                pout.gx0[r] = pout.gy0[r] = pout.gz0[r] = 0.f;
+                                pout.flights[r] = 0; pout.hits[r] = 0;
+                                if (pout.evals) pout.evals[r] = 0;
+                            }
+                            atomicAdd(&bstat[ST_EXITS], 1u);
+                            phase = PH_NEED;
                         } else {
-                            phase = PH_F0;
-                            ray_point(ox, oy, oz, dx, dy, dz, (double)near_t, qx, qy, qz);
+                            // finishGpisSegment step rule (SparseConvolutionNoiseMedium.cpp:289-291)
+                            step = (far_t - near_t) / P.min_step_f;
+                            if (P.step_size < step) step = P.step_size;
+                            t0 = mul_add_exact(step, jit, near_t);
+                            if (COND && P.activate && !first) {
+                                phase = PH_COND;
+                                qx = ox; qy = oy; qz = oz;
+                            } else {
+                                phase = PH_F0;
+                                ray_point(ox, oy, oz, dx, dy, dz, (double)near_t, qx, qy, qz);
+                            }
                         }
                     }
                 }
             }
         }
         if (!__ballot(phase != PH_DEAD)) break;
-        if (phase == PH_NEED || phase == PH_DEAD) continue;
+        if (phase == PH_NEED || phase == PH_DEAD || phase == PH_DONE) continue;
 
         // ---- one evaluation per live lane
         Eval4 e = eval_field<NI, ISO>(P, rs, qx, qy, qz, !COND || phase != PH_COND);
         ++evals_flight;
         const float fv = e.v;
+        // the next query point: t along the ray (unit: along the normalised direction)
+        double tq = 0.0;
+        bool unit = false;
 
         if (COND && phase == PH_COND) {
             condition_coefficients(P, e, ox, oy, oz, prev_v, prev_gx, prev_gy, prev_gz, rs);
             phase = PH_F0;
-            ray_point(ox, oy, oz, dx, dy, dz, (double)near_t, qx, qy, qz);
+            tq = (double)near_t;
         } else if (phase == PH_F0) {
             sign0 = fv < 0.f ? -1 : 1;
             pf = fv;
             k = 0;
             if (t0 < far_t) {
                 phase = PH_MARCH;
-                ray_point(ox, oy, oz, dx, dy, dz, (double)t0, qx, qy, qz);
+                tq = (double)t0;
             } else {
                 phase = PH_FAR;
-                ray_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
+                tq = (double)far_t;
             }
         } else if (phase == PH_MARCH) {
             ++k;
@@ -498,7 +583,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
                 tp = lerp_d(tk - (double)step, tk, fac);
                 tt = tp;
                 phase = PH_SECANT;
-                ray_point(ox, oy, oz, dx, dy, dz, tt, qx, qy, qz);
+                tq = tt;
             }
             if (phase == PH_MARCH) {
                 pf = fv;
@@ -520,10 +605,10 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
                     evals_flight = 0;
                     phase = PH_NEED;
                 } else if (tn < (double)far_t) {
-                    ray_point(ox, oy, oz, dx, dy, dz, tn, qx, qy, qz);
+                    tq = tn;
                 } else {
                     phase = PH_FAR;
-                    ray_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
+                    tq = (double)far_t;
                 }
             }
         } else if (phase == PH_SECANT) {
@@ -538,7 +623,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
                 } else {
                     tp = tt;
                     tt = lerp_d(tk - (double)step, tk, fac);
-                    ray_point(ox, oy, oz, dx, dy, dz, tt, qx, qy, qz);
+                    tq = tt;
                 }
             }
             if (done) {
@@ -546,18 +631,20 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
                 t_end = (float)tp;
                 val_end = 0.f;
                 phase = PH_GRAD;
-                grad_point(ox, oy, oz, dx, dy, dz, tp, qx, qy, qz);
+                tq = tp;
+                unit = true;
             }
         } else if (phase == PH_FAR) {
             hit = 0;
             t_end = far_t;
             val_end = fv;
             phase = PH_GRAD;
-            grad_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
+            tq = (double)far_t;
+            unit = true;
         } else if (phase == PH_GRAD) {
-            // resumeDistance outcome rules (GaussianProcessMedium.cpp:305-348)
-            uint32_t outcome;
-            const float gxv = e.gx, gyv = e.gy, gzv = e.gz;
+            // resumeDistance outcome rules (GaussianProcessMedium.cpp:305-348); the rest of
+            // the finish waits for the batched step at the top of the loop
+            gxv = e.gx; gyv = e.gy; gzv = e.gz;
             if (hit) {
                 if (!isfinite(gxv + gyv + gzv)) outcome = GPIS_FAIL | (GPIS_FAIL_NONFINITE << 8);
                 else if (gxv * dx + gyv * dy + gzv * dz > 0.f) outcome = GPIS_FAIL | (GPIS_FAIL_OUTWARD << 8);
@@ -566,61 +653,10 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
             } else {
                 outcome = GPIS_EXIT;
             }
-            const unsigned oc = outcome & 0xFFu;
-            atomicAdd(&bstat[ST_FLIGHTS], 1u);
-            atomicAdd(&bstat[ST_EVALS], evals_flight);
-            atomicAdd(&bstat[ST_STEPS], (unsigned)k);
-            atomicAdd(&bstat[oc == GPIS_HIT ? ST_HITS : (oc == GPIS_EXIT ? ST_EXITS : ST_FAILS)], 1u);
-            if (!PATHS) {
-                out.t[ray] = t_end; out.value[ray] = val_end;
-                out.gx[ray] = gxv; out.gy[ray] = gyv; out.gz[ray] = gzv;
-                out.outcome[ray] = outcome;
-                if (out.evals) out.evals[ray] = evals_flight;
-                evals_flight = 0;
-                phase = PH_NEED;
-            } else {
-                if (bounce == 0) {
-                    pout.t0[ray] = t_end; pout.outcome0[ray] = outcome;
-                    pout.gx0[ray] = gxv; pout.gy0[ray] = gyv; pout.gz0[ray] = gzv;
-                }
-                path_fh += 1u + (oc == GPIS_HIT ? 0x10000u : 0u);
-                path_evals += evals_flight;
-                evals_flight = 0;
-                phase = PH_NEED;
-                if (oc == GPIS_HIT && bounce < max_bounces) {
-                    ++bounce;
-                    // sample.p = ray.pos() + t * ray.dir() in float
-                    float hx = mul_add_exact(t_end, dx, ox);
-                    float hy = mul_add_exact(t_end, dy, oy);
-                    float hz = mul_add_exact(t_end, dz, oz);
-                    float il = rsqrtf(gxv * gxv + gyv * gyv + gzv * gzv);
-                    float jit;
-                    diffuse_bounce(ray, bounce, bounce_seed, gxv * il, gyv * il, gzv * il, dx, dy, dz, jit);
-                    ox = hx; oy = hy; oz = hz;
-                    near_t = 0.f;
-                    far_t = box_exit(ox, oy, oz, dx, dy, dz, box);
-                    prev_v = 0.f; prev_gx = gxv; prev_gy = gyv; prev_gz = gzv;
-                    first = 0;
-                    rs.cond = 0;
-                    if (far_t <= near_t) {
-                        atomicAdd(&bstat[ST_FLIGHTS], 1u);
-                        atomicAdd(&bstat[ST_EXITS], 1u);
-                        path_fh += 1u;
-                    } else {
-                        step = (far_t - near_t) / P.min_step_f;
-                        if (P.step_size < step) step = P.step_size;
-                        t0 = mul_add_exact(step, jit, near_t);
-                        if (COND && P.activate) { phase = PH_COND; qx = ox; qy = oy; qz = oz; }
-                        else { phase = PH_F0; ray_point(ox, oy, oz, dx, dy, dz, 0.0, qx, qy, qz); }
-                    }
-                }
-                if (phase == PH_NEED) {
-                    pout.flights[ray] = path_fh & 0xFFFFu;
-                    pout.hits[ray] = path_fh >> 16;
-                    if (pout.evals) pout.evals[ray] = path_evals;
-                }
-            }
+            phase = PH_DONE;
         }
+        if (phase == PH_F0 || phase == PH_MARCH || phase == PH_SECANT || phase == PH_FAR || phase == PH_GRAD)
+            point_at(ox, oy, oz, dx, dy, dz, tq, unit, qx, qy, qz);
     }
     // ---- statistics: LDS block totals, one global atomic per counter per block
     __syncthreads();
@@ -978,7 +1014,7 @@ static void launch_flights(gpis_ctx *c, unsigned blocks, unsigned threads, hipSt
                            PathOut pout, unsigned n, int max_bounces, uint32_t bounce_seed, float box) {
 #define GPIS_FLIGHT_LAUNCH(NI, ISO, COND)                                                                         \
     hipLaunchKernelGGL((flight_kernel<NI, ISO, COND, PATHS>), dim3(blocks), dim3(threads), 0, st, c->P, in, out, \
-                       pout, n, c->work, c->counters, max_bounces, bounce_seed, box, c->wave_refill)
+                       pout, n, c->work, c->counters, max_bounces, bounce_seed, box, c->batch_k)
     const bool cond = c->P.activate != 0;
     if (c->P.N == 8 && c->P.iso && !cond) GPIS_FLIGHT_LAUNCH(8, true, false);
     else if (c->P.N == 8 && c->P.iso) GPIS_FLIGHT_LAUNCH(8, true, true);
@@ -1173,10 +1209,10 @@ gpis_status gpis_trace_paths(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, i
     return GPIS_OK;
 }
 
-gpis_status gpis_set_refill(gpis_ctx *c, int32_t policy) {
+gpis_status gpis_set_refill(gpis_ctx *c, int32_t batch) {
     if (!c) return GPIS_INVALID_ARGUMENT;
-    if (policy != 0 && policy != 1) return fail(c, GPIS_INVALID_ARGUMENT, "refill policy must be 0 (lane) or 1 (wave)");
-    c->wave_refill = policy;
+    if (batch < 0 || batch > 64) return fail(c, GPIS_INVALID_ARGUMENT, "refill batch must be in 0..64 lanes");
+    c->batch_k = batch < 1 ? 1 : batch;
     return GPIS_OK;
 }
 

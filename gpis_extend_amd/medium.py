@@ -268,8 +268,9 @@ class GpisDevice:
     def set_launch(self, waves_per_cu=0, block_size=0):
         A.check(self.L.gpis_set_launch(self.h, waves_per_cu, block_size), self.h, "gpis_set_launch")
 
-    def set_refill(self, policy: int):
-        A.check(self.L.gpis_set_refill(self.h, policy), self.h, "gpis_set_refill")
+    def set_refill(self, batch: int):
+        """Lanes per wave served together by the finish/bounce/refill step (1..64)."""
+        A.check(self.L.gpis_set_refill(self.h, batch), self.h, "gpis_set_refill")
 
     def impulse_table(self, lo, dims, seed):
         import torch

@@ -177,11 +177,13 @@ gpis_status gpis_reset_counters(gpis_ctx *ctx);
 /* Launch-shape control (the autotuner's knobs): waves per CU of the persistent
  * flight kernel, and rays per lane batch.  0 = built-in default. */
 gpis_status gpis_set_launch(gpis_ctx *ctx, int32_t waves_per_cu, int32_t block_size);
-/* Refill policy of the persistent flight kernels: 0 = a lane takes the next ray as
- * soon as its flight (or path) ends; 1 = a wave takes 64 new rays only when all of
- * its lanes are done, so rays submitted in spatially coherent groups of 64 (such as
- * 8x8 pixel tiles) march together.  No reference counterpart (a scheduling knob). */
-gpis_status gpis_set_refill(gpis_ctx *ctx, int32_t policy);
+/* Batching of the 3D persistent kernel's finish/bounce/refill step: a lane whose
+ * flight ended waits until `batch` lanes of its wave (1..64) can be finished, bounced
+ * and refilled together, or until no lane of the wave is still marching.  1 (or 0)
+ * serves each lane at once; 64 refills a wave only when all its lanes are done, so
+ * rays submitted in spatially coherent groups of 64 (8x8 pixel tiles) march together.
+ * No reference counterpart (a scheduling knob; results do not depend on it). */
+gpis_status gpis_set_refill(gpis_ctx *ctx, int32_t batch);
 
 #ifdef __cplusplus
 }
