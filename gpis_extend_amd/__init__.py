@@ -6,6 +6,6 @@ the HIP kernels live in ``csrc/`` and are reached through the C ABI declared in
 JSON parameters and sampleDistance/transmittance semantics on top of it.
 """
 from . import _capi  # noqa: F401
-from .medium import GpisDevice, RayBatch, SparseConvolutionNoiseMedium  # noqa: F401
+from .medium import DenseMeanGrid, GpisDevice, MediumConfigError, RayBatch, SparseConvolutionNoiseMedium  # noqa: F401
 
 __all__ = ["GpisDevice", "RayBatch", "SparseConvolutionNoiseMedium"]

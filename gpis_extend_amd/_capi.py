@@ -15,7 +15,7 @@ GPIS_OK, GPIS_INVALID_ARGUMENT, GPIS_DEVICE_ERROR, GPIS_UNSUPPORTED, GPIS_OUT_OF
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "DEVICE_ERROR", 3: "UNSUPPORTED", 4: "OUT_OF_MEMORY"}
 
 COV_SE, COV_PROC_NS = 0, 1
-MEAN_SPHERE, MEAN_BOX, MEAN_PLANE = 0, 1, 2
+MEAN_SPHERE, MEAN_BOX, MEAN_PLANE, MEAN_GRID = 0, 1, 2, 3
 CTX_GLOBAL, CTX_RENEWAL, CTX_RENEWAL_PLUS, CTX_NONE = 0, 1, 2, 3
 SCHEME_UNI, SCHEME_NEE, SCHEME_MIS = 0, 1, 2
 LS_CONST, LS_SINCOS = 0, 1
@@ -27,6 +27,7 @@ EXPORTS = (
     "gpis_create", "gpis_destroy", "gpis_last_error", "gpis_set_domain", "gpis_impulse_table",
     "gpis_eval_points", "gpis_trace_flights", "gpis_trace_paths", "gpis_get_counters",
     "gpis_reset_counters", "gpis_set_launch", "gpis_set_refill", "gpis_trace_flights_1d",
+    "gpis_set_mean_grid",
     # include/gpis_surrogate.h
     "gpis_net_create", "gpis_net_destroy", "gpis_net_last_error", "gpis_net_set_params", "gpis_net_get_params",
     "gpis_net_set_calibration", "gpis_net_features_1d", "gpis_net_classes_1d", "gpis_net_gradient",
@@ -51,6 +52,12 @@ class FieldDesc(C.Structure):
 
 
 _vp = C.c_void_p
+
+
+class MeanGrid(C.Structure):
+    _fields_ = [("voxels", _vp), ("origin", C.c_int32 * 3), ("dims", C.c_int32 * 3),
+                ("world_to_index", C.c_float * 12), ("clamp_min", C.c_float * 3), ("clamp_max", C.c_float * 3),
+                ("offset", C.c_float), ("scale", C.c_float), ("volume", C.c_int32)]
 
 
 class RaySoA(C.Structure):
@@ -125,6 +132,8 @@ def lib():
     L.gpis_set_launch.argtypes = [_vp, C.c_int32, C.c_int32]
     L.gpis_set_refill.restype = st
     L.gpis_set_refill.argtypes = [_vp, C.c_int32]
+    L.gpis_set_mean_grid.restype = st
+    L.gpis_set_mean_grid.argtypes = [_vp, C.POINTER(MeanGrid)]
     L.gpis_net_create.restype = st
     L.gpis_net_create.argtypes = [C.c_uint64, C.POINTER(_vp)]
     L.gpis_net_destroy.restype = st

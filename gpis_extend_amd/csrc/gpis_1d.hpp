@@ -13,6 +13,7 @@
 #pragma once
 
 #include "gpis_device.hpp"
+#include "gpis_exp.hpp"
 
 namespace gpis {
 
@@ -62,6 +63,7 @@ __device__ __forceinline__ Frame3 make_frame(float x, float y, float z) {
 // signed distances), the gradient rounded to float.
 __device__ __forceinline__ double mean_value_d(const FieldParams &P, float x, float y, float z) {
 #pragma clang fp contract(off)
+    if (P.mean_kind == 3) return grid_mean_d(P, x, y, z);
     const double p[3] = {(double)x - P.mcd[0], (double)y - P.mcd[1], (double)z - P.mcd[2]};
     if (P.mean_kind == 1) {
         double o2 = 0.0, mx = -1e300;
@@ -87,7 +89,9 @@ __device__ __forceinline__ void mean_grad_d(const FieldParams &P, float x, float
 #pragma clang fp contract(off)
     const double p[3] = {(double)x - P.mcd[0], (double)y - P.mcd[1], (double)z - P.mcd[2]};
     double g[3] = {0.0, 0.0, 0.0};
-    if (P.mean_kind == 1) {
+    if (P.mean_kind == 3) {
+        grid_mean_grad_d(P, x, y, z, g[0], g[1], g[2]);
+    } else if (P.mean_kind == 1) {
         double q[3], o2 = 0.0;
         int imax = 0;
         for (int i = 0; i < 3; ++i) {
@@ -218,7 +222,7 @@ __device__ __forceinline__ void noise1d_raw(Cells1D<NI> &C, bool &cache_valid, f
             if (to * to < 1.0f) {
                 const float pc = R * ti;
                 const float ab = pq - pc;                // splattingKernel1D: pQuery - pCenter
-                const float e = exp_dr(-(ab * ab / 2.0f));  // exp(-ab^2 / (2 * 1^2))
+                const float e = (float)exp_splat_d((double)(-(ab * ab / 2.0f)));  // exp(-ab^2 / (2 * 1^2))
                 const float g = -ab * e;                 // (-2 ab / 2) * e, exact in double then rounded
                 const float w = copysignf(1.0f, ts);
                 cv += w * e;

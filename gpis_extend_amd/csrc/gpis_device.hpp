@@ -78,6 +78,11 @@ struct FieldParams {
     // mean
     int mean_kind; float mc[3]; float mr; float mh[3]; float pn[3]; float poff;
     double mcd[3], mhd[3], pnd[3], poffd, mrd;  // the same mean parameters in double (exact 1D path)
+    // tabulated mean (mean_kind 3): dense voxel grid, x fastest
+    const float *grid;
+    int gorig[3], gdim[3];
+    float w2i[12], gcmin[3], gcmax[3], goff, gscale;
+    int gvolume;                                 // density -> -log(max(1e-4, density))
     // medium
     uint32_t global_seed; int single; int ctxt; int activate; int one_d; int scheme1d; int corr_xy;
     float step_size; float min_step_f; int max_bounce;
@@ -90,8 +95,80 @@ struct Eval4 { float v, gx, gy, gz; };
 
 // ------------------------------------------------------------------ means --
 // SphericalMean (GPFunctions.hpp:952-996); box SDF and plane are defined in code.
-__device__ __forceinline__ float mean_value(const FieldParams &P, float x, float y, float z, float &gx, float &gy,
-                                            float &gz) {
+// Trilinear density of the tabulated mean's voxel grid at world point p, in the order
+// of the reference's dense mean grid (RaySegmentBatch.cu:58-90).
+__device__ __forceinline__ float grid_voxel(const FieldParams &P, int x, int y, int z) {
+    x = max(0, min(x - P.gorig[0], P.gdim[0] - 1));
+    y = max(0, min(y - P.gorig[1], P.gdim[1] - 1));
+    z = max(0, min(z - P.gorig[2], P.gdim[2] - 1));
+    return P.grid[((size_t)z * P.gdim[1] + y) * P.gdim[0] + x];
+}
+__device__ __forceinline__ float grid_density(const FieldParams &P, float px, float py, float pz) {
+#pragma clang fp contract(off)
+    const float *m = P.w2i;
+    float qx = m[0] * px + m[1] * py + m[2] * pz + m[3];
+    float qy = m[4] * px + m[5] * py + m[6] * pz + m[7];
+    float qz = m[8] * px + m[9] * py + m[10] * pz + m[11];
+    qx = fminf(fmaxf(qx, P.gcmin[0]), P.gcmax[0]);
+    qy = fminf(fmaxf(qy, P.gcmin[1]), P.gcmax[1]);
+    qz = fminf(fmaxf(qz, P.gcmin[2]), P.gcmax[2]);
+    const int x = (int)floorf(qx), y = (int)floorf(qy), z = (int)floorf(qz);
+    const float fx = qx - (float)x, fy = qy - (float)y, fz = qz - (float)z;
+    float layer[2];
+    for (int dz = 0; dz < 2; ++dz) {
+        float row[2];
+        for (int dy = 0; dy < 2; ++dy) {
+            const float a = grid_voxel(P, x, y + dy, z + dz);
+            const float b = grid_voxel(P, x + 1, y + dy, z + dz);
+            row[dy] = a + fx * (b - a);
+        }
+        layer[dz] = row[0] + fy * (row[1] - row[0]);
+    }
+    return layer[0] + fz * (layer[1] - layer[0]);
+}
+
+// TabulatedMean::mean (GPFunctions.cpp:165-172): float lookup point, double arithmetic.
+__device__ __forceinline__ double grid_mean_d(const FieldParams &P, float x, float y, float z) {
+#pragma clang fp contract(off)
+    double res = (double)grid_density(P, x, y, z);
+    if (P.gvolume) res = -log(fmax(0.0001, res));
+    return (res + (double)P.goff) * (double)P.gscale;
+}
+// TabulatedMean::dmean_da (:179-190): forward differences with eps = 1e-3, the shifted
+// point formed in double and looked up in float.
+__device__ __forceinline__ double grid_mean_grad_d(const FieldParams &P, float x, float y, float z, double &gx,
+                                                   double &gy, double &gz) {
+#pragma clang fp contract(off)
+    const double eps = 0.001;
+    double c = 0.0;
+    // one lookup per iteration (the centre, then +x, +y, +z) keeps the inlined code small
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+        const float qx = k == 1 ? (float)((double)x + eps) : x;
+        const float qy = k == 2 ? (float)((double)y + eps) : y;
+        const float qz = k == 3 ? (float)((double)z + eps) : z;
+        const double m = grid_mean_d(P, qx, qy, qz);
+        if (k == 0) c = m;
+        const double d = (m - c) / eps;
+        if (k == 1) gx = d;
+        if (k == 2) gy = d;
+        if (k == 3) gz = d;
+    }
+    return c;
+}
+
+// Tabulated mean and its gradient rounded to float.
+__device__ __forceinline__ float grid_mean_value(const FieldParams &P, float x, float y, float z, float &gx,
+                                                 float &gy, float &gz) {
+    double dx, dy, dz;
+    const double c = grid_mean_grad_d(P, x, y, z, dx, dy, dz);
+    gx = (float)dx; gy = (float)dy; gz = (float)dz;
+    return (float)c;
+}
+
+// The analytic means (sphere, box, plane) and their gradients.
+__device__ __forceinline__ float analytic_mean_value(const FieldParams &P, float x, float y, float z, float &gx,
+                                                     float &gy, float &gz) {
     if (P.mean_kind == 1) {
         float px = x - P.mc[0], py = y - P.mc[1], pz = z - P.mc[2];
         float qx = fabsf(px) - P.mh[0], qy = fabsf(py) - P.mh[1], qz = fabsf(pz) - P.mh[2];
@@ -121,6 +198,16 @@ __device__ __forceinline__ float mean_value(const FieldParams &P, float x, float
     float inv = 1.0f / r;
     gx = px * inv; gy = py * inv; gz = pz * inv;
     return r - P.mr;
+}
+
+// Mean kinds as a compile-time choice: any (runtime switch), analytic only, grid only.
+enum { MEAN_ANY = 0, MEAN_ANALYTIC = 1, MEAN_GRID = 2 };
+
+template <int MEAN = MEAN_ANY>
+__device__ __forceinline__ float mean_value(const FieldParams &P, float x, float y, float z, float &gx, float &gy,
+                                            float &gz) {
+    if (MEAN == MEAN_GRID || (MEAN == MEAN_ANY && P.mean_kind == 3)) return grid_mean_value(P, x, y, z, gx, gy, gz);
+    return analytic_mean_value(P, x, y, z, gx, gy, gz);
 }
 
 }  // namespace gpis

@@ -312,12 +312,12 @@ __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealStat
 __device__ __forceinline__ float amplitude(const FieldParams &P) { return P.amp_const; }
 
 // f(p) = amplitude * noise + mean, and its world gradient (evaluateValue/evaluateGradient).
-template <int NI, bool ISO>
+template <int NI, bool ISO, int MEAN = MEAN_ANY>
 __device__ __forceinline__ Eval4 eval_field(const FieldParams &P, const RealState &rs,
                                             float px, float py, float pz, bool cond) {
     Eval4 n = eval_noise<NI, ISO>(P, rs, px, py, pz, cond);
     float mgx, mgy, mgz;
-    float m = mean_value(P, px, py, pz, mgx, mgy, mgz);
+    float m = mean_value<MEAN>(P, px, py, pz, mgx, mgy, mgz);
     float a = amplitude(P);
     Eval4 r = {a * n.v + m, a * n.gx + mgx, a * n.gy + mgy, a * n.gz + mgz};
     return r;

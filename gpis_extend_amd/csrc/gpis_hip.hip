@@ -35,8 +35,10 @@ struct gpis_ctx {
     int num_cus = 0;
     int waves_per_cu = 0;
     int block_size = 256;
-    int batch_k = 1;  // finish/bounce/refill batch of the 3D kernel (lanes per wave)
+    int batch_k = 8;  // finish/bounce/refill batch of the 3D kernel (lanes per wave)
+    int batch_k_1d = 16;  // ... and of the 1D kernel
     float nf1d = 1.0f;  // sqrt(sparseConvNoiseVariance1D) of the 1D grid
+    float *grid = nullptr;  // device voxels of a tabulated mean
     std::string err;
 };
 
@@ -69,6 +71,7 @@ static gpis_status build_params(gpis_ctx *c) {
         return fail(c, GPIS_UNSUPPORTED, "1D sampling is served for the stationary SE kernel");
     if (d.one_d && d.correlation_xy)
         return fail(c, GPIS_UNSUPPORTED, "1D_gradient_correlationXY (Renewal+ transverse law) is not served");
+    if (d.mean_kind < GPIS_MEAN_SPHERE || d.mean_kind > GPIS_MEAN_GRID) return fail(c, GPIS_UNSUPPORTED, "mean kind");
     P.kind = d.cov_kind;
     P.multires = (d.cov_kind == GPIS_COV_PROC_NS) && d.ns_multires;
     P.sigma = d.sigma;
@@ -317,10 +320,11 @@ __device__ __forceinline__ float box_exit(float ox, float oy, float oz, float dx
 // in FP64.  With one observation point the value and gradient observations decouple
 // (the covariance gradient vanishes at zero lag), so the Schur complement of the
 // value+gradient observation covariance is the diagonal second-derivative block.
+template <int MEAN>
 __device__ __noinline__ void condition_coefficients(const FieldParams &P, const Eval4 &e, float ox, float oy, float oz,
                                                     float pv, float pgx, float pgy, float pgz, RealState &rs) {
     float mgx, mgy, mgz;
-    double mean = mean_value(P, ox, oy, oz, mgx, mgy, mgz);
+    double mean = mean_value<MEAN>(P, ox, oy, oz, mgx, mgy, mgz);
     double amp = P.amp_const;
     double cur_v = ((double)e.v - mean) / amp;
     double a = ((double)pv - mean) / amp - cur_v;
@@ -379,8 +383,9 @@ enum { ST_FLIGHTS = 0, ST_EVALS, ST_STEPS, ST_HITS, ST_EXITS, ST_FAILS, ST_INVAL
 // every iteration each live lane performs one field evaluation.  Lanes whose flight
 // (or path) finished take the next ray indices from a global counter: one atomic
 // per wave, ranks from the ballot mask.  COND: the medium conditions secondary
-// flights (Renewal/Renewal+); PATHS: hits continue as diffuse bounces.
-template <int NI, bool ISO, bool COND, bool PATHS>
+// flights (Renewal/Renewal+); PATHS: hits continue as diffuse bounces; GRID: the mean
+// is tabulated (the analytic means otherwise, whose variants stay free of its registers).
+template <int NI, bool ISO, bool COND, bool PATHS, bool GRID>
 __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(FieldParams P, RayIn in, ReplyOut out,
                                                                             PathOut pout, unsigned int n,
                                                                             unsigned int *work,
@@ -548,7 +553,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
         if (phase == PH_NEED || phase == PH_DEAD || phase == PH_DONE) continue;
 
         // ---- one evaluation per live lane
-        Eval4 e = eval_field<NI, ISO>(P, rs, qx, qy, qz, !COND || phase != PH_COND);
+        Eval4 e = eval_field<NI, ISO, GRID ? MEAN_GRID : MEAN_ANALYTIC>(P, rs, qx, qy, qz, !COND || phase != PH_COND);
         ++evals_flight;
         const float fv = e.v;
         // the next query point: t along the ray (unit: along the normalised direction)
@@ -556,7 +561,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
         bool unit = false;
 
         if (COND && phase == PH_COND) {
-            condition_coefficients(P, e, ox, oy, oz, prev_v, prev_gx, prev_gy, prev_gz, rs);
+            condition_coefficients<GRID ? MEAN_GRID : MEAN_ANALYTIC>(P, e, ox, oy, oz, prev_v, prev_gx, prev_gy, prev_gz, rs);
             phase = PH_F0;
             tq = (double)near_t;
         } else if (phase == PH_F0) {
@@ -756,7 +761,7 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
                                                           unsigned long long *stats, float nf, int use_light,
                                                           float lx, float ly, float lz, Nee1DOut nee,
                                                           const uint32_t *__restrict__ idx,
-                                                          const uint32_t *__restrict__ n_dev) {
+                                                          const uint32_t *__restrict__ n_dev, int batch_k) {
     // idx / n_dev: optional list of ray indices (and its device-side length) to trace
     const unsigned nn = n_dev ? *n_dev : n;
     __shared__ unsigned int bstat[ST_N];
@@ -774,182 +779,28 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
     unsigned evals_flight = 0;
     float gxv = 0, gyv = 0, gzv = 0;
     uint32_t outcome = 0;
+    float bx1 = 0.f, by1 = 0.f;      // Box-Muller pair 0 / sqrt 2 (Renewal+ conditioning)
+    float ndv_end = 0.f;             // derivative along the ray at the endpoint
+    float ndv_nee = 0.f;             // ... and at the NEE point
+    int nee_stage = 0;               // 1 once the NEE evaluation is done
     Ray1D R = {};
     Cells1D<NI> C;
     C.base = 0;
     bool cache_valid = false;
 
     for (;;) {
-        unsigned long long need = __ballot(phase == PH_NEED);
-        if (need) {
-            unsigned cnt = (unsigned)__popcll(need);
-            unsigned leader = (unsigned)(__ffsll((long long)need) - 1);
-            unsigned base = 0;
-            if (lane == leader) base = atomicAdd(work, cnt);
-            base = __shfl(base, leader);
-            if (phase == PH_NEED) {
-                const unsigned rl = base + mask_rank(need);
-                if (rl >= nn) {
-                    phase = PH_DEAD;
-                } else {
-                    const unsigned r = idx ? idx[rl] : rl;
-                    ray = r;
-                    ox = in.ox[r]; oy = in.oy[r]; oz = in.oz[r];
-                    dx = in.dx[r]; dy = in.dy[r]; dz = in.dz[r];
-                    near_t = in.near_t[r]; far_t = in.far_t[r];
-                    float jit = in.jitter[r];
-                    R.seed = in.noise_seed[r];
-                    normal_seed = in.normal_seed[r];
-                    R.cond = 0;
-                    first = (in.flags[r] & 1u) ? 1 : 0;
-                    if (!first) { prev_v = in.pv[r]; prev_gx = in.pgx[r]; prev_gy = in.pgy[r]; prev_gz = in.pgz[r]; }
-                    evals_flight = 0;
-                    cache_valid = false;
-                    ray_scales(P.aniso, P.l_aniso, P.l_aniso_inv, lscale, r, R.la, R.ia);
-                    float ix = R.ia[0] * dx, iy = R.ia[1] * dy, iz = R.ia[2] * dz;
-                    normalize3(ix, iy, iz);
-                    R.F = make_frame(ix, iy, iz);
-                    if (!isfinite(far_t)) far_t = near_t + 2000.f;
-                    bool bad = !isfinite(ox) || !isfinite(oy) || !isfinite(oz) || !isfinite(dx) || !isfinite(dy) ||
-                               !isfinite(dz) || !isfinite(near_t) || !(jit >= 0.f && jit < 1.f) ||
-                               !(R.la[0] > 0.f && R.la[1] > 0.f && R.la[2] > 0.f) || !isfinite(R.la[0] + R.la[1] + R.la[2]);
-                    if (bad || far_t <= near_t) {
-                        out.t[r] = far_t; out.value[r] = (bad || first) ? 0.f : prev_v;
-                        out.gx[r] = out.gy[r] = out.gz[r] = 0.f;
-                        out.outcome[r] = bad ? GPIS_INVALID : GPIS_EXIT;
-                        if (out.evals) out.evals[r] = 0;
-                        if (use_light) { nee.gx[r] = nee.gy[r] = nee.gz[r] = 0.f; nee.pdf[r] = 0.f; }
-                        atomicAdd(&bstat[bad ? ST_INVALID : ST_EXITS], 1u);
-                        phase = PH_NEED;
-                    } else {
-                        step = (far_t - near_t) / P.min_step_f;
-                        if (P.step_size < step) step = P.step_size;
-                        t0 = mul_add_exact(step, jit, near_t);
-                        if (P.activate && !first) {
-                            phase = PH_COND;
-                            qx = ox; qy = oy; qz = oz;
-                        } else {
-                            phase = PH_F0;
-                            ray_point(ox, oy, oz, dx, dy, dz, (double)near_t, qx, qy, qz);
-                        }
-                    }
-                }
-            }
-        }
-        if (!__ballot(phase != PH_DEAD)) break;
-        if (phase == PH_NEED || phase == PH_DEAD) continue;
-
-        float fv, nv, ndv;
-        eval1d<NI>(P, R, C, cache_valid, nf, qx, qy, qz, phase != PH_COND, fv, nv, ndv);
-        ++evals_flight;
-
-        if (phase == PH_COND) {
-            // conditioning1D (:576-656)
-#pragma clang fp contract(off)
-            const float amp = P.amp_const;
-            R.uo = ray_coord(R, ox, oy, oz);
-            R.v1 = 0.f;
-            R.g1z = 0.f;
-            if (amp != 0.f) {
-                const float mean = (float)mean_value_d(P, ox, oy, oz);
-                R.v1 = (prev_v - mean) / amp - nv;
-                if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
-                    float mgx, mgy, mgz;
-                    mean_grad_d(P, ox, oy, oz, mgx, mgy, mgz);
-                    double x1, y1;
-                    transverse_pair(normal_seed, 0, x1, y1);
-                    float cgx, cgy, cgz;
-                    ray_grad_to_world(R, (float)x1, (float)y1, ndv, cgx, cgy, cgz);
-                    float ddx = (prev_gx - mgx) / amp - cgx;
-                    float ddy = (prev_gy - mgy) / amp - cgy;
-                    float ddz = (prev_gz - mgz) / amp - cgz;
-                    // gradient_scale = -2 * frame.toLocal(l_aniso * delta) * scale^2; only z
-                    // survives without the xy correlation
-                    float gsz = dot3(R.F.nx, R.F.ny, R.F.nz, R.la[0] * ddx, R.la[1] * ddy, R.la[2] * ddz);
-                    R.g1z = -2.f * gsz * 1.0f;
-                }
-            }
-            R.cond = 1;
-            phase = PH_F0;
-            ray_point(ox, oy, oz, dx, dy, dz, (double)near_t, qx, qy, qz);
-        } else if (phase == PH_F0) {
-            sign0 = fv < 0.f ? -1 : 1;
-            pf = fv;
-            k = 0;
-            if (t0 < far_t) {
-                phase = PH_MARCH;
-                ray_point(ox, oy, oz, dx, dy, dz, (double)t0, qx, qy, qz);
-            } else {
-                phase = PH_FAR;
-                ray_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
-            }
-        } else if (phase == PH_MARCH) {
-            ++k;
-            int sc = fv < 0.f ? -1 : 1;
-            tk = lattice_t(t0, k - 1, step);
-            if (!first && k == 1) {
-                sign0 = sc;
-            } else if (sc != sign0) {
-                fac = (double)pf / ((double)pf - (double)fv);
-                tp = lerp_d(tk - (double)step, tk, fac);
-                tt = tp;
-                phase = PH_SECANT;
-                ray_point(ox, oy, oz, dx, dy, dz, tt, qx, qy, qz);
-            }
-            if (phase == PH_MARCH) {
-                pf = fv;
-                double tn = lattice_t(t0, k, step);
-                if (k >= kMaxSteps) {
-                    atomicAdd(&bstat[ST_INVALID], 1u);
-                    out.t[ray] = (float)tn; out.value[ray] = fv;
-                    out.gx[ray] = out.gy[ray] = out.gz[ray] = 0.f;
-                    out.outcome[ray] = GPIS_INVALID;
-                    if (out.evals) out.evals[ray] = evals_flight;
-                    if (use_light) { nee.gx[ray] = nee.gy[ray] = nee.gz[ray] = 0.f; nee.pdf[ray] = 0.f; }
-                    phase = PH_NEED;
-                } else if (tn < (double)far_t) {
-                    ray_point(ox, oy, oz, dx, dy, dz, tn, qx, qy, qz);
-                } else {
-                    phase = PH_FAR;
-                    ray_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
-                }
-            }
-        } else if (phase == PH_SECANT) {
-            bool done = false;
-            if ((fv < 0.f ? -1 : 1) == sign0) {
-                done = true;
-            } else {
-                fac *= 0.9;
-                if (fac <= 0.01) {
-                    tp = 0.0;
-                    done = true;
-                } else {
-                    tp = tt;
-                    tt = lerp_d(tk - (double)step, tk, fac);
-                    ray_point(ox, oy, oz, dx, dy, dz, tt, qx, qy, qz);
-                }
-            }
-            if (done) {
-                hit = 1;
-                t_end = (float)tp;
-                val_end = 0.f;
-                phase = PH_GRAD;
-                grad_point(ox, oy, oz, dx, dy, dz, tp, qx, qy, qz);
-            }
-        } else if (phase == PH_FAR) {
-            hit = 0;
-            t_end = far_t;
-            val_end = fv;
-            phase = PH_GRAD;
-            grad_point(ox, oy, oz, dx, dy, dz, (double)far_t, qx, qy, qz);
-        } else if (phase == PH_GRAD || phase == PH_NEE) {
-            if (phase == PH_GRAD) {
+        // ---- finish and refill, batched as in flight_kernel: lanes whose flight ended
+        // (PH_DONE) or that want a ray wait until batch_k of them can be served together.
+        const unsigned long long waiting = __ballot(phase == PH_DONE || phase == PH_NEED);
+        const unsigned long long running = __ballot(phase != PH_DONE && phase != PH_NEED && phase != PH_DEAD);
+        if (waiting && ((unsigned)__popcll(waiting) >= (unsigned)batch_k || !running)) {
+            if (phase == PH_DONE && nee_stage == 0) {
                 // evaluateGradientNoise1D at the endpoint (:204-242): transverse pair 0 when
-                // tSegment == 0, otherwise an independent second pair
+                // tSegment == 0, otherwise an independent second pair; resumeDistance rules
                 double x2, y2;
                 transverse_pair(normal_seed, t_end == 0.0f ? 0 : 1, x2, y2);
                 float wx, wy, wz;
-                ray_grad_to_world(R, (float)x2, (float)y2, ndv, wx, wy, wz);
+                ray_grad_to_world(R, (float)x2, (float)y2, ndv_end, wx, wy, wz);
                 float mgx, mgy, mgz;
                 mean_grad_d(P, qx, qy, qz, mgx, mgy, mgz);
                 const float amp = P.amp_const;
@@ -975,35 +826,219 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
                     outcome = GPIS_EXIT;
                 }
                 if (use_light && (outcome & 0xFFu) == GPIS_HIT) {
-                    // NEE at the scattering point sample.p = pos + t * dir (float)
+                    // one more evaluation at the scattering point sample.p = pos + t * dir (float)
                     phase = PH_NEE;
                     qx = mul_add_exact(t_end, dx, ox);
                     qy = mul_add_exact(t_end, dy, oy);
                     qz = mul_add_exact(t_end, dz, oz);
-                    continue;
+                } else {
+                    if (use_light) { nee.gx[ray] = nee.gy[ray] = nee.gz[ray] = 0.f; nee.pdf[ray] = 0.f; }
+                    nee_stage = 2;
                 }
-                if (use_light) { nee.gx[ray] = nee.gy[ray] = nee.gz[ray] = 0.f; nee.pdf[ray] = 0.f; }
-            } else {
+            } else if (phase == PH_DONE && nee_stage == 1) {
                 // mirror normal reflecting the ray toward the light, then neeGrad / neePDF
                 float wlx = lx - qx, wly = ly - qy, wlz = lz - qz;
                 normalize3(wlx, wly, wlz);
                 float hx = wlx - dx, hy = wly - dy, hz = wlz - dz;
                 normalize3(hx, hy, hz);
                 float ngx, ngy, ngz, pdf;
-                nee1d(P, R, hx, hy, hz, qx, qy, qz, ndv, dx, dy, dz, ngx, ngy, ngz, pdf);
+                nee1d(P, R, hx, hy, hz, qx, qy, qz, ndv_nee, dx, dy, dz, ngx, ngy, ngz, pdf);
                 nee.gx[ray] = ngx; nee.gy[ray] = ngy; nee.gz[ray] = ngz; nee.pdf[ray] = pdf;
+                nee_stage = 2;
             }
-            const unsigned oc = outcome & 0xFFu;
-            atomicAdd(&bstat[ST_FLIGHTS], 1u);
-            atomicAdd(&bstat[ST_EVALS], evals_flight);
-            atomicAdd(&bstat[ST_STEPS], (unsigned)k);
-            atomicAdd(&bstat[oc == GPIS_HIT ? ST_HITS : (oc == GPIS_EXIT ? ST_EXITS : ST_FAILS)], 1u);
-            out.t[ray] = t_end; out.value[ray] = val_end;
-            out.gx[ray] = gxv; out.gy[ray] = gyv; out.gz[ray] = gzv;
-            out.outcome[ray] = outcome;
-            if (out.evals) out.evals[ray] = evals_flight;
-            phase = PH_NEED;
+            if (phase == PH_DONE && nee_stage == 2) {
+                const unsigned oc = outcome & 0xFFu;
+                atomicAdd(&bstat[ST_FLIGHTS], 1u);
+                atomicAdd(&bstat[ST_EVALS], evals_flight);
+                atomicAdd(&bstat[ST_STEPS], (unsigned)k);
+                atomicAdd(&bstat[oc == GPIS_HIT ? ST_HITS : (oc == GPIS_EXIT ? ST_EXITS : ST_FAILS)], 1u);
+                out.t[ray] = t_end; out.value[ray] = val_end;
+                out.gx[ray] = gxv; out.gy[ray] = gyv; out.gz[ray] = gzv;
+                out.outcome[ray] = outcome;
+                if (out.evals) out.evals[ray] = evals_flight;
+                phase = PH_NEED;
+            }
+            const unsigned long long need = __ballot(phase == PH_NEED);
+            if (need) {
+                unsigned cnt = (unsigned)__popcll(need);
+                unsigned leader = (unsigned)(__ffsll((long long)need) - 1);
+                unsigned base = 0;
+                if (lane == leader) base = atomicAdd(work, cnt);
+                base = __shfl(base, leader);
+                if (phase == PH_NEED) {
+                    const unsigned rl = base + mask_rank(need);
+                    if (rl >= nn) {
+                        phase = PH_DEAD;
+                    } else {
+                        const unsigned r = idx ? idx[rl] : rl;
+                        ray = r;
+                        ox = in.ox[r]; oy = in.oy[r]; oz = in.oz[r];
+                        dx = in.dx[r]; dy = in.dy[r]; dz = in.dz[r];
+                        near_t = in.near_t[r]; far_t = in.far_t[r];
+                        float jit = in.jitter[r];
+                        R.seed = in.noise_seed[r];
+                        normal_seed = in.normal_seed[r];
+                        R.cond = 0;
+                        first = (in.flags[r] & 1u) ? 1 : 0;
+                        if (!first) { prev_v = in.pv[r]; prev_gx = in.pgx[r]; prev_gy = in.pgy[r]; prev_gz = in.pgz[r]; }
+                        evals_flight = 0;
+                        nee_stage = 0;
+                        cache_valid = false;
+                        ray_scales(P.aniso, P.l_aniso, P.l_aniso_inv, lscale, r, R.la, R.ia);
+                        float ix = R.ia[0] * dx, iy = R.ia[1] * dy, iz = R.ia[2] * dz;
+                        normalize3(ix, iy, iz);
+                        R.F = make_frame(ix, iy, iz);
+                        if (!isfinite(far_t)) far_t = near_t + 2000.f;
+                        bool bad = !isfinite(ox) || !isfinite(oy) || !isfinite(oz) || !isfinite(dx) || !isfinite(dy) ||
+                                   !isfinite(dz) || !isfinite(near_t) || !(jit >= 0.f && jit < 1.f) ||
+                                   !(R.la[0] > 0.f && R.la[1] > 0.f && R.la[2] > 0.f) || !isfinite(R.la[0] + R.la[1] + R.la[2]);
+                        if (bad || far_t <= near_t) {
+                            out.t[r] = far_t; out.value[r] = (bad || first) ? 0.f : prev_v;
+                            out.gx[r] = out.gy[r] = out.gz[r] = 0.f;
+                            out.outcome[r] = bad ? GPIS_INVALID : GPIS_EXIT;
+                            if (out.evals) out.evals[r] = 0;
+                            if (use_light) { nee.gx[r] = nee.gy[r] = nee.gz[r] = 0.f; nee.pdf[r] = 0.f; }
+                            atomicAdd(&bstat[bad ? ST_INVALID : ST_EXITS], 1u);
+                            phase = PH_NEED;
+                        } else {
+                            step = (far_t - near_t) / P.min_step_f;
+                            if (P.step_size < step) step = P.step_size;
+                            t0 = mul_add_exact(step, jit, near_t);
+                            if (P.activate && !first) {
+                                phase = PH_COND;
+                                qx = ox; qy = oy; qz = oz;
+                                if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
+                                    double x1, y1;
+                                    transverse_pair(normal_seed, 0, x1, y1);
+                                    bx1 = (float)x1; by1 = (float)y1;
+                                }
+                            } else {
+                                phase = PH_F0;
+                                ray_point(ox, oy, oz, dx, dy, dz, (double)near_t, qx, qy, qz);
+                            }
+                        }
+                    }
+                }
+            }
         }
+        if (!__ballot(phase != PH_DEAD)) break;
+        if (phase == PH_NEED || phase == PH_DEAD || phase == PH_DONE) continue;
+
+        float fv, nv, ndv;
+        eval1d<NI>(P, R, C, cache_valid, nf, qx, qy, qz, phase != PH_COND, fv, nv, ndv);
+        ++evals_flight;
+        double tq = 0.0;
+        bool unit = false;
+
+        if (phase == PH_COND) {
+            // conditioning1D (:576-656)
+#pragma clang fp contract(off)
+            const float amp = P.amp_const;
+            R.uo = ray_coord(R, ox, oy, oz);
+            R.v1 = 0.f;
+            R.g1z = 0.f;
+            if (amp != 0.f) {
+                const float mean = (float)mean_value_d(P, ox, oy, oz);
+                R.v1 = (prev_v - mean) / amp - nv;
+                if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
+                    float mgx, mgy, mgz;
+                    mean_grad_d(P, ox, oy, oz, mgx, mgy, mgz);
+                    float cgx, cgy, cgz;
+                    ray_grad_to_world(R, bx1, by1, ndv, cgx, cgy, cgz);
+                    float ddx = (prev_gx - mgx) / amp - cgx;
+                    float ddy = (prev_gy - mgy) / amp - cgy;
+                    float ddz = (prev_gz - mgz) / amp - cgz;
+                    // gradient_scale = -2 * frame.toLocal(l_aniso * delta) * scale^2; only z
+                    // survives without the xy correlation
+                    float gsz = dot3(R.F.nx, R.F.ny, R.F.nz, R.la[0] * ddx, R.la[1] * ddy, R.la[2] * ddz);
+                    R.g1z = -2.f * gsz * 1.0f;
+                }
+            }
+            R.cond = 1;
+            phase = PH_F0;
+            tq = (double)near_t;
+        } else if (phase == PH_F0) {
+            sign0 = fv < 0.f ? -1 : 1;
+            pf = fv;
+            k = 0;
+            if (t0 < far_t) {
+                phase = PH_MARCH;
+                tq = (double)t0;
+            } else {
+                phase = PH_FAR;
+                tq = (double)far_t;
+            }
+        } else if (phase == PH_MARCH) {
+            ++k;
+            int sc = fv < 0.f ? -1 : 1;
+            tk = lattice_t(t0, k - 1, step);
+            if (!first && k == 1) {
+                sign0 = sc;
+            } else if (sc != sign0) {
+                fac = (double)pf / ((double)pf - (double)fv);
+                tp = lerp_d(tk - (double)step, tk, fac);
+                tt = tp;
+                phase = PH_SECANT;
+                tq = tt;
+            }
+            if (phase == PH_MARCH) {
+                pf = fv;
+                double tn = lattice_t(t0, k, step);
+                if (k >= kMaxSteps) {
+                    atomicAdd(&bstat[ST_INVALID], 1u);
+                    out.t[ray] = (float)tn; out.value[ray] = fv;
+                    out.gx[ray] = out.gy[ray] = out.gz[ray] = 0.f;
+                    out.outcome[ray] = GPIS_INVALID;
+                    if (out.evals) out.evals[ray] = evals_flight;
+                    if (use_light) { nee.gx[ray] = nee.gy[ray] = nee.gz[ray] = 0.f; nee.pdf[ray] = 0.f; }
+                    phase = PH_NEED;
+                } else if (tn < (double)far_t) {
+                    tq = tn;
+                } else {
+                    phase = PH_FAR;
+                    tq = (double)far_t;
+                }
+            }
+        } else if (phase == PH_SECANT) {
+            bool done = false;
+            if ((fv < 0.f ? -1 : 1) == sign0) {
+                done = true;
+            } else {
+                fac *= 0.9;
+                if (fac <= 0.01) {
+                    tp = 0.0;
+                    done = true;
+                } else {
+                    tp = tt;
+                    tt = lerp_d(tk - (double)step, tk, fac);
+                    tq = tt;
+                }
+            }
+            if (done) {
+                hit = 1;
+                t_end = (float)tp;
+                val_end = 0.f;
+                phase = PH_GRAD;
+                tq = tp;
+                unit = true;
+            }
+        } else if (phase == PH_FAR) {
+            hit = 0;
+            t_end = far_t;
+            val_end = fv;
+            phase = PH_GRAD;
+            tq = (double)far_t;
+            unit = true;
+        } else if (phase == PH_GRAD) {
+            ndv_end = ndv;      // the gradient itself is formed in the batched finish step
+            phase = PH_DONE;
+        } else if (phase == PH_NEE) {
+            ndv_nee = ndv;
+            nee_stage = 1;
+            phase = PH_DONE;
+        }
+        if (phase == PH_F0 || phase == PH_MARCH || phase == PH_SECANT || phase == PH_FAR || phase == PH_GRAD)
+            point_at(ox, oy, oz, dx, dy, dz, tq, unit, qx, qy, qz);
     }
     __syncthreads();
     if (threadIdx.x < ST_N && bstat[threadIdx.x]) atomicAdd(&stats[threadIdx.x], (unsigned long long)bstat[threadIdx.x]);
@@ -1012,10 +1047,19 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
 template <bool PATHS>
 static void launch_flights(gpis_ctx *c, unsigned blocks, unsigned threads, hipStream_t st, RayIn in, ReplyOut out,
                            PathOut pout, unsigned n, int max_bounces, uint32_t bounce_seed, float box) {
-#define GPIS_FLIGHT_LAUNCH(NI, ISO, COND)                                                                         \
-    hipLaunchKernelGGL((flight_kernel<NI, ISO, COND, PATHS>), dim3(blocks), dim3(threads), 0, st, c->P, in, out, \
-                       pout, n, c->work, c->counters, max_bounces, bounce_seed, box, c->batch_k)
+#define GPIS_FLIGHT_LAUNCH(NI, ISO, COND)                                                                          \
+    do {                                                                                                           \
+        if (grid)                                                                                                  \
+            hipLaunchKernelGGL((flight_kernel<NI, ISO, COND, PATHS, true>), dim3(blocks), dim3(threads), 0, st,    \
+                               c->P, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box,        \
+                               c->batch_k);                                                                        \
+        else                                                                                                       \
+            hipLaunchKernelGGL((flight_kernel<NI, ISO, COND, PATHS, false>), dim3(blocks), dim3(threads), 0, st,   \
+                               c->P, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box,        \
+                               c->batch_k);                                                                        \
+    } while (0)
     const bool cond = c->P.activate != 0;
+    const bool grid = c->P.mean_kind == GPIS_MEAN_GRID;
     if (c->P.N == 8 && c->P.iso && !cond) GPIS_FLIGHT_LAUNCH(8, true, false);
     else if (c->P.N == 8 && c->P.iso) GPIS_FLIGHT_LAUNCH(8, true, true);
     else if (c->P.N == 8 && !cond) GPIS_FLIGHT_LAUNCH(8, false, false);
@@ -1062,6 +1106,7 @@ gpis_status gpis_destroy(gpis_ctx *c) {
     if (!c) return GPIS_INVALID_ARGUMENT;
     (void)hipDeviceSynchronize();
     if (c->table) (void)hipFree(c->table);
+    if (c->grid) (void)hipFree(c->grid);
     if (c->counters) (void)hipFree(c->counters);
     if (c->work) (void)hipFree(c->work);
     delete c;
@@ -1069,6 +1114,46 @@ gpis_status gpis_destroy(gpis_ctx *c) {
 }
 
 const char *gpis_last_error(const gpis_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+gpis_status gpis_set_mean_grid(gpis_ctx *c, const gpis_mean_grid *g) {
+    if (!c || !g || !g->voxels) return fail(c, GPIS_INVALID_ARGUMENT, "null argument");
+    if (c->P.mean_kind != GPIS_MEAN_GRID) return fail(c, GPIS_INVALID_ARGUMENT, "the medium's mean is not tabulated");
+    size_t count = 1;
+    for (int i = 0; i < 3; ++i) {
+        // the same shape rule as the reference's dense mean grid (RaySegmentBatch.cu:312-313)
+        if (g->dims[i] < 2) return fail(c, GPIS_INVALID_ARGUMENT, "grid dimensions must be at least 2");
+        if (!(g->clamp_min[i] <= g->clamp_max[i])) return fail(c, GPIS_INVALID_ARGUMENT, "empty clamp box");
+        count *= (size_t)g->dims[i];
+    }
+    if (count > ((size_t)1 << 34)) return fail(c, GPIS_INVALID_ARGUMENT, "grid too large");
+    for (int i = 0; i < 12; ++i)
+        if (!std::isfinite(g->world_to_index[i])) return fail(c, GPIS_INVALID_ARGUMENT, "non-finite transform");
+    if (!std::isfinite(g->offset) || !std::isfinite(g->scale)) return fail(c, GPIS_INVALID_ARGUMENT, "non-finite offset or scale");
+    HIP_TRY(c, hipDeviceSynchronize());
+    if (c->grid) { HIP_TRY(c, hipFree(c->grid)); c->grid = nullptr; }
+    c->P.grid = nullptr;
+    HIP_TRY(c, hipMalloc((void **)&c->grid, count * sizeof(float)));
+    HIP_TRY(c, hipMemcpy(c->grid, g->voxels, count * sizeof(float), hipMemcpyHostToDevice));
+    FieldParams &P = c->P;
+    P.grid = c->grid;
+    for (int i = 0; i < 3; ++i) {
+        P.gorig[i] = g->origin[i];
+        P.gdim[i] = g->dims[i];
+        P.gcmin[i] = g->clamp_min[i];
+        P.gcmax[i] = g->clamp_max[i];
+    }
+    for (int i = 0; i < 12; ++i) P.w2i[i] = g->world_to_index[i];
+    P.goff = g->offset;
+    P.gscale = g->scale;
+    P.gvolume = g->volume != 0;
+    return GPIS_OK;
+}
+
+static gpis_status mean_ready(gpis_ctx *c) {
+    if (c->P.mean_kind == GPIS_MEAN_GRID && !c->P.grid)
+        return fail(c, GPIS_INVALID_ARGUMENT, "tabulated mean: call gpis_set_mean_grid first");
+    return GPIS_OK;
+}
 
 gpis_status gpis_set_domain(gpis_ctx *c, const float lo[3], const float hi[3], uint32_t seed) {
     if (!c || !lo || !hi) return fail(c, GPIS_INVALID_ARGUMENT, "null argument");
@@ -1124,6 +1209,7 @@ gpis_status gpis_eval_points(gpis_ctx *c, const float *x, const float *y, const 
     if (!c || n < 0) return fail(c, GPIS_INVALID_ARGUMENT, "bad argument");
     if (n == 0) return GPIS_OK;
     if (!x || !y || !z || !v || !gx || !gy || !gz) return fail(c, GPIS_INVALID_ARGUMENT, "null pointer");
+    if (gpis_status s = mean_ready(c)) return s;
     int threads = 256;
     unsigned blocks = (unsigned)((n + threads - 1) / threads);
 #define GPIS_EVAL_LAUNCH(NI, ISO)                                                                              \
@@ -1148,7 +1234,7 @@ static gpis_status check_rays(gpis_ctx *c, const gpis_ray_soa *r, int64_t n) {
     if (!r->prev_value || !r->prev_gx || !r->prev_gy || !r->prev_gz)
         return fail(c, GPIS_INVALID_ARGUMENT, "null previous-state field");
     if (r->length_scale) return fail(c, GPIS_UNSUPPORTED, "per-ray length scale is a 1D feature");
-    return GPIS_OK;
+    return mean_ready(c);
 }
 
 static void launch_shape(gpis_ctx *c, int64_t n, unsigned &blocks, unsigned &threads) {
@@ -1213,6 +1299,7 @@ gpis_status gpis_set_refill(gpis_ctx *c, int32_t batch) {
     if (!c) return GPIS_INVALID_ARGUMENT;
     if (batch < 0 || batch > 64) return fail(c, GPIS_INVALID_ARGUMENT, "refill batch must be in 0..64 lanes");
     c->batch_k = batch < 1 ? 1 : batch;
+    c->batch_k_1d = c->batch_k;
     return GPIS_OK;
 }
 
@@ -1229,7 +1316,8 @@ static gpis_status launch_flight1d(gpis_ctx *c, const gpis_ray_soa *rays, int64_
     unsigned blocks = (unsigned)(need < want ? need : (want < 1 ? 1 : want));
     hipLaunchKernelGGL((flight1d_kernel<8>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
                        rays->length_scale, out, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
-                       light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, nee, idx, n_dev);
+                       light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, nee, idx, n_dev,
+                       c->batch_k_1d);
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }
@@ -1249,6 +1337,7 @@ gpis_status gpis_trace_flights_1d(gpis_ctx *c, const gpis_ray_soa *rays, int64_t
     if (light && (!nee || !nee->gx || !nee->gy || !nee->gz || !nee->pdf))
         return fail(c, GPIS_INVALID_ARGUMENT, "NEE needs all four output arrays");
     if (c->P.N != 8) return fail(c, GPIS_UNSUPPORTED, "1D flights are compiled for 8 impulses per cell");
+    if (gpis_status s = mean_ready(c)) return s;
     if (n == 0) return GPIS_OK;
     ReplyOut out = {reply->t, reply->value, reply->gx, reply->gy, reply->gz, reply->outcome, reply->evals};
     Nee1DOut no = {};
@@ -1539,6 +1628,7 @@ gpis_status gpis_net_features_1d(gpis_ctx *c, const gpis_ray_soa *rays, int64_t 
     if (!c || !rays || !feats) return fail(c, GPIS_INVALID_ARGUMENT, "null argument");
     if (!c->P.one_d) return fail(c, GPIS_INVALID_ARGUMENT, "the medium is not 1D_sampling");
     if (n < 0 || n >= (1LL << 31)) return fail(c, GPIS_INVALID_ARGUMENT, "ray count out of range");
+    if (gpis_status s = mean_ready(c)) return s;
     if (n == 0) return GPIS_OK;
     hipLaunchKernelGGL(features1d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        c->P, make_in(rays), rays->length_scale, (int)n, feats);
@@ -1620,6 +1710,7 @@ gpis_status gpis_surrogate_flights_1d(gpis_ctx *c, gpis_net *net, const gpis_ray
     if (!rays || !reply || !reply->t || !reply->value || !reply->gx || !reply->gy || !reply->gz || !reply->outcome)
         return fail(c, GPIS_INVALID_ARGUMENT, "null ray or reply field");
     if (n < 0 || n >= (1LL << 31)) return fail(c, GPIS_INVALID_ARGUMENT, "ray count out of range");
+    if (gpis_status ms = mean_ready(c)) return ms;
     if (n_fallback) *n_fallback = 0;
     if (n == 0) return GPIS_OK;
     if (c->P.N != 8) return fail(c, GPIS_UNSUPPORTED, "1D flights are compiled for 8 impulses per cell");

@@ -1,5 +1,5 @@
 // gpis_mlp.hpp — the learned flight sampler on gfx950 BF16 MFMA: an MLP
-// 8 -> 128 -> 128 -> 128 -> 128 -> 32 (GELU) whose softmax is a piecewise-linear CDF over
+// 8 -> 128 -> 128 -> 128 -> 128 -> 32 (tanh-form GELU) whose softmax is a piecewise-linear CDF over
 // the normalised next-crossing distance (31 bins + "exit").  No counterpart in the
 // reference: it stands in for the Renewal flight sampler of
 // SparseConvolutionNoiseMedium.cpp:265-347 and is judged against it.
@@ -87,9 +87,19 @@ struct Calibration {
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU in its tanh form, 0.5 z (1 + tanh(u)) with u = sqrt(2/pi) (z + 0.044715 z^3),
+// written as z * sigmoid(2u): one exp2 and one reciprocal.
+constexpr float kGeluC1 = -2.0f * 0.79788456080286536f * 1.4426950408889634f;  // -2 sqrt(2/pi) log2(e)
+constexpr float kGeluC3 = kGeluC1 * 0.044715f;
+__device__ __forceinline__ float gelu_sig(float z) {
+    const float a = z * fmaf(kGeluC3, z * z, kGeluC1);
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(a));
+}
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sig(z); }
 __device__ __forceinline__ float gelu_grad(float z) {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
+    const float s = gelu_sig(z);
+    // d/dz z s(2u) = s + z s (1 - s) 2u',  2u' = 2 sqrt(2/pi) (1 + 3 * 0.044715 z^2)
+    return s + z * s * (1.0f - s) * (1.5957691216057308f * fmaf(0.134145f, z * z, 1.0f));
 }
 __device__ __forceinline__ unsigned lane64() { return __lane_id(); }
 

@@ -82,6 +82,16 @@ bool HipRayBatchBackend::setDomain(const float lo[3], const float hi[3], std::ui
     return gpis_set_domain(ctx_, lo, hi, seed) == GPIS_OK;
 }
 
+bool HipRayBatchBackend::setMeanGrid(const gpis_mean_grid &grid) {
+    if (!ok()) return false;
+    std::lock_guard<std::mutex> lk(deviceMutex_);
+    if (gpis_set_mean_grid(ctx_, &grid) != GPIS_OK) {
+        error_ = gpis_last_error(ctx_);
+        return false;
+    }
+    return true;
+}
+
 std::vector<RaySegmentReply> HipRayBatchBackend::process(const std::vector<RaySegmentRequest> &requests) {
     std::vector<RaySegmentReply> replies(requests.size());
     if (!ok()) return replies;  // all invalid: the caller traces them on the CPU

@@ -78,6 +78,9 @@ public:
 
     // Optional: precompute the packed impulse table of a single-realization medium.
     bool setDomain(const float lo[3], const float hi[3], std::uint32_t seed);
+    // Required for a tabulated mean (mean_kind GPIS_MEAN_GRID) before the first trace:
+    // the dense export of the mean's grid (the reference's makeHostMeanGrid output).
+    bool setMeanGrid(const gpis_mean_grid &grid);
 
     bool trace(const RaySegmentRequest &request, RaySegmentResult &result);
     std::future<std::vector<RaySegmentReply>> submitBatch(const std::vector<RaySegmentRequest> &requests,

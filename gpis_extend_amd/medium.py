@@ -32,6 +32,79 @@ _SCHEME = {"uni": A.SCHEME_UNI, "UNI": A.SCHEME_UNI, "nee": A.SCHEME_NEE, "NEE":
 
 
 @dataclass
+class DenseMeanGrid:
+    """A tabulated mean's voxel grid in the dense layout of the reference's GPU export
+    (HostMeanGrid, src/gpu/cuda/RaySegmentBatch.hpp:19-30, filled by makeHostMeanGrid,
+    src/gpu/VdbMeanGrid.cpp:9-32): ``voxels[z][y][x]`` hold the density at integer index
+    coordinates ``origin + (x, y, z)``; a world point p is looked up at the index point
+    ``world_to_index @ (p, 1)`` clamped to ``[clamp_min, clamp_max]``."""
+    voxels: np.ndarray
+    origin: tuple = (0, 0, 0)
+    world_to_index: tuple = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0)
+    clamp_min: tuple = (0.0, 0.0, 0.0)
+    clamp_max: tuple = (0.0, 0.0, 0.0)
+
+    def __post_init__(self):
+        self.voxels = np.ascontiguousarray(self.voxels, np.float32)
+        if self.voxels.ndim != 3 or min(self.voxels.shape) < 2:
+            raise MediumConfigError("a dense mean grid needs at least 2 voxels along each axis")
+        self.origin = tuple(int(v) for v in self.origin)
+        w = np.asarray(self.world_to_index, np.float64).reshape(-1)
+        if w.size == 16:
+            w = w[:12]
+        if w.size != 12 or not np.all(np.isfinite(w)):
+            raise MediumConfigError("world_to_index must be a finite 3x4 (or 4x4) row-major matrix")
+        self.world_to_index = tuple(float(v) for v in w)
+        self.clamp_min = tuple(float(v) for v in self.clamp_min)
+        self.clamp_max = tuple(float(v) for v in self.clamp_max)
+        if any(a > b for a, b in zip(self.clamp_min, self.clamp_max)):
+            raise MediumConfigError("empty clamp box")
+
+    @property
+    def dims(self) -> tuple:
+        return (self.voxels.shape[2], self.voxels.shape[1], self.voxels.shape[0])
+
+    @classmethod
+    def sample(cls, fn, lo, hi, spacing: float) -> "DenseMeanGrid":
+        """Tabulate fn(x, y, z) (vectorised, world units) on the lattice lo + spacing * i
+        covering [lo, hi]; lookups clamp to the lattice."""
+        lo = np.asarray(lo, np.float64)
+        n = np.floor((np.asarray(hi, np.float64) - lo) / spacing + 1e-9).astype(int) + 1
+        axes = [lo[i] + spacing * np.arange(n[i]) for i in range(3)]
+        z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+        vox = np.asarray(fn(x, y, z), np.float32)
+        inv = 1.0 / spacing
+        w2i = (inv, 0.0, 0.0, -lo[0] * inv, 0.0, inv, 0.0, -lo[1] * inv, 0.0, 0.0, inv, -lo[2] * inv)
+        return cls(vox, (0, 0, 0), w2i, (0.0, 0.0, 0.0), tuple(float(v - 1) for v in n))
+
+    @classmethod
+    def from_json(cls, js: dict, base_dir: Optional[str] = None) -> "DenseMeanGrid":
+        """``{"type": "dense", "file": "<voxels .npy, [z][y][x]>", "origin": [...],
+        "world_to_index": [12 numbers], "clamp_min": [...], "clamp_max": [...]}``."""
+        import os
+        path = js["file"]
+        if base_dir and not os.path.isabs(path):
+            path = os.path.join(base_dir, path)
+        vox = np.load(path, allow_pickle=False)
+        shape = vox.shape
+        return cls(vox, tuple(js.get("origin", (0, 0, 0))),
+                   tuple(js.get("world_to_index", cls.world_to_index)),
+                   tuple(js.get("clamp_min", (0.0, 0.0, 0.0))),
+                   tuple(js.get("clamp_max", (shape[2] - 1.0, shape[1] - 1.0, shape[0] - 1.0))))
+
+    def to_c(self, offset: float, scale: float, volume: bool) -> A.MeanGrid:
+        g = A.MeanGrid()
+        g.voxels = self.voxels.ctypes.data
+        for i in range(3):
+            g.origin[i], g.dims[i] = self.origin[i], self.dims[i]
+            g.clamp_min[i], g.clamp_max[i] = self.clamp_min[i], self.clamp_max[i]
+        for i in range(12):
+            g.world_to_index[i] = self.world_to_index[i]
+        g.offset, g.scale, g.volume = offset, scale, int(bool(volume))
+        return g
+
+
+@dataclass
 class SparseConvolutionNoiseMedium:
     """Parameters of one sparse_conv_noise medium.  Defaults are the reference
     constructor's (SparseConvolutionNoiseMedium.cpp:62-79, Medium.cpp:24,
@@ -65,10 +138,19 @@ class SparseConvolutionNoiseMedium:
     mean_half: tuple = (0.5, 0.5, 0.5)
     plane_normal: tuple = (1.0, 0.0, 0.0)
     plane_offset: float = 0.0
+    # TabulatedMean (GPFunctions.cpp:143-153): grid, offset, scale, volume
+    mean_grid: Optional[DenseMeanGrid] = None
+    mean_offset: float = 0.0
+    mean_scale: float = 1.0
+    mean_volume: bool = False
     extra: Dict = field(default_factory=dict)
 
     @classmethod
-    def from_json(cls, js: dict) -> "SparseConvolutionNoiseMedium":
+    def from_json(cls, js: dict, mean_grid: Optional[DenseMeanGrid] = None,
+                  base_dir: Optional[str] = None) -> "SparseConvolutionNoiseMedium":
+        """``mean_grid`` supplies a tabulated mean's voxels when the JSON names a grid the
+        host application reads itself (a "vdb" grid); "dense" grids are read from their
+        .npy file, relative to ``base_dir``."""
         if js.get("type", "sparse_conv_noise") != "sparse_conv_noise":
             raise MediumConfigError(f"not a sparse_conv_noise medium: {js.get('type')!r}")
         m = cls()
@@ -108,6 +190,20 @@ class SparseConvolutionNoiseMedium:
             m.mean_kind = A.MEAN_PLANE
             m.plane_normal = tuple(float(x) for x in mean.get("normal", (1.0, 0.0, 0.0)))
             m.plane_offset = float(mean.get("offset", 0.0))
+        elif mt == "tabulated":
+            m.mean_kind = A.MEAN_GRID
+            m.mean_offset = float(mean.get("offset", 0.0))
+            m.mean_scale = float(mean.get("scale", 1.0))
+            m.mean_volume = bool(mean.get("volume", False))
+            g = mean.get("grid", {})
+            if mean_grid is not None:
+                m.mean_grid = mean_grid
+            elif g.get("type") == "dense":
+                m.mean_grid = DenseMeanGrid.from_json(g, base_dir)
+            else:
+                raise MediumConfigError(
+                    f"tabulated mean grid of type {g.get('type')!r}: pass its dense export "
+                    "(origin, dims, voxels, world_to_index, clamp box) as mean_grid")
         else:
             raise MediumConfigError(f"mean type {mt!r} is not served by the GPU flight path")
         cov = gp.get("covariance", {"type": "squared_exponential"})
@@ -242,6 +338,11 @@ class GpisDevice:
         h = C.c_void_p()
         A.check(self.L.gpis_create(C.byref(self.desc), C.byref(h)), None, "gpis_create")
         self.h = h
+        if medium.mean_kind == A.MEAN_GRID:
+            if medium.mean_grid is None:
+                raise A.GpisError("tabulated mean without a grid")
+            g = medium.mean_grid.to_c(medium.mean_offset, medium.mean_scale, medium.mean_volume)
+            A.check(self.L.gpis_set_mean_grid(self.h, C.byref(g)), self.h, "gpis_set_mean_grid")
 
     def close(self):
         if getattr(self, "h", None):

@@ -11,7 +11,7 @@ import math
 import numpy as np
 
 from . import _capi as A
-from .medium import SparseConvolutionNoiseMedium
+from .medium import DenseMeanGrid, SparseConvolutionNoiseMedium
 
 SEED_POINTS = 0x5EED0000
 SEED_RAYS = 0x5EED0001
@@ -76,6 +76,28 @@ def config3_json():
         "isotropic_3D_sampling": False, "1D_sampling": True, "1D_sampling_scheme": "mis",
         "1D_gradient_correlationXY": False,
     }
+
+
+def with_tabulated_mean(js, offset=0.0, scale=1.0, volume=False):
+    """The medium `js` with its mean replaced by a TabulatedMean over a VDB grid, in the
+    reference's scene form (data/example-scenes/lattice-cubic-gpis/shaderball.json); the
+    grid's voxels come separately, as a DenseMeanGrid."""
+    import copy
+    out = copy.deepcopy(js)
+    out["gaussian_process"]["mean"] = {
+        "type": "tabulated", "volume": bool(volume), "scale": float(scale), "offset": float(offset),
+        "grid": {"type": "vdb", "file": "sdf.vdb", "density_name": "sdf", "interpolate": "linear"}}
+    return out
+
+
+def sphere_sdf_grid(radius=0.5, center=(0.0, 0.0, 0.0), half=1.06, spacing=0.02):
+    """Signed distance to a sphere tabulated on [-half, half]^3 with the given voxel spacing
+    (the dense export of an SDF grid like the reference's shaderball_sdf.vdb)."""
+    c = np.asarray(center, np.float64)
+
+    def sdf(x, y, z):
+        return np.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) - radius
+    return DenseMeanGrid.sample(sdf, (-half,) * 3, (half,) * 3, spacing)
 
 
 def points_workload(n=1 << 20, seed=SEED_POINTS):

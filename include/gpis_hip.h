@@ -34,8 +34,9 @@ typedef enum gpis_status {
 /* Covariance kinds: SquaredExponentialCovariance (GPFunctions.hpp:1880-1990) and
  * ProceduralNonstationaryCovariance (GPFunctions.hpp:2459-2505). */
 enum { GPIS_COV_SE = 0, GPIS_COV_PROC_NS = 1 };
-/* Mean kinds: SphericalMean (GPFunctions.hpp:952-996); box SDF and plane in code. */
-enum { GPIS_MEAN_SPHERE = 0, GPIS_MEAN_BOX = 1, GPIS_MEAN_PLANE = 2 };
+/* Mean kinds: SphericalMean (GPFunctions.hpp:952-996); box SDF and plane in code;
+ * TabulatedMean over a dense grid (GPFunctions.cpp:165-190), set by gpis_set_mean_grid. */
+enum { GPIS_MEAN_SPHERE = 0, GPIS_MEAN_BOX = 1, GPIS_MEAN_PLANE = 2, GPIS_MEAN_GRID = 3 };
 /* GPCorrelationContext (GaussianProcessMedium.cpp:20-42). */
 enum { GPIS_CTX_GLOBAL = 0, GPIS_CTX_RENEWAL = 1, GPIS_CTX_RENEWAL_PLUS = 2, GPIS_CTX_NONE = 3 };
 /* SparseConv1DSamplingScheme (Medium.hpp:42-46). */
@@ -170,6 +171,26 @@ gpis_status gpis_trace_flights_1d(gpis_ctx *ctx, const gpis_ray_soa *rays, int64
                                   const gpis_reply_soa *reply, const float *light,
                                   const gpis_nee_out *nee, void *stream);
 
+/* A tabulated mean on a dense voxel grid: the fields of the reference's HostMeanGrid
+ * (src/gpu/cuda/RaySegmentBatch.hpp:19-30; made from a VDB grid by makeHostMeanGrid,
+ * src/gpu/VdbMeanGrid.cpp:9-32).  density(p) is the trilinear interpolation of the
+ * voxels at q = world_to_index * p clamped to [clamp_min, clamp_max] (voxel indices
+ * clamped to the grid); mean(p) = (density + offset) * scale in double, and its gradient is
+ * TabulatedMean::dmean_da's forward difference with eps = 1e-3 (GPFunctions.cpp:178-190). */
+typedef struct gpis_mean_grid {
+    const float *voxels;          /* host memory: dims[0]*dims[1]*dims[2] values, x fastest */
+    int32_t origin[3];            /* index of voxels[0] */
+    int32_t dims[3];
+    float world_to_index[12];     /* row-major 3x4 affine map */
+    float clamp_min[3], clamp_max[3];
+    float offset, scale;
+    int32_t volume;               /* nonzero: density -> -log(max(1e-4, density)) first
+                                     (TabulatedMean's "volume" option) */
+} gpis_mean_grid;
+/* Copies the grid to the device; required (and only allowed) for media created with
+ * mean_kind = GPIS_MEAN_GRID.  Replaces TabulatedMean::loadResources (:192-194). */
+gpis_status gpis_set_mean_grid(gpis_ctx *ctx, const gpis_mean_grid *grid);
+
 /* Accumulated device counters (synchronises the context's stream). */
 gpis_status gpis_get_counters(gpis_ctx *ctx, gpis_counters *out);
 gpis_status gpis_reset_counters(gpis_ctx *ctx);
@@ -177,7 +198,8 @@ gpis_status gpis_reset_counters(gpis_ctx *ctx);
 /* Launch-shape control (the autotuner's knobs): waves per CU of the persistent
  * flight kernel, and rays per lane batch.  0 = built-in default. */
 gpis_status gpis_set_launch(gpis_ctx *ctx, int32_t waves_per_cu, int32_t block_size);
-/* Batching of the 3D persistent kernel's finish/bounce/refill step: a lane whose
+/* Batching of the persistent flight kernels' finish/bounce/refill step (defaults: 8 lanes
+ * for the 3D kernel, 16 for the 1D kernel; this call sets both): a lane whose
  * flight ended waits until `batch` lanes of its wave (1..64) can be finished, bounced
  * and refilled together, or until no lane of the wave is still marching.  1 (or 0)
  * serves each lane at once; 64 refills a wave only when all its lanes are done, so

@@ -5,7 +5,7 @@
  * media, and the plain method (gpis_trace_flights_1d) is both its training-data
  * source and its fallback.
  *
- * Network: 8 features -> 128 -> 128 -> 128 -> 128 -> 32 logits, GELU (erf form),
+ * Network: 8 features -> 128 -> 128 -> 128 -> 128 -> 32 logits, GELU (tanh form),
  * BF16 weights and activations, FP32 accumulation (v_mfma_f32_16x16x32_bf16), FP32
  * master weights and Adam state.  Softmax over 32 classes = a piecewise-linear CDF
  * over s = (t - near) / (far - near): classes 0..30 are bins [edges[b], edges[b+1])

@@ -402,10 +402,65 @@ static float cov1d_grad3n(const orc_field *f, float pq, float pc, v3f pqw, v3f p
     return se_cov1d_gg3(pq - pc, ns_cov_splat_cov1d(f, pqw, pcw));
 }
 
+/* ---- tabulated mean on a dense grid --------------------------------- */
+/* voxel(): index relative to the grid origin, clamped to the grid (RaySegmentBatch.cu:64-71) */
+static float grid_voxel(const orc_medium_desc *d, int x, int y, int z) {
+    x -= d->grid_origin[0]; y -= d->grid_origin[1]; z -= d->grid_origin[2];
+    x = x < 0 ? 0 : (x > d->grid_dims[0] - 1 ? d->grid_dims[0] - 1 : x);
+    y = y < 0 ? 0 : (y > d->grid_dims[1] - 1 ? d->grid_dims[1] - 1 : y);
+    z = z < 0 ? 0 : (z > d->grid_dims[2] - 1 ? d->grid_dims[2] - 1 : z);
+    return d->grid[((size_t)z * (size_t)d->grid_dims[1] + (size_t)y) * (size_t)d->grid_dims[0] + (size_t)x];
+}
+static float fclampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+/* meanValue()'s density: world -> index affine map, clamp to [clampMin, clampMax] (the
+ * VdbGrid::density clamp to bounds+2 / bounds-3, VdbGrid.cpp:413 and :458-459), then
+ * trilinear interpolation along x, then y, then z (RaySegmentBatch.cu:73-95). */
+float orc_grid_density(const orc_medium_desc *d, const float p[3]) {
+    const float *m = d->grid_w2i;
+    float q[3];
+    for (int r = 0; r < 3; ++r) q[r] = m[4 * r] * p[0] + m[4 * r + 1] * p[1] + m[4 * r + 2] * p[2] + m[4 * r + 3];
+    for (int i = 0; i < 3; ++i) q[i] = fclampf(q[i], d->grid_clamp_min[i], d->grid_clamp_max[i]);
+    int x = (int)floorf(q[0]), y = (int)floorf(q[1]), z = (int)floorf(q[2]);
+    float fx = q[0] - (float)x, fy = q[1] - (float)y, fz = q[2] - (float)z;
+    float layer[2];
+    for (int dz = 0; dz < 2; ++dz) {
+        float row[2];
+        for (int dy = 0; dy < 2; ++dy) {
+            float a = grid_voxel(d, x, y + dy, z + dz);
+            float b = grid_voxel(d, x + 1, y + dy, z + dz);
+            row[dy] = a + fx * (b - a);
+        }
+        layer[dz] = row[0] + fy * (row[1] - row[0]);
+    }
+    return layer[0] + fz * (layer[1] - layer[0]);
+}
+/* TabulatedMean::mean (GPFunctions.cpp:165-172): the point rounded to float, the density
+ * widened to double, the "volume" option's -log(max(1e-4, .)), then (res + offset) * scale. */
+double orc_grid_mean(const orc_medium_desc *d, const double a[3]) {
+    float p[3] = {(float)a[0], (float)a[1], (float)a[2]};
+    double res = (double)orc_grid_density(d, p);
+    if (d->grid_volume) res = -log(res > 0.0001 ? res : 0.0001);
+    return (res + (double)d->grid_offset) * (double)d->grid_scale;
+}
+/* TabulatedMean::dmean_da (:179-190): forward differences with eps = 0.001 in double. */
+void orc_grid_mean_grad(const orc_medium_desc *d, const double a[3], double g[3]) {
+    const double eps = 0.001;
+    double c = orc_grid_mean(d, a);
+    for (int i = 0; i < 3; ++i) {
+        double b[3] = {a[0], a[1], a[2]};
+        b[i] = a[i] + eps;
+        g[i] = (orc_grid_mean(d, b) - c) / eps;
+    }
+}
+
 /* ---- means (GPFunctions.hpp:952-996 and code-defined box/plane) ----- */
 static double mean_value(const orc_field *f, v3d p) {
     const orc_medium_desc *d = &f->d;
     switch (d->mean_kind) {
+    case ORC_MEAN_GRID: {
+        double a[3] = {p.x, p.y, p.z};
+        return orc_grid_mean(d, a);
+    }
     case ORC_MEAN_BOX: {
         double q[3], o2 = 0.0, mx = -1e300;
         double pp[3] = {p.x - d->mean_center[0], p.y - d->mean_center[1], p.z - d->mean_center[2]};
@@ -428,6 +483,11 @@ static double mean_value(const orc_field *f, v3d p) {
 static v3d mean_grad(const orc_field *f, v3d p) {
     const orc_medium_desc *d = &f->d;
     switch (d->mean_kind) {
+    case ORC_MEAN_GRID: {
+        double a[3] = {p.x, p.y, p.z}, g[3];
+        orc_grid_mean_grad(d, a, g);
+        return D3(g[0], g[1], g[2]);
+    }
     case ORC_MEAN_BOX: {
         double pp[3] = {p.x - d->mean_center[0], p.y - d->mean_center[1], p.z - d->mean_center[2]};
         double q[3], g[3] = {0, 0, 0}, o2 = 0.0;

@@ -44,7 +44,7 @@ void orc_tangent_frame(const float n[3], float out9[9]); /* tangent, bitangent, 
 
 /* ---- medium description --------------------------------------------- */
 enum { ORC_COV_SE = 0, ORC_COV_PROC_NS = 1 };
-enum { ORC_MEAN_SPHERE = 0, ORC_MEAN_BOX = 1, ORC_MEAN_PLANE = 2 };
+enum { ORC_MEAN_SPHERE = 0, ORC_MEAN_BOX = 1, ORC_MEAN_PLANE = 2, ORC_MEAN_GRID = 3 };
 enum { ORC_CTX_GLOBAL = 0, ORC_CTX_RENEWAL = 1, ORC_CTX_RENEWAL_PLUS = 2, ORC_CTX_NONE = 3 };
 enum { ORC_SCHEME_UNI = 0, ORC_SCHEME_NEE = 1, ORC_SCHEME_MIS = 2 };
 enum { ORC_LS_CONST = 0, ORC_LS_SINCOS = 1 };
@@ -78,7 +78,21 @@ typedef struct orc_medium_desc {
     float step_size;
     uint32_t min_step;
     int32_t max_bounce;
+    /* TabulatedMean over a dense voxel grid (the HostMeanGrid fields of
+     * src/gpu/cuda/RaySegmentBatch.hpp:19-30); the voxels stay owned by the caller. */
+    const float *grid;          /* x fastest */
+    int32_t grid_origin[3], grid_dims[3];
+    float grid_w2i[12];         /* row-major 3x4 world -> index */
+    float grid_clamp_min[3], grid_clamp_max[3];
+    float grid_offset, grid_scale;
+    int32_t grid_volume;
 } orc_medium_desc;
+
+/* Dense-grid density at a world point (trilinear, clamped), and TabulatedMean::mean
+ * and dmean_da at double points (GPFunctions.cpp:165-190). */
+float orc_grid_density(const orc_medium_desc *d, const float p[3]);
+double orc_grid_mean(const orc_medium_desc *d, const double a[3]);
+void orc_grid_mean_grad(const orc_medium_desc *d, const double a[3], double g[3]);
 
 /* One flight request: the fields of GpRaySegmentRequest plus MediumState. */
 typedef struct orc_ray {

@@ -4,7 +4,7 @@ bench.py's CPU-baseline leg import it; the product path never does).
 The surrogate has no counterpart in the reference.  Its specification is the
 flight sampler it stands in for: a multilayer perceptron
 
-    8 features -> 128 -> 128 -> 128 -> 128 -> 32 logits, GELU (erf form) between layers,
+    8 features -> 128 -> 128 -> 128 -> 128 -> 32 logits, GELU (tanh form) between layers,
 
 whose softmax over 32 classes is a piecewise-linear CDF over the normalised
 next-crossing distance s = (t - near) / (far - near): classes 0..30 are distance
@@ -23,11 +23,6 @@ from __future__ import annotations
 import math
 
 import numpy as np
-
-try:
-    from scipy.special import erf as _erf
-except Exception:  # pragma: no cover - scipy is part of the test image
-    _erf = np.vectorize(math.erf)
 
 N_IN = 8
 WIDTH = 128
@@ -76,12 +71,24 @@ def init_params(seed=11):
     return theta
 
 
+_K = math.sqrt(2.0 / math.pi)
+
+
+def _sig2u(z):
+    """sigmoid(2u) = 0.5 (1 + tanh(u)), u = sqrt(2/pi) (z + 0.044715 z^3)."""
+    z = np.asarray(z, np.float64)
+    with np.errstate(over="ignore"):
+        return 1.0 / (1.0 + np.exp(-2.0 * _K * (z + 0.044715 * z ** 3)))
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + _erf(z / math.sqrt(2.0)))
+    """GELU in its tanh form (the network's activation)."""
+    return z * _sig2u(z)
 
 
 def gelu_grad(z):
-    return 0.5 * (1.0 + _erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2.0 * math.pi)
+    s = _sig2u(z)
+    return s + z * s * (1.0 - s) * 2.0 * _K * (1.0 + 3.0 * 0.044715 * np.asarray(z, np.float64) ** 2)
 
 
 def _mm(a, b, exact):

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "build", "libgpis_oracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libgpis_ref_leaf.so")
 
 COV_SE, COV_PROC_NS = 0, 1
-MEAN_SPHERE, MEAN_BOX, MEAN_PLANE = 0, 1, 2
+MEAN_SPHERE, MEAN_BOX, MEAN_PLANE, MEAN_GRID = 0, 1, 2, 3
 CTX_GLOBAL, CTX_RENEWAL, CTX_RENEWAL_PLUS, CTX_NONE = 0, 1, 2, 3
 SCHEME_UNI, SCHEME_NEE, SCHEME_MIS = 0, 1, 2
 LS_CONST, LS_SINCOS = 0, 1
@@ -34,6 +34,9 @@ class MediumDesc(C.Structure):
         ("global_seed", C.c_uint32), ("impulse_density", C.c_float), ("single_realization", C.c_int32),
         ("ctxt", C.c_int32), ("one_d", C.c_int32), ("scheme", C.c_int32), ("correlation_xy", C.c_int32),
         ("step_size", C.c_float), ("min_step", C.c_uint32), ("max_bounce", C.c_int32),
+        ("grid", C.c_void_p), ("grid_origin", C.c_int32 * 3), ("grid_dims", C.c_int32 * 3),
+        ("grid_w2i", C.c_float * 12), ("grid_clamp_min", C.c_float * 3), ("grid_clamp_max", C.c_float * 3),
+        ("grid_offset", C.c_float), ("grid_scale", C.c_float), ("grid_volume", C.c_int32),
     ]
 
 
@@ -83,6 +86,11 @@ def lib():
         L.orc_cell3d_impulses.argtypes = [u32, u32, u32, u32, u32, f32p]
         L.orc_cell1d_impulses.argtypes = [u32, u32, u32, f32p]
         L.orc_standard_normals.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_double)]
+        L.orc_grid_density.restype = C.c_float
+        L.orc_grid_density.argtypes = [vp, f32p]
+        L.orc_grid_mean.restype = C.c_double
+        L.orc_grid_mean.argtypes = [vp, vp]
+        L.orc_grid_mean_grad.argtypes = [vp, vp, vp]
         L.orc_tangent_frame.argtypes = [f32p, f32p]
         L.orc_field_create.restype = vp; L.orc_field_create.argtypes = [C.POINTER(MediumDesc)]
         L.orc_field_destroy.argtypes = [vp]
@@ -196,7 +204,51 @@ def make_desc(**kw) -> MediumDesc:
     d.step_size = kw.get("step_size", 0.01)
     d.min_step = kw.get("min_step", 8)
     d.max_bounce = kw.get("max_bounce", 1 << 30)
+    g = kw.get("grid")
+    if g is not None:
+        set_grid(d, g.voxels, g.origin, g.world_to_index, g.clamp_min, g.clamp_max,
+                 kw.get("grid_offset", 0.0), kw.get("grid_scale", 1.0), kw.get("grid_volume", False))
     return d
+
+
+def set_grid(d: MediumDesc, voxels, origin, world_to_index, clamp_min, clamp_max, offset=0.0, scale=1.0,
+             volume=False):
+    """Attach a dense tabulated-mean grid (voxels indexed [z][y][x]) to a description;
+    the description keeps the array alive."""
+    v = np.ascontiguousarray(voxels, np.float32)
+    assert v.ndim == 3
+    d._grid_keep = v
+    d.grid = v.ctypes.data
+    for i in range(3):
+        d.grid_origin[i] = int(origin[i])
+        d.grid_dims[i] = v.shape[2 - i]
+        d.grid_clamp_min[i] = clamp_min[i]
+        d.grid_clamp_max[i] = clamp_max[i]
+    w = np.asarray(world_to_index, np.float32).reshape(12)
+    for i in range(12):
+        d.grid_w2i[i] = w[i]
+    d.grid_offset, d.grid_scale, d.grid_volume = offset, scale, int(bool(volume))
+    return d
+
+
+def grid_density(d: MediumDesc, p) -> np.ndarray:
+    p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+    return np.array([lib().orc_grid_density(C.byref(d), _fp(p[i].copy())) for i in range(len(p))], np.float32)
+
+
+def grid_mean(d: MediumDesc, a) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.float64).reshape(-1, 3)
+    return np.array([lib().orc_grid_mean(C.byref(d), a[i].ctypes.data) for i in range(len(a))])
+
+
+def grid_mean_grad(d: MediumDesc, a) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.float64).reshape(-1, 3)
+    g = np.zeros((len(a), 3))
+    for i in range(len(a)):
+        row = np.zeros(3)
+        lib().orc_grid_mean_grad(C.byref(d), a[i].ctypes.data, row.ctypes.data)
+        g[i] = row
+    return g
 
 
 class Field:

@@ -17,7 +17,8 @@ def oracle_desc(med):
         impulse_density=med.impulse_density, single_realization=int(med.single_realization),
         ctxt=med.correlation_context, one_d=int(med.one_d), scheme=med.scheme,
         correlation_xy=int(med.correlation_xy), step_size=med.step_size, min_step=med.min_step,
-        max_bounce=med.max_bounces)
+        max_bounce=med.max_bounces, grid=med.mean_grid, grid_offset=med.mean_offset, grid_scale=med.mean_scale,
+        grid_volume=med.mean_volume)
     return d
 
 

@@ -18,14 +18,14 @@ pytestmark = pytest.mark.gpu
 NTH = min(16, os.cpu_count() or 1)
 
 
-def _medium(js):
+def _medium(js, grid=None):
     from gpis_extend_amd import SparseConvolutionNoiseMedium
-    return SparseConvolutionNoiseMedium.from_json(js)
+    return SparseConvolutionNoiseMedium.from_json(js, mean_grid=grid)
 
 
-def _run(js, rays, light):
+def _run(js, rays, light, grid=None):
     from gpis_extend_amd import GpisDevice, RayBatch
-    med = _medium(js)
+    med = _medium(js, grid)
     dev = GpisDevice(med)
     rep = dev.trace_flights_1d(RayBatch.from_numpy(rays), light=light)
     torch.cuda.synchronize()
@@ -53,10 +53,14 @@ def _fragile_fn(desc, orays, ls, ref, rel_tol=1e-4, trials=16):
 
 
 def _grad_point(rays, t):
-    """ro + normalize(dir in double) * t, rounded to float (the gradient position)."""
+    """ro + normalize(dir in double) * t, rounded to float (the gradient position), with
+    the normalisation as a multiplication by 1/|d|: the same point to the last bit, which
+    a tabulated mean's forward-difference gradient needs (its float rounding noise is
+    ~1e-4 of |grad| between neighbouring points)."""
     o = np.stack([rays["ox"], rays["oy"], rays["oz"]], 1).astype(np.float64)
     d = np.stack([rays["dx"], rays["dy"], rays["dz"]], 1).astype(np.float64)
-    d /= np.sqrt((d * d).sum(1, keepdims=True))
+    l2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    d = d * (1.0 / np.sqrt(l2))[:, None]
     return (o + d * t.astype(np.float64)[:, None]).astype(np.float32)
 
 
@@ -82,8 +86,8 @@ def _mirror_normal(rays, p, light):
     return _normalize_f32((wl - d).astype(np.float32))
 
 
-def _check_1d(js, rays, light, name, min_agree=0.9999):
-    med, got, counters = _run(js, rays, light)
+def _check_1d(js, rays, light, name, min_agree=0.9999, grid=None):
+    med, got, counters = _run(js, rays, light, grid)
     desc = oracle_desc(med)
     orays = oracle_rays(rays, med)
     ls = rays.get("length_scale")
@@ -109,7 +113,9 @@ def _check_1d(js, rays, light, name, min_agree=0.9999):
     # hit gradients and exit gradients: the oracle's conditioned realization at the
     # point the HIP path evaluated (same t), t_segment = t
     sel = np.nonzero(agree & ((oc == po.HIT) | (oc == po.EXIT)))[0]
-    q = _grad_point(rays, got["t"])[sel]
+    # the point from the oracle's double t: the HIP path forms it from its own double t,
+    # which the float t in the reply only rounds
+    q = _grad_point(rays, ref["t"])[sel]
     sub = orays[sel]
     ref4 = po.conditioned_eval_ls(desc, sub, None if ls is None else ls[sel], q, got["t"][sel].astype(np.float32))
     g_gpu = np.stack([got["gx"], got["gy"], got["gz"]], 1)[sel].astype(np.float64)
