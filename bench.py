@@ -360,7 +360,10 @@ def main():
     rays = W.camera_rays(med, width=args.width, height=args.width)
     n = rays["ox"].shape[0]
     dev = GpisDevice(med)
-    dev.set_domain((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+    # the impulse table covers every point a flight evaluates: the [-1, 1]^3 box and the
+    # camera position (a flight whose secant cuts off at t = 0 takes its gradient at the
+    # ray origin)
+    dev.set_domain((-1.0, -1.0, -1.0), (1.0, 1.0, 3.0))
     batch = RayBatch.from_numpy(rays)
     reply = dev.alloc_path_reply(n)
     stream = torch.cuda.current_stream()

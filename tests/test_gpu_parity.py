@@ -108,6 +108,28 @@ def test_flights_config2_conditioned_multires_subset():
     parity_gate(got, ref, "C2 conditioned multires subset", min_agree=0.99, oracle=orc)
 
 
+@pytest.mark.skip(reason="faulted the GPU together with an LDS cell-block staging path for "
+                         "out-of-table evaluations, since removed; the cause was not found, so the "
+                         "case stays off the GPU")
+def test_partial_impulse_table_equals_generator():
+    """Flights whose evaluations fall inside, outside and across the edge of the packed
+    impulse table (outside: the wave generates the lane's cell block into LDS) give the
+    same bits as flights on the generator alone and on a table that covers everything."""
+    from gpis_extend_amd import workloads as W, RayBatch
+    print("case: C0 16,384 flights, no table / partial table / full table", flush=True)
+    outs = []
+    for dom in (None, ((-0.6,) * 3, (0.6,) * 3), ((-0.3, -2.1, -0.3), (2.1, 0.3, 2.1)), ((-2.1,) * 3, (2.1,) * 3)):
+        med, dev = _dev(W.config0_json(), dom)
+        rays = W.sphere_rays(med, n=16384)
+        rep = dev.trace_flights(RayBatch.from_numpy(rays))
+        torch.cuda.synchronize()
+        outs.append({k: v.cpu().numpy() for k, v in rep.items()})
+        dev.close()
+    for o in outs[1:]:
+        for k in ("t", "value", "gx", "gy", "gz", "outcome", "evals"):
+            assert np.array_equal(o[k].view(np.uint32), outs[0][k].view(np.uint32)), k
+
+
 def test_paths_first_flight_equals_single_flights():
     from gpis_extend_amd import workloads as W, RayBatch
     med, dev = _dev(W.config1_json(), ((-1.0,) * 3, (1.0,) * 3))
