@@ -8,10 +8,14 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(PKG, "csrc", "gpis_hip.hip")
 OUT = os.path.join(PKG, "libgpis_hip.so")
 
-# Packed FP32 forms are disabled for every HIP object of this project.
+# Packed FP32 forms are disabled for every HIP object of this project.  The two-address
+# v_fmac_f64 form is disabled too: with it the compiler copies each loop-invariant FP64
+# coefficient into the accumulator before every fused multiply-add (a v_mov_b64 per
+# Horner step of the splat exponential); v_fma_f64 takes the coefficient as a third source.
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Xclang", "-target-feature", "-Xclang", "-fmacf64-inst",
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Wno-pass-failed",
 ]
