@@ -253,18 +253,45 @@ __device__ __forceinline__ void cond_splat1d(const Ray1D &R, float u, float &v, 
     dv += vd + gd;
 }
 
-// Box-Muller pair m (0 or 1) after set_state(normal_seed), in FP64 (Gaussian.cpp:22-34),
-// divided by sqrt(2.f) as evaluateGradientNoise1D does.
-__device__ __forceinline__ void transverse_pair(uint32_t normal_seed, int m, double &x, double &y) {
+// Box-Muller pair m (0 or 1) after set_state(normal_seed), in FP64 (Gaussian.cpp:22-34).
+__device__ __forceinline__ void normal_pair(uint32_t normal_seed, int m, double &x, double &y) {
     Pcg g;
     g.set((uint64_t)normal_seed);
     double u1 = u01(g.next()), u2 = u01(g.next());
     if (m == 1) { u1 = u01(g.next()); u2 = u01(g.next()); }
     double r = sqrt(-2.0 * log(1.0 - u1));
     double a = (double)(2.0f * kPiF) * u2;
+    x = r * cos(a);
+    y = r * sin(a);
+}
+// The same pair divided by sqrt(2.f) as evaluateGradientNoise1D does (:204-222).
+__device__ __forceinline__ void transverse_pair(uint32_t normal_seed, int m, double &x, double &y) {
+    normal_pair(normal_seed, m, x, y);
     const double s2 = (double)sqrtf(2.0f);
-    x = r * cos(a) / s2;
-    y = r * sin(a) / s2;
+    x = x / s2;
+    y = y / s2;
+}
+
+// seTransverseGradientMoments (SparseConvolutionNoise.cpp:78-99) for a 1D SE Renewal+
+// medium with 1D_gradient_correlationXY: the transverse gradient at p is Gaussian with
+// mean rho * (observed transverse gradient at the origin) and variance
+// -expm1(-q2) / 2, q2 = |T (p - o)|^2 / 2; without a gradient observation mean 0 and
+// variance 1/2.  obs_x, obs_y: frame.toLocal(localToWorld^T g_obs) of the conditioning.
+__device__ __forceinline__ void transverse_moments(const Ray1D &R, bool observed, float obs_x, float obs_y, float ox,
+                                                   float oy, float oz, float px, float py, float pz, double &mx,
+                                                   double &my, double &var) {
+#pragma clang fp contract(off)
+    mx = 0.0; my = 0.0; var = 0.5;
+    if (!observed) return;
+    const float ex = R.ia[0] * (px - ox), ey = R.ia[1] * (py - oy), ez = R.ia[2] * (pz - oz);
+    double l2 = (double)ex * (double)ex;
+    l2 += (double)ey * (double)ey;
+    l2 += (double)ez * (double)ez;
+    const double q2 = 0.5 * l2;
+    const double rho = exp(-0.5 * q2);
+    var = -0.5 * expm1(-q2);
+    mx = rho * (double)obs_x;
+    my = rho * (double)obs_y;
 }
 
 // World gradient of the noise from (transverse x, y; longitudinal dv) in isotropic

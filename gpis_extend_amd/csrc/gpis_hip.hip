@@ -69,8 +69,6 @@ static gpis_status build_params(gpis_ctx *c) {
     if (d.cov_kind != GPIS_COV_SE && d.cov_kind != GPIS_COV_PROC_NS) return fail(c, GPIS_UNSUPPORTED, "covariance kind");
     if (d.one_d && d.cov_kind != GPIS_COV_SE)
         return fail(c, GPIS_UNSUPPORTED, "1D sampling is served for the stationary SE kernel");
-    if (d.one_d && d.correlation_xy)
-        return fail(c, GPIS_UNSUPPORTED, "1D_gradient_correlationXY (Renewal+ transverse law) is not served");
     if (d.mean_kind < GPIS_MEAN_SPHERE || d.mean_kind > GPIS_MEAN_GRID) return fail(c, GPIS_UNSUPPORTED, "mean kind");
     P.kind = d.cov_kind;
     P.multires = (d.cov_kind == GPIS_COV_PROC_NS) && d.ns_multires;
@@ -694,11 +692,12 @@ __device__ __forceinline__ void eval1d(const FieldParams &P, const Ray1D &R, Cel
 }
 
 // neeShared + neeGrad + neePDF (SparseConvolutionNoise.cpp:662-821) for the single-level
-// stationary kernel without the transverse law: the mirror normal `h`, the hit point p,
-// the conditioned derivative along the ray `dvp`; float arithmetic as the reference.
+// stationary kernel: the mirror normal `h`, the hit point p, the conditioned derivative
+// along the ray `dvp`; with `tr` the transverse density is the correlated law of
+// seTransverseGradientMoments (tmx, tmy, tvar).  Float arithmetic as the reference.
 __device__ __forceinline__ void nee1d(const FieldParams &P, const Ray1D &R, float hx, float hy, float hz, float px,
-                                   float py, float pz, float dvp, float dx, float dy, float dz, float &gx, float &gy,
-                                   float &gz, float &pdf) {
+                                   float py, float pz, float dvp, float dx, float dy, float dz, bool tr, double tmx,
+                                   double tmy, double tvar, float &gx, float &gy, float &gz, float &pdf) {
 #pragma clang fp contract(off)
     float mgx, mgy, mgz;
     mean_grad_d(P, px, py, pz, mgx, mgy, mgz);
@@ -742,11 +741,19 @@ __device__ __forceinline__ void nee1d(const FieldParams &P, const Ray1D &R, floa
     g3x += I[0][2] * rz;
     float g3y = I[1][0] * rx + I[1][1] * ry;
     g3y += I[1][2] * rz;
-    // transverse draw density with variance 1/2 per axis (no correlation with the origin)
-    float smx = (float)((double)g3x * 1.4142135623730951 * 1.0);
-    float smy = (float)((double)g3y * 1.4142135623730951 * 1.0);
-    float pdf_area = (float)(exp((double)(-smx * smx) / 2.0) * exp((double)(-smy * smy) / 2.0) / (2.0 * M_PI) * 2.0 *
-                             1.0 * 1.0);
+    float pdf_area;
+    if (tr) {
+        // the correlated transverse law (neePDF :742-752): a delta at the origin
+        if (!(tvar > 0.0)) { pdf = 0.0f; return; }
+        const double ex = (double)g3x - tmx, ey = (double)g3y - tmy;
+        pdf_area = (float)(exp(-0.5 * (ex * ex + ey * ey) / tvar) / (2.0 * M_PI * tvar));
+    } else {
+        // transverse draw density with variance 1/2 per axis (no correlation with the origin)
+        float smx = (float)((double)g3x * 1.4142135623730951 * 1.0);
+        float smy = (float)((double)g3y * 1.4142135623730951 * 1.0);
+        pdf_area = (float)(exp((double)(-smx * smx) / 2.0) * exp((double)(-smy * smy) / 2.0) / (2.0 * M_PI) * 2.0 *
+                           1.0 * 1.0);
+    }
     float cosl = fabsf(dot3(hx, hy, hz, nx, ny, nz));
     float pdf_n = pdf_area * (isect * isect) / cosl / stretch;
     pdf = (float)(pdf_n / (4.0 * (double)dot3(hx, hy, hz, -dx, -dy, -dz)));
@@ -779,7 +786,10 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
     unsigned evals_flight = 0;
     float gxv = 0, gyv = 0, gzv = 0;
     uint32_t outcome = 0;
-    float bx1 = 0.f, by1 = 0.f;      // Box-Muller pair 0 / sqrt 2 (Renewal+ conditioning)
+    float bx1 = 0.f, by1 = 0.f;      // transverse gradient of the unconditioned field at the origin
+    float obs_x = 0.f, obs_y = 0.f;  // observed transverse gradient at the origin (correlated law)
+    bool observed = false;           // coeff_1D.has_gradient_observation
+    const bool corr = P.corr_xy && P.ctxt == GPIS_CTX_RENEWAL_PLUS;  // seTransverseGradientMoments applies
     float ndv_end = 0.f;             // derivative along the ray at the endpoint
     float ndv_nee = 0.f;             // ... and at the NEE point
     int nee_stage = 0;               // 1 once the NEE evaluation is done
@@ -797,10 +807,23 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
             if (phase == PH_DONE && nee_stage == 0) {
                 // evaluateGradientNoise1D at the endpoint (:204-242): transverse pair 0 when
                 // tSegment == 0, otherwise an independent second pair; resumeDistance rules
-                double x2, y2;
-                transverse_pair(normal_seed, t_end == 0.0f ? 0 : 1, x2, y2);
+                float tx2, ty2;
+                if (corr) {
+                    // the correlated law: pair 0 about the conditional mean (:186-196)
+                    double mx, my, var, z0, z1;
+                    transverse_moments(R, observed, obs_x, obs_y, ox, oy, oz, qx, qy, qz, mx, my, var);
+                    normal_pair(normal_seed, 0, z0, z1);
+                    const double sd = sqrt(var);
+                    tx2 = (float)(mx + sd * z0);
+                    ty2 = (float)(my + sd * z1);
+                } else {
+                    double x2, y2;
+                    transverse_pair(normal_seed, t_end == 0.0f ? 0 : 1, x2, y2);
+                    tx2 = (float)x2;
+                    ty2 = (float)y2;
+                }
                 float wx, wy, wz;
-                ray_grad_to_world(R, (float)x2, (float)y2, ndv_end, wx, wy, wz);
+                ray_grad_to_world(R, tx2, ty2, ndv_end, wx, wy, wz);
                 float mgx, mgy, mgz;
                 mean_grad_d(P, qx, qy, qz, mgx, mgy, mgz);
                 const float amp = P.amp_const;
@@ -841,8 +864,10 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
                 normalize3(wlx, wly, wlz);
                 float hx = wlx - dx, hy = wly - dy, hz = wlz - dz;
                 normalize3(hx, hy, hz);
+                double tmx = 0.0, tmy = 0.0, tvar = 0.5;
+                if (corr) transverse_moments(R, observed, obs_x, obs_y, ox, oy, oz, qx, qy, qz, tmx, tmy, tvar);
                 float ngx, ngy, ngz, pdf;
-                nee1d(P, R, hx, hy, hz, qx, qy, qz, ndv_nee, dx, dy, dz, ngx, ngy, ngz, pdf);
+                nee1d(P, R, hx, hy, hz, qx, qy, qz, ndv_nee, dx, dy, dz, corr, tmx, tmy, tvar, ngx, ngy, ngz, pdf);
                 nee.gx[ray] = ngx; nee.gy[ray] = ngy; nee.gz[ray] = ngz; nee.pdf[ray] = pdf;
                 nee_stage = 2;
             }
@@ -883,6 +908,7 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
                         if (!first) { prev_v = in.pv[r]; prev_gx = in.pgx[r]; prev_gy = in.pgy[r]; prev_gz = in.pgz[r]; }
                         evals_flight = 0;
                         nee_stage = 0;
+                        observed = false;
                         cache_valid = false;
                         ray_scales(P.aniso, P.l_aniso, P.l_aniso_inv, lscale, r, R.la, R.ia);
                         float ix = R.ia[0] * dx, iy = R.ia[1] * dy, iz = R.ia[2] * dz;
@@ -907,7 +933,14 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
                             if (P.activate && !first) {
                                 phase = PH_COND;
                                 qx = ox; qy = oy; qz = oz;
-                                if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
+                                if (corr) {
+                                    // evaluateGradientNoise1D(conditioning = false): mean 0, variance 1/2
+                                    double z0, z1;
+                                    normal_pair(normal_seed, 0, z0, z1);
+                                    const double sd = sqrt(0.5);
+                                    bx1 = (float)(0.0 + sd * z0);
+                                    by1 = (float)(0.0 + sd * z1);
+                                } else if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
                                     double x1, y1;
                                     transverse_pair(normal_seed, 0, x1, y1);
                                     bx1 = (float)x1; by1 = (float)y1;
@@ -945,13 +978,20 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
                     mean_grad_d(P, ox, oy, oz, mgx, mgy, mgz);
                     float cgx, cgy, cgz;
                     ray_grad_to_world(R, bx1, by1, ndv, cgx, cgy, cgz);
-                    float ddx = (prev_gx - mgx) / amp - cgx;
-                    float ddy = (prev_gy - mgy) / amp - cgy;
-                    float ddz = (prev_gz - mgz) / amp - cgz;
-                    // gradient_scale = -2 * frame.toLocal(l_aniso * delta) * scale^2; only z
-                    // survives without the xy correlation
+                    const float tgx = (prev_gx - mgx) / amp, tgy = (prev_gy - mgy) / amp, tgz = (prev_gz - mgz) / amp;
+                    float ddx = tgx - cgx;
+                    float ddy = tgy - cgy;
+                    float ddz = tgz - cgz;
+                    // gradient_scale = -2 * frame.toLocal(l_aniso * delta) * scale^2; only its
+                    // z component enters the value and the derivative along the ray (the
+                    // transverse components are the correlated law's, through the moments)
                     float gsz = dot3(R.F.nx, R.F.ny, R.F.nz, R.la[0] * ddx, R.la[1] * ddy, R.la[2] * ddz);
                     R.g1z = -2.f * gsz * 1.0f;
+                    // target_gradient_noise in isotropic ray space (seTransverseGradientMoments :94-97)
+                    const float gwx = R.la[0] * tgx, gwy = R.la[1] * tgy, gwz = R.la[2] * tgz;
+                    obs_x = dot3(R.F.tx, R.F.ty, R.F.tz, gwx, gwy, gwz);
+                    obs_y = dot3(R.F.bx, R.F.by, R.F.bz, gwx, gwy, gwz);
+                    observed = true;
                 }
             }
             R.cond = 1;

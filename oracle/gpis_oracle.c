@@ -1075,6 +1075,20 @@ void orc_conditioned_eval(const orc_field *f, const orc_ray *ray, const float q[
     out4[1] = g.x; out4[2] = g.y; out4[3] = g.z;
 }
 
+/* seTransverseGradientMoments and samplingScheme at p for the flight's realization
+ * (conditioned at the ray origin as prepare() does); returns 1 when the correlated
+ * transverse law applies (SparseConvolutionNoise.cpp:68-99). */
+int orc_transverse_moments(const orc_field *f, const orc_ray *ray, const float p_in[3], double mean_out[2],
+                           double *var_out, int *scheme_out) {
+    real_t r;
+    prepare(&r, f, ray);
+    v3f dir = V3(ray->dir[0], ray->dir[1], ray->dir[2]);
+    v3f p = V3(p_in[0], p_in[1], p_in[2]);
+    int ok = transverse_moments(&r, p, dir, 1, mean_out, var_out);
+    *scheme_out = sampling_scheme(&r, p);
+    return ok;
+}
+
 /* neeShared / neeGrad / neePDF (SparseConvolutionNoise.cpp:662-821), single-level SE */
 void orc_nee(const orc_field *f, const orc_ray *ray, const float normal_in[3], const float p_in[3],
              float t_segment, float grad_out[3], float *pdf_out) {

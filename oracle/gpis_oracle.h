@@ -150,6 +150,8 @@ void orc_trace_flight_perturbed(const orc_field *f, const orc_ray *ray, float ep
 void orc_conditioned_eval(const orc_field *f, const orc_ray *ray, const float q[3],
                           float t_segment, float out4[4]);
 /* 1D NEE (neeGrad / neePDF) after conditioning on the ray's origin state. */
+int orc_transverse_moments(const orc_field *f, const orc_ray *ray, const float p[3], double mean[2], double *var,
+                           int *scheme);
 void orc_nee(const orc_field *f, const orc_ray *ray, const float normal[3],
              const float p[3], float t_segment, float grad_out[3], float *pdf_out);
 

@@ -86,6 +86,8 @@ def lib():
         L.orc_cell3d_impulses.argtypes = [u32, u32, u32, u32, u32, f32p]
         L.orc_cell1d_impulses.argtypes = [u32, u32, u32, f32p]
         L.orc_standard_normals.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_double)]
+        L.orc_transverse_moments.restype = C.c_int
+        L.orc_transverse_moments.argtypes = [vp, vp, f32p, vp, vp, vp]
         L.orc_grid_density.restype = C.c_float
         L.orc_grid_density.argtypes = [vp, f32p]
         L.orc_grid_mean.restype = C.c_double
@@ -299,6 +301,18 @@ class Field:
         out = np.zeros(4, np.float32)
         lib().orc_conditioned_eval(self.h, C.byref(r), _fp(qq), t_segment, _fp(out))
         return out
+
+    def transverse_moments(self, ray_rec, p):
+        """(applies, mean[2], variance, sampling scheme) of the correlated transverse
+        gradient law at p for the ray's conditioned realization."""
+        r = Ray.from_buffer_copy(np.ascontiguousarray(ray_rec, RAY_DTYPE).tobytes())
+        pp = np.asarray(p, np.float32).copy()
+        mu = np.zeros(2)
+        var = np.zeros(1)
+        sch = C.c_int(0)
+        ok = lib().orc_transverse_moments(self.h, C.byref(r), _fp(pp), mu.ctypes.data, var.ctypes.data,
+                                          C.byref(sch))
+        return bool(ok), mu, float(var[0]), sch.value
 
     def nee(self, ray_rec, normal, p, t_segment):
         r = Ray.from_buffer_copy(np.ascontiguousarray(ray_rec, RAY_DTYPE).tobytes())

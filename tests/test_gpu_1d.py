@@ -176,6 +176,21 @@ def test_flights_1d_renewal_without_gradient():
     _check_1d(js, rays, W.LIGHT_POS, "1D Renewal (16,384 flights + NEE)")
 
 
+def test_flights_1d_renewal_plus_correlation_xy_with_nee():
+    """1D_gradient_correlationXY: the transverse gradient at the endpoint and the NEE
+    density follow the correlated law of seTransverseGradientMoments."""
+    from gpis_extend_amd import workloads as W
+    print("case: 1D Renewal+ correlationXY, 32,768 flights + NEE", flush=True)
+    js = W.config3_json()
+    js["1D_gradient_correlationXY"] = True
+    med = _medium(js)
+    rays = W.renewal1d_rays(med, n=32768, seed=7)
+    got, ref = _check_1d(js, rays, W.LIGHT_POS, "1D Renewal+ correlationXY (32,768 flights + NEE)")
+    oc = got["outcome"] & 0xFF
+    same = oc == ref["outcome"]
+    assert np.all(got["t"][same].view(np.uint32) == ref["t"][same].astype(np.float32).view(np.uint32))
+
+
 def test_entry_points_reject_wrong_dimension():
     from gpis_extend_amd import GpisDevice, RayBatch, workloads as W
     from gpis_extend_amd._capi import GpisError
@@ -190,7 +205,3 @@ def test_entry_points_reject_wrong_dimension():
     with pytest.raises(GpisError, match="gpis_trace_flights_1d"):
         d1.trace_flights(r3)
     d1.close()
-    js = W.config3_json()
-    js["1D_gradient_correlationXY"] = True
-    with pytest.raises(GpisError, match="UNSUPPORTED"):
-        GpisDevice(_medium(js))
