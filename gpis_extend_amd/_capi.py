@@ -48,6 +48,7 @@ class FieldDesc(C.Structure):
         ("global_seed", C.c_uint32), ("impulse_density", C.c_float), ("single_realization", C.c_int32),
         ("ctxt", C.c_int32), ("one_d", C.c_int32), ("scheme", C.c_int32), ("correlation_xy", C.c_int32),
         ("step_size", C.c_float), ("min_step", C.c_uint32), ("max_bounce", C.c_int32),
+        ("isotropic_3d", C.c_int32),
     ]
 
 

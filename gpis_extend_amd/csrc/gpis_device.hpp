@@ -62,6 +62,11 @@ struct FieldParams {
     float amp_const;     // amplitude when it is spatially constant
     float l_aniso[3], l_aniso_inv[3];
     float aniso[3];  // anisotropy factors of the SE length scale (1D rays rescale by their own l)
+    // isotropic ray space of 3D media (isotropic_3D_sampling): kernel radius, exponent
+    // factor (-log2 e * R^2 / 2), gradient factor (-R) and 1/sqrt(variance) of the
+    // identity-space kernel
+    int iso3d;
+    float iso_R, iso_c0, iso_g, iso_nf_inv;
     float length_scale;  // the medium's SE lengthScale
     float l_max;         // max(l_aniso) (splattingKernelRadius)
     float kernel_scale;  // _kernelScale (3)

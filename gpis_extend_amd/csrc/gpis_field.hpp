@@ -11,6 +11,7 @@
 // SE kernels and normalisation: GPFunctions.cpp:555-580, 706-843.
 #pragma once
 
+#include "gpis_1d.hpp"
 #include "gpis_device.hpp"
 
 #ifndef GPIS_PREFETCH_CELLS
@@ -258,10 +259,69 @@ __device__ __forceinline__ float ls_field(const FieldParams &P, float x, float y
     return P.ls_params[0];
 }
 
-// Normalised noise value and world gradient (evaluateNoise3D, world space).
-template <int NI, bool ISO>
+// The isotropic ray space of a flight along (dx, dy, dz): the frame of
+// normalize(worldToLocal d) (evaluateNoise3DIsotropicRayNormalized :349-352).
+__device__ __forceinline__ Frame3 iso_ray_frame(const FieldParams &P, float dx, float dy, float dz) {
+#pragma clang fp contract(off)
+    float ix = P.l_aniso_inv[0] * dx, iy = P.l_aniso_inv[1] * dy, iz = P.l_aniso_inv[2] * dz;
+    normalize3(ix, iy, iz);
+    return make_frame(ix, iy, iz);
+}
+// A world point in the isotropic ray space: frame.toLocal(worldToLocal p).
+__device__ __forceinline__ void to_iso_ray(const FieldParams &P, const Frame3 &F, float px, float py, float pz,
+                                           float &rx, float &ry, float &rz) {
+#pragma clang fp contract(off)
+    const float ix = P.l_aniso_inv[0] * px, iy = P.l_aniso_inv[1] * py, iz = P.l_aniso_inv[2] * pz;
+    rx = dot3(F.tx, F.ty, F.tz, ix, iy, iz);
+    ry = dot3(F.bx, F.by, F.bz, ix, iy, iz);
+    rz = dot3(F.nx, F.ny, F.nz, ix, iy, iz);
+}
+// A gradient from the isotropic ray space to world space: localToWorld^-T frame.toGlobal g.
+__device__ __forceinline__ void from_iso_ray(const FieldParams &P, const Frame3 &F, float gx, float gy, float gz,
+                                             float &wx, float &wy, float &wz) {
+#pragma clang fp contract(off)
+    float x = F.tx * gx + F.bx * gy;
+    x += F.nx * gz;
+    float y = F.ty * gx + F.by * gy;
+    y += F.ny * gz;
+    float z = F.tz * gx + F.bz * gy;
+    z += F.nz * gz;
+    wx = P.l_aniso_inv[0] * x; wy = P.l_aniso_inv[1] * y; wz = P.l_aniso_inv[2] * z;
+}
+
+// Normalised noise value and world gradient (evaluateNoise3D :142-165): world space, or
+// with IRAY the isotropic ray space of the flight along (dx, dy, dz).
+template <int NI, bool ISO, bool IRAY = false>
 __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealState &rs, 
-                                            float px, float py, float pz, bool cond) {
+                                            float px, float py, float pz, bool cond, float dx = 0.f, float dy = 0.f,
+                                            float dz = 1.f) {
+    if (IRAY) {
+        // single-level SE: noise3D on frame.toLocal(p / l_aniso) with the identity kernel
+        const Frame3 F = iso_ray_frame(P, dx, dy, dz);
+        float rx, ry, rz;
+        to_iso_ray(P, F, px, py, pz, rx, ry, rz);
+        Eval4 n = noise3d_raw<NI, true>(P, rx, ry, rz, rs.seed, P.iso_R, P.iso_c0, P.iso_c0, P.iso_c0);
+        float wx, wy, wz;
+        from_iso_ray(P, F, n.gx * P.iso_g, n.gy * P.iso_g, n.gz * P.iso_g, wx, wy, wz);
+        Eval4 r = {n.v * P.iso_nf_inv, wx * P.iso_nf_inv, wy * P.iso_nf_inv, wz * P.iso_nf_inv};
+        if (cond && rs.cond) {
+            // conditioning splats in the ray space: isotropic covariance kernel exp(-|d|^2 / 4)
+            float ox, oy, oz;
+            to_iso_ray(P, F, rs.ox, rs.oy, rs.oz, ox, oy, oz);
+            const float ax = rx - ox, ay = ry - oy, az = rz - oz;
+            const float k = __expf(-(ax * ax * 0.25f + ay * ay * 0.25f + az * az * 0.25f));
+            const float d0 = -0.5f * ax, d1 = -0.5f * ay, d2 = -0.5f * az;
+            const float bd = rs.bx * d0 + rs.by * d1 + rs.bz * d2;
+            const float v = (rs.a + bd) * k;
+            const float g0 = (rs.a * d0 + bd * d0 - 0.5f * rs.bx) * k;
+            const float g1 = (rs.a * d1 + bd * d1 - 0.5f * rs.by) * k;
+            const float g2 = (rs.a * d2 + bd * d2 - 0.5f * rs.bz) * k;
+            float cx, cy, cz;
+            from_iso_ray(P, F, g0, g1, g2, cx, cy, cz);
+            r.v += v; r.gx += cx; r.gy += cy; r.gz += cz;
+        }
+        return r;
+    }
     if (!P.multires) {
         Eval4 r;
         float ls = 1.0f, gs = 1.0f;
@@ -312,10 +372,11 @@ __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealStat
 __device__ __forceinline__ float amplitude(const FieldParams &P) { return P.amp_const; }
 
 // f(p) = amplitude * noise + mean, and its world gradient (evaluateValue/evaluateGradient).
-template <int NI, bool ISO, int MEAN = MEAN_ANY>
+template <int NI, bool ISO, int MEAN = MEAN_ANY, bool IRAY = false>
 __device__ __forceinline__ Eval4 eval_field(const FieldParams &P, const RealState &rs,
-                                            float px, float py, float pz, bool cond) {
-    Eval4 n = eval_noise<NI, ISO>(P, rs, px, py, pz, cond);
+                                            float px, float py, float pz, bool cond, float dx = 0.f,
+                                            float dy = 0.f, float dz = 1.f) {
+    Eval4 n = eval_noise<NI, ISO, IRAY>(P, rs, px, py, pz, cond, dx, dy, dz);
     float mgx, mgy, mgz;
     float m = mean_value<MEAN>(P, px, py, pz, mgx, mgy, mgz);
     float a = amplitude(P);

@@ -70,6 +70,20 @@ static gpis_status build_params(gpis_ctx *c) {
     if (d.one_d && d.cov_kind != GPIS_COV_SE)
         return fail(c, GPIS_UNSUPPORTED, "1D sampling is served for the stationary SE kernel");
     if (d.mean_kind < GPIS_MEAN_SPHERE || d.mean_kind > GPIS_MEAN_GRID) return fail(c, GPIS_UNSUPPORTED, "mean kind");
+    if (d.isotropic_3d && !d.one_d) {
+        if (d.cov_kind != GPIS_COV_SE)
+            return fail(c, GPIS_UNSUPPORTED, "isotropic_3D_sampling is served for the stationary SE kernel");
+        if (d.mean_kind == GPIS_MEAN_GRID)
+            return fail(c, GPIS_UNSUPPORTED, "isotropic_3D_sampling with a tabulated mean is not served");
+        // splattingKernelRadius(true, 1) and sparseConvNoiseVariance3D(N, R, identity)
+        // (GPFunctions.cpp:706-719, 751-764); the identity-space splat exp(-|R to|^2 / 2)
+        P.iso3d = 1;
+        P.iso_R = d.kernel_scale;
+        P.iso_c0 = -1.4426950408889634f * (P.iso_R * P.iso_R * 0.5f);
+        P.iso_g = -P.iso_R;
+        const double var = (double)((float)(unsigned)d.impulse_density / (P.iso_R * P.iso_R * P.iso_R)) * pow(M_PI, 1.5);
+        P.iso_nf_inv = (float)(1.0 / sqrt((double)(float)var));
+    }
     P.kind = d.cov_kind;
     P.multires = (d.cov_kind == GPIS_COV_PROC_NS) && d.ns_multires;
     P.sigma = d.sigma;
@@ -318,9 +332,10 @@ __device__ __forceinline__ float box_exit(float ox, float oy, float oz, float dx
 // in FP64.  With one observation point the value and gradient observations decouple
 // (the covariance gradient vanishes at zero lag), so the Schur complement of the
 // value+gradient observation covariance is the diagonal second-derivative block.
-template <int MEAN>
+template <int MEAN, bool IRAY>
 __device__ __noinline__ void condition_coefficients(const FieldParams &P, const Eval4 &e, float ox, float oy, float oz,
-                                                    float pv, float pgx, float pgy, float pgz, RealState &rs) {
+                                                    float pv, float pgx, float pgy, float pgz, float dx, float dy,
+                                                    float dz, RealState &rs) {
     float mgx, mgy, mgz;
     double mean = mean_value<MEAN>(P, ox, oy, oz, mgx, mgy, mgz);
     double amp = P.amp_const;
@@ -343,10 +358,20 @@ __device__ __noinline__ void condition_coefficients(const FieldParams &P, const 
         double del0 = ((double)pgx - mgx) / amp - ((double)e.gx - mgx) / amp;
         double del1 = ((double)pgy - mgy) / amp - ((double)e.gy - mgy) / amp;
         double del2 = ((double)pgz - mgz) / amp - ((double)e.gz - mgz) / amp;
-        // inverse second derivative of the covariance at zero lag: -2 / l_aniso_inv_i^2
-        b0 = -2.0 / ((double)P.l_aniso_inv[0] * P.l_aniso_inv[0]) * del0 / gsum;
-        b1 = -2.0 / ((double)P.l_aniso_inv[1] * P.l_aniso_inv[1]) * del1 / gsum;
-        b2 = -2.0 / ((double)P.l_aniso_inv[2] * P.l_aniso_inv[2]) * del2 / gsum;
+        if (IRAY) {
+            // the isotropic ray space (:539-552): -2 frame.toLocal(localToWorld^T delta)
+            const Frame3 F = iso_ray_frame(P, dx, dy, dz);
+            const double l0 = (double)P.l_aniso[0] * del0, l1 = (double)P.l_aniso[1] * del1,
+                         l2 = (double)P.l_aniso[2] * del2;
+            b0 = -2.0 * ((double)F.tx * l0 + (double)F.ty * l1 + (double)F.tz * l2);
+            b1 = -2.0 * ((double)F.bx * l0 + (double)F.by * l1 + (double)F.bz * l2);
+            b2 = -2.0 * ((double)F.nx * l0 + (double)F.ny * l1 + (double)F.nz * l2);
+        } else {
+            // inverse second derivative of the covariance at zero lag: -2 / l_aniso_inv_i^2
+            b0 = -2.0 / ((double)P.l_aniso_inv[0] * P.l_aniso_inv[0]) * del0 / gsum;
+            b1 = -2.0 / ((double)P.l_aniso_inv[1] * P.l_aniso_inv[1]) * del1 / gsum;
+            b2 = -2.0 / ((double)P.l_aniso_inv[2] * P.l_aniso_inv[2]) * del2 / gsum;
+        }
     }
     rs.ox = ox; rs.oy = oy; rs.oz = oz;
     rs.a = (float)(a / wsum);
@@ -382,8 +407,9 @@ enum { ST_FLIGHTS = 0, ST_EVALS, ST_STEPS, ST_HITS, ST_EXITS, ST_FAILS, ST_INVAL
 // (or path) finished take the next ray indices from a global counter: one atomic
 // per wave, ranks from the ballot mask.  COND: the medium conditions secondary
 // flights (Renewal/Renewal+); PATHS: hits continue as diffuse bounces; GRID: the mean
-// is tabulated (the analytic means otherwise, whose variants stay free of its registers).
-template <int NI, bool ISO, bool COND, bool PATHS, bool GRID>
+// is tabulated (the analytic means otherwise, whose variants stay free of its registers);
+// IRAY: the noise lives in each flight's isotropic ray space (isotropic_3D_sampling).
+template <int NI, bool ISO, bool COND, bool PATHS, bool GRID, bool IRAY = false>
 __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(FieldParams P, RayIn in, ReplyOut out,
                                                                             PathOut pout, unsigned int n,
                                                                             unsigned int *work,
@@ -551,7 +577,8 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
         if (phase == PH_NEED || phase == PH_DEAD || phase == PH_DONE) continue;
 
         // ---- one evaluation per live lane
-        Eval4 e = eval_field<NI, ISO, GRID ? MEAN_GRID : MEAN_ANALYTIC>(P, rs, qx, qy, qz, !COND || phase != PH_COND);
+        Eval4 e = eval_field<NI, ISO, GRID ? MEAN_GRID : MEAN_ANALYTIC, IRAY>(P, rs, qx, qy, qz,
+                                                                               !COND || phase != PH_COND, dx, dy, dz);
         ++evals_flight;
         const float fv = e.v;
         // the next query point: t along the ray (unit: along the normalised direction)
@@ -559,7 +586,8 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
         bool unit = false;
 
         if (COND && phase == PH_COND) {
-            condition_coefficients<GRID ? MEAN_GRID : MEAN_ANALYTIC>(P, e, ox, oy, oz, prev_v, prev_gx, prev_gy, prev_gz, rs);
+            condition_coefficients<GRID ? MEAN_GRID : MEAN_ANALYTIC, IRAY>(P, e, ox, oy, oz, prev_v, prev_gx, prev_gy,
+                                                                          prev_gz, dx, dy, dz, rs);
             phase = PH_F0;
             tq = (double)near_t;
         } else if (phase == PH_F0) {
@@ -1100,6 +1128,16 @@ static void launch_flights(gpis_ctx *c, unsigned blocks, unsigned threads, hipSt
     } while (0)
     const bool cond = c->P.activate != 0;
     const bool grid = c->P.mean_kind == GPIS_MEAN_GRID;
+    if (c->P.iso3d) {
+        // the isotropic ray space: generic impulse count, isotropic kernel, analytic means
+        if (cond)
+            hipLaunchKernelGGL((flight_kernel<0, true, true, PATHS, false, true>), dim3(blocks), dim3(threads), 0, st,
+                               c->P, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box, c->batch_k);
+        else
+            hipLaunchKernelGGL((flight_kernel<0, true, false, PATHS, false, true>), dim3(blocks), dim3(threads), 0, st,
+                               c->P, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box, c->batch_k);
+        return;
+    }
     if (c->P.N == 8 && c->P.iso && !cond) GPIS_FLIGHT_LAUNCH(8, true, false);
     else if (c->P.N == 8 && c->P.iso) GPIS_FLIGHT_LAUNCH(8, true, true);
     else if (c->P.N == 8 && !cond) GPIS_FLIGHT_LAUNCH(8, false, false);
@@ -1198,6 +1236,7 @@ static gpis_status mean_ready(gpis_ctx *c) {
 gpis_status gpis_set_domain(gpis_ctx *c, const float lo[3], const float hi[3], uint32_t seed) {
     if (!c || !lo || !hi) return fail(c, GPIS_INVALID_ARGUMENT, "null argument");
     if (c->P.multires) return fail(c, GPIS_UNSUPPORTED, "impulse tables cover the single-level grid");
+    if (c->P.iso3d) return fail(c, GPIS_UNSUPPORTED, "the isotropic ray space has one lattice per flight direction");
     if (c->table) { HIP_TRY(c, hipFree(c->table)); c->table = nullptr; }
     c->P.has_table = 0;
     c->P.table = nullptr;
@@ -1250,6 +1289,7 @@ gpis_status gpis_eval_points(gpis_ctx *c, const float *x, const float *y, const 
     if (n == 0) return GPIS_OK;
     if (!x || !y || !z || !v || !gx || !gy || !gz) return fail(c, GPIS_INVALID_ARGUMENT, "null pointer");
     if (gpis_status s = mean_ready(c)) return s;
+    if (c->P.iso3d) return fail(c, GPIS_UNSUPPORTED, "the isotropic ray space needs a flight direction");
     int threads = 256;
     unsigned blocks = (unsigned)((n + threads - 1) / threads);
 #define GPIS_EVAL_LAUNCH(NI, ISO)                                                                              \

@@ -231,8 +231,6 @@ class SparseConvolutionNoiseMedium:
         m.aniso = tuple(float(x) for x in (a if isinstance(a, (list, tuple)) else (a, a, a)))
         if cov.get("useAnisoMtx", False):
             raise MediumConfigError("useAnisoMtx is not served by the GPU flight path")
-        if m.isotropic_3D_sampling and not m.one_d:
-            raise MediumConfigError("isotropic_3D_sampling is not served by the GPU flight path")
         return m
 
     def to_desc(self) -> A.FieldDesc:
@@ -258,6 +256,7 @@ class SparseConvolutionNoiseMedium:
         d.scheme = self.scheme
         d.correlation_xy = int(self.correlation_xy)
         d.step_size, d.min_step, d.max_bounce = self.step_size, self.min_step, self.max_bounces
+        d.isotropic_3d = int(self.isotropic_3D_sampling and not self.one_d)
         return d
 
     @property

@@ -71,6 +71,8 @@ typedef struct gpis_field_desc {
     float step_size;                           /* "step_size" */
     uint32_t min_step;                         /* "min_step" */
     int32_t max_bounce;                        /* Medium "max_bounces" */
+    int32_t isotropic_3d;                      /* "isotropic_3D_sampling": 3D noise in the isotropic
+                                                  ray space of each flight (stationary SE kernel) */
 } gpis_field_desc;
 
 /* A wavefront of flight requests in structure-of-arrays form

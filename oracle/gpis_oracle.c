@@ -540,7 +540,7 @@ static v4f cell3d(const orc_field *f, v3f pw, uint32_t ci, uint32_t cj, uint32_t
         float w = pcg_1d(&g) < 0.5f ? -1.f : 1.f;
         v3f to = vsub(p, pi);
         if (vlen2(to) < 1.0f) {
-            sum = v4add(sum, v4scale(w, splat3d(f, vscale(R, p), vscale(R, pi), 0, 0, kss, pw)));
+            sum = v4add(sum, v4scale(w, splat3d(f, vscale(R, p), vscale(R, pi), 0, f->d.iso3d != 0, kss, pw)));
         }
     }
     if (pairs) *pairs += (int)n;
@@ -574,10 +574,43 @@ static v4f noise3d_normalized(const real_t *r, v3f p, float N, float R, float ks
     }
     return nz;
 }
-/* evaluateNoise3D, world space (:142-153) */
-static v4f eval_noise3d(const real_t *r, v3f p, int cond) {
+/* evaluateNoise3DIsotropicRayNormalized (:345-377): the point in the isotropic space,
+ * rotated into the frame of the flight's isotropic direction, noise3D there with the
+ * identity-space kernel radius, the gradient rotated and scaled back to world space. */
+static v4f noise3d_iso_ray_normalized(const real_t *r, v3f p, v3f dir, float N, float R, float kss, int cond) {
+    const orc_field *f = r->f;
+    v3f dir_iso = vnorm(w2l(f, dir, 1.0f));
+    frame_t F = make_frame(dir_iso);
+    v3f piso = w2l(f, p, kss);
+    v3f pir = to_local(&F, piso);
+    int addl = ilog_level(f, kss);
+    v4f nir = noise3d(f, p, pir, r->seed + (uint32_t)addl, N, R, 1.0f);
+    v3f gw = grad_l2w(f, to_global(&F, V3(nir.y, nir.z, nir.w)), kss);
+    v4f nw = V4(nir.x, gw.x, gw.y, gw.z);
+    float nf = sqrtf(cov_var3d(f, p, N, R, 1, 1.0f));
+    nw = v4divs(nw, nf);
+    if (f->activate && cond) {
+        v3f oir = to_local(&F, w2l(f, r->c.origin, kss));
+        v4f a = v4scale(r->c.value_scale, splat3d(f, pir, oir, 1, 1, 1.0f, p));
+        v4f b = splat3d_grad(f, pir, oir, r->c.grad_scale, 1, 1, 1.0f, p);
+        v4f d = v4add(a, b);
+        v3f gd = grad_l2w(f, to_global(&F, V3(d.y, d.z, d.w)), kss);
+        nw = v4add(nw, V4(d.x, gd.x, gd.y, gd.z));
+    }
+    return nw;
+}
+/* evaluateNoise3D (:142-165): world space, or the isotropic ray space of the flight */
+static v4f eval_noise3d(const real_t *r, v3f p, v3f dir, int cond) {
     const orc_field *f = r->f;
     float N = f->d.impulse_density;
+    if (f->d.iso3d) {
+        float R = cov_kernel_radius(f, 1, 1.0f);
+        if (!f->multires) return noise3d_iso_ray_normalized(r, p, dir, N, R, 1.0f, cond);
+        v4f info = level_ratio(f, p);
+        v4f lo = noise3d_iso_ray_normalized(r, p, dir, N, R, info.x, cond);
+        v4f hi = noise3d_iso_ray_normalized(r, p, dir, N, R, info.y, cond);
+        return v4add(v4scale(info.z, lo), v4scale(info.w, hi));
+    }
     if (!f->multires) {
         float kss = cov_world_sampling_scale(f);
         return noise3d_normalized(r, p, N, cov_kernel_radius(f, 0, 1.0f), kss, cond);
@@ -762,7 +795,7 @@ static v3f eval_grad_noise1d(const real_t *r, v3f p, float tseg, v3f dir, int co
 static float eval_value(const real_t *r, v3f p, v3f dir) {
     const orc_field *f = r->f;
     float amp = cov_amplitude(f, p);
-    float nv = f->d.one_d ? eval_value_noise1d(r, p, dir, 1) : eval_noise3d(r, p, 1).x;
+    float nv = f->d.one_d ? eval_value_noise1d(r, p, dir, 1) : eval_noise3d(r, p, dir, 1).x;
     double m = mean_value(f, d_of(p));
     return (float)((double)(amp * nv) + m);
 }
@@ -771,7 +804,7 @@ static v3f eval_gradient(const real_t *r, v3f p, float t, v3f dir) {
     float amp = cov_amplitude(f, p);
     v3f ng;
     if (f->d.one_d) ng = eval_grad_noise1d(r, p, t, dir, 1);
-    else { v4f n4 = eval_noise3d(r, p, 1); ng = V3(n4.y, n4.z, n4.w); }
+    else { v4f n4 = eval_noise3d(r, p, dir, 1); ng = V3(n4.y, n4.z, n4.w); }
     v3f mg = f_of(mean_grad(f, d_of(p)));
     return vadd(vscale(amp, ng), mg);
 }
@@ -790,7 +823,6 @@ static float march_value(real_t *r, v3f p, v3f dir) {
 /* conditioning3D (:498-574), world space */
 static void conditioning3d(real_t *r, v3f p, v3f dir, float tv, v3f tg) {
     const orc_field *f = r->f;
-    (void)dir;
     r->c.value_scale = 0.f; r->c.grad_scale = V3(0.f, 0.f, 0.f); r->c.origin = p;
     v4f ksi = V4(0, 0, 0, 0);
     if (f->multires) ksi = level_ratio(f, p);
@@ -799,23 +831,38 @@ static void conditioning3d(real_t *r, v3f p, v3f dir, float tv, v3f tg) {
         amp = cov_amplitude(f, p);
         if (amp == 0) return;
         float mean = (float)mean_value(f, d_of(p));
-        float cur = eval_noise3d(r, p, 0).x;
+        float cur = eval_noise3d(r, p, dir, 0).x;
         r->c.value_scale = (tv - mean) / amp - cur;
         if (f->multires) r->c.value_scale /= ksi.z + ksi.w;
     }
     if (f->d.ctxt == ORC_CTX_RENEWAL_PLUS) {
         v3f mg = f_of(mean_grad(f, d_of(p)));
-        v4f n4 = eval_noise3d(r, p, 0);
+        v4f n4 = eval_noise3d(r, p, dir, 0);
         v3f cur = V3(n4.y, n4.z, n4.w);
         v3f delta = vsub(vdivs(vsub(tg, mg), amp), cur);
-        m3f inv = one_over_second(f, 0);
-        r->c.grad_scale = m3_mul(&inv, delta);
-        if (f->multires) {
-            float den = ksi.z / (ksi.x * ksi.x) + ksi.w / (ksi.y * ksi.y);
-            r->c.grad_scale = vdivs(r->c.grad_scale, den);
+        if (f->d.iso3d) {
+            /* the isotropic (ray) space: localToWorld^T delta in the flight's frame (:539-556) */
+            v3f gsi = grad_w2l(f, delta, 1.0f);
+            frame_t F = make_frame(vnorm(w2l(f, dir, 1.0f)));
+            gsi = to_local(&F, gsi);
+            m3f inv = one_over_second(f, 1);
+            r->c.grad_scale = m3_mul(&inv, gsi);
+            if (f->multires) {
+                r->c.grad_scale = vdivs(r->c.grad_scale, ksi.z / ksi.x + ksi.w / ksi.y);
+            } else {
+                float s = sqrf(cov_splat_scale(f, p));
+                r->c.grad_scale = V3(r->c.grad_scale.x * s, r->c.grad_scale.y * s, r->c.grad_scale.z * s);
+            }
         } else {
-            float s = sqrf(cov_lateral(f, p));
-            r->c.grad_scale = V3(r->c.grad_scale.x * s, r->c.grad_scale.y * s, r->c.grad_scale.z * s);
+            m3f inv = one_over_second(f, 0);
+            r->c.grad_scale = m3_mul(&inv, delta);
+            if (f->multires) {
+                float den = ksi.z / (ksi.x * ksi.x) + ksi.w / (ksi.y * ksi.y);
+                r->c.grad_scale = vdivs(r->c.grad_scale, den);
+            } else {
+                float s = sqrf(cov_lateral(f, p));
+                r->c.grad_scale = V3(r->c.grad_scale.x * s, r->c.grad_scale.y * s, r->c.grad_scale.z * s);
+            }
         }
     }
 }
@@ -907,37 +954,44 @@ static void real_init(real_t *r, const orc_field *f, const uint32_t pss[4], uint
 
 void orc_noise3d(const orc_field *f, const float *p, uint32_t seed, float out4[4]) {
     real_t r; memset(&r, 0, sizeof r); r.f = f; r.seed = seed;
-    v4f n = eval_noise3d(&r, V3(p[0], p[1], p[2]), 0);
+    v4f n = eval_noise3d(&r, V3(p[0], p[1], p[2]), V3(0.f, 0.f, 1.f), 0);
     out4[0] = n.x; out4[1] = n.y; out4[2] = n.z; out4[3] = n.w;
 }
 
 typedef struct {
     const orc_field *f; const float *xyz; float *out; int64_t n0, n1; uint32_t seed;
-    const orc_ray *rays; orc_flight *fl;
+    const orc_ray *rays; orc_flight *fl; v3f dir;
 } job_t;
 
-static void eval_point(const orc_field *f, uint32_t seed, const float *q, float *o) {
+/* evaluateValue / evaluateGradient of the unconditioned realization `seed` at q for a
+ * flight along `dir` (the direction matters only to the isotropic ray space). */
+static void eval_point(const orc_field *f, uint32_t seed, const float *q, v3f dir, float *o) {
     real_t r; memset(&r, 0, sizeof r); r.f = f; r.seed = seed;
     v3f p = V3(q[0], q[1], q[2]);
-    o[0] = eval_value(&r, p, V3(0.f, 0.f, 1.f));
-    v3f g = eval_gradient(&r, p, 1.0f, V3(0.f, 0.f, 1.f));
+    o[0] = eval_value(&r, p, dir);
+    v3f g = eval_gradient(&r, p, 1.0f, dir);
     o[1] = g.x; o[2] = g.y; o[3] = g.z;
 }
 static void *eval_worker(void *arg) {
     job_t *j = (job_t *)arg;
-    for (int64_t i = j->n0; i < j->n1; ++i) eval_point(j->f, j->seed, j->xyz + 3 * i, j->out + 4 * i);
+    for (int64_t i = j->n0; i < j->n1; ++i) eval_point(j->f, j->seed, j->xyz + 3 * i, j->dir, j->out + 4 * i);
     return NULL;
 }
-void orc_eval_points(const orc_field *f, const float *xyz, int64_t n, uint32_t seed, float *out4, int nthreads) {
+void orc_eval_points_dir(const orc_field *f, const float *xyz, int64_t n, uint32_t seed, const float dir[3],
+                         float *out4, int nthreads) {
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
     pthread_t th[256]; job_t jb[256];
     for (int t = 0; t < nthreads; ++t) {
-        jb[t].f = f; jb[t].xyz = xyz; jb[t].out = out4; jb[t].seed = seed;
+        jb[t].f = f; jb[t].xyz = xyz; jb[t].out = out4; jb[t].seed = seed; jb[t].dir = V3(dir[0], dir[1], dir[2]);
         jb[t].n0 = n * t / nthreads; jb[t].n1 = n * (t + 1) / nthreads;
         pthread_create(&th[t], NULL, eval_worker, &jb[t]);
     }
     for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+}
+void orc_eval_points(const orc_field *f, const float *xyz, int64_t n, uint32_t seed, float *out4, int nthreads) {
+    const float dz[3] = {0.f, 0.f, 1.f};
+    orc_eval_points_dir(f, xyz, n, seed, dz, out4, nthreads);
 }
 
 /* prepareGpisSegment (SparseConvolutionNoiseMedium.cpp:199-263) */

@@ -86,6 +86,9 @@ typedef struct orc_medium_desc {
     float grid_clamp_min[3], grid_clamp_max[3];
     float grid_offset, grid_scale;
     int32_t grid_volume;
+    /* "isotropic_3D_sampling": 3D noise in the isotropic ray space of each flight
+     * (evaluateNoise3DIsotropicRayNormalized, SparseConvolutionNoise.cpp:345-377) */
+    int32_t iso3d;
 } orc_medium_desc;
 
 /* Dense-grid density at a world point (trilinear, clamped), and TabulatedMean::mean
@@ -132,6 +135,8 @@ uint32_t orc_compute_seed(const orc_field *f, const uint32_t pss[4], uint32_t sc
 /* Unconditioned normalised noise + world gradient (evaluateNoise3D), per point. */
 void orc_noise3d(const orc_field *f, const float *p, uint32_t seed, float out4[4]);
 /* f(p) and grad f(p) as evaluateValue/evaluateGradient with no conditioning. */
+void orc_eval_points_dir(const orc_field *f, const float *xyz, int64_t n, uint32_t seed, const float dir[3],
+                         float *out4, int nthreads);
 void orc_eval_points(const orc_field *f, const float *xyz, int64_t n,
                      uint32_t seed, float *out4, int nthreads);
 /* World-space cell size used by the 3D grid (splattingKernelRadius(false, 1)). */

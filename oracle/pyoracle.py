@@ -37,6 +37,7 @@ class MediumDesc(C.Structure):
         ("grid", C.c_void_p), ("grid_origin", C.c_int32 * 3), ("grid_dims", C.c_int32 * 3),
         ("grid_w2i", C.c_float * 12), ("grid_clamp_min", C.c_float * 3), ("grid_clamp_max", C.c_float * 3),
         ("grid_offset", C.c_float), ("grid_scale", C.c_float), ("grid_volume", C.c_int32),
+        ("iso3d", C.c_int32),
     ]
 
 
@@ -100,6 +101,7 @@ def lib():
         L.orc_compute_seed.argtypes = [vp, C.POINTER(C.c_uint32), u32]
         L.orc_noise3d.argtypes = [vp, f32p, u32, f32p]
         L.orc_eval_points.argtypes = [vp, f32p, C.c_int64, u32, f32p, C.c_int]
+        L.orc_eval_points_dir.argtypes = [vp, f32p, C.c_int64, u32, f32p, f32p, C.c_int]
         L.orc_kernel_radius.restype = C.c_float; L.orc_kernel_radius.argtypes = [vp]
         L.orc_trace_flight.argtypes = [vp, C.POINTER(Ray), C.POINTER(Flight)]
         L.orc_trace_flights.argtypes = [vp, vp, C.c_int64, vp, C.c_int]
@@ -206,6 +208,7 @@ def make_desc(**kw) -> MediumDesc:
     d.step_size = kw.get("step_size", 0.01)
     d.min_step = kw.get("min_step", 8)
     d.max_bounce = kw.get("max_bounce", 1 << 30)
+    d.iso3d = int(kw.get("iso3d", 0))
     g = kw.get("grid")
     if g is not None:
         set_grid(d, g.voxels, g.origin, g.world_to_index, g.clamp_min, g.clamp_max,
@@ -281,6 +284,15 @@ class Field:
         xyz = np.ascontiguousarray(xyz, np.float32)
         out = np.zeros((xyz.shape[0], 4), np.float32)
         lib().orc_eval_points(self.h, _fp(xyz), xyz.shape[0], seed, _fp(out), nthreads)
+        return out
+
+    def eval_points_dir(self, xyz, seed, direction, nthreads=1):
+        """Value and gradient at points for flights along `direction` (the isotropic
+        ray space of isotropic_3D_sampling media depends on it)."""
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        d = np.asarray(direction, np.float32).copy()
+        out = np.zeros((xyz.shape[0], 4), np.float32)
+        lib().orc_eval_points_dir(self.h, _fp(xyz), xyz.shape[0], seed, _fp(d), _fp(out), nthreads)
         return out
 
     def trace(self, rays: np.ndarray, nthreads=1) -> np.ndarray:

@@ -18,7 +18,7 @@ def oracle_desc(med):
         ctxt=med.correlation_context, one_d=int(med.one_d), scheme=med.scheme,
         correlation_xy=int(med.correlation_xy), step_size=med.step_size, min_step=med.min_step,
         max_bounce=med.max_bounces, grid=med.mean_grid, grid_offset=med.mean_offset, grid_scale=med.mean_scale,
-        grid_volume=med.mean_volume)
+        grid_volume=med.mean_volume, iso3d=int(med.isotropic_3D_sampling and not med.one_d))
     return d
 
 
