@@ -20,14 +20,15 @@ CTX_GLOBAL, CTX_RENEWAL, CTX_RENEWAL_PLUS, CTX_NONE = 0, 1, 2, 3
 SCHEME_UNI, SCHEME_NEE, SCHEME_MIS = 0, 1, 2
 LS_CONST, LS_SINCOS = 0, 1
 EXIT, HIT, FAIL, INVALID = 0, 1, 2, 3
-FAIL_NONE, FAIL_OUTWARD, FAIL_ZERO_GRAD, FAIL_NONFINITE = 0, 1, 2, 3
+FAIL_NONE, FAIL_OUTWARD, FAIL_ZERO_GRAD, FAIL_NONFINITE, FAIL_BOUNCE_LIMIT = 0, 1, 2, 3, 4
+PHASE_MIRROR, PHASE_LAMBERT = 0, 1
 
 #: every symbol include/gpis_hip.h declares
 EXPORTS = (
     "gpis_create", "gpis_destroy", "gpis_last_error", "gpis_set_domain", "gpis_impulse_table",
     "gpis_eval_points", "gpis_trace_flights", "gpis_trace_paths", "gpis_get_counters",
     "gpis_reset_counters", "gpis_set_launch", "gpis_set_refill", "gpis_trace_flights_1d",
-    "gpis_set_mean_grid",
+    "gpis_set_mean_grid", "gpis_trace_paths_1d",
     # include/gpis_surrogate.h
     "gpis_net_create", "gpis_net_destroy", "gpis_net_last_error", "gpis_net_set_params", "gpis_net_get_params",
     "gpis_net_set_calibration", "gpis_net_features_1d", "gpis_net_classes_1d", "gpis_net_gradient",
@@ -73,6 +74,15 @@ class ReplySoA(C.Structure):
 
 class PathReply(C.Structure):
     _fields_ = [(n, _vp) for n in ("t0", "gx0", "gy0", "gz0", "outcome0", "flights", "hits", "evals")]
+
+
+class PathSeeds(C.Structure):
+    _fields_ = [("px", _vp), ("py", _vp), ("spp", _vp), ("bounce", _vp), ("scene_seed", C.c_uint32)]
+
+
+class Path1DReply(C.Structure):
+    _fields_ = [(n, _vp) for n in ("t0", "gx0", "gy0", "gz0", "outcome0", "flights", "hits", "shadows", "visible",
+                                   "nee", "t_last", "outcome_last", "evals")]
 
 
 class NeeOut(C.Structure):
@@ -122,6 +132,10 @@ def lib():
     L.gpis_trace_paths.restype = st
     L.gpis_trace_paths.argtypes = [_vp, C.POINTER(RaySoA), C.c_int64, C.c_int32, C.c_uint32, C.c_float,
                                    C.POINTER(PathReply), _vp]
+    L.gpis_trace_paths_1d.restype = st
+    L.gpis_trace_paths_1d.argtypes = [_vp, C.POINTER(RaySoA), C.POINTER(PathSeeds), C.c_int64, C.c_int32,
+                                      C.c_int32, C.c_uint32, C.c_float, C.POINTER(C.c_float),
+                                      C.POINTER(Path1DReply), _vp]
     L.gpis_trace_flights_1d.restype = st
     L.gpis_trace_flights_1d.argtypes = [_vp, C.POINTER(RaySoA), C.c_int64, C.POINTER(ReplySoA),
                                         C.POINTER(C.c_float), C.POINTER(NeeOut), _vp]

@@ -141,6 +141,8 @@ static gpis_status build_params(gpis_ctx *c) {
     P.activate = !d.single_realization && (d.ctxt == GPIS_CTX_RENEWAL || d.ctxt == GPIS_CTX_RENEWAL_PLUS);
     P.one_d = d.one_d;
     P.corr_xy = d.correlation_xy;
+    // samplingScheme's configured scheme (SparseConvolutionNoise.cpp:25-54): UNI unless 1D and not single
+    P.scheme1d = (!d.single_realization && d.one_d) ? d.scheme : GPIS_SCHEME_UNI;
     P.step_size = d.step_size;
     P.min_step_f = (float)d.min_step;
     P.max_bounce = d.max_bounce;
@@ -787,16 +789,80 @@ __device__ __forceinline__ void nee1d(const FieldParams &P, const Ray1D &R, floa
     pdf = (float)(pdf_n / (4.0 * (double)dot3(hx, hy, hz, -dx, -dy, -dz)));
 }
 
+// Inputs and outputs of device-resident 1D paths (gpis_trace_paths_1d).
+struct Path1DIn {
+    const uint32_t *px, *py, *spp, *w0;  // RayInfo::pixelSampleSegment of each path's first flight
+    uint32_t scene_hash;                 // xxhash32(sceneSeed)
+    int max_bounces, phase;
+    uint32_t bounce_seed;
+    float box;
+};
+struct Path1DOut {
+    float *t0, *gx0, *gy0, *gz0;
+    uint32_t *outcome0, *flights, *hits, *shadows, *visible;
+    float *nee, *t_last;
+    uint32_t *outcome_last, *evals;
+};
+
+// Seeds of the flight with pixelSampleSegment.w = w of path r: computeSeed
+// (SparseConvolutionNoise.cpp:57-66) and the normal seed of prepareGpisSegment
+// (SparseConvolutionNoiseMedium.cpp:235-238).
+__device__ __forceinline__ void path_seeds(const FieldParams &P, const Path1DIn &pin, unsigned r, uint32_t w,
+                                           uint32_t &noise, uint32_t &normal) {
+    const uint32_t a = pin.px[r], b = pin.py[r], s = pin.spp[r];
+    const uint32_t h = xxh4(a, b, s, w);
+    normal = h + pin.scene_hash + 1u;
+    noise = P.global_seed;
+    if (!P.single) noise += (P.ctxt == GPIS_CTX_GLOBAL ? xxh4(a, b, s, 0u) : h) + pin.scene_hash;
+}
+
+// The scattering frame of BRDFPhaseFunction::setEventIsectInfo (BRDFPhaseFunction.cpp:29-49):
+// TangentFrame(Vec3f(aniso.normalized())), the gradient normalised in double, and
+// wi = frame.toLocal(-dir).normalized().
+__device__ __forceinline__ void hit_frame(float gx, float gy, float gz, float dx, float dy, float dz, Frame3 &F,
+                                          float &wix, float &wiy, float &wiz) {
+#pragma clang fp contract(off)
+    const double ex = gx, ey = gy, ez = gz;
+    double l2 = ex * ex;
+    l2 += ey * ey;
+    l2 += ez * ez;
+    const double inv = 1.0 / sqrt(l2);
+    F = make_frame((float)(ex * inv), (float)(ey * inv), (float)(ez * inv));
+    wix = dot3(F.tx, F.ty, F.tz, -dx, -dy, -dz);
+    wiy = dot3(F.bx, F.by, F.bz, -dx, -dy, -dz);
+    wiz = dot3(F.nx, F.ny, F.nz, -dx, -dy, -dz);
+    normalize3(wix, wiy, wiz);
+}
+// TangentFrame::toGlobal: tangent * x + bitangent * y + normal * z, each product rounded.
+__device__ __forceinline__ void frame_to_global(const Frame3 &F, float x, float y, float z, float &ox, float &oy,
+                                                float &oz) {
+#pragma clang fp contract(off)
+    const float ax = F.tx * x, ay = F.ty * x, az = F.tz * x;
+    const float bx = F.bx * y, by = F.by * y, bz = F.bz * y;
+    const float cx = F.nx * z, cy = F.ny * z, cz = F.nz * z;
+    ox = (ax + bx) + cx;
+    oy = (ay + by) + cy;
+    oz = (az + bz) + cz;
+}
+// Angle.hpp:8-14 in float: PI = 3.1415926536f, TWO_PI = PI * 2.0f, INV_PI = 1.0f / PI
+#define kTwoPiF 0x1.921fb6p+2f
+#define kInvPiF 0x1.45f306p-2f
+
 // The 1D persistent flight kernel: same lane state machine as flight_kernel, with the
 // 1D field (cells cached per lane), conditioning1D at the origin, the Box-Muller
 // transverse gradient at the endpoint, and optional NEE toward a point light.
-template <int NI>
+// PATHS: each lane traces a whole path instead (gpis_trace_paths_1d): at a hit the
+// hit-time phase runs in the batched finish step (the light sample with its shadow
+// flight, then the mirror or Lambert bounce) and the next flight starts from the hit,
+// conditioned on (0, gradient), without leaving the kernel.
+template <int NI, bool PATHS>
 __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn in, const float *__restrict__ lscale,
                                                           ReplyOut out, unsigned int n, unsigned int *work,
                                                           unsigned long long *stats, float nf, int use_light,
                                                           float lx, float ly, float lz, Nee1DOut nee,
                                                           const uint32_t *__restrict__ idx,
-                                                          const uint32_t *__restrict__ n_dev, int batch_k) {
+                                                          const uint32_t *__restrict__ n_dev, int batch_k,
+                                                          Path1DIn pin, Path1DOut pout) {
     // idx / n_dev: optional list of ray indices (and its device-side length) to trace
     const unsigned nn = n_dev ? *n_dev : n;
     __shared__ unsigned int bstat[ST_N];
@@ -820,11 +886,106 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
     const bool corr = P.corr_xy && P.ctxt == GPIS_CTX_RENEWAL_PLUS;  // seTransverseGradientMoments applies
     float ndv_end = 0.f;             // derivative along the ray at the endpoint
     float ndv_nee = 0.f;             // ... and at the NEE point
-    int nee_stage = 0;               // 1 once the NEE evaluation is done
+    // 0: flight running or its gradient pending; 1: NEE evaluation done; 2: reply ready
+    // (flights); 3: the bounce is next (paths)
+    int nee_stage = 0;
     Ray1D R = {};
     Cells1D<NI> C;
     C.base = 0;
     bool cache_valid = false;
+    // path state (PATHS)
+    int pb = 0;                 // flight index within the path (pixelSampleSegment.w - w0)
+    bool shadow = false;        // the running flight is a light sample's shadow flight
+    uint32_t w0 = 0;
+    float hx = 0.f, hy = 0.f, hz = 0.f;     // scattering point of the last hit (sample.p)
+    float hgx = 0.f, hgy = 0.f, hgz = 0.f;  // its gradient (sample.aniso)
+    float mdx = 0.f, mdy = 0.f, mdz = 0.f;  // direction of the flight that hit
+    float hnx = 0.f, hny = 0.f, hnz = 0.f;  // the mirror's half-vector normal toward the light
+    float nee_f = 0.f, nee_rsq = 1.f, nee_acc = 0.f, t_last = 0.f;
+    uint32_t o_last = 0;
+    unsigned pfl = 0, phit = 0, psh = 0, pvis = 0, pev = 0;
+
+    // Set up the lane's flight from ox..dz, near_t, far_t (finite, > near_t), first, the
+    // previous value/gradient, R.seed and normal_seed: ray frame, step rule and the first
+    // phase (conditioning at the origin, or f(nearT)).
+    auto begin_flight = [&](float jit) __attribute__((always_inline)) {
+        evals_flight = 0;
+        nee_stage = 0;
+        observed = false;
+        cache_valid = false;
+        R.cond = 0;
+        float ix = R.ia[0] * dx, iy = R.ia[1] * dy, iz = R.ia[2] * dz;
+        normalize3(ix, iy, iz);
+        R.F = make_frame(ix, iy, iz);
+        // finishGpisSegment step rule (SparseConvolutionNoiseMedium.cpp:289-291)
+        step = (far_t - near_t) / P.min_step_f;
+        if (P.step_size < step) step = P.step_size;
+        t0 = mul_add_exact(step, jit, near_t);
+        if (P.activate && !first) {
+            phase = PH_COND;
+            qx = ox; qy = oy; qz = oz;
+            if (corr) {
+                // evaluateGradientNoise1D(conditioning = false): mean 0, variance 1/2
+                double z0, z1;
+                normal_pair(normal_seed, 0, z0, z1);
+                const double sd = sqrt(0.5);
+                bx1 = (float)(0.0 + sd * z0);
+                by1 = (float)(0.0 + sd * z1);
+            } else if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
+                double x1, y1;
+                transverse_pair(normal_seed, 0, x1, y1);
+                bx1 = (float)x1; by1 = (float)y1;
+            }
+        } else {
+            phase = PH_F0;
+            ray_point(ox, oy, oz, dx, dy, dz, (double)near_t, qx, qy, qz);
+        }
+    };
+    auto end_path = [&]() __attribute__((always_inline)) {
+        pout.flights[ray] = pfl; pout.hits[ray] = phit;
+        pout.shadows[ray] = psh; pout.visible[ray] = pvis;
+        pout.nee[ray] = nee_acc; pout.t_last[ray] = t_last; pout.outcome_last[ray] = o_last;
+        if (pout.evals) pout.evals[ray] = pev;
+        phase = PH_NEED;
+    };
+    // A flight that ends before its march (empty interval or medium bounce limit).
+    auto no_march_flight = [&](uint32_t oc) __attribute__((always_inline)) {
+        if (pb == 0) {
+            pout.t0[ray] = far_t; pout.gx0[ray] = pout.gy0[ray] = pout.gz0[ray] = 0.f;
+            pout.outcome0[ray] = oc;
+        }
+        ++pfl;
+        t_last = far_t;
+        o_last = oc;
+        atomicAdd(&bstat[(oc & 0xFFu) == GPIS_EXIT ? ST_EXITS : ST_FAILS], 1u);
+        end_path();
+    };
+    // volumeLightSample's shadow flight (TraceBase.cpp:812-828): from the scattering point
+    // toward the light (Point::sampleDirect, Point.cpp:98-106) over min(distance, box
+    // exit), conditioned on (0, sg), with this flight's seeds.
+    auto start_shadow = [&](float sgx, float sgy, float sgz) __attribute__((always_inline)) {
+        float dlx = lx - hx, dly = ly - hy, dlz = lz - hz;
+        const float dist = sqrtf(dot3(dlx, dly, dlz, dlx, dly, dlz));
+        dlx = dlx / dist; dly = dly / dist; dlz = dlz / dist;
+        ox = hx; oy = hy; oz = hz;
+        dx = dlx; dy = dly; dz = dlz;
+        near_t = 0.f;
+        far_t = fminf(dist, box_exit(hx, hy, hz, dlx, dly, dlz, pin.box));
+        first = 0;
+        prev_v = 0.f; prev_gx = sgx; prev_gy = sgy; prev_gz = sgz;
+        Pcg g;
+        g.set((uint64_t)(uint32_t)(xxh4(ray, (uint32_t)pb, pin.bounce_seed, 1u) + 1u));
+        const float jit = u01(g.next());
+        if (far_t <= near_t) {
+            // transmittance over an empty segment is 1
+            ++psh; ++pvis;
+            nee_acc += nee_f / nee_rsq;
+            nee_stage = 3;
+        } else {
+            shadow = true;
+            begin_flight(jit);
+        }
+    };
 
     for (;;) {
         // ---- finish and refill, batched as in flight_kernel: lanes whose flight ended
@@ -832,7 +993,22 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
         const unsigned long long waiting = __ballot(phase == PH_DONE || phase == PH_NEED);
         const unsigned long long running = __ballot(phase != PH_DONE && phase != PH_NEED && phase != PH_DEAD);
         if (waiting && ((unsigned)__popcll(waiting) >= (unsigned)batch_k || !running)) {
-            if (phase == PH_DONE && nee_stage == 0) {
+            if (PATHS && phase == PH_DONE && nee_stage == 0 && shadow) {
+                // the shadow flight ended: any crossing occludes (resumeTransmittance,
+                // GaussianProcessMedium.cpp:394-421), otherwise f * e / pdf with e = 1
+                atomicAdd(&bstat[ST_FLIGHTS], 1u);
+                atomicAdd(&bstat[ST_EVALS], evals_flight);
+                atomicAdd(&bstat[ST_STEPS], (unsigned)k);
+                atomicAdd(&bstat[hit ? ST_HITS : ST_EXITS], 1u);
+                pev += evals_flight;
+                ++psh;
+                if (!hit) {
+                    ++pvis;
+                    nee_acc += nee_f / nee_rsq;
+                }
+                shadow = false;
+                nee_stage = 3;
+            } else if (phase == PH_DONE && nee_stage == 0) {
                 // evaluateGradientNoise1D at the endpoint (:204-242): transverse pair 0 when
                 // tSegment == 0, otherwise an independent second pair; resumeDistance rules
                 float tx2, ty2;
@@ -876,30 +1052,109 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
                 } else {
                     outcome = GPIS_EXIT;
                 }
-                if (use_light && (outcome & 0xFFu) == GPIS_HIT) {
-                    // one more evaluation at the scattering point sample.p = pos + t * dir (float)
-                    phase = PH_NEE;
-                    qx = mul_add_exact(t_end, dx, ox);
-                    qy = mul_add_exact(t_end, dy, oy);
-                    qz = mul_add_exact(t_end, dz, oz);
+                if (!PATHS) {
+                    if (use_light && (outcome & 0xFFu) == GPIS_HIT) {
+                        // one more evaluation at the scattering point sample.p = pos + t * dir (float)
+                        phase = PH_NEE;
+                        qx = mul_add_exact(t_end, dx, ox);
+                        qy = mul_add_exact(t_end, dy, oy);
+                        qz = mul_add_exact(t_end, dz, oz);
+                    } else {
+                        if (use_light) { nee.gx[ray] = nee.gy[ray] = nee.gz[ray] = 0.f; nee.pdf[ray] = 0.f; }
+                        nee_stage = 2;
+                    }
                 } else {
-                    if (use_light) { nee.gx[ray] = nee.gy[ray] = nee.gz[ray] = 0.f; nee.pdf[ray] = 0.f; }
-                    nee_stage = 2;
+                    const unsigned oc = outcome & 0xFFu;
+                    atomicAdd(&bstat[ST_FLIGHTS], 1u);
+                    atomicAdd(&bstat[ST_EVALS], evals_flight);
+                    atomicAdd(&bstat[ST_STEPS], (unsigned)k);
+                    atomicAdd(&bstat[oc == GPIS_HIT ? ST_HITS : (oc == GPIS_EXIT ? ST_EXITS : ST_FAILS)], 1u);
+                    pev += evals_flight;
+                    evals_flight = 0;
+                    ++pfl;
+                    if (pb == 0) {
+                        pout.t0[ray] = t_end; pout.gx0[ray] = gxv; pout.gy0[ray] = gyv; pout.gz0[ray] = gzv;
+                        pout.outcome0[ray] = outcome;
+                    }
+                    t_last = t_end;
+                    o_last = outcome;
+                    if (oc == GPIS_HIT) {
+                        ++phit;
+                        // finishDistance: sample.p = ray.pos() + sample.t * ray.dir() in float
+                        hx = mul_add_exact(t_end, dx, ox);
+                        hy = mul_add_exact(t_end, dy, oy);
+                        hz = mul_add_exact(t_end, dz, oz);
+                        hgx = gxv; hgy = gyv; hgz = gzv;
+                        mdx = dx; mdy = dy; mdz = dz;
+                        nee_stage = 3;  // the bounce, unless a light sample comes first
+                        if (use_light) {
+                            Frame3 Fh;
+                            float wix, wiy, wiz;
+                            hit_frame(hgx, hgy, hgz, mdx, mdy, mdz, Fh, wix, wiy, wiz);
+                            float dlx = lx - hx, dly = ly - hy, dlz = lz - hz;
+                            nee_rsq = dot3(dlx, dly, dlz, dlx, dly, dlz);
+                            const float dist = sqrtf(nee_rsq);
+                            dlx = dlx / dist; dly = dly / dist; dlz = dlz / dist;
+                            float wox = dot3(Fh.tx, Fh.ty, Fh.tz, dlx, dly, dlz);
+                            float woy = dot3(Fh.bx, Fh.by, Fh.bz, dlx, dly, dlz);
+                            float woz = dot3(Fh.nx, Fh.ny, Fh.nz, dlx, dly, dlz);
+                            normalize3(wox, woy, woz);
+                            if (pin.phase == GPIS_PHASE_MIRROR) {
+                                // samplingScheme at the endpoint (:68-76); a specular phase
+                                // under UNI takes no light sample (TraceBase.cpp:806-807)
+                                float sx, sy, sz;
+                                ray_point(ox, oy, oz, dx, dy, dz, tp, sx, sy, sz);
+                                const bool uni = P.scheme1d == GPIS_SCHEME_UNI || amp < 1e-6f ||
+                                                 (P.corr_xy && observed && sx == ox && sy == oy && sz == oz);
+                                if (!uni) {
+                                    // MirrorBsdf::eval / evalGrad: the half-vector normal in world space
+                                    const float nlx = (wix + wox) * 0.5f, nly = (wiy + woy) * 0.5f,
+                                                nlz = (wiz + woz) * 0.5f;
+                                    frame_to_global(Fh, nlx, nly, nlz, hnx, hny, hnz);
+                                    normalize3(hnx, hny, hnz);
+                                    phase = PH_NEE;
+                                    nee_stage = 0;
+                                    qx = hx; qy = hy; qz = hz;
+                                }
+                            } else if (wiz > 0.0f && woz > 0.0f) {
+                                nee_f = kInvPiF * woz;  // LambertBsdf::eval (LambertBsdf.cpp:40-47)
+                                start_shadow(hgx, hgy, hgz);
+                            }
+                        }
+                    } else {
+                        end_path();
+                    }
                 }
             } else if (phase == PH_DONE && nee_stage == 1) {
                 // mirror normal reflecting the ray toward the light, then neeGrad / neePDF
-                float wlx = lx - qx, wly = ly - qy, wlz = lz - qz;
-                normalize3(wlx, wly, wlz);
-                float hx = wlx - dx, hy = wly - dy, hz = wlz - dz;
-                normalize3(hx, hy, hz);
+                float hnx_, hny_, hnz_;
+                if (PATHS) {
+                    hnx_ = hnx; hny_ = hny; hnz_ = hnz;
+                } else {
+                    float wlx = lx - qx, wly = ly - qy, wlz = lz - qz;
+                    normalize3(wlx, wly, wlz);
+                    hnx_ = wlx - dx; hny_ = wly - dy; hnz_ = wlz - dz;
+                    normalize3(hnx_, hny_, hnz_);
+                }
                 double tmx = 0.0, tmy = 0.0, tvar = 0.5;
                 if (corr) transverse_moments(R, observed, obs_x, obs_y, ox, oy, oz, qx, qy, qz, tmx, tmy, tvar);
                 float ngx, ngy, ngz, pdf;
-                nee1d(P, R, hx, hy, hz, qx, qy, qz, ndv_nee, dx, dy, dz, corr, tmx, tmy, tvar, ngx, ngy, ngz, pdf);
-                nee.gx[ray] = ngx; nee.gy[ray] = ngy; nee.gz[ray] = ngz; nee.pdf[ray] = pdf;
-                nee_stage = 2;
+                nee1d(P, R, hnx_, hny_, hnz_, qx, qy, qz, ndv_nee, dx, dy, dz, corr, tmx, tmy, tvar, ngx, ngy, ngz,
+                      pdf);
+                if (!PATHS) {
+                    nee.gx[ray] = ngx; nee.gy[ray] = ngy; nee.gz[ray] = ngz; nee.pdf[ray] = pdf;
+                    nee_stage = 2;
+                } else {
+                    // the evaluation at the scattering point counts to the path
+                    atomicAdd(&bstat[ST_EVALS], evals_flight);
+                    pev += evals_flight;
+                    evals_flight = 0;
+                    nee_f = pdf;  // albedo 1 times neePDF (MirrorBsdf::eval)
+                    if (nee_f != 0.0f) start_shadow(ngx, ngy, ngz);
+                    else nee_stage = 3;
+                }
             }
-            if (phase == PH_DONE && nee_stage == 2) {
+            if (!PATHS && phase == PH_DONE && nee_stage == 2) {
                 const unsigned oc = outcome & 0xFFu;
                 atomicAdd(&bstat[ST_FLIGHTS], 1u);
                 atomicAdd(&bstat[ST_EVALS], evals_flight);
@@ -910,6 +1165,50 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
                 out.outcome[ray] = outcome;
                 if (out.evals) out.evals[ray] = evals_flight;
                 phase = PH_NEED;
+            }
+            if (PATHS && phase == PH_DONE && nee_stage == 3) {
+                // the phase sample and the next flight of the path (BRDFPhaseFunction::sample)
+                bool more = pb < pin.max_bounces;
+                float wx = 0.f, wy = 0.f, wz = 0.f, jit = 0.f;
+                if (more) {
+                    Frame3 Fh;
+                    float wix, wiy, wiz;
+                    hit_frame(hgx, hgy, hgz, mdx, mdy, mdz, Fh, wix, wiy, wiz);
+                    Pcg g;
+                    g.set((uint64_t)(uint32_t)(xxh4(ray, (uint32_t)(pb + 1), pin.bounce_seed, 0u) + 1u));
+                    const float u1 = u01(g.next()), u2 = u01(g.next());
+                    jit = u01(g.next());
+                    float ax, ay, az;
+                    if (pin.phase == GPIS_PHASE_MIRROR) {
+                        ax = -wix; ay = -wiy; az = wiz;  // MirrorBsdf::sample
+                    } else {
+                        // LambertBsdf::sample: SampleWarp::cosineHemisphere(next2D())
+                        more = wiz > 0.0f;
+                        const float ph = u1 * kTwoPiF, rr = sqrtf(u2);
+                        ax = cosf(ph) * rr;
+                        ay = sinf(ph) * rr;
+                        az = sqrtf(fmaxf(1.0f - u2, 0.0f));
+                    }
+                    frame_to_global(Fh, ax, ay, az, wx, wy, wz);
+                    normalize3(wx, wy, wz);
+                }
+                if (!more) {
+                    end_path();
+                } else {
+                    ++pb;
+                    ox = hx; oy = hy; oz = hz;
+                    dx = wx; dy = wy; dz = wz;
+                    near_t = 0.f;
+                    far_t = box_exit(hx, hy, hz, wx, wy, wz, pin.box);
+                    first = 0;
+                    prev_v = 0.f; prev_gx = hgx; prev_gy = hgy; prev_gz = hgz;
+                    path_seeds(P, pin, ray, w0 + (uint32_t)pb, R.seed, normal_seed);
+                    // beginDistance (GaussianProcessMedium.cpp:244-265): medium bounce limit,
+                    // empty interval
+                    if ((int)(w0 + (uint32_t)pb) >= P.max_bounce) no_march_flight(GPIS_FAIL | (GPIS_FAIL_BOUNCE_LIMIT << 8));
+                    else if (far_t <= near_t) no_march_flight(GPIS_EXIT);
+                    else begin_flight(jit);
+                }
             }
             const unsigned long long need = __ballot(phase == PH_NEED);
             if (need) {
@@ -929,24 +1228,36 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
                         dx = in.dx[r]; dy = in.dy[r]; dz = in.dz[r];
                         near_t = in.near_t[r]; far_t = in.far_t[r];
                         float jit = in.jitter[r];
-                        R.seed = in.noise_seed[r];
-                        normal_seed = in.normal_seed[r];
-                        R.cond = 0;
                         first = (in.flags[r] & 1u) ? 1 : 0;
                         if (!first) { prev_v = in.pv[r]; prev_gx = in.pgx[r]; prev_gy = in.pgy[r]; prev_gz = in.pgz[r]; }
-                        evals_flight = 0;
-                        nee_stage = 0;
-                        observed = false;
-                        cache_valid = false;
+                        if (PATHS) {
+                            w0 = pin.w0[r];
+                            pb = 0;
+                            shadow = false;
+                            pfl = phit = psh = pvis = pev = 0;
+                            nee_acc = 0.f;
+                            path_seeds(P, pin, r, w0, R.seed, normal_seed);
+                        } else {
+                            R.seed = in.noise_seed[r];
+                            normal_seed = in.normal_seed[r];
+                        }
                         ray_scales(P.aniso, P.l_aniso, P.l_aniso_inv, lscale, r, R.la, R.ia);
-                        float ix = R.ia[0] * dx, iy = R.ia[1] * dy, iz = R.ia[2] * dz;
-                        normalize3(ix, iy, iz);
-                        R.F = make_frame(ix, iy, iz);
                         if (!isfinite(far_t)) far_t = near_t + 2000.f;
                         bool bad = !isfinite(ox) || !isfinite(oy) || !isfinite(oz) || !isfinite(dx) || !isfinite(dy) ||
                                    !isfinite(dz) || !isfinite(near_t) || !(jit >= 0.f && jit < 1.f) ||
                                    !(R.la[0] > 0.f && R.la[1] > 0.f && R.la[2] > 0.f) || !isfinite(R.la[0] + R.la[1] + R.la[2]);
-                        if (bad || far_t <= near_t) {
+                        if (PATHS && bad) {
+                            pout.t0[r] = far_t; pout.gx0[r] = pout.gy0[r] = pout.gz0[r] = 0.f;
+                            pout.outcome0[r] = GPIS_INVALID;
+                            t_last = far_t;
+                            o_last = GPIS_INVALID;
+                            atomicAdd(&bstat[ST_INVALID], 1u);
+                            end_path();
+                        } else if (PATHS && (int)w0 >= P.max_bounce) {
+                            no_march_flight(GPIS_FAIL | (GPIS_FAIL_BOUNCE_LIMIT << 8));
+                        } else if (PATHS && far_t <= near_t) {
+                            no_march_flight(GPIS_EXIT);
+                        } else if (bad || far_t <= near_t) {
                             out.t[r] = far_t; out.value[r] = (bad || first) ? 0.f : prev_v;
                             out.gx[r] = out.gy[r] = out.gz[r] = 0.f;
                             out.outcome[r] = bad ? GPIS_INVALID : GPIS_EXIT;
@@ -955,28 +1266,7 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
                             atomicAdd(&bstat[bad ? ST_INVALID : ST_EXITS], 1u);
                             phase = PH_NEED;
                         } else {
-                            step = (far_t - near_t) / P.min_step_f;
-                            if (P.step_size < step) step = P.step_size;
-                            t0 = mul_add_exact(step, jit, near_t);
-                            if (P.activate && !first) {
-                                phase = PH_COND;
-                                qx = ox; qy = oy; qz = oz;
-                                if (corr) {
-                                    // evaluateGradientNoise1D(conditioning = false): mean 0, variance 1/2
-                                    double z0, z1;
-                                    normal_pair(normal_seed, 0, z0, z1);
-                                    const double sd = sqrt(0.5);
-                                    bx1 = (float)(0.0 + sd * z0);
-                                    by1 = (float)(0.0 + sd * z1);
-                                } else if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
-                                    double x1, y1;
-                                    transverse_pair(normal_seed, 0, x1, y1);
-                                    bx1 = (float)x1; by1 = (float)y1;
-                                }
-                            } else {
-                                phase = PH_F0;
-                                ray_point(ox, oy, oz, dx, dy, dz, (double)near_t, qx, qy, qz);
-                            }
+                            begin_flight(jit);
                         }
                     }
                 }
@@ -1054,12 +1344,27 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
                 double tn = lattice_t(t0, k, step);
                 if (k >= kMaxSteps) {
                     atomicAdd(&bstat[ST_INVALID], 1u);
-                    out.t[ray] = (float)tn; out.value[ray] = fv;
-                    out.gx[ray] = out.gy[ray] = out.gz[ray] = 0.f;
-                    out.outcome[ray] = GPIS_INVALID;
-                    if (out.evals) out.evals[ray] = evals_flight;
-                    if (use_light) { nee.gx[ray] = nee.gy[ray] = nee.gz[ray] = 0.f; nee.pdf[ray] = 0.f; }
-                    phase = PH_NEED;
+                    if (!PATHS) {
+                        out.t[ray] = (float)tn; out.value[ray] = fv;
+                        out.gx[ray] = out.gy[ray] = out.gz[ray] = 0.f;
+                        out.outcome[ray] = GPIS_INVALID;
+                        if (out.evals) out.evals[ray] = evals_flight;
+                        if (use_light) { nee.gx[ray] = nee.gy[ray] = nee.gz[ray] = 0.f; nee.pdf[ray] = 0.f; }
+                        phase = PH_NEED;
+                    } else {
+                        // step limit (RaySegmentStatus::StepLimit): the path ends invalid
+                        if (!shadow) {
+                            if (pb == 0) {
+                                pout.t0[ray] = (float)tn; pout.gx0[ray] = pout.gy0[ray] = pout.gz0[ray] = 0.f;
+                                pout.outcome0[ray] = GPIS_INVALID;
+                            }
+                            ++pfl;
+                        }
+                        pev += evals_flight;
+                        t_last = (float)tn;
+                        o_last = GPIS_INVALID;
+                        end_path();
+                    }
                 } else if (tn < (double)far_t) {
                     tq = tn;
                 } else {
@@ -1086,7 +1391,8 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
                 hit = 1;
                 t_end = (float)tp;
                 val_end = 0.f;
-                phase = PH_GRAD;
+                // a shadow flight needs only whether it crossed
+                phase = (PATHS && shadow) ? PH_DONE : PH_GRAD;
                 tq = tp;
                 unit = true;
             }
@@ -1094,7 +1400,7 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
             hit = 0;
             t_end = far_t;
             val_end = fv;
-            phase = PH_GRAD;
+            phase = (PATHS && shadow) ? PH_DONE : PH_GRAD;
             tq = (double)far_t;
             unit = true;
         } else if (phase == PH_GRAD) {
@@ -1394,10 +1700,57 @@ static gpis_status launch_flight1d(gpis_ctx *c, const gpis_ray_soa *rays, int64_
     long long want = (long long)c->num_cus * wpc * 64 / threads;
     long long need = (n + threads - 1) / threads;
     unsigned blocks = (unsigned)(need < want ? need : (want < 1 ? 1 : want));
-    hipLaunchKernelGGL((flight1d_kernel<8>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+    hipLaunchKernelGGL((flight1d_kernel<8, false>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
                        rays->length_scale, out, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
                        light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, nee, idx, n_dev,
-                       c->batch_k_1d);
+                       c->batch_k_1d, Path1DIn{}, Path1DOut{});
+    HIP_TRY(c, hipGetLastError());
+    return GPIS_OK;
+}
+
+gpis_status gpis_trace_paths_1d(gpis_ctx *c, const gpis_ray_soa *rays, const gpis_path_seeds *seeds, int64_t n,
+                                int32_t max_bounces, int32_t phase, uint32_t bounce_seed, float box,
+                                const float *light, const gpis_path1d_reply *reply, void *stream) {
+    if (!c) return GPIS_INVALID_ARGUMENT;
+    if (!c->P.one_d) return fail(c, GPIS_INVALID_ARGUMENT, "the medium is not 1D_sampling");
+    if (!rays || !seeds || !reply) return fail(c, GPIS_INVALID_ARGUMENT, "null argument");
+    if (n < 0 || n >= (1LL << 31)) return fail(c, GPIS_INVALID_ARGUMENT, "path count out of range");
+    if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz || !rays->near_t ||
+        !rays->far_t || !rays->jitter || !rays->flags || !rays->prev_value || !rays->prev_gx || !rays->prev_gy ||
+        !rays->prev_gz)
+        return fail(c, GPIS_INVALID_ARGUMENT, "null ray field");
+    if (!seeds->px || !seeds->py || !seeds->spp || !seeds->bounce)
+        return fail(c, GPIS_INVALID_ARGUMENT, "null seed field");
+    if (!reply->t0 || !reply->gx0 || !reply->gy0 || !reply->gz0 || !reply->outcome0 || !reply->flights ||
+        !reply->hits || !reply->shadows || !reply->visible || !reply->nee || !reply->t_last || !reply->outcome_last)
+        return fail(c, GPIS_INVALID_ARGUMENT, "null reply field");
+    if (max_bounces < 0 || max_bounces > 64) return fail(c, GPIS_INVALID_ARGUMENT, "max_bounces out of range");
+    if (phase != GPIS_PHASE_MIRROR && phase != GPIS_PHASE_LAMBERT)
+        return fail(c, GPIS_INVALID_ARGUMENT, "phase must be GPIS_PHASE_MIRROR or GPIS_PHASE_LAMBERT");
+    if (!(box > 0.f) || !std::isfinite(box)) return fail(c, GPIS_INVALID_ARGUMENT, "box must be positive");
+    if (light && !(std::isfinite(light[0]) && std::isfinite(light[1]) && std::isfinite(light[2])))
+        return fail(c, GPIS_INVALID_ARGUMENT, "light position must be finite");
+    if (c->P.kind != GPIS_COV_SE)
+        return fail(c, GPIS_UNSUPPORTED, "1D paths serve the stationary SE kernel");
+    if (c->P.N != 8) return fail(c, GPIS_UNSUPPORTED, "1D flights are compiled for 8 impulses per cell");
+    if (gpis_status s = mean_ready(c)) return s;
+    if (n == 0) return GPIS_OK;
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(c, hipMemsetAsync(c->work, 0, sizeof(unsigned), st));
+    unsigned threads = 256;
+    int wpc = c->waves_per_cu > 0 ? c->waves_per_cu : 8;
+    long long want = (long long)c->num_cus * wpc * 64 / threads;
+    long long need = (n + threads - 1) / threads;
+    unsigned blocks = (unsigned)(need < want ? need : (want < 1 ? 1 : want));
+    Path1DIn pin = {seeds->px, seeds->py, seeds->spp, seeds->bounce, xxh1(seeds->scene_seed), max_bounces, phase,
+                    bounce_seed, box};
+    Path1DOut pout = {reply->t0,      reply->gx0,   reply->gy0,    reply->gz0,          reply->outcome0,
+                      reply->flights, reply->hits,  reply->shadows, reply->visible,     reply->nee,
+                      reply->t_last,  reply->outcome_last, reply->evals};
+    hipLaunchKernelGGL((flight1d_kernel<8, true>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+                       rays->length_scale, ReplyOut{}, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
+                       light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, Nee1DOut{}, nullptr,
+                       nullptr, c->batch_k_1d, pin, pout);
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }

@@ -436,6 +436,38 @@ class GpisDevice:
                 self.h, "gpis_trace_flights_1d")
         return reply
 
+    def trace_paths_1d(self, rays: RayBatch, pss, scene_seed: int, max_bounces: int, phase: str = "mirror",
+                       bounce_seed: int = 7, box: float = 1.0, light=None):
+        """Device-resident paths of a 1D_sampling medium (gpis_trace_paths_1d): each ray
+        starts a path whose hits run the hit-time phase ("mirror" or "lambert" BSDF of
+        albedo 1) and, with `light`, the light sample with its shadow flight toward that
+        point light.  `pss` holds each path's pixelSampleSegment (n x 4, uint32): flight
+        b uses (px, py, spp, w + b) and `scene_seed` for its seeds."""
+        import torch
+        kind = {"mirror": A.PHASE_MIRROR, "lambert": A.PHASE_LAMBERT}.get(phase)
+        if kind is None:
+            raise MediumConfigError(f"unknown phase {phase!r} (mirror, lambert)")
+        p = np.ascontiguousarray(np.asarray(pss, np.uint32).reshape(rays.n, 4))
+        dev = rays.t["ox"].device
+        cols = [torch.from_numpy(np.ascontiguousarray(p[:, i]).view(np.int32)).to(dev) for i in range(4)]
+        seeds = A.PathSeeds(cols[0].data_ptr(), cols[1].data_ptr(), cols[2].data_ptr(), cols[3].data_ptr(),
+                            int(scene_seed) & 0xFFFFFFFF)
+        f = dict(dtype=torch.float32, device=dev)
+        i = dict(dtype=torch.int32, device=dev)
+        reply = {k: torch.empty(rays.n, **f) for k in ("t0", "gx0", "gy0", "gz0", "nee", "t_last")}
+        reply.update({k: torch.empty(rays.n, **i) for k in ("outcome0", "flights", "hits", "shadows", "visible",
+                                                            "outcome_last", "evals")})
+        r = A.Path1DReply()
+        for k, _ in A.Path1DReply._fields_:
+            setattr(r, k, reply[k].data_ptr())
+        soa = rays.soa()
+        lp = None if light is None else (C.c_float * 3)(*[float(x) for x in light])
+        A.check(self.L.gpis_trace_paths_1d(self.h, C.byref(soa), C.byref(seeds), rays.n, max_bounces, kind,
+                                           bounce_seed & 0xFFFFFFFF, box, lp, C.byref(r), self._stream()),
+                self.h, "gpis_trace_paths_1d")
+        reply["_seeds"] = cols  # device inputs stay referenced until the caller synchronises
+        return reply
+
     def alloc_path_reply(self, n, device="cuda"):
         import torch
         f = dict(dtype=torch.float32, device=device)

@@ -92,7 +92,8 @@ typedef struct gpis_ray_soa {
 
 /* Outcomes of one flight (DistanceStatus and sample.exited, GaussianProcessMedium.cpp:289-369). */
 enum { GPIS_EXIT = 0, GPIS_HIT = 1, GPIS_FAIL = 2, GPIS_INVALID = 3 };
-enum { GPIS_FAIL_NONE = 0, GPIS_FAIL_OUTWARD = 1, GPIS_FAIL_ZERO_GRAD = 2, GPIS_FAIL_NONFINITE = 3 };
+enum { GPIS_FAIL_NONE = 0, GPIS_FAIL_OUTWARD = 1, GPIS_FAIL_ZERO_GRAD = 2, GPIS_FAIL_NONFINITE = 3,
+       GPIS_FAIL_BOUNCE_LIMIT = 4 /* the medium's max_bounces (beginDistance, GaussianProcessMedium.cpp:252-255) */ };
 
 typedef struct gpis_reply_soa {
     float *t;                         /* sample distance; farT on exit */
@@ -172,6 +173,49 @@ typedef struct gpis_nee_out {
 gpis_status gpis_trace_flights_1d(gpis_ctx *ctx, const gpis_ray_soa *rays, int64_t n,
                                   const gpis_reply_soa *reply, const float *light,
                                   const gpis_nee_out *nee, void *stream);
+
+/* Device-resident paths of a 1D_sampling medium: the hit-time phase and Renewal+
+ * chaining of PathTracer's medium branch (PathTracer.cpp:380-438) without host round
+ * trips.  Path i starts with the flight of rays[i]; at every hit the kernel
+ *   - forms the scattering frame of BRDFPhaseFunction::setEventIsectInfo
+ *     (BRDFPhaseFunction.cpp:29-49: TangentFrame of the normalised gradient);
+ *   - when `light` is non-NULL, takes volumeLightSample's light sample toward the point
+ *     light at light[0..2] (TraceBase.cpp:798-839, Point::sampleDirect Point.cpp:98-106,
+ *     emission 1): f = the phase's eval (mirror: MirrorBsdf::eval = neePDF of the
+ *     half-vector normal, none under the UNI sampling scheme; Lambert: cos/pi), then a
+ *     shadow flight over min(distance, box exit) conditioned on (0, neeGrad) (mirror) or
+ *     (0, gradient) (Lambert) whose any crossing occludes (resumeTransmittance,
+ *     GaussianProcessMedium.cpp:394-421); an unoccluded sample adds f / |light - p|^2;
+ *   - samples the phase (MirrorBsdf::sample, MirrorBsdf.cpp:30-49, or LambertBsdf::sample,
+ *     LambertBsdf.cpp:27-38, albedo 1) and traces the next flight from the hit to the
+ *     exit of [-box, box]^3, conditioned on (0, gradient) (prepareGpisSegment,
+ *     SparseConvolutionNoiseMedium.cpp:199-241), with pixelSampleSegment.w one higher.
+ * A path ends at an exit, a failure, a refused Lambert sample or after 1 + max_bounces
+ * flights (PathTracer's maxBounces = 1 + max_bounces).  Flight b of path i uses the
+ * seeds of pixelSampleSegment (px, py, spp, bounce + b) and scene_seed (computeSeed
+ * SparseConvolutionNoise.cpp:57-66, prepareGpisSegment :235-238): rays->noise_seed and
+ * normal_seed are not read.  The path sampler's draws are PCG32 streams seeded by
+ * xxhash32(i, b, bounce_seed, s) + 1: stream 0 of flight b >= 1 gives (u1, u2, jitter)
+ * of its phase sample and flight, stream 1 of flight b the jitter of its shadow flight.
+ * SE kernel only; rays->length_scale may give each path its own length scale. */
+enum { GPIS_PHASE_MIRROR = 0, GPIS_PHASE_LAMBERT = 1 };
+typedef struct gpis_path_seeds {
+    const uint32_t *px, *py, *spp, *bounce;  /* RayInfo::pixelSampleSegment of each path's first flight */
+    uint32_t scene_seed;                     /* Scene::_seed */
+} gpis_path_seeds;
+typedef struct gpis_path1d_reply {
+    float *t0, *gx0, *gy0, *gz0;      /* first flight: t and gradient */
+    uint32_t *outcome0;
+    uint32_t *flights, *hits;         /* path flights, hits among them */
+    uint32_t *shadows, *visible;      /* shadow flights, unoccluded among them */
+    float *nee;                       /* sum of f / |light - p|^2 over unoccluded light samples */
+    float *t_last;                    /* last path flight: t and outcome */
+    uint32_t *outcome_last;
+    uint32_t *evals;                  /* field evaluations of the path, shadow flights included (may be NULL) */
+} gpis_path1d_reply;
+gpis_status gpis_trace_paths_1d(gpis_ctx *ctx, const gpis_ray_soa *rays, const gpis_path_seeds *seeds, int64_t n,
+                                int32_t max_bounces, int32_t phase, uint32_t bounce_seed, float box,
+                                const float *light, const gpis_path1d_reply *reply, void *stream);
 
 /* A tabulated mean on a dense voxel grid: the fields of the reference's HostMeanGrid
  * (src/gpu/cuda/RaySegmentBatch.hpp:19-30; made from a VDB grid by makeHostMeanGrid,

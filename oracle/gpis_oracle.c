@@ -1252,3 +1252,156 @@ void orc_conditioned_eval_ls(const orc_medium_desc *desc, const orc_ray *rays, c
         orc_field_destroy(f);
     }
 }
+
+/* ===================================================================== */
+/* 1D paths: flights chained through the hit-time phase                  */
+/* ===================================================================== */
+
+/* Exit distance of o + t d from the box [-b, b]^3 (the harness's scene bound). */
+static float box_exit_f(v3f o, v3f d, float b) {
+    float tx = d.x != 0.f ? fmaxf((b - o.x) / d.x, (-b - o.x) / d.x) : 3.4e38f;
+    float ty = d.y != 0.f ? fmaxf((b - o.y) / d.y, (-b - o.y) / d.y) : 3.4e38f;
+    float tz = d.z != 0.f ? fmaxf((b - o.z) / d.z, (-b - o.z) / d.z) : 3.4e38f;
+    return fmaxf(fminf(tx, fminf(ty, tz)), 0.f);
+}
+static void path_stream(pcg_t *g, uint32_t path, uint32_t b, uint32_t seed, uint32_t stream) {
+    pcg_set(g, (uint64_t)(uint32_t)(orc_xxhash32_4(path, b, seed, stream) + 1u));
+}
+static void trace_path1d(const orc_field *f, const orc_ray *ray0, uint32_t path, int max_bounces, int phase,
+                         uint32_t bseed, float box, const float *light, orc_path *out) {
+    const float INV_PI_F = 1.0f / PI_F, TWO_PI_F = PI_F * 2.0f; /* Angle.hpp:8-14 */
+    memset(out, 0, sizeof *out);
+    orc_ray ray = *ray0;
+    const uint32_t w0 = ray0->pss[3];
+    for (int b = 0;; ++b) {
+        ray.pss[3] = w0 + (uint32_t)b;
+        ray.bounce = (int32_t)(w0 + (uint32_t)b);
+        orc_flight fl;
+        orc_trace_flight(f, &ray, &fl);
+        out->flights++;
+        out->evals += fl.evals;
+        if (b == 0) {
+            out->t0 = (float)fl.t;
+            out->grad0[0] = fl.grad[0]; out->grad0[1] = fl.grad[1]; out->grad0[2] = fl.grad[2];
+            out->outcome0 = fl.outcome; out->fail0 = fl.fail_reason;
+        }
+        out->t_last = (float)fl.t;
+        out->outcome_last = fl.outcome; out->fail_last = fl.fail_reason;
+        for (int a = 0; a < 3; ++a) { out->pos_last[a] = ray.origin[a]; out->dir_last[a] = ray.dir[a]; }
+        if (fl.outcome != ORC_HIT) break;
+        out->hits++;
+        /* finishDistance: sample.p = ray.pos() + sample.t * ray.dir() in float */
+        const float t = (float)fl.t;
+        const v3f o = V3(ray.origin[0], ray.origin[1], ray.origin[2]);
+        const v3f d = V3(ray.dir[0], ray.dir[1], ray.dir[2]);
+        const v3f p = vadd(o, vscale(t, d));
+        const v3f g = V3(fl.grad[0], fl.grad[1], fl.grad[2]);
+        /* setEventIsectInfo: TangentFrame(Vec3f(aniso.normalized())), wi = toLocal(-w).normalized() */
+        const frame_t F = make_frame(f_of(dnorm(d_of(g))));
+        const v3f wi = vnorm(to_local(&F, V3(-d.x, -d.y, -d.z)));
+        if (light) {
+            v3f dl = V3(light[0] - p.x, light[1] - p.y, light[2] - p.z);
+            const float rsq = vlen2(dl);
+            const float dist = sqrtf(rsq);
+            dl = vdivs(dl, dist);
+            float fv = 0.f;
+            v3f sg = g;
+            const v3f wo = vnorm(to_local(&F, dl));
+            if (phase == ORC_PHASE_MIRROR) {
+                /* specular with the UNI scheme: no light sample (TraceBase.cpp:806-807) */
+                if (fl.scheme != ORC_SCHEME_UNI) {
+                    /* MirrorBsdf::eval / evalGrad: the half-vector normal in world space */
+                    const v3f nl = vscale(0.5f, vadd(wi, wo));
+                    const v3f h = vnorm(to_global(&F, nl));
+                    const float hh[3] = {h.x, h.y, h.z}, pp[3] = {p.x, p.y, p.z};
+                    float gg[3], pdf;
+                    orc_nee(f, &ray, hh, pp, t, gg, &pdf);
+                    fv = pdf;
+                    sg = V3(gg[0], gg[1], gg[2]);
+                }
+            } else if (wi.z > 0.0f && wo.z > 0.0f) {
+                fv = INV_PI_F * wo.z; /* LambertBsdf::eval */
+            }
+            if (fv != 0.0f) {
+                orc_ray sh = ray;
+                sh.origin[0] = p.x; sh.origin[1] = p.y; sh.origin[2] = p.z;
+                sh.dir[0] = dl.x; sh.dir[1] = dl.y; sh.dir[2] = dl.z;
+                sh.near_t = 0.0f;
+                sh.far_t = fminf(dist, box_exit_f(p, dl, box));
+                sh.first_scatter = 0;
+                sh.last_val = 0.0f;
+                sh.last_aniso[0] = sg.x; sh.last_aniso[1] = sg.y; sh.last_aniso[2] = sg.z;
+                sh.bounce = 0; /* transmittance has no bounce limit */
+                pcg_t gs;
+                path_stream(&gs, path, (uint32_t)b, bseed, 1u);
+                sh.jitter = pcg_1d(&gs);
+                orc_flight sf;
+                orc_trace_flight(f, &sh, &sf);
+                out->shadows++;
+                out->evals += sf.evals;
+                if (sf.outcome == ORC_EXIT) { /* any crossing occludes */
+                    out->visible++;
+                    out->nee += fv / rsq;
+                }
+            }
+        }
+        if (b == max_bounces) break;
+        /* phase sample of the next flight */
+        pcg_t gb;
+        path_stream(&gb, path, (uint32_t)(b + 1), bseed, 0u);
+        const float u1 = pcg_1d(&gb), u2 = pcg_1d(&gb), jit = pcg_1d(&gb);
+        v3f wl;
+        if (phase == ORC_PHASE_MIRROR) {
+            wl = V3(-wi.x, -wi.y, wi.z); /* MirrorBsdf::sample */
+        } else {
+            if (wi.z <= 0.0f) break; /* LambertBsdf::sample refuses */
+            const float phi = u1 * TWO_PI_F, r = sqrtf(u2); /* SampleWarp::cosineHemisphere */
+            wl = V3(cosf(phi) * r, sinf(phi) * r, sqrtf(fmaxf(1.0f - u2, 0.0f)));
+        }
+        const v3f w = vnorm(to_global(&F, wl));
+        ray.origin[0] = p.x; ray.origin[1] = p.y; ray.origin[2] = p.z;
+        ray.dir[0] = w.x; ray.dir[1] = w.y; ray.dir[2] = w.z;
+        ray.near_t = 0.0f;
+        ray.far_t = box_exit_f(p, w, box);
+        ray.jitter = jit;
+        ray.first_scatter = 0;
+        ray.last_val = 0.0f;
+        ray.last_aniso[0] = g.x; ray.last_aniso[1] = g.y; ray.last_aniso[2] = g.z;
+    }
+}
+
+typedef struct {
+    const orc_medium_desc *d; const orc_ray *rays; const float *ls; orc_path *out; int64_t n0, n1;
+    int max_bounces, phase; uint32_t bseed; float box; const float *light;
+} path_job_t;
+static void *path_worker(void *arg) {
+    path_job_t *j = (path_job_t *)arg;
+    orc_field *shared = j->ls ? NULL : orc_field_create(j->d);
+    for (int64_t i = j->n0; i < j->n1; ++i) {
+        orc_field *f = shared;
+        if (!f) {
+            orc_medium_desc d = *j->d;
+            d.length_scale = j->ls[i];
+            f = orc_field_create(&d);
+        }
+        trace_path1d(f, &j->rays[i], (uint32_t)i, j->max_bounces, j->phase, j->bseed, j->box, j->light, &j->out[i]);
+        if (!shared) orc_field_destroy(f);
+    }
+    if (shared) orc_field_destroy(shared);
+    return NULL;
+}
+void orc_trace_paths1d(const orc_medium_desc *desc, const orc_ray *rays, const float *ls, int64_t n,
+                       int32_t max_bounces, int32_t phase, uint32_t bounce_seed, float box,
+                       const float *light, orc_path *out, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256]; path_job_t jb[256];
+    for (int t = 0; t < nthreads; ++t) {
+        jb[t].d = desc; jb[t].rays = rays; jb[t].ls = ls; jb[t].out = out;
+        jb[t].max_bounces = max_bounces; jb[t].phase = phase; jb[t].bseed = bounce_seed;
+        jb[t].box = box; jb[t].light = light;
+        jb[t].n0 = n * t / nthreads; jb[t].n1 = n * (t + 1) / nthreads;
+        pthread_create(&th[t], NULL, path_worker, &jb[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+}

@@ -160,6 +160,41 @@ int orc_transverse_moments(const orc_field *f, const orc_ray *ray, const float p
 void orc_nee(const orc_field *f, const orc_ray *ray, const float normal[3],
              const float p[3], float t_segment, float grad_out[3], float *pdf_out);
 
+/* ---- 1D paths: flights chained through the hit-time phase ------------ */
+/* A path of a 1D_sampling medium inside the box [-box, box]^3 lit by one point light
+ * (Point::sampleDirect, src/core/primitives/Point.cpp:98-106, emission 1), following
+ * PathTracer's medium branch (PathTracer.cpp:380-438): flight b of the path has
+ * pixelSampleSegment.w = w0 + b (w0 = pss[3] of the path's first ray); at a hit the
+ * phase is BRDFPhaseFunction (BRDFPhaseFunction.cpp:29-100) with a mirror
+ * (MirrorBsdf.cpp:30-113) or Lambert (LambertBsdf.cpp:27-47) BSDF of albedo 1;
+ * next-event estimation is volumeLightSample (TraceBase.cpp:798-839): f = phase.eval
+ * (the mirror's neePDF of the half-vector normal, zero under the UNI scheme), a
+ * shadow flight conditioned on (0, neeGrad) (mirror) or (0, hit gradient) (Lambert)
+ * toward the light over min(distance, box exit) whose any crossing occludes
+ * (resumeTransmittance, GaussianProcessMedium.cpp:394-421), and f / |light - p|^2 is
+ * added when unoccluded.  The next flight starts at the hit conditioned on (0,
+ * gradient) along the phase-sampled direction to the box exit.  The path sampler's
+ * draws are replaced by PCG32 streams seeded by xxhash32(path, b, bounce_seed, s) + 1:
+ * stream 0 of flight b >= 1 gives (u1, u2, jitter) of its phase sample and flight,
+ * stream 1 of flight b the jitter of its shadow flight. */
+enum { ORC_PHASE_MIRROR = 0, ORC_PHASE_LAMBERT = 1 };
+typedef struct orc_path {
+    float t0, grad0[3];       /* first flight */
+    int32_t outcome0, fail0;
+    int32_t flights, hits;    /* path flights, hits among them */
+    int32_t shadows, visible; /* shadow flights, unoccluded among them */
+    float nee;                /* sum over scattering events of f * V / |light - p|^2 */
+    float t_last;             /* last path flight */
+    int32_t outcome_last, fail_last;
+    int32_t evals;            /* field evaluations, shadow flights included */
+    float pos_last[3], dir_last[3]; /* origin and direction of the last path flight */
+} orc_path;
+/* Up to 1 + max_bounces flights per path (PathTracer's maxBounces = 1 + max_bounces);
+ * ls: optional per-path SE length scale; light: NULL for no next-event estimation. */
+void orc_trace_paths1d(const orc_medium_desc *desc, const orc_ray *rays, const float *ls, int64_t n,
+                       int32_t max_bounces, int32_t phase, uint32_t bounce_seed, float box,
+                       const float *light, orc_path *out, int nthreads);
+
 /* Per-ray SE length scale (1D media whose rays each draw their own l): the same
  * flights and NEE as above, each ray on the field of `desc` with length_scale =
  * ls[i].  `normals`/`points`/`t_segment` are per ray (3, 3, 1 values). */

@@ -21,6 +21,7 @@ CTX_GLOBAL, CTX_RENEWAL, CTX_RENEWAL_PLUS, CTX_NONE = 0, 1, 2, 3
 SCHEME_UNI, SCHEME_NEE, SCHEME_MIS = 0, 1, 2
 LS_CONST, LS_SINCOS = 0, 1
 EXIT, HIT, FAIL = 0, 1, 2
+FAIL_NONE, FAIL_OUTWARD, FAIL_ZERO_GRAD, FAIL_NONFINITE, FAIL_BOUNCE_LIMIT = 0, 1, 2, 3, 4
 
 
 class MediumDesc(C.Structure):
@@ -67,6 +68,12 @@ FLIGHT_DTYPE = np.dtype([
     ("t", "<f8"), ("value", "<f4"), ("grad", "<f4", 3), ("outcome", "<i4"), ("fail_reason", "<i4"),
     ("steps", "<i4"), ("evals", "<i4"), ("scheme", "<i4"), ("min_abs_f", "<f4"),
     ("bracket", "<f4", 2)], align=True)
+PATH_DTYPE = np.dtype([
+    ("t0", "<f4"), ("grad0", "<f4", 3), ("outcome0", "<i4"), ("fail0", "<i4"),
+    ("flights", "<i4"), ("hits", "<i4"), ("shadows", "<i4"), ("visible", "<i4"),
+    ("nee", "<f4"), ("t_last", "<f4"), ("outcome_last", "<i4"), ("fail_last", "<i4"),
+    ("evals", "<i4"), ("pos_last", "<f4", 3), ("dir_last", "<f4", 3)], align=True)
+PHASE_MIRROR, PHASE_LAMBERT = 0, 1
 assert RAY_DTYPE.itemsize == C.sizeof(Ray), (RAY_DTYPE.itemsize, C.sizeof(Ray))
 assert FLIGHT_DTYPE.itemsize == C.sizeof(Flight), (FLIGHT_DTYPE.itemsize, C.sizeof(Flight))
 
@@ -106,6 +113,8 @@ def lib():
         L.orc_trace_flight.argtypes = [vp, C.POINTER(Ray), C.POINTER(Flight)]
         L.orc_trace_flights.argtypes = [vp, vp, C.c_int64, vp, C.c_int]
         L.orc_trace_flights_ls.argtypes = [vp, vp, vp, C.c_int64, vp, C.c_int]
+        L.orc_trace_paths1d.argtypes = [vp, vp, vp, C.c_int64, C.c_int32, C.c_int32, u32, C.c_float, vp, vp,
+                                        C.c_int]
         L.orc_nee_ls.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int64, vp, vp]
         L.orc_conditioned_eval_ls.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp]
         L.orc_trace_flight_perturbed.argtypes = [vp, C.POINTER(Ray), C.c_float, C.c_uint32, C.POINTER(Flight)]
@@ -343,6 +352,19 @@ def trace_ls(desc: MediumDesc, rays: np.ndarray, ls, nthreads=1) -> np.ndarray:
     lsa = None if ls is None else np.ascontiguousarray(ls, np.float32)
     lib().orc_trace_flights_ls(C.byref(desc), rays.ctypes.data, None if lsa is None else lsa.ctypes.data,
                                rays.shape[0], out.ctypes.data, nthreads)
+    return out
+
+
+def trace_paths1d(desc: MediumDesc, rays: np.ndarray, ls, max_bounces, phase, bounce_seed, box, light=None,
+                  nthreads=1) -> np.ndarray:
+    """1D paths (orc_trace_paths1d): path i's draws use path index i."""
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    out = np.zeros(rays.shape[0], PATH_DTYPE)
+    lsa = None if ls is None else np.ascontiguousarray(ls, np.float32)
+    la = None if light is None else np.ascontiguousarray(light, np.float32)
+    lib().orc_trace_paths1d(C.byref(desc), rays.ctypes.data, None if lsa is None else lsa.ctypes.data,
+                            rays.shape[0], int(max_bounces), int(phase), int(bounce_seed) & 0xFFFFFFFF, float(box),
+                            None if la is None else la.ctypes.data, out.ctypes.data, nthreads)
     return out
 
 
