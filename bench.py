@@ -8,10 +8,11 @@ the box [-1,1]^3, each traced as a path of one primary flight plus up to 2
 cosine-weighted diffuse bounces (bounce seed 7), all in one persistent launch.
 Inputs are generated on the host, copied to HBM before timing, and stay there.
 
-The same line carries, under "plain_1d" and "surrogate", the 1D Renewal+ flights
-with NEE (2,097,152 rays) and the learned surrogate against the plain method
-(trained on 2^24 plain flights, served on a 512x512 image at 16 spp), each with
-its own timing.
+The same line carries, under "plain_1d", "paths_1d" and "surrogate", the 1D
+Renewal+ flights with NEE (2,097,152 rays), the device-resident 1D mirror paths
+with light samples started from the same rays, and the learned surrogate against
+the plain method (trained on 2^24 plain flights, served on a 512x512 image at 16
+spp), each with its own timing.
 
 Prints one JSON line (see DESIGN.md for the field definitions).
 """
@@ -206,6 +207,97 @@ def run_1d(args):
     return line
 
 
+def cpu_baseline_paths_1d(med, rays, light, max_bounces, max_seconds=12.0):
+    """The oracle's path restatement on a bounded uniform sample of the same paths."""
+    from oracle import pyoracle as po
+    from tests.helpers import oracle_desc, oracle_rays
+    from gpis_extend_amd import workloads as W
+
+    desc = oracle_desc(med)
+    cores = min(os.cpu_count() or 1, 64)
+    n_total = rays["ox"].shape[0]
+    order = np.random.default_rng(12345).permutation(n_total)
+    sample, paths, spent, start_idx = 8192, 0, 0.0, 0
+    while spent < max_seconds and start_idx < n_total:
+        idx = np.sort(order[start_idx:start_idx + sample])
+        start_idx += len(idx)
+        cur = W.rays_subset(rays, idx)
+        orr = oracle_rays(cur, med)
+        t0 = time.perf_counter()
+        po.trace_paths1d(desc, orr, cur["length_scale"], max_bounces, po.PHASE_MIRROR, 7, 1.0, light,
+                         nthreads=cores)
+        spent += time.perf_counter() - t0
+        paths += len(idx)
+    return {"value": paths / spent, "unit": "paths/s", "cores": cores, "kind": "port",
+            "sample": f"{paths} mirror paths of the same workload through the C oracle's path restatement "
+                      f"on {cores} host threads, {spent:.1f} s"}
+
+
+def run_paths_1d(args):
+    """Device-resident 1D paths on the renewal workload: each ray starts a mirror path of
+    up to 5 flights; every hit takes a light sample (NEE normal, pdf and a conditioned
+    shadow flight) and bounces, all inside one launch (one step = one launch)."""
+    import torch
+    from gpis_extend_amd import GpisDevice, RayBatch, SparseConvolutionNoiseMedium, workloads as W
+
+    med = SparseConvolutionNoiseMedium.from_json(W.config3_json())
+    rays = W.renewal1d_rays(med, n=args.rays_1d)
+    n = rays["ox"].shape[0]
+    dev = GpisDevice(med)
+    batch = RayBatch.from_numpy(rays)
+    stream = torch.cuda.current_stream()
+    max_bounces = 4
+
+    seeds = dev.path_seeds(rays["pss"])
+    reply = dev.alloc_path1d_reply(n)
+
+    def step():
+        return dev.trace_paths_1d(batch, seeds, rays["scene_seed"], max_bounces, phase="mirror",
+                                  bounce_seed=7, box=1.0, light=W.LIGHT_POS, reply=reply)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    dev.reset_counters()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep = None
+    for i in range(args.steps):
+        starts[i].record(stream)
+        rep = step()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(starts, ends)]))
+    c = dev.counters()
+    flights = int(rep["flights"].sum().item())
+    shadows = int(rep["shadows"].sum().item())
+    evals_per_step = c["evals"] / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    fpe = flops_per_eval_1d(int(med.impulse_density))
+    achieved = evals_per_step * fpe / (kernel_ms / 1e3) / 1e12
+    line = {
+        "metric": "GPIS 1D paths/s on 1 GPU (mirror paths of up to 5 flights, light sample at each hit)",
+        "value": n / (ms_per_step / 1e3), "unit": "paths/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "1D Renewal+ mirror paths from the renewal workload's rays: up to 1 + 4 flights, "
+                               "each hit with neeGrad/neePDF toward (0, 2, 0) and a conditioned shadow flight, "
+                               "bounces inside [-1, 1]^3",
+                   "paths": n, "flights_per_step": flights, "shadow_flights_per_step": shadows,
+                   "evals_per_step": evals_per_step,
+                   "flights_per_s": (flights + shadows) / (ms_per_step / 1e3)},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "flight1d_kernel<8,true>", "kernel_ms": kernel_ms, "flops_per_eval": fpe},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline_paths_1d(med, rays, W.LIGHT_POS, max_bounces)
+    dev.close()
+    return line
+
+
 MLP_FLOPS_PER_ROW = 2 * (8 * 128 + 3 * 128 * 128 + 128 * 32)  # 108,544 (forward)
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 
@@ -325,8 +417,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--width", type=int, default=2048)
-    ap.add_argument("--workload", choices=("c1", "c3", "c4"), default="c1",
+    ap.add_argument("--workload", choices=("c1", "c3", "c3paths", "c4"), default="c1",
                     help="c1: camera-ray paths on the 3D field (default); c3: 1D Renewal+ flights with NEE; "
+                         "c3paths: device-resident 1D mirror paths with light samples; "
                          "c4: the learned surrogate against the plain method")
     ap.add_argument("--train-flights", type=int, default=1 << 24)
     ap.add_argument("--epochs", type=int, default=20)
@@ -348,6 +441,9 @@ def main():
 
     if args.workload == "c3":
         print(json.dumps(run_1d(args)), flush=True)
+        return
+    if args.workload == "c3paths":
+        print(json.dumps(run_paths_1d(args)), flush=True)
         return
     if args.workload == "c4":
         print(json.dumps(run_surrogate(args)), flush=True)
@@ -437,7 +533,7 @@ def main():
         keep = ("value", "unit", "ms_per_step", "config", "roofline", "cpu_baseline", "plain", "speedup_vs_plain",
                 "fallback_fraction", "ks_vs_plain", "pit_ks", "image_rel_error", "image_rel_error_plain_vs_plain",
                 "train", "metric", "dtype")
-        for key, fn in (("plain_1d", run_1d), ("surrogate", run_surrogate)):
+        for key, fn in (("plain_1d", run_1d), ("paths_1d", run_paths_1d), ("surrogate", run_surrogate)):
             sub = fn(args)
             line[key] = {k: v for k, v in sub.items() if k in keep}
     print(json.dumps(line), flush=True)

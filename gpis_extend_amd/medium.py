@@ -436,27 +436,42 @@ class GpisDevice:
                 self.h, "gpis_trace_flights_1d")
         return reply
 
+    @staticmethod
+    def path_seeds(pss, device="cuda"):
+        """Upload each path's pixelSampleSegment (n x 4, uint32) as the four device
+        columns gpis_trace_paths_1d reads."""
+        import torch
+        p = np.ascontiguousarray(np.asarray(pss, np.uint32).reshape(-1, 4))
+        return [torch.from_numpy(np.ascontiguousarray(p[:, i]).view(np.int32)).to(device) for i in range(4)]
+
+    def alloc_path1d_reply(self, n, device="cuda"):
+        import torch
+        f = dict(dtype=torch.float32, device=device)
+        i = dict(dtype=torch.int32, device=device)
+        reply = {k: torch.empty(n, **f) for k in ("t0", "gx0", "gy0", "gz0", "nee", "t_last")}
+        reply.update({k: torch.empty(n, **i) for k in ("outcome0", "flights", "hits", "shadows", "visible",
+                                                       "outcome_last", "evals")})
+        return reply
+
     def trace_paths_1d(self, rays: RayBatch, pss, scene_seed: int, max_bounces: int, phase: str = "mirror",
-                       bounce_seed: int = 7, box: float = 1.0, light=None):
+                       bounce_seed: int = 7, box: float = 1.0, light=None, reply=None):
         """Device-resident paths of a 1D_sampling medium (gpis_trace_paths_1d): each ray
         starts a path whose hits run the hit-time phase ("mirror" or "lambert" BSDF of
         albedo 1) and, with `light`, the light sample with its shadow flight toward that
-        point light.  `pss` holds each path's pixelSampleSegment (n x 4, uint32): flight
-        b uses (px, py, spp, w + b) and `scene_seed` for its seeds."""
-        import torch
+        point light.  `pss` holds each path's pixelSampleSegment, as an n x 4 uint32
+        array or as the device columns of path_seeds(): flight b uses (px, py, spp,
+        w + b) and `scene_seed` for its seeds."""
         kind = {"mirror": A.PHASE_MIRROR, "lambert": A.PHASE_LAMBERT}.get(phase)
         if kind is None:
             raise MediumConfigError(f"unknown phase {phase!r} (mirror, lambert)")
-        p = np.ascontiguousarray(np.asarray(pss, np.uint32).reshape(rays.n, 4))
         dev = rays.t["ox"].device
-        cols = [torch.from_numpy(np.ascontiguousarray(p[:, i]).view(np.int32)).to(dev) for i in range(4)]
+        cols = pss if isinstance(pss, (list, tuple)) else self.path_seeds(pss, dev)
+        if len(cols) != 4 or any(int(c.shape[0]) != rays.n for c in cols):
+            raise A.GpisError("pss needs four device columns of one entry per path")
         seeds = A.PathSeeds(cols[0].data_ptr(), cols[1].data_ptr(), cols[2].data_ptr(), cols[3].data_ptr(),
                             int(scene_seed) & 0xFFFFFFFF)
-        f = dict(dtype=torch.float32, device=dev)
-        i = dict(dtype=torch.int32, device=dev)
-        reply = {k: torch.empty(rays.n, **f) for k in ("t0", "gx0", "gy0", "gz0", "nee", "t_last")}
-        reply.update({k: torch.empty(rays.n, **i) for k in ("outcome0", "flights", "hits", "shadows", "visible",
-                                                            "outcome_last", "evals")})
+        if reply is None:
+            reply = self.alloc_path1d_reply(rays.n, dev)
         r = A.Path1DReply()
         for k, _ in A.Path1DReply._fields_:
             setattr(r, k, reply[k].data_ptr())
