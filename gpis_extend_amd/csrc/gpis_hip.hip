@@ -84,6 +84,14 @@ static gpis_status build_params(gpis_ctx *c) {
         const double var = (double)((float)(unsigned)d.impulse_density / (P.iso_R * P.iso_R * P.iso_R)) * pow(M_PI, 1.5);
         P.iso_nf_inv = (float)(1.0 / sqrt((double)(float)var));
     }
+    // The brute-force nonstationary kernel conditions the gradient with the local scale
+    // (gradient_scale *= sqr(sparseConvNoiseLateralScale(p)), SparseConvolutionNoise.cpp:563-565),
+    // which the 3D flight kernels do not apply: refused rather than served wrong.
+    if (d.cov_kind == GPIS_COV_PROC_NS && !d.ns_multires && !d.one_d && !d.single_realization &&
+        d.ctxt == GPIS_CTX_RENEWAL_PLUS)
+        return fail(c, GPIS_UNSUPPORTED,
+                    "Renewal+ conditioning of the brute-force nonstationary kernel is not served "
+                    "(multiResolutionGrid and the Renewal context are)");
     P.kind = d.cov_kind;
     P.multires = (d.cov_kind == GPIS_COV_PROC_NS) && d.ns_multires;
     P.sigma = d.sigma;

@@ -66,6 +66,34 @@ def test_eval_points_config0(table):
     assert np.max(dg / np.maximum(gmag, 1.0)) < 1e-4
 
 
+@pytest.mark.parametrize("multires", [True, False])
+def test_eval_points_nonstationary(multires):
+    """The procedural nonstationary field (l(x) of the nonstationary workload) at 8,192
+    points: the multi-resolution blend and the brute-force local-scale splats."""
+    from gpis_extend_amd import workloads as W
+    print(f"case: nonstationary eval_points, multiResolutionGrid={multires}", flush=True)
+    js = W.config2_json()
+    js["gaussian_process"]["covariance"]["multiResolutionGrid"] = multires
+    js["single_realization"] = True
+    got, ref = _eval_compare(js, 8192, None, W.SEED_POINTS)
+    dv = np.abs(got[:, 0] - ref[:, 0])
+    gmag = np.linalg.norm(ref[:, 1:], axis=1)
+    dg = np.linalg.norm(got[:, 1:] - ref[:, 1:], axis=1) / np.maximum(gmag, 1.0)
+    off = (dv > 1e-4) | (dg > 1e-4)
+    print(f"value max|d|={dv.max():.3e}  grad max|d|/max(|g|,1)={dg.max():.3e} (median |g| {np.median(gmag):.1f}); "
+          f"points beyond 1e-4: {off.mean():.5f}")
+    # FP32: the value to 1e-4 absolute, the gradient (|g| up to ~1e2 at l = 0.02) to 1e-4 of
+    # max(|g|, 1).  The multi-resolution levels take their scales 2.5^L and the level index
+    # from the device's powf/logf: where those differ from the C library's in the last bit, an
+    # impulse at the support radius is in on one side and out on the other, a jump of one
+    # truncated splat (<= 0.01 in value); such points stay rare.
+    if multires:
+        assert off.mean() < 0.01
+        assert dv.max() < 0.02
+    else:
+        assert not off.any()
+
+
 def _flight_compare(js, rays, domain, name):
     from gpis_extend_amd import RayBatch
     med, dev = _dev(js, domain)
@@ -106,6 +134,25 @@ def test_flights_config2_conditioned_multires_subset():
     rays = W.box_surface_rays(med, n=512)
     got, ref, orc = _flight_compare(W.config2_json(), rays, None, "C2")
     parity_gate(got, ref, "C2 conditioned multires subset", min_agree=0.99, oracle=orc)
+
+
+def test_flights_config2_conditioned_brute_force_subset():
+    """The nonstationary workload with multiResolutionGrid false: every splat at the
+    local scale l(p) / l_max on the lattice of l_max (GPFunctions.cpp:1225-1345), flights
+    conditioned on the value (Renewal); Renewal+ is refused for this kernel."""
+    from gpis_extend_amd import GpisDevice, workloads as W
+    from gpis_extend_amd import SparseConvolutionNoiseMedium
+    from gpis_extend_amd._capi import GpisError
+    print("case: C2 brute-force nonstationary, 512 Renewal flights", flush=True)
+    js = W.config2_json()
+    js["gaussian_process"]["covariance"]["multiResolutionGrid"] = False
+    with pytest.raises(GpisError, match="UNSUPPORTED"):
+        GpisDevice(SparseConvolutionNoiseMedium.from_json(js))
+    js["correlation_context"] = "renewal"
+    med = SparseConvolutionNoiseMedium.from_json(js)
+    rays = W.box_surface_rays(med, n=512)
+    got, ref, orc = _flight_compare(js, rays, None, "C2bf")
+    parity_gate(got, ref, "C2 Renewal brute-force subset", min_agree=0.99, oracle=orc)
 
 
 @pytest.mark.skip(reason="faulted the GPU together with an LDS cell-block staging path for "
