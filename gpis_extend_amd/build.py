@@ -30,6 +30,7 @@ def hipcc() -> str:
 
 HOST_DIR = os.path.join(PKG, "host")
 SELFTEST = os.path.join(HOST_DIR, "backend_selftest")
+POOLTEST = os.path.join(HOST_DIR, "path_pool_selftest")
 
 
 def build(verbose: bool = False) -> str:
@@ -47,6 +48,16 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(SELFTEST + ".tmp", SELFTEST)
+    # the wavefront path pool over the backend and its self-test driver (no contraction:
+    # the host's hit-time phase rounds as the device's)
+    cmd = [hipcc(), "-O2", "-std=c++17", "-ffp-contract=off", "-Wno-unused-result",
+           os.path.join(HOST_DIR, "gpis_backend.cpp"), os.path.join(HOST_DIR, "path_pool.cpp"),
+           os.path.join(HOST_DIR, "path_pool_selftest.cpp"),
+           "-L" + PKG, "-lgpis_hip", "-Wl,-rpath,$ORIGIN/..", "-lpthread", "-o", POOLTEST + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(POOLTEST + ".tmp", POOLTEST)
     return OUT
 
 
