@@ -14,7 +14,8 @@ with light samples started from the same rays, and the learned surrogate against
 the plain method (trained on 2^24 plain flights, served on a 512x512 image at 16
 spp), each with its own timing.
 
-Prints one JSON line (see DESIGN.md for the field definitions).
+Prints one JSON line: the metric and its value, timing, the workload, the roofline of the
+dominant kernel and the CPU baseline.
 """
 from __future__ import annotations
 
@@ -436,7 +437,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(flights shard by ray index; see DESIGN.md)", file=sys.stderr)
+              "(flights shard by ray index: one process per GPU over disjoint ray ranges)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "c3":

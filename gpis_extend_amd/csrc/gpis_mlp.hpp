@@ -2,7 +2,7 @@
 // 8 -> 128 -> 128 -> 128 -> 128 -> 32 (tanh-form GELU) whose softmax is a piecewise-linear CDF over
 // the normalised next-crossing distance (31 bins + "exit").  No counterpart in the
 // reference: it stands in for the Renewal flight sampler of
-// SparseConvolutionNoiseMedium.cpp:265-347 and is judged against it.
+// SparseConvolutionNoiseMedium.cpp:265-347 and is measured against it.
 //
 // Register-resident layout.  Every layer is computed transposed, D = W^T X^T, with
 // v_mfma_f32_16x16x32_bf16: the weights are the A operand (16 output features x 32

@@ -2,7 +2,7 @@
 
 A small MLP maps the state of a conditioned 1D flight to a piecewise-linear CDF over
 its normalised next-crossing distance; it is trained on flights of the plain GPU
-method and sampled with the flight's own jitter.  The plain method stays the judge
+method and sampled with the flight's own jitter.  The plain method stays the reference
 (KS distance, image error) and the fallback (out-of-hull, high-entropy and
 unconditioned rays).  All compute runs in libgpis_hip.so; torch only holds device
 memory and streams and gathers training batches.

@@ -56,7 +56,7 @@ def test_unit_variance_and_se_covariance():
 def test_gradient_is_gradient_of_value():
     # The truncated splats make the field jump by ~0.009 where an impulse leaves the
     # support radius (about 25 such boundaries per cell length), so the difference
-    # step is kept short (2e-5) and judged by quantiles.
+    # step is kept short (2e-5) and checked by quantiles.
     F = se_field(l=0.1)
     rng = np.random.default_rng(4)
     x = rng.uniform(-1, 1, (2000, 3)).astype(np.float64)

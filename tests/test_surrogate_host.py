@@ -1,5 +1,5 @@
 """Host logic of the learned flight sampler (no GPU): the minimax bin edges and the
-KS helpers used to judge the surrogate."""
+KS helpers used to measure the surrogate."""
 import numpy as np
 
 from gpis_extend_amd import surrogate as S
