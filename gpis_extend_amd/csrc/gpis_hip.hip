@@ -856,6 +856,92 @@ __device__ __forceinline__ void frame_to_global(const Frame3 &F, float x, float 
 #define kTwoPiF 0x1.921fb6p+2f
 #define kInvPiF 0x1.45f306p-2f
 
+// conditioning1D (SparseConvolutionNoise.cpp:576-656) from the unconditioned evaluation at
+// the origin (normalised value nv, derivative along the ray ndv): the value coefficient v1
+// and, for Renewal+, the derivative coefficient g1z and the observed transverse gradient
+// of the correlated law.
+__device__ __forceinline__ void condition1d(const FieldParams &P, Ray1D &R, float ox, float oy, float oz, float nv,
+                                            float ndv, float prev_v, float prev_gx, float prev_gy, float prev_gz,
+                                            float bx1, float by1, float &obs_x, float &obs_y, bool &observed) {
+#pragma clang fp contract(off)
+    const float amp = P.amp_const;
+    R.uo = ray_coord(R, ox, oy, oz);
+    R.v1 = 0.f;
+    R.g1z = 0.f;
+    if (amp != 0.f) {
+        const float mean = (float)mean_value_d(P, ox, oy, oz);
+        R.v1 = (prev_v - mean) / amp - nv;
+        if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
+            float mgx, mgy, mgz;
+            mean_grad_d(P, ox, oy, oz, mgx, mgy, mgz);
+            float cgx, cgy, cgz;
+            ray_grad_to_world(R, bx1, by1, ndv, cgx, cgy, cgz);
+            const float tgx = (prev_gx - mgx) / amp, tgy = (prev_gy - mgy) / amp, tgz = (prev_gz - mgz) / amp;
+            float ddx = tgx - cgx;
+            float ddy = tgy - cgy;
+            float ddz = tgz - cgz;
+            // gradient_scale = -2 * frame.toLocal(l_aniso * delta) * scale^2; only its
+            // z component enters the value and the derivative along the ray (the
+            // transverse components are the correlated law's, through the moments)
+            float gsz = dot3(R.F.nx, R.F.ny, R.F.nz, R.la[0] * ddx, R.la[1] * ddy, R.la[2] * ddz);
+            R.g1z = -2.f * gsz * 1.0f;
+            // target_gradient_noise in isotropic ray space (seTransverseGradientMoments :94-97)
+            const float gwx = R.la[0] * tgx, gwy = R.la[1] * tgy, gwz = R.la[2] * tgz;
+            obs_x = dot3(R.F.tx, R.F.ty, R.F.tz, gwx, gwy, gwz);
+            obs_y = dot3(R.F.bx, R.F.by, R.F.bz, gwx, gwy, gwz);
+            observed = true;
+        }
+    }
+    R.cond = 1;
+}
+
+// evaluateGradientNoise1D at the endpoint q (:204-242) — transverse pair 0 when tSegment == 0,
+// otherwise an independent second pair, or the correlated law's pair 0 about the conditional
+// mean (:186-196) — then resumeDistance's rejection rules in double
+// (GaussianProcessMedium.cpp:289-369).  Returns the outcome.
+__device__ __forceinline__ uint32_t finish1d(const FieldParams &P, const Ray1D &R, bool corr, bool observed,
+                                             float obs_x, float obs_y, float ox, float oy, float oz, float qx,
+                                             float qy, float qz, float dx, float dy, float dz, uint32_t normal_seed,
+                                             float t_end, float ndv_end, int hit, float &gxv, float &gyv,
+                                             float &gzv) {
+    float tx2, ty2;
+    if (corr) {
+        double mx, my, var, z0, z1;
+        transverse_moments(R, observed, obs_x, obs_y, ox, oy, oz, qx, qy, qz, mx, my, var);
+        normal_pair(normal_seed, 0, z0, z1);
+        const double sd = sqrt(var);
+        tx2 = (float)(mx + sd * z0);
+        ty2 = (float)(my + sd * z1);
+    } else {
+        double x2, y2;
+        transverse_pair(normal_seed, t_end == 0.0f ? 0 : 1, x2, y2);
+        tx2 = (float)x2;
+        ty2 = (float)y2;
+    }
+    float wx, wy, wz;
+    ray_grad_to_world(R, tx2, ty2, ndv_end, wx, wy, wz);
+    float mgx, mgy, mgz;
+    mean_grad_d(P, qx, qy, qz, mgx, mgy, mgz);
+    const float amp = P.amp_const;
+    {
+#pragma clang fp contract(off)
+        gxv = amp * wx + mgx; gyv = amp * wy + mgy; gzv = amp * wz + mgz;
+    }
+    if (!hit) return GPIS_EXIT;
+    const double gx = gxv, gy = gyv, gz = gzv;
+    double gd = gx * (double)dx;
+    gd += gy * (double)dy;
+    gd += gz * (double)dz;
+    double g2 = 0.0;
+    g2 += gx * gx;
+    g2 += gy * gy;
+    g2 += gz * gz;
+    if (!isfinite((gx + gy + gz) * (1.0 / 3.0))) return GPIS_FAIL | (GPIS_FAIL_NONFINITE << 8);
+    if (gd > 0.0) return GPIS_FAIL | (GPIS_FAIL_OUTWARD << 8);
+    if (g2 < (double)1e-7f) return GPIS_FAIL | (GPIS_FAIL_ZERO_GRAD << 8);
+    return GPIS_HIT;
+}
+
 // The 1D persistent flight kernel: same lane state machine as flight_kernel, with the
 // 1D field (cells cached per lane), conditioning1D at the origin, the Box-Muller
 // transverse gradient at the endpoint, and optional NEE toward a point light.
@@ -1017,49 +1103,8 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
                 shadow = false;
                 nee_stage = 3;
             } else if (phase == PH_DONE && nee_stage == 0) {
-                // evaluateGradientNoise1D at the endpoint (:204-242): transverse pair 0 when
-                // tSegment == 0, otherwise an independent second pair; resumeDistance rules
-                float tx2, ty2;
-                if (corr) {
-                    // the correlated law: pair 0 about the conditional mean (:186-196)
-                    double mx, my, var, z0, z1;
-                    transverse_moments(R, observed, obs_x, obs_y, ox, oy, oz, qx, qy, qz, mx, my, var);
-                    normal_pair(normal_seed, 0, z0, z1);
-                    const double sd = sqrt(var);
-                    tx2 = (float)(mx + sd * z0);
-                    ty2 = (float)(my + sd * z1);
-                } else {
-                    double x2, y2;
-                    transverse_pair(normal_seed, t_end == 0.0f ? 0 : 1, x2, y2);
-                    tx2 = (float)x2;
-                    ty2 = (float)y2;
-                }
-                float wx, wy, wz;
-                ray_grad_to_world(R, tx2, ty2, ndv_end, wx, wy, wz);
-                float mgx, mgy, mgz;
-                mean_grad_d(P, qx, qy, qz, mgx, mgy, mgz);
-                const float amp = P.amp_const;
-                {
-#pragma clang fp contract(off)
-                    gxv = amp * wx + mgx; gyv = amp * wy + mgy; gzv = amp * wz + mgz;
-                }
-                if (hit) {
-                    // resumeDistance rejection rules, in double (GaussianProcessMedium.cpp:289-369)
-                    const double gx = gxv, gy = gyv, gz = gzv;
-                    double gd = gx * (double)dx;
-                    gd += gy * (double)dy;
-                    gd += gz * (double)dz;
-                    double g2 = 0.0;
-                    g2 += gx * gx;
-                    g2 += gy * gy;
-                    g2 += gz * gz;
-                    if (!isfinite((gx + gy + gz) * (1.0 / 3.0))) outcome = GPIS_FAIL | (GPIS_FAIL_NONFINITE << 8);
-                    else if (gd > 0.0) outcome = GPIS_FAIL | (GPIS_FAIL_OUTWARD << 8);
-                    else if (g2 < (double)1e-7f) outcome = GPIS_FAIL | (GPIS_FAIL_ZERO_GRAD << 8);
-                    else outcome = GPIS_HIT;
-                } else {
-                    outcome = GPIS_EXIT;
-                }
+                outcome = finish1d(P, R, corr, observed, obs_x, obs_y, ox, oy, oz, qx, qy, qz, dx, dy, dz, normal_seed,
+                                   t_end, ndv_end, hit, gxv, gyv, gzv);
                 if (!PATHS) {
                     if (use_light && (outcome & 0xFFu) == GPIS_HIT) {
                         // one more evaluation at the scattering point sample.p = pos + t * dir (float)
@@ -1112,7 +1157,7 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
                                 // under UNI takes no light sample (TraceBase.cpp:806-807)
                                 float sx, sy, sz;
                                 ray_point(ox, oy, oz, dx, dy, dz, tp, sx, sy, sz);
-                                const bool uni = P.scheme1d == GPIS_SCHEME_UNI || amp < 1e-6f ||
+                                const bool uni = P.scheme1d == GPIS_SCHEME_UNI || P.amp_const < 1e-6f ||
                                                  (P.corr_xy && observed && sx == ox && sy == oy && sz == oz);
                                 if (!uni) {
                                     // MirrorBsdf::eval / evalGrad: the half-vector normal in world space
@@ -1290,37 +1335,7 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
         bool unit = false;
 
         if (phase == PH_COND) {
-            // conditioning1D (:576-656)
-#pragma clang fp contract(off)
-            const float amp = P.amp_const;
-            R.uo = ray_coord(R, ox, oy, oz);
-            R.v1 = 0.f;
-            R.g1z = 0.f;
-            if (amp != 0.f) {
-                const float mean = (float)mean_value_d(P, ox, oy, oz);
-                R.v1 = (prev_v - mean) / amp - nv;
-                if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
-                    float mgx, mgy, mgz;
-                    mean_grad_d(P, ox, oy, oz, mgx, mgy, mgz);
-                    float cgx, cgy, cgz;
-                    ray_grad_to_world(R, bx1, by1, ndv, cgx, cgy, cgz);
-                    const float tgx = (prev_gx - mgx) / amp, tgy = (prev_gy - mgy) / amp, tgz = (prev_gz - mgz) / amp;
-                    float ddx = tgx - cgx;
-                    float ddy = tgy - cgy;
-                    float ddz = tgz - cgz;
-                    // gradient_scale = -2 * frame.toLocal(l_aniso * delta) * scale^2; only its
-                    // z component enters the value and the derivative along the ray (the
-                    // transverse components are the correlated law's, through the moments)
-                    float gsz = dot3(R.F.nx, R.F.ny, R.F.nz, R.la[0] * ddx, R.la[1] * ddy, R.la[2] * ddz);
-                    R.g1z = -2.f * gsz * 1.0f;
-                    // target_gradient_noise in isotropic ray space (seTransverseGradientMoments :94-97)
-                    const float gwx = R.la[0] * tgx, gwy = R.la[1] * tgy, gwz = R.la[2] * tgz;
-                    obs_x = dot3(R.F.tx, R.F.ty, R.F.tz, gwx, gwy, gwz);
-                    obs_y = dot3(R.F.bx, R.F.by, R.F.bz, gwx, gwy, gwz);
-                    observed = true;
-                }
-            }
-            R.cond = 1;
+            condition1d(P, R, ox, oy, oz, nv, ndv, prev_v, prev_gx, prev_gy, prev_gz, bx1, by1, obs_x, obs_y, observed);
             phase = PH_F0;
             tq = (double)near_t;
         } else if (phase == PH_F0) {
@@ -1421,6 +1436,255 @@ __global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn i
         }
         if (phase == PH_F0 || phase == PH_MARCH || phase == PH_SECANT || phase == PH_FAR || phase == PH_GRAD)
             point_at(ox, oy, oz, dx, dy, dz, tq, unit, qx, qy, qz);
+    }
+    __syncthreads();
+    if (threadIdx.x < ST_N && bstat[threadIdx.x]) atomicAdd(&stats[threadIdx.x], (unsigned long long)bstat[threadIdx.x]);
+}
+
+// Flights of a short index list (the learned sampler's fallback rays): a group of 16 lanes
+// (four groups per wave) traces one flight.  The march evaluates 16 consecutive lattice
+// points at once, one per lane; conditioning, f(nearT), the secant, f(farT) and the endpoint
+// gradient evaluate one point on all 16 lanes.  The points, their order of decisions and
+// the arithmetic are flight1d_kernel's, so every reply is the same bit for bit (evals
+// reports the plain march's count), while a long flight takes about steps/16 evaluation
+// rounds: a launch over a few thousand rays no longer waits ~steps rounds for its longest
+// flight.  Ray indices come from idx[0 .. *n_dev).
+constexpr int kGroup = 16;
+template <int NI>
+__global__ __launch_bounds__(256, 2) void flight1d_group_kernel(FieldParams P, RayIn in,
+                                                                const float *__restrict__ lscale, ReplyOut out,
+                                                                unsigned int *work, unsigned long long *stats,
+                                                                float nf, const uint32_t *__restrict__ idx,
+                                                                const uint32_t *__restrict__ n_dev) {
+    const unsigned nn = *n_dev;
+    __shared__ unsigned int bstat[ST_N];
+    if (threadIdx.x < ST_N) bstat[threadIdx.x] = 0;
+    __syncthreads();
+    const unsigned lane = lane_id();
+    const unsigned sub = lane % kGroup;
+    const unsigned gshift = lane - sub;  // first lane of this lane's group
+    const unsigned long long below = gshift ? ((1ull << gshift) - 1ull) : 0ull;
+    int phase = PH_NEED;
+    unsigned ray = 0;
+    float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, near_t = 0, far_t = 0, step = 0, t0 = 0;
+    float pf = 0.f, t_end = 0.f, val_end = 0.f;
+    int first = 1, sign0 = 1, hit = 0;
+    unsigned kb = 0, k = 0, evals_u = 0, evals_m = 0;
+    double tk = 0, fac = 0, tp = 0, tt = 0, tq = 0;
+    bool unit = false;
+    float prev_v = 0, prev_gx = 0, prev_gy = 0, prev_gz = 0;
+    uint32_t normal_seed = 0;
+    float bx1 = 0.f, by1 = 0.f, obs_x = 0.f, obs_y = 0.f;
+    bool observed = false;
+    const bool corr = P.corr_xy && P.ctxt == GPIS_CTX_RENEWAL_PLUS;
+    Ray1D R = {};
+    Cells1D<NI> C;
+    C.base = 0;
+    bool cache_valid = false;
+
+    for (;;) {
+        // ---- refill: every group that wants a ray takes the next index (one atomic per wave)
+        const unsigned long long need = __ballot(phase == PH_NEED && sub == 0);
+        if (need) {
+            const unsigned cnt = (unsigned)__popcll(need);
+            const unsigned leader = (unsigned)(__ffsll((long long)need) - 1);
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(work, cnt);
+            base = __shfl(base, leader);
+            if (phase == PH_NEED) {
+                const unsigned rl = base + (unsigned)__popcll(need & below);
+                if (rl >= nn) {
+                    phase = PH_DEAD;
+                } else {
+                    const unsigned r = idx[rl];
+                    ray = r;
+                    ox = in.ox[r]; oy = in.oy[r]; oz = in.oz[r];
+                    dx = in.dx[r]; dy = in.dy[r]; dz = in.dz[r];
+                    near_t = in.near_t[r]; far_t = in.far_t[r];
+                    const float jit = in.jitter[r];
+                    R.seed = in.noise_seed[r];
+                    normal_seed = in.normal_seed[r];
+                    R.cond = 0;
+                    first = (in.flags[r] & 1u) ? 1 : 0;
+                    if (!first) { prev_v = in.pv[r]; prev_gx = in.pgx[r]; prev_gy = in.pgy[r]; prev_gz = in.pgz[r]; }
+                    evals_u = 0; evals_m = 0; k = 0;
+                    observed = false;
+                    cache_valid = false;
+                    ray_scales(P.aniso, P.l_aniso, P.l_aniso_inv, lscale, r, R.la, R.ia);
+                    float ix = R.ia[0] * dx, iy = R.ia[1] * dy, iz = R.ia[2] * dz;
+                    normalize3(ix, iy, iz);
+                    R.F = make_frame(ix, iy, iz);
+                    if (!isfinite(far_t)) far_t = near_t + 2000.f;
+                    const bool bad = !isfinite(ox) || !isfinite(oy) || !isfinite(oz) || !isfinite(dx) ||
+                                     !isfinite(dy) || !isfinite(dz) || !isfinite(near_t) || !(jit >= 0.f && jit < 1.f) ||
+                                     !(R.la[0] > 0.f && R.la[1] > 0.f && R.la[2] > 0.f) ||
+                                     !isfinite(R.la[0] + R.la[1] + R.la[2]);
+                    if (bad || far_t <= near_t) {
+                        if (sub == 0) {
+                            out.t[r] = far_t; out.value[r] = (bad || first) ? 0.f : prev_v;
+                            out.gx[r] = out.gy[r] = out.gz[r] = 0.f;
+                            out.outcome[r] = bad ? GPIS_INVALID : GPIS_EXIT;
+                            if (out.evals) out.evals[r] = 0;
+                            atomicAdd(&bstat[bad ? ST_INVALID : ST_EXITS], 1u);
+                        }
+                        phase = PH_NEED;
+                    } else {
+                        // finishGpisSegment step rule (SparseConvolutionNoiseMedium.cpp:289-291)
+                        step = (far_t - near_t) / P.min_step_f;
+                        if (P.step_size < step) step = P.step_size;
+                        t0 = mul_add_exact(step, jit, near_t);
+                        unit = false;
+                        if (P.activate && !first) {
+                            phase = PH_COND;
+                            if (corr) {
+                                double z0, z1;
+                                normal_pair(normal_seed, 0, z0, z1);
+                                const double sd = sqrt(0.5);
+                                bx1 = (float)(0.0 + sd * z0);
+                                by1 = (float)(0.0 + sd * z1);
+                            } else if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
+                                double x1, y1;
+                                transverse_pair(normal_seed, 0, x1, y1);
+                                bx1 = (float)x1; by1 = (float)y1;
+                            }
+                        } else {
+                            phase = PH_F0;
+                            tq = (double)near_t;
+                        }
+                    }
+                }
+            }
+        }
+        if (!__ballot(phase != PH_DEAD)) break;
+        if (phase == PH_NEED || phase == PH_DEAD) continue;
+
+        // ---- one evaluation per lane: its own lattice point in the march, else the group's point
+        float qx, qy, qz;
+        if (phase == PH_COND) {
+            qx = ox; qy = oy; qz = oz;
+        } else if (phase == PH_MARCH) {
+            point_at(ox, oy, oz, dx, dy, dz, lattice_t(t0, (int)(kb + sub), step), false, qx, qy, qz);
+        } else {
+            point_at(ox, oy, oz, dx, dy, dz, tq, unit, qx, qy, qz);
+        }
+        float fv, nv, ndv;
+        eval1d<NI>(P, R, C, cache_valid, nf, qx, qy, qz, phase != PH_COND, fv, nv, ndv);
+
+        if (phase == PH_COND) {
+            ++evals_u;
+            condition1d(P, R, ox, oy, oz, nv, ndv, prev_v, prev_gx, prev_gy, prev_gz, bx1, by1, obs_x, obs_y, observed);
+            phase = PH_F0;
+            tq = (double)near_t;
+            unit = false;
+        } else if (phase == PH_F0) {
+            ++evals_u;
+            sign0 = fv < 0.f ? -1 : 1;
+            pf = fv;
+            kb = 0;
+            if (t0 < far_t) {
+                phase = PH_MARCH;
+            } else {
+                phase = PH_FAR;
+                tq = (double)far_t;
+            }
+        } else if (phase == PH_MARCH) {
+            // lattice point kb + sub; it exists while t < farT (the plain march's loop bound)
+            const double tl = lattice_t(t0, (int)(kb + sub), step);
+            const bool valid = tl < (double)far_t && kb + sub < (unsigned)kMaxSteps;
+            const int sc = fv < 0.f ? -1 : 1;
+            // a conditioned flight's first march point only sets the reference sign
+            if (!first && kb == 0) sign0 = __shfl(sc, (int)gshift);
+            const bool cand = valid && sc != sign0 && (first || kb + sub != 0);
+            const unsigned cm = (unsigned)((__ballot(cand) >> gshift) & 0xFFFFull);
+            const unsigned vm = (unsigned)((__ballot(valid) >> gshift) & 0xFFFFull);
+            const float f_last = __shfl(fv, (int)(gshift + kGroup - 1));
+            if (cm) {
+                const unsigned j = (unsigned)(__ffs((int)cm) - 1);
+                const float fj = __shfl(fv, (int)(gshift + j));
+                const float fjm1 = __shfl(fv, (int)(gshift + (j > 0 ? j - 1 : 0)));
+                const float pfx = j > 0 ? fjm1 : pf;
+                const unsigned kc = kb + j;  // lattice index of the first sign change
+                evals_m = kc + 1;
+                k = kc + 1;
+                // secant with 0.9 shrink (SparseConvolutionNoiseMedium.cpp:309-329), in FP64
+                tk = lattice_t(t0, (int)kc, step);
+                fac = (double)pfx / ((double)pfx - (double)fj);
+                tp = lerp_d(tk - (double)step, tk, fac);
+                tt = tp;
+                phase = PH_SECANT;
+                tq = tt;
+                unit = false;
+            } else if (vm != 0xFFFFu) {
+                evals_m = kb + (unsigned)__popc(vm);
+                k = evals_m;
+                phase = PH_FAR;
+                tq = (double)far_t;
+                unit = false;
+            } else {
+                pf = f_last;
+                kb += kGroup;
+                if (kb >= (unsigned)kMaxSteps) {
+                    // step limit (RaySegmentStatus::StepLimit): reported invalid, as the plain kernel
+                    if (sub == 0) {
+                        out.t[ray] = (float)lattice_t(t0, kMaxSteps, step); out.value[ray] = f_last;
+                        out.gx[ray] = out.gy[ray] = out.gz[ray] = 0.f;
+                        out.outcome[ray] = GPIS_INVALID;
+                        if (out.evals) out.evals[ray] = evals_u + (unsigned)kMaxSteps;
+                        atomicAdd(&bstat[ST_INVALID], 1u);
+                    }
+                    phase = PH_NEED;
+                }
+            }
+        } else if (phase == PH_SECANT) {
+            ++evals_u;
+            bool done = false;
+            if ((fv < 0.f ? -1 : 1) == sign0) {
+                done = true;
+            } else {
+                fac *= 0.9;
+                if (fac <= 0.01) {
+                    tp = 0.0;
+                    done = true;
+                } else {
+                    tp = tt;
+                    tt = lerp_d(tk - (double)step, tk, fac);
+                    tq = tt;
+                }
+            }
+            if (done) {
+                hit = 1;
+                t_end = (float)tp;
+                val_end = 0.f;
+                phase = PH_GRAD;
+                tq = tp;
+                unit = true;
+            }
+        } else if (phase == PH_FAR) {
+            ++evals_u;
+            hit = 0;
+            t_end = far_t;
+            val_end = fv;
+            phase = PH_GRAD;
+            tq = (double)far_t;
+            unit = true;
+        } else if (phase == PH_GRAD) {
+            ++evals_u;
+            float gxv, gyv, gzv;
+            const uint32_t outcome = finish1d(P, R, corr, observed, obs_x, obs_y, ox, oy, oz, qx, qy, qz, dx, dy, dz,
+                                              normal_seed, t_end, ndv, hit, gxv, gyv, gzv);
+            if (sub == 0) {
+                const unsigned oc = outcome & 0xFFu;
+                atomicAdd(&bstat[ST_FLIGHTS], 1u);
+                atomicAdd(&bstat[ST_EVALS], evals_u + evals_m);
+                atomicAdd(&bstat[ST_STEPS], k);
+                atomicAdd(&bstat[oc == GPIS_HIT ? ST_HITS : (oc == GPIS_EXIT ? ST_EXITS : ST_FAILS)], 1u);
+                out.t[ray] = t_end; out.value[ray] = val_end;
+                out.gx[ray] = gxv; out.gy[ray] = gyv; out.gz[ray] = gzv;
+                out.outcome[ray] = outcome;
+                if (out.evals) out.evals[ray] = evals_u + evals_m;
+            }
+            phase = PH_NEED;
+        }
     }
     __syncthreads();
     if (threadIdx.x < ST_N && bstat[threadIdx.x]) atomicAdd(&stats[threadIdx.x], (unsigned long long)bstat[threadIdx.x]);
@@ -1759,6 +2023,20 @@ gpis_status gpis_trace_paths_1d(gpis_ctx *c, const gpis_ray_soa *rays, const gpi
                        rays->length_scale, ReplyOut{}, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
                        light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, Nee1DOut{}, nullptr,
                        nullptr, c->batch_k_1d, pin, pout);
+    HIP_TRY(c, hipGetLastError());
+    return GPIS_OK;
+}
+
+// Launch the group kernel over the first *n_dev entries of the index list idx (n bounds them).
+static gpis_status launch_flight1d_groups(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, const ReplyOut &out,
+                                          const uint32_t *idx, const uint32_t *n_dev, hipStream_t st) {
+    HIP_TRY(c, hipMemsetAsync(c->work, 0, sizeof(unsigned), st));
+    const unsigned threads = 256, groups_per_block = threads / kGroup;
+    const long long want = (long long)c->num_cus * 2;  // two blocks of four waves per CU (2 waves per SIMD)
+    const long long need = (n + groups_per_block - 1) / groups_per_block;
+    const unsigned blocks = (unsigned)(need < want ? need : (want < 1 ? 1 : want));
+    hipLaunchKernelGGL((flight1d_group_kernel<8>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+                       rays->length_scale, out, c->work, c->counters, c->nf1d, idx, n_dev);
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }
@@ -2167,7 +2445,8 @@ gpis_status gpis_surrogate_flights_1d(gpis_ctx *c, gpis_net *net, const gpis_ray
                   (const float *)rays->jitter, (const uint32_t *)rays->flags, o, st);
     if (s != GPIS_OK) return s;
     ReplyOut out = {reply->t, reply->value, reply->gx, reply->gy, reply->gz, reply->outcome, reply->evals};
-    s = launch_flight1d(c, rays, n, out, nullptr, Nee1DOut{}, net->fallback, net->n_fallback, st);
+    // the fallback rays are few and their flights long: a group of lanes per flight
+    s = launch_flight1d_groups(c, rays, n, out, net->fallback, net->n_fallback, st);
     if (s != GPIS_OK) return s;
     if (n_fallback) {
         HIP_TRY(c, hipMemcpyAsync(n_fallback, net->n_fallback, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
