@@ -247,6 +247,11 @@ enum Phase : int {
 #ifndef GPIS_FLIGHT_MIN_WAVES
 #define GPIS_FLIGHT_MIN_WAVES 3
 #endif
+// The same for the 1D flight kernels (their FP64 splat exponentials need ~185 VGPRs; at 3
+// waves per SIMD they spill and the renewal workload ran 5.72 ms against 5.51).
+#ifndef GPIS_FLIGHT1D_MIN_WAVES
+#define GPIS_FLIGHT1D_MIN_WAVES 2
+#endif
 
 // Lattice steps after which a flight is abandoned and reported invalid.
 constexpr int kMaxSteps = 1 << 22;
@@ -953,7 +958,7 @@ __device__ __forceinline__ uint32_t finish1d(const FieldParams &P, const Ray1D &
 // flight, then the mirror or Lambert bounce) and the next flight starts from the hit,
 // conditioned on (0, gradient), without leaving the kernel.
 template <int NI, bool PATHS>
-__global__ __launch_bounds__(256, 2) void flight1d_kernel(FieldParams P, RayIn in, const float *__restrict__ lscale,
+__global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(FieldParams P, RayIn in, const float *__restrict__ lscale,
                                                           ReplyOut out, unsigned int n, unsigned int *work,
                                                           unsigned long long *stats, float nf, int use_light,
                                                           float lx, float ly, float lz, Nee1DOut nee,
