@@ -62,19 +62,26 @@ __device__ __forceinline__ void pair3(float lx, float ly, float lz, float4 im, f
 // The same pair with an impulse from the packed table: (x with the weight in its
 // sign bit, y, z).  |x| is an operand modifier and the weight is applied with a
 // bit insert, so the packed form costs no extra instruction.
+// The splat runs only when some lane of the wave has the impulse in support: a
+// wave-uniform branch on the ballot of the support test.  The lanes it skips would add
+// exact zeros, so the sums are the same bit for bit; a wave whose lanes query nearby points
+// (camera rays of one pixel tile) skips most of the 27 cells' impulses.
 template <bool ISO>
 __device__ __forceinline__ void pair3p(float lx, float ly, float lz, float xs, float iy, float iz, float c0, float c1,
                                        float c2, float &v, float &ax, float &ay, float &az) {
     float tx = lx - fabsf(xs), ty = ly - iy, tz = lz - iz;
     float r2 = support_r2(tx, ty, tz);
-    float e;
-    if (ISO) e = __builtin_amdgcn_exp2f(c0 * r2);
-    else e = __builtin_amdgcn_exp2f(c0 * (tx * tx) + c1 * (ty * ty) + c2 * (tz * tz));
-    float we = (r2 < 1.0f) ? copysignf(e, xs) : 0.0f;
-    v += we;
-    ax += we * tx;
-    ay += we * ty;
-    az += we * tz;
+    const bool in = r2 < 1.0f;
+    if (__builtin_amdgcn_ballot_w64(in)) {
+        float e;
+        if (ISO) e = __builtin_amdgcn_exp2f(c0 * r2);
+        else e = __builtin_amdgcn_exp2f(c0 * (tx * tx) + c1 * (ty * ty) + c2 * (tz * tz));
+        float we = in ? copysignf(e, xs) : 0.0f;
+        v += we;
+        ax += we * tx;
+        ay += we * ty;
+        az += we * tz;
+    }
 }
 
 // A cell at offset d in {-1,0,1} from the query's cell holds impulses whose |to| along
