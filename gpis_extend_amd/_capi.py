@@ -15,7 +15,10 @@ GPIS_OK, GPIS_INVALID_ARGUMENT, GPIS_DEVICE_ERROR, GPIS_UNSUPPORTED, GPIS_OUT_OF
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "DEVICE_ERROR", 3: "UNSUPPORTED", 4: "OUT_OF_MEMORY"}
 
 COV_SE, COV_PROC_NS = 0, 1
-MEAN_SPHERE, MEAN_BOX, MEAN_PLANE, MEAN_GRID = 0, 1, 2, 3
+MEAN_SPHERE, MEAN_BOX, MEAN_PLANE, MEAN_GRID, MEAN_PROCEDURAL = 0, 1, 2, 3, 4
+# SdfFunctions::Function (SdfFunctions.hpp:13-22)
+SDF_FUNCTIONS = ("knob", "knob_inner", "knob_outer", "two_spheres", "plane", "rounded_u_groove", "two_slabs",
+                 "three_slabs")
 CTX_GLOBAL, CTX_RENEWAL, CTX_RENEWAL_PLUS, CTX_NONE = 0, 1, 2, 3
 SCHEME_UNI, SCHEME_NEE, SCHEME_MIS = 0, 1, 2
 LS_CONST, LS_SINCOS = 0, 1
@@ -50,6 +53,8 @@ class FieldDesc(C.Structure):
         ("ctxt", C.c_int32), ("one_d", C.c_int32), ("scheme", C.c_int32), ("correlation_xy", C.c_int32),
         ("step_size", C.c_float), ("min_step", C.c_uint32), ("max_bounce", C.c_int32),
         ("isotropic_3d", C.c_int32),
+        ("sdf_function", C.c_int32), ("sdf_inverse_transform", C.c_float * 12),
+        ("sdf_min", C.c_float), ("sdf_scale", C.c_float), ("sdf_offset", C.c_float),
     ]
 
 

@@ -61,9 +61,13 @@ __device__ __forceinline__ Frame3 make_frame(float x, float y, float z) {
 // The mean and its gradient evaluated in double as the reference does
 // (GaussianProcess::mean, SphericalMean GPFunctions.hpp:952-996, and the box/plane
 // signed distances), the gradient rounded to float.
+// PROC = false leaves the procedural mean's out-of-line call out of a kernel (a call
+// changes the register allocation of the whole kernel); mean_kind 4 then must not reach it.
+template <bool PROC = true>
 __device__ __forceinline__ double mean_value_d(const FieldParams &P, float x, float y, float z) {
 #pragma clang fp contract(off)
     if (P.mean_kind == 3) return grid_mean_d(P, x, y, z);
+    if (PROC && P.mean_kind == 4) return proc_mean_d(P.sdf, (double)x, (double)y, (double)z, nullptr);
     const double p[3] = {(double)x - P.mcd[0], (double)y - P.mcd[1], (double)z - P.mcd[2]};
     if (P.mean_kind == 1) {
         double o2 = 0.0, mx = -1e300;
@@ -84,6 +88,7 @@ __device__ __forceinline__ double mean_value_d(const FieldParams &P, float x, fl
     s += p[2] * p[2];
     return sqrt(s) - P.mrd;
 }
+template <bool PROC = true>
 __device__ __forceinline__ void mean_grad_d(const FieldParams &P, float x, float y, float z, float &gx, float &gy,
                                             float &gz) {
 #pragma clang fp contract(off)
@@ -91,6 +96,8 @@ __device__ __forceinline__ void mean_grad_d(const FieldParams &P, float x, float
     double g[3] = {0.0, 0.0, 0.0};
     if (P.mean_kind == 3) {
         grid_mean_grad_d(P, x, y, z, g[0], g[1], g[2]);
+    } else if (PROC && P.mean_kind == 4) {
+        proc_mean_d(P.sdf, (double)x, (double)y, (double)z, g);
     } else if (P.mean_kind == 1) {
         double q[3], o2 = 0.0;
         int imax = 0;

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "gpis_sdf.hpp"
+
 namespace gpis {
 
 // ---------------------------------------------------------------- leaves --
@@ -88,6 +90,8 @@ struct FieldParams {
     int gorig[3], gdim[3];
     float w2i[12], gcmin[3], gcmax[3], goff, gscale;
     int gvolume;                                 // density -> -log(max(1e-4, density))
+    // procedural SDF mean (mean_kind 4): its parameters in device memory
+    const sdf::MeanParams *sdf;
     // medium
     uint32_t global_seed; int single; int ctxt; int activate; int one_d; int scheme1d; int corr_xy;
     float step_size; float min_step_f; int max_bounce;
@@ -205,13 +209,29 @@ __device__ __forceinline__ float analytic_mean_value(const FieldParams &P, float
     return r - P.mr;
 }
 
-// Mean kinds as a compile-time choice: any (runtime switch), analytic only, grid only.
-enum { MEAN_ANY = 0, MEAN_ANALYTIC = 1, MEAN_GRID = 2 };
+// ProceduralMean (mean_kind 4) at a double point, in double; g may be null. Out of line:
+// the shapes are long and every flight kernel would otherwise inline them at each call.
+__device__ __noinline__ double proc_mean_d(const sdf::MeanParams *M, double x, double y, double z, double *g) {
+    return sdf::mean(*M, x, y, z, g);
+}
+__device__ __forceinline__ float proc_mean_value(const FieldParams &P, float x, float y, float z, float &gx,
+                                                 float &gy, float &gz) {
+    double g[3];
+    const double v = proc_mean_d(P.sdf, (double)x, (double)y, (double)z, g);
+    gx = (float)g[0]; gy = (float)g[1]; gz = (float)g[2];
+    return (float)v;
+}
+
+// Mean kinds as a compile-time choice: any (runtime switch), analytic only (sphere, box,
+// plane), or the evaluated means (tabulated grid or procedural SDF, switched at run time).
+enum { MEAN_ANY = 0, MEAN_ANALYTIC = 1, MEAN_EXT = 2 };
 
 template <int MEAN = MEAN_ANY>
 __device__ __forceinline__ float mean_value(const FieldParams &P, float x, float y, float z, float &gx, float &gy,
                                             float &gz) {
-    if (MEAN == MEAN_GRID || (MEAN == MEAN_ANY && P.mean_kind == 3)) return grid_mean_value(P, x, y, z, gx, gy, gz);
+    if (MEAN == MEAN_ANALYTIC) return analytic_mean_value(P, x, y, z, gx, gy, gz);
+    if (P.mean_kind == 3) return grid_mean_value(P, x, y, z, gx, gy, gz);
+    if (P.mean_kind == 4) return proc_mean_value(P, x, y, z, gx, gy, gz);
     return analytic_mean_value(P, x, y, z, gx, gy, gz);
 }
 

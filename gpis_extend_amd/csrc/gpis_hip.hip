@@ -39,6 +39,8 @@ struct gpis_ctx {
     int batch_k_1d = 16;  // ... and of the 1D kernel
     float nf1d = 1.0f;  // sqrt(sparseConvNoiseVariance1D) of the 1D grid
     float *grid = nullptr;  // device voxels of a tabulated mean
+    sdf::MeanParams sdf_host;             // a procedural mean's parameters ...
+    sdf::MeanParams *sdf_dev = nullptr;   // ... and their device copy
     std::string err;
 };
 
@@ -59,6 +61,59 @@ static gpis_status hip_fail(gpis_ctx *c, hipError_t e, const char *where) {
 // Host restatement of floor(log(s)/log(2.5)) in float (SparseConvolutionNoise.cpp:309).
 static int host_level_index(float s) { return (int)floorf(logf(s) / logf(2.5f)); }
 
+// Mat4f::rotAxis((1, 0, 0), deg) rows 0-2 (Mat4f.cpp:132-148), in float with the C
+// library's sinf/cosf.
+static void rot_axis_x(float deg, float r[9]) {
+#pragma clang fp contract(off)
+    const float x = 1.0f, y = 0.0f, z = 0.0f;
+    const float a = deg * (3.1415926536f / 180.0f);
+    const float sn = sinf(a), cs = cosf(a), c1 = 1.0f - cs;
+    const float m[9] = {cs + x * x * c1,     x * y * c1 - z * sn, x * z * c1 + y * sn,
+                        y * x * c1 + z * sn, cs + y * y * c1,     y * z * c1 - x * sn,
+                        z * x * c1 - y * sn, z * y * c1 + x * sn, cs + z * z * c1};
+    for (int i = 0; i < 9; ++i) r[i] = m[i];
+}
+
+// The procedural SDF mean's parameters (ProceduralMean::fromJson and prepareSdfQueries,
+// GPFunctions.cpp:250-266, ProceduralMeanSdf.cpp:11-23), kept on the host until
+// gpis_create copies them to the device.
+static gpis_status build_sdf_mean(gpis_ctx *c) {
+    const gpis_field_desc &d = c->desc;
+    sdf::MeanParams &M = c->sdf_host;
+    memset(&M, 0, sizeof M);
+    if (d.sdf_function < GPIS_SDF_KNOB || d.sdf_function > GPIS_SDF_THREE_SLABS)
+        return fail(c, GPIS_INVALID_ARGUMENT, "procedural mean: unknown SDF function");
+    M.fn = d.sdf_function;
+    M.queries = sdf::supports_queries(M.fn) ? 1 : 0;
+    double bound = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        M.t[i] = (double)d.sdf_inverse_transform[4 * i + 3];
+        double row = 0.0;
+        for (int j = 0; j < 3; ++j) {
+            M.w[3 * i + j] = (double)d.sdf_inverse_transform[4 * i + j];
+            row += fabs(M.w[3 * i + j]);
+        }
+        bound = fmax(bound, row);
+    }
+    for (int i = 0; i < 12; ++i) M.inv[i] = d.sdf_inverse_transform[i];
+    M.minv = d.sdf_min;
+    M.scale = d.sdf_scale;
+    M.offset = d.sdf_offset;
+    // prepareSdfQueries refuses a non-finite or zero transform, scale, offset or minimum
+    // for the functions with queries
+    if (M.queries) {
+        bool finite = std::isfinite(d.sdf_min) && std::isfinite(d.sdf_scale) && std::isfinite(d.sdf_offset);
+        for (int i = 0; i < 12; ++i) finite = finite && std::isfinite(d.sdf_inverse_transform[i]);
+        if (!finite || !(bound > 0.0))
+            return fail(c, GPIS_INVALID_ARGUMENT,
+                        "procedural SDF requires a finite invertible transform, scale, offset and minimum");
+    }
+    rot_axis_x(90.0f, M.rot_p90);
+    rot_axis_x(-90.0f, M.rot_m90);
+    rot_axis_x(-45.0f, M.rot_m45);
+    return GPIS_OK;
+}
+
 static gpis_status build_params(gpis_ctx *c) {
     const gpis_field_desc &d = c->desc;
     FieldParams &P = c->P;
@@ -69,12 +124,16 @@ static gpis_status build_params(gpis_ctx *c) {
     if (d.cov_kind != GPIS_COV_SE && d.cov_kind != GPIS_COV_PROC_NS) return fail(c, GPIS_UNSUPPORTED, "covariance kind");
     if (d.one_d && d.cov_kind != GPIS_COV_SE)
         return fail(c, GPIS_UNSUPPORTED, "1D sampling is served for the stationary SE kernel");
-    if (d.mean_kind < GPIS_MEAN_SPHERE || d.mean_kind > GPIS_MEAN_GRID) return fail(c, GPIS_UNSUPPORTED, "mean kind");
+    if (d.mean_kind < GPIS_MEAN_SPHERE || d.mean_kind > GPIS_MEAN_PROCEDURAL) return fail(c, GPIS_UNSUPPORTED, "mean kind");
+    if (d.mean_kind == GPIS_MEAN_PROCEDURAL) {
+        gpis_status s = build_sdf_mean(c);
+        if (s != GPIS_OK) return s;
+    }
     if (d.isotropic_3d && !d.one_d) {
         if (d.cov_kind != GPIS_COV_SE)
             return fail(c, GPIS_UNSUPPORTED, "isotropic_3D_sampling is served for the stationary SE kernel");
-        if (d.mean_kind == GPIS_MEAN_GRID)
-            return fail(c, GPIS_UNSUPPORTED, "isotropic_3D_sampling with a tabulated mean is not served");
+        if (d.mean_kind == GPIS_MEAN_GRID || d.mean_kind == GPIS_MEAN_PROCEDURAL)
+            return fail(c, GPIS_UNSUPPORTED, "isotropic_3D_sampling with a tabulated or procedural mean is not served");
         // splattingKernelRadius(true, 1) and sparseConvNoiseVariance3D(N, R, identity)
         // (GPFunctions.cpp:706-719, 751-764); the identity-space splat exp(-|R to|^2 / 2)
         P.iso3d = 1;
@@ -425,7 +484,8 @@ enum { ST_FLIGHTS = 0, ST_EVALS, ST_STEPS, ST_HITS, ST_EXITS, ST_FAILS, ST_INVAL
 // (or path) finished take the next ray indices from a global counter: one atomic
 // per wave, ranks from the ballot mask.  COND: the medium conditions secondary
 // flights (Renewal/Renewal+); PATHS: hits continue as diffuse bounces; GRID: the mean
-// is tabulated (the analytic means otherwise, whose variants stay free of its registers);
+// is evaluated (tabulated or procedural SDF; the analytic means otherwise, whose variants
+// stay free of its registers);
 // IRAY: the noise lives in each flight's isotropic ray space (isotropic_3D_sampling).
 template <int NI, bool ISO, bool COND, bool PATHS, bool GRID, bool IRAY = false>
 __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(FieldParams P, RayIn in, ReplyOut out,
@@ -595,7 +655,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
         if (phase == PH_NEED || phase == PH_DEAD || phase == PH_DONE) continue;
 
         // ---- one evaluation per live lane
-        Eval4 e = eval_field<NI, ISO, GRID ? MEAN_GRID : MEAN_ANALYTIC, IRAY>(P, rs, qx, qy, qz,
+        Eval4 e = eval_field<NI, ISO, GRID ? MEAN_EXT : MEAN_ANALYTIC, IRAY>(P, rs, qx, qy, qz,
                                                                                !COND || phase != PH_COND, dx, dy, dz);
         ++evals_flight;
         const float fv = e.v;
@@ -604,7 +664,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
         bool unit = false;
 
         if (COND && phase == PH_COND) {
-            condition_coefficients<GRID ? MEAN_GRID : MEAN_ANALYTIC, IRAY>(P, e, ox, oy, oz, prev_v, prev_gx, prev_gy,
+            condition_coefficients<GRID ? MEAN_EXT : MEAN_ANALYTIC, IRAY>(P, e, ox, oy, oz, prev_v, prev_gx, prev_gy,
                                                                           prev_gz, dx, dy, dz, rs);
             phase = PH_F0;
             tq = (double)near_t;
@@ -721,7 +781,7 @@ struct Nee1DOut {
 // Per-lane 1D field evaluation at q: normalised noise value and derivative along the
 // ray (conditioned when `cond`), then f = (float)(amplitude * noise + mean in double),
 // the reference's evaluateValue rounding.
-template <int NI>
+template <int NI, bool PROC>
 __device__ __forceinline__ void eval1d(const FieldParams &P, const Ray1D &R, Cells1D<NI> &C, bool &cache_valid,
                                        float nf, float qx, float qy, float qz, bool cond, float &f, float &nv,
                                        float &ndv) {
@@ -734,19 +794,20 @@ __device__ __forceinline__ void eval1d(const FieldParams &P, const Ray1D &R, Cel
     if (cond && R.cond) cond_splat1d(R, u, v, dv);
     nv = v;
     ndv = dv;
-    f = (float)((double)(P.amp_const * v) + mean_value_d(P, qx, qy, qz));
+    f = (float)((double)(P.amp_const * v) + mean_value_d<PROC>(P, qx, qy, qz));
 }
 
 // neeShared + neeGrad + neePDF (SparseConvolutionNoise.cpp:662-821) for the single-level
 // stationary kernel: the mirror normal `h`, the hit point p, the conditioned derivative
 // along the ray `dvp`; with `tr` the transverse density is the correlated law of
 // seTransverseGradientMoments (tmx, tmy, tvar).  Float arithmetic as the reference.
+template <bool PROC>
 __device__ __forceinline__ void nee1d(const FieldParams &P, const Ray1D &R, float hx, float hy, float hz, float px,
                                    float py, float pz, float dvp, float dx, float dy, float dz, bool tr, double tmx,
                                    double tmy, double tvar, float &gx, float &gy, float &gz, float &pdf) {
 #pragma clang fp contract(off)
     float mgx, mgy, mgz;
-    mean_grad_d(P, px, py, pz, mgx, mgy, mgz);
+    mean_grad_d<PROC>(P, px, py, pz, mgx, mgy, mgz);
     const float amp = P.amp_const;
     const float s0 = amp * R.ia[0], s1 = amp * R.ia[1], s2 = amp * R.ia[2];
     // M = amp * diag(1/l_aniso) * [tangent bitangent normal]
@@ -868,6 +929,7 @@ __device__ __forceinline__ void frame_to_global(const Frame3 &F, float x, float 
 // the origin (normalised value nv, derivative along the ray ndv): the value coefficient v1
 // and, for Renewal+, the derivative coefficient g1z and the observed transverse gradient
 // of the correlated law.
+template <bool PROC>
 __device__ __forceinline__ void condition1d(const FieldParams &P, Ray1D &R, float ox, float oy, float oz, float nv,
                                             float ndv, float prev_v, float prev_gx, float prev_gy, float prev_gz,
                                             float bx1, float by1, float &obs_x, float &obs_y, bool &observed) {
@@ -877,11 +939,11 @@ __device__ __forceinline__ void condition1d(const FieldParams &P, Ray1D &R, floa
     R.v1 = 0.f;
     R.g1z = 0.f;
     if (amp != 0.f) {
-        const float mean = (float)mean_value_d(P, ox, oy, oz);
+        const float mean = (float)mean_value_d<PROC>(P, ox, oy, oz);
         R.v1 = (prev_v - mean) / amp - nv;
         if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
             float mgx, mgy, mgz;
-            mean_grad_d(P, ox, oy, oz, mgx, mgy, mgz);
+            mean_grad_d<PROC>(P, ox, oy, oz, mgx, mgy, mgz);
             float cgx, cgy, cgz;
             ray_grad_to_world(R, bx1, by1, ndv, cgx, cgy, cgz);
             const float tgx = (prev_gx - mgx) / amp, tgy = (prev_gy - mgy) / amp, tgz = (prev_gz - mgz) / amp;
@@ -907,6 +969,7 @@ __device__ __forceinline__ void condition1d(const FieldParams &P, Ray1D &R, floa
 // otherwise an independent second pair, or the correlated law's pair 0 about the conditional
 // mean (:186-196) — then resumeDistance's rejection rules in double
 // (GaussianProcessMedium.cpp:289-369).  Returns the outcome.
+template <bool PROC>
 __device__ __forceinline__ uint32_t finish1d(const FieldParams &P, const Ray1D &R, bool corr, bool observed,
                                              float obs_x, float obs_y, float ox, float oy, float oz, float qx,
                                              float qy, float qz, float dx, float dy, float dz, uint32_t normal_seed,
@@ -929,7 +992,7 @@ __device__ __forceinline__ uint32_t finish1d(const FieldParams &P, const Ray1D &
     float wx, wy, wz;
     ray_grad_to_world(R, tx2, ty2, ndv_end, wx, wy, wz);
     float mgx, mgy, mgz;
-    mean_grad_d(P, qx, qy, qz, mgx, mgy, mgz);
+    mean_grad_d<PROC>(P, qx, qy, qz, mgx, mgy, mgz);
     const float amp = P.amp_const;
     {
 #pragma clang fp contract(off)
@@ -957,7 +1020,7 @@ __device__ __forceinline__ uint32_t finish1d(const FieldParams &P, const Ray1D &
 // hit-time phase runs in the batched finish step (the light sample with its shadow
 // flight, then the mirror or Lambert bounce) and the next flight starts from the hit,
 // conditioned on (0, gradient), without leaving the kernel.
-template <int NI, bool PATHS>
+template <int NI, bool PATHS, bool PROC>
 __global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(FieldParams P, RayIn in, const float *__restrict__ lscale,
                                                           ReplyOut out, unsigned int n, unsigned int *work,
                                                           unsigned long long *stats, float nf, int use_light,
@@ -1111,7 +1174,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(
                 shadow = false;
                 nee_stage = 3;
             } else if (phase == PH_DONE && nee_stage == 0) {
-                outcome = finish1d(P, R, corr, observed, obs_x, obs_y, ox, oy, oz, qx, qy, qz, dx, dy, dz, normal_seed,
+                outcome = finish1d<PROC>(P, R, corr, observed, obs_x, obs_y, ox, oy, oz, qx, qy, qz, dx, dy, dz, normal_seed,
                                    t_end, ndv_end, hit, gxv, gyv, gzv);
                 if (!PATHS) {
                     if (use_light && (outcome & 0xFFu) == GPIS_HIT) {
@@ -1200,7 +1263,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(
                 double tmx = 0.0, tmy = 0.0, tvar = 0.5;
                 if (corr) transverse_moments(R, observed, obs_x, obs_y, ox, oy, oz, qx, qy, qz, tmx, tmy, tvar);
                 float ngx, ngy, ngz, pdf;
-                nee1d(P, R, hnx_, hny_, hnz_, qx, qy, qz, ndv_nee, dx, dy, dz, corr, tmx, tmy, tvar, ngx, ngy, ngz,
+                nee1d<PROC>(P, R, hnx_, hny_, hnz_, qx, qy, qz, ndv_nee, dx, dy, dz, corr, tmx, tmy, tvar, ngx, ngy, ngz,
                       pdf);
                 if (!PATHS) {
                     nee.gx[ray] = ngx; nee.gy[ray] = ngy; nee.gz[ray] = ngz; nee.pdf[ray] = pdf;
@@ -1337,13 +1400,13 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(
         if (phase == PH_NEED || phase == PH_DEAD || phase == PH_DONE) continue;
 
         float fv, nv, ndv;
-        eval1d<NI>(P, R, C, cache_valid, nf, qx, qy, qz, phase != PH_COND, fv, nv, ndv);
+        eval1d<NI, PROC>(P, R, C, cache_valid, nf, qx, qy, qz, phase != PH_COND, fv, nv, ndv);
         ++evals_flight;
         double tq = 0.0;
         bool unit = false;
 
         if (phase == PH_COND) {
-            condition1d(P, R, ox, oy, oz, nv, ndv, prev_v, prev_gx, prev_gy, prev_gz, bx1, by1, obs_x, obs_y, observed);
+            condition1d<PROC>(P, R, ox, oy, oz, nv, ndv, prev_v, prev_gx, prev_gy, prev_gz, bx1, by1, obs_x, obs_y, observed);
             phase = PH_F0;
             tq = (double)near_t;
         } else if (phase == PH_F0) {
@@ -1458,7 +1521,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(
 // rounds: a launch over a few thousand rays no longer waits ~steps rounds for its longest
 // flight.  Ray indices come from idx[0 .. *n_dev).
 constexpr int kGroup = 16;
-template <int NI>
+template <int NI, bool PROC>
 __global__ __launch_bounds__(256, 2) void flight1d_group_kernel(FieldParams P, RayIn in,
                                                                 const float *__restrict__ lscale, ReplyOut out,
                                                                 unsigned int *work, unsigned long long *stats,
@@ -1576,11 +1639,11 @@ __global__ __launch_bounds__(256, 2) void flight1d_group_kernel(FieldParams P, R
             point_at(ox, oy, oz, dx, dy, dz, tq, unit, qx, qy, qz);
         }
         float fv, nv, ndv;
-        eval1d<NI>(P, R, C, cache_valid, nf, qx, qy, qz, phase != PH_COND, fv, nv, ndv);
+        eval1d<NI, PROC>(P, R, C, cache_valid, nf, qx, qy, qz, phase != PH_COND, fv, nv, ndv);
 
         if (phase == PH_COND) {
             ++evals_u;
-            condition1d(P, R, ox, oy, oz, nv, ndv, prev_v, prev_gx, prev_gy, prev_gz, bx1, by1, obs_x, obs_y, observed);
+            condition1d<PROC>(P, R, ox, oy, oz, nv, ndv, prev_v, prev_gx, prev_gy, prev_gz, bx1, by1, obs_x, obs_y, observed);
             phase = PH_F0;
             tq = (double)near_t;
             unit = false;
@@ -1678,7 +1741,7 @@ __global__ __launch_bounds__(256, 2) void flight1d_group_kernel(FieldParams P, R
         } else if (phase == PH_GRAD) {
             ++evals_u;
             float gxv, gyv, gzv;
-            const uint32_t outcome = finish1d(P, R, corr, observed, obs_x, obs_y, ox, oy, oz, qx, qy, qz, dx, dy, dz,
+            const uint32_t outcome = finish1d<PROC>(P, R, corr, observed, obs_x, obs_y, ox, oy, oz, qx, qy, qz, dx, dy, dz,
                                               normal_seed, t_end, ndv, hit, gxv, gyv, gzv);
             if (sub == 0) {
                 const unsigned oc = outcome & 0xFFu;
@@ -1713,7 +1776,7 @@ static void launch_flights(gpis_ctx *c, unsigned blocks, unsigned threads, hipSt
                                c->batch_k);                                                                        \
     } while (0)
     const bool cond = c->P.activate != 0;
-    const bool grid = c->P.mean_kind == GPIS_MEAN_GRID;
+    const bool grid = c->P.mean_kind == GPIS_MEAN_GRID || c->P.mean_kind == GPIS_MEAN_PROCEDURAL;
     if (c->P.iso3d) {
         // the isotropic ray space: generic impulse count, isotropic kernel, analytic means
         if (cond)
@@ -1755,10 +1818,16 @@ gpis_status gpis_create(const gpis_field_desc *desc, gpis_ctx **out) {
     if (e == hipSuccess) e = hipMalloc((void **)&c->counters, 16 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->counters, 0, 16 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void **)&c->work, 64);
+    if (e == hipSuccess && c->desc.mean_kind == GPIS_MEAN_PROCEDURAL) {
+        e = hipMalloc((void **)&c->sdf_dev, sizeof(sdf::MeanParams));
+        if (e == hipSuccess) e = hipMemcpy(c->sdf_dev, &c->sdf_host, sizeof(sdf::MeanParams), hipMemcpyHostToDevice);
+        c->P.sdf = c->sdf_dev;
+    }
     if (e != hipSuccess) {
         fprintf(stderr, "gpis_create: %s\n", hipGetErrorString(e));
         if (c->counters) (void)hipFree(c->counters);
         if (c->work) (void)hipFree(c->work);
+        if (c->sdf_dev) (void)hipFree(c->sdf_dev);
         delete c;
         return GPIS_DEVICE_ERROR;
     }
@@ -1771,6 +1840,7 @@ gpis_status gpis_destroy(gpis_ctx *c) {
     (void)hipDeviceSynchronize();
     if (c->table) (void)hipFree(c->table);
     if (c->grid) (void)hipFree(c->grid);
+    if (c->sdf_dev) (void)hipFree(c->sdf_dev);
     if (c->counters) (void)hipFree(c->counters);
     if (c->work) (void)hipFree(c->work);
     delete c;
@@ -1980,10 +2050,16 @@ static gpis_status launch_flight1d(gpis_ctx *c, const gpis_ray_soa *rays, int64_
     long long want = (long long)c->num_cus * wpc * 64 / threads;
     long long need = (n + threads - 1) / threads;
     unsigned blocks = (unsigned)(need < want ? need : (want < 1 ? 1 : want));
-    hipLaunchKernelGGL((flight1d_kernel<8, false>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
-                       rays->length_scale, out, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
-                       light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, nee, idx, n_dev,
-                       c->batch_k_1d, Path1DIn{}, Path1DOut{});
+    if (c->P.mean_kind == GPIS_MEAN_PROCEDURAL)
+        hipLaunchKernelGGL((flight1d_kernel<8, false, true>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+                           rays->length_scale, out, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
+                           light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, nee, idx, n_dev,
+                           c->batch_k_1d, Path1DIn{}, Path1DOut{});
+    else
+        hipLaunchKernelGGL((flight1d_kernel<8, false, false>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+                           rays->length_scale, out, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
+                           light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, nee, idx, n_dev,
+                           c->batch_k_1d, Path1DIn{}, Path1DOut{});
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }
@@ -2027,10 +2103,16 @@ gpis_status gpis_trace_paths_1d(gpis_ctx *c, const gpis_ray_soa *rays, const gpi
     Path1DOut pout = {reply->t0,      reply->gx0,   reply->gy0,    reply->gz0,          reply->outcome0,
                       reply->flights, reply->hits,  reply->shadows, reply->visible,     reply->nee,
                       reply->t_last,  reply->outcome_last, reply->evals};
-    hipLaunchKernelGGL((flight1d_kernel<8, true>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
-                       rays->length_scale, ReplyOut{}, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
-                       light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, Nee1DOut{}, nullptr,
-                       nullptr, c->batch_k_1d, pin, pout);
+    if (c->P.mean_kind == GPIS_MEAN_PROCEDURAL)
+        hipLaunchKernelGGL((flight1d_kernel<8, true, true>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+                           rays->length_scale, ReplyOut{}, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
+                           light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, Nee1DOut{}, nullptr,
+                           nullptr, c->batch_k_1d, pin, pout);
+    else
+        hipLaunchKernelGGL((flight1d_kernel<8, true, false>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+                           rays->length_scale, ReplyOut{}, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
+                           light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, Nee1DOut{}, nullptr,
+                           nullptr, c->batch_k_1d, pin, pout);
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }
@@ -2043,8 +2125,12 @@ static gpis_status launch_flight1d_groups(gpis_ctx *c, const gpis_ray_soa *rays,
     const long long want = (long long)c->num_cus * 2;  // two blocks of four waves per CU (2 waves per SIMD)
     const long long need = (n + groups_per_block - 1) / groups_per_block;
     const unsigned blocks = (unsigned)(need < want ? need : (want < 1 ? 1 : want));
-    hipLaunchKernelGGL((flight1d_group_kernel<8>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
-                       rays->length_scale, out, c->work, c->counters, c->nf1d, idx, n_dev);
+    if (c->P.mean_kind == GPIS_MEAN_PROCEDURAL)
+        hipLaunchKernelGGL((flight1d_group_kernel<8, true>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+                           rays->length_scale, out, c->work, c->counters, c->nf1d, idx, n_dev);
+    else
+        hipLaunchKernelGGL((flight1d_group_kernel<8, false>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+                           rays->length_scale, out, c->work, c->counters, c->nf1d, idx, n_dev);
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }

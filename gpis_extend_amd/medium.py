@@ -19,6 +19,7 @@ from typing import Dict, Optional
 import numpy as np
 
 from . import _capi as A
+from . import mat4f
 from .hashing import xxhash32_1, xxhash32_4
 
 
@@ -143,6 +144,13 @@ class SparseConvolutionNoiseMedium:
     mean_offset: float = 0.0
     mean_scale: float = 1.0
     mean_volume: bool = False
+    # ProceduralMean (GPFunctions.cpp:250-266): SdfFunctions function, the float inverse of
+    # its transform (rows 0-2, row-major), min, scale, offset
+    sdf_function: int = 0
+    sdf_inverse_transform: tuple = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0)
+    sdf_min: float = -3.4028234663852886e38
+    sdf_scale: float = 1.0
+    sdf_offset: float = 0.0
     extra: Dict = field(default_factory=dict)
 
     @classmethod
@@ -204,6 +212,26 @@ class SparseConvolutionNoiseMedium:
                 raise MediumConfigError(
                     f"tabulated mean grid of type {g.get('type')!r}: pass its dense export "
                     "(origin, dims, voxels, world_to_index, clamp box) as mean_grid")
+        elif mt == "procedural":
+            m.mean_kind = A.MEAN_PROCEDURAL
+            if "func" in mean:
+                fn = mean["func"]
+            elif "f" in mean:
+                raise MediumConfigError("a procedural mean over a procedural scalar \"f\" is not served; "
+                                        "name an SDF function in \"func\"")
+            else:
+                fn = "knob"
+            if fn not in A.SDF_FUNCTIONS:
+                raise MediumConfigError(f"Invalid sdf function: '{fn}'")
+            m.sdf_function = A.SDF_FUNCTIONS.index(fn)
+            try:
+                inv = mat4f.invert(mat4f.from_json(mean.get("transform")))
+            except ValueError as e:
+                raise MediumConfigError(f"procedural mean transform: {e}") from None
+            m.sdf_inverse_transform = tuple(float(v) for v in inv[:12])
+            m.sdf_min = float(np.float32(mean.get("min", m.sdf_min)))
+            m.sdf_scale = float(mean.get("scale", 1.0))
+            m.sdf_offset = float(mean.get("offset", 0.0))
         else:
             raise MediumConfigError(f"mean type {mt!r} is not served by the GPU flight path")
         cov = gp.get("covariance", {"type": "squared_exponential"})
@@ -216,10 +244,17 @@ class SparseConvolutionNoiseMedium:
                 m.ns_ls_field = A.LS_SINCOS
                 m.ns_ls_params = (float(ls["base"]), float(ls["amplitude"]), float(ls["frequency"]), 0.0)
                 m.ns_ls_max = float(ls["base"]) + float(ls["amplitude"])
-            else:
+            elif ls.get("type", "constant") == "constant":
                 v = float(ls.get("value", 1.0))
                 m.ns_ls_field, m.ns_ls_params, m.ns_ls_max = A.LS_CONST, (v, 0.0, 0.0, 0.0), v
-            m.ns_var = float(cov.get("var", 1.0))
+            else:
+                raise MediumConfigError(f"length-scale field {ls.get('type')!r} is not served by the GPU flight path")
+            var = cov.get("var", 1.0)
+            if isinstance(var, dict):
+                if var.get("type", "constant") != "constant":
+                    raise MediumConfigError(f"variance field {var.get('type')!r} is not served by the GPU flight path")
+                var = var.get("value", 1.0)
+            m.ns_var = float(var)
             cov = cov.get("cov", {"type": "squared_exponential"})
             ct = cov.get("type")
         if ct != "squared_exponential":
@@ -257,6 +292,10 @@ class SparseConvolutionNoiseMedium:
         d.correlation_xy = int(self.correlation_xy)
         d.step_size, d.min_step, d.max_bounce = self.step_size, self.min_step, self.max_bounces
         d.isotropic_3d = int(self.isotropic_3D_sampling and not self.one_d)
+        d.sdf_function = self.sdf_function
+        for i in range(12):
+            d.sdf_inverse_transform[i] = self.sdf_inverse_transform[i]
+        d.sdf_min, d.sdf_scale, d.sdf_offset = self.sdf_min, self.sdf_scale, self.sdf_offset
         return d
 
     @property

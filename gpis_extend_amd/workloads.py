@@ -90,6 +90,21 @@ def with_tabulated_mean(js, offset=0.0, scale=1.0, volume=False):
     return out
 
 
+def with_procedural_mean(js, func="knob", transform=None, scale=1.0, offset=0.0, minimum=None):
+    """The medium `js` with its mean replaced by a ProceduralMean over an SdfFunctions
+    function, in the reference's scene form (data/example-scenes/two-spheres-backlight/
+    scene.json: func "two_spheres", transform {"scale": 0.1}, scale 0.1)."""
+    import copy
+    out = copy.deepcopy(js)
+    mean = {"type": "procedural", "func": func, "scale": float(scale), "offset": float(offset)}
+    if transform is not None:
+        mean["transform"] = transform
+    if minimum is not None:
+        mean["min"] = float(minimum)
+    out["gaussian_process"]["mean"] = mean
+    return out
+
+
 def sphere_sdf_grid(radius=0.5, center=(0.0, 0.0, 0.0), half=1.06, spacing=0.02):
     """Signed distance to a sphere tabulated on [-half, half]^3 with the given voxel spacing
     (the dense export of an SDF grid like the reference's shaderball_sdf.vdb)."""

@@ -35,8 +35,12 @@ typedef enum gpis_status {
  * ProceduralNonstationaryCovariance (GPFunctions.hpp:2459-2505). */
 enum { GPIS_COV_SE = 0, GPIS_COV_PROC_NS = 1 };
 /* Mean kinds: SphericalMean (GPFunctions.hpp:952-996); box SDF and plane in code;
- * TabulatedMean over a dense grid (GPFunctions.cpp:165-190), set by gpis_set_mean_grid. */
-enum { GPIS_MEAN_SPHERE = 0, GPIS_MEAN_BOX = 1, GPIS_MEAN_PLANE = 2, GPIS_MEAN_GRID = 3 };
+ * TabulatedMean over a dense grid (GPFunctions.cpp:165-190), set by gpis_set_mean_grid;
+ * ProceduralMean over an SdfFunctions function ("func", GPFunctions.cpp:250-298). */
+enum { GPIS_MEAN_SPHERE = 0, GPIS_MEAN_BOX = 1, GPIS_MEAN_PLANE = 2, GPIS_MEAN_GRID = 3, GPIS_MEAN_PROCEDURAL = 4 };
+/* SdfFunctions::Function (SdfFunctions.hpp:13-22), the "func" of a procedural mean. */
+enum { GPIS_SDF_KNOB = 0, GPIS_SDF_KNOB_INNER = 1, GPIS_SDF_KNOB_OUTER = 2, GPIS_SDF_TWO_SPHERES = 3,
+       GPIS_SDF_PLANE = 4, GPIS_SDF_ROUNDED_U_GROOVE = 5, GPIS_SDF_TWO_SLABS = 6, GPIS_SDF_THREE_SLABS = 7 };
 /* GPCorrelationContext (GaussianProcessMedium.cpp:20-42). */
 enum { GPIS_CTX_GLOBAL = 0, GPIS_CTX_RENEWAL = 1, GPIS_CTX_RENEWAL_PLUS = 2, GPIS_CTX_NONE = 3 };
 /* SparseConv1DSamplingScheme (Medium.hpp:42-46). */
@@ -73,6 +77,10 @@ typedef struct gpis_field_desc {
     int32_t max_bounce;                        /* Medium "max_bounces" */
     int32_t isotropic_3d;                      /* "isotropic_3D_sampling": 3D noise in the isotropic
                                                   ray space of each flight (stationary SE kernel) */
+    /* GPIS_MEAN_PROCEDURAL (ProceduralMean::fromJson, GPFunctions.cpp:250-266) */
+    int32_t sdf_function;                      /* "func", GPIS_SDF_* */
+    float sdf_inverse_transform[12];           /* _invConfigTransform rows 0-2, row-major */
+    float sdf_min, sdf_scale, sdf_offset;      /* "min" (-FLT_MAX), "scale" (1), "offset" (0) */
 } gpis_field_desc;
 
 /* A wavefront of flight requests in structure-of-arrays form

@@ -461,6 +461,10 @@ static double mean_value(const orc_field *f, v3d p) {
         double a[3] = {p.x, p.y, p.z};
         return orc_grid_mean(d, a);
     }
+    case ORC_MEAN_PROC: {
+        double a[3] = {p.x, p.y, p.z};
+        return orc_sdf_mean(d, a, NULL);
+    }
     case ORC_MEAN_BOX: {
         double q[3], o2 = 0.0, mx = -1e300;
         double pp[3] = {p.x - d->mean_center[0], p.y - d->mean_center[1], p.z - d->mean_center[2]};
@@ -488,6 +492,11 @@ static v3d mean_grad(const orc_field *f, v3d p) {
         orc_grid_mean_grad(d, a, g);
         return D3(g[0], g[1], g[2]);
     }
+    case ORC_MEAN_PROC: {
+        double a[3] = {p.x, p.y, p.z}, g[3];
+        orc_sdf_mean(d, a, g);
+        return D3(g[0], g[1], g[2]);
+    }
     case ORC_MEAN_BOX: {
         double pp[3] = {p.x - d->mean_center[0], p.y - d->mean_center[1], p.z - d->mean_center[2]};
         double q[3], g[3] = {0, 0, 0}, o2 = 0.0;
@@ -509,6 +518,279 @@ static v3d mean_grad(const orc_field *f, v3d p) {
         return D3(d->plane_normal[0], d->plane_normal[1], d->plane_normal[2]);
     default:
         return dnorm(D3(p.x - d->mean_center[0], p.y - d->mean_center[1], p.z - d->mean_center[2]));
+    }
+}
+
+/* ---- procedural SDF means (GPFunctions.cpp:250-298, ProceduralMeanSdf.cpp:11-38,
+ *      SdfFunctionsQueries.cpp:14-466) ---------------------------------
+ * The shapes are evaluated in one of two modes, as the reference instantiates its CSG
+ * templates twice: point jets (value, gradient and the constant/affine flags of
+ * PointJet, :127-187) for the functions with queries, plain doubles (the float API,
+ * :374-394) for knob_inner and knob_outer. */
+typedef struct { double v, g[3]; int cst, aff; } sj;
+typedef struct { int jet; float rp90[9], rm90[9], rm45[9]; } sdf_mode;
+
+/* std::hypot(x, y, z) of libstdc++ 11: scale by the largest magnitude. */
+static double sdf_hypot3(double x, double y, double z) {
+    double m;
+    x = fabs(x); y = fabs(y); z = fabs(z);
+    m = x < y ? (y < z ? z : y) : (x < z ? z : x);
+    if (m == 0.0) return 0.0;
+    return m * sqrt((x / m) * (x / m) + (y / m) * (y / m) + (z / m) * (z / m));
+}
+/* guard(v) = 128 eps (1 + |v|) (SdfFunctionsQueries.cpp:15) */
+static double sdf_guard(double v) { return 128.0 * 2.220446049250313e-16 * (1.0 + fabs(v)); }
+static sj sj_c(double v) { sj r; r.v = v; r.g[0] = r.g[1] = r.g[2] = 0.0; r.cst = 1; r.aff = 1; return r; }
+static sj sj_add(sj a, sj b) {
+    sj r;
+    r.v = a.v + b.v;
+    for (int i = 0; i < 3; ++i) r.g[i] = a.g[i] + b.g[i];
+    r.cst = a.cst && b.cst;
+    r.aff = a.aff && b.aff;
+    return r;
+}
+static sj sj_mul(const sdf_mode *m, sj a, double s) {
+    if (m->jet && s == 0.0) return sj_c(0.0);
+    a.v = a.v * s;
+    for (int i = 0; i < 3; ++i) a.g[i] = a.g[i] * s;
+    return a;
+}
+static sj sj_neg(const sdf_mode *m, sj a) {
+    if (!m->jet) { a.v = -a.v; return a; }
+    return sj_mul(m, a, -1.0);
+}
+static sj sj_sub(const sdf_mode *m, sj a, sj b) {
+    if (!m->jet) { a.v = a.v - b.v; return a; }
+    return sj_add(a, sj_neg(m, b));
+}
+static sj sj_min(const sdf_mode *m, sj a, sj b) {
+    sj r;
+    if (!m->jet) return b.v < a.v ? b : a;                    /* std::min */
+    if (a.cst && b.cst) return sj_c(b.v < a.v ? b.v : a.v);
+    if (a.v + sdf_guard(a.v) < b.v - sdf_guard(b.v)) return a;
+    if (b.v + sdf_guard(b.v) < a.v - sdf_guard(a.v)) return b;
+    r = a.v <= b.v ? a : b;
+    r.cst = 0; r.aff = 0;
+    return r;
+}
+static sj sj_max(const sdf_mode *m, sj a, sj b) {
+    if (!m->jet) return a.v < b.v ? b : a;                    /* std::max */
+    return sj_neg(m, sj_min(m, sj_neg(m, a), sj_neg(m, b)));
+}
+static int sj_zero(sj a) { return a.v == 0.0 && a.cst; }
+static sj sj_norm(const sdf_mode *m, sj a, sj b, sj c) {
+    sj r, q[3];
+    if (!m->jet) return sj_c(sdf_hypot3(a.v, b.v, c.v));
+    if (sj_zero(b) && sj_zero(c)) return sj_max(m, a, sj_neg(m, a));
+    if (sj_zero(a) && sj_zero(c)) return sj_max(m, b, sj_neg(m, b));
+    if (sj_zero(a) && sj_zero(b)) return sj_max(m, c, sj_neg(m, c));
+    q[0] = a; q[1] = b; q[2] = c;
+    r = sj_c(sdf_hypot3(a.v, b.v, c.v));
+    if (r.v > 0.0)
+        for (int k = 0; k < 3; ++k) {
+            double ratio = q[k].v / r.v;
+            for (int i = 0; i < 3; ++i) r.g[i] = r.g[i] + q[k].g[i] * ratio;
+        }
+    r.cst = a.cst && b.cst && c.cst;
+    if (a.aff && b.aff && c.aff) {
+        int z = 1;
+        for (int k = 0; k < 3; ++k)
+            for (int i = 0; i < 3; ++i) z = z && q[k].g[i] == 0.0;
+        r.cst = z;
+    }
+    r.aff = r.cst;
+    return r;
+}
+static sj sj_smooth_union(const sdf_mode *m, sj a, sj b, double k) {
+    double d = b.v - a.v, h;
+    sj r;
+    if (m->jet) {
+        double err = sdf_guard(a.v) + sdf_guard(b.v);
+        if (d - err >= k) return a;
+        if (d + err <= -k) return b;
+    }
+    h = 0.5 + 0.5 * d / k;
+    if (h < 0.0) h = 0.0;
+    else if (h > 1.0) h = 1.0;
+    r = sj_c(b.v + (a.v - b.v) * h - k * h * (1.0 - h));
+    for (int i = 0; i < 3; ++i) r.g[i] = h * a.g[i] + (1.0 - h) * b.g[i];
+    r.cst = a.cst && b.cst;
+    r.aff = r.cst;
+    return r;
+}
+static sj sj_subtract(const sdf_mode *m, sj a, sj b, double k) {
+    return sj_neg(m, sj_smooth_union(m, a, sj_neg(m, b), k));
+}
+static sj sj_abs(const sdf_mode *m, sj a) { return sj_max(m, a, sj_neg(m, a)); }
+static void sj_shift(sj out[3], const sj *p, double x, double y, double z) {
+    out[0] = sj_add(p[0], sj_c(x));
+    out[1] = sj_add(p[1], sj_c(y));
+    out[2] = sj_add(p[2], sj_c(z));
+}
+static void sj_rotate(const sdf_mode *m, sj out[3], const sj *p, const float r[9]) {
+    for (int i = 0; i < 3; ++i)
+        out[i] = sj_add(sj_add(sj_mul(m, p[0], (double)r[3 * i]), sj_mul(m, p[1], (double)r[3 * i + 1])),
+                        sj_mul(m, p[2], (double)r[3 * i + 2]));
+}
+static sj sj_sphere(const sdf_mode *m, const sj *p, double r) {
+    return sj_sub(m, sj_norm(m, p[0], p[1], p[2]), sj_c(r));
+}
+static sj sj_cone(const sdf_mode *m, const sj *p) {
+    return sj_mul(m, sj_add(sj_norm(m, p[0], p[1], sj_c(0.0)), p[2]), (double)(3.1415926536f / 3.f));
+}
+static sj sj_cylinder(const sdf_mode *m, const sj *p, double h, double r) {
+    sj x = sj_sub(m, sj_norm(m, p[0], p[2], sj_c(0.0)), sj_c(h));
+    sj y = sj_sub(m, sj_abs(m, p[1]), sj_c(r));
+    return sj_add(sj_min(m, sj_max(m, x, y), sj_c(0.0)),
+                  sj_norm(m, sj_max(m, x, sj_c(0.0)), sj_max(m, y, sj_c(0.0)), sj_c(0.0)));
+}
+static sj sj_prism(const sdf_mode *m, const sj *p) {
+    sj zside = sj_sub(m, sj_abs(m, p[2]), sj_c((double)0.3f));
+    sj slant = sj_add(sj_mul(m, sj_abs(m, p[0]), (double)0.866025f), sj_mul(m, p[1], 0.5));
+    return sj_max(m, zside, sj_sub(m, sj_max(m, slant, sj_neg(m, p[1])), sj_c((double)(1.2f * 0.5f))));
+}
+/* knobGeometry (SdfFunctionsQueries.cpp:245-267) */
+static sj sj_knob(const sdf_mode *m, const sj *p0, int variant) {
+    const double size = (double)0.8f;
+    sj p[3], t[3], u[3], b[3], outer, cutout, etch, d, inner, base, cone_a, cone_b;
+    for (int i = 0; i < 3; ++i) p[i] = sj_mul(m, p0[i], (double)(float)(1.0 / size));
+    if (variant == ORC_SDF_KNOB_INNER) return sj_mul(m, sj_sphere(m, p, 0.75), size);
+    outer = sj_sphere(m, p, 1.0);
+    sj_shift(t, p, 0.0, -(double)0.5f, -(double)0.5f);
+    cutout = sj_sphere(m, t, (double)0.7f);
+    sj_shift(t, p, 0.0, -(double)0.2f, -(double)0.2f);
+    sj_rotate(m, u, t, m->rm45);
+    etch = sj_sub(m, sj_norm(m, sj_sub(m, sj_norm(m, u[0], u[2], sj_c(0.0)), sj_c((double)1.0f)), u[1], sj_c(0.0)),
+                  sj_c((double)0.05f));
+    d = sj_subtract(m, cutout, outer, (double)0.1f);
+    inner = sj_sphere(m, p, 0.75);
+    d = variant == ORC_SDF_KNOB_OUTER ? sj_max(m, d, sj_neg(m, inner)) : sj_min(m, d, inner);
+    d = sj_max(m, sj_neg(m, etch), d);
+    sj_shift(b, p, 0.0, (double)0.775f, 0.0);
+    sj_shift(t, b, 0.0, (double)0.9f, 0.0);
+    sj_rotate(m, u, t, m->rm90);
+    cone_a = sj_cone(m, u);
+    sj_shift(t, b, 0.0, -(double)0.9f, 0.0);
+    sj_rotate(m, u, t, m->rp90);
+    cone_b = sj_cone(m, u);
+    base = sj_smooth_union(m, cone_a, cone_b, (double)0.02f);
+    base = sj_mul(m, sj_max(m, base, sj_cylinder(m, b, (double)1.1f, (double)0.25f)), (double)0.7f);
+    base = sj_max(m, sj_neg(m, sj_cylinder(m, b, (double)0.6f, (double)0.3f)), base);
+    sj_shift(t, b, 0.0, 0.0, -1.0);
+    sj_rotate(m, u, t, m->rp90);
+    base = sj_max(m, sj_neg(m, sj_prism(m, u)), base);
+    return sj_mul(m, sj_min(m, sj_subtract(m, outer, base, (double)0.1f), d), size);
+}
+static sj sj_slab(const sdf_mode *m, const sj *p) {
+    sj x = sj_sub(m, sj_abs(m, p[0]), sj_c(0.6));
+    sj y = sj_sub(m, sj_abs(m, p[1]), sj_c(0.01));
+    sj z = sj_sub(m, sj_abs(m, p[2]), sj_c(0.6));
+    return sj_max(m, sj_max(m, x, y), z);
+}
+static sj sj_two_slabs(const sdf_mode *m, const sj *p) {
+    sj a[3], b[3];
+    sj_shift(a, p, -0.18, -0.02, 0.0);
+    sj_shift(b, p, 0.18, 0.02, 0.0);
+    return sj_min(m, sj_slab(m, a), sj_slab(m, b));
+}
+/* Mat4f::rotAxis((1, 0, 0), angle) rows 0-2 (Mat4f.cpp:132-148), in float. */
+static void sdf_rot_x(float deg, float r[9]) {
+    const float x = 1.0f, y = 0.0f, z = 0.0f;
+    float a = deg * (3.1415926536f / 180.0f), s = sinf(a), c = cosf(a), c1 = 1.0f - c;
+    r[0] = c + x * x * c1;     r[1] = x * y * c1 - z * s; r[2] = x * z * c1 + y * s;
+    r[3] = y * x * c1 + z * s; r[4] = c + y * y * c1;     r[5] = y * z * c1 - x * s;
+    r[6] = z * x * c1 - y * s; r[7] = z * y * c1 + x * s; r[8] = c + z * z * c1;
+}
+static void sdf_mode_init(sdf_mode *m, int jet) {
+    m->jet = jet;
+    sdf_rot_x(90.0f, m->rp90);
+    sdf_rot_x(-90.0f, m->rm90);
+    sdf_rot_x(-45.0f, m->rm45);
+}
+double orc_sdf_jet(int fn, const double p[3], double g[3]) {
+    sdf_mode m;
+    sj q[3], r, s[3];
+    if (fn == ORC_SDF_ROUNDED_U_GROOVE) {
+        /* roundedUJet (:323-327) */
+        double qy = 0.0 < p[1] ? 0.0 : p[1], rr = sqrt(p[0] * p[0] + qy * qy);
+        g[0] = rr > 0.0 ? -p[0] / rr : 0.0;
+        g[1] = rr > 0.0 ? -qy / rr : 0.0;
+        g[2] = rr > 0.0 ? -0.0 / rr : 0.0;
+        return 1.0 - rr;
+    }
+    sdf_mode_init(&m, 1);
+    for (int i = 0; i < 3; ++i) {
+        q[i] = sj_c(p[i]);
+        q[i].g[i] = 1.0;
+        q[i].cst = 0;
+    }
+    if (fn == ORC_SDF_TWO_SLABS) r = sj_two_slabs(&m, q);
+    else if (fn == ORC_SDF_THREE_SLABS) {
+        sj_shift(s, q, -0.18, 0.06, 0.0);
+        r = sj_min(&m, sj_two_slabs(&m, q), sj_slab(&m, s));
+    } else if (fn == ORC_SDF_TWO_SPHERES) {
+        s[0] = q[0];
+        s[1] = sj_sub(&m, sj_abs(&m, q[1]), sj_c(10.0));
+        s[2] = q[2];
+        r = sj_sphere(&m, s, 9.5);
+    } else r = sj_knob(&m, q, ORC_SDF_KNOB);
+    for (int i = 0; i < 3; ++i) g[i] = r.g[i];
+    return r.v;
+}
+float orc_sdf_eval(int fn, const float p[3]) {
+    sdf_mode m;
+    sj q[3];
+    if (fn == ORC_SDF_PLANE) return p[1];
+    sdf_mode_init(&m, 0);
+    for (int i = 0; i < 3; ++i) q[i] = sj_c((double)p[i]);
+    return (float)sj_knob(&m, q, fn).v;
+}
+static int sdf_has_queries(int fn) {
+    return fn == ORC_SDF_KNOB || fn == ORC_SDF_ROUNDED_U_GROOVE || fn == ORC_SDF_TWO_SPHERES ||
+           fn == ORC_SDF_TWO_SLABS || fn == ORC_SDF_THREE_SLABS;
+}
+/* ProceduralMean::mean without queries (GPFunctions.cpp:282-289) */
+static double sdf_mean_plain(const orc_medium_desc *d, const double a[3]) {
+    const float *t = d->sdf_inverse_transform;
+    float p[3] = {(float)a[0], (float)a[1], (float)a[2]}, l[3], v;
+    for (int i = 0; i < 3; ++i) l[i] = t[4 * i] * p[0] + t[4 * i + 1] * p[1] + t[4 * i + 2] * p[2] + t[4 * i + 3];
+    v = orc_sdf_eval(d->sdf_function, l);
+    v *= d->sdf_scale;
+    v = v + d->sdf_offset;
+    return (double)(d->sdf_min > v ? d->sdf_min : v);
+}
+double orc_sdf_mean(const orc_medium_desc *d, const double a[3], double g[3]) {
+    const float *t = d->sdf_inverse_transform;
+    if (sdf_has_queries(d->sdf_function)) {
+        /* ProceduralMean::evaluateJet (ProceduralMeanSdf.cpp:32-38); the 3x3 product in the
+         * order Eigen evaluates it for doubles with SSE2 (rows 0-1 left to right, row 2 as
+         * x + (y + z)); the gradient's rows left to right with the scale folded in first. */
+        double w[9], l[3], q[3], value, s = (double)d->sdf_scale;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) w[3 * i + j] = (double)t[4 * i + j];
+        l[0] = ((w[0] * a[0] + w[1] * a[1]) + w[2] * a[2]) + (double)t[3];
+        l[1] = ((w[3] * a[0] + w[4] * a[1]) + w[5] * a[2]) + (double)t[7];
+        l[2] = (w[6] * a[0] + (w[7] * a[1] + w[8] * a[2])) + (double)t[11];
+        value = orc_sdf_jet(d->sdf_function, l, q) * s + (double)d->sdf_offset;
+        if (value < (double)d->sdf_min || d->sdf_scale == 0.0f) {
+            if (g) g[0] = g[1] = g[2] = 0.0;
+            return value < (double)d->sdf_min ? (double)d->sdf_min : value;
+        }
+        if (g)
+            for (int i = 0; i < 3; ++i) g[i] = ((s * w[i]) * q[0] + (s * w[3 + i]) * q[1]) + (s * w[6 + i]) * q[2];
+        return value;
+    }
+    {
+        /* dmean_da (:291-298): forward differences with eps = double(0.001f) */
+        const double eps = (double)0.001f;
+        double v0 = sdf_mean_plain(d, a);
+        if (g)
+            for (int i = 0; i < 3; ++i) {
+                double b[3] = {a[0] + 0.0, a[1] + 0.0, a[2] + 0.0};
+                b[i] = a[i] + eps;
+                g[i] = (sdf_mean_plain(d, b) - v0) / eps;
+            }
+        return v0;
     }
 }
 

@@ -15,6 +15,8 @@
  *   src/core/math/GPFunctions.cpp:555-875          SE splat / covariance kernels
  *   src/core/math/GPFunctions.cpp:1225-1345,1600-1760  nonstationary covariance
  *   src/core/math/GPFunctions.hpp:952-996          spherical mean
+ *   src/core/math/GPFunctions.cpp:250-298, ProceduralMeanSdf.cpp:11-38,
+ *   src/core/math/SdfFunctionsQueries.cpp:14-466   procedural SDF means
  *   src/core/math/SparseConvolutionNoise.cpp       realization (all)
  *   src/core/media/SparseConvolutionNoiseMedium.cpp:199-347  prepare/finish
  *   src/core/media/GaussianProcessMedium.cpp:222-440         outcome rules
@@ -44,7 +46,10 @@ void orc_tangent_frame(const float n[3], float out9[9]); /* tangent, bitangent, 
 
 /* ---- medium description --------------------------------------------- */
 enum { ORC_COV_SE = 0, ORC_COV_PROC_NS = 1 };
-enum { ORC_MEAN_SPHERE = 0, ORC_MEAN_BOX = 1, ORC_MEAN_PLANE = 2, ORC_MEAN_GRID = 3 };
+enum { ORC_MEAN_SPHERE = 0, ORC_MEAN_BOX = 1, ORC_MEAN_PLANE = 2, ORC_MEAN_GRID = 3, ORC_MEAN_PROC = 4 };
+/* SdfFunctions::Function (SdfFunctions.hpp:13-22) */
+enum { ORC_SDF_KNOB = 0, ORC_SDF_KNOB_INNER = 1, ORC_SDF_KNOB_OUTER = 2, ORC_SDF_TWO_SPHERES = 3,
+       ORC_SDF_PLANE = 4, ORC_SDF_ROUNDED_U_GROOVE = 5, ORC_SDF_TWO_SLABS = 6, ORC_SDF_THREE_SLABS = 7 };
 enum { ORC_CTX_GLOBAL = 0, ORC_CTX_RENEWAL = 1, ORC_CTX_RENEWAL_PLUS = 2, ORC_CTX_NONE = 3 };
 enum { ORC_SCHEME_UNI = 0, ORC_SCHEME_NEE = 1, ORC_SCHEME_MIS = 2 };
 enum { ORC_LS_CONST = 0, ORC_LS_SINCOS = 1 };
@@ -89,6 +94,11 @@ typedef struct orc_medium_desc {
     /* "isotropic_3D_sampling": 3D noise in the isotropic ray space of each flight
      * (evaluateNoise3DIsotropicRayNormalized, SparseConvolutionNoise.cpp:345-377) */
     int32_t iso3d;
+    /* ProceduralMean over an SdfFunctions function (GPFunctions.cpp:250-298): "func",
+     * rows 0-2 of _invConfigTransform (row-major), "min", "scale", "offset" */
+    int32_t sdf_function;
+    float sdf_inverse_transform[12];
+    float sdf_min, sdf_scale, sdf_offset;
 } orc_medium_desc;
 
 /* Dense-grid density at a world point (trilinear, clamped), and TabulatedMean::mean
@@ -96,6 +106,16 @@ typedef struct orc_medium_desc {
 float orc_grid_density(const orc_medium_desc *d, const float p[3]);
 double orc_grid_mean(const orc_medium_desc *d, const double a[3]);
 void orc_grid_mean_grad(const orc_medium_desc *d, const double a[3], double g[3]);
+
+/* SdfFunctions::evaluateJet at a local point (SdfFunctionsQueries.cpp:444-450): value,
+ * gradient into g.  Only for the functions with queries (knob, two_spheres, two_slabs,
+ * three_slabs, rounded_u_groove). */
+double orc_sdf_jet(int fn, const double p[3], double g[3]);
+/* SdfFunctions::eval, the float API (SdfFunctions.hpp:64-75, SdfFunctionsQueries.cpp:374-394). */
+float orc_sdf_eval(int fn, const float p[3]);
+/* ProceduralMean::mean and dmean_da (GPFunctions.cpp:282-298, ProceduralMeanSdf.cpp:32-38)
+ * at a double point; g may be NULL. */
+double orc_sdf_mean(const orc_medium_desc *d, const double a[3], double g[3]);
 
 /* One flight request: the fields of GpRaySegmentRequest plus MediumState. */
 typedef struct orc_ray {

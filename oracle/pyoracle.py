@@ -16,7 +16,10 @@ LIB_PATH = os.path.join(HERE, "build", "libgpis_oracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libgpis_ref_leaf.so")
 
 COV_SE, COV_PROC_NS = 0, 1
-MEAN_SPHERE, MEAN_BOX, MEAN_PLANE, MEAN_GRID = 0, 1, 2, 3
+MEAN_SPHERE, MEAN_BOX, MEAN_PLANE, MEAN_GRID, MEAN_PROC = 0, 1, 2, 3, 4
+# SdfFunctions::Function (SdfFunctions.hpp:13-22)
+(SDF_KNOB, SDF_KNOB_INNER, SDF_KNOB_OUTER, SDF_TWO_SPHERES, SDF_PLANE, SDF_ROUNDED_U_GROOVE, SDF_TWO_SLABS,
+ SDF_THREE_SLABS) = range(8)
 CTX_GLOBAL, CTX_RENEWAL, CTX_RENEWAL_PLUS, CTX_NONE = 0, 1, 2, 3
 SCHEME_UNI, SCHEME_NEE, SCHEME_MIS = 0, 1, 2
 LS_CONST, LS_SINCOS = 0, 1
@@ -39,6 +42,8 @@ class MediumDesc(C.Structure):
         ("grid_w2i", C.c_float * 12), ("grid_clamp_min", C.c_float * 3), ("grid_clamp_max", C.c_float * 3),
         ("grid_offset", C.c_float), ("grid_scale", C.c_float), ("grid_volume", C.c_int32),
         ("iso3d", C.c_int32),
+        ("sdf_function", C.c_int32), ("sdf_inverse_transform", C.c_float * 12),
+        ("sdf_min", C.c_float), ("sdf_scale", C.c_float), ("sdf_offset", C.c_float),
     ]
 
 
@@ -218,6 +223,12 @@ def make_desc(**kw) -> MediumDesc:
     d.min_step = kw.get("min_step", 8)
     d.max_bounce = kw.get("max_bounce", 1 << 30)
     d.iso3d = int(kw.get("iso3d", 0))
+    d.sdf_function = int(kw.get("sdf_function", SDF_KNOB))
+    for i, a in enumerate(kw.get("sdf_inverse_transform", (1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0))):
+        d.sdf_inverse_transform[i] = a
+    d.sdf_min = kw.get("sdf_min", -3.4028234663852886e38)
+    d.sdf_scale = kw.get("sdf_scale", 1.0)
+    d.sdf_offset = kw.get("sdf_offset", 0.0)
     g = kw.get("grid")
     if g is not None:
         set_grid(d, g.voxels, g.origin, g.world_to_index, g.clamp_min, g.clamp_max,

@@ -11,7 +11,11 @@
 //   src/core/math/BitManip.hpp:47-50         normalizedUint
 //   src/core/math/TangentFrame.hpp:74-85     TangentFrame(n)
 //   src/core/sampling/Gaussian.cpp:22-34,89-104  rand_normal_2, sample_standard_normal
+//   src/core/math/SdfFunctionsQueries.cpp        SdfFunctions::evaluateJet, eval (float API)
+//   src/core/math/Mat4f.{hpp,cpp}                rotYXZ, invert, transformVector
+#include "math/Mat4f.hpp"
 #include "math/MathUtil.hpp"
+#include "math/SdfFunctions.hpp"
 #include "math/TangentFrame.hpp"
 #include "sampling/Gaussian.hpp"
 #include "sampling/UniformSampler.hpp"
@@ -78,6 +82,59 @@ void ref_pcg_words(uint64_t state, int n, uint32_t *out) {
     UniformSampler sampler;
     sampler.set_state(state);
     for (int q = 0; q < n; ++q) out[q] = sampler.nextI();
+}
+
+// SdfFunctions::evaluateJet at a local point: value, gradient into g.
+double ref_sdf_jet(int fn, const double p[3], double g[3]) {
+    const SdfJet q = SdfFunctions::evaluateJet(SdfFunctions::Function(fn), Eigen::Vector3d(p[0], p[1], p[2]));
+    g[0] = q.gradient.x(); g[1] = q.gradient.y(); g[2] = q.gradient.z();
+    return q.value;
+}
+
+// SdfFunctions::eval, the float API.
+float ref_sdf_eval(int fn, const float p[3]) {
+    int mat = 0;
+    return SdfFunctions::eval(SdfFunctions::Function(fn), Vec3f(p[0], p[1], p[2]), mat);
+}
+
+// The statements of ProceduralMean::prepareSdfQueries and evaluateJet
+// (ProceduralMeanSdf.cpp:11-16, 32-38) on the reference's Eigen, which fix the order of
+// the 3x3 double products; inv16 is _invConfigTransform, row-major.
+double ref_proc_mean_jet(int fn, const float inv16[16], float minv, float scale, float offset, const double a[3],
+                         double g[3]) {
+    Eigen::Matrix3d w2l;
+    Eigen::Vector3d tr;
+    for (int i = 0; i < 3; ++i) {
+        tr[i] = inv16[4 * i + 3];
+        for (int j = 0; j < 3; ++j) w2l(i, j) = inv16[4 * i + j];
+    }
+    const Eigen::Vector3d p(a[0], a[1], a[2]);
+    const SdfJet q = SdfFunctions::evaluateJet(SdfFunctions::Function(fn), w2l * p + tr);
+    const double value = q.value * double(scale) + double(offset);
+    Eigen::Vector3d grad = Eigen::Vector3d::Zero();
+    double out = value;
+    if (value < minv || scale == 0) out = std::max(value, double(minv));
+    else grad = double(scale) * w2l.transpose() * q.gradient;
+    g[0] = grad.x(); g[1] = grad.y(); g[2] = grad.z();
+    return out;
+}
+
+// Mat4f::rotYXZ(rot), Mat4f::invert and Mat4f * Vec3f, row-major 4x4.
+void ref_mat4f_rot_yxz(const float rot[3], float out16[16]) {
+    const Mat4f m = Mat4f::rotYXZ(Vec3f(rot[0], rot[1], rot[2]));
+    for (int i = 0; i < 16; ++i) out16[i] = m[i];
+}
+void ref_mat4f_invert(const float m16[16], float out16[16]) {
+    Mat4f m;
+    for (int i = 0; i < 16; ++i) m[i] = m16[i];
+    const Mat4f inv = m.invert();
+    for (int i = 0; i < 16; ++i) out16[i] = inv[i];
+}
+void ref_mat4f_transform_vector(const float m16[16], const float v[3], float out[3]) {
+    Mat4f m;
+    for (int i = 0; i < 16; ++i) m[i] = m16[i];
+    const Vec3f r = m * Vec3f(v[0], v[1], v[2]);
+    out[0] = r.x(); out[1] = r.y(); out[2] = r.z();
 }
 
 }  // extern "C"

@@ -18,7 +18,9 @@ def oracle_desc(med):
         ctxt=med.correlation_context, one_d=int(med.one_d), scheme=med.scheme,
         correlation_xy=int(med.correlation_xy), step_size=med.step_size, min_step=med.min_step,
         max_bounce=med.max_bounces, grid=med.mean_grid, grid_offset=med.mean_offset, grid_scale=med.mean_scale,
-        grid_volume=med.mean_volume, iso3d=int(med.isotropic_3D_sampling and not med.one_d))
+        grid_volume=med.mean_volume, iso3d=int(med.isotropic_3D_sampling and not med.one_d),
+        sdf_function=med.sdf_function, sdf_inverse_transform=med.sdf_inverse_transform, sdf_min=med.sdf_min,
+        sdf_scale=med.sdf_scale, sdf_offset=med.sdf_offset)
     return d
 
 
