@@ -268,7 +268,7 @@ __global__ void table_build_kernel(int lox, int loy, int loz, int dx, int dy, in
     }
 }
 
-template <int NI, bool ISO>
+template <int NI, bool ISO, int MEAN>
 __global__ __launch_bounds__(256) void eval_points_kernel(FieldParams P, const float *__restrict__ x,
                                                           const float *__restrict__ y, const float *__restrict__ z,
                                                           long long n, uint32_t seed, float *__restrict__ v,
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(256) void eval_points_kernel(FieldParams P, const f
     if (i >= n) return;
     RealState rs = {};
     rs.seed = seed;
-    Eval4 e = eval_field<NI, ISO>(P, rs, x[i], y[i], z[i], false);
+    Eval4 e = eval_field<NI, ISO, MEAN>(P, rs, x[i], y[i], z[i], false);
     v[i] = e.v;
     gx[i] = e.gx;
     gy[i] = e.gy;
@@ -1949,8 +1949,15 @@ gpis_status gpis_eval_points(gpis_ctx *c, const float *x, const float *y, const 
     int threads = 256;
     unsigned blocks = (unsigned)((n + threads - 1) / threads);
 #define GPIS_EVAL_LAUNCH(NI, ISO)                                                                              \
-    hipLaunchKernelGGL((eval_points_kernel<NI, ISO>), dim3(blocks), dim3(threads), 0, (hipStream_t)stream, c->P, x, \
-                       y, z, (long long)n, seed, v, gx, gy, gz)
+    do {                                                                                                           \
+        if (ext)                                                                                                   \
+            hipLaunchKernelGGL((eval_points_kernel<NI, ISO, MEAN_EXT>), dim3(blocks), dim3(threads), 0,            \
+                               (hipStream_t)stream, c->P, x, y, z, (long long)n, seed, v, gx, gy, gz);            \
+        else                                                                                                       \
+            hipLaunchKernelGGL((eval_points_kernel<NI, ISO, MEAN_ANALYTIC>), dim3(blocks), dim3(threads), 0,       \
+                               (hipStream_t)stream, c->P, x, y, z, (long long)n, seed, v, gx, gy, gz);            \
+    } while (0)
+    const bool ext = c->P.mean_kind == GPIS_MEAN_GRID || c->P.mean_kind == GPIS_MEAN_PROCEDURAL;
     if (c->P.N == 8 && c->P.iso) GPIS_EVAL_LAUNCH(8, true);
     else if (c->P.N == 8) GPIS_EVAL_LAUNCH(8, false);
     else if (c->P.iso) GPIS_EVAL_LAUNCH(0, true);
