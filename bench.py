@@ -412,16 +412,66 @@ def run_surrogate(args):
     return line
 
 
+def run_two_spheres(args):
+    """Primary flights through the sparse_conv_noise medium of the reference's two-spheres
+    scene (data/example-scenes/two-spheres-backlight/scene.json): procedural two_spheres
+    mean (transform scale 0.1, scale 0.1), SE sigma 0.08, l 0.2, 10 impulses per cell,
+    step 0.0025, a fresh realization per flight (no impulse table)."""
+    import torch
+    from gpis_extend_amd import GpisDevice, RayBatch, SparseConvolutionNoiseMedium, workloads as W
+
+    js = W.with_procedural_mean(W.config0_json(), "two_spheres", {"scale": 0.1}, scale=0.1)
+    js["gaussian_process"]["covariance"] = {"type": "squared_exponential", "sigma": 0.08, "lengthScale": 0.2}
+    js.update({"step_size": 0.0025, "min_step": 8, "seed": 17429, "impulse_density": 10,
+               "single_realization": False, "correlation_context": "global"})
+    med = SparseConvolutionNoiseMedium.from_json(js)
+    width = min(args.width, 1024)
+    rays = W.camera_rays(med, width=width, height=width, eye=(0.0, 0.0, 4.0), box=2.0)
+    n = rays["ox"].shape[0]
+    dev = GpisDevice(med)
+    batch = RayBatch.from_numpy(rays)
+    reply = dev.alloc_reply(n)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        dev.trace_flights(batch, reply)
+    torch.cuda.synchronize()
+    dev.reset_counters()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        dev.trace_flights(batch, reply)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    c = dev.counters()
+    dev.close()
+    flights = c["flights"] / args.steps
+    evals = c["evals"] / args.steps
+    ms = wall * 1e3 / args.steps
+    return {"metric": "GPIS flights/s on 1 GPU (two-spheres scene medium, procedural SDF mean)",
+            "value": flights / (ms / 1e3), "unit": "flights/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 (mean in f64)", "data": "synthetic",
+            "config": {"workload": f"{width}x{width} pinhole primary flights from (0, 0, 4) through the "
+                                   "two-spheres-backlight medium", "rays": n, "flights_per_step": flights,
+                       "evals_per_step": evals, "hits_per_step": c["hits"] / args.steps},
+            "kernel_ms": kernel_ms, "evals_per_s": evals / (kernel_ms / 1e3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--width", type=int, default=2048)
-    ap.add_argument("--workload", choices=("c1", "c3", "c3paths", "c4"), default="c1",
+    ap.add_argument("--workload", choices=("c1", "c3", "c3paths", "c4", "twospheres"), default="c1",
                     help="c1: camera-ray paths on the 3D field (default); c3: 1D Renewal+ flights with NEE; "
                          "c3paths: device-resident 1D mirror paths with light samples; "
-                         "c4: the learned surrogate against the plain method")
+                         "c4: the learned surrogate against the plain method; "
+                         "twospheres: primary flights through the two-spheres scene medium")
     ap.add_argument("--train-flights", type=int, default=1 << 24)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--no-extras", action="store_true",
@@ -448,6 +498,9 @@ def main():
         return
     if args.workload == "c4":
         print(json.dumps(run_surrogate(args)), flush=True)
+        return
+    if args.workload == "twospheres":
+        print(json.dumps(run_two_spheres(args)), flush=True)
         return
 
     import torch
