@@ -4,12 +4,42 @@
 // with the reference; the noise arithmetic is FP32.
 #pragma once
 
-#include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#ifdef GPIS_HOST_CHECK
+// Host build of the field code for the CPU checks under tests/ (one lane per wave): the
+// few device intrinsics the field evaluation uses, restated for one lane.
+#include <math.h>
+#define __host__
+#define __device__
+#define __global__
+#define __forceinline__ inline
+#define __noinline__
+#define __expf expf
+#define __logf logf
+#define __powf powf
+static inline float rsqrtf(float x) { return 1.0f / sqrtf(x); }
+static inline int min(int a, int b) { return a < b ? a : b; }
+static inline int max(int a, int b) { return a > b ? a : b; }
+#else
+#include <hip/hip_runtime.h>
+#endif
+
+#include "gpis_lists.hpp"
 #include "gpis_sdf.hpp"
 
 namespace gpis {
+
+// exp2 on the transcendental unit (v_exp_f32), and "some lane of the wave" of a predicate.
+#ifdef GPIS_HOST_CHECK
+static inline float hw_exp2(float x) { return exp2f(x); }
+#define GPIS_CHECK_INDEX(i, n) gpis_check_index((size_t)(i), (size_t)(n), __FILE__, __LINE__)
+static inline bool wave_any(bool p) { return p; }
+#else
+__device__ __forceinline__ float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+#define GPIS_CHECK_INDEX(i, n) ((void)0)
+#endif
 
 // ---------------------------------------------------------------- leaves --
 // xxhash32 on 1, 2 and 4 words: src/core/math/MathUtil.hpp:179-224.
@@ -98,6 +128,12 @@ struct FieldParams {
     // precomputed impulse table (single realization)
     // packed impulse table: per cell 3N floats (x with the weight in its sign bit, y, z)
     const float *table; int tlo[3]; int tdim[3]; uint32_t tseed; int has_table;
+    uint64_t table_n;  // floats in the table
+    // support lists of the table's interior cells (gpis_lists.hpp): fixed-point impulses
+    // (4 int32 each), list entries (byte offsets into lfix, groups of four), and per
+    // sub-cell (first group, number of groups); lsub sub-cells per cell edge
+    const int32_t *lfix; const uint32_t *lent; const uint32_t *lrange; int lsub; int has_lists;
+    uint32_t lfix_bytes, lent_n, lrange_n;  // their sizes (bytes, entries, sub-cells)
 };
 
 struct Eval4 { float v, gx, gy, gz; };

@@ -50,8 +50,8 @@ __device__ __forceinline__ void pair3(float lx, float ly, float lz, float4 im, f
     float tx = lx - im.x, ty = ly - im.y, tz = lz - im.z;
     float r2 = support_r2(tx, ty, tz);
     float e;
-    if (ISO) e = __builtin_amdgcn_exp2f(c0 * r2);
-    else e = __builtin_amdgcn_exp2f(c0 * (tx * tx) + c1 * (ty * ty) + c2 * (tz * tz));
+    if (ISO) e = hw_exp2(c0 * r2);
+    else e = hw_exp2(c0 * (tx * tx) + c1 * (ty * ty) + c2 * (tz * tz));
     float we = (r2 < 1.0f) ? im.w * e : 0.0f;
     v += we;
     ax += we * tx;
@@ -72,10 +72,10 @@ __device__ __forceinline__ void pair3p(float lx, float ly, float lz, float xs, f
     float tx = lx - fabsf(xs), ty = ly - iy, tz = lz - iz;
     float r2 = support_r2(tx, ty, tz);
     const bool in = r2 < 1.0f;
-    if (__builtin_amdgcn_ballot_w64(in)) {
+    if (wave_any(in)) {
         float e;
-        if (ISO) e = __builtin_amdgcn_exp2f(c0 * r2);
-        else e = __builtin_amdgcn_exp2f(c0 * (tx * tx) + c1 * (ty * ty) + c2 * (tz * tz));
+        if (ISO) e = hw_exp2(c0 * r2);
+        else e = hw_exp2(c0 * (tx * tx) + c1 * (ty * ty) + c2 * (tz * tz));
         float we = in ? copysignf(e, xs) : 0.0f;
         v += we;
         ax += we * tx;
@@ -114,6 +114,98 @@ __device__ __forceinline__ void cell_generated(int cx, int cy, int cz, int n, ui
     }
 }
 
+// One fixed-point impulse of the support lists: (X, Y, Z) in 2^-23 grid units and the
+// weight's float bits, read from `base` at byte offset `off`.
+struct Imp4 { int32_t x, y, z, w; };
+__device__ __forceinline__ Imp4 load_imp(const int32_t *base, uint32_t off) {
+#ifdef GPIS_HOST_CHECK
+    const int32_t *q = (const int32_t *)((const char *)base + off);
+    Imp4 r = {q[0], q[1], q[2], q[3]};
+#else
+    const int4 q = *reinterpret_cast<const int4 *>((const char *)base + off);
+    Imp4 r = {q.x, q.y, q.z, q.w};
+#endif
+    return r;
+}
+__device__ __forceinline__ void load_group(const uint32_t *ent, uint32_t e[4]) {
+#ifdef GPIS_HOST_CHECK
+    for (int i = 0; i < 4; ++i) e[i] = ent[i];
+#else
+    const uint4 q = *reinterpret_cast<const uint4 *>(ent);
+    e[0] = q.x; e[1] = q.y; e[2] = q.z; e[3] = q.w;
+#endif
+}
+__device__ __forceinline__ int32_t fix_coord(float g) {
+#ifdef GPIS_HOST_CHECK
+    return (int32_t)lrintf(g * lists::kFixScale);
+#else
+    return __float2int_rn(g * lists::kFixScale);
+#endif
+}
+
+// One query-impulse pair of a support list, in 2^-23 grid units: t = query - impulse is
+// the exact integer difference rounded once to float; |t|^2 < 2^46 is the support test.
+// `near` collects pairs whose |t|^2 lies within a relative kNearRel of the radius (their
+// decision is left to the exact path).  s0..s2 are the exponent factors scaled by 2^-46.
+template <bool ISO>
+__device__ __forceinline__ void pair_fix(int32_t Gx, int32_t Gy, int32_t Gz, const Imp4 &q, float s0, float s1,
+                                         float s2, float &v, float &ax, float &ay, float &az, bool &near) {
+    const float tx = (float)(int32_t)((uint32_t)Gx - (uint32_t)q.x);
+    const float ty = (float)(int32_t)((uint32_t)Gy - (uint32_t)q.y);
+    const float tz = (float)(int32_t)((uint32_t)Gz - (uint32_t)q.z);
+    const float r2 = tx * tx + ty * ty + tz * tz;
+    const bool in = r2 < 0x1p46f;
+    near = near || fabsf(r2 - 0x1p46f) < 0x1p46f * lists::kNearRel;
+    float e;
+    if (ISO) e = hw_exp2(s0 * r2);
+    else e = hw_exp2(s0 * (tx * tx) + s1 * (ty * ty) + s2 * (tz * tz));
+    union { int32_t i; float f; } w;
+    w.i = q.w;
+    const float we = in ? e * w.f : 0.0f;
+    v += we;
+    ax += we * tx;
+    ay += we * ty;
+    az += we * tz;
+}
+
+// The 27-cell sum over the support list of the query's sub-cell (the query's cell `lc` is
+// an interior cell of the table).  (gx, gy, gz): the query in grid units, (rx, ry, rz) its
+// in-cell coordinates.  Returns true when a pair fell near the support radius: the caller
+// then evaluates the query on the exact path.
+template <bool ISO>
+__device__ __forceinline__ bool list_sum(const FieldParams &P, unsigned lc, float gx, float gy, float gz, float rx,
+                                         float ry, float rz, float c0, float c1, float c2, float &v, float &ax,
+                                         float &ay, float &az) {
+    const unsigned S = (unsigned)P.lsub;
+    const unsigned sc = ((lc * S + (unsigned)lists::sub_index(rz, S)) * S + (unsigned)lists::sub_index(ry, S)) * S +
+                        (unsigned)lists::sub_index(rx, S);
+    GPIS_CHECK_INDEX(sc, P.lrange_n);
+    const unsigned first = P.lrange[2 * sc], ng = P.lrange[2 * sc + 1];
+    const int32_t Gx = fix_coord(gx), Gy = fix_coord(gy), Gz = fix_coord(gz);
+    const float s0 = c0 * 0x1p-46f, s1 = c1 * 0x1p-46f, s2 = c2 * 0x1p-46f;
+    bool near = false;
+    v = ax = ay = az = 0.0f;
+    const uint32_t *ent = P.lent + 4 * (size_t)first;
+#pragma unroll 1
+    for (unsigned j = 0; j < ng; ++j) {
+        GPIS_CHECK_INDEX(4 * ((size_t)first + j) + 3, (size_t)P.lent_n);
+        uint32_t e[4];
+        load_group(ent + 4 * j, e);
+        Imp4 q[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            GPIS_CHECK_INDEX((size_t)e[i] + 15, (size_t)P.lfix_bytes);
+            q[i] = load_imp(P.lfix, e[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) pair_fix<ISO>(Gx, Gy, Gz, q[i], s0, s1, s2, v, ax, ay, az, near);
+    }
+    ax *= 0x1p-23f;
+    ay *= 0x1p-23f;
+    az *= 0x1p-23f;
+    return near;
+}
+
 // Raw 27-cell sum at point p for one grid level: returns (sum w*h, sum w*h*to).
 // c0..c2 are the per-axis exponent factors already scaled by -log2(e).
 // When all 27 cells lie in the precomputed table the loads of cell j+1 are issued
@@ -130,8 +222,19 @@ __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, flo
     const int N = NI > 0 ? NI : P.N;
     bool tab = P.has_table && seed == P.tseed;
     unsigned bx = (unsigned)(ix - 1 - P.tlo[0]), by = (unsigned)(iy - 1 - P.tlo[1]), bz = (unsigned)(iz - 1 - P.tlo[2]);
-    if (NI > 0 && (NI * 3) % 4 == 0 && tab && bx + 2 < (unsigned)P.tdim[0] && by + 2 < (unsigned)P.tdim[1] &&
-        bz + 2 < (unsigned)P.tdim[2]) {
+    const bool inner = tab && bx + 2 < (unsigned)P.tdim[0] && by + 2 < (unsigned)P.tdim[1] && bz + 2 < (unsigned)P.tdim[2];
+    if (inner && P.has_lists) {
+        // the query's support list; a query with a pair at the support radius runs the
+        // exact path below instead
+        const unsigned lc = ((bz + 1) * (unsigned)P.tdim[1] + (by + 1)) * (unsigned)P.tdim[0] + (bx + 1);
+        const bool near = list_sum<ISO>(P, lc, gx, gy, gz, rx, ry, rz, c0, c1, c2, v, ax, ay, az);
+        if (!wave_any(near) || !near) {
+            Eval4 r = {v, ax, ay, az};
+            return r;
+        }
+        v = ax = ay = az = 0.0f;
+    }
+    if (NI > 0 && (NI * 3) % 4 == 0 && inner) {
         // fast path: the whole 3x3x3 block is in the packed table; the loads of cell
         // j+1 are in flight while cell j is summed
         constexpr int Q = NI > 0 ? NI * 3 / 4 : 1;  // float4 per cell
@@ -174,6 +277,7 @@ __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, flo
             for (int dzi = 0; dzi < 3; ++dzi) {
                 const float bzz = dzi == 0 ? bz0 : (dzi == 1 ? 0.0f : bz2);
                 if (bxy + bzz >= kCullR2) continue;
+                GPIS_CHECK_INDEX((size_t)(row - P.table) + (size_t)dzi * sz + cs - 1, (size_t)P.table_n);
                 const float4 *q = reinterpret_cast<const float4 *>(row + (size_t)dzi * sz);
                 float4 cur[Q];
 #pragma unroll
@@ -197,8 +301,9 @@ __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, flo
                 float lx = rx - (float)dx, ly = ry - (float)dy, lz = rz - (float)dz;
                 unsigned ux = (unsigned)(cx - P.tlo[0]), uy = (unsigned)(cy - P.tlo[1]), uz = (unsigned)(cz - P.tlo[2]);
                 if (tab && ux < (unsigned)P.tdim[0] && uy < (unsigned)P.tdim[1] && uz < (unsigned)P.tdim[2]) {
-                    const float *q =
-                        P.table + ((size_t)(uz * (unsigned)P.tdim[1] + uy) * (unsigned)P.tdim[0] + ux) * (3 * N);
+                    const size_t qi = ((size_t)(uz * (unsigned)P.tdim[1] + uy) * (unsigned)P.tdim[0] + ux) * (3 * N);
+                    GPIS_CHECK_INDEX(qi + 3 * N - 1, (size_t)P.table_n);
+                    const float *q = P.table + qi;
 #pragma unroll
                     for (int k = 0; k < N; ++k)
                         pair3p<ISO>(lx, ly, lz, q[3 * k], q[3 * k + 1], q[3 * k + 2], c0, c1, c2, v, ax, ay, az);

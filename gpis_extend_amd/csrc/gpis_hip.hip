@@ -29,6 +29,12 @@ struct gpis_ctx {
     FieldParams P;
     float *table = nullptr;  // packed impulse table (3N floats per cell)
     size_t table_cells = 0;
+    int table_lo[3] = {0, 0, 0}, table_dim[3] = {0, 0, 0};
+    // support lists of the table (gpis_lists.hpp): fixed-point impulses, entries, ranges
+    int32_t *lfix = nullptr;
+    uint32_t *lent = nullptr;
+    uint32_t *lrange = nullptr;
+    int lists_sub = 3;  // sub-cells per cell edge; 0: no lists
     unsigned long long *counters = nullptr;  // device: flights evals steps hits exits fails invalid launches
     unsigned int *work = nullptr;            // device work counter of the persistent kernels
     int device = 0;
@@ -43,6 +49,9 @@ struct gpis_ctx {
     sdf::MeanParams *sdf_dev = nullptr;   // ... and their device copy
     std::string err;
 };
+
+static void free_lists(gpis_ctx *c);
+static gpis_status build_lists(gpis_ctx *c);
 
 static gpis_status fail(gpis_ctx *c, gpis_status s, const char *msg) {
     if (c) c->err = msg;
@@ -266,6 +275,51 @@ __global__ void table_build_kernel(int lox, int loy, int loz, int dx, int dy, in
         q[3 * k + 1] = iy;
         q[3 * k + 2] = iz;
     }
+}
+
+// Fixed-point copy of the packed table (gpis_lists.hpp): impulse i at lfix[4i..4i+3], the
+// sentinel after the last impulse.  One thread per impulse.
+__global__ void fix_build_kernel(const float *__restrict__ table, int lox, int loy, int loz, int dx, int dy,
+                                 int n, long long total, int32_t *__restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > total) return;
+    if (i == total) {
+        lists::sentinel_impulse(out + 4 * i);
+        return;
+    }
+    const long long cell = i / n;
+    const int x = (int)(cell % dx), y = (int)((cell / dx) % dy), z = (int)(cell / ((long long)dx * dy));
+    lists::fix_impulse(lox + x, loy + y, loz + z, table[3 * i], table[3 * i + 1], table[3 * i + 2], out + 4 * i);
+}
+
+// Support lists, one thread per sub-cell of the table: first the number of groups of four
+// entries (0 for cells on the table's border), then, with the groups' positions, the entries.
+__global__ void list_count_kernel(const float *__restrict__ table, int lox, int loy, int loz, int dx, int dy, int dz,
+                                  int n, int S, long long nsub, uint32_t *__restrict__ groups) {
+    const long long sc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sc >= nsub) return;
+    const int lo[3] = {lox, loy, loz}, dim[3] = {dx, dy, dz};
+    const long long cell = sc / (S * S * S);
+    const int s = (int)(sc % (S * S * S));
+    const int x = lox + (int)(cell % dx), y = loy + (int)((cell / dx) % dy), z = loz + (int)(cell / ((long long)dx * dy));
+    uint32_t g = 0;
+    if (lists::interior(lo, dim, x, y, z))
+        g = (uint32_t)((lists::sub_cell_list(table, lo, dim, n, S, x, y, z, s % S, (s / S) % S, s / (S * S), nullptr,
+                                             0u) + 3) / 4);
+    groups[sc] = g;
+}
+__global__ void list_fill_kernel(const float *__restrict__ table, int lox, int loy, int loz, int dx, int dy, int dz,
+                                 int n, int S, long long nsub, const uint32_t *__restrict__ range,
+                                 uint32_t *__restrict__ ent, uint32_t sentinel_off) {
+    const long long sc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sc >= nsub) return;
+    if (range[2 * sc + 1] == 0) return;
+    const int lo[3] = {lox, loy, loz}, dim[3] = {dx, dy, dz};
+    const long long cell = sc / (S * S * S);
+    const int s = (int)(sc % (S * S * S));
+    const int x = lox + (int)(cell % dx), y = loy + (int)((cell / dx) % dy), z = loz + (int)(cell / ((long long)dx * dy));
+    lists::sub_cell_list(table, lo, dim, n, S, x, y, z, s % S, (s / S) % S, s / (S * S), ent + 4 * (size_t)range[2 * sc],
+                         sentinel_off);
 }
 
 template <int NI, bool ISO, int MEAN>
@@ -1838,6 +1892,7 @@ gpis_status gpis_create(const gpis_field_desc *desc, gpis_ctx **out) {
 gpis_status gpis_destroy(gpis_ctx *c) {
     if (!c) return GPIS_INVALID_ARGUMENT;
     (void)hipDeviceSynchronize();
+    free_lists(c);
     if (c->table) (void)hipFree(c->table);
     if (c->grid) (void)hipFree(c->grid);
     if (c->sdf_dev) (void)hipFree(c->sdf_dev);
@@ -1889,13 +1944,80 @@ static gpis_status mean_ready(gpis_ctx *c) {
     return GPIS_OK;
 }
 
+static void free_lists(gpis_ctx *c) {
+    if (c->lfix) (void)hipFree(c->lfix);
+    if (c->lent) (void)hipFree(c->lent);
+    if (c->lrange) (void)hipFree(c->lrange);
+    c->lfix = nullptr; c->lent = nullptr; c->lrange = nullptr;
+    FieldParams &P = c->P;
+    P.lfix = nullptr; P.lent = nullptr; P.lrange = nullptr;
+    P.has_lists = 0; P.lsub = 0; P.lfix_bytes = P.lent_n = P.lrange_n = 0;
+}
+
+// Support lists of the table's interior cells (gpis_lists.hpp), or none when lists are off,
+// there is no table, or the table is beyond the fixed-point range.  Synchronous.
+static gpis_status build_lists(gpis_ctx *c) {
+    HIP_TRY(c, hipDeviceSynchronize());
+    free_lists(c);
+    const int S = c->lists_sub;
+    if (S <= 0 || !c->table) return GPIS_OK;
+    const int *lo = c->table_lo, *dim = c->table_dim;
+    for (int i = 0; i < 3; ++i)
+        if (lo[i] < -lists::kMaxAbsCell || lo[i] + dim[i] - 1 > lists::kMaxAbsCell) return GPIS_OK;
+    const int N = c->P.N;
+    const long long cells = (long long)c->table_cells;
+    const long long imps = cells * N;
+    if ((imps + 1) * 16 >= (1LL << 32)) return GPIS_OK;
+    const long long nsub = cells * S * S * S;
+    HIP_TRY(c, hipMalloc((void **)&c->lfix, (size_t)(imps + 1) * 16));
+    HIP_TRY(c, hipMalloc((void **)&c->lrange, (size_t)nsub * 2 * sizeof(uint32_t)));
+    const int threads = 256;
+    hipLaunchKernelGGL(fix_build_kernel, dim3((unsigned)((imps + 1 + threads - 1) / threads)), dim3(threads), 0, 0,
+                       c->table, lo[0], lo[1], lo[2], dim[0], dim[1], N, imps, c->lfix);
+    HIP_TRY(c, hipGetLastError());
+    uint32_t *groups = nullptr;
+    HIP_TRY(c, hipMalloc((void **)&groups, (size_t)nsub * sizeof(uint32_t)));
+    hipLaunchKernelGGL(list_count_kernel, dim3((unsigned)((nsub + threads - 1) / threads)), dim3(threads), 0, 0,
+                       c->table, lo[0], lo[1], lo[2], dim[0], dim[1], dim[2], N, S, nsub, groups);
+    hipError_t e = hipGetLastError();
+    std::vector<uint32_t> g((size_t)nsub), range((size_t)nsub * 2);
+    if (e == hipSuccess) e = hipMemcpy(g.data(), groups, (size_t)nsub * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    (void)hipFree(groups);
+    if (e != hipSuccess) return hip_fail(c, e, "list_count_kernel");
+    unsigned long long total = 0;
+    for (long long i = 0; i < nsub; ++i) {
+        range[2 * i] = (uint32_t)total;
+        range[2 * i + 1] = g[i];
+        total += g[i];
+    }
+    if (total == 0 || total * 4 >= (1ULL << 32)) { free_lists(c); return GPIS_OK; }
+    HIP_TRY(c, hipMemcpy(c->lrange, range.data(), range.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMalloc((void **)&c->lent, (size_t)total * 4 * sizeof(uint32_t)));
+    hipLaunchKernelGGL(list_fill_kernel, dim3((unsigned)((nsub + threads - 1) / threads)), dim3(threads), 0, 0,
+                       c->table, lo[0], lo[1], lo[2], dim[0], dim[1], dim[2], N, S, nsub, c->lrange, c->lent,
+                       (uint32_t)(imps * 16));
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipDeviceSynchronize());
+    FieldParams &P = c->P;
+    P.lfix = c->lfix; P.lent = c->lent; P.lrange = c->lrange;
+    P.lsub = S;
+    P.lfix_bytes = (uint32_t)((imps + 1) * 16);
+    P.lent_n = (uint32_t)(total * 4);
+    P.lrange_n = (uint32_t)nsub;
+    P.has_lists = 1;
+    return GPIS_OK;
+}
+
 gpis_status gpis_set_domain(gpis_ctx *c, const float lo[3], const float hi[3], uint32_t seed) {
     if (!c || !lo || !hi) return fail(c, GPIS_INVALID_ARGUMENT, "null argument");
     if (c->P.multires) return fail(c, GPIS_UNSUPPORTED, "impulse tables cover the single-level grid");
     if (c->P.iso3d) return fail(c, GPIS_UNSUPPORTED, "the isotropic ray space has one lattice per flight direction");
+    HIP_TRY(c, hipDeviceSynchronize());
+    free_lists(c);
     if (c->table) { HIP_TRY(c, hipFree(c->table)); c->table = nullptr; }
     c->P.has_table = 0;
     c->P.table = nullptr;
+    c->table_cells = 0;
     float R = c->P.R;
     int clo[3], dims[3];
     size_t cells = 1;
@@ -1916,10 +2038,14 @@ gpis_status gpis_set_domain(gpis_ctx *c, const float lo[3], const float hi[3], u
     HIP_TRY(c, hipDeviceSynchronize());
     c->table_cells = cells;
     c->P.table = c->table;
-    for (int i = 0; i < 3; ++i) { c->P.tlo[i] = clo[i]; c->P.tdim[i] = dims[i]; }
+    for (int i = 0; i < 3; ++i) {
+        c->P.tlo[i] = clo[i]; c->P.tdim[i] = dims[i];
+        c->table_lo[i] = clo[i]; c->table_dim[i] = dims[i];
+    }
     c->P.tseed = seed;
+    c->P.table_n = (uint64_t)cells * c->P.N * 3;
     c->P.has_table = 1;
-    return GPIS_OK;
+    return build_lists(c);
 }
 
 gpis_status gpis_impulse_table(gpis_ctx *c, const int32_t lo[3], const int32_t dims[3], uint32_t seed, float *out,
@@ -2036,6 +2162,13 @@ gpis_status gpis_trace_paths(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, i
     launch_flights<true>(c, blocks, threads, st, make_in(rays), out, pout, (unsigned)n, max_bounces, bounce_seed, box);
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
+}
+
+gpis_status gpis_set_lists(gpis_ctx *c, int32_t sub) {
+    if (!c) return GPIS_INVALID_ARGUMENT;
+    if (sub < 0 || sub > lists::kMaxSub) return fail(c, GPIS_INVALID_ARGUMENT, "sub-cells per cell edge must be in 0..6");
+    c->lists_sub = sub;
+    return build_lists(c);
 }
 
 gpis_status gpis_set_refill(gpis_ctx *c, int32_t batch) {

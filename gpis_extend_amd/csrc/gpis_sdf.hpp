@@ -15,9 +15,34 @@
 // Arithmetic is double, unfused (#pragma clang fp contract(off)), in the reference's order.
 // The three-argument hypot is the C++17 std::hypot of libstdc++ 11 (scale by the largest
 // magnitude, then sqrt of the sum of the three squared ratios, left to right).
+// The SDF shapes below (the knob, two-spheres, slab and groove programs) follow the shape
+// formulas of the reference's src/core/math/SdfFunctions.cpp, which carries these notices:
+//
+//   Copyright 2020 Towaki Takikawa @yongyuanxi
+//   The MIT License
+//
+//   The MIT License (MIT)
+//   Copyright (c) 2021, NVIDIA CORPORATION.
+//   Permission is hereby granted, free of charge, to any person obtaining a copy of
+//   this software and associated documentation files (the "Software"), to deal in
+//   the Software without restriction, including without limitation the rights to
+//   use, copy, modify, merge, publish, distribute, sublicense, and/or sell copies of
+//   the Software, and to permit persons to whom the Software is furnished to do so,
+//   subject to the following conditions:
+//   The above copyright notice and this permission notice shall be included in all
+//   copies or substantial portions of the Software.
+//   THE SOFTWARE IS PROVIDED "AS IS", WITHOUT WARRANTY OF ANY KIND, EXPRESS OR
+//   IMPLIED, INCLUDING BUT NOT LIMITED TO THE WARRANTIES OF MERCHANTABILITY, FITNESS
+//   FOR A PARTICULAR PURPOSE AND NONINFRINGEMENT. IN NO EVENT SHALL THE AUTHORS OR
+//   COPYRIGHT HOLDERS BE LIABLE FOR ANY CLAIM, DAMAGES OR OTHER LIABILITY, WHETHER
+//   IN AN ACTION OF CONTRACT, TORT OR OTHERWISE, ARISING FROM, OUT OF OR IN
+//   CONNECTION WITH THE SOFTWARE OR THE USE OR OTHER DEALINGS IN THE SOFTWARE.
+//
 #pragma once
 
+#ifndef GPIS_HOST_CHECK
 #include <hip/hip_runtime.h>
+#endif
 
 #include <cmath>
 

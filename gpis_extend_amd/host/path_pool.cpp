@@ -247,6 +247,8 @@ bool PathPool::run(const std::vector<PathRequest> &paths, std::vector<PathResult
         return false;
     }
     if (!backend_.ok()) { error_ = "backend disabled: " + backend_.error(); return false; }
+    if (set_.batchSize == 0) { error_ = "batchSize must be positive"; return false; }
+    if (set_.poolSize == 0 && !paths.empty()) { error_ = "poolSize must be positive"; return false; }
     results.assign(paths.size(), PathResult());
     out_ = &results;
     std::vector<State> pool(std::min(set_.poolSize, paths.size()));
@@ -276,6 +278,14 @@ bool PathPool::run(const std::vector<PathRequest> &paths, std::vector<PathResult
         std::size_t k = 0;
         for (auto &f : futs) {
             const std::vector<RaySegmentReply> rep = f.get();
+            // replies come back in request order, one per request of the batch
+            const std::size_t expect = std::min(set_.batchSize, who.size() - k);
+            if (rep.size() != expect) {
+                error_ = "batch returned " + std::to_string(rep.size()) + " replies for " + std::to_string(expect) +
+                         " requests";
+                out_ = nullptr;
+                return false;
+            }
             for (const RaySegmentReply &r : rep) {
                 State &s = pool[who[k++]];
                 if (!r.valid) {

@@ -261,6 +261,16 @@ gpis_status gpis_set_launch(gpis_ctx *ctx, int32_t waves_per_cu, int32_t block_s
  * No reference counterpart (a scheduling knob; results do not depend on it). */
 gpis_status gpis_set_refill(gpis_ctx *ctx, int32_t batch);
 
+/* Support lists of the impulse table (the autotuner's knob "sub-cells per cell edge",
+ * 1..6, default 3; 0 = none).  For every cell of the table whose 27 neighbours are in the
+ * table, each of its sub^3 sub-cells lists the neighbour impulses within one grid unit
+ * of the sub-cell, in cell3D's visiting order (SparseConvolutionNoise.cpp:419-452); a
+ * lane sums only its query's list.  Results are the same as without lists (same support
+ * decisions, same order of the sums).  Rebuilds the lists of an existing table
+ * (synchronous); later gpis_set_domain calls build them too.  No reference counterpart
+ * (a data structure of the single-realization table). */
+gpis_status gpis_set_lists(gpis_ctx *ctx, int32_t sub);
+
 #ifdef __cplusplus
 }
 #endif

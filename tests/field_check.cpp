@@ -1,0 +1,299 @@
+// Host build of the device field code (gpis_field.hpp, GPIS_HOST_CHECK: one lane per wave)
+// with every table, list and fixed-point index checked against its array.  Built and run by
+// tests/test_field_check.py.
+//
+// Checks, on a stationary SE field with the C1 length scale (R = 3 l / sqrt 2, l = 0.05,
+// 8 impulses per cell, seed 1) over a table of the box [-1, 1]^3:
+//   1. the impulse field of a query is the same, bit for bit, whether its cells come from
+//      the generator (no table), a partial table (covering only x < 0) or the full table;
+//   2. with support lists (1..6 sub-cells per cell edge) the support decisions are the
+//      exact path's: with the splat set to 1 (exponent factor 0) the value is the integer
+//      sum of the in-support weights, which must be identical, and the gradient sums agree;
+//      with the SE exponent the value agrees to 2e-6 of the absolute sum;
+//   3. every list holds each impulse within reach of its sub-cell exactly once, in the
+//      reference's order, and nothing out of reach of the whole sub-cell.
+// Queries: uniform in the table's interior, near the origin planes (|p / R| < 1, where the
+// fixed-point query is rounded), on cell and sub-cell faces, and at distance 1 +- 2^-22
+// grid units from an impulse (the support radius).
+#include <cassert>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+struct float4 { float x, y, z, w; };
+struct int4 { int x, y, z, w; };
+struct uint4 { unsigned x, y, z, w; };
+
+static long long g_checks = 0;
+static void gpis_check_index(size_t i, size_t n, const char *file, int line) {
+    ++g_checks;
+    if (i >= n) {
+        fprintf(stderr, "index %zu out of range %zu at %s:%d\n", i, n, file, line);
+        abort();
+    }
+}
+
+#define GPIS_HOST_CHECK 1
+#define GPIS_LHD inline
+#include "../gpis_extend_amd/csrc/gpis_field.hpp"
+
+using namespace gpis;
+
+struct Table {
+    int lo[3], dim[3];
+    std::vector<float> packed;
+    std::vector<int32_t> fix;
+    std::vector<uint32_t> ent, range;
+    int S = 0;
+};
+
+// The packed table of table_build_kernel: per cell N x (x with the weight's sign, y, z).
+static void build_table(Table &T, const int lo[3], const int dim[3], uint32_t seed, int N) {
+    for (int i = 0; i < 3; ++i) { T.lo[i] = lo[i]; T.dim[i] = dim[i]; }
+    const long long cells = (long long)dim[0] * dim[1] * dim[2];
+    T.packed.assign(cells * 3 * N, 0.f);
+    for (long long c = 0; c < cells; ++c) {
+        const int x = lo[0] + (int)(c % dim[0]), y = lo[1] + (int)((c / dim[0]) % dim[1]),
+                  z = lo[2] + (int)(c / ((long long)dim[0] * dim[1]));
+        Pcg g;
+        g.set((uint64_t)(uint32_t)(xxh4((uint32_t)z, (uint32_t)y, (uint32_t)x, seed) + 1u));
+        for (int k = 0; k < N; ++k) {
+            const float iz = u01(g.next()), iy = u01(g.next()), ix = u01(g.next());
+            const float w = u01(g.next()) < 0.5f ? -1.0f : 1.0f;
+            T.packed[(c * N + k) * 3] = copysignf(ix, w);
+            T.packed[(c * N + k) * 3 + 1] = iy;
+            T.packed[(c * N + k) * 3 + 2] = iz;
+        }
+    }
+}
+
+// build_lists of gpis_hip.hip on the host: fixed-point table, counts, ranges, entries.
+static void build_lists(Table &T, int N, int S) {
+    T.S = S;
+    const long long cells = (long long)T.dim[0] * T.dim[1] * T.dim[2];
+    const long long imps = cells * N;
+    T.fix.assign((imps + 1) * 4, 0);
+    for (long long i = 0; i < imps; ++i) {
+        const long long c = i / N;
+        const int x = T.lo[0] + (int)(c % T.dim[0]), y = T.lo[1] + (int)((c / T.dim[0]) % T.dim[1]),
+                  z = T.lo[2] + (int)(c / ((long long)T.dim[0] * T.dim[1]));
+        lists::fix_impulse(x, y, z, T.packed[3 * i], T.packed[3 * i + 1], T.packed[3 * i + 2], &T.fix[4 * i]);
+    }
+    lists::sentinel_impulse(&T.fix[4 * imps]);
+    const long long nsub = cells * S * S * S;
+    T.range.assign(nsub * 2, 0);
+    unsigned long long total = 0;
+    for (long long sc = 0; sc < nsub; ++sc) {
+        const long long c = sc / (S * S * S);
+        const int s = (int)(sc % (S * S * S));
+        const int x = T.lo[0] + (int)(c % T.dim[0]), y = T.lo[1] + (int)((c / T.dim[0]) % T.dim[1]),
+                  z = T.lo[2] + (int)(c / ((long long)T.dim[0] * T.dim[1]));
+        uint32_t g = 0;
+        if (lists::interior(T.lo, T.dim, x, y, z))
+            g = (uint32_t)((lists::sub_cell_list(T.packed.data(), T.lo, T.dim, N, S, x, y, z, s % S, (s / S) % S,
+                                                 s / (S * S), nullptr, 0u) + 3) / 4);
+        T.range[2 * sc] = (uint32_t)total;
+        T.range[2 * sc + 1] = g;
+        total += g;
+    }
+    T.ent.assign(total * 4, 0xFFFFFFFFu);
+    for (long long sc = 0; sc < nsub; ++sc) {
+        if (!T.range[2 * sc + 1]) continue;
+        const long long c = sc / (S * S * S);
+        const int s = (int)(sc % (S * S * S));
+        const int x = T.lo[0] + (int)(c % T.dim[0]), y = T.lo[1] + (int)((c / T.dim[0]) % T.dim[1]),
+                  z = T.lo[2] + (int)(c / ((long long)T.dim[0] * T.dim[1]));
+        const int n = lists::sub_cell_list(T.packed.data(), T.lo, T.dim, N, S, x, y, z, s % S, (s / S) % S,
+                                           s / (S * S), &T.ent[4 * (size_t)T.range[2 * sc]], (uint32_t)(imps * 16));
+        assert((uint32_t)((n + 3) / 4) == T.range[2 * sc + 1]);
+    }
+    for (uint32_t e : T.ent) assert(e != 0xFFFFFFFFu && e % 16 == 0 && e <= imps * 16);
+}
+
+static FieldParams params(const Table *T, bool lists_on, float R, uint32_t seed, int N) {
+    FieldParams P;
+    memset(&P, 0, sizeof P);
+    P.N = N;
+    P.Nf = (float)N;
+    P.R = R;
+    if (T) {
+        P.has_table = 1;
+        P.table = T->packed.data();
+        P.table_n = T->packed.size();
+        P.tseed = seed;
+        for (int i = 0; i < 3; ++i) { P.tlo[i] = T->lo[i]; P.tdim[i] = T->dim[i]; }
+        if (lists_on && T->S) {
+            P.has_lists = 1;
+            P.lsub = T->S;
+            P.lfix = T->fix.data();
+            P.lent = T->ent.data();
+            P.lrange = T->range.data();
+            P.lfix_bytes = (uint32_t)(T->fix.size() * 4);
+            P.lent_n = (uint32_t)T->ent.size();
+            P.lrange_n = (uint32_t)(T->range.size() / 2);
+        }
+    }
+    return P;
+}
+
+static bool same_bits(const Eval4 &a, const Eval4 &b) {
+    return memcmp(&a, &b, sizeof a) == 0;
+}
+
+int main(int argc, char **argv) {
+    const int npts = argc > 1 ? atoi(argv[1]) : 20000;
+    const int N = 8;
+    const uint32_t seed = 1;
+    const float l_conv = 0.05f * sqrtf(2.f) / 2;
+    const float R = 3.0f * l_conv;
+    const float c_se = -1.4426950408889634f * (R * R * (1.0f / (l_conv * l_conv) * 0.5f));
+    int lo[3], dim[3];
+    for (int i = 0; i < 3; ++i) {
+        lo[i] = (int)floorf(-1.0f / R) - 1;
+        dim[i] = (int)floorf(1.0f / R) + 1 - lo[i] + 1;
+    }
+    Table full;
+    build_table(full, lo, dim, seed, N);
+    // the partial table: cells with x < 0 only
+    Table part;
+    {
+        int plo[3] = {lo[0], lo[1], lo[2]}, pdim[3] = {-lo[0], dim[1], dim[2]};
+        build_table(part, plo, pdim, seed, N);
+    }
+
+    std::mt19937 rng(2024);
+    std::uniform_real_distribution<float> U(-0.999f, 0.999f), Usmall(-0.99f * R, 0.99f * R), U01(0.f, 1.f);
+    std::vector<float> pts;
+    for (int i = 0; i < npts; ++i) {
+        float p[3];
+        const int kind = i % 5;
+        for (int a = 0; a < 3; ++a) p[a] = U(rng);
+        if (kind == 1) p[i % 3] = Usmall(rng);                       // near an origin plane
+        if (kind == 2) {                                             // on a cell or sub-cell face
+            const int a = i % 3;
+            const float g = floorf(p[a] / R) + (float)(rng() % 7) / 6.0f;
+            p[a] = g * R;
+        }
+        if (kind == 3) {                                             // at the support radius of an impulse
+            const long long c = rng() % (full.packed.size() / (3 * N));
+            const int k = rng() % N;
+            const int cx = lo[0] + (int)(c % dim[0]), cy = lo[1] + (int)((c / dim[0]) % dim[1]),
+                      cz = lo[2] + (int)(c / ((long long)dim[0] * dim[1]));
+            const double ix = cx + fabs(full.packed[(c * N + k) * 3]), iy = cy + full.packed[(c * N + k) * 3 + 1],
+                         iz = cz + full.packed[(c * N + k) * 3 + 2];
+            double ux = U01(rng) - 0.5, uy = U01(rng) - 0.5, uz = U01(rng) - 0.5;
+            const double ul = sqrt(ux * ux + uy * uy + uz * uz);
+            const double rad = 1.0 + ((int)(rng() % 5) - 2) * ldexp(1.0, -22);
+            p[0] = (float)((ix + rad * ux / ul) * R);
+            p[1] = (float)((iy + rad * uy / ul) * R);
+            p[2] = (float)((iz + rad * uz / ul) * R);
+            if (fabsf(p[0]) > 0.999f || fabsf(p[1]) > 0.999f || fabsf(p[2]) > 0.999f) { --i; continue; }
+        }
+        pts.insert(pts.end(), p, p + 3);
+    }
+
+    // 1. generator, partial table and full table agree bit for bit
+    const FieldParams Pn = params(nullptr, false, R, seed, N), Pp = params(&part, false, R, seed, N),
+                      Pf = params(&full, false, R, seed, N);
+    long long table_mismatch = 0;
+    for (int i = 0; i < npts; ++i) {
+        const float *p = &pts[3 * i];
+        const Eval4 a = noise3d_raw<8, true>(Pn, p[0], p[1], p[2], seed, R, c_se, c_se, c_se);
+        const Eval4 b = noise3d_raw<8, true>(Pp, p[0], p[1], p[2], seed, R, c_se, c_se, c_se);
+        const Eval4 c = noise3d_raw<8, true>(Pf, p[0], p[1], p[2], seed, R, c_se, c_se, c_se);
+        const Eval4 d = noise3d_raw<0, true>(Pf, p[0], p[1], p[2], seed, R, c_se, c_se, c_se);
+        if (!same_bits(a, b) || !same_bits(a, c) || !same_bits(a, d)) ++table_mismatch;
+    }
+    printf("table_mismatch %lld\n", table_mismatch);
+
+    // 2./3. lists
+    long long support_mismatch = 0, value_bad = 0, near = 0, evals = 0, near_radius = 0, evals_radius = 0;
+    double max_rel = 0.0, mean_list = 0.0;
+    long long list_cnt = 0, list_missing = 0, list_extra = 0;
+    for (int S = 1; S <= lists::kMaxSub; ++S) {
+        build_lists(full, N, S);
+        const FieldParams Pl = params(&full, true, R, seed, N);
+        for (int i = 0; i < npts; ++i) {
+            const float *p = &pts[3 * i];
+            // indicator splats: v = sum of in-support weights (exact integers on both paths)
+            const Eval4 e0 = noise3d_raw<8, true>(Pf, p[0], p[1], p[2], seed, R, 0.f, 0.f, 0.f);
+            const Eval4 e1 = noise3d_raw<8, true>(Pl, p[0], p[1], p[2], seed, R, 0.f, 0.f, 0.f);
+            if (e0.v != e1.v || fabsf(e0.gx - e1.gx) > 1e-5f * (1.f + fabsf(e0.gx)) ||
+                fabsf(e0.gy - e1.gy) > 1e-5f * (1.f + fabsf(e0.gy)) || fabsf(e0.gz - e1.gz) > 1e-5f * (1.f + fabsf(e0.gz)))
+                ++support_mismatch;
+            const Eval4 s0 = noise3d_raw<8, true>(Pf, p[0], p[1], p[2], seed, R, c_se, c_se, c_se);
+            const Eval4 s1 = noise3d_raw<8, true>(Pl, p[0], p[1], p[2], seed, R, c_se, c_se, c_se);
+            const double rel = fabs((double)s0.v - s1.v) + fabs((double)s0.gx - s1.gx) +
+                               fabs((double)s0.gy - s1.gy) + fabs((double)s0.gz - s1.gz);
+            if (rel > 2e-5) ++value_bad;
+            if (rel > max_rel) max_rel = rel;
+            // near flags of the list path (the lane would run the exact path)
+            float gx = p[0] / R, gy = p[1] / R, gz = p[2] / R;
+            float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+            int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+            if (lists::interior(full.lo, full.dim, ix, iy, iz)) {
+                const unsigned lc = (unsigned)lists::cell_linear(full.lo, full.dim, ix, iy, iz);
+                float v, ax, ay, az;
+                const bool nr =
+                    list_sum<true>(Pl, lc, gx, gy, gz, gx - fx, gy - fy, gz - fz, c_se, c_se, c_se, v, ax, ay, az);
+                if (i % 5 == 3) {
+                    near_radius += nr;  // queries placed at the support radius
+                    ++evals_radius;
+                } else {
+                    near += nr;
+                    ++evals;
+                }
+            }
+        }
+        // list contents against a brute-force scan of every interior cell's sub-cells
+        const int step = S >= 4 ? 7 : 1;
+        for (long long c = 0; c < (long long)dim[0] * dim[1] * dim[2]; c += step) {
+            const int x = lo[0] + (int)(c % dim[0]), y = lo[1] + (int)((c / dim[0]) % dim[1]),
+                      z = lo[2] + (int)(c / ((long long)dim[0] * dim[1]));
+            if (!lists::interior(lo, dim, x, y, z)) continue;
+            for (int s = 0; s < S * S * S; ++s) {
+                const long long sc = c * S * S * S + s;
+                const uint32_t *e = &full.ent[4 * (size_t)full.range[2 * sc]];
+                const int ng = (int)full.range[2 * sc + 1];
+                const int sx = s % S, sy = (s / S) % S, sz = s / (S * S);
+                int at = 0;
+                for (int dx = -1; dx <= 1; ++dx)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dz = -1; dz <= 1; ++dz)
+                            for (int k = 0; k < N; ++k) {
+                                const long long cn = lists::cell_linear(lo, dim, x + dx, y + dy, z + dz);
+                                const float *q = &full.packed[(cn * N + k) * 3];
+                                // exact distance from the impulse to the unpadded sub-cell box
+                                double d2 = 0;
+                                const double e3[3] = {dx + (double)fabsf(q[0]), dy + (double)q[1], dz + (double)q[2]};
+                                const int s3[3] = {sx, sy, sz};
+                                for (int a = 0; a < 3; ++a) {
+                                    const double b0 = (double)s3[a] / S, b1 = (double)(s3[a] + 1) / S;
+                                    const double g = e3[a] < b0 ? b0 - e3[a] : (e3[a] > b1 ? e3[a] - b1 : 0.0);
+                                    d2 += g * g;
+                                }
+                                const uint32_t off = (uint32_t)((cn * N + k) * 16);
+                                const bool listed = at < 4 * ng && e[at] == off;
+                                if (listed) ++at;
+                                if (d2 < 1.0 && !listed) ++list_missing;
+                                if (d2 > 1.001 && listed) ++list_extra;
+                            }
+                for (; at < 4 * ng; ++at)
+                    if (e[at] != (uint32_t)(full.packed.size() / 3 * 16)) ++list_extra;  // only sentinels remain
+                list_cnt += 1;
+                mean_list += 4.0 * ng;
+            }
+        }
+        printf("S %d support_mismatch %lld value_bad %lld near %lld of %lld (at the radius: %lld of %lld) "
+               "max_abs_diff %.3g mean_list %.1f\n", S, support_mismatch, value_bad, near, evals, near_radius,
+               evals_radius, max_rel, mean_list / (list_cnt ? list_cnt : 1));
+    }
+    printf("list_missing %lld list_extra %lld checks %lld\n", list_missing, list_extra, g_checks);
+    const bool ok = table_mismatch == 0 && support_mismatch == 0 && value_bad == 0 && list_missing == 0 &&
+                    list_extra == 0 && near * 100 < evals && near_radius * 10 > evals_radius * 9;
+    printf("%s\n", ok ? "OK" : "FAIL");
+    return ok ? 0 : 1;
+}

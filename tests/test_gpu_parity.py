@@ -177,6 +177,47 @@ def test_partial_impulse_table_equals_generator():
             assert np.array_equal(o[k].view(np.uint32), outs[0][k].view(np.uint32)), k
 
 
+@pytest.mark.parametrize("sub", [1, 3, 6])
+def test_support_lists_keep_fields_and_flights(sub):
+    """Support lists (sub-cells per cell edge 1, 3, 6) against the table's exact 27-cell
+    path: the same support decisions in the same order, so field values differ only in the
+    rounding of the splat arguments, and flights keep their outcomes and distances."""
+    from gpis_extend_amd import workloads as W, RayBatch
+    print(f"case: support lists sub={sub}, C1 eval_points 65,536 and 16,384 camera paths", flush=True)
+    med, dev = _dev(W.config1_json(), ((-1.0,) * 3, (1.0,) * 3))
+    x = torch.from_numpy(W.points_workload(65536, seed=W.SEED_POINTS)).cuda()
+    rays = W.camera_rays(med, width=128, height=128)
+    rb = RayBatch.from_numpy(rays)
+    outs = []
+    for s in (0, sub):
+        dev.set_lists(s)
+        v, gx, gy, gz = dev.eval_points(x[:, 0].contiguous(), x[:, 1].contiguous(), x[:, 2].contiguous(), med.seed)
+        pa = dev.trace_paths(rb, 2, 7, 1.0)
+        torch.cuda.synchronize()
+        outs.append((torch.stack([v, gx, gy, gz], 1).cpu().numpy(), {k: t.cpu().numpy() for k, t in pa.items()}))
+    dev.close()
+    (f0, p0), (f1, p1) = outs
+    scale = np.maximum(np.abs(f0).max(axis=1, keepdims=True), 1.0)
+    dmax = float(np.max(np.abs(f1 - f0) / scale))
+    agree = float(np.mean(p0["outcome0"] == p1["outcome0"]))
+    both = (p0["outcome0"] & 0xFF) == 1
+    dt = np.abs(p1["t0"][both] - p0["t0"][both]) / np.maximum(np.abs(p0["t0"][both]), 1e-6)
+    same_paths = float(np.mean((p0["flights"] == p1["flights"]) & (p0["hits"] == p1["hits"])))
+    far = float(np.mean(dt > 1e-5))
+    print(f"field max rel diff {dmax:.2e}; first-flight outcome agreement {agree:.5f}, hits with rel dt > 1e-5: "
+          f"{far:.5f} (median {np.median(dt):.1e}); paths with the same flights and hits {same_paths:.5f}")
+    # FP32 rounding of the splat arguments only (the support decisions are the same).  A
+    # flight whose secant shrink ends right at the 0.01 cutoff (t = 0, the ray origin) can
+    # switch sides on a last-bit difference: such flights stay below 5 in 10,000.
+    sel = dt > 1e-5
+    print("differing hits (t without lists, t with lists):", list(zip(p0["t0"][both][sel][:8].tolist(),
+                                                                      p1["t0"][both][sel][:8].tolist())))
+    assert dmax < 2e-5
+    assert agree >= 0.9999
+    assert far <= 5e-4
+    assert same_paths >= 0.999
+
+
 def test_paths_first_flight_equals_single_flights():
     from gpis_extend_amd import workloads as W, RayBatch
     med, dev = _dev(W.config1_json(), ((-1.0,) * 3, (1.0,) * 3))
