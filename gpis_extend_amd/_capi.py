@@ -101,7 +101,8 @@ class NetCalibration(C.Structure):
 
 class Counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
-        "flights", "evals", "steps", "hits", "exits", "fails", "invalid", "kernel_launches")]
+        "flights", "evals", "steps", "hits", "exits", "fails", "invalid", "kernel_launches", "spec_evals", "passes",
+        "paired")]
 
 
 _lib = None
@@ -153,7 +154,7 @@ def lib():
     L.gpis_set_refill.restype = st
     L.gpis_set_refill.argtypes = [_vp, C.c_int32]
     L.gpis_set_lists.restype = st
-    L.gpis_set_lists.argtypes = [_vp, C.c_int32]
+    L.gpis_set_lists.argtypes = [_vp, C.c_int32, C.c_int32]
     L.gpis_set_mean_grid.restype = st
     L.gpis_set_mean_grid.argtypes = [_vp, C.POINTER(MeanGrid)]
     L.gpis_net_create.restype = st

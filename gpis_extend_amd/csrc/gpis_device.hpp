@@ -134,6 +134,8 @@ struct FieldParams {
     // sub-cell (first group, number of groups); lsub sub-cells per cell edge
     const int32_t *lfix; const uint32_t *lent; const uint32_t *lrange; int lsub; int has_lists;
     uint32_t lfix_bytes, lent_n, lrange_n;  // their sizes (bytes, entries, sub-cells)
+    int lmargin;        // cells whose lists exist: those at least lmargin cells inside the table
+    float lpair2;       // lists serve a second query within sqrt(lpair2) grid units (0: one query)
 };
 
 struct Eval4 { float v, gx, gy, gz; };

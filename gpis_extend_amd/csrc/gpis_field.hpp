@@ -143,6 +143,25 @@ __device__ __forceinline__ int32_t fix_coord(float g) {
 #endif
 }
 
+// The splat of one pair given t = query - impulse in 2^-23 grid units and the weight's bits.
+template <bool ISO>
+__device__ __forceinline__ void pair_splat(float tx, float ty, float tz, int32_t wbits, float s0, float s1, float s2,
+                                           float &v, float &ax, float &ay, float &az, bool &near) {
+    const float r2 = tx * tx + ty * ty + tz * tz;
+    const bool in = r2 < 0x1p46f;
+    near = near || fabsf(r2 - 0x1p46f) < 0x1p46f * lists::kNearRel;
+    float e;
+    if (ISO) e = hw_exp2(s0 * r2);
+    else e = hw_exp2(s0 * (tx * tx) + s1 * (ty * ty) + s2 * (tz * tz));
+    union { int32_t i; float f; } w;
+    w.i = wbits;
+    const float we = in ? e * w.f : 0.0f;
+    v += we;
+    ax += we * tx;
+    ay += we * ty;
+    az += we * tz;
+}
+
 // One query-impulse pair of a support list, in 2^-23 grid units: t = query - impulse is
 // the exact integer difference rounded once to float; |t|^2 < 2^46 is the support test.
 // `near` collects pairs whose |t|^2 lies within a relative kNearRel of the radius (their
@@ -153,19 +172,7 @@ __device__ __forceinline__ void pair_fix(int32_t Gx, int32_t Gy, int32_t Gz, con
     const float tx = (float)(int32_t)((uint32_t)Gx - (uint32_t)q.x);
     const float ty = (float)(int32_t)((uint32_t)Gy - (uint32_t)q.y);
     const float tz = (float)(int32_t)((uint32_t)Gz - (uint32_t)q.z);
-    const float r2 = tx * tx + ty * ty + tz * tz;
-    const bool in = r2 < 0x1p46f;
-    near = near || fabsf(r2 - 0x1p46f) < 0x1p46f * lists::kNearRel;
-    float e;
-    if (ISO) e = hw_exp2(s0 * r2);
-    else e = hw_exp2(s0 * (tx * tx) + s1 * (ty * ty) + s2 * (tz * tz));
-    union { int32_t i; float f; } w;
-    w.i = q.w;
-    const float we = in ? e * w.f : 0.0f;
-    v += we;
-    ax += we * tx;
-    ay += we * ty;
-    az += we * tz;
+    pair_splat<ISO>(tx, ty, tz, q.w, s0, s1, s2, v, ax, ay, az, near);
 }
 
 // The 27-cell sum over the support list of the query's sub-cell (the query's cell `lc` is
@@ -206,6 +213,63 @@ __device__ __forceinline__ bool list_sum(const FieldParams &P, unsigned lc, floa
     return near;
 }
 
+// The table index of cell (ix, iy, iz) when it has support lists.
+__device__ __forceinline__ bool list_cell(const FieldParams &P, int ix, int iy, int iz, unsigned &lc) {
+    // signed offsets: a cell below the table's low corner must fail the test, not wrap
+    const int m = P.lmargin;
+    const int ux = ix - P.tlo[0], uy = iy - P.tlo[1], uz = iz - P.tlo[2];
+    if (!(ux >= m && uy >= m && uz >= m && ux + m < P.tdim[0] && uy + m < P.tdim[1] && uz + m < P.tdim[2]))
+        return false;
+    lc = ((unsigned)uz * (unsigned)P.tdim[1] + (unsigned)uy) * (unsigned)P.tdim[0] + (unsigned)ux;
+    return true;
+}
+
+// Two queries a and b summed over the support list of a's sub-cell (lists of reach 1 + e,
+// b within e of a): every impulse is read once for both.  Returns the two near flags.
+template <bool ISO>
+__device__ __forceinline__ void list_sum2(const FieldParams &P, unsigned lc, const float ga[3], const float ra[3],
+                                          const float gb[3], float c0, float c1, float c2, Eval4 &A, Eval4 &B,
+                                          bool &near_a, bool &near_b) {
+    const unsigned S = (unsigned)P.lsub;
+    const unsigned sc = ((lc * S + (unsigned)lists::sub_index(ra[2], S)) * S + (unsigned)lists::sub_index(ra[1], S)) *
+                            S + (unsigned)lists::sub_index(ra[0], S);
+    GPIS_CHECK_INDEX(sc, P.lrange_n);
+    const unsigned first = P.lrange[2 * sc], ng = P.lrange[2 * sc + 1];
+    const int32_t Ax = fix_coord(ga[0]), Ay = fix_coord(ga[1]), Az = fix_coord(ga[2]);
+    const int32_t Bx = fix_coord(gb[0]), By = fix_coord(gb[1]), Bz = fix_coord(gb[2]);
+    const float dbx = (float)(Bx - Ax), dby = (float)(By - Ay), dbz = (float)(Bz - Az);
+    const float s0 = c0 * 0x1p-46f, s1 = c1 * 0x1p-46f, s2 = c2 * 0x1p-46f;
+    near_a = near_b = false;
+    A.v = A.gx = A.gy = A.gz = 0.0f;
+    B.v = B.gx = B.gy = B.gz = 0.0f;
+    const uint32_t *ent = P.lent + 4 * (size_t)first;
+#pragma unroll 1
+    for (unsigned j = 0; j < ng; ++j) {
+        GPIS_CHECK_INDEX(4 * ((size_t)first + j) + 3, (size_t)P.lent_n);
+        uint32_t e[4];
+        load_group(ent + 4 * j, e);
+        Imp4 q[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            GPIS_CHECK_INDEX((size_t)e[i] + 15, (size_t)P.lfix_bytes);
+            q[i] = load_imp(P.lfix, e[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float tx = (float)(int32_t)((uint32_t)Ax - (uint32_t)q[i].x);
+            const float ty = (float)(int32_t)((uint32_t)Ay - (uint32_t)q[i].y);
+            const float tz = (float)(int32_t)((uint32_t)Az - (uint32_t)q[i].z);
+            pair_splat<ISO>(tx, ty, tz, q[i].w, s0, s1, s2, A.v, A.gx, A.gy, A.gz, near_a);
+            // b - impulse = (a - impulse) + (b - a): integers below 2^24 in magnitude for
+            // every impulse within reach, so the float sum is exact and equals the rounded
+            // integer difference (far sentinel impulses round, and stay out of support)
+            pair_splat<ISO>(tx + dbx, ty + dby, tz + dbz, q[i].w, s0, s1, s2, B.v, B.gx, B.gy, B.gz, near_b);
+        }
+    }
+    A.gx *= 0x1p-23f; A.gy *= 0x1p-23f; A.gz *= 0x1p-23f;
+    B.gx *= 0x1p-23f; B.gy *= 0x1p-23f; B.gz *= 0x1p-23f;
+}
+
 // Raw 27-cell sum at point p for one grid level: returns (sum w*h, sum w*h*to).
 // c0..c2 are the per-axis exponent factors already scaled by -log2(e).
 // When all 27 cells lie in the precomputed table the loads of cell j+1 are issued
@@ -213,7 +277,7 @@ __device__ __forceinline__ bool list_sum(const FieldParams &P, unsigned lc, floa
 // than one per cell; otherwise each cell comes from the table or the generator.
 template <int NI, bool ISO>
 __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, float py, float pz, uint32_t seed, float R,
-                                             float c0, float c1, float c2) {
+                                             float c0, float c1, float c2, bool use_lists = true) {
     float gx = px / R, gy = py / R, gz = pz / R;
     float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
     float rx = gx - fx, ry = gy - fy, rz = gz - fz;
@@ -221,12 +285,16 @@ __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, flo
     float v = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
     const int N = NI > 0 ? NI : P.N;
     bool tab = P.has_table && seed == P.tseed;
-    unsigned bx = (unsigned)(ix - 1 - P.tlo[0]), by = (unsigned)(iy - 1 - P.tlo[1]), bz = (unsigned)(iz - 1 - P.tlo[2]);
-    const bool inner = tab && bx + 2 < (unsigned)P.tdim[0] && by + 2 < (unsigned)P.tdim[1] && bz + 2 < (unsigned)P.tdim[2];
-    if (inner && P.has_lists) {
+    // the 3x3x3 block's low cell relative to the table; signed, so that a query below the
+    // table's low corner fails the test instead of wrapping around to a huge offset
+    const int sbx = ix - 1 - P.tlo[0], sby = iy - 1 - P.tlo[1], sbz = iz - 1 - P.tlo[2];
+    const bool inner = tab && sbx >= 0 && sby >= 0 && sbz >= 0 && sbx + 2 < P.tdim[0] && sby + 2 < P.tdim[1] &&
+                       sbz + 2 < P.tdim[2];
+    const unsigned bx = (unsigned)sbx, by = (unsigned)sby, bz = (unsigned)sbz;
+    unsigned lc = 0;
+    if (use_lists && tab && P.has_lists && list_cell(P, ix, iy, iz, lc)) {
         // the query's support list; a query with a pair at the support radius runs the
         // exact path below instead
-        const unsigned lc = ((bz + 1) * (unsigned)P.tdim[1] + (by + 1)) * (unsigned)P.tdim[0] + (bx + 1);
         const bool near = list_sum<ISO>(P, lc, gx, gy, gz, rx, ry, rz, c0, c1, c2, v, ax, ay, az);
         if (!wave_any(near) || !near) {
             Eval4 r = {v, ax, ay, az};
@@ -315,6 +383,42 @@ __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, flo
     }
     Eval4 r = {v, ax, ay, az};
     return r;
+}
+
+// Two queries at once (a and, when `two`, b): when lists serve two queries and b lies within
+// their extra reach of a, one pass over a's list sums both; otherwise each is summed on its
+// own.  Queries with a pair at the support radius are summed again on the exact path.
+template <int NI, bool ISO>
+__device__ __forceinline__ void noise3d_raw2(const FieldParams &P, const float pa[3], const float pb[3], bool two,
+                                             uint32_t seed, float R, float c0, float c1, float c2, Eval4 &A,
+                                             Eval4 &B) {
+    // a query without a second point rides the same pass with b = a (its second sums are
+    // discarded), so that the lanes of a wave stay on one path
+    const float ga[3] = {pa[0] / R, pa[1] / R, pa[2] / R};
+    const float gb[3] = {two ? pb[0] / R : ga[0], two ? pb[1] / R : ga[1], two ? pb[2] / R : ga[2]};
+    const float fa[3] = {floorf(ga[0]), floorf(ga[1]), floorf(ga[2])};
+    const float ra[3] = {ga[0] - fa[0], ga[1] - fa[1], ga[2] - fa[2]};
+    const float dx = gb[0] - ga[0], dy = gb[1] - ga[1], dz = gb[2] - ga[2];
+    unsigned lc = 0;
+    const bool pair = P.lpair2 > 0.0f && P.has_table && seed == P.tseed && P.has_lists &&
+                      dx * dx + dy * dy + dz * dz <= P.lpair2 && list_cell(P, (int)fa[0], (int)fa[1], (int)fa[2], lc);
+    bool near_a = false, near_b = false;
+    if (wave_any(pair)) {
+        if (pair) list_sum2<ISO>(P, lc, ga, ra, gb, c0, c1, c2, A, B, near_a, near_b);
+    }
+    // the rest one query at a time (one inlined copy of the single-query sum): queries off the
+    // pair path on their own lists or the table, near-radius queries on the exact path
+#pragma unroll 1
+    for (int w = 0; w < 2; ++w) {
+        const bool need = w == 0 ? (!pair || near_a) : (two && (!pair || near_b));
+        if (!wave_any(need)) continue;
+        if (need) {
+            const Eval4 r = noise3d_raw<NI, ISO>(P, w ? pb[0] : pa[0], w ? pb[1] : pa[1], w ? pb[2] : pa[2], seed, R, c0,
+                                                 c1, c2, !pair);
+            if (w) B = r;
+            else A = r;
+        }
+    }
 }
 
 // Grid constants of one level with spatial scale gs (multi-resolution level
@@ -494,6 +598,50 @@ __device__ __forceinline__ Eval4 eval_field(const FieldParams &P, const RealStat
     float a = amplitude(P);
     Eval4 r = {a * n.v + m, a * n.gx + mgx, a * n.gy + mgy, a * n.gz + mgz};
     return r;
+}
+
+// f and its gradient at two points of one flight (a, and b when `two`): eval_field at each.
+// Stationary single-level media in world space sum both points' noise in one pass over the
+// support list of a (noise3d_raw2); the other media evaluate the points one after the other.
+template <int NI, bool ISO, int MEAN = MEAN_ANY, bool IRAY = false>
+__device__ __forceinline__ void eval_field2(const FieldParams &P, const RealState &rs, const float pa[3],
+                                            const float pb[3], bool two, bool cond, float dx, float dy, float dz,
+                                            Eval4 &A, Eval4 &B) {
+    if (!IRAY && !P.multires && P.kind == 0) {
+        Eval4 na, nb;
+        noise3d_raw2<NI, ISO>(P, pa, pb, two, rs.seed, P.R, P.c2[0], P.c2[1], P.c2[2], na, nb);
+        A.v = na.v * P.nf_inv;
+        A.gx = na.gx * P.gfac[0] * P.nf_inv;
+        A.gy = na.gy * P.gfac[1] * P.nf_inv;
+        A.gz = na.gz * P.gfac[2] * P.nf_inv;
+        B.v = nb.v * P.nf_inv;
+        B.gx = nb.gx * P.gfac[0] * P.nf_inv;
+        B.gy = nb.gy * P.gfac[1] * P.nf_inv;
+        B.gz = nb.gz * P.gfac[2] * P.nf_inv;
+#pragma unroll 1
+        for (int w = 0; w < 2; ++w) {
+            if (w == 1 && !two) break;
+            Eval4 &E = w ? B : A;
+            const float *p = w ? pb : pa;
+            if (cond && rs.cond) cond_splat(P, rs, p[0], p[1], p[2], 1.0f, 1.0f, E);
+            float mgx, mgy, mgz;
+            const float m = mean_value<MEAN>(P, p[0], p[1], p[2], mgx, mgy, mgz);
+            const float a = amplitude(P);
+            E.v = a * E.v + m;
+            E.gx = a * E.gx + mgx;
+            E.gy = a * E.gy + mgy;
+            E.gz = a * E.gz + mgz;
+        }
+        return;
+    }
+#pragma unroll 1
+    for (int w = 0; w < 2; ++w) {
+        if (w == 1 && !two) break;
+        const float *p = w ? pb : pa;
+        const Eval4 e = eval_field<NI, ISO, MEAN, IRAY>(P, rs, p[0], p[1], p[2], cond, dx, dy, dz);
+        if (w) B = e;
+        else A = e;
+    }
 }
 
 }  // namespace gpis

@@ -34,7 +34,8 @@ struct gpis_ctx {
     int32_t *lfix = nullptr;
     uint32_t *lent = nullptr;
     uint32_t *lrange = nullptr;
-    int lists_sub = 3;  // sub-cells per cell edge; 0: no lists
+    int lists_sub = 3;     // sub-cells per cell edge; 0: no lists
+    int lists_points = 2;  // queries one list pass serves (1, or 2 with lists of reach kPairReach)
     unsigned long long *counters = nullptr;  // device: flights evals steps hits exits fails invalid launches
     unsigned int *work = nullptr;            // device work counter of the persistent kernels
     int device = 0;
@@ -295,7 +296,7 @@ __global__ void fix_build_kernel(const float *__restrict__ table, int lox, int l
 // Support lists, one thread per sub-cell of the table: first the number of groups of four
 // entries (0 for cells on the table's border), then, with the groups' positions, the entries.
 __global__ void list_count_kernel(const float *__restrict__ table, int lox, int loy, int loz, int dx, int dy, int dz,
-                                  int n, int S, long long nsub, uint32_t *__restrict__ groups) {
+                                  int n, int S, double reach, long long nsub, uint32_t *__restrict__ groups) {
     const long long sc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (sc >= nsub) return;
     const int lo[3] = {lox, loy, loz}, dim[3] = {dx, dy, dz};
@@ -303,13 +304,13 @@ __global__ void list_count_kernel(const float *__restrict__ table, int lox, int 
     const int s = (int)(sc % (S * S * S));
     const int x = lox + (int)(cell % dx), y = loy + (int)((cell / dx) % dy), z = loz + (int)(cell / ((long long)dx * dy));
     uint32_t g = 0;
-    if (lists::interior(lo, dim, x, y, z))
-        g = (uint32_t)((lists::sub_cell_list(table, lo, dim, n, S, x, y, z, s % S, (s / S) % S, s / (S * S), nullptr,
-                                             0u) + 3) / 4);
+    if (lists::interior(lo, dim, x, y, z, lists::list_margin(reach)))
+        g = (uint32_t)((lists::sub_cell_list(table, lo, dim, n, S, reach, x, y, z, s % S, (s / S) % S, s / (S * S),
+                                             nullptr, 0u) + 3) / 4);
     groups[sc] = g;
 }
 __global__ void list_fill_kernel(const float *__restrict__ table, int lox, int loy, int loz, int dx, int dy, int dz,
-                                 int n, int S, long long nsub, const uint32_t *__restrict__ range,
+                                 int n, int S, double reach, long long nsub, const uint32_t *__restrict__ range,
                                  uint32_t *__restrict__ ent, uint32_t sentinel_off) {
     const long long sc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (sc >= nsub) return;
@@ -318,8 +319,8 @@ __global__ void list_fill_kernel(const float *__restrict__ table, int lox, int l
     const long long cell = sc / (S * S * S);
     const int s = (int)(sc % (S * S * S));
     const int x = lox + (int)(cell % dx), y = loy + (int)((cell / dx) % dy), z = loz + (int)(cell / ((long long)dx * dy));
-    lists::sub_cell_list(table, lo, dim, n, S, x, y, z, s % S, (s / S) % S, s / (S * S), ent + 4 * (size_t)range[2 * sc],
-                         sentinel_off);
+    lists::sub_cell_list(table, lo, dim, n, S, reach, x, y, z, s % S, (s / S) % S, s / (S * S),
+                         ent + 4 * (size_t)range[2 * sc], sentinel_off);
 }
 
 template <int NI, bool ISO, int MEAN>
@@ -531,7 +532,10 @@ __device__ __noinline__ void diffuse_bounce(uint32_t ray, int bounce, uint32_t b
 }
 
 // Block statistics kept in LDS and flushed once per block.
-enum { ST_FLIGHTS = 0, ST_EVALS, ST_STEPS, ST_HITS, ST_EXITS, ST_FAILS, ST_INVALID, ST_N };
+// (counter 7 is the launch count; ST_SPEC counts evaluations made ahead for a point the
+// flight did not ask for)
+enum { ST_FLIGHTS = 0, ST_EVALS, ST_STEPS, ST_HITS, ST_EXITS, ST_FAILS, ST_INVALID, ST_SPEC = 8, ST_PASSES, ST_PAIRED,
+       ST_N };
 
 // The persistent flight kernel.  Lanes run independent per-flight state machines;
 // every iteration each live lane performs one field evaluation.  Lanes whose flight
@@ -555,7 +559,10 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
     int phase = PH_NEED;
     unsigned ray = 0;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, near_t = 0, far_t = 0, step = 0, t0 = 0;
-    float qx = 0, qy = 0, qz = 0;
+    double tq_p = 0.0;  // the lane's next evaluation point: t along the ray (unit_p: unit direction)
+    bool unit_p = false;
+    const bool pair_evals = P.lpair2 > 0.0f;
+    unsigned lane_passes = 0, lane_paired = 0;  // evaluation passes of this lane, passes with two used
     float pf = 0.f, t_end = 0.f, val_end = 0.f;
     int first = 1, sign0 = 1, k = 0, hit = 0;
     double tk = 0, fac = 0, tp = 0, tt = 0;
@@ -621,8 +628,9 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
                             step = (far_t - near_t) / P.min_step_f;
                             if (P.step_size < step) step = P.step_size;
                             t0 = mul_add_exact(step, jit, near_t);
-                            if (COND && P.activate) { phase = PH_COND; qx = ox; qy = oy; qz = oz; }
-                            else { phase = PH_F0; ray_point(ox, oy, oz, dx, dy, dz, 0.0, qx, qy, qz); }
+                            phase = COND && P.activate ? PH_COND : PH_F0;
+                            tq_p = 0.0;  // the origin, or f(nearT) at nearT = 0
+                            unit_p = false;
                         }
                     }
                     if (phase == PH_NEED) {
@@ -695,11 +703,12 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
                             t0 = mul_add_exact(step, jit, near_t);
                             if (COND && P.activate && !first) {
                                 phase = PH_COND;
-                                qx = ox; qy = oy; qz = oz;
+                                tq_p = 0.0;  // the origin
                             } else {
                                 phase = PH_F0;
-                                ray_point(ox, oy, oz, dx, dy, dz, (double)near_t, qx, qy, qz);
+                                tq_p = (double)near_t;
                             }
+                            unit_p = false;
                         }
                     }
                 }
@@ -708,120 +717,159 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(Fiel
         if (!__ballot(phase != PH_DEAD)) break;
         if (phase == PH_NEED || phase == PH_DEAD || phase == PH_DONE) continue;
 
-        // ---- one evaluation per live lane
-        Eval4 e = eval_field<NI, ISO, GRID ? MEAN_EXT : MEAN_ANALYTIC, IRAY>(P, rs, qx, qy, qz,
-                                                                               !COND || phase != PH_COND, dx, dy, dz);
-        ++evals_flight;
-        const float fv = e.v;
+        // ---- up to two field evaluations per live lane: the point its flight needs next
+        // (tq_p along the ray, along the normalised direction when unit_p; the origin in the
+        // conditioning phase) and, when the point after it is already determined by the
+        // march (the next lattice point, the next secant candidate, the gradient point at
+        // farT), that one too: the two share one pass over the impulses.  The second result
+        // is used only when the flight's state after the first asks for exactly that point.
+        float pa[3], pb[3] = {0.f, 0.f, 0.f};
+        point_at(ox, oy, oz, dx, dy, dz, tq_p, unit_p, pa[0], pa[1], pa[2]);
+        double spec_t = 0.0;
+        int spec_phase = PH_DEAD;
+        bool spec_unit = false;
+        if (pair_evals) {
+            if (phase == PH_F0 && t0 < far_t) {
+                spec_t = (double)t0; spec_phase = PH_MARCH;
+            } else if (phase == PH_MARCH) {
+                const double tn = lattice_t(t0, k + 1, step);
+                if (tn < (double)far_t) { spec_t = tn; spec_phase = PH_MARCH; }
+            } else if (phase == PH_SECANT) {
+                const double fb = fac * 0.9;
+                if (fb > 0.01) { spec_t = lerp_d(tk - (double)step, tk, fb); spec_phase = PH_SECANT; }
+            } else if (phase == PH_FAR) {
+                spec_t = (double)far_t; spec_phase = PH_GRAD; spec_unit = true;
+            }
+        }
+        const bool two = spec_phase != PH_DEAD;
+        if (two) point_at(ox, oy, oz, dx, dy, dz, spec_t, spec_unit, pb[0], pb[1], pb[2]);
+        Eval4 ea, eb;
+        eval_field2<NI, ISO, GRID ? MEAN_EXT : MEAN_ANALYTIC, IRAY>(P, rs, pa, pb, two, !COND || phase != PH_COND, dx,
+                                                                     dy, dz, ea, eb);
         // the next query point: t along the ray (unit: along the normalised direction)
         double tq = 0.0;
         bool unit = false;
-
-        if (COND && phase == PH_COND) {
-            condition_coefficients<GRID ? MEAN_EXT : MEAN_ANALYTIC, IRAY>(P, e, ox, oy, oz, prev_v, prev_gx, prev_gy,
-                                                                          prev_gz, dx, dy, dz, rs);
-            phase = PH_F0;
-            tq = (double)near_t;
-        } else if (phase == PH_F0) {
-            sign0 = fv < 0.f ? -1 : 1;
-            pf = fv;
-            k = 0;
-            if (t0 < far_t) {
-                phase = PH_MARCH;
-                tq = (double)t0;
-            } else {
-                phase = PH_FAR;
-                tq = (double)far_t;
-            }
-        } else if (phase == PH_MARCH) {
-            ++k;
-            int sc = fv < 0.f ? -1 : 1;
-            // lattice t_k = t0 + k*step, exact in double as the reference's `t += step_size`
-            tk = lattice_t(t0, k - 1, step);
-            if (!first && k == 1) {
-                sign0 = sc;
-            } else if (sc != sign0) {
-                // secant with 0.9 shrink (SparseConvolutionNoiseMedium.cpp:309-329), in FP64
-                fac = (double)pf / ((double)pf - (double)fv);
-                tp = lerp_d(tk - (double)step, tk, fac);
-                tt = tp;
-                phase = PH_SECANT;
-                tq = tt;
-            }
-            if (phase == PH_MARCH) {
+        auto consume = [&](const Eval4 &e) {
+            ++evals_flight;
+            const float fv = e.v;
+            if (COND && phase == PH_COND) {
+                condition_coefficients<GRID ? MEAN_EXT : MEAN_ANALYTIC, IRAY>(P, e, ox, oy, oz, prev_v, prev_gx, prev_gy,
+                                                                              prev_gz, dx, dy, dz, rs);
+                phase = PH_F0;
+                tq = (double)near_t;
+            } else if (phase == PH_F0) {
+                sign0 = fv < 0.f ? -1 : 1;
                 pf = fv;
-                double tn = lattice_t(t0, k, step);
-                if (k >= kMaxSteps) {
-                    // step limit (RaySegmentStatus::StepLimit, RaySegmentBatch.cu:281-284): reported invalid
-                    atomicAdd(&bstat[ST_INVALID], 1u);
-                    atomicAdd(&bstat[ST_STEPS], (unsigned)k);
-                    if (!PATHS) {
-                        out.t[ray] = (float)tn; out.value[ray] = fv;
-                        out.gx[ray] = out.gy[ray] = out.gz[ray] = 0.f;
-                        out.outcome[ray] = GPIS_INVALID;
-                        if (out.evals) out.evals[ray] = evals_flight;
-                    } else {
-                        if (bounce == 0) { pout.t0[ray] = (float)tn; pout.outcome0[ray] = GPIS_INVALID; }
-                        pout.flights[ray] = path_fh & 0xFFFFu; pout.hits[ray] = path_fh >> 16;
-                        if (pout.evals) pout.evals[ray] = path_evals + evals_flight;
-                    }
-                    evals_flight = 0;
-                    phase = PH_NEED;
-                } else if (tn < (double)far_t) {
-                    tq = tn;
+                k = 0;
+                if (t0 < far_t) {
+                    phase = PH_MARCH;
+                    tq = (double)t0;
                 } else {
                     phase = PH_FAR;
                     tq = (double)far_t;
                 }
-            }
-        } else if (phase == PH_SECANT) {
-            bool done = false;
-            if ((fv < 0.f ? -1 : 1) == sign0) {
-                done = true;
-            } else {
-                fac *= 0.9;
-                if (fac <= 0.01) {
-                    tp = 0.0;
-                    done = true;
-                } else {
-                    tp = tt;
-                    tt = lerp_d(tk - (double)step, tk, fac);
+            } else if (phase == PH_MARCH) {
+                ++k;
+                int sc = fv < 0.f ? -1 : 1;
+                // lattice t_k = t0 + k*step, exact in double as the reference's `t += step_size`
+                tk = lattice_t(t0, k - 1, step);
+                if (!first && k == 1) {
+                    sign0 = sc;
+                } else if (sc != sign0) {
+                    // secant with 0.9 shrink (SparseConvolutionNoiseMedium.cpp:309-329), in FP64
+                    fac = (double)pf / ((double)pf - (double)fv);
+                    tp = lerp_d(tk - (double)step, tk, fac);
+                    tt = tp;
+                    phase = PH_SECANT;
                     tq = tt;
                 }
-            }
-            if (done) {
-                hit = 1;
-                t_end = (float)tp;
-                val_end = 0.f;
+                if (phase == PH_MARCH) {
+                    pf = fv;
+                    double tn = lattice_t(t0, k, step);
+                    if (k >= kMaxSteps) {
+                        // step limit (RaySegmentStatus::StepLimit, RaySegmentBatch.cu:281-284): reported invalid
+                        atomicAdd(&bstat[ST_INVALID], 1u);
+                        atomicAdd(&bstat[ST_STEPS], (unsigned)k);
+                        if (!PATHS) {
+                            out.t[ray] = (float)tn; out.value[ray] = fv;
+                            out.gx[ray] = out.gy[ray] = out.gz[ray] = 0.f;
+                            out.outcome[ray] = GPIS_INVALID;
+                            if (out.evals) out.evals[ray] = evals_flight;
+                        } else {
+                            if (bounce == 0) { pout.t0[ray] = (float)tn; pout.outcome0[ray] = GPIS_INVALID; }
+                            pout.flights[ray] = path_fh & 0xFFFFu; pout.hits[ray] = path_fh >> 16;
+                            if (pout.evals) pout.evals[ray] = path_evals + evals_flight;
+                        }
+                        evals_flight = 0;
+                        phase = PH_NEED;
+                    } else if (tn < (double)far_t) {
+                        tq = tn;
+                    } else {
+                        phase = PH_FAR;
+                        tq = (double)far_t;
+                    }
+                }
+            } else if (phase == PH_SECANT) {
+                bool done = false;
+                if ((fv < 0.f ? -1 : 1) == sign0) {
+                    done = true;
+                } else {
+                    fac *= 0.9;
+                    if (fac <= 0.01) {
+                        tp = 0.0;
+                        done = true;
+                    } else {
+                        tp = tt;
+                        tt = lerp_d(tk - (double)step, tk, fac);
+                        tq = tt;
+                    }
+                }
+                if (done) {
+                    hit = 1;
+                    t_end = (float)tp;
+                    val_end = 0.f;
+                    phase = PH_GRAD;
+                    tq = tp;
+                    unit = true;
+                }
+            } else if (phase == PH_FAR) {
+                hit = 0;
+                t_end = far_t;
+                val_end = fv;
                 phase = PH_GRAD;
-                tq = tp;
+                tq = (double)far_t;
                 unit = true;
+            } else if (phase == PH_GRAD) {
+                // resumeDistance outcome rules (GaussianProcessMedium.cpp:305-348); the rest of
+                // the finish waits for the batched step at the top of the loop
+                gxv = e.gx; gyv = e.gy; gzv = e.gz;
+                if (hit) {
+                    if (!isfinite(gxv + gyv + gzv)) outcome = GPIS_FAIL | (GPIS_FAIL_NONFINITE << 8);
+                    else if (gxv * dx + gyv * dy + gzv * dz > 0.f) outcome = GPIS_FAIL | (GPIS_FAIL_OUTWARD << 8);
+                    else if (gxv * gxv + gyv * gyv + gzv * gzv < 1e-7f) outcome = GPIS_FAIL | (GPIS_FAIL_ZERO_GRAD << 8);
+                    else outcome = GPIS_HIT;
+                } else {
+                    outcome = GPIS_EXIT;
+                }
+                phase = PH_DONE;
             }
-        } else if (phase == PH_FAR) {
-            hit = 0;
-            t_end = far_t;
-            val_end = fv;
-            phase = PH_GRAD;
-            tq = (double)far_t;
-            unit = true;
-        } else if (phase == PH_GRAD) {
-            // resumeDistance outcome rules (GaussianProcessMedium.cpp:305-348); the rest of
-            // the finish waits for the batched step at the top of the loop
-            gxv = e.gx; gyv = e.gy; gzv = e.gz;
-            if (hit) {
-                if (!isfinite(gxv + gyv + gzv)) outcome = GPIS_FAIL | (GPIS_FAIL_NONFINITE << 8);
-                else if (gxv * dx + gyv * dy + gzv * dz > 0.f) outcome = GPIS_FAIL | (GPIS_FAIL_OUTWARD << 8);
-                else if (gxv * gxv + gyv * gyv + gzv * gzv < 1e-7f) outcome = GPIS_FAIL | (GPIS_FAIL_ZERO_GRAD << 8);
-                else outcome = GPIS_HIT;
+        };
+        consume(ea);
+        ++lane_passes;
+        if (two) {
+            if (phase == spec_phase && tq == spec_t && unit == spec_unit) {
+                consume(eb);
+                ++lane_paired;
             } else {
-                outcome = GPIS_EXIT;
+                atomicAdd(&bstat[ST_SPEC], 1u);
             }
-            phase = PH_DONE;
         }
-        if (phase == PH_F0 || phase == PH_MARCH || phase == PH_SECANT || phase == PH_FAR || phase == PH_GRAD)
-            point_at(ox, oy, oz, dx, dy, dz, tq, unit, qx, qy, qz);
+        tq_p = tq;
+        unit_p = unit;
     }
     // ---- statistics: LDS block totals, one global atomic per counter per block
+    atomicAdd(&bstat[ST_PASSES], lane_passes);
+    atomicAdd(&bstat[ST_PAIRED], lane_paired);
     __syncthreads();
     if (threadIdx.x < ST_N && bstat[threadIdx.x]) atomicAdd(&stats[threadIdx.x], (unsigned long long)bstat[threadIdx.x]);
 }
@@ -1952,6 +2000,7 @@ static void free_lists(gpis_ctx *c) {
     FieldParams &P = c->P;
     P.lfix = nullptr; P.lent = nullptr; P.lrange = nullptr;
     P.has_lists = 0; P.lsub = 0; P.lfix_bytes = P.lent_n = P.lrange_n = 0;
+    P.lmargin = 1; P.lpair2 = 0.0f;
 }
 
 // Support lists of the table's interior cells (gpis_lists.hpp), or none when lists are off,
@@ -1961,6 +2010,7 @@ static gpis_status build_lists(gpis_ctx *c) {
     free_lists(c);
     const int S = c->lists_sub;
     if (S <= 0 || !c->table) return GPIS_OK;
+    const double reach = c->lists_points == 2 ? (double)lists::kPairReach : 1.0;
     const int *lo = c->table_lo, *dim = c->table_dim;
     for (int i = 0; i < 3; ++i)
         if (lo[i] < -lists::kMaxAbsCell || lo[i] + dim[i] - 1 > lists::kMaxAbsCell) return GPIS_OK;
@@ -1978,7 +2028,7 @@ static gpis_status build_lists(gpis_ctx *c) {
     uint32_t *groups = nullptr;
     HIP_TRY(c, hipMalloc((void **)&groups, (size_t)nsub * sizeof(uint32_t)));
     hipLaunchKernelGGL(list_count_kernel, dim3((unsigned)((nsub + threads - 1) / threads)), dim3(threads), 0, 0,
-                       c->table, lo[0], lo[1], lo[2], dim[0], dim[1], dim[2], N, S, nsub, groups);
+                       c->table, lo[0], lo[1], lo[2], dim[0], dim[1], dim[2], N, S, reach, nsub, groups);
     hipError_t e = hipGetLastError();
     std::vector<uint32_t> g((size_t)nsub), range((size_t)nsub * 2);
     if (e == hipSuccess) e = hipMemcpy(g.data(), groups, (size_t)nsub * sizeof(uint32_t), hipMemcpyDeviceToHost);
@@ -1994,7 +2044,7 @@ static gpis_status build_lists(gpis_ctx *c) {
     HIP_TRY(c, hipMemcpy(c->lrange, range.data(), range.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMalloc((void **)&c->lent, (size_t)total * 4 * sizeof(uint32_t)));
     hipLaunchKernelGGL(list_fill_kernel, dim3((unsigned)((nsub + threads - 1) / threads)), dim3(threads), 0, 0,
-                       c->table, lo[0], lo[1], lo[2], dim[0], dim[1], dim[2], N, S, nsub, c->lrange, c->lent,
+                       c->table, lo[0], lo[1], lo[2], dim[0], dim[1], dim[2], N, S, reach, nsub, c->lrange, c->lent,
                        (uint32_t)(imps * 16));
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipDeviceSynchronize());
@@ -2004,6 +2054,10 @@ static gpis_status build_lists(gpis_ctx *c) {
     P.lfix_bytes = (uint32_t)((imps + 1) * 16);
     P.lent_n = (uint32_t)(total * 4);
     P.lrange_n = (uint32_t)nsub;
+    P.lmargin = lists::list_margin(reach);
+    // a second query within (reach - 1) of the first is served by the same pass (with a
+    // margin below the lists' own)
+    P.lpair2 = reach > 1.0 ? (float)((reach - 1.0) * (reach - 1.0) * 0.999) : 0.0f;
     P.has_lists = 1;
     return GPIS_OK;
 }
@@ -2023,8 +2077,10 @@ gpis_status gpis_set_domain(gpis_ctx *c, const float lo[3], const float hi[3], u
     size_t cells = 1;
     for (int i = 0; i < 3; ++i) {
         if (!(hi[i] > lo[i])) return fail(c, GPIS_INVALID_ARGUMENT, "empty domain");
-        clo[i] = (int)floorf(lo[i] / R) - 1;
-        int chi = (int)floorf(hi[i] / R) + 1;
+        // two cells of margin: the 3x3x3 block of every query in the box, and the 5x5x5
+        // neighbourhood of the support lists that serve two queries per pass
+        clo[i] = (int)floorf(lo[i] / R) - 2;
+        int chi = (int)floorf(hi[i] / R) + 2;
         dims[i] = chi - clo[i] + 1;
         cells *= (size_t)dims[i];
     }
@@ -2164,10 +2220,12 @@ gpis_status gpis_trace_paths(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, i
     return GPIS_OK;
 }
 
-gpis_status gpis_set_lists(gpis_ctx *c, int32_t sub) {
+gpis_status gpis_set_lists(gpis_ctx *c, int32_t sub, int32_t points) {
     if (!c) return GPIS_INVALID_ARGUMENT;
     if (sub < 0 || sub > lists::kMaxSub) return fail(c, GPIS_INVALID_ARGUMENT, "sub-cells per cell edge must be in 0..6");
+    if (points != 1 && points != 2) return fail(c, GPIS_INVALID_ARGUMENT, "points per list pass must be 1 or 2");
     c->lists_sub = sub;
+    c->lists_points = points;
     return build_lists(c);
 }
 
@@ -2305,6 +2363,7 @@ gpis_status gpis_get_counters(gpis_ctx *c, gpis_counters *o) {
     HIP_TRY(c, hipMemcpy(h, c->counters, sizeof h, hipMemcpyDeviceToHost));
     o->flights = h[0]; o->evals = h[1]; o->steps = h[2]; o->hits = h[3];
     o->exits = h[4]; o->fails = h[5]; o->invalid = h[6]; o->kernel_launches = h[7];
+    o->spec_evals = h[8]; o->passes = h[9]; o->paired = h[10];
     return GPIS_OK;
 }
 

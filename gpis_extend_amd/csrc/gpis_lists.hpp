@@ -43,7 +43,8 @@ namespace lists {
 constexpr float kFixScale = 8388608.0f;      // 2^23
 constexpr int kMaxAbsCell = 120;             // |cell index| bound: query - impulse and query - sentinel fit int32
 constexpr int32_t kSentinel = 1 << 30;       // sentinel impulse coordinate (2^7 cells away)
-constexpr double kReach2 = 1.0001;           // list membership: squared distance to the sub-cell box
+constexpr double kReach2 = 1.0001;           // list membership: squared distance margin (relative)
+constexpr float kPairReach = 1.125f;         // reach of lists that serve two queries per pass
 constexpr double kBoxPad = 1e-5;             // sub-cell box padding (rounding of frac * S)
 constexpr float kNearRel = 4e-6f;            // pairs with | |to|^2 - 1 | below this are decided again
 constexpr int kMaxSub = 6;                   // sub-cells per cell edge: 1..kMaxSub
@@ -53,11 +54,14 @@ GPIS_LHD long long cell_linear(const int lo[3], const int dim[3], int x, int y, 
     return ((long long)(z - lo[2]) * dim[1] + (y - lo[1])) * dim[0] + (x - lo[0]);
 }
 
-// Interior cells: all 27 neighbours lie in the table.
-GPIS_LHD bool interior(const int lo[3], const int dim[3], int x, int y, int z) {
-    return x - 1 >= lo[0] && x + 1 < lo[0] + dim[0] && y - 1 >= lo[1] && y + 1 < lo[1] + dim[1] &&
-           z - 1 >= lo[2] && z + 1 < lo[2] + dim[2];
+// Interior cells: every cell within m cells along each axis lies in the table.
+GPIS_LHD bool interior(const int lo[3], const int dim[3], int x, int y, int z, int m) {
+    return x - m >= lo[0] && x + m < lo[0] + dim[0] && y - m >= lo[1] && y + m < lo[1] + dim[1] &&
+           z - m >= lo[2] && z + m < lo[2] + dim[2];
 }
+// Neighbourhood of a list: one cell around the query's cell for a reach of one grid unit,
+// two when the list also serves a second query up to (reach - 1) away.
+GPIS_LHD int list_margin(double reach) { return reach > 1.0 ? 2 : 1; }
 
 // Fixed-point impulse k of cell (cx, cy, cz) from its packed form (x with the weight in its
 // sign bit, y, z): out = (X, Y, Z, bits of the weight).
@@ -85,16 +89,26 @@ GPIS_LHD double gap2(double e, double lo, double hi) {
 // The support list of sub-cell (sx, sy, sz) of interior cell (cx, cy, cz): the number of
 // entries (before padding), and when `out` is given the entries themselves (byte offsets of
 // fixed-point impulses; impulse i of the table is at 16 i) padded with `sentinel_off` to a
-// multiple of four.  `table` is the packed table (3N floats per cell).
-GPIS_LHD int sub_cell_list(const float *table, const int lo[3], const int dim[3], int N, int S, int cx, int cy,
-                           int cz, int sx, int sy, int sz, uint32_t *out, uint32_t sentinel_off) {
+// multiple of four.  `table` is the packed table (3N floats per cell).  An impulse is listed
+// when its distance to the sub-cell box is below `reach` grid units (1: the support of the
+// queries inside the sub-cell; 1 + e: also of every query within e of one inside it).  The
+// cells are visited in the order of noise3D's loops (x, then y, then z), which orders the
+// list the same way for every query it serves.
+GPIS_LHD int sub_cell_list(const float *table, const int lo[3], const int dim[3], int N, int S, double reach, int cx,
+                           int cy, int cz, int sx, int sy, int sz, uint32_t *out, uint32_t sentinel_off) {
     const double bx0 = (double)sx / S - kBoxPad, bx1 = (double)(sx + 1) / S + kBoxPad;
     const double by0 = (double)sy / S - kBoxPad, by1 = (double)(sy + 1) / S + kBoxPad;
     const double bz0 = (double)sz / S - kBoxPad, bz1 = (double)(sz + 1) / S + kBoxPad;
+    const double r2 = reach * reach * kReach2;
+    const int m = list_margin(reach);
     int n = 0;
-    for (int dx = -1; dx <= 1; ++dx)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dz = -1; dz <= 1; ++dz) {
+    for (int dx = -m; dx <= m; ++dx)
+        for (int dy = -m; dy <= m; ++dy)
+            for (int dz = -m; dz <= m; ++dz) {
+                // cells whose box is out of reach of the sub-cell box hold no listed impulse
+                const double cb = gap2(dx + 0.5, bx0 - 0.5, bx1 + 0.5) + gap2(dy + 0.5, by0 - 0.5, by1 + 0.5) +
+                                  gap2(dz + 0.5, bz0 - 0.5, bz1 + 0.5);
+                if (cb >= r2) continue;
                 const long long c = cell_linear(lo, dim, cx + dx, cy + dy, cz + dz);
                 const float *q = table + c * 3 * N;
                 for (int k = 0; k < N; ++k) {
@@ -102,7 +116,7 @@ GPIS_LHD int sub_cell_list(const float *table, const int lo[3], const int dim[3]
                     const double ey = dy + (double)q[3 * k + 1];
                     const double ez = dz + (double)q[3 * k + 2];
                     const double d2 = gap2(ex, bx0, bx1) + gap2(ey, by0, by1) + gap2(ez, bz0, bz1);
-                    if (d2 < kReach2) {
+                    if (d2 < r2) {
                         if (out) out[n] = (uint32_t)((c * N + k) * 16);
                         ++n;
                     }

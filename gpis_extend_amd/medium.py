@@ -411,9 +411,10 @@ class GpisDevice:
         """Lanes per wave served together by the finish/bounce/refill step (1..64)."""
         A.check(self.L.gpis_set_refill(self.h, batch), self.h, "gpis_set_refill")
 
-    def set_lists(self, sub: int):
-        """Support lists of the impulse table: sub-cells per cell edge (1..6), 0 for none."""
-        A.check(self.L.gpis_set_lists(self.h, sub), self.h, "gpis_set_lists")
+    def set_lists(self, sub: int, points: int = 2):
+        """Support lists of the impulse table: sub-cells per cell edge (1..6, 0 for none) and
+        evaluation points per list pass (1 or 2)."""
+        A.check(self.L.gpis_set_lists(self.h, sub, points), self.h, "gpis_set_lists")
 
     def impulse_table(self, lo, dims, seed):
         import torch

@@ -114,6 +114,10 @@ typedef struct gpis_reply_soa {
 typedef struct gpis_counters {
     uint64_t flights, evals, steps, hits, exits, fails, invalid;
     uint64_t kernel_launches;
+    uint64_t spec_evals;              /* evaluations made ahead (two points per list pass) that the
+                                         flight then did not ask for; not counted in evals */
+    uint64_t passes, paired;          /* 3D flight kernels: evaluation passes of the lanes, and the
+                                         passes whose second point the flight used */
 } gpis_counters;
 
 typedef struct gpis_ctx gpis_ctx;
@@ -261,15 +265,18 @@ gpis_status gpis_set_launch(gpis_ctx *ctx, int32_t waves_per_cu, int32_t block_s
  * No reference counterpart (a scheduling knob; results do not depend on it). */
 gpis_status gpis_set_refill(gpis_ctx *ctx, int32_t batch);
 
-/* Support lists of the impulse table (the autotuner's knob "sub-cells per cell edge",
- * 1..6, default 3; 0 = none).  For every cell of the table whose 27 neighbours are in the
- * table, each of its sub^3 sub-cells lists the neighbour impulses within one grid unit
- * of the sub-cell, in cell3D's visiting order (SparseConvolutionNoise.cpp:419-452); a
- * lane sums only its query's list.  Results are the same as without lists (same support
- * decisions, same order of the sums).  Rebuilds the lists of an existing table
- * (synchronous); later gpis_set_domain calls build them too.  No reference counterpart
- * (a data structure of the single-realization table). */
-gpis_status gpis_set_lists(gpis_ctx *ctx, int32_t sub);
+/* Support lists of the impulse table (the autotuner's knobs: "sub-cells per cell edge",
+ * 1..6, default 3, 0 = none; "points per list pass", 1 or 2, default 2).  For every cell of
+ * the table whose neighbourhood is in the table, each of its sub^3 sub-cells lists the
+ * impulses within reach of the sub-cell (one grid unit, or 1.125 when a pass serves two
+ * points), in cell3D's visiting order (SparseConvolutionNoise.cpp:419-452); a lane sums only
+ * its query's list, and with points = 2 the 3D flight kernels evaluate the next point the
+ * march already determines (next lattice point, next secant candidate, the gradient at
+ * farT) in the same pass.  Results are the same as without lists (same support decisions,
+ * same order of the sums; replies and evaluation counts unchanged).  Rebuilds the lists of
+ * an existing table (synchronous); later gpis_set_domain calls build them too.  No
+ * reference counterpart (a data structure of the single-realization table). */
+gpis_status gpis_set_lists(gpis_ctx *ctx, int32_t sub, int32_t points);
 
 #ifdef __cplusplus
 }

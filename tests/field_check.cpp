@@ -13,8 +13,9 @@
 //   3. every list holds each impulse within reach of its sub-cell exactly once, in the
 //      reference's order, and nothing out of reach of the whole sub-cell.
 // Queries: uniform in the table's interior, near the origin planes (|p / R| < 1, where the
-// fixed-point query is rounded), on cell and sub-cell faces, and at distance 1 +- 2^-22
-// grid units from an impulse (the support radius).
+// fixed-point query is rounded), on cell and sub-cell faces, at distance 1 +- 2^-22 grid
+// units from an impulse (the support radius), and up to four cells below the table's low
+// corner (cells the table does not hold; their neighbour blocks must not be read from it).
 #include <cassert>
 #include <cmath>
 #include <cstdio>
@@ -48,6 +49,7 @@ struct Table {
     std::vector<int32_t> fix;
     std::vector<uint32_t> ent, range;
     int S = 0;
+    double reach = 1.0;
 };
 
 // The packed table of table_build_kernel: per cell N x (x with the weight's sign, y, z).
@@ -71,8 +73,10 @@ static void build_table(Table &T, const int lo[3], const int dim[3], uint32_t se
 }
 
 // build_lists of gpis_hip.hip on the host: fixed-point table, counts, ranges, entries.
-static void build_lists(Table &T, int N, int S) {
+static void build_lists(Table &T, int N, int S, double reach) {
     T.S = S;
+    T.reach = reach;
+    const int m = lists::list_margin(reach);
     const long long cells = (long long)T.dim[0] * T.dim[1] * T.dim[2];
     const long long imps = cells * N;
     T.fix.assign((imps + 1) * 4, 0);
@@ -92,9 +96,9 @@ static void build_lists(Table &T, int N, int S) {
         const int x = T.lo[0] + (int)(c % T.dim[0]), y = T.lo[1] + (int)((c / T.dim[0]) % T.dim[1]),
                   z = T.lo[2] + (int)(c / ((long long)T.dim[0] * T.dim[1]));
         uint32_t g = 0;
-        if (lists::interior(T.lo, T.dim, x, y, z))
-            g = (uint32_t)((lists::sub_cell_list(T.packed.data(), T.lo, T.dim, N, S, x, y, z, s % S, (s / S) % S,
-                                                 s / (S * S), nullptr, 0u) + 3) / 4);
+        if (lists::interior(T.lo, T.dim, x, y, z, m))
+            g = (uint32_t)((lists::sub_cell_list(T.packed.data(), T.lo, T.dim, N, S, reach, x, y, z, s % S,
+                                                 (s / S) % S, s / (S * S), nullptr, 0u) + 3) / 4);
         T.range[2 * sc] = (uint32_t)total;
         T.range[2 * sc + 1] = g;
         total += g;
@@ -106,7 +110,7 @@ static void build_lists(Table &T, int N, int S) {
         const int s = (int)(sc % (S * S * S));
         const int x = T.lo[0] + (int)(c % T.dim[0]), y = T.lo[1] + (int)((c / T.dim[0]) % T.dim[1]),
                   z = T.lo[2] + (int)(c / ((long long)T.dim[0] * T.dim[1]));
-        const int n = lists::sub_cell_list(T.packed.data(), T.lo, T.dim, N, S, x, y, z, s % S, (s / S) % S,
+        const int n = lists::sub_cell_list(T.packed.data(), T.lo, T.dim, N, S, reach, x, y, z, s % S, (s / S) % S,
                                            s / (S * S), &T.ent[4 * (size_t)T.range[2 * sc]], (uint32_t)(imps * 16));
         assert((uint32_t)((n + 3) / 4) == T.range[2 * sc + 1]);
     }
@@ -134,6 +138,8 @@ static FieldParams params(const Table *T, bool lists_on, float R, uint32_t seed,
             P.lfix_bytes = (uint32_t)(T->fix.size() * 4);
             P.lent_n = (uint32_t)T->ent.size();
             P.lrange_n = (uint32_t)(T->range.size() / 2);
+            P.lmargin = lists::list_margin(T->reach);
+            P.lpair2 = T->reach > 1.0 ? (float)((T->reach - 1.0) * (T->reach - 1.0) * 0.999) : 0.0f;
         }
     }
     return P;
@@ -172,6 +178,7 @@ int main(int argc, char **argv) {
         const int kind = i % 5;
         for (int a = 0; a < 3; ++a) p[a] = U(rng);
         if (kind == 1) p[i % 3] = Usmall(rng);                       // near an origin plane
+        if (kind == 4 && (i / 5) % 4 == 0) p[i % 3] = -1.0f - 0.45f * U01(rng);  // below the table's low corner
         if (kind == 2) {                                             // on a cell or sub-cell face
             const int a = i % 3;
             const float g = floorf(p[a] / R) + (float)(rng() % 7) / 6.0f;
@@ -213,8 +220,12 @@ int main(int argc, char **argv) {
     long long support_mismatch = 0, value_bad = 0, near = 0, evals = 0, near_radius = 0, evals_radius = 0;
     double max_rel = 0.0, mean_list = 0.0;
     long long list_cnt = 0, list_missing = 0, list_extra = 0;
-    for (int S = 1; S <= lists::kMaxSub; ++S) {
-        build_lists(full, N, S);
+    long long pair_mismatch = 0, pair_evals = 0;
+    for (int variant = 0; variant < 2 * lists::kMaxSub; ++variant) {
+        const int S = variant % lists::kMaxSub + 1;
+        const double reach = variant < lists::kMaxSub ? 1.0 : (double)lists::kPairReach;
+        const int mg = lists::list_margin(reach);
+        build_lists(full, N, S, reach);
         const FieldParams Pl = params(&full, true, R, seed, N);
         for (int i = 0; i < npts; ++i) {
             const float *p = &pts[3 * i];
@@ -234,7 +245,7 @@ int main(int argc, char **argv) {
             float gx = p[0] / R, gy = p[1] / R, gz = p[2] / R;
             float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
             int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-            if (lists::interior(full.lo, full.dim, ix, iy, iz)) {
+            if (lists::interior(full.lo, full.dim, ix, iy, iz, mg)) {
                 const unsigned lc = (unsigned)lists::cell_linear(full.lo, full.dim, ix, iy, iz);
                 float v, ax, ay, az;
                 const bool nr =
@@ -248,21 +259,49 @@ int main(int argc, char **argv) {
                 }
             }
         }
+        // two queries per pass: b within the lists' extra reach of a, against each on its own
+        if (Pl.lpair2 > 0.0f) {
+            for (int i = 0; i < npts; ++i) {
+                const float *p = &pts[3 * i];
+                float u[3] = {U01(rng) - 0.5f, U01(rng) - 0.5f, U01(rng) - 0.5f};
+                const float ul = sqrtf(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+                const float len = sqrtf(Pl.lpair2) * U01(rng) * R;
+                const float pb[3] = {p[0] + len * u[0] / ul, p[1] + len * u[1] / ul, p[2] + len * u[2] / ul};
+                for (int cz = 0; cz < 2; ++cz) {
+                    const float cc = cz ? c_se : 0.f;
+                    Eval4 A, B;
+                    noise3d_raw2<8, true>(Pl, p, pb, true, seed, R, cc, cc, cc, A, B);
+                    const Eval4 a0 = noise3d_raw<8, true>(Pf, p[0], p[1], p[2], seed, R, cc, cc, cc);
+                    const Eval4 b0 = noise3d_raw<8, true>(Pf, pb[0], pb[1], pb[2], seed, R, cc, cc, cc);
+                    const double d = fabs((double)A.v - a0.v) + fabs((double)B.v - b0.v) + fabs((double)A.gx - a0.gx) +
+                                     fabs((double)B.gz - b0.gz);
+                    if ((cz == 0 && (A.v != a0.v || B.v != b0.v)) || d > 2e-5) ++pair_mismatch;
+                    ++pair_evals;
+                    // a query without a second point
+                    Eval4 A1, B1;
+                    noise3d_raw2<8, true>(Pl, p, pb, false, seed, R, cc, cc, cc, A1, B1);
+                    if ((cz == 0 && A1.v != a0.v) || fabs((double)A1.v - a0.v) + fabs((double)A1.gy - a0.gy) > 2e-5)
+                        ++pair_mismatch;
+                }
+            }
+        }
         // list contents against a brute-force scan of every interior cell's sub-cells
         const int step = S >= 4 ? 7 : 1;
+        mean_list = 0.0;
+        list_cnt = 0;
         for (long long c = 0; c < (long long)dim[0] * dim[1] * dim[2]; c += step) {
             const int x = lo[0] + (int)(c % dim[0]), y = lo[1] + (int)((c / dim[0]) % dim[1]),
                       z = lo[2] + (int)(c / ((long long)dim[0] * dim[1]));
-            if (!lists::interior(lo, dim, x, y, z)) continue;
+            if (!lists::interior(lo, dim, x, y, z, mg)) continue;
             for (int s = 0; s < S * S * S; ++s) {
                 const long long sc = c * S * S * S + s;
                 const uint32_t *e = &full.ent[4 * (size_t)full.range[2 * sc]];
                 const int ng = (int)full.range[2 * sc + 1];
                 const int sx = s % S, sy = (s / S) % S, sz = s / (S * S);
                 int at = 0;
-                for (int dx = -1; dx <= 1; ++dx)
-                    for (int dy = -1; dy <= 1; ++dy)
-                        for (int dz = -1; dz <= 1; ++dz)
+                for (int dx = -mg; dx <= mg; ++dx)
+                    for (int dy = -mg; dy <= mg; ++dy)
+                        for (int dz = -mg; dz <= mg; ++dz)
                             for (int k = 0; k < N; ++k) {
                                 const long long cn = lists::cell_linear(lo, dim, x + dx, y + dy, z + dz);
                                 const float *q = &full.packed[(cn * N + k) * 3];
@@ -278,8 +317,8 @@ int main(int argc, char **argv) {
                                 const uint32_t off = (uint32_t)((cn * N + k) * 16);
                                 const bool listed = at < 4 * ng && e[at] == off;
                                 if (listed) ++at;
-                                if (d2 < 1.0 && !listed) ++list_missing;
-                                if (d2 > 1.001 && listed) ++list_extra;
+                                if (d2 < reach * reach && !listed) ++list_missing;
+                                if (d2 > reach * reach * 1.001 && listed) ++list_extra;
                             }
                 for (; at < 4 * ng; ++at)
                     if (e[at] != (uint32_t)(full.packed.size() / 3 * 16)) ++list_extra;  // only sentinels remain
@@ -287,12 +326,13 @@ int main(int argc, char **argv) {
                 mean_list += 4.0 * ng;
             }
         }
-        printf("S %d support_mismatch %lld value_bad %lld near %lld of %lld (at the radius: %lld of %lld) "
-               "max_abs_diff %.3g mean_list %.1f\n", S, support_mismatch, value_bad, near, evals, near_radius,
+        printf("reach %.3f S %d support_mismatch %lld value_bad %lld near %lld of %lld (at the radius: %lld of %lld) "
+               "max_abs_diff %.3g mean_list %.1f\n", reach, S, support_mismatch, value_bad, near, evals, near_radius,
                evals_radius, max_rel, mean_list / (list_cnt ? list_cnt : 1));
     }
+    printf("pair_mismatch %lld of %lld\n", pair_mismatch, pair_evals);
     printf("list_missing %lld list_extra %lld checks %lld\n", list_missing, list_extra, g_checks);
-    const bool ok = table_mismatch == 0 && support_mismatch == 0 && value_bad == 0 && list_missing == 0 &&
+    const bool ok = table_mismatch == 0 && support_mismatch == 0 && pair_mismatch == 0 && pair_evals > 0 && value_bad == 0 && list_missing == 0 &&
                     list_extra == 0 && near * 100 < evals && near_radius * 10 > evals_radius * 9;
     printf("%s\n", ok ? "OK" : "FAIL");
     return ok ? 0 : 1;

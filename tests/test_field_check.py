@@ -16,7 +16,7 @@ def test_field_tables_and_support_lists_on_the_host(tmp_path):
     exe = tmp_path / "field_check"
     subprocess.run(["g++", "-O2", "-std=c++17", "-Wno-unknown-pragmas", "-ffp-contract=off",
                     os.path.join(HERE, "field_check.cpp"), "-o", str(exe)], check=True)
-    out = subprocess.run([str(exe), "4000"], capture_output=True, text=True)
+    out = subprocess.run([str(exe), "2000"], capture_output=True, text=True)
     print(out.stdout)
     assert out.returncode == 0, out.stdout + out.stderr
     lines = out.stdout.split("\n")

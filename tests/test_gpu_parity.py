@@ -155,18 +155,19 @@ def test_flights_config2_conditioned_brute_force_subset():
     parity_gate(got, ref, "C2 Renewal brute-force subset", min_agree=0.99, oracle=orc)
 
 
-@pytest.mark.skip(reason="faulted the GPU together with an LDS cell-block staging path for "
-                         "out-of-table evaluations, since removed; the cause was not found, so the "
-                         "case stays off the GPU")
 def test_partial_impulse_table_equals_generator():
     """Flights whose evaluations fall inside, outside and across the edge of the packed
-    impulse table (outside: the wave generates the lane's cell block into LDS) give the
-    same bits as flights on the generator alone and on a table that covers everything."""
+    impulse table give the same bits as flights on the generator alone and on a table that
+    covers everything (support lists off: the exact 27-cell path).  Queries one or two cells
+    below a table's low corner once passed the table test by unsigned wrap-around and read
+    far outside the table (found by the bounds-checked host build, tests/field_check.cpp)."""
     from gpis_extend_amd import workloads as W, RayBatch
     print("case: C0 16,384 flights, no table / partial table / full table", flush=True)
     outs = []
     for dom in (None, ((-0.6,) * 3, (0.6,) * 3), ((-0.3, -2.1, -0.3), (2.1, 0.3, 2.1)), ((-2.1,) * 3, (2.1,) * 3)):
         med, dev = _dev(W.config0_json(), dom)
+        if dom is not None:
+            dev.set_lists(0)
         rays = W.sphere_rays(med, n=16384)
         rep = dev.trace_flights(RayBatch.from_numpy(rays))
         torch.cuda.synchronize()
@@ -177,20 +178,22 @@ def test_partial_impulse_table_equals_generator():
             assert np.array_equal(o[k].view(np.uint32), outs[0][k].view(np.uint32)), k
 
 
-@pytest.mark.parametrize("sub", [1, 3, 6])
-def test_support_lists_keep_fields_and_flights(sub):
-    """Support lists (sub-cells per cell edge 1, 3, 6) against the table's exact 27-cell
-    path: the same support decisions in the same order, so field values differ only in the
-    rounding of the splat arguments, and flights keep their outcomes and distances."""
+@pytest.mark.parametrize("sub,points", [(1, 1), (3, 1), (3, 2), (6, 2)])
+def test_support_lists_keep_fields_and_flights(sub, points):
+    """Support lists (sub-cells per cell edge; one or two evaluation points per list pass)
+    against the table's exact 27-cell path: the same support decisions in the same order, so
+    field values differ only in the rounding of the splat arguments, and flights keep their
+    outcomes, distances and evaluation counts."""
     from gpis_extend_amd import workloads as W, RayBatch
-    print(f"case: support lists sub={sub}, C1 eval_points 65,536 and 16,384 camera paths", flush=True)
+    print(f"case: support lists sub={sub} points={points}, C1 eval_points 65,536 and 16,384 camera paths",
+          flush=True)
     med, dev = _dev(W.config1_json(), ((-1.0,) * 3, (1.0,) * 3))
     x = torch.from_numpy(W.points_workload(65536, seed=W.SEED_POINTS)).cuda()
     rays = W.camera_rays(med, width=128, height=128)
     rb = RayBatch.from_numpy(rays)
     outs = []
     for s in (0, sub):
-        dev.set_lists(s)
+        dev.set_lists(s, points)
         v, gx, gy, gz = dev.eval_points(x[:, 0].contiguous(), x[:, 1].contiguous(), x[:, 2].contiguous(), med.seed)
         pa = dev.trace_paths(rb, 2, 7, 1.0)
         torch.cuda.synchronize()
@@ -203,9 +206,11 @@ def test_support_lists_keep_fields_and_flights(sub):
     both = (p0["outcome0"] & 0xFF) == 1
     dt = np.abs(p1["t0"][both] - p0["t0"][both]) / np.maximum(np.abs(p0["t0"][both]), 1e-6)
     same_paths = float(np.mean((p0["flights"] == p1["flights"]) & (p0["hits"] == p1["hits"])))
+    same_evals = float(np.mean(p0["evals"] == p1["evals"]))
     far = float(np.mean(dt > 1e-5))
     print(f"field max rel diff {dmax:.2e}; first-flight outcome agreement {agree:.5f}, hits with rel dt > 1e-5: "
-          f"{far:.5f} (median {np.median(dt):.1e}); paths with the same flights and hits {same_paths:.5f}")
+          f"{far:.5f} (median {np.median(dt):.1e}); paths with the same flights and hits {same_paths:.5f}, "
+          f"the same evaluation count {same_evals:.5f}")
     # FP32 rounding of the splat arguments only (the support decisions are the same).  A
     # flight whose secant shrink ends right at the 0.01 cutoff (t = 0, the ray origin) can
     # switch sides on a last-bit difference: such flights stay below 5 in 10,000.
@@ -216,6 +221,7 @@ def test_support_lists_keep_fields_and_flights(sub):
     assert agree >= 0.9999
     assert far <= 5e-4
     assert same_paths >= 0.999
+    assert same_evals >= 0.999
 
 
 def test_paths_first_flight_equals_single_flights():
