@@ -27,6 +27,7 @@ using namespace gpis;
 struct gpis_ctx {
     gpis_field_desc desc;
     FieldParams P;
+    FieldParams *P_dev = nullptr;  // device copy of P, read by every kernel (sync_params)
     float *table = nullptr;  // packed impulse table (3N floats per cell)
     size_t table_cells = 0;
     int table_lo[3] = {0, 0, 0}, table_dim[3] = {0, 0, 0};
@@ -53,6 +54,7 @@ struct gpis_ctx {
 
 static void free_lists(gpis_ctx *c);
 static gpis_status build_lists(gpis_ctx *c);
+static gpis_status sync_params(gpis_ctx *c);
 
 static gpis_status fail(gpis_ctx *c, gpis_status s, const char *msg) {
     if (c) c->err = msg;
@@ -67,6 +69,15 @@ static gpis_status hip_fail(gpis_ctx *c, hipError_t e, const char *where) {
         hipError_t _e = (expr);                           \
         if (_e != hipSuccess) return hip_fail(c, _e, #expr); \
     } while (0)
+
+// The kernels read the field parameters from the context's device copy; every call that
+// changes c->P copies it again before returning (synchronous).
+static gpis_status sync_params(gpis_ctx *c) {
+    if (!c->P_dev) return GPIS_OK;
+    HIP_TRY(c, hipDeviceSynchronize());
+    HIP_TRY(c, hipMemcpy(c->P_dev, &c->P, sizeof(FieldParams), hipMemcpyHostToDevice));
+    return GPIS_OK;
+}
 
 // Host restatement of floor(log(s)/log(2.5)) in float (SparseConvolutionNoise.cpp:309).
 static int host_level_index(float s) { return (int)floorf(logf(s) / logf(2.5f)); }
@@ -324,11 +335,12 @@ __global__ void list_fill_kernel(const float *__restrict__ table, int lox, int l
 }
 
 template <int NI, bool ISO, int MEAN>
-__global__ __launch_bounds__(256) void eval_points_kernel(FieldParams P, const float *__restrict__ x,
+__global__ __launch_bounds__(256) void eval_points_kernel(const FieldParams *__restrict__ Pg, const float *__restrict__ x,
                                                           const float *__restrict__ y, const float *__restrict__ z,
                                                           long long n, uint32_t seed, float *__restrict__ v,
                                                           float *__restrict__ gx, float *__restrict__ gy,
                                                           float *__restrict__ gz) {
+    const FieldParams &P = *Pg;
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     RealState rs = {};
@@ -546,12 +558,13 @@ enum { ST_FLIGHTS = 0, ST_EVALS, ST_STEPS, ST_HITS, ST_EXITS, ST_FAILS, ST_INVAL
 // stay free of its registers);
 // IRAY: the noise lives in each flight's isotropic ray space (isotropic_3D_sampling).
 template <int NI, bool ISO, bool COND, bool PATHS, bool GRID, bool IRAY = false>
-__global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(FieldParams P, RayIn in, ReplyOut out,
+__global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(const FieldParams *__restrict__ Pg, RayIn in, ReplyOut out,
                                                                             PathOut pout, unsigned int n,
                                                                             unsigned int *work,
                                                                             unsigned long long *stats,
                                                                             int max_bounces, uint32_t bounce_seed,
                                                                             float box, int batch_k) {
+    const FieldParams &P = *Pg;
     __shared__ unsigned int bstat[ST_N];
     if (threadIdx.x < ST_N) bstat[threadIdx.x] = 0;
     __syncthreads();
@@ -1123,13 +1136,14 @@ __device__ __forceinline__ uint32_t finish1d(const FieldParams &P, const Ray1D &
 // flight, then the mirror or Lambert bounce) and the next flight starts from the hit,
 // conditioned on (0, gradient), without leaving the kernel.
 template <int NI, bool PATHS, bool PROC>
-__global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(FieldParams P, RayIn in, const float *__restrict__ lscale,
+__global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(const FieldParams *__restrict__ Pg, RayIn in, const float *__restrict__ lscale,
                                                           ReplyOut out, unsigned int n, unsigned int *work,
                                                           unsigned long long *stats, float nf, int use_light,
                                                           float lx, float ly, float lz, Nee1DOut nee,
                                                           const uint32_t *__restrict__ idx,
                                                           const uint32_t *__restrict__ n_dev, int batch_k,
                                                           Path1DIn pin, Path1DOut pout) {
+    const FieldParams &P = *Pg;
     // idx / n_dev: optional list of ray indices (and its device-side length) to trace
     const unsigned nn = n_dev ? *n_dev : n;
     __shared__ unsigned int bstat[ST_N];
@@ -1624,11 +1638,12 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(
 // flight.  Ray indices come from idx[0 .. *n_dev).
 constexpr int kGroup = 16;
 template <int NI, bool PROC>
-__global__ __launch_bounds__(256, 2) void flight1d_group_kernel(FieldParams P, RayIn in,
+__global__ __launch_bounds__(256, 2) void flight1d_group_kernel(const FieldParams *__restrict__ Pg, RayIn in,
                                                                 const float *__restrict__ lscale, ReplyOut out,
                                                                 unsigned int *work, unsigned long long *stats,
                                                                 float nf, const uint32_t *__restrict__ idx,
                                                                 const uint32_t *__restrict__ n_dev) {
+    const FieldParams &P = *Pg;
     const unsigned nn = *n_dev;
     __shared__ unsigned int bstat[ST_N];
     if (threadIdx.x < ST_N) bstat[threadIdx.x] = 0;
@@ -1870,11 +1885,11 @@ static void launch_flights(gpis_ctx *c, unsigned blocks, unsigned threads, hipSt
     do {                                                                                                           \
         if (grid)                                                                                                  \
             hipLaunchKernelGGL((flight_kernel<NI, ISO, COND, PATHS, true>), dim3(blocks), dim3(threads), 0, st,    \
-                               c->P, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box,        \
+                               c->P_dev, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box,        \
                                c->batch_k);                                                                        \
         else                                                                                                       \
             hipLaunchKernelGGL((flight_kernel<NI, ISO, COND, PATHS, false>), dim3(blocks), dim3(threads), 0, st,   \
-                               c->P, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box,        \
+                               c->P_dev, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box,        \
                                c->batch_k);                                                                        \
     } while (0)
     const bool cond = c->P.activate != 0;
@@ -1883,10 +1898,10 @@ static void launch_flights(gpis_ctx *c, unsigned blocks, unsigned threads, hipSt
         // the isotropic ray space: generic impulse count, isotropic kernel, analytic means
         if (cond)
             hipLaunchKernelGGL((flight_kernel<0, true, true, PATHS, false, true>), dim3(blocks), dim3(threads), 0, st,
-                               c->P, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box, c->batch_k);
+                               c->P_dev, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box, c->batch_k);
         else
             hipLaunchKernelGGL((flight_kernel<0, true, false, PATHS, false, true>), dim3(blocks), dim3(threads), 0, st,
-                               c->P, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box, c->batch_k);
+                               c->P_dev, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box, c->batch_k);
         return;
     }
     if (c->P.N == 8 && c->P.iso && !cond) GPIS_FLIGHT_LAUNCH(8, true, false);
@@ -1925,11 +1940,14 @@ gpis_status gpis_create(const gpis_field_desc *desc, gpis_ctx **out) {
         if (e == hipSuccess) e = hipMemcpy(c->sdf_dev, &c->sdf_host, sizeof(sdf::MeanParams), hipMemcpyHostToDevice);
         c->P.sdf = c->sdf_dev;
     }
+    if (e == hipSuccess) e = hipMalloc((void **)&c->P_dev, sizeof(FieldParams));
+    if (e == hipSuccess) e = hipMemcpy(c->P_dev, &c->P, sizeof(FieldParams), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         fprintf(stderr, "gpis_create: %s\n", hipGetErrorString(e));
         if (c->counters) (void)hipFree(c->counters);
         if (c->work) (void)hipFree(c->work);
         if (c->sdf_dev) (void)hipFree(c->sdf_dev);
+        if (c->P_dev) (void)hipFree(c->P_dev);
         delete c;
         return GPIS_DEVICE_ERROR;
     }
@@ -1946,13 +1964,14 @@ gpis_status gpis_destroy(gpis_ctx *c) {
     if (c->sdf_dev) (void)hipFree(c->sdf_dev);
     if (c->counters) (void)hipFree(c->counters);
     if (c->work) (void)hipFree(c->work);
+    if (c->P_dev) (void)hipFree(c->P_dev);
     delete c;
     return GPIS_OK;
 }
 
 const char *gpis_last_error(const gpis_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
-gpis_status gpis_set_mean_grid(gpis_ctx *c, const gpis_mean_grid *g) {
+static gpis_status gpis_set_mean_grid_impl(gpis_ctx *c, const gpis_mean_grid *g) {
     if (!c || !g || !g->voxels) return fail(c, GPIS_INVALID_ARGUMENT, "null argument");
     if (c->P.mean_kind != GPIS_MEAN_GRID) return fail(c, GPIS_INVALID_ARGUMENT, "the medium's mean is not tabulated");
     size_t count = 1;
@@ -1984,6 +2003,13 @@ gpis_status gpis_set_mean_grid(gpis_ctx *c, const gpis_mean_grid *g) {
     P.gscale = g->scale;
     P.gvolume = g->volume != 0;
     return GPIS_OK;
+}
+
+gpis_status gpis_set_mean_grid(gpis_ctx *c, const gpis_mean_grid *g) {
+    if (!c) return GPIS_INVALID_ARGUMENT;
+    const gpis_status s = gpis_set_mean_grid_impl(c, g);
+    const gpis_status t = sync_params(c);
+    return s != GPIS_OK ? s : t;
 }
 
 static gpis_status mean_ready(gpis_ctx *c) {
@@ -2062,7 +2088,7 @@ static gpis_status build_lists(gpis_ctx *c) {
     return GPIS_OK;
 }
 
-gpis_status gpis_set_domain(gpis_ctx *c, const float lo[3], const float hi[3], uint32_t seed) {
+static gpis_status gpis_set_domain_impl(gpis_ctx *c, const float lo[3], const float hi[3], uint32_t seed) {
     if (!c || !lo || !hi) return fail(c, GPIS_INVALID_ARGUMENT, "null argument");
     if (c->P.multires) return fail(c, GPIS_UNSUPPORTED, "impulse tables cover the single-level grid");
     if (c->P.iso3d) return fail(c, GPIS_UNSUPPORTED, "the isotropic ray space has one lattice per flight direction");
@@ -2104,6 +2130,13 @@ gpis_status gpis_set_domain(gpis_ctx *c, const float lo[3], const float hi[3], u
     return build_lists(c);
 }
 
+gpis_status gpis_set_domain(gpis_ctx *c, const float lo[3], const float hi[3], uint32_t seed) {
+    if (!c) return GPIS_INVALID_ARGUMENT;
+    const gpis_status s = gpis_set_domain_impl(c, lo, hi, seed);
+    const gpis_status t = sync_params(c);
+    return s != GPIS_OK ? s : t;
+}
+
 gpis_status gpis_impulse_table(gpis_ctx *c, const int32_t lo[3], const int32_t dims[3], uint32_t seed, float *out,
                                void *stream) {
     if (!c || !lo || !dims || !out) return fail(c, GPIS_INVALID_ARGUMENT, "null argument");
@@ -2134,10 +2167,10 @@ gpis_status gpis_eval_points(gpis_ctx *c, const float *x, const float *y, const 
     do {                                                                                                           \
         if (ext)                                                                                                   \
             hipLaunchKernelGGL((eval_points_kernel<NI, ISO, MEAN_EXT>), dim3(blocks), dim3(threads), 0,            \
-                               (hipStream_t)stream, c->P, x, y, z, (long long)n, seed, v, gx, gy, gz);            \
+                               (hipStream_t)stream, c->P_dev, x, y, z, (long long)n, seed, v, gx, gy, gz);            \
         else                                                                                                       \
             hipLaunchKernelGGL((eval_points_kernel<NI, ISO, MEAN_ANALYTIC>), dim3(blocks), dim3(threads), 0,       \
-                               (hipStream_t)stream, c->P, x, y, z, (long long)n, seed, v, gx, gy, gz);            \
+                               (hipStream_t)stream, c->P_dev, x, y, z, (long long)n, seed, v, gx, gy, gz);            \
     } while (0)
     const bool ext = c->P.mean_kind == GPIS_MEAN_GRID || c->P.mean_kind == GPIS_MEAN_PROCEDURAL;
     if (c->P.N == 8 && c->P.iso) GPIS_EVAL_LAUNCH(8, true);
@@ -2220,13 +2253,20 @@ gpis_status gpis_trace_paths(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, i
     return GPIS_OK;
 }
 
-gpis_status gpis_set_lists(gpis_ctx *c, int32_t sub, int32_t points) {
+static gpis_status gpis_set_lists_impl(gpis_ctx *c, int32_t sub, int32_t points) {
     if (!c) return GPIS_INVALID_ARGUMENT;
     if (sub < 0 || sub > lists::kMaxSub) return fail(c, GPIS_INVALID_ARGUMENT, "sub-cells per cell edge must be in 0..6");
     if (points != 1 && points != 2) return fail(c, GPIS_INVALID_ARGUMENT, "points per list pass must be 1 or 2");
     c->lists_sub = sub;
     c->lists_points = points;
     return build_lists(c);
+}
+
+gpis_status gpis_set_lists(gpis_ctx *c, int32_t sub, int32_t points) {
+    if (!c) return GPIS_INVALID_ARGUMENT;
+    const gpis_status s = gpis_set_lists_impl(c, sub, points);
+    const gpis_status t = sync_params(c);
+    return s != GPIS_OK ? s : t;
 }
 
 gpis_status gpis_set_refill(gpis_ctx *c, int32_t batch) {
@@ -2249,12 +2289,12 @@ static gpis_status launch_flight1d(gpis_ctx *c, const gpis_ray_soa *rays, int64_
     long long need = (n + threads - 1) / threads;
     unsigned blocks = (unsigned)(need < want ? need : (want < 1 ? 1 : want));
     if (c->P.mean_kind == GPIS_MEAN_PROCEDURAL)
-        hipLaunchKernelGGL((flight1d_kernel<8, false, true>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+        hipLaunchKernelGGL((flight1d_kernel<8, false, true>), dim3(blocks), dim3(threads), 0, st, c->P_dev, make_in(rays),
                            rays->length_scale, out, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
                            light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, nee, idx, n_dev,
                            c->batch_k_1d, Path1DIn{}, Path1DOut{});
     else
-        hipLaunchKernelGGL((flight1d_kernel<8, false, false>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+        hipLaunchKernelGGL((flight1d_kernel<8, false, false>), dim3(blocks), dim3(threads), 0, st, c->P_dev, make_in(rays),
                            rays->length_scale, out, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
                            light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, nee, idx, n_dev,
                            c->batch_k_1d, Path1DIn{}, Path1DOut{});
@@ -2302,12 +2342,12 @@ gpis_status gpis_trace_paths_1d(gpis_ctx *c, const gpis_ray_soa *rays, const gpi
                       reply->flights, reply->hits,  reply->shadows, reply->visible,     reply->nee,
                       reply->t_last,  reply->outcome_last, reply->evals};
     if (c->P.mean_kind == GPIS_MEAN_PROCEDURAL)
-        hipLaunchKernelGGL((flight1d_kernel<8, true, true>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+        hipLaunchKernelGGL((flight1d_kernel<8, true, true>), dim3(blocks), dim3(threads), 0, st, c->P_dev, make_in(rays),
                            rays->length_scale, ReplyOut{}, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
                            light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, Nee1DOut{}, nullptr,
                            nullptr, c->batch_k_1d, pin, pout);
     else
-        hipLaunchKernelGGL((flight1d_kernel<8, true, false>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+        hipLaunchKernelGGL((flight1d_kernel<8, true, false>), dim3(blocks), dim3(threads), 0, st, c->P_dev, make_in(rays),
                            rays->length_scale, ReplyOut{}, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
                            light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, Nee1DOut{}, nullptr,
                            nullptr, c->batch_k_1d, pin, pout);
@@ -2324,10 +2364,10 @@ static gpis_status launch_flight1d_groups(gpis_ctx *c, const gpis_ray_soa *rays,
     const long long need = (n + groups_per_block - 1) / groups_per_block;
     const unsigned blocks = (unsigned)(need < want ? need : (want < 1 ? 1 : want));
     if (c->P.mean_kind == GPIS_MEAN_PROCEDURAL)
-        hipLaunchKernelGGL((flight1d_group_kernel<8, true>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+        hipLaunchKernelGGL((flight1d_group_kernel<8, true>), dim3(blocks), dim3(threads), 0, st, c->P_dev, make_in(rays),
                            rays->length_scale, out, c->work, c->counters, c->nf1d, idx, n_dev);
     else
-        hipLaunchKernelGGL((flight1d_group_kernel<8, false>), dim3(blocks), dim3(threads), 0, st, c->P, make_in(rays),
+        hipLaunchKernelGGL((flight1d_group_kernel<8, false>), dim3(blocks), dim3(threads), 0, st, c->P_dev, make_in(rays),
                            rays->length_scale, out, c->work, c->counters, c->nf1d, idx, n_dev);
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
@@ -2387,8 +2427,9 @@ gpis_status gpis_set_launch(gpis_ctx *c, int32_t waves_per_cu, int32_t block_siz
 
 // ===================================================================== surrogate --
 // Features of a conditioned 1D flight (order documented in include/gpis_surrogate.h).
-__global__ void features1d_kernel(FieldParams P, RayIn in, const float *__restrict__ lscale, int n,
+__global__ void features1d_kernel(const FieldParams *__restrict__ Pg, RayIn in, const float *__restrict__ lscale, int n,
                                   float *__restrict__ feats) {
+    const FieldParams &P = *Pg;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const float ox = in.ox[r], oy = in.oy[r], oz = in.oz[r];
@@ -2643,7 +2684,7 @@ gpis_status gpis_net_features_1d(gpis_ctx *c, const gpis_ray_soa *rays, int64_t 
     if (gpis_status s = mean_ready(c)) return s;
     if (n == 0) return GPIS_OK;
     hipLaunchKernelGGL(features1d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                       c->P, make_in(rays), rays->length_scale, (int)n, feats);
+                       c->P_dev, make_in(rays), rays->length_scale, (int)n, feats);
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }
