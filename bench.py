@@ -522,7 +522,7 @@ def main():
         dev.set_launch(args.waves_per_cu, 0)
     elif not args.no_autotune:
         from gpis_extend_amd import autotune
-        tuned = autotune.tune(dev, "paths", n, lambda: dev.trace_paths(batch, 2, 7, 1.0, reply))
+        tuned = autotune.tune(dev, "paths", n, lambda: dev.trace_paths(batch, 2, 7, 1.0, reply), lists=True)
 
     for _ in range(args.warmup):
         dev.trace_paths(batch, 2, 7, 1.0, reply)
@@ -564,7 +564,10 @@ def main():
                                f"{args.width}x{args.width} pinhole rays + 2 diffuse bounces",
                    "rays": n, "flights_per_step": flights_per_step, "evals_per_step": evals_per_step,
                    "hits_per_step": c["hits"] / args.steps, "fails_per_step": c["fails"] / args.steps,
-                   "launch": {"refill_batch": tuned[0], "waves_per_cu": tuned[1], "block": tuned[2]} if tuned else None},
+                   "launch": {"refill_batch": tuned[0], "waves_per_cu": tuned[1], "block": tuned[2],
+                              "list_subcells": tuned[3], "points_per_list_pass": 2} if tuned else None,
+                   "list_passes_per_step": c["passes"] / args.steps, "paired_per_step": c["paired"] / args.steps,
+                   "spec_evals_per_step": c["spec_evals"] / args.steps},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "frac_scalar_issue_peak": achieved / PEAK_FP32_SCALAR_TFLOPS,

@@ -762,15 +762,17 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(cons
         // the next query point: t along the ray (unit: along the normalised direction)
         double tq = 0.0;
         bool unit = false;
-        auto consume = [&](const Eval4 &e) {
+        // `second`: the result of the point made ahead, which only a flight in the march, the
+        // secant or the gradient phase asks for (the other phases' code drops out)
+        auto consume = [&](const Eval4 &e, const bool second) {
             ++evals_flight;
             const float fv = e.v;
-            if (COND && phase == PH_COND) {
+            if (COND && !second && phase == PH_COND) {
                 condition_coefficients<GRID ? MEAN_EXT : MEAN_ANALYTIC, IRAY>(P, e, ox, oy, oz, prev_v, prev_gx, prev_gy,
                                                                               prev_gz, dx, dy, dz, rs);
                 phase = PH_F0;
                 tq = (double)near_t;
-            } else if (phase == PH_F0) {
+            } else if (!second && phase == PH_F0) {
                 sign0 = fv < 0.f ? -1 : 1;
                 pf = fv;
                 k = 0;
@@ -845,7 +847,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(cons
                     tq = tp;
                     unit = true;
                 }
-            } else if (phase == PH_FAR) {
+            } else if (!second && phase == PH_FAR) {
                 hit = 0;
                 t_end = far_t;
                 val_end = fv;
@@ -867,11 +869,11 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(cons
                 phase = PH_DONE;
             }
         };
-        consume(ea);
+        consume(ea, false);
         ++lane_passes;
         if (two) {
             if (phase == spec_phase && tq == spec_t && unit == spec_unit) {
-                consume(eb);
+                consume(eb, true);
                 ++lane_paired;
             } else {
                 atomicAdd(&bstat[ST_SPEC], 1u);

@@ -22,7 +22,7 @@
 // below 2^-23 (|p / R| >= 1) this is the reference's to = (frac - d) - p_i to the bit,
 // whatever the neighbour offset d.  Near the origin planes (|p / R| < 1) the query is
 // rounded to 2^-23 first; the lane then flags any pair whose |to|^2 lies within a relative
-// 4e-6 of 1 and the caller decides that query again on the exact 27-cell path, so the
+// 2e-6 of 1 and the caller decides that query again on the exact 27-cell path, so the
 // support decisions stay the reference's for every query.
 //
 // Lists are padded to groups of four entries with a sentinel impulse placed 2^7 cells away
@@ -46,7 +46,11 @@ constexpr int32_t kSentinel = 1 << 30;       // sentinel impulse coordinate (2^7
 constexpr double kReach2 = 1.0001;           // list membership: squared distance margin (relative)
 constexpr float kPairReach = 1.125f;         // reach of lists that serve two queries per pass
 constexpr double kBoxPad = 1e-5;             // sub-cell box padding (rounding of frac * S)
-constexpr float kNearRel = 4e-6f;            // pairs with | |to|^2 - 1 | below this are decided again
+// Pairs with | |to|^2 - 1 | below kNearRel are decided again on the exact path.  The list
+// path's |to|^2 differs from the reference's by the fused products (a few ulp, < 3e-7) and,
+// for queries with |p / R| < 1, by the rounding of the query to 2^-23 (|to| moves by at most
+// 2^-23 per axis: < 8e-7); 2e-6 covers both with a margin.
+constexpr float kNearRel = 2e-6f;
 constexpr int kMaxSub = 6;                   // sub-cells per cell edge: 1..kMaxSub
 
 // Linear index of the table cell (x, y, z) (absolute cell indices), x fastest.
