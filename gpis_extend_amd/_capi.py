@@ -35,7 +35,7 @@ EXPORTS = (
     # include/gpis_surrogate.h
     "gpis_net_create", "gpis_net_destroy", "gpis_net_last_error", "gpis_net_set_params", "gpis_net_get_params",
     "gpis_net_set_calibration", "gpis_net_features_1d", "gpis_net_classes_1d", "gpis_net_gradient",
-    "gpis_net_train_step", "gpis_net_forward", "gpis_surrogate_flights_1d",
+    "gpis_net_train_step", "gpis_net_forward", "gpis_surrogate_flights_1d", "gpis_surrogate_flights_1d_noalloc",
 )
 
 NET_FEATURES, NET_CLASSES, NET_PARAMS = 8, 32, 54816
@@ -181,6 +181,9 @@ def lib():
     L.gpis_net_forward.argtypes = [_vp, _vp, C.c_int64, _vp, _vp]
     L.gpis_surrogate_flights_1d.restype = st
     L.gpis_surrogate_flights_1d.argtypes = [_vp, _vp, C.POINTER(RaySoA), C.c_int64, C.POINTER(ReplySoA),
+                                            C.POINTER(C.c_uint32), _vp]
+    L.gpis_surrogate_flights_1d_noalloc.restype = st
+    L.gpis_surrogate_flights_1d_noalloc.argtypes = [_vp, _vp, C.POINTER(RaySoA), C.c_int64, C.POINTER(ReplySoA),
                                             C.POINTER(C.c_uint32), _vp]
     _lib = L
     return L

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include <random>
@@ -2359,18 +2360,23 @@ gpis_status gpis_trace_paths_1d(gpis_ctx *c, const gpis_ray_soa *rays, const gpi
 
 // Launch the group kernel over the first *n_dev entries of the index list idx (n bounds them).
 static gpis_status launch_flight1d_groups(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, const ReplyOut &out,
-                                          const uint32_t *idx, const uint32_t *n_dev, hipStream_t st) {
-    HIP_TRY(c, hipMemsetAsync(c->work, 0, sizeof(unsigned), st));
+                                          const uint32_t *idx, const uint32_t *n_dev, hipStream_t st,
+                                          unsigned *work = nullptr) {
+    // the persistent grid's work counter: the context's, or one the caller cleared
+    if (!work) {
+        work = c->work;
+        HIP_TRY(c, hipMemsetAsync(work, 0, sizeof(unsigned), st));
+    }
     const unsigned threads = 256, groups_per_block = threads / kGroup;
     const long long want = (long long)c->num_cus * 2;  // two blocks of four waves per CU (2 waves per SIMD)
     const long long need = (n + groups_per_block - 1) / groups_per_block;
     const unsigned blocks = (unsigned)(need < want ? need : (want < 1 ? 1 : want));
     if (c->P.mean_kind == GPIS_MEAN_PROCEDURAL)
         hipLaunchKernelGGL((flight1d_group_kernel<8, true>), dim3(blocks), dim3(threads), 0, st, c->P_dev, make_in(rays),
-                           rays->length_scale, out, c->work, c->counters, c->nf1d, idx, n_dev);
+                           rays->length_scale, out, work, c->counters, c->nf1d, idx, n_dev);
     else
         hipLaunchKernelGGL((flight1d_group_kernel<8, false>), dim3(blocks), dim3(threads), 0, st, c->P_dev, make_in(rays),
-                           rays->length_scale, out, c->work, c->counters, c->nf1d, idx, n_dev);
+                           rays->length_scale, out, work, c->counters, c->nf1d, idx, n_dev);
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }
@@ -2786,6 +2792,85 @@ gpis_status gpis_surrogate_flights_1d(gpis_ctx *c, gpis_net *net, const gpis_ray
     if (s != GPIS_OK) return s;
     if (n_fallback) {
         HIP_TRY(c, hipMemcpyAsync(n_fallback, net->n_fallback, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIP_TRY(c, hipStreamSynchronize(st));
+    }
+    return GPIS_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ allocation-free surrogate flights --
+// Scratch of gpis_surrogate_flights_1d_noalloc: the features of the rays, the list of
+// rays that fall back to the plain method, its length and the plain kernel's work
+// counter.  One copy per GPU, held until the process ends; host code uses the address
+// hipGetSymbolAddress returns on each device (the array's name is only a host stand-in).
+constexpr int64_t kNoAllocMaxRays = 1 << 18;
+constexpr size_t kNoAllocFeatBytes = (size_t)kNoAllocMaxRays * mlp::kIn * sizeof(float);  // 8 MiB
+constexpr size_t kNoAllocListBytes = (size_t)kNoAllocMaxRays * sizeof(uint32_t);           // 1 MiB
+constexpr size_t kNoAllocBytes = kNoAllocFeatBytes + kNoAllocListBytes + 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static unsigned char *g_scratch[kNoAllocMaxDevices];
+static std::mutex g_scratch_mutex;
+
+// The device address of `scratch` on the current device (looked up once per device).
+static hipError_t noalloc_scratch(unsigned char **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch[dev];
+    return hipSuccess;
+}
+
+extern "C" {
+
+gpis_status gpis_surrogate_flights_1d_noalloc(gpis_ctx *c, gpis_net *net, const gpis_ray_soa *rays, int64_t n,
+                                              const gpis_reply_soa *reply, uint32_t *n_fallback, void *stream) {
+    if (!c || !net) return GPIS_INVALID_ARGUMENT;
+    if (!net->calibrated) return nfail(net, GPIS_INVALID_ARGUMENT, "calibration not set");
+    if (!c->P.one_d) return fail(c, GPIS_INVALID_ARGUMENT, "the medium is not 1D_sampling");
+    if (!rays || !reply || !reply->t || !reply->value || !reply->gx || !reply->gy || !reply->gz || !reply->outcome)
+        return fail(c, GPIS_INVALID_ARGUMENT, "null ray or reply field");
+    if (n < 0 || n > kNoAllocMaxRays)
+        return fail(c, GPIS_INVALID_ARGUMENT, "the allocation-free entry point takes at most 262144 rays per call");
+    if (gpis_status ms = mean_ready(c)) return ms;
+    if (n_fallback) *n_fallback = 0;
+    if (n == 0) return GPIS_OK;
+    if (c->P.N != 8) return fail(c, GPIS_UNSUPPORTED, "1D flights are compiled for 8 impulses per cell");
+    int dev = -1;
+    HIP_TRY(c, hipGetDevice(&dev));
+    if (dev != c->device || dev != net->device)
+        return fail(c, GPIS_INVALID_ARGUMENT, "the context, the network and the current device must agree");
+    unsigned char *sp = nullptr;
+    HIP_TRY(c, noalloc_scratch(&sp));
+    float *feats = (float *)sp;
+    uint32_t *fallback = (uint32_t *)(sp + kNoAllocFeatBytes);
+    uint32_t *counts = (uint32_t *)(sp + kNoAllocFeatBytes + kNoAllocListBytes);  // [0] fallback rays, [1] work
+    hipStream_t st = (hipStream_t)stream;
+    gpis_status s = gpis_net_features_1d(c, rays, n, feats, stream);
+    if (s != GPIS_OK) return s;
+    // the counters start at zero (a fill kernel: the entry point records no memset)
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(32), 0, st, counts, 2, 0u);
+    HIP_TRY(c, hipGetLastError());
+    mlp::InferOut o = {reply->t, reply->value, reply->gx, reply->gy, reply->gz, reply->outcome, reply->evals,
+                       nullptr, fallback, counts};
+    s = net_infer(net, feats, n, (const float *)rays->near_t, (const float *)rays->far_t,
+                  (const float *)rays->jitter, (const uint32_t *)rays->flags, o, st);
+    if (s != GPIS_OK) return s;
+    ReplyOut out = {reply->t, reply->value, reply->gx, reply->gy, reply->gz, reply->outcome, reply->evals};
+    s = launch_flight1d_groups(c, rays, n, out, fallback, counts, st, counts + 1);
+    if (s != GPIS_OK) return s;
+    if (n_fallback) {
+        HIP_TRY(c, hipMemcpyAsync(n_fallback, counts, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         HIP_TRY(c, hipStreamSynchronize(st));
     }
     return GPIS_OK;

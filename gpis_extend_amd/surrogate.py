@@ -243,8 +243,10 @@ class FlightSurrogate:
                     "gpis_net_forward")
         return p
 
-    def flights(self, dev: GpisDevice, rays: RayBatch, reply=None, count_fallback: bool = True):
-        """Surrogate flights with the plain method as fallback; returns (reply, n_fallback)."""
+    def flights(self, dev: GpisDevice, rays: RayBatch, reply=None, count_fallback: bool = True,
+                no_alloc: bool = False):
+        """Surrogate flights with the plain method as fallback; returns (reply, n_fallback).
+        `no_alloc`: the allocation-free entry point (at most 262,144 rays per call)."""
         if reply is None:
             reply = dev.alloc_reply(rays.n)
         r = A.ReplySoA()
@@ -252,12 +254,13 @@ class FlightSurrogate:
             setattr(r, k, reply[k].data_ptr())
         soa = rays.soa()
         nfb = C.c_uint32(0)
-        st = self.L.gpis_surrogate_flights_1d(dev.h, self.h, C.byref(soa), rays.n, C.byref(r),
-                                              C.byref(nfb) if count_fallback else None, self._stream())
+        fn = self.L.gpis_surrogate_flights_1d_noalloc if no_alloc else self.L.gpis_surrogate_flights_1d
+        st = fn(dev.h, self.h, C.byref(soa), rays.n, C.byref(r), C.byref(nfb) if count_fallback else None,
+                self._stream())
         if st != A.GPIS_OK:
             msg = self.L.gpis_net_last_error(self.h).decode(errors="replace") or \
                 self.L.gpis_last_error(dev.h).decode(errors="replace")
-            raise A.GpisError(f"gpis_surrogate_flights_1d: {A.STATUS_NAMES.get(st, st)} {msg}")
+            raise A.GpisError(f"{fn.__name__}: {A.STATUS_NAMES.get(st, st)} {msg}")
         return reply, int(nfb.value)
 
 

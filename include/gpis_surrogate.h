@@ -87,6 +87,19 @@ gpis_status gpis_net_forward(gpis_net *net, const float *feats, int64_t n, float
 gpis_status gpis_surrogate_flights_1d(gpis_ctx *ctx, gpis_net *net, const gpis_ray_soa *rays, int64_t n,
                                       const gpis_reply_soa *reply, uint32_t *n_fallback, void *stream);
 
+/* gpis_surrogate_flights_1d without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same inputs, outputs and kernels as
+ * gpis_surrogate_flights_1d (replies identical bit for bit), but the ray features, the
+ * fallback list and its counters live in a fixed device array of the library (9 MiB per
+ * GPU, held until the process ends) instead of buffers grown on demand.  At most 262,144
+ * rays per call (larger n: GPIS_INVALID_ARGUMENT); ctx, net and the current device must be
+ * the same GPU.  Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (other streams, other host threads) race.  n_fallback, when given, synchronises the
+ * stream; pass NULL under graph capture. */
+gpis_status gpis_surrogate_flights_1d_noalloc(gpis_ctx *ctx, gpis_net *net, const gpis_ray_soa *rays, int64_t n,
+                                              const gpis_reply_soa *reply, uint32_t *n_fallback, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
