@@ -47,94 +47,123 @@ PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector peak (packed FMA issue)
 PEAK_FP32_SCALAR_TFLOPS = 78.6    # one FMA per lane per cycle, no packed issue
 
 
-def cpu_baseline(med, rays, max_seconds=12.0):
-    """The oracle (a CPU port of the reference algorithm) on a bounded sample of the
-    same workload: whole paths (primary flight + up to 2 bounces) of the first
-    camera rays, all host cores, flights/s."""
-    from oracle import pyoracle as po
+_ORACLE_BUILD = None
+
+
+def _oracle_build() -> str:
+    """Builds the oracle for this host (-O3 -march=native, the same strict rounding as the
+    checking build) into a fresh temporary folder and loads it for the CPU baselines;
+    falls back to the portable -O2 build if the compiler is unavailable."""
+    global _ORACLE_BUILD
+    if _ORACLE_BUILD is None:
+        import subprocess
+        import tempfile
+        from oracle import pyoracle as po
+        out = tempfile.mkdtemp(prefix="gpis_oracle_native_")
+        try:
+            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"NATIVE_DIR={out}"],
+                           check=True, timeout=180, capture_output=True)
+            po.use_library(os.path.join(out, "libgpis_oracle.so"))
+            _ORACLE_BUILD = "gcc -O3 -march=native -ffp-contract=off (built on this host)"
+        except Exception as e:  # noqa: BLE001 - any build failure keeps the portable build
+            _ORACLE_BUILD = f"gcc -O2 -ffp-contract=off (portable build; native build failed: {type(e).__name__})"
+    return _ORACLE_BUILD
+
+
+def _host_cpus(max_threads=16):
+    """(threads used, physical cores visible, logical CPUs visible, CPU model): one worker
+    per physical core of this process's CPU set, at most `max_threads` (a GPU's share of
+    the host)."""
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        try:
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/core_id") as f:
+                core = f.read().strip()
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/physical_package_id") as f:
+                pkg = f.read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("?", c))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return max(1, min(len(cores), max_threads)), len(cores), len(cpus), model
+
+
+def cpu_rates(prepare, run, n_max, unit, what, target_s=0.8, runs=5, n0=256):
+    """The CPU baseline: the oracle on uniform samples of the workload, single-thread and
+    with one worker per physical core, each the median of `runs` timed runs after a
+    warm-up run on the same sample.  prepare(n) builds the inputs of an n-unit sample
+    (untimed); run(inputs, threads) runs it and returns the units done.  Each sample is
+    sized from a first run so that one timed run takes about `target_s`."""
+    build = _oracle_build()
+    threads, phys, logical, model = _host_cpus()
+    res = {}
+    for label, nth in (("single", 1), ("all", threads)):
+        n = min(n_max, n0)
+        inp = prepare(n)
+        t0 = time.perf_counter()
+        run(inp, nth)
+        el = time.perf_counter() - t0
+        n = int(min(n_max, max(n, n * target_s / max(el, 1e-4))))
+        inp = prepare(n)
+        run(inp, nth)  # warm-up on the timed sample
+        rates = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            units = run(inp, nth)
+            rates.append(units / (time.perf_counter() - t0))
+        res[label] = (float(np.median(rates)), n, units)
+    return {"value": res["all"][0], "unit": unit, "cores": threads, "kind": "port",
+            "single_thread": res["single"][0], "runs": runs, "statistic": "median",
+            "cpu_model": model, "physical_cores_visible": phys, "logical_cpus_visible": logical, "build": build,
+            "sample": f"{what}: {res['single'][1]} (single thread, {res['single'][2]} {unit.split('/')[0]}) and "
+                      f"{res['all'][1]} ({threads} threads, {res['all'][2]} {unit.split('/')[0]}) drawn uniformly "
+                      f"from the workload, through the C oracle; median of {runs} runs after a warm-up run"}
+
+
+def cpu_baseline(med, rays):
+    """The oracle's path restatement (orc_trace_paths3d: the GPU paths' chaining and bounce
+    rule) on uniform samples of the camera-ray paths: primary flight + up to 2 bounces."""
     from tests.helpers import oracle_field, oracle_rays
     from gpis_extend_amd import workloads as W
-    from gpis_extend_amd.hashing import Pcg32, xxhash32_4
 
     F = oracle_field(med)
-    cores = min(os.cpu_count() or 1, 64)
-    n_total = rays["ox"].shape[0]
-    order = np.random.default_rng(12345).permutation(n_total)  # uniform sample over the image
-    sample = 2048
-    flights = 0
-    spent = 0.0
-    done_rays = 0
-    start_idx = 0
-    while spent < max_seconds and start_idx < n_total:
-        idx = np.sort(order[start_idx:start_idx + sample])
-        start_idx += len(idx)
-        cur = W.rays_subset(rays, idx)
-        ray_id = idx.astype(np.uint32)
-        for bounce in range(3):
-            orr = oracle_rays(cur, med)
-            t0 = time.perf_counter()
-            res = F.trace(orr, nthreads=cores)
-            spent += time.perf_counter() - t0
-            flights += len(res)
-            hit = res["outcome"] == po.HIT
-            if bounce == 2 or not hit.any():
-                break
-            # cosine-weighted bounce about the normalised gradient (same rule as the GPU path kernel)
-            sel = np.nonzero(hit)[0]
-            g = res["grad"][sel].astype(np.float32)
-            nrm = g / np.linalg.norm(g, axis=1, keepdims=True)
-            h = xxhash32_4(ray_id[sel], np.full(len(sel), bounce + 1, np.uint32), np.full(len(sel), 7, np.uint32),
-                           np.zeros(len(sel), np.uint32)) + np.uint32(1)
-            pcg = Pcg32(h.astype(np.uint64))
-            u1, u2, jit = pcg.next_1d(), pcg.next_1d(), pcg.next_1d()
-            r_, ph = np.sqrt(u1), np.float32(2 * np.pi) * u2
-            loc = np.stack([r_ * np.cos(ph), r_ * np.sin(ph), np.sqrt(np.maximum(0, 1 - u1))], 1)
-            sgn = np.where(nrm[:, 2] < 0, -1.0, 1.0)
-            a = -1.0 / (sgn + nrm[:, 2]); b = nrm[:, 0] * nrm[:, 1] * a
-            tt = np.stack([1 + sgn * nrm[:, 0] ** 2 * a, sgn * b, -sgn * nrm[:, 0]], 1)
-            bb = np.stack([b, sgn + nrm[:, 1] ** 2 * a, -nrm[:, 1]], 1)
-            d = loc[:, :1] * tt + loc[:, 1:2] * bb + loc[:, 2:] * nrm
-            d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
-            o = (np.stack([cur["ox"][sel], cur["oy"][sel], cur["oz"][sel]], 1)
-                 + res["t"][sel, None].astype(np.float32) * np.stack([cur["dx"][sel], cur["dy"][sel], cur["dz"][sel]], 1)).astype(np.float32)
-            _, tf = W.box_interval(o.astype(np.float64), d.astype(np.float64), 1.0)
-            nxt = W._empty_rays(len(sel))
-            nxt["ox"], nxt["oy"], nxt["oz"] = o[:, 0].copy(), o[:, 1].copy(), o[:, 2].copy()
-            nxt["dx"], nxt["dy"], nxt["dz"] = d[:, 0].copy(), d[:, 1].copy(), d[:, 2].copy()
-            nxt["far_t"] = tf.astype(np.float32)
-            nxt["jitter"] = jit.astype(np.float32)
-            nxt["flags"] = np.zeros(len(sel), np.uint32)
-            nxt["prev_gx"], nxt["prev_gy"], nxt["prev_gz"] = g[:, 0].copy(), g[:, 1].copy(), g[:, 2].copy()
-            nxt["pss"] = cur["pss"][sel]
-            nxt["noise_seed"] = cur["noise_seed"][sel]
-            nxt["normal_seed"] = cur["normal_seed"][sel]
-            cur = nxt
-            ray_id = ray_id[sel]
-        done_rays += len(idx)
-    return {"value": flights / spent, "unit": "flights/s", "cores": cores, "kind": "port",
-            "sample": f"{done_rays} camera-ray paths (primary + up to 2 bounces, {flights} flights) of the same "
-                      f"workload through the C oracle on {cores} host threads, {spent:.1f} s"}
+    order = np.random.default_rng(12345).permutation(rays["ox"].shape[0])
+
+    def prepare(n):
+        return oracle_rays(W.rays_subset(rays, np.sort(order[:n])), med)
+
+    def run(orr, nth):
+        return int(F.trace_paths3d(orr, 2, 7, 1.0, nthreads=nth)["flights"].sum())
+
+    return cpu_rates(prepare, run, len(order), "flights/s", "camera-ray paths (primary + up to 2 bounces)")
 
 
-def cpu_baseline_1d(med, rays, light, max_seconds=12.0):
-    """The oracle on a bounded uniform sample of the 1D workload: conditioned 1D flights
-    with their per-ray length scales, then NEE toward the light at every hit."""
+def cpu_baseline_1d(med, rays, light):
+    """The oracle on uniform samples of the 1D workload: conditioned 1D flights with their
+    per-ray length scales, then NEE toward the light at every hit."""
     from oracle import pyoracle as po
     from tests.helpers import oracle_desc, oracle_rays
     from gpis_extend_amd import workloads as W
 
     desc = oracle_desc(med)
-    cores = min(os.cpu_count() or 1, 64)
-    n_total = rays["ox"].shape[0]
-    order = np.random.default_rng(12345).permutation(n_total)
-    sample, flights, spent, start_idx = 8192, 0, 0.0, 0
-    while spent < max_seconds and start_idx < n_total:
-        idx = np.sort(order[start_idx:start_idx + sample])
-        start_idx += len(idx)
-        cur = W.rays_subset(rays, idx)
-        orr = oracle_rays(cur, med)
-        t0 = time.perf_counter()
-        res = po.trace_ls(desc, orr, cur["length_scale"], nthreads=cores)
+    order = np.random.default_rng(12345).permutation(rays["ox"].shape[0])
+
+    def prepare(n):
+        cur = W.rays_subset(rays, np.sort(order[:n]))
+        return cur, oracle_rays(cur, med)
+
+    def run(inp, nth):
+        cur, orr = inp
+        res = po.trace_ls(desc, orr, cur["length_scale"], nthreads=nth)
         hit = np.nonzero(res["outcome"] == po.HIT)[0]
         t = res["t"][hit].astype(np.float32)
         p = np.stack([cur["ox"][hit] + t * cur["dx"][hit], cur["oy"][hit] + t * cur["dy"][hit],
@@ -144,11 +173,10 @@ def cpu_baseline_1d(med, rays, light, max_seconds=12.0):
         d = np.stack([cur["dx"][hit], cur["dy"][hit], cur["dz"][hit]], 1)
         h = (wl - d) / np.linalg.norm(wl - d, axis=1, keepdims=True)
         po.nee_ls(desc, orr[hit], cur["length_scale"][hit], h.astype(np.float32), p, t)
-        spent += time.perf_counter() - t0
-        flights += len(idx)
-    return {"value": flights / spent, "unit": "flights/s", "cores": cores, "kind": "port",
-            "sample": f"{flights} conditioned 1D flights (+ NEE at each hit) of the same workload through "
-                      f"the C oracle, flights on {cores} host threads and NEE on 1, {spent:.1f} s"}
+        return len(orr)
+
+    return cpu_rates(prepare, run, len(order), "flights/s",
+                     "conditioned 1D flights with NEE at each hit (flights threaded, NEE on one thread)", n0=2048)
 
 
 def run_1d(args):
@@ -208,30 +236,25 @@ def run_1d(args):
     return line
 
 
-def cpu_baseline_paths_1d(med, rays, light, max_bounces, max_seconds=12.0):
-    """The oracle's path restatement on a bounded uniform sample of the same paths."""
+def cpu_baseline_paths_1d(med, rays, light, max_bounces):
+    """The oracle's path restatement on uniform samples of the same 1D paths."""
     from oracle import pyoracle as po
     from tests.helpers import oracle_desc, oracle_rays
     from gpis_extend_amd import workloads as W
 
     desc = oracle_desc(med)
-    cores = min(os.cpu_count() or 1, 64)
-    n_total = rays["ox"].shape[0]
-    order = np.random.default_rng(12345).permutation(n_total)
-    sample, paths, spent, start_idx = 8192, 0, 0.0, 0
-    while spent < max_seconds and start_idx < n_total:
-        idx = np.sort(order[start_idx:start_idx + sample])
-        start_idx += len(idx)
-        cur = W.rays_subset(rays, idx)
-        orr = oracle_rays(cur, med)
-        t0 = time.perf_counter()
-        po.trace_paths1d(desc, orr, cur["length_scale"], max_bounces, po.PHASE_MIRROR, 7, 1.0, light,
-                         nthreads=cores)
-        spent += time.perf_counter() - t0
-        paths += len(idx)
-    return {"value": paths / spent, "unit": "paths/s", "cores": cores, "kind": "port",
-            "sample": f"{paths} mirror paths of the same workload through the C oracle's path restatement "
-                      f"on {cores} host threads, {spent:.1f} s"}
+    order = np.random.default_rng(12345).permutation(rays["ox"].shape[0])
+
+    def prepare(n):
+        cur = W.rays_subset(rays, np.sort(order[:n]))
+        return cur, oracle_rays(cur, med)
+
+    def run(inp, nth):
+        cur, orr = inp
+        po.trace_paths1d(desc, orr, cur["length_scale"], max_bounces, po.PHASE_MIRROR, 7, 1.0, light, nthreads=nth)
+        return len(orr)
+
+    return cpu_rates(prepare, run, len(order), "paths/s", "1D mirror paths with light samples", n0=1024)
 
 
 def run_paths_1d(args):
@@ -295,6 +318,140 @@ def run_paths_1d(args):
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_paths_1d(med, rays, W.LIGHT_POS, max_bounces)
+    dev.close()
+    return line
+
+
+def _timed_steps(args, step, stream):
+    """Warm-up steps, then `args.steps` timed steps: (wall ms per step, mean HIP-event ms
+    per step on `stream`)."""
+    import torch
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        step()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall * 1e3 / args.steps, float(np.mean([a.elapsed_time(b) for a, b in zip(starts, ends)]))
+
+
+def run_c2(args):
+    """The nonstationary Renewal+ chains: 1,048,576 rays born on the box surface, each a
+    device-resident chain of up to 16 flights (gpis_trace_paths), every flight conditioned
+    on (0, the previous hit's gradient), per-flight seeds Vec4u(rayId, 0, 0, flight)."""
+    import torch
+    from gpis_extend_amd import GpisDevice, RayBatch, SparseConvolutionNoiseMedium, workloads as W
+
+    med = SparseConvolutionNoiseMedium.from_json(W.config2_json())
+    rays = W.box_surface_rays(med, n=args.rays_c2)
+    n = rays["ox"].shape[0]
+    dev = GpisDevice(med)
+    batch = RayBatch.from_numpy(rays)
+    seeds = dev.path_seeds(rays["pss"])
+    reply = dev.alloc_path_reply(n)
+    stream = torch.cuda.current_stream()
+    max_bounces = 15
+
+    def step():
+        dev.trace_paths(batch, max_bounces, 7, 1.0, reply, pss=seeds, scene_seed=int(rays["scene_seed"]))
+
+    tuned = None
+    if not args.no_autotune:
+        from gpis_extend_amd import autotune
+        tuned = autotune.tune(dev, "chains", n, step)
+    dev.reset_counters()
+    ms_per_step, kernel_ms = _timed_steps(args, step, stream)
+    c = dev.counters()
+    steps = args.steps + args.warmup  # the counters run from the reset, warm-up included
+    flights_per_step = c["flights"] / steps
+    evals_per_step = c["evals"] / steps
+    fpe = 2 * flops_per_eval_3d(int(med.impulse_density)) + 40 + 60
+    achieved = evals_per_step * fpe / (kernel_ms / 1e3) / 1e12
+    line = {
+        "metric": "GPIS flights/s on 1 GPU (nonstationary Renewal+ chains)",
+        "value": flights_per_step / (ms_per_step / 1e3), "unit": "flights/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "nonstationary multi-resolution Renewal+ chains: box SDF (half extent 0.6), "
+                               "procedural l(x), origins uniform on the box surface, up to 16 conditioned flights "
+                               "per ray, diffuse bounces inside [-1, 1]^3",
+                   "rays": n, "flights_per_step": flights_per_step, "evals_per_step": evals_per_step,
+                   "hits_per_step": c["hits"] / steps, "fails_per_step": c["fails"] / steps,
+                   "launch": {"refill_batch": tuned[0], "waves_per_cu": tuned[1], "block": tuned[2]} if tuned else None},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "flight_kernel<0,true,COND,PATHS>", "kernel_ms": kernel_ms, "flops_per_eval": fpe},
+    }
+    if not args.no_cpu_baseline:
+        from tests.helpers import oracle_field, oracle_rays
+        F = oracle_field(med)
+        order = np.random.default_rng(12345).permutation(n)
+
+        def prepare(k):
+            return oracle_rays(W.rays_subset(rays, np.sort(order[:k])), med)
+
+        def run(orr, nth):
+            return int(F.trace_paths3d(orr, max_bounces, 7, 1.0, seeded=True, nthreads=nth)["flights"].sum())
+
+        line["cpu_baseline"] = cpu_rates(prepare, run, n, "flights/s", "Renewal+ chains of up to 16 flights")
+    dev.close()
+    return line
+
+
+def run_points(args):
+    """Value + gradient of the stationary field (C0: SE l = 0.1, 8 impulses per cell,
+    single realization) at 2^20 points uniform in [-1, 1]^3: one gpis_eval_points launch."""
+    import torch
+    from gpis_extend_amd import GpisDevice, SparseConvolutionNoiseMedium, workloads as W
+
+    med = SparseConvolutionNoiseMedium.from_json(W.config0_json())
+    pts = W.points_workload(args.points)
+    n = pts.shape[0]
+    dev = GpisDevice(med)
+    x = torch.from_numpy(np.ascontiguousarray(pts[:, 0])).cuda()
+    y = torch.from_numpy(np.ascontiguousarray(pts[:, 1])).cuda()
+    z = torch.from_numpy(np.ascontiguousarray(pts[:, 2])).cuda()
+    stream = torch.cuda.current_stream()
+
+    def step():
+        dev.eval_points(x, y, z, med.seed)
+
+    ms_per_step, kernel_ms = _timed_steps(args, step, stream)
+    fpe = flops_per_eval_3d(int(med.impulse_density))
+    achieved = n * fpe / (kernel_ms / 1e3) / 1e12
+    line = {
+        "metric": "GPIS field evaluations/s on 1 GPU (value + gradient)",
+        "value": n / (ms_per_step / 1e3), "unit": "evaluations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "stationary SE field (sigma 1, l 0.1, 8 impulses per cell, single realization, "
+                               "seed 1): value and gradient at points uniform in [-1, 1]^3",
+                   "points": n},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "hbm_bytes_algorithmic": n * 28, "kernel": "eval_points_kernel<8>", "kernel_ms": kernel_ms,
+                     "flops_per_eval": fpe},
+    }
+    if not args.no_cpu_baseline:
+        from tests.helpers import oracle_field
+        F = oracle_field(med)
+
+        def prepare(k):
+            return np.ascontiguousarray(pts[:k])
+
+        def run(xyz, nth):
+            F.eval_points(xyz, med.seed, nthreads=nth)
+            return xyz.shape[0]
+
+        line["cpu_baseline"] = cpu_rates(prepare, run, n, "evaluations/s", "points of the same workload",
+                                         n0=4096)
     dev.close()
     return line
 
@@ -467,8 +624,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--width", type=int, default=2048)
-    ap.add_argument("--workload", choices=("c1", "c3", "c3paths", "c4", "twospheres"), default="c1",
-                    help="c1: camera-ray paths on the 3D field (default); c3: 1D Renewal+ flights with NEE; "
+    ap.add_argument("--workload", choices=("c1", "c2", "c0w1", "c3", "c3paths", "c4", "twospheres"), default="c1",
+                    help="c1: camera-ray paths on the 3D field (default); c2: nonstationary Renewal+ chains; "
+                         "c0w1: field value + gradient at 2^20 points; c3: 1D Renewal+ flights with NEE; "
                          "c3paths: device-resident 1D mirror paths with light samples; "
                          "c4: the learned surrogate against the plain method; "
                          "twospheres: primary flights through the two-spheres scene medium")
@@ -477,6 +635,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="report only the camera-ray paths workload (skip the 1D and surrogate legs)")
     ap.add_argument("--rays-1d", type=int, default=1 << 21)
+    ap.add_argument("--rays-c2", type=int, default=1 << 20)
+    ap.add_argument("--points", type=int, default=1 << 20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true",
@@ -490,8 +650,16 @@ def main():
               "(flights shard by ray index: one process per GPU over disjoint ray ranges)", file=sys.stderr)
         sys.exit(2)
 
+    if not args.no_cpu_baseline:
+        _oracle_build()  # before anything loads the oracle: the baselines time the host build
     if args.workload == "c3":
         print(json.dumps(run_1d(args)), flush=True)
+        return
+    if args.workload == "c2":
+        print(json.dumps(run_c2(args)), flush=True)
+        return
+    if args.workload == "c0w1":
+        print(json.dumps(run_points(args)), flush=True)
         return
     if args.workload == "c3paths":
         print(json.dumps(run_paths_1d(args)), flush=True)
@@ -585,12 +753,14 @@ def main():
         line["cpu_baseline"] = cpu_baseline(med, rays)
     dev.close()
     if not args.no_extras:
-        # the other two headline comparisons: plain 1D flights with NEE, and the learned
-        # surrogate against the plain method (its own timing, KS and image error)
+        # the other configurations, each with its own timing, roofline and CPU baseline:
+        # the nonstationary chains, the field at 2^20 points, plain 1D flights with NEE,
+        # 1D paths, and the learned surrogate against the plain method (KS, image error)
         keep = ("value", "unit", "ms_per_step", "config", "roofline", "cpu_baseline", "plain", "speedup_vs_plain",
                 "fallback_fraction", "ks_vs_plain", "pit_ks", "image_rel_error", "image_rel_error_plain_vs_plain",
                 "train", "metric", "dtype")
-        for key, fn in (("plain_1d", run_1d), ("paths_1d", run_paths_1d), ("surrogate", run_surrogate)):
+        for key, fn in (("chains_c2", run_c2), ("points_c0w1", run_points), ("plain_1d", run_1d),
+                        ("paths_1d", run_paths_1d), ("surrogate", run_surrogate)):
             sub = fn(args)
             line[key] = {k: v for k, v in sub.items() if k in keep}
     print(json.dumps(line), flush=True)

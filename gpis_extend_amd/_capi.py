@@ -78,7 +78,8 @@ class ReplySoA(C.Structure):
 
 
 class PathReply(C.Structure):
-    _fields_ = [(n, _vp) for n in ("t0", "gx0", "gy0", "gz0", "outcome0", "flights", "hits", "evals")]
+    _fields_ = [(n, _vp) for n in ("t0", "gx0", "gy0", "gz0", "outcome0", "flights", "hits", "evals", "t_last",
+                                   "outcome_last")]
 
 
 class PathSeeds(C.Structure):
@@ -136,8 +137,8 @@ def lib():
     L.gpis_trace_flights.restype = st
     L.gpis_trace_flights.argtypes = [_vp, C.POINTER(RaySoA), C.c_int64, C.POINTER(ReplySoA), _vp]
     L.gpis_trace_paths.restype = st
-    L.gpis_trace_paths.argtypes = [_vp, C.POINTER(RaySoA), C.c_int64, C.c_int32, C.c_uint32, C.c_float,
-                                   C.POINTER(PathReply), _vp]
+    L.gpis_trace_paths.argtypes = [_vp, C.POINTER(RaySoA), C.POINTER(PathSeeds), C.c_int64, C.c_int32, C.c_uint32,
+                                   C.c_float, C.POINTER(PathReply), _vp]
     L.gpis_trace_paths_1d.restype = st
     L.gpis_trace_paths_1d.argtypes = [_vp, C.POINTER(RaySoA), C.POINTER(PathSeeds), C.c_int64, C.c_int32,
                                       C.c_int32, C.c_uint32, C.c_float, C.POINTER(C.c_float),

@@ -85,6 +85,87 @@ __host__ __device__ __forceinline__ float u01(uint32_t i) {
     return c.f - 1.0f;
 }
 
+// ------------------------------------------------------------ bounces --
+// sin and cos of 2 pi u for u in [0, 1), in float operations that round the same on the
+// host and the device (no library calls, no contraction): the quadrant and octant of u by
+// exact reductions, then Taylor polynomials of sin and cos on [0, pi/4] (truncation error
+// below 2e-9, Horner form).  Used where a host restatement must reproduce the device bit
+// for bit (the diffuse bounce of the 3D paths).
+__host__ __device__ __forceinline__ void sincos_turn(float u, float &s, float &c) {
+#pragma clang fp contract(off)
+    const float x4 = u * 4.0f;                  // exact
+    const float qf = floorf(x4);
+    const int q = (int)qf & 3;
+    const float r = x4 - qf;                    // exact, in [0, 1): angle r * pi / 2 in the quadrant
+    const bool swap = r > 0.5f;
+    const float m = swap ? 1.0f - r : r;        // exact, in [0, 0.5]
+    const float h = m * 1.57079632679489662f;   // angle in [0, pi / 4]
+    const float h2 = h * h;
+    float ps = -2.50521083854417188e-8f;        // -1/11!
+    ps = ps * h2 + 2.75573192239858907e-6f;     //  1/9!
+    ps = ps * h2 - 1.98412698412698413e-4f;     // -1/7!
+    ps = ps * h2 + 8.33333333333333333e-3f;     //  1/5!
+    ps = ps * h2 - 1.66666666666666667e-1f;     // -1/3!
+    float sn = ps * h2;
+    sn = sn * h;
+    sn = sn + h;
+    float pc = 2.08767569878680990e-9f;         //  1/12!
+    pc = pc * h2 - 2.75573192239858907e-7f;     // -1/10!
+    pc = pc * h2 + 2.48015873015873016e-5f;     //  1/8!
+    pc = pc * h2 - 1.38888888888888889e-3f;     // -1/6!
+    pc = pc * h2 + 4.16666666666666667e-2f;     //  1/4!
+    pc = pc * h2 - 0.5f;
+    float cs = pc * h2;
+    cs = cs + 1.0f;
+    const float S = swap ? cs : sn, C = swap ? sn : cs;  // sin, cos of r * pi / 2
+    s = q == 0 ? S : (q == 1 ? C : (q == 2 ? -S : -C));
+    c = q == 0 ? C : (q == 1 ? -S : (q == 2 ? -C : S));
+}
+
+// The direction of a cosine-weighted diffuse bounce about the unit normal n (TangentFrame's
+// basis, TangentFrame.hpp:74-85) and the new flight's jitter: PCG32 seeded by
+// xxhash32(ray, bounce, bounce_seed, 0) + 1 gives u1, u2 and the jitter; local direction
+// (sqrt(u1) cos 2 pi u2, sqrt(u1) sin 2 pi u2, sqrt(1 - u1)).  Correctly rounded square roots
+// and divisions only, so the host's restatement gives the same bits.
+__host__ __device__ __forceinline__ void diffuse_direction(uint32_t ray, uint32_t bounce, uint32_t bounce_seed,
+                                                            float nx, float ny, float nz, float &dx, float &dy,
+                                                            float &dz, float &jit) {
+#pragma clang fp contract(off)
+    Pcg g;
+    g.set((uint64_t)(uint32_t)(xxh4(ray, bounce, bounce_seed, 0u) + 1u));
+    const float u1 = u01(g.next()), u2 = u01(g.next());
+    jit = u01(g.next());
+    const float rr = sqrtf(u1);
+    float sn, cs;
+    sincos_turn(u2, sn, cs);
+    const float lx = rr * cs, ly = rr * sn;
+    const float lz = sqrtf(fmaxf(0.f, 1.f - u1));
+    const float sgn = copysignf(1.0f, nz);
+    const float aa = -1.0f / (sgn + nz);
+    const float bb = (nx * ny) * aa;
+    const float tx = 1.0f + ((sgn * nx) * nx) * aa, ty = sgn * bb, tz = -sgn * nx;
+    const float bx = bb, by = sgn + (ny * ny) * aa, bz = -ny;
+    float x = tx * lx + bx * ly;
+    x = x + nx * lz;
+    float y = ty * lx + by * ly;
+    y = y + ny * lz;
+    float z = tz * lx + bz * ly;
+    z = z + nz * lz;
+    float l2 = x * x + y * y;
+    l2 = l2 + z * z;
+    const float inv = 1.0f / sqrtf(l2);
+    dx = x * inv; dy = y * inv; dz = z * inv;
+}
+
+// The unit normal of a hit from its gradient, as the bounce uses it.
+__host__ __device__ __forceinline__ void unit_normal(float gx, float gy, float gz, float &nx, float &ny, float &nz) {
+#pragma clang fp contract(off)
+    float l2 = gx * gx + gy * gy;
+    l2 = l2 + gz * gz;
+    const float inv = 1.0f / sqrtf(l2);
+    nx = gx * inv; ny = gy * inv; nz = gz * inv;
+}
+
 // ------------------------------------------------------------ parameters --
 struct FieldParams {
     // covariance
@@ -106,7 +187,7 @@ struct FieldParams {
     int N; float Nf;
     int iso;                  // isotropic kernel: exponent = c * r2
     int seed_add_single;      // floor(log(kss)/log(2.5)) for the single-level grid
-    int level_seed_add[64];   // the same for multi-resolution level L, indexed L + 32
+    int level_seed_add[65];   // the same for multi-resolution level L = -32 .. 32, indexed L + 32
     // single-level world-space constants (stationary SE)
     float R, invR;
     float c2[3];         // R^2 * m_i : exponent factor per axis in grid units

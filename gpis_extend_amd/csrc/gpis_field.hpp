@@ -467,12 +467,54 @@ __device__ __forceinline__ void cond_splat(const FieldParams &P, const RealState
     n.gz += (rs.a * d2 + bd * d2 - 2.f * m2 * rs.bz) * k;
 }
 
-// Procedural length-scale field of the nonstationary covariance.
+// Procedural length-scale field of the nonstationary covariance, evaluated in double at
+// the double-widened point and rounded once to float (the covariance's lateral scale
+// getKernelScale, GPFunctions.cpp:1734-1740, takes a Vec3d).
 __device__ __forceinline__ float ls_field(const FieldParams &P, float x, float y, float z) {
+#pragma clang fp contract(off)
     (void)y;
-    if (P.ls_field == 1)
-        return P.ls_params[0] + P.ls_params[1] * (0.5f + 0.5f * sinf(P.ls_params[2] * x) * cosf(P.ls_params[2] * z));
+    if (P.ls_field == 1) {
+        const double fr = P.ls_params[2];
+        return (float)((double)P.ls_params[0] +
+                       (double)P.ls_params[1] * (0.5 + 0.5 * sin(fr * (double)x) * cos(fr * (double)z)));
+    }
     return P.ls_params[0];
+}
+
+// Levels of the multi-resolution ladder of base 2.5 (kernelScaleLevelRatio,
+// SparseConvolutionNoise.cpp:101-111) at lateral scale lat:
+//   L = floor(log(lat) / log(2.5)), scales 2.5^L and 2.5^(L+1), weights
+//   (1 - fr) / sqrt(1 - 2 fr + 2 fr^2) and fr / sqrt(...), fr = (lat - 2.5^L) / (2.5^(L+1) - 2.5^L).
+// The scales are exact roundings of 2.5^L (5^L and 2^L are exact in double for the clamped
+// range, and the one division or power-of-two scaling rounds once), so every level's grid
+// and support radius are those of the float pow in the reference.  The float logarithm is
+// rounded from the double one: over every float lat in [2^-12, 2^12] its floor quotient
+// equals the one of the float logf.  L is clamped to [-32, 31], the seed-offset table.
+constexpr float kLogLevelBase = 0.91629076f;  // logf(2.5f)
+struct LevelRatio {
+    float slo, shi, wlo, whi;
+    int li;
+};
+__device__ __forceinline__ float pow_level_base(int L) {
+    const int a = L < 0 ? -L : L;
+    double v = 1.0;
+    for (int i = 0; i < a; ++i) v *= 5.0;
+    return L >= 0 ? (float)ldexp(v, -L) : (float)(ldexp(1.0, a) / v);
+}
+__device__ __forceinline__ LevelRatio level_ratio(float lat) {
+#pragma clang fp contract(off)
+    float lvl = floorf((float)log((double)lat) / kLogLevelBase);
+    lvl = fminf(fmaxf(lvl, -32.0f), 31.0f);
+    LevelRatio k;
+    k.li = (int)lvl;
+    k.slo = pow_level_base(k.li);
+    k.shi = pow_level_base(k.li + 1);
+    const float fr = (lat - k.slo) / (k.shi - k.slo);
+    const double fd = fr;
+    const double den = sqrt(1.0 - 2.0 * fd + 2.0 * fd * fd);
+    k.wlo = (float)((1.0 - fd) / den);
+    k.whi = (float)(fd / den);
+    return k;
 }
 
 // The isotropic ray space of a flight along (dx, dy, dz): the frame of
@@ -561,16 +603,9 @@ __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealStat
         return r;
     }
     // multi-resolution: two adjacent levels of the base-2.5 ladder (kernelScaleLevelRatio)
-    float lat = ls_field(P, px, py, pz);
-    float lvl = floorf(__logf(lat) / 0.91629073187415506f);
-    int li = (int)lvl;
-    float slo = __powf(2.5f, lvl), shi = __powf(2.5f, lvl + 1.0f);
-    float fr = (lat - slo) / (shi - slo);
-    float den = rsqrtf(1.0f - 2.0f * fr + 2.0f * fr * fr);
-    float wlo = (1.0f - fr) * den, whi = fr * den;
-    int ilo = li + 32, ihi = li + 33;
-    ilo = ilo < 0 ? 0 : (ilo > 63 ? 63 : ilo);
-    ihi = ihi < 0 ? 0 : (ihi > 63 ? 63 : ihi);
+    const LevelRatio lr = level_ratio(ls_field(P, px, py, pz));
+    const float slo = lr.slo, shi = lr.shi, wlo = lr.wlo, whi = lr.whi;
+    const int ilo = lr.li + 32, ihi = lr.li + 33;  // in [0, 64]: the table holds L = -32 .. 32
     Level A = make_level(P, slo, 1.0f, (uint32_t)P.level_seed_add[ilo]);
     Level B = make_level(P, shi, 1.0f, (uint32_t)P.level_seed_add[ihi]);
     Eval4 na = noise3d_raw<NI, ISO>(P, px, py, pz, rs.seed + A.seed_add, A.R, A.c0, A.c1, A.c2);

@@ -198,7 +198,7 @@ static gpis_status build_params(gpis_ctx *c) {
     // single-level grid (stationary SE: kss = 1)
     float kss = d.cov_kind == GPIS_COV_SE ? 1.0f : P.ls_max;
     P.seed_add_single = host_level_index(kss);
-    for (int L = -32; L < 32; ++L) P.level_seed_add[L + 32] = host_level_index(powf(2.5f, (float)L));
+    for (int L = -32; L <= 32; ++L) P.level_seed_add[L + 32] = host_level_index(powf(2.5f, (float)L));
     P.R = P.kernel_scale * 1.0f * P.l_max;
     P.invR = 1.0f / P.R;
     for (int i = 0; i < 3; ++i) {
@@ -395,7 +395,24 @@ struct ReplyOut {
 struct PathOut {
     float *t0, *gx0, *gy0, *gz0;
     uint32_t *outcome0, *flights, *hits, *evals;
+    float *t_last;             // last flight of the path: t and outcome (may be null)
+    uint32_t *outcome_last;
 };
+// Per-flight realization seeds of 3D paths: flight b of path r has pixelSampleSegment
+// (px, py, spp, w0 + b) (computeSeed, SparseConvolutionNoise.cpp:57-66); `on` = 0: every
+// flight of a path uses the ray's noise_seed.
+struct PathSeeds3D {
+    const uint32_t *px, *py, *spp, *w0;
+    uint32_t scene_hash;  // xxhash32(sceneSeed)
+    int on;
+};
+__device__ __forceinline__ uint32_t flight_seed3d(const FieldParams &P, const PathSeeds3D &ps, unsigned r,
+                                                  uint32_t w) {
+    const uint32_t a = ps.px[r], b = ps.py[r], s = ps.spp[r];
+    uint32_t noise = P.global_seed;
+    if (!P.single) noise += (P.ctxt == GPIS_CTX_GLOBAL ? xxh4(a, b, s, 0u) : xxh4(a, b, s, w)) + ps.scene_hash;
+    return noise;
+}
 
 __device__ __forceinline__ unsigned lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 __device__ __forceinline__ unsigned mask_rank(unsigned long long m) {
@@ -490,14 +507,10 @@ __device__ __noinline__ void condition_coefficients(const FieldParams &P, const 
     double wsum = 1.0, gsum = 1.0;
     if (P.multires) {
         // kernelScaleLevelRatio at the origin: value /= wlo + whi, gradient /= wlo/slo^2 + whi/shi^2
-        double lat = ls_field(P, ox, oy, oz);
-        double lvl = floor(log(lat) / log(2.5));
-        double slo = pow(2.5, lvl), shi = pow(2.5, lvl + 1.0);
-        double fr = (lat - slo) / (shi - slo);
-        double den = sqrt(1.0 - 2.0 * fr + 2.0 * fr * fr);
-        double wlo = (1.0 - fr) / den, whi = fr / den;
-        wsum = wlo + whi;
-        gsum = wlo / (slo * slo) + whi / (shi * shi);
+        // (the sums in float, as the weights and scales are floats)
+        const LevelRatio k = level_ratio(ls_field(P, ox, oy, oz));
+        wsum = k.wlo + k.whi;
+        gsum = k.wlo / (k.slo * k.slo) + k.whi / (k.shi * k.shi);
     }
     if (P.ctxt == GPIS_CTX_RENEWAL_PLUS) {
         double del0 = ((double)pgx - mgx) / amp - ((double)e.gx - mgx) / amp;
@@ -527,21 +540,7 @@ __device__ __noinline__ void condition_coefficients(const FieldParams &P, const 
 // Cosine-weighted diffuse direction about the unit normal n (TangentFrame basis).
 __device__ __noinline__ void diffuse_bounce(uint32_t ray, int bounce, uint32_t bounce_seed, float nx, float ny,
                                             float nz, float &dx, float &dy, float &dz, float &jit) {
-    Pcg g;
-    g.set((uint64_t)(uint32_t)(xxh4(ray, (uint32_t)bounce, bounce_seed, 0u) + 1u));
-    float u1 = u01(g.next()), u2 = u01(g.next());
-    jit = u01(g.next());
-    float rr = sqrtf(u1), ph = 6.2831853071795864f * u2;
-    float lx = rr * cosf(ph), ly = rr * sinf(ph), lz = sqrtf(fmaxf(0.f, 1.f - u1));
-    float sgn = copysignf(1.0f, nz);
-    float aa = -1.0f / (sgn + nz), bb = nx * ny * aa;
-    float tx = 1.0f + sgn * nx * nx * aa, ty = sgn * bb, tz = -sgn * nx;
-    float bx = bb, by = sgn + ny * ny * aa, bz = -ny;
-    dx = tx * lx + bx * ly + nx * lz;
-    dy = ty * lx + by * ly + ny * lz;
-    dz = tz * lx + bz * ly + nz * lz;
-    float inl = rsqrtf(dx * dx + dy * dy + dz * dz);
-    dx *= inl; dy *= inl; dz *= inl;
+    diffuse_direction(ray, (uint32_t)bounce, bounce_seed, nx, ny, nz, dx, dy, dz, jit);
 }
 
 // Block statistics kept in LDS and flushed once per block.
@@ -560,7 +559,8 @@ enum { ST_FLIGHTS = 0, ST_EVALS, ST_STEPS, ST_HITS, ST_EXITS, ST_FAILS, ST_INVAL
 // IRAY: the noise lives in each flight's isotropic ray space (isotropic_3D_sampling).
 template <int NI, bool ISO, bool COND, bool PATHS, bool GRID, bool IRAY = false>
 __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(const FieldParams *__restrict__ Pg, RayIn in, ReplyOut out,
-                                                                            PathOut pout, unsigned int n,
+                                                                            PathOut pout, PathSeeds3D pseeds,
+                                                                            unsigned int n,
                                                                             unsigned int *work,
                                                                             unsigned long long *stats,
                                                                             int max_bounces, uint32_t bounce_seed,
@@ -625,19 +625,23 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(cons
                         float hx = mul_add_exact(t_end, dx, ox);
                         float hy = mul_add_exact(t_end, dy, oy);
                         float hz = mul_add_exact(t_end, dz, oz);
-                        float il = rsqrtf(gxv * gxv + gyv * gyv + gzv * gzv);
-                        float jit;
-                        diffuse_bounce(ray, bounce, bounce_seed, gxv * il, gyv * il, gzv * il, dx, dy, dz, jit);
+                        float nx, ny, nz, jit;
+                        unit_normal(gxv, gyv, gzv, nx, ny, nz);
+                        diffuse_bounce(ray, bounce, bounce_seed, nx, ny, nz, dx, dy, dz, jit);
                         ox = hx; oy = hy; oz = hz;
                         near_t = 0.f;
                         far_t = box_exit(ox, oy, oz, dx, dy, dz, box);
                         prev_v = 0.f; prev_gx = gxv; prev_gy = gyv; prev_gz = gzv;
                         first = 0;
                         rs.cond = 0;
+                        if (pseeds.on) rs.seed = flight_seed3d(P, pseeds, ray, pseeds.w0[ray] + (uint32_t)bounce);
                         if (far_t <= near_t) {
+                            // an empty segment (the hit on the box's boundary): an exit at farT
                             atomicAdd(&bstat[ST_FLIGHTS], 1u);
                             atomicAdd(&bstat[ST_EXITS], 1u);
                             path_fh += 1u;
+                            t_end = far_t;
+                            outcome = GPIS_EXIT;
                         } else {
                             step = (far_t - near_t) / P.min_step_f;
                             if (P.step_size < step) step = P.step_size;
@@ -651,6 +655,10 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(cons
                         pout.flights[ray] = path_fh & 0xFFFFu;
                         pout.hits[ray] = path_fh >> 16;
                         if (pout.evals) pout.evals[ray] = path_evals;
+                        if (pout.t_last) {
+                            pout.t_last[ray] = t_end;
+                            pout.outcome_last[ray] = outcome;
+                        }
                     }
                 }
             }
@@ -673,7 +681,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(cons
                         dx = in.dx[r]; dy = in.dy[r]; dz = in.dz[r];
                         near_t = in.near_t[r]; far_t = in.far_t[r];
                         float jit = in.jitter[r];
-                        rs.seed = in.noise_seed[r];
+                        rs.seed = PATHS && pseeds.on ? flight_seed3d(P, pseeds, r, pseeds.w0[r]) : in.noise_seed[r];
                         rs.cond = 0;
                         first = (in.flags[r] & 1u) ? 1 : 0;
                         if (COND && !first) { prev_v = in.pv[r]; prev_gx = in.pgx[r]; prev_gy = in.pgy[r]; prev_gz = in.pgz[r]; }
@@ -692,6 +700,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(cons
                                 pout.gx0[r] = pout.gy0[r] = pout.gz0[r] = 0.f;
                                 pout.flights[r] = 0; pout.hits[r] = 0;
                                 if (pout.evals) pout.evals[r] = 0;
+                                if (pout.t_last) { pout.t_last[r] = far_t; pout.outcome_last[r] = GPIS_INVALID; }
                             }
                             atomicAdd(&bstat[ST_INVALID], 1u);
                             phase = PH_NEED;
@@ -707,6 +716,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(cons
                                 pout.gx0[r] = pout.gy0[r] = pout.gz0[r] = 0.f;
                                 pout.flights[r] = 0; pout.hits[r] = 0;
                                 if (pout.evals) pout.evals[r] = 0;
+                                if (pout.t_last) { pout.t_last[r] = far_t; pout.outcome_last[r] = GPIS_EXIT; }
                             }
                             atomicAdd(&bstat[ST_EXITS], 1u);
                             phase = PH_NEED;
@@ -813,6 +823,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(cons
                             if (out.evals) out.evals[ray] = evals_flight;
                         } else {
                             if (bounce == 0) { pout.t0[ray] = (float)tn; pout.outcome0[ray] = GPIS_INVALID; }
+                        if (pout.t_last) { pout.t_last[ray] = (float)tn; pout.outcome_last[ray] = GPIS_INVALID; }
                             pout.flights[ray] = path_fh & 0xFFFFu; pout.hits[ray] = path_fh >> 16;
                             if (pout.evals) pout.evals[ray] = path_evals + evals_flight;
                         }
@@ -1883,16 +1894,17 @@ __global__ __launch_bounds__(256, 2) void flight1d_group_kernel(const FieldParam
 
 template <bool PATHS>
 static void launch_flights(gpis_ctx *c, unsigned blocks, unsigned threads, hipStream_t st, RayIn in, ReplyOut out,
-                           PathOut pout, unsigned n, int max_bounces, uint32_t bounce_seed, float box) {
+                           PathOut pout, PathSeeds3D pseeds, unsigned n, int max_bounces, uint32_t bounce_seed,
+                           float box) {
 #define GPIS_FLIGHT_LAUNCH(NI, ISO, COND)                                                                          \
     do {                                                                                                           \
         if (grid)                                                                                                  \
             hipLaunchKernelGGL((flight_kernel<NI, ISO, COND, PATHS, true>), dim3(blocks), dim3(threads), 0, st,    \
-                               c->P_dev, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box,        \
+                               c->P_dev, in, out, pout, pseeds, n, c->work, c->counters, max_bounces, bounce_seed, box,        \
                                c->batch_k);                                                                        \
         else                                                                                                       \
             hipLaunchKernelGGL((flight_kernel<NI, ISO, COND, PATHS, false>), dim3(blocks), dim3(threads), 0, st,   \
-                               c->P_dev, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box,        \
+                               c->P_dev, in, out, pout, pseeds, n, c->work, c->counters, max_bounces, bounce_seed, box,        \
                                c->batch_k);                                                                        \
     } while (0)
     const bool cond = c->P.activate != 0;
@@ -1901,10 +1913,10 @@ static void launch_flights(gpis_ctx *c, unsigned blocks, unsigned threads, hipSt
         // the isotropic ray space: generic impulse count, isotropic kernel, analytic means
         if (cond)
             hipLaunchKernelGGL((flight_kernel<0, true, true, PATHS, false, true>), dim3(blocks), dim3(threads), 0, st,
-                               c->P_dev, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box, c->batch_k);
+                               c->P_dev, in, out, pout, pseeds, n, c->work, c->counters, max_bounces, bounce_seed, box, c->batch_k);
         else
             hipLaunchKernelGGL((flight_kernel<0, true, false, PATHS, false, true>), dim3(blocks), dim3(threads), 0, st,
-                               c->P_dev, in, out, pout, n, c->work, c->counters, max_bounces, bounce_seed, box, c->batch_k);
+                               c->P_dev, in, out, pout, pseeds, n, c->work, c->counters, max_bounces, bounce_seed, box, c->batch_k);
         return;
     }
     if (c->P.N == 8 && c->P.iso && !cond) GPIS_FLIGHT_LAUNCH(8, true, false);
@@ -2228,19 +2240,25 @@ gpis_status gpis_trace_flights(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n,
     launch_shape(c, n, blocks, threads);
     ReplyOut out = {reply->t, reply->value, reply->gx, reply->gy, reply->gz, reply->outcome, reply->evals};
     PathOut pout = {};
-    launch_flights<false>(c, blocks, threads, st, make_in(rays), out, pout, (unsigned)n, 0, 0u, 0.f);
+    PathSeeds3D ps = {};
+    launch_flights<false>(c, blocks, threads, st, make_in(rays), out, pout, ps, (unsigned)n, 0, 0u, 0.f);
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }
 
-gpis_status gpis_trace_paths(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, int32_t max_bounces,
-                             uint32_t bounce_seed, float box, const gpis_path_reply *reply, void *stream) {
+gpis_status gpis_trace_paths(gpis_ctx *c, const gpis_ray_soa *rays, const gpis_path_seeds *seeds, int64_t n,
+                             int32_t max_bounces, uint32_t bounce_seed, float box, const gpis_path_reply *reply,
+                             void *stream) {
     if (!c) return GPIS_INVALID_ARGUMENT;
     gpis_status s = check_rays(c, rays, n);
     if (s != GPIS_OK) return s;
     if (!reply || !reply->t0 || !reply->gx0 || !reply->gy0 || !reply->gz0 || !reply->outcome0 || !reply->flights ||
         !reply->hits)
         return fail(c, GPIS_INVALID_ARGUMENT, "null reply field");
+    if ((reply->t_last == nullptr) != (reply->outcome_last == nullptr))
+        return fail(c, GPIS_INVALID_ARGUMENT, "t_last and outcome_last are given together");
+    if (seeds && (!seeds->px || !seeds->py || !seeds->spp || !seeds->bounce))
+        return fail(c, GPIS_INVALID_ARGUMENT, "null seed field");
     if (max_bounces < 0 || max_bounces > 64) return fail(c, GPIS_INVALID_ARGUMENT, "max_bounces out of range");
     if (!(box > 0.f)) return fail(c, GPIS_INVALID_ARGUMENT, "box must be positive");
     if (n == 0) return GPIS_OK;
@@ -2250,8 +2268,11 @@ gpis_status gpis_trace_paths(gpis_ctx *c, const gpis_ray_soa *rays, int64_t n, i
     launch_shape(c, n, blocks, threads);
     ReplyOut out = {};
     PathOut pout = {reply->t0, reply->gx0, reply->gy0, reply->gz0, reply->outcome0, reply->flights, reply->hits,
-                    reply->evals};
-    launch_flights<true>(c, blocks, threads, st, make_in(rays), out, pout, (unsigned)n, max_bounces, bounce_seed, box);
+                    reply->evals, reply->t_last, reply->outcome_last};
+    PathSeeds3D ps = {};
+    if (seeds) ps = {seeds->px, seeds->py, seeds->spp, seeds->bounce, xxh1(seeds->scene_seed), 1};
+    launch_flights<true>(c, blocks, threads, st, make_in(rays), out, pout, ps, (unsigned)n, max_bounces, bounce_seed,
+                         box);
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }

@@ -533,17 +533,33 @@ class GpisDevice:
         i = dict(dtype=torch.int32, device=device)
         return {"t0": torch.empty(n, **f), "gx0": torch.empty(n, **f), "gy0": torch.empty(n, **f),
                 "gz0": torch.empty(n, **f), "outcome0": torch.empty(n, **i), "flights": torch.empty(n, **i),
-                "hits": torch.empty(n, **i), "evals": torch.empty(n, **i)}
+                "hits": torch.empty(n, **i), "evals": torch.empty(n, **i), "t_last": torch.empty(n, **f),
+                "outcome_last": torch.empty(n, **i)}
 
-    def trace_paths(self, rays: RayBatch, max_bounces: int, bounce_seed: int, box: float, reply=None):
+    def trace_paths(self, rays: RayBatch, max_bounces: int, bounce_seed: int, box: float, reply=None, pss=None,
+                    scene_seed: int = 0):
+        """Paths of up to 1 + max_bounces 3D flights with diffuse bounces (gpis_trace_paths).
+        `pss` (optional): each path's pixelSampleSegment, as an n x 4 uint32 array or the
+        device columns of path_seeds(); flight b then uses the realization seed of (px, py,
+        spp, w + b) and `scene_seed`.  Without it every flight uses the ray's noise_seed."""
         if reply is None:
             reply = self.alloc_path_reply(rays.n)
         r = A.PathReply()
-        for k in ("t0", "gx0", "gy0", "gz0", "outcome0", "flights", "hits", "evals"):
-            setattr(r, k, reply[k].data_ptr())
+        for k, _ in A.PathReply._fields_:
+            setattr(r, k, reply[k].data_ptr() if k in reply else None)
         soa = rays.soa()
-        A.check(self.L.gpis_trace_paths(self.h, C.byref(soa), rays.n, max_bounces, bounce_seed & 0xFFFFFFFF,
-                                        box, C.byref(r), self._stream()), self.h, "gpis_trace_paths")
+        seeds = None
+        if pss is not None:
+            dev = rays.t["ox"].device
+            cols = pss if isinstance(pss, (list, tuple)) else self.path_seeds(pss, dev)
+            if len(cols) != 4 or any(int(c.shape[0]) != rays.n for c in cols):
+                raise A.GpisError("pss needs four device columns of one entry per path")
+            seeds = A.PathSeeds(cols[0].data_ptr(), cols[1].data_ptr(), cols[2].data_ptr(), cols[3].data_ptr(),
+                                int(scene_seed) & 0xFFFFFFFF)
+            reply["_seeds"] = cols  # device inputs stay referenced until the caller synchronises
+        A.check(self.L.gpis_trace_paths(self.h, C.byref(soa), C.byref(seeds) if seeds is not None else None,
+                                        rays.n, max_bounces, bounce_seed & 0xFFFFFFFF, box, C.byref(r),
+                                        self._stream()), self.h, "gpis_trace_paths")
         return reply
 
     def counters(self) -> Dict[str, int]:

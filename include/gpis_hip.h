@@ -154,18 +154,28 @@ gpis_status gpis_trace_flights(gpis_ctx *ctx, const gpis_ray_soa *rays, int64_t 
 
 /* Trace n paths of up to 1 + max_bounces flights each in one persistent launch:
  * each hit continues as a cosine-weighted diffuse bounce about the normalised
- * gradient (secondary flights use the step-1 sign rule), bounded by the box
- * [-box, box]^3.  Per path: number of flights, flights per outcome, and the
- * first flight's reply.  Bounce directions use PCG32 seeded by
- * xxhash32(ray index, bounce, bounce_seed). */
+ * gradient (secondary flights use the step-1 sign rule; a conditioned medium conditions
+ * each on (0, the hit's gradient), prepareGpisSegment SparseConvolutionNoiseMedium.cpp:199-241),
+ * bounded by the box [-box, box]^3.  Per path: number of flights, flights per outcome,
+ * the first and the last flight's reply.  Bounce directions and jitters use PCG32
+ * seeded by xxhash32(ray index, bounce, bounce_seed, 0) + 1 (u1, u2, jitter; local
+ * direction (sqrt(u1) cos 2 pi u2, sqrt(u1) sin 2 pi u2, sqrt(1 - u1)) in TangentFrame's
+ * basis, with correctly rounded square roots and divisions and a fixed sin/cos polynomial,
+ * so a host restatement reproduces it).  seeds (may be NULL): flight b of path i uses
+ * the realization seed of pixelSampleSegment (px, py, spp, bounce + b) and scene_seed
+ * (computeSeed, SparseConvolutionNoise.cpp:57-66); NULL: every flight of a path uses
+ * rays->noise_seed. */
 typedef struct gpis_path_reply {
     float *t0, *gx0, *gy0, *gz0;      /* first flight: t and gradient */
     uint32_t *outcome0;
     uint32_t *flights;                /* flights traced for this path */
     uint32_t *hits;                   /* hit flights on this path */
     uint32_t *evals;                  /* field evaluations on this path (may be NULL) */
+    float *t_last;                    /* last flight: t and outcome (may be NULL, both or neither) */
+    uint32_t *outcome_last;
 } gpis_path_reply;
-gpis_status gpis_trace_paths(gpis_ctx *ctx, const gpis_ray_soa *rays, int64_t n,
+typedef struct gpis_path_seeds gpis_path_seeds;
+gpis_status gpis_trace_paths(gpis_ctx *ctx, const gpis_ray_soa *rays, const gpis_path_seeds *seeds, int64_t n,
                              int32_t max_bounces, uint32_t bounce_seed, float box,
                              const gpis_path_reply *reply, void *stream);
 
@@ -211,10 +221,10 @@ gpis_status gpis_trace_flights_1d(gpis_ctx *ctx, const gpis_ray_soa *rays, int64
  * of its phase sample and flight, stream 1 of flight b the jitter of its shadow flight.
  * SE kernel only; rays->length_scale may give each path its own length scale. */
 enum { GPIS_PHASE_MIRROR = 0, GPIS_PHASE_LAMBERT = 1 };
-typedef struct gpis_path_seeds {
+struct gpis_path_seeds {
     const uint32_t *px, *py, *spp, *bounce;  /* RayInfo::pixelSampleSegment of each path's first flight */
     uint32_t scene_seed;                     /* Scene::_seed */
-} gpis_path_seeds;
+};
 typedef struct gpis_path1d_reply {
     float *t0, *gx0, *gy0, *gz0;      /* first flight: t and gradient */
     uint32_t *outcome0;

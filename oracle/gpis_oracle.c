@@ -1712,3 +1712,157 @@ void orc_trace_paths1d(const orc_medium_desc *desc, const orc_ray *rays, const f
     }
     for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
 }
+
+/* ===================================================================== */
+/* 3D paths: flights chained through cosine-weighted diffuse bounces     */
+/* ===================================================================== */
+/* The harness's bounce rule of the device path kernel (the reference has no diffuse
+ * bounce on this path; the rule is restated operation for operation so that both give
+ * the same bits): sin and cos of 2 pi u from exact quadrant and octant reductions and
+ * Taylor polynomials on [0, pi/4] in Horner form; correctly rounded square roots and
+ * divisions only. */
+static void sincos_turn(float u, float *s, float *c) {
+    const float x4 = u * 4.0f;
+    const float qf = floorf(x4);
+    const int q = (int)qf & 3;
+    const float r = x4 - qf;
+    const int swap = r > 0.5f;
+    const float m = swap ? 1.0f - r : r;
+    const float h = m * 1.57079632679489662f;
+    const float h2 = h * h;
+    float ps = -2.50521083854417188e-8f;
+    ps = ps * h2 + 2.75573192239858907e-6f;
+    ps = ps * h2 - 1.98412698412698413e-4f;
+    ps = ps * h2 + 8.33333333333333333e-3f;
+    ps = ps * h2 - 1.66666666666666667e-1f;
+    float sn = ps * h2;
+    sn = sn * h;
+    sn = sn + h;
+    float pc = 2.08767569878680990e-9f;
+    pc = pc * h2 - 2.75573192239858907e-7f;
+    pc = pc * h2 + 2.48015873015873016e-5f;
+    pc = pc * h2 - 1.38888888888888889e-3f;
+    pc = pc * h2 + 4.16666666666666667e-2f;
+    pc = pc * h2 - 0.5f;
+    float cs = pc * h2;
+    cs = cs + 1.0f;
+    const float S = swap ? cs : sn, C = swap ? sn : cs;
+    *s = q == 0 ? S : (q == 1 ? C : (q == 2 ? -S : -C));
+    *c = q == 0 ? C : (q == 1 ? -S : (q == 2 ? -C : S));
+}
+void orc_sincos_turn(float u, float *s, float *c) { sincos_turn(u, s, c); }
+
+static v3f unit_normal(v3f g) {
+    float l2 = g.x * g.x + g.y * g.y;
+    l2 = l2 + g.z * g.z;
+    const float inv = 1.0f / sqrtf(l2);
+    return V3(g.x * inv, g.y * inv, g.z * inv);
+}
+static v3f diffuse_direction(uint32_t ray, uint32_t bounce, uint32_t bseed, v3f n, float *jit) {
+    pcg_t g;
+    pcg_set(&g, (uint64_t)(uint32_t)(orc_xxhash32_4(ray, bounce, bseed, 0u) + 1u));
+    const float u1 = pcg_1d(&g), u2 = pcg_1d(&g);
+    *jit = pcg_1d(&g);
+    const float rr = sqrtf(u1);
+    float sn, cs;
+    sincos_turn(u2, &sn, &cs);
+    const float lx = rr * cs, ly = rr * sn;
+    const float lz = sqrtf(fmaxf(0.f, 1.f - u1));
+    const float sgn = copysignf(1.0f, n.z);
+    const float aa = -1.0f / (sgn + n.z);
+    const float bb = (n.x * n.y) * aa;
+    const float tx = 1.0f + ((sgn * n.x) * n.x) * aa, ty = sgn * bb, tz = -sgn * n.x;
+    const float bx = bb, by = sgn + (n.y * n.y) * aa, bz = -n.y;
+    float x = tx * lx + bx * ly;
+    x = x + n.x * lz;
+    float y = ty * lx + by * ly;
+    y = y + n.y * lz;
+    float z = tz * lx + bz * ly;
+    z = z + n.z * lz;
+    float l2 = x * x + y * y;
+    l2 = l2 + z * z;
+    const float inv = 1.0f / sqrtf(l2);
+    return V3(x * inv, y * inv, z * inv);
+}
+void orc_diffuse_direction(uint32_t ray, uint32_t bounce, uint32_t bseed, const float n[3], float d[3],
+                           float *jit) {
+    const v3f r = diffuse_direction(ray, bounce, bseed, V3(n[0], n[1], n[2]), jit);
+    d[0] = r.x; d[1] = r.y; d[2] = r.z;
+}
+
+/* One path: the first flight is the request; every hit continues (up to max_bounces
+ * times) from sample.p = origin + t dir (float, each operation rounded) along a diffuse
+ * bounce about the unit gradient, to the exit of [-box, box]^3, conditioned on (0, the
+ * hit's gradient) with the step-1 sign rule (a secondary flight, prepareGpisSegment
+ * SparseConvolutionNoiseMedium.cpp:199-241); an empty segment ends the path as an exit
+ * at farT.  `seeded`: flight b uses pixelSampleSegment.w = w0 + b, else the request's. */
+static void trace_path3d(const orc_field *f, const orc_ray *ray0, uint32_t path, int max_bounces, uint32_t bseed,
+                         float box, int seeded, float perturb, uint32_t pseed, orc_path *out) {
+    memset(out, 0, sizeof *out);
+    orc_ray ray = *ray0;
+    for (int b = 0;; ++b) {
+        orc_flight fl;
+        trace_flight_impl(f, &ray, &fl, perturb, pseed ^ (path * 0x9E3779B9u) ^ ((uint32_t)b << 24));
+        out->flights += 1;
+        out->evals += fl.evals;
+        if (b == 0) {
+            out->t0 = (float)fl.t;
+            out->grad0[0] = fl.grad[0]; out->grad0[1] = fl.grad[1]; out->grad0[2] = fl.grad[2];
+            out->outcome0 = fl.outcome; out->fail0 = fl.fail_reason;
+        }
+        out->t_last = (float)fl.t;
+        out->outcome_last = fl.outcome; out->fail_last = fl.fail_reason;
+        out->pos_last[0] = ray.origin[0]; out->pos_last[1] = ray.origin[1]; out->pos_last[2] = ray.origin[2];
+        out->dir_last[0] = ray.dir[0]; out->dir_last[1] = ray.dir[1]; out->dir_last[2] = ray.dir[2];
+        if (fl.outcome != ORC_HIT) return;
+        out->hits += 1;
+        if (b >= max_bounces) return;
+        const float t = (float)fl.t;
+        const v3f o = V3(ray.origin[0], ray.origin[1], ray.origin[2]), d = V3(ray.dir[0], ray.dir[1], ray.dir[2]);
+        const v3f p = V3(o.x + t * d.x, o.y + t * d.y, o.z + t * d.z);
+        const v3f g = V3(fl.grad[0], fl.grad[1], fl.grad[2]);
+        float jit;
+        const v3f w = diffuse_direction(path, (uint32_t)(b + 1), bseed, unit_normal(g), &jit);
+        orc_ray nx = *ray0;
+        nx.origin[0] = p.x; nx.origin[1] = p.y; nx.origin[2] = p.z;
+        nx.dir[0] = w.x; nx.dir[1] = w.y; nx.dir[2] = w.z;
+        nx.near_t = 0.f;
+        nx.far_t = box_exit_f(p, w, box);
+        nx.jitter = jit;
+        nx.first_scatter = 0;
+        nx.last_val = 0.f;
+        nx.last_aniso[0] = g.x; nx.last_aniso[1] = g.y; nx.last_aniso[2] = g.z;
+        if (seeded) nx.pss[3] = ray0->pss[3] + (uint32_t)(b + 1);
+        if (nx.far_t <= nx.near_t) {
+            out->flights += 1;
+            out->t_last = nx.far_t;
+            out->outcome_last = ORC_EXIT; out->fail_last = ORC_FAIL_NONE;
+            return;
+        }
+        ray = nx;
+    }
+}
+typedef struct {
+    const orc_field *f; const orc_ray *rays; orc_path *out; int max_bounces; uint32_t bseed; float box;
+    int seeded; float perturb; uint32_t pseed; int64_t n0, n1;
+} path3d_job_t;
+static void *path3d_worker(void *arg) {
+    path3d_job_t *j = (path3d_job_t *)arg;
+    for (int64_t i = j->n0; i < j->n1; ++i)
+        trace_path3d(j->f, &j->rays[i], (uint32_t)i, j->max_bounces, j->bseed, j->box, j->seeded, j->perturb, j->pseed,
+                     &j->out[i]);
+    return NULL;
+}
+void orc_trace_paths3d(const orc_field *f, const orc_ray *rays, int64_t n, int32_t max_bounces, uint32_t bounce_seed,
+                       float box, int32_t seeded, float perturb, uint32_t pseed, orc_path *out, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256]; path3d_job_t jb[256];
+    for (int t = 0; t < nthreads; ++t) {
+        jb[t].f = f; jb[t].rays = rays; jb[t].out = out; jb[t].max_bounces = max_bounces; jb[t].bseed = bounce_seed;
+        jb[t].box = box; jb[t].seeded = seeded; jb[t].perturb = perturb; jb[t].pseed = pseed;
+        jb[t].n0 = n * t / nthreads; jb[t].n1 = n * (t + 1) / nthreads;
+        pthread_create(&th[t], NULL, path3d_worker, &jb[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+}

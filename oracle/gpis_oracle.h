@@ -215,6 +215,21 @@ void orc_trace_paths1d(const orc_medium_desc *desc, const orc_ray *rays, const f
                        int32_t max_bounces, int32_t phase, uint32_t bounce_seed, float box,
                        const float *light, orc_path *out, int nthreads);
 
+/* ---- 3D paths: flights chained through diffuse bounces ---------------- */
+/* The device path kernel's chaining (gpis_trace_paths): every hit continues along a
+ * cosine-weighted diffuse bounce about the unit gradient (PCG32 seeded by
+ * xxhash32(path, bounce, bounce_seed, 0) + 1 gives u1, u2 and the jitter) from the hit to
+ * the exit of [-box, box]^3, conditioned on (0, the hit's gradient); at most
+ * 1 + max_bounces flights.  seeded: flight b uses pixelSampleSegment.w = w0 + b.
+ * perturb (test hook, 0 = off): +-perturb with hashed signs (from pseed, the path and the
+ * flight) added to every marched value, as orc_trace_flight_perturbed. */
+void orc_trace_paths3d(const orc_field *f, const orc_ray *rays, int64_t n, int32_t max_bounces,
+                       uint32_t bounce_seed, float box, int32_t seeded, float perturb, uint32_t pseed,
+                       orc_path *out, int nthreads);
+void orc_sincos_turn(float u, float *s, float *c);
+void orc_diffuse_direction(uint32_t ray, uint32_t bounce, uint32_t bseed, const float n[3], float d[3],
+                           float *jit);
+
 /* Per-ray SE length scale (1D media whose rays each draw their own l): the same
  * flights and NEE as above, each ray on the field of `desc` with length_scale =
  * ls[i].  `normals`/`points`/`t_segment` are per ray (3, 3, 1 values). */

@@ -86,6 +86,15 @@ _lib = None
 _ref = None
 
 
+def use_library(path):
+    """Load the oracle from `path` (another build of the same source) instead of the default
+    build; only before the first call into the oracle."""
+    global LIB_PATH
+    if _lib is not None and os.path.abspath(path) != os.path.abspath(LIB_PATH):
+        raise RuntimeError("the oracle library is already loaded")
+    LIB_PATH = path
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -121,6 +130,10 @@ def lib():
         L.orc_trace_paths1d.argtypes = [vp, vp, vp, C.c_int64, C.c_int32, C.c_int32, u32, C.c_float, vp, vp,
                                         C.c_int]
         L.orc_nee_ls.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int64, vp, vp]
+        L.orc_trace_paths3d.argtypes = [vp, vp, C.c_int64, C.c_int32, u32, C.c_float, C.c_int32, C.c_float, u32, vp,
+                                        C.c_int]
+        L.orc_sincos_turn.argtypes = [C.c_float, f32p, f32p]
+        L.orc_diffuse_direction.argtypes = [u32, u32, u32, f32p, f32p, f32p]
         L.orc_conditioned_eval_ls.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp]
         L.orc_trace_flight_perturbed.argtypes = [vp, C.POINTER(Ray), C.c_float, C.c_uint32, C.POINTER(Flight)]
         L.orc_conditioned_eval.argtypes = [vp, C.POINTER(Ray), f32p, C.c_float, f32p]
@@ -321,6 +334,17 @@ class Field:
         lib().orc_trace_flights(self.h, rays.ctypes.data, rays.shape[0], out.ctypes.data, nthreads)
         return out
 
+    def trace_paths3d(self, rays: np.ndarray, max_bounces, bounce_seed, box, seeded=False, nthreads=1,
+                      perturb=0.0, pseed=0) -> np.ndarray:
+        """3D paths (orc_trace_paths3d): path i's bounces use path index i.  `perturb`:
+        +-perturb added to every marched value (signs hashed from pseed, path, flight)."""
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        out = np.zeros(rays.shape[0], PATH_DTYPE)
+        lib().orc_trace_paths3d(self.h, rays.ctypes.data, rays.shape[0], int(max_bounces),
+                                int(bounce_seed) & 0xFFFFFFFF, float(box), int(bool(seeded)), float(perturb),
+                                int(pseed) & 0xFFFFFFFF, out.ctypes.data, nthreads)
+        return out
+
     def trace_perturbed(self, ray_rec, eps, pseed):
         r = Ray.from_buffer_copy(np.ascontiguousarray(ray_rec, RAY_DTYPE).tobytes())
         out = np.zeros(1, FLIGHT_DTYPE)
@@ -377,6 +401,23 @@ def trace_paths1d(desc: MediumDesc, rays: np.ndarray, ls, max_bounces, phase, bo
                             rays.shape[0], int(max_bounces), int(phase), int(bounce_seed) & 0xFFFFFFFF, float(box),
                             None if la is None else la.ctypes.data, out.ctypes.data, nthreads)
     return out
+
+
+def sincos_turn(u: float):
+    """(sin, cos) of 2 pi u by the bounce's fixed polynomial."""
+    s, c = C.c_float(), C.c_float()
+    lib().orc_sincos_turn(float(u), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def diffuse_direction(ray, bounce, bounce_seed, normal):
+    """(direction, jitter) of the paths' diffuse bounce about a unit normal."""
+    n = np.ascontiguousarray(normal, np.float32)
+    d = np.zeros(3, np.float32)
+    j = C.c_float()
+    lib().orc_diffuse_direction(int(ray) & 0xFFFFFFFF, int(bounce) & 0xFFFFFFFF, int(bounce_seed) & 0xFFFFFFFF,
+                                _fp(n), _fp(d), C.byref(j))
+    return d, j.value
 
 
 def nee_ls(desc: MediumDesc, rays: np.ndarray, ls, normals, points, t_segment):

@@ -11,7 +11,9 @@
 //      sum of the in-support weights, which must be identical, and the gradient sums agree;
 //      with the SE exponent the value agrees to 2e-6 of the absolute sum;
 //   3. every list holds each impulse within reach of its sub-cell exactly once, in the
-//      reference's order, and nothing out of reach of the whole sub-cell.
+//      reference's order, and nothing out of reach of the whole sub-cell;
+//   4. the multi-resolution level index, level scales and weights of every float lateral
+//      scale in [2^-12, 2^12] are those of the float logf/powf formula.
 // Queries: uniform in the table's interior, near the origin planes (|p / R| < 1, where the
 // fixed-point query is rounded), on cell and sub-cell faces, at distance 1 +- 2^-22 grid
 // units from an impulse (the support radius), and up to four cells below the table's low
@@ -332,7 +334,44 @@ int main(int argc, char **argv) {
     }
     printf("pair_mismatch %lld of %lld\n", pair_mismatch, pair_evals);
     printf("list_missing %lld list_extra %lld checks %lld\n", list_missing, list_extra, g_checks);
-    const bool ok = table_mismatch == 0 && support_mismatch == 0 && pair_mismatch == 0 && pair_evals > 0 && value_bad == 0 && list_missing == 0 &&
+
+    // 4. multi-resolution levels: float lateral scales in [2^-12, 2^12] (every `lstep`-th
+    //    float, and every float within 2^14 of each level boundary 2.5^L) get the level
+    //    index and the scales 2.5^L, 2.5^(L+1) of the float logf/powf formula
+    //    (SparseConvolutionNoise.cpp:101-111), bit for bit, and its weights
+    long long level_bad = 0, weight_bad = 0, level_n = 0;
+    {
+        auto check_level = [&](uint32_t u) {
+            float lat;
+            memcpy(&lat, &u, 4);
+            const LevelRatio k = level_ratio(lat);
+            const float lo = floorf(logf(lat) / logf(2.5f)), slo = powf(2.5f, lo), shi = powf(2.5f, lo + 1.0f);
+            const float fr = (lat - slo) / (shi - slo);
+            const float wlo = (float)((1.0 - fr) / sqrt(1.0 - 2.0 * fr + 2.0 * fr * fr));
+            const float whi = (float)(fr / sqrt(1.0 - 2.0 * fr + 2.0 * fr * fr));
+            if (k.li != (int)lo || k.slo != slo || k.shi != shi) {
+                if (level_bad < 4) printf("level: lat %.9g device (%d, %.9g, %.9g) float formula (%d, %.9g, %.9g)\n",
+                                          lat, k.li, k.slo, k.shi, (int)lo, slo, shi);
+                ++level_bad;
+            }
+            if (k.wlo != wlo || k.whi != whi) ++weight_bad;
+            ++level_n;
+        };
+        const int lstep = argc > 2 ? atoi(argv[2]) : 1;
+        uint32_t u0, u1;
+        const float f0 = ldexpf(1.0f, -12), f1 = ldexpf(1.0f, 12);
+        memcpy(&u0, &f0, 4);
+        memcpy(&u1, &f1, 4);
+        for (uint32_t u = u0; u <= u1; u += (uint32_t)lstep) check_level(u);
+        for (int L = -9; L <= 9; ++L) {
+            const float b = powf(2.5f, (float)L);
+            uint32_t ub;
+            memcpy(&ub, &b, 4);
+            for (uint32_t u = ub - (1u << 14); u <= ub + (1u << 14); ++u) check_level(u);
+        }
+    }
+    printf("level_mismatch %lld weight_mismatch %lld of %lld\n", level_bad, weight_bad, level_n);
+    const bool ok = level_bad == 0 && weight_bad == 0 && table_mismatch == 0 && support_mismatch == 0 && pair_mismatch == 0 && pair_evals > 0 && value_bad == 0 && list_missing == 0 &&
                     list_extra == 0 && near * 100 < evals && near_radius * 10 > evals_radius * 9;
     printf("%s\n", ok ? "OK" : "FAIL");
     return ok ? 0 : 1;

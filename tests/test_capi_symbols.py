@@ -30,4 +30,4 @@ def test_struct_layouts_match_header_sizes():
     # field counts of the C structs mirrored in _capi
     assert ctypes.sizeof(_capi.RaySoA) == 17 * ctypes.sizeof(ctypes.c_void_p)
     assert ctypes.sizeof(_capi.ReplySoA) == 7 * ctypes.sizeof(ctypes.c_void_p)
-    assert ctypes.sizeof(_capi.PathReply) == 8 * ctypes.sizeof(ctypes.c_void_p)
+    assert ctypes.sizeof(_capi.PathReply) == 10 * ctypes.sizeof(ctypes.c_void_p)

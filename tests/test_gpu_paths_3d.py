@@ -1,0 +1,174 @@
+"""3D paths on the GPU (gpis_trace_paths: flights chained through cosine-weighted diffuse
+bounces, conditioned on the previous hit in Renewal+ media) against the oracle's
+restatement of the same chaining (orc_trace_paths3d), and the chained flights of the
+nonstationary Renewal+ workload generation by generation against the oracle on identical
+inputs (each generation built from the oracle's previous one)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.helpers import oracle_field, oracle_rays
+from tests.parity import parity_gate
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+NTH = min(16, os.cpu_count() or 1)
+BOUNCE_SEED = 7
+
+
+def _paths_gpu(med, rays, max_bounces, seeded, domain=None):
+    from gpis_extend_amd import GpisDevice, RayBatch
+    dev = GpisDevice(med)
+    if domain is not None:
+        dev.set_domain(*domain)
+    rb = RayBatch.from_numpy(rays)
+    rep = dev.trace_paths(rb, max_bounces, BOUNCE_SEED, 1.0, pss=rays["pss"] if seeded else None,
+                          scene_seed=int(rays["scene_seed"]) if seeded else 0)
+    torch.cuda.synchronize()
+    out = {k: v.cpu().numpy() for k, v in rep.items() if not k.startswith("_")}
+    dev.close()
+    return out
+
+
+def _path_agreement(got, ref, name):
+    """Fractions of paths with the same structure (flights, hits and last outcome) and of
+    paths that are identical (the structure, and the last distance within 1e-4 relative,
+    1e-6 absolute floor)."""
+    struct = (got["flights"] == ref["flights"]) & (got["hits"] == ref["hits"]) & \
+        ((got["outcome_last"] & 0xFF) == ref["outcome_last"])
+    dt = np.abs(got["t_last"].astype(np.float64) - ref["t_last"])
+    same = struct & (dt <= 1e-4 * np.abs(ref["t_last"]) + 1e-6)
+    first = (got["outcome0"] & 0xFF) == ref["outcome0"]
+    first &= np.abs(got["t0"].astype(np.float64) - ref["t0"]) <= 1e-4 * np.abs(ref["t0"]) + 1e-6
+    print(f"{name}: {len(same)} paths, {ref['flights'].sum()} oracle flights "
+          f"({ref['flights'].mean():.2f} per path, up to {ref['flights'].max()}); first flights identical "
+          f"{first.mean():.5f}; same structure {struct.mean():.5f}; identical {same.mean():.5f}; "
+          f"paths of another structure by the oracle's last outcome (exit, hit, fail): "
+          f"{[int((~struct & (ref['outcome_last'] == c)).sum()) for c in (0, 1, 2)]}")
+    for i in np.nonzero(~same)[0][:6]:
+        print(f"   path {i}: gpu flights {got['flights'][i]} hits {got['hits'][i]} last ({got['outcome_last'][i] & 0xFF}, "
+              f"{got['t_last'][i]:.6f}); oracle flights {ref['flights'][i]} hits {ref['hits'][i]} last "
+              f"({ref['outcome_last'][i]}, {ref['t_last'][i]:.6f})")
+    return float(struct.mean()), float(same.mean())
+
+
+def test_config1_paths_match_oracle():
+    """Camera rays of the primary-flight workload with two diffuse bounces: every path
+    flight by flight as the oracle chains it."""
+    from gpis_extend_amd import SparseConvolutionNoiseMedium, workloads as W
+    print("case: C1 16,384 camera paths, 2 bounces, GPU vs oracle", flush=True)
+    med = SparseConvolutionNoiseMedium.from_json(W.config1_json())
+    rays = W.camera_rays(med, width=512, height=512)
+    rays = W.rays_subset(rays, np.arange(0, rays["ox"].shape[0], 16))
+    got = _paths_gpu(med, rays, 2, False, ((-1.0,) * 3, (1.0,) * 3))
+    ref = oracle_field(med).trace_paths3d(oracle_rays(rays, med), 2, BOUNCE_SEED, 1.0, nthreads=NTH)
+    struct, same = _path_agreement(got, ref, "C1 paths")
+    # a path differs only after one of its flights differs; flights differ only at grazing
+    # crossings (about 1 in 10^4, test_gpu_parity.py), so three-flight paths agree >= 99.9%
+    assert struct >= 0.999 and same >= 0.999
+
+
+def test_config2_chained_paths_match_oracle():
+    """The nonstationary Renewal+ workload as device-resident chains: up to 16 flights per
+    ray, each conditioned on the previous hit's gradient, per-flight seeds
+    Vec4u(rayId, 0, 0, flight)."""
+    from gpis_extend_amd import SparseConvolutionNoiseMedium, workloads as W
+    print("case: C2 16,384 chained Renewal+ paths of up to 16 flights, GPU vs oracle", flush=True)
+    med = SparseConvolutionNoiseMedium.from_json(W.config2_json())
+    rays = W.box_surface_rays(med, n=16384)
+    got = _paths_gpu(med, rays, 15, True)
+    ref = oracle_field(med).trace_paths3d(oracle_rays(rays, med), 15, BOUNCE_SEED, 1.0, seeded=True, nthreads=NTH)
+    struct, same = _path_agreement(got, ref, "C2 chained paths")
+    assert np.all(got["flights"] >= 1)
+    # Surface-born conditioned flights start at f = 0, so a crossing within the first
+    # lattice step is decided by |f| ~ 1e-4 and its secant is ill-conditioned (every such
+    # disagreement is classified grazing in the generation test below).  Each hit's
+    # position seeds the next flight, so a chain amplifies rounding: the yardstick is the
+    # oracle against itself with +-1e-6 (below the measured 1.4e-6 FP32 evaluation error)
+    # added to every marched value.  The GPU must stay at least as close to the oracle.
+    alt = oracle_field(med).trace_paths3d(oracle_rays(rays, med), 15, BOUNCE_SEED, 1.0, seeded=True, nthreads=NTH,
+                                          perturb=1e-6, pseed=3)
+    alt_struct, alt_same = _path_agreement(alt, ref, "C2 chained paths, oracle +-1e-6 against oracle")
+    assert struct >= alt_struct - 0.005 and same >= alt_same - 0.01
+    assert struct >= 0.95
+
+
+def _unit_normal(g):
+    g = g.astype(np.float32)
+    l2 = g[:, 0] * g[:, 0] + g[:, 1] * g[:, 1]
+    l2 = l2 + g[:, 2] * g[:, 2]
+    inv = np.float32(1.0) / np.sqrt(l2)
+    return g * inv[:, None]
+
+
+def _box_exit(o, d, b=np.float32(1.0)):
+    o = o.astype(np.float32); d = d.astype(np.float32)
+    t = []
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for a in range(3):
+            ta = np.maximum((b - o[:, a]) / d[:, a], (-b - o[:, a]) / d[:, a])
+            t.append(np.where(d[:, a] != 0, ta, np.float32(3.4e38)).astype(np.float32))
+    return np.maximum(np.minimum(t[0], np.minimum(t[1], t[2])), np.float32(0)).astype(np.float32)
+
+
+def test_config2_chain_generation_by_generation():
+    """Each generation of the chained Renewal+ flights traced by the GPU and the oracle on
+    the same requests (built from the oracle's previous generation with the paths' bounce
+    rule): the north-star gate per generation."""
+    from gpis_extend_amd import GpisDevice, RayBatch, SparseConvolutionNoiseMedium, workloads as W
+    print("case: C2 chain, 16,384 rays, up to 16 generations, GPU vs oracle per generation", flush=True)
+    med = SparseConvolutionNoiseMedium.from_json(W.config2_json())
+    rays = W.box_surface_rays(med, n=16384)
+    scene = int(rays["scene_seed"])
+    F = oracle_field(med)
+    dev = GpisDevice(med)
+    cur = dict(rays)
+    ids = np.arange(rays["ox"].shape[0], dtype=np.uint32)
+    total, differ = 0, 0
+    for gen in range(16):
+        orr = oracle_rays(cur, med)
+        ref = F.trace(orr, nthreads=NTH)
+        rep = dev.trace_flights(RayBatch.from_numpy(cur))
+        torch.cuda.synchronize()
+        got = {k: v.cpu().numpy() for k, v in rep.items()}
+        # every disagreement grazing-classified (parity_gate); the rate, over all
+        # generations, is bounded below
+        bad = parity_gate(got, ref, f"C2 generation {gen}", min_agree=0.0, oracle=(F, orr))
+        total += len(bad)
+        differ += int(bad.sum())
+        hit = np.nonzero(ref["outcome"] == po.HIT)[0]
+        if gen == 15 or hit.size == 0:
+            break
+        t = ref["t"][hit].astype(np.float32)
+        o = np.stack([cur["ox"][hit], cur["oy"][hit], cur["oz"][hit]], 1).astype(np.float32)
+        d = np.stack([cur["dx"][hit], cur["dy"][hit], cur["dz"][hit]], 1).astype(np.float32)
+        p = (o + t[:, None] * d).astype(np.float32)
+        g = ref["grad"][hit].astype(np.float32)
+        nrm = _unit_normal(g)
+        nd = np.zeros_like(p)
+        jit = np.zeros(len(hit), np.float32)
+        for k in range(len(hit)):
+            nd[k], jit[k] = po.diffuse_direction(ids[hit[k]], gen + 1, BOUNCE_SEED, nrm[k])
+        far = _box_exit(p, nd)
+        keep = far > 0
+        sel = hit[keep]
+        nxt = W._empty_rays(int(keep.sum()))
+        nxt["ox"], nxt["oy"], nxt["oz"] = p[keep, 0].copy(), p[keep, 1].copy(), p[keep, 2].copy()
+        nxt["dx"], nxt["dy"], nxt["dz"] = nd[keep, 0].copy(), nd[keep, 1].copy(), nd[keep, 2].copy()
+        nxt["far_t"] = far[keep]
+        nxt["jitter"] = jit[keep]
+        nxt["flags"] = np.zeros(len(sel), np.uint32)
+        nxt["prev_value"] = np.zeros(len(sel), np.float32)
+        nxt["prev_gx"], nxt["prev_gy"], nxt["prev_gz"] = g[keep, 0].copy(), g[keep, 1].copy(), g[keep, 2].copy()
+        pss = cur["pss"][sel].copy()
+        pss[:, 3] = gen + 1
+        W._fill_seeds(nxt, med, pss, scene)
+        ids = ids[sel]
+        cur = nxt
+    dev.close()
+    print(f"C2 chain: {total} flights over {gen + 1} generations, {differ} differing (all grazing)")
+    assert total > 16384
+    # about 1e-3 of the surface-born flights cross within the first lattice step at |f| ~ 1e-4
+    assert differ <= 2e-3 * total
