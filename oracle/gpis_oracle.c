@@ -13,12 +13,33 @@
  *
  * Citations are file:line in the reference tree.
  */
-#include "gpis_oracle.h"
-
 #include <math.h>
 #include <pthread.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+#ifdef ORC_FP64_TRUTH
+/* The FP64 truth build (oracle/Makefile target fp64): the same algorithm with every float
+ * of the field arithmetic a double and every float libm call its double version, so that
+ * the only remaining rounding is double's.  What the reference defines as binary32 data
+ * keeps its value: the generator's draws (u01 below), float constants of the reference
+ * and the request/reply structs of the header (orc_f32).  Pointer arguments that the
+ * header declares as float are double in this build. */
+#define float double
+#define sqrtf sqrt
+#define floorf floor
+#define fmaxf fmax
+#define fminf fmin
+#define logf log
+#define powf pow
+#define sinf sin
+#define cosf cos
+#define copysignf copysign
+#define fabsf fabs
+#endif
+
+#include "gpis_oracle.h"
 
 #define PI_F 3.1415926536f /* src/core/math/Angle.hpp:8 (float) */
 
@@ -98,7 +119,7 @@ static inline uint32_t pcg_next(pcg_t *g) {
 }
 /* BitManip::normalizedUint (BitManip.hpp:47-50) */
 static inline float u01(uint32_t i) {
-    union { uint32_t u; float f; } c; c.u = (i >> 9u) | 0x3F800000u; return c.f - 1.0f;
+    union { uint32_t u; orc_f32 f; } c; c.u = (i >> 9u) | 0x3F800000u; return c.f - 1.0f;
 }
 static inline float pcg_1d(pcg_t *g) { return u01(pcg_next(g)); }
 /* set_state: state = seed, then next2D() discards two draws (UniformSampler.hpp:71-75) */
@@ -228,7 +249,7 @@ static int is_se(const orc_field *f) { return f->d.cov_kind == ORC_COV_SE; }
 
 /* ---- procedural nonstationary fields (D3: defined in code) ---------- */
 static double ns_ls_value(const orc_field *f, v3d p) {
-    const float *q = f->d.ns_ls_params;
+    const orc_f32 *q = f->d.ns_ls_params;
     switch (f->d.ns_ls_field) {
     case ORC_LS_SINCOS: {
         double fr = q[2];
@@ -416,7 +437,7 @@ static float fclampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), h
  * VdbGrid::density clamp to bounds+2 / bounds-3, VdbGrid.cpp:413 and :458-459), then
  * trilinear interpolation along x, then y, then z (RaySegmentBatch.cu:73-95). */
 float orc_grid_density(const orc_medium_desc *d, const float p[3]) {
-    const float *m = d->grid_w2i;
+    const orc_f32 *m = d->grid_w2i;
     float q[3];
     for (int r = 0; r < 3; ++r) q[r] = m[4 * r] * p[0] + m[4 * r + 1] * p[1] + m[4 * r + 2] * p[2] + m[4 * r + 3];
     for (int i = 0; i < 3; ++i) q[i] = fclampf(q[i], d->grid_clamp_min[i], d->grid_clamp_max[i]);
@@ -776,7 +797,7 @@ static int sdf_has_queries(int fn) {
 }
 /* ProceduralMean::mean without queries (GPFunctions.cpp:282-289) */
 static double sdf_mean_plain(const orc_medium_desc *d, const double a[3]) {
-    const float *t = d->sdf_inverse_transform;
+    const orc_f32 *t = d->sdf_inverse_transform;
     float p[3] = {(float)a[0], (float)a[1], (float)a[2]}, l[3], v;
     for (int i = 0; i < 3; ++i) l[i] = t[4 * i] * p[0] + t[4 * i + 1] * p[1] + t[4 * i + 2] * p[2] + t[4 * i + 3];
     v = orc_sdf_eval(d->sdf_function, l);
@@ -785,7 +806,7 @@ static double sdf_mean_plain(const orc_medium_desc *d, const double a[3]) {
     return (double)(d->sdf_min > v ? d->sdf_min : v);
 }
 double orc_sdf_mean(const orc_medium_desc *d, const double a[3], double g[3]) {
-    const float *t = d->sdf_inverse_transform;
+    const orc_f32 *t = d->sdf_inverse_transform;
     if (sdf_has_queries(d->sdf_function)) {
         /* ProceduralMean::evaluateJet (ProceduralMeanSdf.cpp:32-38); the 3x3 product in the
          * order Eigen evaluates it for doubles with SSE2 (rows 0-1 left to right, row 2 as

@@ -26,6 +26,11 @@
 
 #include <stdint.h>
 
+/* A binary32 float, also in the FP64 truth build of the oracle (gpis_oracle.c compiled
+ * with ORC_FP64_TRUTH, where the word float names double): the layout of every struct
+ * below is the same in both builds. */
+typedef __typeof__(1.0f) orc_f32;
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -57,39 +62,39 @@ enum { ORC_LS_CONST = 0, ORC_LS_SINCOS = 1 };
 typedef struct orc_medium_desc {
     /* covariance (SquaredExponentialCovariance JSON: sigma, lengthScale, localScale, aniso) */
     int32_t cov_kind;
-    float sigma, length_scale, kernel_scale;
-    float aniso[3];
+    orc_f32 sigma, length_scale, kernel_scale;
+    orc_f32 aniso[3];
     /* ProceduralNonstationaryCovariance: ls field, multiResolutionGrid */
     int32_t ns_multires;
     int32_t ns_ls_field;
-    float ns_ls_params[4];   /* SINCOS: base, amplitude, frequency (x0, x2) */
-    float ns_ls_max;         /* ProceduralVector::maxVal() of the ls field */
-    float ns_var;            /* constant variance field */
+    orc_f32 ns_ls_params[4];   /* SINCOS: base, amplitude, frequency (x0, x2) */
+    orc_f32 ns_ls_max;         /* ProceduralVector::maxVal() of the ls field */
+    orc_f32 ns_var;            /* constant variance field */
     /* mean */
     int32_t mean_kind;
-    float mean_radius;
+    orc_f32 mean_radius;
     double mean_center[3];
     double mean_half[3];     /* BOX half extents */
     double plane_normal[3];  /* PLANE: mean = dot(n, p) - offset */
     double plane_offset;
     /* SparseConvolutionNoiseMedium JSON */
     uint32_t global_seed;
-    float impulse_density;
+    orc_f32 impulse_density;
     int32_t single_realization;
     int32_t ctxt;
     int32_t one_d;
     int32_t scheme;
     int32_t correlation_xy;
-    float step_size;
+    orc_f32 step_size;
     uint32_t min_step;
     int32_t max_bounce;
     /* TabulatedMean over a dense voxel grid (the HostMeanGrid fields of
      * src/gpu/cuda/RaySegmentBatch.hpp:19-30); the voxels stay owned by the caller. */
-    const float *grid;          /* x fastest */
+    const orc_f32 *grid;          /* x fastest */
     int32_t grid_origin[3], grid_dims[3];
-    float grid_w2i[12];         /* row-major 3x4 world -> index */
-    float grid_clamp_min[3], grid_clamp_max[3];
-    float grid_offset, grid_scale;
+    orc_f32 grid_w2i[12];         /* row-major 3x4 world -> index */
+    orc_f32 grid_clamp_min[3], grid_clamp_max[3];
+    orc_f32 grid_offset, grid_scale;
     int32_t grid_volume;
     /* "isotropic_3D_sampling": 3D noise in the isotropic ray space of each flight
      * (evaluateNoise3DIsotropicRayNormalized, SparseConvolutionNoise.cpp:345-377) */
@@ -97,8 +102,8 @@ typedef struct orc_medium_desc {
     /* ProceduralMean over an SdfFunctions function (GPFunctions.cpp:250-298): "func",
      * rows 0-2 of _invConfigTransform (row-major), "min", "scale", "offset" */
     int32_t sdf_function;
-    float sdf_inverse_transform[12];
-    float sdf_min, sdf_scale, sdf_offset;
+    orc_f32 sdf_inverse_transform[12];
+    orc_f32 sdf_min, sdf_scale, sdf_offset;
 } orc_medium_desc;
 
 /* Dense-grid density at a world point (trilinear, clamped), and TabulatedMean::mean
@@ -119,12 +124,12 @@ double orc_sdf_mean(const orc_medium_desc *d, const double a[3], double g[3]);
 
 /* One flight request: the fields of GpRaySegmentRequest plus MediumState. */
 typedef struct orc_ray {
-    float origin[3], dir[3];
-    float near_t, far_t, jitter;
+    orc_f32 origin[3], dir[3];
+    orc_f32 near_t, far_t, jitter;
     uint32_t pss[4];          /* RayInfo::pixelSampleSegment */
     uint32_t scene_seed;
     int32_t first_scatter;
-    float last_val;           /* MediumState::lastVal */
+    orc_f32 last_val;           /* MediumState::lastVal */
     double last_aniso[3];     /* MediumState::lastAniso */
     int32_t bounce;
 } orc_ray;
@@ -135,15 +140,15 @@ enum { ORC_FAIL_NONE = 0, ORC_FAIL_OUTWARD = 1, ORC_FAIL_ZERO_GRAD = 2,
 
 typedef struct orc_flight {
     double t;            /* sampled distance (hit) or farT (exit) */
-    float value;         /* MediumState::lastVal after the flight */
-    float grad[3];       /* sample.aniso: gradient at the hit, or at farT on exit */
+    orc_f32 value;         /* MediumState::lastVal after the flight */
+    orc_f32 grad[3];       /* sample.aniso: gradient at the hit, or at farT on exit */
     int32_t outcome;     /* ORC_EXIT / ORC_HIT / ORC_FAIL */
     int32_t fail_reason;
     int32_t steps;       /* lattice steps taken */
     int32_t evals;       /* field evaluations performed (value or value+gradient) */
     int32_t scheme;      /* sparseConv1DSamplingScheme at the endpoint */
-    float min_abs_f;     /* smallest |f| of any evaluation whose sign decided the march */
-    float bracket[2];    /* f at the lattice points bracketing the crossing (pf, fc) */
+    orc_f32 min_abs_f;     /* smallest |f| of any evaluation whose sign decided the march */
+    orc_f32 bracket[2];    /* f at the lattice points bracketing the crossing (pf, fc) */
 } orc_flight;
 
 /* ---- field API ------------------------------------------------------- */
@@ -199,15 +204,15 @@ void orc_nee(const orc_field *f, const orc_ray *ray, const float normal[3],
  * stream 1 of flight b the jitter of its shadow flight. */
 enum { ORC_PHASE_MIRROR = 0, ORC_PHASE_LAMBERT = 1 };
 typedef struct orc_path {
-    float t0, grad0[3];       /* first flight */
+    orc_f32 t0, grad0[3];       /* first flight */
     int32_t outcome0, fail0;
     int32_t flights, hits;    /* path flights, hits among them */
     int32_t shadows, visible; /* shadow flights, unoccluded among them */
-    float nee;                /* sum over scattering events of f * V / |light - p|^2 */
-    float t_last;             /* last path flight */
+    orc_f32 nee;                /* sum over scattering events of f * V / |light - p|^2 */
+    orc_f32 t_last;             /* last path flight */
     int32_t outcome_last, fail_last;
     int32_t evals;            /* field evaluations, shadow flights included */
-    float pos_last[3], dir_last[3]; /* origin and direction of the last path flight */
+    orc_f32 pos_last[3], dir_last[3]; /* origin and direction of the last path flight */
 } orc_path;
 /* Up to 1 + max_bounces flights per path (PathTracer's maxBounces = 1 + max_bounces);
  * ls: optional per-path SE length scale; light: NULL for no next-event estimation. */

@@ -446,3 +446,79 @@ def conditioned_eval_ls(desc: MediumDesc, rays: np.ndarray, ls, q, t_segment):
     lib().orc_conditioned_eval_ls(C.byref(desc), rays.ctypes.data, None if lsa is None else lsa.ctypes.data,
                                   qq.ctypes.data, ts.ctypes.data, n, out.ctypes.data)
     return out
+
+
+# ---- FP64 truth build (gpis_oracle.c with ORC_FP64_TRUTH) ----------------
+LIB64_PATH = os.path.join(HERE, "build", "libgpis_oracle_fp64.so")
+_lib64 = None
+
+
+def lib64():
+    """The FP64 truth build: the same algorithm with every float of the field arithmetic
+    in double; the request/reply structs are those of the float build, the pointer
+    arguments that the float build takes as float are double here."""
+    global _lib64
+    if _lib64 is None:
+        if not os.path.exists(LIB64_PATH):
+            raise RuntimeError(f"FP64 oracle not built: {LIB64_PATH} (run make -C oracle)")
+        L = C.CDLL(LIB64_PATH)
+        vp, u32 = C.c_void_p, C.c_uint32
+        L.orc_field_create.restype = vp
+        L.orc_field_create.argtypes = [C.POINTER(MediumDesc)]
+        L.orc_field_destroy.argtypes = [vp]
+        L.orc_eval_points.argtypes = [vp, vp, C.c_int64, u32, vp, C.c_int]
+        L.orc_trace_flights.argtypes = [vp, vp, C.c_int64, vp, C.c_int]
+        L.orc_trace_flights_ls.argtypes = [vp, vp, vp, C.c_int64, vp, C.c_int]
+        L.orc_conditioned_eval_ls.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp]
+        _lib64 = L
+    return _lib64
+
+
+class Truth64:
+    """A field of the FP64 truth build: value and gradient in double, flights whose
+    distance t is the double march's (value and gradient of the reply rounded to float)."""
+
+    def __init__(self, desc: MediumDesc):
+        self.desc = desc
+        self.h = lib64().orc_field_create(C.byref(desc))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib64().orc_field_destroy(self.h)
+            self.h = None
+
+    def eval_points(self, xyz, seed, nthreads=1):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        out = np.zeros((xyz.shape[0], 4), np.float64)
+        lib64().orc_eval_points(self.h, xyz.ctypes.data, xyz.shape[0], int(seed) & 0xFFFFFFFF, out.ctypes.data,
+                                nthreads)
+        return out
+
+    def trace(self, rays: np.ndarray, nthreads=1) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        out = np.zeros(rays.shape[0], FLIGHT_DTYPE)
+        lib64().orc_trace_flights(self.h, rays.ctypes.data, rays.shape[0], out.ctypes.data, nthreads)
+        return out
+
+
+def trace_ls64(desc: MediumDesc, rays: np.ndarray, ls, nthreads=1) -> np.ndarray:
+    """trace_ls in the FP64 truth build."""
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    out = np.zeros(rays.shape[0], FLIGHT_DTYPE)
+    lsa = None if ls is None else np.ascontiguousarray(ls, np.float64)
+    lib64().orc_trace_flights_ls(C.byref(desc), rays.ctypes.data, None if lsa is None else lsa.ctypes.data,
+                                 rays.shape[0], out.ctypes.data, nthreads)
+    return out
+
+
+def conditioned_eval_ls64(desc: MediumDesc, rays: np.ndarray, ls, q, t_segment):
+    """conditioned_eval_ls in the FP64 truth build (double in and out)."""
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    n = rays.shape[0]
+    lsa = None if ls is None else np.ascontiguousarray(ls, np.float64)
+    qq = np.ascontiguousarray(q, np.float64).reshape(n, 3)
+    ts = np.ascontiguousarray(t_segment, np.float64).reshape(n)
+    out = np.zeros((n, 4), np.float64)
+    lib64().orc_conditioned_eval_ls(C.byref(desc), rays.ctypes.data, None if lsa is None else lsa.ctypes.data,
+                                    qq.ctypes.data, ts.ctypes.data, n, out.ctypes.data)
+    return out

@@ -92,3 +92,30 @@ def test_unit_variance_under_a_varying_length_scale(multires):
         m = int(sel.sum())
         assert m > 500
         assert abs(v[sel].var() - 1.0) < 8 * math.sqrt(2.0 / m) + 0.04, (lo, hi, v[sel].var())
+
+
+@pytest.mark.parametrize("lvl", [-3, -2])
+def test_multires_levels_are_independent(lvl):
+    """A constant length scale at a rung of the ladder, c = 2.5^L, gives weights (1, 0): the
+    field is level L alone; c = 2.5^(L+1) gives level L+1 alone, with that level's own seed
+    offset.  Both media share the global seed, so they expose the two levels of one
+    realization: their cross-covariance at equal and nearby points must vanish (z-test)."""
+    c_lo = float(np.float32(2.5) ** np.float32(lvl))
+    c_hi = float(np.float32(2.5) ** np.float32(lvl + 1))
+    A = po.Field(_ns_desc(c_lo, 0.0, True, c_lo))
+    B = po.Field(_ns_desc(c_hi, 0.0, True, c_hi))
+    rng = np.random.default_rng(11 + lvl)
+    # a box of 80 coarse length scales: the samples are nearly independent
+    x = rng.uniform(-40.0 * c_hi, 40.0 * c_hi, (N, 3))
+    va = _values(A, x, 5)
+    vb = _values(B, x, 5)
+    # each is a unit-variance level on its own
+    assert abs(va.var() - 1.0) < 0.03 and abs(vb.var() - 1.0) < 0.03
+    for lag in (0.0, 0.5 * c_lo, c_lo):
+        u = rng.normal(size=(N, 3))
+        u = u / np.linalg.norm(u, axis=1, keepdims=True) * lag
+        vb2 = _values(B, x + u, 5) if lag > 0 else vb
+        rho = float(np.mean(va * vb2))
+        z = rho * math.sqrt(N)
+        print(f"level {lvl} x level {lvl + 1}, lag {lag:.3f}: covariance {rho:+.4f} (z = {z:+.2f})")
+        assert abs(z) < 5.0, (lag, rho)
