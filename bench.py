@@ -213,6 +213,18 @@ def run_1d(args):
     flights_per_step = c["flights"] / args.steps
     evals_per_step = c["evals"] / args.steps
     ms_per_step = wall * 1e3 / args.steps
+    # the reference-rounding mode on the same rays, for comparison (not the headline)
+    dev.set_precision("exact")
+    dev.trace_flights_1d(batch, light=W.LIGHT_POS, reply=reply)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(3):
+        dev.trace_flights_1d(batch, light=W.LIGHT_POS, reply=reply)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    exact_ms = e0.elapsed_time(e1) / 3
+    dev.set_precision("fast")
     fpe = flops_per_eval_1d(int(med.impulse_density))
     achieved = evals_per_step * fpe / (kernel_ms / 1e3) / 1e12
     line = {
@@ -224,11 +236,14 @@ def run_1d(args):
                                "sphere SDF mean, v0 ~ N(0.3, 0.2^2), directional derivative ~ N(0, 1), "
                                "NEE toward a point light at (0, 2, 0)",
                    "rays": n, "flights_per_step": flights_per_step, "evals_per_step": evals_per_step,
-                   "hits_per_step": c["hits"] / args.steps, "fails_per_step": c["fails"] / args.steps},
+                   "hits_per_step": c["hits"] / args.steps, "fails_per_step": c["fails"] / args.steps,
+                   "precision": "fast (FP32 splats and mean)",
+                   "exact_rounding_mode": {"kernel_ms": exact_ms,
+                                           "flights_per_s": flights_per_step / (exact_ms / 1e3)}},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "frac_scalar_issue_peak": achieved / PEAK_FP32_SCALAR_TFLOPS,
-                     "kernel": "flight1d_kernel<8>", "kernel_ms": kernel_ms, "flops_per_eval": fpe},
+                     "kernel": "flight1d_kernel<8,fast>", "kernel_ms": kernel_ms, "flops_per_eval": fpe},
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_1d(med, rays, W.LIGHT_POS)

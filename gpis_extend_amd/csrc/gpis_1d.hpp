@@ -88,6 +88,21 @@ __device__ __forceinline__ double mean_value_d(const FieldParams &P, float x, fl
     s += p[2] * p[2];
     return sqrt(s) - P.mrd;
 }
+// The analytic means in FP32 (sphere, box, plane); the tabulated and procedural means keep
+// their double evaluation, rounded once.
+template <bool PROC = true>
+__device__ __forceinline__ float mean_value_f(const FieldParams &P, float x, float y, float z) {
+    if (P.mean_kind == 3 || P.mean_kind == 4) return (float)mean_value_d<PROC>(P, x, y, z);
+    const float px = x - (float)P.mcd[0], py = y - (float)P.mcd[1], pz = z - (float)P.mcd[2];
+    if (P.mean_kind == 1) {
+        const float qx = fabsf(px) - (float)P.mhd[0], qy = fabsf(py) - (float)P.mhd[1], qz = fabsf(pz) - (float)P.mhd[2];
+        const float cx = fmaxf(qx, 0.f), cy = fmaxf(qy, 0.f), cz = fmaxf(qz, 0.f);
+        return sqrtf(cx * cx + cy * cy + cz * cz) + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.f);
+    }
+    if (P.mean_kind == 2)
+        return (float)P.pnd[0] * x + (float)P.pnd[1] * y + (float)P.pnd[2] * z - (float)P.poffd;
+    return sqrtf(px * px + py * py + pz * pz) - (float)P.mrd;
+}
 template <bool PROC = true>
 __device__ __forceinline__ void mean_grad_d(const FieldParams &P, float x, float y, float z, float &gx, float &gy,
                                             float &gz) {
@@ -201,10 +216,16 @@ __device__ __forceinline__ void cells_for(Cells1D<NI> &C, int i, uint32_t seed, 
 // kernels compute it ((float)exp((double)x), GPFunctions.cpp:845-875).
 __device__ __forceinline__ float exp_dr(float x) { return (float)exp((double)x); }
 
+// exp(-x^2 / 2) in FP32 by the hardware exponential: exp2(-x^2 log2(e) / 2).
+constexpr float kNegHalfLog2e = -0.72134752044448170f;
+__device__ __forceinline__ float gauss_f32(float x) { return hw_exp2(kNegHalfLog2e * (x * x)); }
+
 // noise1D (:454-485) at coordinate u with kernel radius R = 3: the raw sums of w*h and
-// w*dh/du, bit-identical to the reference's arithmetic (per-cell partial sums added
-// cell by cell, splat products rounded as written, exponentials rounded from double).
-template <int NI>
+// w*dh/du.  FAST = false: bit-identical to the reference's arithmetic (per-cell partial
+// sums added cell by cell, splat products rounded as written, exponentials rounded from
+// double).  FAST = true: the same support decisions and summation order, the splat by
+// the FP32 hardware exponential (a few ulp).
+template <int NI, bool FAST = false>
 __device__ __forceinline__ void noise1d_raw(Cells1D<NI> &C, bool &cache_valid, float u, uint32_t seed,
                                             float &v, float &dv) {
 #pragma clang fp contract(off)
@@ -229,7 +250,7 @@ __device__ __forceinline__ void noise1d_raw(Cells1D<NI> &C, bool &cache_valid, f
             if (to * to < 1.0f) {
                 const float pc = R * ti;
                 const float ab = pq - pc;                // splattingKernel1D: pQuery - pCenter
-                const float e = (float)exp_splat_d((double)(-(ab * ab / 2.0f)));  // exp(-ab^2 / (2 * 1^2))
+                const float e = FAST ? gauss_f32(ab) : (float)exp_splat_d((double)(-(ab * ab / 2.0f)));
                 const float g = -ab * e;                 // (-2 ab / 2) * e, exact in double then rounded
                 const float w = copysignf(1.0f, ts);
                 cv += w * e;
@@ -246,9 +267,19 @@ __device__ __forceinline__ void noise1d_raw(Cells1D<NI> &C, bool &cache_valid, f
 // Covariance splats of the 1D conditioning (covarianceKernel1D / 1DGrad with the
 // stationary kernel of length sqrt(2), GPFunctions.cpp:590-605, 845-875), rounded as
 // the reference: value v1 * kappa + g1z * kappa', derivative v1 * kappa' + g1z * kappa''.
+template <bool FAST = false>
 __device__ __forceinline__ void cond_splat1d(const Ray1D &R, float u, float &v, float &dv) {
 #pragma clang fp contract(off)
     const float d = u - R.uo;
+    if (FAST) {
+        // kappa = exp(-d^2 / 4), kappa' = -d/2 kappa, kappa'' = (d^2/4 - 1/2) kappa in FP32
+        const float k = hw_exp2(-0.36067376022224085f * (d * d));
+        const float h = -0.5f * d;
+        const float k1 = h * k, k2 = (h * h - 0.5f) * k;
+        v += R.v1 * k + R.g1z * k1;
+        dv += R.v1 * k1 + R.g1z * k2;
+        return;
+    }
     const float den2 = 3.9999998f;                     // (float)(2 * sqr((float)(1 * sqrt(2))))
     const float k = exp_dr(-(d * d / den2));
     const float k1 = (float)((-2.0 * (double)d) / (double)den2 * (double)k);

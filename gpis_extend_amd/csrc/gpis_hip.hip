@@ -47,6 +47,7 @@ struct gpis_ctx {
     int batch_k = 8;  // finish/bounce/refill batch of the 3D kernel (lanes per wave)
     int batch_k_1d = 16;  // ... and of the 1D kernel
     float nf1d = 1.0f;  // sqrt(sparseConvNoiseVariance1D) of the 1D grid
+    int fast1d = 1;     // 1D arithmetic: FP32 fast path (1) or the reference's rounding (0)
     float *grid = nullptr;  // device voxels of a tabulated mean
     sdf::MeanParams sdf_host;             // a procedural mean's parameters ...
     sdf::MeanParams *sdf_dev = nullptr;   // ... and their device copy
@@ -909,21 +910,29 @@ struct Nee1DOut {
 
 // Per-lane 1D field evaluation at q: normalised noise value and derivative along the
 // ray (conditioned when `cond`), then f = (float)(amplitude * noise + mean in double),
-// the reference's evaluateValue rounding.
-template <int NI, bool PROC>
+// the reference's evaluateValue rounding.  FAST: FP32 splats, conditioning splats and
+// analytic mean (the normalisation as a multiplication by 1/nf).
+template <int NI, bool PROC, bool FAST = false>
 __device__ __forceinline__ void eval1d(const FieldParams &P, const Ray1D &R, Cells1D<NI> &C, bool &cache_valid,
                                        float nf, float qx, float qy, float qz, bool cond, float &f, float &nv,
                                        float &ndv) {
 #pragma clang fp contract(off)
     float u = ray_coord(R, qx, qy, qz);
     float v, dv;
-    noise1d_raw<NI>(C, cache_valid, u, R.seed, v, dv);
-    v = v / nf;
-    dv = dv / nf;
-    if (cond && R.cond) cond_splat1d(R, u, v, dv);
+    noise1d_raw<NI, FAST>(C, cache_valid, u, R.seed, v, dv);
+    if (FAST) {
+        const float inf = 1.0f / nf;
+        v *= inf;
+        dv *= inf;
+    } else {
+        v = v / nf;
+        dv = dv / nf;
+    }
+    if (cond && R.cond) cond_splat1d<FAST>(R, u, v, dv);
     nv = v;
     ndv = dv;
-    f = (float)((double)(P.amp_const * v) + mean_value_d<PROC>(P, qx, qy, qz));
+    if (FAST) f = P.amp_const * v + mean_value_f<PROC>(P, qx, qy, qz);
+    else f = (float)((double)(P.amp_const * v) + mean_value_d<PROC>(P, qx, qy, qz));
 }
 
 // neeShared + neeGrad + neePDF (SparseConvolutionNoise.cpp:662-821) for the single-level
@@ -1149,7 +1158,7 @@ __device__ __forceinline__ uint32_t finish1d(const FieldParams &P, const Ray1D &
 // hit-time phase runs in the batched finish step (the light sample with its shadow
 // flight, then the mirror or Lambert bounce) and the next flight starts from the hit,
 // conditioned on (0, gradient), without leaving the kernel.
-template <int NI, bool PATHS, bool PROC>
+template <int NI, bool PATHS, bool PROC, bool FAST>
 __global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(const FieldParams *__restrict__ Pg, RayIn in, const float *__restrict__ lscale,
                                                           ReplyOut out, unsigned int n, unsigned int *work,
                                                           unsigned long long *stats, float nf, int use_light,
@@ -1530,7 +1539,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(
         if (phase == PH_NEED || phase == PH_DEAD || phase == PH_DONE) continue;
 
         float fv, nv, ndv;
-        eval1d<NI, PROC>(P, R, C, cache_valid, nf, qx, qy, qz, phase != PH_COND, fv, nv, ndv);
+        eval1d<NI, PROC, FAST>(P, R, C, cache_valid, nf, qx, qy, qz, phase != PH_COND, fv, nv, ndv);
         ++evals_flight;
         double tq = 0.0;
         bool unit = false;
@@ -1651,7 +1660,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(
 // rounds: a launch over a few thousand rays no longer waits ~steps rounds for its longest
 // flight.  Ray indices come from idx[0 .. *n_dev).
 constexpr int kGroup = 16;
-template <int NI, bool PROC>
+template <int NI, bool PROC, bool FAST>
 __global__ __launch_bounds__(256, 2) void flight1d_group_kernel(const FieldParams *__restrict__ Pg, RayIn in,
                                                                 const float *__restrict__ lscale, ReplyOut out,
                                                                 unsigned int *work, unsigned long long *stats,
@@ -1770,7 +1779,7 @@ __global__ __launch_bounds__(256, 2) void flight1d_group_kernel(const FieldParam
             point_at(ox, oy, oz, dx, dy, dz, tq, unit, qx, qy, qz);
         }
         float fv, nv, ndv;
-        eval1d<NI, PROC>(P, R, C, cache_valid, nf, qx, qy, qz, phase != PH_COND, fv, nv, ndv);
+        eval1d<NI, PROC, FAST>(P, R, C, cache_valid, nf, qx, qy, qz, phase != PH_COND, fv, nv, ndv);
 
         if (phase == PH_COND) {
             ++evals_u;
@@ -2293,12 +2302,39 @@ gpis_status gpis_set_lists(gpis_ctx *c, int32_t sub, int32_t points) {
     return s != GPIS_OK ? s : t;
 }
 
+gpis_status gpis_set_precision(gpis_ctx *c, int32_t mode) {
+    if (!c) return GPIS_INVALID_ARGUMENT;
+    if (mode != GPIS_PRECISION_FAST && mode != GPIS_PRECISION_EXACT)
+        return fail(c, GPIS_INVALID_ARGUMENT, "precision mode must be GPIS_PRECISION_FAST or GPIS_PRECISION_EXACT");
+    c->fast1d = mode == GPIS_PRECISION_FAST;
+    return GPIS_OK;
+}
+
 gpis_status gpis_set_refill(gpis_ctx *c, int32_t batch) {
     if (!c) return GPIS_INVALID_ARGUMENT;
     if (batch < 0 || batch > 64) return fail(c, GPIS_INVALID_ARGUMENT, "refill batch must be in 0..64 lanes");
     c->batch_k = batch < 1 ? 1 : batch;
     c->batch_k_1d = c->batch_k;
     return GPIS_OK;
+}
+
+// The persistent 1D kernel for the medium's mean (procedural or not) and the precision
+// mode of the context (FP32 fast path or the reference's exact rounding).
+extern "C++" template <bool PATHS>
+static void launch_flight1d_kernel(gpis_ctx *c, unsigned blocks, hipStream_t st, const gpis_ray_soa *rays, unsigned n,
+                                   const ReplyOut &out, const float *light, const Nee1DOut &nee, const uint32_t *idx,
+                                   const uint32_t *n_dev, const Path1DIn &pin, const Path1DOut &pout) {
+    const float lx = light ? light[0] : 0.f, ly = light ? light[1] : 0.f, lz = light ? light[2] : 0.f;
+#define GPIS_1D_LAUNCH(PROC, FAST)                                                                               \
+    hipLaunchKernelGGL((flight1d_kernel<8, PATHS, PROC, FAST>), dim3(blocks), dim3(256), 0, st, c->P_dev,          \
+                       make_in(rays), rays->length_scale, out, n, c->work, c->counters, c->nf1d, light ? 1 : 0, lx,   \
+                       ly, lz, nee, idx, n_dev, c->batch_k_1d, pin, pout)
+    const bool proc = c->P.mean_kind == GPIS_MEAN_PROCEDURAL;
+    if (proc && c->fast1d) GPIS_1D_LAUNCH(true, true);
+    else if (proc) GPIS_1D_LAUNCH(true, false);
+    else if (c->fast1d) GPIS_1D_LAUNCH(false, true);
+    else GPIS_1D_LAUNCH(false, false);
+#undef GPIS_1D_LAUNCH
 }
 
 // Launch the persistent 1D kernel over n rays, or over the first *n_dev entries of the
@@ -2312,16 +2348,8 @@ static gpis_status launch_flight1d(gpis_ctx *c, const gpis_ray_soa *rays, int64_
     long long want = (long long)c->num_cus * wpc * 64 / threads;
     long long need = (n + threads - 1) / threads;
     unsigned blocks = (unsigned)(need < want ? need : (want < 1 ? 1 : want));
-    if (c->P.mean_kind == GPIS_MEAN_PROCEDURAL)
-        hipLaunchKernelGGL((flight1d_kernel<8, false, true>), dim3(blocks), dim3(threads), 0, st, c->P_dev, make_in(rays),
-                           rays->length_scale, out, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
-                           light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, nee, idx, n_dev,
-                           c->batch_k_1d, Path1DIn{}, Path1DOut{});
-    else
-        hipLaunchKernelGGL((flight1d_kernel<8, false, false>), dim3(blocks), dim3(threads), 0, st, c->P_dev, make_in(rays),
-                           rays->length_scale, out, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
-                           light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, nee, idx, n_dev,
-                           c->batch_k_1d, Path1DIn{}, Path1DOut{});
+    launch_flight1d_kernel<false>(c, blocks, st, rays, (unsigned)n, out, light, nee, idx, n_dev, Path1DIn{},
+                                  Path1DOut{});
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }
@@ -2365,16 +2393,8 @@ gpis_status gpis_trace_paths_1d(gpis_ctx *c, const gpis_ray_soa *rays, const gpi
     Path1DOut pout = {reply->t0,      reply->gx0,   reply->gy0,    reply->gz0,          reply->outcome0,
                       reply->flights, reply->hits,  reply->shadows, reply->visible,     reply->nee,
                       reply->t_last,  reply->outcome_last, reply->evals};
-    if (c->P.mean_kind == GPIS_MEAN_PROCEDURAL)
-        hipLaunchKernelGGL((flight1d_kernel<8, true, true>), dim3(blocks), dim3(threads), 0, st, c->P_dev, make_in(rays),
-                           rays->length_scale, ReplyOut{}, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
-                           light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, Nee1DOut{}, nullptr,
-                           nullptr, c->batch_k_1d, pin, pout);
-    else
-        hipLaunchKernelGGL((flight1d_kernel<8, true, false>), dim3(blocks), dim3(threads), 0, st, c->P_dev, make_in(rays),
-                           rays->length_scale, ReplyOut{}, (unsigned)n, c->work, c->counters, c->nf1d, light ? 1 : 0,
-                           light ? light[0] : 0.f, light ? light[1] : 0.f, light ? light[2] : 0.f, Nee1DOut{}, nullptr,
-                           nullptr, c->batch_k_1d, pin, pout);
+    launch_flight1d_kernel<true>(c, blocks, st, rays, (unsigned)n, ReplyOut{}, light, Nee1DOut{}, nullptr, nullptr, pin,
+                                 pout);
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }
@@ -2392,12 +2412,15 @@ static gpis_status launch_flight1d_groups(gpis_ctx *c, const gpis_ray_soa *rays,
     const long long want = (long long)c->num_cus * 2;  // two blocks of four waves per CU (2 waves per SIMD)
     const long long need = (n + groups_per_block - 1) / groups_per_block;
     const unsigned blocks = (unsigned)(need < want ? need : (want < 1 ? 1 : want));
-    if (c->P.mean_kind == GPIS_MEAN_PROCEDURAL)
-        hipLaunchKernelGGL((flight1d_group_kernel<8, true>), dim3(blocks), dim3(threads), 0, st, c->P_dev, make_in(rays),
-                           rays->length_scale, out, work, c->counters, c->nf1d, idx, n_dev);
-    else
-        hipLaunchKernelGGL((flight1d_group_kernel<8, false>), dim3(blocks), dim3(threads), 0, st, c->P_dev, make_in(rays),
-                           rays->length_scale, out, work, c->counters, c->nf1d, idx, n_dev);
+#define GPIS_1DG_LAUNCH(PROC, FAST)                                                                              \
+    hipLaunchKernelGGL((flight1d_group_kernel<8, PROC, FAST>), dim3(blocks), dim3(threads), 0, st, c->P_dev,          \
+                       make_in(rays), rays->length_scale, out, work, c->counters, c->nf1d, idx, n_dev)
+    const bool proc = c->P.mean_kind == GPIS_MEAN_PROCEDURAL;
+    if (proc && c->fast1d) GPIS_1DG_LAUNCH(true, true);
+    else if (proc) GPIS_1DG_LAUNCH(true, false);
+    else if (c->fast1d) GPIS_1DG_LAUNCH(false, true);
+    else GPIS_1DG_LAUNCH(false, false);
+#undef GPIS_1DG_LAUNCH
     HIP_TRY(c, hipGetLastError());
     return GPIS_OK;
 }

@@ -92,6 +92,16 @@ bool HipRayBatchBackend::setMeanGrid(const gpis_mean_grid &grid) {
     return true;
 }
 
+bool HipRayBatchBackend::setPrecision(std::int32_t mode) {
+    if (!ok()) return false;
+    std::lock_guard<std::mutex> lk(deviceMutex_);
+    if (gpis_set_precision(ctx_, mode) != GPIS_OK) {
+        error_ = gpis_last_error(ctx_);
+        return false;
+    }
+    return true;
+}
+
 std::vector<RaySegmentReply> HipRayBatchBackend::process(const std::vector<RaySegmentRequest> &requests) {
     std::vector<RaySegmentReply> replies(requests.size());
     if (!ok()) return replies;  // all invalid: the caller traces them on the CPU

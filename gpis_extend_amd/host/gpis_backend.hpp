@@ -81,6 +81,8 @@ public:
     // Required for a tabulated mean (mean_kind GPIS_MEAN_GRID) before the first trace:
     // the dense export of the mean's grid (the reference's makeHostMeanGrid output).
     bool setMeanGrid(const gpis_mean_grid &grid);
+    // Arithmetic of 1D flights (gpis_set_precision: GPIS_PRECISION_FAST or _EXACT).
+    bool setPrecision(std::int32_t mode);
 
     bool trace(const RaySegmentRequest &request, RaySegmentResult &result);
     std::future<std::vector<RaySegmentReply>> submitBatch(const std::vector<RaySegmentRequest> &requests,

@@ -19,6 +19,12 @@
 using namespace gpis_host;
 
 int main(int argc, char **argv) {
+    // an optional last argument "exact" or "fast" selects the 1D arithmetic
+    int precision = GPIS_PRECISION_FAST;
+    if (argc > 1 && (std::strcmp(argv[argc - 1], "exact") == 0 || std::strcmp(argv[argc - 1], "fast") == 0)) {
+        precision = std::strcmp(argv[argc - 1], "exact") == 0 ? GPIS_PRECISION_EXACT : GPIS_PRECISION_FAST;
+        --argc;
+    }
     if (argc < 12) {
         std::fprintf(stderr, "usage: %s desc.bin paths.bin n results.bin max_bounces phase bounce_seed box "
                              "scene_seed pool batch [lx ly lz]\n", argv[0]);
@@ -58,7 +64,7 @@ int main(int argc, char **argv) {
         for (int a = 0; a < 3; ++a) s.light[a] = (float)std::atof(argv[12 + a]);
     }
     HipRayBatchBackend be(desc, s.batchSize);
-    if (!be.ok()) { std::fprintf(stderr, "backend: %s\n", be.error().c_str()); return 3; }
+    if (!be.ok() || !be.setPrecision(precision)) { std::fprintf(stderr, "backend: %s\n", be.error().c_str()); return 3; }
     PathPool pool(be, desc, s);
     std::vector<PathResult> res;
     if (!pool.run(paths, res)) { std::fprintf(stderr, "path pool: %s\n", pool.error().c_str()); return 4; }

@@ -416,6 +416,14 @@ class GpisDevice:
         evaluation points per list pass (1 or 2)."""
         A.check(self.L.gpis_set_lists(self.h, sub, points), self.h, "gpis_set_lists")
 
+    def set_precision(self, mode: str):
+        """Arithmetic of the 1D flights: "fast" (FP32 splats and analytic mean, the default)
+        or "exact" (the reference's rounding step by step, bit-identical with the oracle)."""
+        m = {"fast": 0, "exact": 1}.get(mode)
+        if m is None:
+            raise MediumConfigError(f"unknown precision {mode!r} (fast, exact)")
+        A.check(self.L.gpis_set_precision(self.h, m), self.h, "gpis_set_precision")
+
     def impulse_table(self, lo, dims, seed):
         import torch
         n = int(self.medium.impulse_density)

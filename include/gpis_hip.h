@@ -288,6 +288,18 @@ gpis_status gpis_set_refill(gpis_ctx *ctx, int32_t batch);
  * reference counterpart (a data structure of the single-realization table). */
 gpis_status gpis_set_lists(gpis_ctx *ctx, int32_t sub, int32_t points);
 
+/* Arithmetic of the 1D along-ray flights (gpis_trace_flights_1d, gpis_trace_paths_1d and
+ * the surrogate's fallback flights).  GPIS_PRECISION_FAST (the default): the splats,
+ * the conditioning splats and the analytic means in FP32 with the hardware exponential,
+ * within the flight tolerance of the reference (hit distances to 1e-4 relative, outcomes
+ * identical but at grazing crossings).  GPIS_PRECISION_EXACT: the reference's rounding
+ * step by step (exponentials rounded from double, mean in double), bit-identical with
+ * the CPU restatement.  Support decisions, summation order, conditioning, the secant and
+ * the transverse gradient are the same in both.  No reference counterpart (the reference
+ * has one arithmetic). */
+enum { GPIS_PRECISION_FAST = 0, GPIS_PRECISION_EXACT = 1 };
+gpis_status gpis_set_precision(gpis_ctx *ctx, int32_t mode);
+
 #ifdef __cplusplus
 }
 #endif

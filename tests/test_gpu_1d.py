@@ -23,10 +23,11 @@ def _medium(js, grid=None):
     return SparseConvolutionNoiseMedium.from_json(js, mean_grid=grid)
 
 
-def _run(js, rays, light, grid=None):
+def _run(js, rays, light, grid=None, precision="exact"):
     from gpis_extend_amd import GpisDevice, RayBatch
     med = _medium(js, grid)
     dev = GpisDevice(med)
+    dev.set_precision(precision)
     rep = dev.trace_flights_1d(RayBatch.from_numpy(rays), light=light)
     torch.cuda.synchronize()
     got = {k: v.cpu().numpy() for k, v in rep.items()}
@@ -86,8 +87,10 @@ def _mirror_normal(rays, p, light):
     return _normalize_f32((wl - d).astype(np.float32))
 
 
-def _check_1d(js, rays, light, name, min_agree=0.9999, grid=None):
-    med, got, counters = _run(js, rays, light, grid)
+def _check_1d(js, rays, light, name, min_agree=0.9999, grid=None, precision="exact"):
+    """precision "exact": the reference's rounding (bit-identical distances for most
+    flights, identical evaluation counts); "fast": the FP32 path, through the same gate."""
+    med, got, counters = _run(js, rays, light, grid, precision)
     desc = oracle_desc(med)
     orays = oracle_rays(rays, med)
     ls = rays.get("length_scale")
@@ -105,7 +108,12 @@ def _check_1d(js, rays, light, name, min_agree=0.9999, grid=None):
     # for a conditioned flight and one NEE evaluation per hit when a light is given
     cond = (rays["flags"] & 1) == 0
     extra = cond.astype(np.int64) + ((oc == po.HIT) & (light is not None))
-    assert np.array_equal(got["evals"][agree].astype(np.int64), ref["evals"][agree] + extra[agree])
+    same_evals = got["evals"][agree].astype(np.int64) == ref["evals"][agree] + extra[agree]
+    if precision == "exact":
+        assert same_evals.all()
+    else:
+        # a secant candidate within FP32 rounding of f = 0 may take one more shrink step
+        assert same_evals.mean() >= 0.999
     assert counters["flights"] == n
     assert counters["evals"] == int(got["evals"].astype(np.int64).sum())
     assert counters["hits"] == int((oc == po.HIT).sum())
@@ -150,21 +158,24 @@ def _check_1d(js, rays, light, name, min_agree=0.9999, grid=None):
     return got, ref
 
 
-def test_flights_1d_renewal_plus_per_ray_length_scale_with_nee():
+@pytest.mark.parametrize("precision", ["exact", "fast"])
+def test_flights_1d_renewal_plus_per_ray_length_scale_with_nee(precision):
     from gpis_extend_amd import workloads as W
+    print(f"case: C3 65,536 1D flights + NEE, precision {precision}", flush=True)
     js = W.config3_json()
     med = _medium(js)
     rays = W.renewal1d_rays(med, n=65536)
-    _check_1d(js, rays, W.LIGHT_POS, "C3 (65,536 1D flights + NEE)")
+    _check_1d(js, rays, W.LIGHT_POS, f"C3 (65,536 1D flights + NEE, {precision})", precision=precision)
 
 
-def test_flights_1d_primary_rays_medium_length_scale():
+@pytest.mark.parametrize("precision", ["exact", "fast"])
+def test_flights_1d_primary_rays_medium_length_scale(precision):
     from gpis_extend_amd import workloads as W
     js = W.config3_json()
     js["1D_sampling_scheme"] = "uni"
     med = _medium(js)
     rays = W.sphere_rays(med, n=16384)
-    _check_1d(js, rays, None, "1D primary (16,384 flights)")
+    _check_1d(js, rays, None, f"1D primary (16,384 flights, {precision})", precision=precision)
 
 
 def test_flights_1d_renewal_without_gradient():

@@ -61,7 +61,7 @@ def _write_paths(path, rays):
     raw.tofile(path)
 
 
-def _run_pool(tmp_path, med, rays, phase, light, max_bounces=4, pool=3000, batch=1024):
+def _run_pool(tmp_path, med, rays, phase, light, max_bounces=4, pool=3000, batch=1024, precision="fast"):
     (tmp_path / "desc.bin").write_bytes(bytes(med.to_desc()))
     _write_paths(tmp_path / "paths.bin", rays)
     n = rays["ox"].shape[0]
@@ -69,6 +69,7 @@ def _run_pool(tmp_path, med, rays, phase, light, max_bounces=4, pool=3000, batch
             str(max_bounces), str(phase), "7", "1.0", str(int(rays["scene_seed"])), str(pool), str(batch)]
     if light is not None:
         args += [repr(float(x)) for x in light]
+    args.append(precision)
     out = subprocess.run(args, capture_output=True, text=True, timeout=300)
     print(out.stdout.strip(), out.stderr.strip())
     assert out.returncode == 0
@@ -114,7 +115,7 @@ def test_path_pool_lambert_light_samples_equal_the_oracle(tmp_path):
     med = SparseConvolutionNoiseMedium.from_json(W.config3_json())
     rays = W.renewal1d_rays(med, n=8192, seed=42)
     del rays["length_scale"]
-    pool = _run_pool(tmp_path, med, rays, 1, W.LIGHT_POS)
+    pool = _run_pool(tmp_path, med, rays, 1, W.LIGHT_POS, precision="exact")
     ref = po.trace_paths1d(oracle_desc(med), oracle_rays(rays, med), None, 4, po.PHASE_LAMBERT, 7, 1.0, W.LIGHT_POS,
                            nthreads=min(16, os.cpu_count() or 1))
     same = ((pool["flights"] == ref["flights"]) & (pool["hits"] == ref["hits"]) &

@@ -31,6 +31,8 @@ def _medium(js):
 def _gpu_paths(med, rays, max_bounces, phase, light, bounce_seed=7):
     from gpis_extend_amd import GpisDevice, RayBatch
     dev = GpisDevice(med)
+    # the reference's rounding: every flight of a path bit-identical with the oracle's
+    dev.set_precision("exact")
     rb = RayBatch.from_numpy(rays)
     dev.reset_counters()
     rep = dev.trace_paths_1d(rb, rays["pss"], rays["scene_seed"], max_bounces, phase=phase,
