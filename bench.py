@@ -43,6 +43,17 @@ def flops_per_eval_1d(n_imp: int) -> float:
     return 3 * n_imp * 4 + 2 * n_imp * 10 + 32
 
 
+# HBM bytes per launch of the dominant kernels, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs of
+# this bench (KiB per dispatch, scaled by what the counters read for a 256 MiB device copy in
+# the same run: FETCH x2.0, WRITE x1.0).  Counters cannot be read from inside the timed process,
+# so the figures are the profiled ones, named with the profile summary they come from.
+MEASURED_TRAFFIC = {
+    "flight_kernel<8,true>": {"bytes": (8.49e6 * 2.0 + 1.044e6) * 1024, "fetch_bytes": 8.49e6 * 2.0 * 1024,
+                              "write_bytes": 1.044e6 * 1024, "source": "profiles/flight_kernel_c1_paths_2048.md"},
+    "flight1d_kernel<8,fast>": {"bytes": (2.40e5 * 2.0 + 6.09e5) * 1024, "fetch_bytes": 2.40e5 * 2.0 * 1024,
+                                "write_bytes": 6.09e5 * 1024, "source": "profiles/flight1d_kernel_c3_2m.md"},
+}
+
 PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector peak (packed FMA issue)
 PEAK_FP32_SCALAR_TFLOPS = 78.6    # one FMA per lane per cycle, no packed issue
 
@@ -241,7 +252,10 @@ def run_1d(args):
                    "exact_rounding_mode": {"kernel_ms": exact_ms,
                                            "flights_per_s": flights_per_step / (exact_ms / 1e3)}},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": MEASURED_TRAFFIC["flight1d_kernel<8,fast>"]["bytes"],
+                     "traffic_detail": MEASURED_TRAFFIC["flight1d_kernel<8,fast>"],
+                     "hbm_bytes_algorithmic": n * 112,
                      "frac_scalar_issue_peak": achieved / PEAK_FP32_SCALAR_TFLOPS,
                      "kernel": "flight1d_kernel<8,fast>", "kernel_ms": kernel_ms, "flops_per_eval": fpe},
     }
@@ -652,6 +666,20 @@ def run_two_spheres(args):
             "kernel_ms": kernel_ms, "evals_per_s": evals / (kernel_ms / 1e3)}
 
 
+def _copy_probe(args):
+    """One device-to-device copy of a known size (a reference kernel for calibrating the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counters of the same run)."""
+    if not args.copy_probe_mib:
+        return
+    import torch
+    src = torch.empty(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    src.fill_(1.0)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -675,6 +703,9 @@ def main():
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the launch-shape sweep that runs before the warmup steps")
+    ap.add_argument("--launch", default="",
+                    help="R,W,B[,S]: refill batch, waves per CU, threads per block and support-list "
+                         "sub-cells for the camera-ray paths; replaces the autotuner's sweep")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed region, copy this many MiB device-to-device once "
                          "(a known-size kernel for calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -688,21 +719,27 @@ def main():
         _oracle_build()  # before anything loads the oracle: the baselines time the host build
     if args.workload == "c3":
         print(json.dumps(run_1d(args)), flush=True)
+        _copy_probe(args)
         return
     if args.workload == "c2":
         print(json.dumps(run_c2(args)), flush=True)
+        _copy_probe(args)
         return
     if args.workload == "c0w1":
         print(json.dumps(run_points(args)), flush=True)
+        _copy_probe(args)
         return
     if args.workload == "c3paths":
         print(json.dumps(run_paths_1d(args)), flush=True)
+        _copy_probe(args)
         return
     if args.workload == "c4":
         print(json.dumps(run_surrogate(args)), flush=True)
+        _copy_probe(args)
         return
     if args.workload == "twospheres":
         print(json.dumps(run_two_spheres(args)), flush=True)
+        _copy_probe(args)
         return
 
     import torch
@@ -720,7 +757,12 @@ def main():
     reply = dev.alloc_path_reply(n)
     stream = torch.cuda.current_stream()
     tuned = None
-    if args.waves_per_cu:
+    if args.launch:
+        v = [int(x) for x in args.launch.split(",")]
+        tuned = (v[0], v[1], v[2], v[3] if len(v) > 3 else 3)
+        from gpis_extend_amd import autotune
+        autotune._apply(dev, tuned, True)
+    elif args.waves_per_cu:
         dev.set_launch(args.waves_per_cu, 0)
     elif not args.no_autotune:
         from gpis_extend_amd import autotune
@@ -771,18 +813,15 @@ def main():
                    "list_passes_per_step": c["passes"] / args.steps, "paired_per_step": c["paired"] / args.steps,
                    "spec_evals_per_step": c["spec_evals"] / args.steps},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": MEASURED_TRAFFIC["flight_kernel<8,true>"]["bytes"],
+                     "traffic_detail": MEASURED_TRAFFIC["flight_kernel<8,true>"],
+                     "hbm_bytes_algorithmic": n * 96,
                      "frac_scalar_issue_peak": achieved / PEAK_FP32_SCALAR_TFLOPS,
                      "kernel": "flight_kernel<8,true>", "kernel_ms": kernel_ms,
                      "flops_per_eval": fpe},
     }
-    if args.copy_probe_mib:
-        src = torch.empty(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        src.fill_(1.0)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
+    _copy_probe(args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(med, rays)
     dev.close()

@@ -166,14 +166,6 @@ static gpis_status build_params(gpis_ctx *c) {
         const double var = (double)((float)(unsigned)d.impulse_density / (P.iso_R * P.iso_R * P.iso_R)) * pow(M_PI, 1.5);
         P.iso_nf_inv = (float)(1.0 / sqrt((double)(float)var));
     }
-    // The brute-force nonstationary kernel conditions the gradient with the local scale
-    // (gradient_scale *= sqr(sparseConvNoiseLateralScale(p)), SparseConvolutionNoise.cpp:563-565),
-    // which the 3D flight kernels do not apply: refused rather than served wrong.
-    if (d.cov_kind == GPIS_COV_PROC_NS && !d.ns_multires && !d.one_d && !d.single_realization &&
-        d.ctxt == GPIS_CTX_RENEWAL_PLUS)
-        return fail(c, GPIS_UNSUPPORTED,
-                    "Renewal+ conditioning of the brute-force nonstationary kernel is not served "
-                    "(multiResolutionGrid and the Renewal context are)");
     P.kind = d.cov_kind;
     P.multires = (d.cov_kind == GPIS_COV_PROC_NS) && d.ns_multires;
     P.sigma = d.sigma;
@@ -530,6 +522,13 @@ __device__ __noinline__ void condition_coefficients(const FieldParams &P, const 
             b0 = -2.0 / ((double)P.l_aniso_inv[0] * P.l_aniso_inv[0]) * del0 / gsum;
             b1 = -2.0 / ((double)P.l_aniso_inv[1] * P.l_aniso_inv[1]) * del1 / gsum;
             b2 = -2.0 / ((double)P.l_aniso_inv[2] * P.l_aniso_inv[2]) * del2 / gsum;
+            if (P.kind == GPIS_COV_PROC_NS && !P.multires) {
+                // the brute-force kernel splats at l(o): gradient_scale *= sqr(lateral scale)
+                // (SparseConvolutionNoise.cpp:563-565), the square formed in float
+                const float lat = ls_field(P, ox, oy, oz);
+                const double s2 = (double)(lat * lat);
+                b0 *= s2; b1 *= s2; b2 *= s2;
+            }
         }
     }
     rs.ox = ox; rs.oy = oy; rs.oz = oz;

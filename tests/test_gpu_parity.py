@@ -131,23 +131,24 @@ def test_flights_config2_conditioned_multires_subset():
     parity_gate(got, ref, "C2 conditioned multires subset", min_agree=0.99, oracle=orc)
 
 
-def test_flights_config2_conditioned_brute_force_subset():
+@pytest.mark.parametrize("ctxt", ["renewal", "renewal+"])
+def test_flights_config2_conditioned_brute_force_subset(ctxt):
     """The nonstationary workload with multiResolutionGrid false: every splat at the
     local scale l(p) / l_max on the lattice of l_max (GPFunctions.cpp:1225-1345), flights
-    conditioned on the value (Renewal); Renewal+ is refused for this kernel."""
-    from gpis_extend_amd import GpisDevice, workloads as W
-    from gpis_extend_amd import SparseConvolutionNoiseMedium
-    from gpis_extend_amd._capi import GpisError
-    print("case: C2 brute-force nonstationary, 512 Renewal flights", flush=True)
+    conditioned on the value (Renewal) or on the value and the gradient (Renewal+, whose
+    gradient coefficients carry the factor l(o)^2, SparseConvolutionNoise.cpp:563-565)."""
+    from gpis_extend_amd import workloads as W
+    print(f"case: C2 brute-force nonstationary, 2,048 {ctxt} flights", flush=True)
     js = W.config2_json()
     js["gaussian_process"]["covariance"]["multiResolutionGrid"] = False
-    with pytest.raises(GpisError, match="UNSUPPORTED"):
-        GpisDevice(SparseConvolutionNoiseMedium.from_json(js))
-    js["correlation_context"] = "renewal"
+    js["correlation_context"] = ctxt
+    from gpis_extend_amd import SparseConvolutionNoiseMedium
     med = SparseConvolutionNoiseMedium.from_json(js)
-    rays = W.box_surface_rays(med, n=512)
+    rays = W.box_surface_rays(med, n=2048)
     got, ref, orc = _flight_compare(js, rays, None, "C2bf")
-    parity_gate(got, ref, "C2 Renewal brute-force subset", min_agree=0.99, oracle=orc)
+    # surface-born conditioned flights start at f = 0: their first sign decisions are
+    # intrinsically grazing, and the gate's grazing test names every disagreement
+    parity_gate(got, ref, f"C2 {ctxt} brute-force subset", min_agree=0.99, oracle=orc)
 
 
 def test_partial_impulse_table_equals_generator():
