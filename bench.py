@@ -525,7 +525,7 @@ def run_surrogate(args):
     t0 = time.perf_counter()
     feats, s, ex, oc = S.flight_dataset(dev, net, med, args.train_flights, seed=S.SEED_TRAIN)
     log(f"training set: {feats.shape[0]} plain flights in {time.perf_counter() - t0:.1f} s")
-    cal = S.calibrate(feats, s[~ex])
+    cal = S.calibrate(feats, s, ex)
     net.set_calibration(cal)
     cls = S.classes_of(net, s, oc)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -571,23 +571,44 @@ def run_surrogate(args):
     _, fwd_ms = timed(lambda: net.forward(fe))
     torch.cuda.synchronize()
 
-    sp, exp_, _ = S.normalised_distance(rb, plain)
-    ss, _, _ = S.normalised_distance(rb, sur)
+    sp, exp_, ocp = S.normalised_distance(rb, plain)
+    ss, _, ocs = S.normalised_distance(rb, sur)
     sp_np, ss_np = sp.cpu().numpy(), ss.cpu().numpy()
     ks = S.ks_two_sample(sp_np, ss_np)
     # PIT of the plain distances under the network's CDFs, on the rays it serves
     served = (sur["evals"] == 0).cpu().numpy()
-    probs = net.forward(fe).cpu().numpy()
+    probs_t = net.forward(fe)
+    probs = probs_t.cpu().numpy()
     rng = np.random.default_rng(5)
     pit = S.pit_ks(probs[served], cal.edges, sp_np[served], exp_.cpu().numpy()[served], rng)
-    # image error: per-pixel mean normalised distance over the 16 samples
-    img_p = sp_np.reshape(-1, 16).mean(1)
-    img_s = ss_np.reshape(-1, 16).mean(1)
+    # a second plain render of the same rays with independent realizations: the noise floor
     rays2 = W.renewal1d_image_rays(med, width=512, spp=16, seed=S.SEED_INFER, scene_seed=4)
     rb2 = RayBatch.from_numpy(rays2)
     plain2 = dev.trace_flights_1d(rb2)
-    sp2, _, _ = S.normalised_distance(rb2, plain2)
-    img_p2 = sp2.cpu().numpy().reshape(-1, 16).mean(1)
+    sp2, _, ocp2 = S.normalised_distance(rb2, plain2)
+    sp2_np = sp2.cpu().numpy()
+    # the conditional test: KS inside the 27 quantile strata of (value, derivative, length
+    # scale), against the plain-vs-plain floor and the input-free sampler (the pooled law)
+    F = fe.cpu().numpy()
+    sid = S.strata_of(F[:, list(S.EDGE_STRATA_FEATURES)], 3)
+    ks_strata = S.stratified_ks(sid, sp_np, ss_np)
+    ks_floor = S.stratified_ks(sid, sp_np, sp2_np)
+    ks_free = S.stratified_ks(sid, sp_np, sp_np[np.random.default_rng(6).permutation(n)])
+    # outcome classes of the same rays, and the cross-entropy of the plain classes under
+    # the network against the input-free predictor's (the marginal class frequencies)
+    frac = lambda o: {k: float((o == v).float().mean().item()) for k, v in (("hit", 1), ("exit", 0), ("fail", 2))}
+    cls = S.classes_of(net, sp, ocp)
+    ok = cls >= 0
+    ce_net = float(-(probs_t[ok].gather(1, cls[ok].long()[:, None]).clamp_min(1e-30).log()).mean().item())
+    pm = torch.bincount(cls[ok].long(), minlength=S.N_CLASS).double()
+    pm = pm / pm.sum()
+    ce_free = float(-(pm[pm > 0] * pm[pm > 0].log()).sum().item())
+    # image error: per-pixel mean normalised distance over the 16 samples
+    img_p = sp_np.reshape(-1, 16).mean(1)
+    img_s = ss_np.reshape(-1, 16).mean(1)
+    img_p2 = sp2_np.reshape(-1, 16).mean(1)
+    bias_s, limit_s = S.image_bias(ss_np.reshape(-1, 16), sp_np.reshape(-1, 16))
+    bias_p, limit_p = S.image_bias(sp2_np.reshape(-1, 16), sp_np.reshape(-1, 16))
     rel = lambda a, b: float(np.mean(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)))
     line = {
         "metric": "GPIS 1D flights/s on 1 GPU, surrogate (plain fallback) vs plain",
@@ -602,7 +623,12 @@ def run_surrogate(args):
         "speedup_vs_plain": plain_wall / sur_wall,
         "fallback_fraction": nfb / n,
         "ks_vs_plain": ks, "pit_ks": pit,
+        "ks_strata": ks_strata, "ks_strata_plain_vs_plain": ks_floor, "ks_strata_input_free": ks_free,
+        "outcomes": {"plain": frac(ocp), "surrogate": frac(ocs), "plain_second_render": frac(ocp2)},
+        "cross_entropy": {"network": ce_net, "input_free": ce_free},
         "image_rel_error": rel(img_s, img_p), "image_rel_error_plain_vs_plain": rel(img_p2, img_p),
+        "image_rel_bias": {"surrogate_vs_plain": bias_s, "plain_vs_plain": bias_p,
+                           "detection_limit": max(limit_s, limit_p)},
         "train": {"seconds": train_s, "samples_per_s": samples / train_s, "loss_first_epoch": losses[0],
                   "loss_last_epoch": losses[-1],
                   "tflops": samples * 3 * MLP_FLOPS_PER_ROW / train_s / 1e12},
@@ -830,8 +856,9 @@ def main():
         # the nonstationary chains, the field at 2^20 points, plain 1D flights with NEE,
         # 1D paths, and the learned surrogate against the plain method (KS, image error)
         keep = ("value", "unit", "ms_per_step", "config", "roofline", "cpu_baseline", "plain", "speedup_vs_plain",
-                "fallback_fraction", "ks_vs_plain", "pit_ks", "image_rel_error", "image_rel_error_plain_vs_plain",
-                "train", "metric", "dtype")
+                "fallback_fraction", "ks_vs_plain", "pit_ks", "ks_strata", "ks_strata_plain_vs_plain",
+                "ks_strata_input_free", "outcomes", "cross_entropy", "image_rel_error",
+                "image_rel_error_plain_vs_plain", "image_rel_bias", "train", "metric", "dtype")
         for key, fn in (("chains_c2", run_c2), ("points_c0w1", run_points), ("plain_1d", run_1d),
                         ("paths_1d", run_paths_1d), ("surrogate", run_surrogate)):
             sub = fn(args)

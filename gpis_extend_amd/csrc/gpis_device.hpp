@@ -285,13 +285,13 @@ __device__ __forceinline__ double grid_mean_grad_d(const FieldParams &P, float x
     return c;
 }
 
-// Tabulated mean and its gradient rounded to float.
-__device__ __forceinline__ float grid_mean_value(const FieldParams &P, float x, float y, float z, float &gx,
-                                                 float &gy, float &gz) {
+// Tabulated mean (double, as TabulatedMean::mean returns it) and its gradient rounded to float.
+__device__ __forceinline__ double grid_mean_value(const FieldParams &P, float x, float y, float z, float &gx,
+                                                  float &gy, float &gz) {
     double dx, dy, dz;
     const double c = grid_mean_grad_d(P, x, y, z, dx, dy, dz);
     gx = (float)dx; gy = (float)dy; gz = (float)dz;
-    return (float)c;
+    return c;
 }
 
 // The analytic means (sphere, box, plane) and their gradients.
@@ -333,25 +333,33 @@ __device__ __forceinline__ float analytic_mean_value(const FieldParams &P, float
 __device__ __noinline__ double proc_mean_d(const sdf::MeanParams *M, double x, double y, double z, double *g) {
     return sdf::mean(*M, x, y, z, g);
 }
-__device__ __forceinline__ float proc_mean_value(const FieldParams &P, float x, float y, float z, float &gx,
-                                                 float &gy, float &gz) {
+__device__ __forceinline__ double proc_mean_value(const FieldParams &P, float x, float y, float z, float &gx,
+                                                  float &gy, float &gz) {
     double g[3];
     const double v = proc_mean_d(P.sdf, (double)x, (double)y, (double)z, g);
     gx = (float)g[0]; gy = (float)g[1]; gz = (float)g[2];
-    return (float)v;
+    return v;
 }
 
 // Mean kinds as a compile-time choice: any (runtime switch), analytic only (sphere, box,
 // plane), or the evaluated means (tabulated grid or procedural SDF, switched at run time).
 enum { MEAN_ANY = 0, MEAN_ANALYTIC = 1, MEAN_EXT = 2 };
 
+// The mean in double (the evaluated means as the reference returns them; the analytic means'
+// float value widened) and its gradient in float.
+template <int MEAN = MEAN_ANY>
+__device__ __forceinline__ double mean_value_d(const FieldParams &P, float x, float y, float z, float &gx,
+                                               float &gy, float &gz) {
+    if (MEAN == MEAN_ANALYTIC) return (double)analytic_mean_value(P, x, y, z, gx, gy, gz);
+    if (P.mean_kind == 3) return grid_mean_value(P, x, y, z, gx, gy, gz);
+    if (P.mean_kind == 4) return proc_mean_value(P, x, y, z, gx, gy, gz);
+    return (double)analytic_mean_value(P, x, y, z, gx, gy, gz);
+}
 template <int MEAN = MEAN_ANY>
 __device__ __forceinline__ float mean_value(const FieldParams &P, float x, float y, float z, float &gx, float &gy,
                                             float &gz) {
     if (MEAN == MEAN_ANALYTIC) return analytic_mean_value(P, x, y, z, gx, gy, gz);
-    if (P.mean_kind == 3) return grid_mean_value(P, x, y, z, gx, gy, gz);
-    if (P.mean_kind == 4) return proc_mean_value(P, x, y, z, gx, gy, gz);
-    return analytic_mean_value(P, x, y, z, gx, gy, gz);
+    return (float)mean_value_d<MEAN>(P, x, y, z, gx, gy, gz);
 }
 
 }  // namespace gpis

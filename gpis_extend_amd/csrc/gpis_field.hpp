@@ -622,16 +622,34 @@ __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealStat
 
 __device__ __forceinline__ float amplitude(const FieldParams &P) { return P.amp_const; }
 
-// f(p) = amplitude * noise + mean, and its world gradient (evaluateValue/evaluateGradient).
+// f(p) = amplitude * noise + mean and its world gradient, rounded as evaluateValue and
+// evaluateGradient round them (SparseConvolutionNoise.cpp:113-140): the float product
+// amplitude * noise added to the double mean and rounded once; the gradient's float
+// product and float mean gradient added unfused.  With an analytic (float) mean the double
+// sum of two floats is exact, so the float addition rounds the same.
+template <int MEAN>
+__device__ __forceinline__ void add_mean(const FieldParams &P, float px, float py, float pz, Eval4 &r) {
+#pragma clang fp contract(off)
+    float mgx, mgy, mgz;
+    const float a = amplitude(P);
+    const float an = a * r.v;
+    if (MEAN == MEAN_ANALYTIC) {
+        r.v = an + analytic_mean_value(P, px, py, pz, mgx, mgy, mgz);
+    } else {
+        const double m = mean_value_d<MEAN>(P, px, py, pz, mgx, mgy, mgz);
+        r.v = (float)((double)an + m);
+    }
+    r.gx = a * r.gx + mgx;
+    r.gy = a * r.gy + mgy;
+    r.gz = a * r.gz + mgz;
+}
+
 template <int NI, bool ISO, int MEAN = MEAN_ANY, bool IRAY = false>
 __device__ __forceinline__ Eval4 eval_field(const FieldParams &P, const RealState &rs,
                                             float px, float py, float pz, bool cond, float dx = 0.f,
                                             float dy = 0.f, float dz = 1.f) {
-    Eval4 n = eval_noise<NI, ISO, IRAY>(P, rs, px, py, pz, cond, dx, dy, dz);
-    float mgx, mgy, mgz;
-    float m = mean_value<MEAN>(P, px, py, pz, mgx, mgy, mgz);
-    float a = amplitude(P);
-    Eval4 r = {a * n.v + m, a * n.gx + mgx, a * n.gy + mgy, a * n.gz + mgz};
+    Eval4 r = eval_noise<NI, ISO, IRAY>(P, rs, px, py, pz, cond, dx, dy, dz);
+    add_mean<MEAN>(P, px, py, pz, r);
     return r;
 }
 
@@ -659,13 +677,7 @@ __device__ __forceinline__ void eval_field2(const FieldParams &P, const RealStat
             Eval4 &E = w ? B : A;
             const float *p = w ? pb : pa;
             if (cond && rs.cond) cond_splat(P, rs, p[0], p[1], p[2], 1.0f, 1.0f, E);
-            float mgx, mgy, mgz;
-            const float m = mean_value<MEAN>(P, p[0], p[1], p[2], mgx, mgy, mgz);
-            const float a = amplitude(P);
-            E.v = a * E.v + m;
-            E.gx = a * E.gx + mgx;
-            E.gy = a * E.gy + mgy;
-            E.gz = a * E.gz + mgz;
+            add_mean<MEAN>(P, p[0], p[1], p[2], E);
         }
         return;
     }

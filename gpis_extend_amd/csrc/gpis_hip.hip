@@ -2531,6 +2531,212 @@ __global__ void classes1d_kernel(RayIn in, const float *__restrict__ t, const ui
     cls[r] = c;
 }
 
+// ------------------------------------------------- the endpoint of a served flight --
+// The learned sampler returns a distance, not a realization.  resumeDistance still draws a
+// gradient at the endpoint and applies its rejection rules (GaussianProcessMedium.cpp:300-330),
+// and an exit carries the field value at the bound (the next flight's conditioning value,
+// SparseConvolutionNoiseMedium.cpp:339-346).  A served endpoint takes them from the law of
+// the flight's conditioned 1D SE process psi(u) (isotropic ray units u = k s, k =
+// |worldToLocal(d)|, correlation exp(-u^2/4)), f(s) = m(s) + sigma psi(k s), observed at the
+// origin as psi(0) = a and, for Renewal+, psi'(0) = b (conditioning1D,
+// SparseConvolutionNoise.cpp:576-656):
+//   * the crossing's sense is the sign of f at the first lattice point t0 = near + step *
+//     jitter, where a conditioned flight takes its sign reference (:304-306), drawn from its
+//     conditional law;
+//   * at a crossing (f(t) = 0) the derivative along the ray has the Rice density
+//     |y| N(y; mu, s^2) on the sense's side (y < 0: a down-crossing, a hit; y > 0: the
+//     outward crossing the rules reject);
+//   * at the t = 0 atom (the secant's cut-off returns the ray origin) the derivative is the
+//     conditioned one there (b under Renewal+, N(0, 1/2) under Renewal);
+//   * at an exit the value follows the conditional law of f(far) on the sense's side and
+//     the derivative its law given that value.
+// The transverse gradient is the plain method's own Box-Muller draw (finish1d), so the
+// served gradient differs from a plain one of the same ray only along the ray.  The law's
+// draws use the normal seed's PCG stream after the four draws of the plain method's pairs.
+
+// Conditional mean and covariance of (psi(x), psi'(x)) at lag x (isotropic units) given
+// psi(0) = a and, when `plus`, psi'(0) = b.
+struct Law1D { double e1, e2, c11, c12, c22; };
+__device__ __forceinline__ Law1D cond_law1d(double dl, double a, double b, bool plus) {
+    const double x = 0.5 * dl * dl;
+    const double rho = exp(-0.5 * x), r2 = rho * rho;
+    Law1D L;
+    if (plus) {
+        L.e1 = rho * (a + dl * b);
+        L.e2 = rho * (-0.5 * dl * a + (1.0 - x) * b);
+        // 1 - e^{-x} (1 + x) by its series where the difference cancels
+        L.c11 = x < 1e-3 ? x * x * (0.5 + x * (-1.0 / 3.0 + x * (1.0 / 8.0 + x * (-1.0 / 30.0 + x / 144.0))))
+                         : 1.0 - r2 * (1.0 + x);
+        L.c22 = 0.5 * (1.0 - r2 * (1.0 - x + x * x));
+        L.c12 = 0.25 * dl * dl * dl * r2;
+    } else {
+        L.e1 = rho * a;
+        L.e2 = -0.5 * dl * rho * a;
+        L.c11 = -expm1(-x);
+        L.c22 = 0.5 * (1.0 - x * r2);
+        L.c12 = 0.5 * dl * r2;
+    }
+    return L;
+}
+
+// Draws of the endpoint law: uniforms of a PCG stream and Box-Muller normals (FP32).
+struct LawDraws {
+    Pcg g;
+    __device__ __forceinline__ float uni() { return u01(g.next()); }
+    __device__ __forceinline__ float normal() {
+        const float u1 = uni(), u2 = uni();
+        return sqrtf(-2.0f * __logf(1.0f - u1)) * __cosf(6.28318530718f * u2);
+    }
+};
+
+// X >= 0 with density proportional to x exp(-x^2/2 + m x), the size-biased normal N(m, 1)
+// on [0, inf), by rejection: for m < 0 from a Gamma(2) proposal of rate 1/mode, accepted with
+// exp(-(x - mode)^2 / 2); for m >= 0 from a unit normal about m + 1/mode, accepted with
+// (x/mode) exp(1 - x/mode).  Both acceptances are at least about 0.4; one round draws both
+// proposals so the lanes of a wave stay on one path whatever their m.
+__device__ __forceinline__ float rice_draw(float m, LawDraws &D) {
+    m = fminf(fmaxf(m, -1e6f), 1e6f);
+    const float sq = sqrtf(m * m + 4.0f);
+    const float xm = m < 0.f ? 2.0f / (sq - m) : 0.5f * (m + sq);
+    const float inv_xm = 1.0f / xm;
+    const bool gam = m < 0.f;
+    for (int it = 0; it < 64; ++it) {
+        const float u1 = D.uni(), u2 = D.uni(), u3 = D.uni();
+        // Gamma(2): -log(u u') * mode; normal: Box-Muller of (u1, u2)
+        const float lg = __logf((1.0f - u1) * (1.0f - u2));
+        const float xn = m + inv_xm + sqrtf(-2.0f * __logf(1.0f - u1)) * __cosf(6.28318530718f * u2);
+        const float x = gam ? -lg * xm : xn;
+        const float r = x * inv_xm;
+        const float acc = gam ? __expf(-0.5f * (x - xm) * (x - xm)) : r * __expf(1.0f - r);
+        if (x > 0.f && u3 < acc) return x;
+    }
+    return xm;
+}
+
+// Z ~ N(0, 1) restricted to z > alpha (FP32): the inverse CDF for alpha < 3, Robert's
+// exponential rejection above.
+__device__ __forceinline__ float normal_above(float alpha, LawDraws &D) {
+    alpha = fminf(alpha, 1e6f);
+    if (alpha < 3.0f) {
+        const float q = 0.5f * erfcf(alpha * 0.70710678f);  // P(Z > alpha)
+        const float u = 1.0f - D.uni();                      // (0, 1]
+        return 1.41421356f * erfcinvf(2.0f * u * q);
+    }
+    for (int it = 0; it < 64; ++it) {
+        const float z = alpha - __logf(1.0f - D.uni()) / alpha;
+        const float dz = z - alpha;
+        if (D.uni() < __expf(-0.5f * dz * dz)) return z;
+    }
+    return alpha + 1.0f / alpha;
+}
+
+template <bool PROC>
+__global__ __launch_bounds__(256) void served_finish_kernel(const FieldParams *__restrict__ Pg, RayIn in,
+                                                            const float *__restrict__ lscale, int n, ReplyOut out) {
+#pragma clang fp contract(off)
+    const FieldParams &P = *Pg;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t code = out.outcome[r];
+    if (!(code & mlp::kServedPending)) return;
+    const int cls = (int)(code & 0xFFu);
+    const float ox = in.ox[r], oy = in.oy[r], oz = in.oz[r];
+    const float dx = in.dx[r], dy = in.dy[r], dz = in.dz[r];
+    const float near_t = in.near_t[r], far_t = in.far_t[r];
+    const uint32_t normal_seed = in.normal_seed[r];
+    Ray1D R = {};
+    ray_scales(P.aniso, P.l_aniso, P.l_aniso_inv, lscale, (unsigned)r, R.la, R.ia);
+    float ix = R.ia[0] * dx, iy = R.ia[1] * dy, iz = R.ia[2] * dz;
+    const double k = sqrt((double)ix * ix + (double)iy * iy + (double)iz * iz);  // du/ds
+    normalize3(ix, iy, iz);
+    R.F = make_frame(ix, iy, iz);
+    R.cond = 1;
+    const double sig = P.amp_const;
+    const bool plus = P.ctxt == GPIS_CTX_RENEWAL_PLUS;
+    const bool corr = P.corr_xy && plus;
+    // the observations at the origin (conditioning1D's targets)
+    float mgx, mgy, mgz;
+    mean_grad_d<PROC>(P, ox, oy, oz, mgx, mgy, mgz);
+    const double a = ((double)in.pv[r] - mean_value_d<PROC>(P, ox, oy, oz)) / sig;
+    const float amp = P.amp_const;
+    const float tgx = (in.pgx[r] - mgx) / amp, tgy = (in.pgy[r] - mgy) / amp, tgz = (in.pgz[r] - mgz) / amp;
+    const float gwx = R.la[0] * tgx, gwy = R.la[1] * tgy, gwz = R.la[2] * tgz;
+    const double b = plus ? (double)dot3(R.F.nx, R.F.ny, R.F.nz, gwx, gwy, gwz) : 0.0;
+    float obs_x = 0.f, obs_y = 0.f;
+    if (plus) {
+        obs_x = dot3(R.F.tx, R.F.ty, R.F.tz, gwx, gwy, gwz);
+        obs_y = dot3(R.F.bx, R.F.by, R.F.bz, gwx, gwy, gwz);
+    }
+    LawDraws D;
+    D.g.set((uint64_t)normal_seed);
+    for (int i = 0; i < 4; ++i) (void)D.g.next();  // past the plain method's two pairs
+    // the sense: the sign of f at the first lattice point
+    float step = (far_t - near_t) / P.min_step_f;
+    if (P.step_size < step) step = P.step_size;
+    const float t0 = mul_add_exact(step, in.jitter[r], near_t);
+    float qx, qy, qz;
+    ray_point(ox, oy, oz, dx, dy, dz, (double)t0, qx, qy, qz);
+    const Law1D L0 = cond_law1d(k * (double)t0, a, b, plus);
+    const double f0 = mean_value_d<PROC>(P, qx, qy, qz) + sig * (L0.e1 + sqrt(fmax(L0.c11, 0.0)) * (double)D.normal());
+    const bool down = !(f0 < 0.0);  // a positive side looks for a down-crossing
+    float t_end, ndv, value = 0.f;
+    int hit = 1;
+    if (cls == 0) {
+        // the secant's cut-off: the ray origin and the conditioned derivative there
+        t_end = 0.f;
+        const Law1D L = cond_law1d(0.0, a, b, plus);
+        ndv = (float)(L.e2 + sqrt(fmax(L.c22, 0.0)) * (double)D.normal());
+    } else {
+        t_end = cls >= mlp::kBins ? far_t : out.t[r];
+        hit = cls < mlp::kBins;
+    }
+    grad_point(ox, oy, oz, dx, dy, dz, (double)t_end, qx, qy, qz);
+    if (cls != 0) {
+        const double m = mean_value_d<PROC>(P, qx, qy, qz);
+        float ex, ey, ez;
+        mean_grad_d<PROC>(P, qx, qy, qz, ex, ey, ez);
+        const double ms = (double)ex * dx + (double)ey * dy + (double)ez * dz;  // mean slope
+        const Law1D L = cond_law1d(k * (double)t_end, a, b, plus);
+        const bool ok = L.c11 > 1e-300;
+        const double q = ok ? L.c12 / L.c11 : 0.0;
+        const double vs = fmax(L.c22 - q * L.c12, 0.0);
+        const double sk = sig * k;
+        if (hit) {
+            // the Rice law of y = f'(t) at f(t) = 0, on the sense's side
+            const double mu = L.e2 + (ok ? q * (-m / sig - L.e1) : 0.0);
+            const double muy = ms + sk * mu, sy = sk * sqrt(vs);
+            double y;
+            if (sy > 1e-30 * (fabs(muy) + 1.0)) {
+                const double w = (double)rice_draw((float)((down ? -muy : muy) / sy), D) * sy;
+                y = down ? -w : w;
+            } else {
+                y = muy;
+            }
+            ndv = (float)((y - ms) / sk);
+        } else {
+            // the value at the bound on the sense's side, then the derivative given it
+            const double sd = sqrt(fmax(L.c11, 0.0));
+            double psi = L.e1;
+            if (sd > 0.0) {
+                const double alpha = (-m / sig - L.e1) / sd;  // psi > x0 keeps f > 0
+                psi = L.e1 + sd * (double)(down ? normal_above((float)alpha, D) : -normal_above((float)-alpha, D));
+            }
+            value = (float)(m + sig * psi);
+            const double mu = L.e2 + (ok ? q * (psi - L.e1) : 0.0);
+            ndv = (float)(mu + sqrt(vs) * (double)D.normal());
+        }
+    }
+    float gx, gy, gz;
+    const uint32_t oc = finish1d<PROC>(P, R, corr, plus, obs_x, obs_y, ox, oy, oz, qx, qy, qz, dx, dy, dz,
+                                       normal_seed, t_end, ndv, hit, gx, gy, gz);
+    out.t[r] = t_end;
+    out.value[r] = value;
+    out.gx[r] = gx;
+    out.gy[r] = gy;
+    out.gz[r] = gz;
+    out.outcome[r] = oc;
+}
+
 __global__ void fill_u32_kernel(uint32_t *p, int n, uint32_t v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -2554,6 +2760,10 @@ struct gpis_net {
     int64_t fb_rows = 0;
     float *loss = nullptr;
     int64_t step = 0;
+    // served endpoints run on a side stream beside the fallback flights (fork and join
+    // through two events; the side stream has the default priority)
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     std::string err;
 };
 
@@ -2642,6 +2852,9 @@ gpis_status gpis_net_create(uint64_t seed, gpis_net **out) {
     if (e == hipSuccess) e = hipMalloc((void **)&n->calib, sizeof(mlp::Calibration));
     if (e == hipSuccess) e = hipMalloc((void **)&n->loss, 64);
     if (e == hipSuccess) e = hipMalloc((void **)&n->n_fallback, 64);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&n->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&n->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&n->join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipFuncSetAttribute((const void *)mlp::infer_kernel,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)(mlp::kFwdElems * sizeof(mlp::bf16) +
@@ -2675,6 +2888,9 @@ gpis_status gpis_net_destroy(gpis_net *n) {
                     n->xt, n->feats, n->fallback, n->n_fallback, n->loss};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (n->fork) (void)hipEventDestroy(n->fork);
+    if (n->join) (void)hipEventDestroy(n->join);
+    if (n->side) (void)hipStreamDestroy(n->side);
     delete n;
     return GPIS_OK;
 }
@@ -2806,11 +3022,48 @@ gpis_status gpis_net_forward(gpis_net *n, const float *feats, int64_t rows, floa
     return net_infer(n, feats, rows, nullptr, nullptr, nullptr, nullptr, o, (hipStream_t)stream);
 }
 
+}  // extern "C"
+
+// The media the learned sampler serves: conditioned (Renewal, Renewal+) single-level
+// stationary SE flights with a non-zero amplitude, the law the served endpoint assumes.
+static gpis_status surrogate_medium_ok(gpis_ctx *c) {
+    if (!c->P.one_d) return fail(c, GPIS_INVALID_ARGUMENT, "the medium is not 1D_sampling");
+    if (c->P.ctxt != GPIS_CTX_RENEWAL && c->P.ctxt != GPIS_CTX_RENEWAL_PLUS)
+        return fail(c, GPIS_UNSUPPORTED, "the learned sampler serves Renewal and Renewal+ flights");
+    if (c->P.kind != GPIS_COV_SE || c->P.multires)
+        return fail(c, GPIS_UNSUPPORTED, "the learned sampler serves the stationary single-level SE kernel");
+    if (!(c->P.amp_const > 0.f)) return fail(c, GPIS_UNSUPPORTED, "the learned sampler needs sigma > 0");
+    return GPIS_OK;
+}
+
+// After the inference kernel on `st`: the served endpoints on the side stream, the
+// fallback flights (the plain method, one group of lanes per flight) on `st`, joined.
+static gpis_status launch_served_and_fallback(gpis_ctx *c, gpis_net *net, const gpis_ray_soa *rays, int64_t n,
+                                              const ReplyOut &out, const uint32_t *fallback, const uint32_t *n_fb,
+                                              hipStream_t st, unsigned *work) {
+    HIP_TRY(c, hipEventRecord(net->fork, st));
+    HIP_TRY(c, hipStreamWaitEvent(net->side, net->fork, 0));
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (c->P.mean_kind == GPIS_MEAN_PROCEDURAL)
+        hipLaunchKernelGGL(served_finish_kernel<true>, dim3(blocks), dim3(256), 0, net->side, c->P_dev, make_in(rays),
+                           rays->length_scale, (int)n, out);
+    else
+        hipLaunchKernelGGL(served_finish_kernel<false>, dim3(blocks), dim3(256), 0, net->side, c->P_dev,
+                           make_in(rays), rays->length_scale, (int)n, out);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipEventRecord(net->join, net->side));
+    gpis_status s = launch_flight1d_groups(c, rays, n, out, fallback, n_fb, st, work);
+    HIP_TRY(c, hipStreamWaitEvent(st, net->join, 0));
+    return s;
+}
+
+extern "C" {
+
 gpis_status gpis_surrogate_flights_1d(gpis_ctx *c, gpis_net *net, const gpis_ray_soa *rays, int64_t n,
                                       const gpis_reply_soa *reply, uint32_t *n_fallback, void *stream) {
     if (!c || !net) return GPIS_INVALID_ARGUMENT;
     if (!net->calibrated) return nfail(net, GPIS_INVALID_ARGUMENT, "calibration not set");
-    if (!c->P.one_d) return fail(c, GPIS_INVALID_ARGUMENT, "the medium is not 1D_sampling");
+    if (gpis_status ms = surrogate_medium_ok(c)) return ms;
     if (!rays || !reply || !reply->t || !reply->value || !reply->gx || !reply->gy || !reply->gz || !reply->outcome)
         return fail(c, GPIS_INVALID_ARGUMENT, "null ray or reply field");
     if (n < 0 || n >= (1LL << 31)) return fail(c, GPIS_INVALID_ARGUMENT, "ray count out of range");
@@ -2831,7 +3084,7 @@ gpis_status gpis_surrogate_flights_1d(gpis_ctx *c, gpis_net *net, const gpis_ray
     if (s != GPIS_OK) return s;
     ReplyOut out = {reply->t, reply->value, reply->gx, reply->gy, reply->gz, reply->outcome, reply->evals};
     // the fallback rays are few and their flights long: a group of lanes per flight
-    s = launch_flight1d_groups(c, rays, n, out, net->fallback, net->n_fallback, st);
+    s = launch_served_and_fallback(c, net, rays, n, out, net->fallback, net->n_fallback, st, nullptr);
     if (s != GPIS_OK) return s;
     if (n_fallback) {
         HIP_TRY(c, hipMemcpyAsync(n_fallback, net->n_fallback, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -2880,7 +3133,7 @@ gpis_status gpis_surrogate_flights_1d_noalloc(gpis_ctx *c, gpis_net *net, const 
                                               const gpis_reply_soa *reply, uint32_t *n_fallback, void *stream) {
     if (!c || !net) return GPIS_INVALID_ARGUMENT;
     if (!net->calibrated) return nfail(net, GPIS_INVALID_ARGUMENT, "calibration not set");
-    if (!c->P.one_d) return fail(c, GPIS_INVALID_ARGUMENT, "the medium is not 1D_sampling");
+    if (gpis_status ms = surrogate_medium_ok(c)) return ms;
     if (!rays || !reply || !reply->t || !reply->value || !reply->gx || !reply->gy || !reply->gz || !reply->outcome)
         return fail(c, GPIS_INVALID_ARGUMENT, "null ray or reply field");
     if (n < 0 || n > kNoAllocMaxRays)
@@ -2910,7 +3163,7 @@ gpis_status gpis_surrogate_flights_1d_noalloc(gpis_ctx *c, gpis_net *net, const 
                   (const float *)rays->jitter, (const uint32_t *)rays->flags, o, st);
     if (s != GPIS_OK) return s;
     ReplyOut out = {reply->t, reply->value, reply->gx, reply->gy, reply->gz, reply->outcome, reply->evals};
-    s = launch_flight1d_groups(c, rays, n, out, fallback, counts, st, counts + 1);
+    s = launch_served_and_fallback(c, net, rays, n, out, fallback, counts, st, counts + 1);
     if (s != GPIS_OK) return s;
     if (n_fallback) {
         HIP_TRY(c, hipMemcpyAsync(n_fallback, counts, sizeof(uint32_t), hipMemcpyDeviceToHost, st));

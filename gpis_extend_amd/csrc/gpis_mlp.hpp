@@ -371,6 +371,9 @@ __global__ void adam_kernel(const float *__restrict__ part, int chunks, float in
 }
 
 // ---------------------------------------------------------------- inference --
+// Outcome word of a served row until its endpoint is completed: this bit and the class.
+constexpr uint32_t kServedPending = 0x80000000u;
+
 struct InferOut {
     float *t;
     float *value;
@@ -382,7 +385,13 @@ struct InferOut {
     uint32_t *n_fallback;  // their count
 };
 
-constexpr int kInferWaves = 8;  // waves per block; one block per CU holds the weights in LDS
+#ifndef GPIS_INFER_WAVES
+#define GPIS_INFER_WAVES 12
+#endif
+// waves per block (3 per SIMD: the weights' 112 KB of LDS allow one block per CU, and the
+// epilogue's dependent MFMA, exponential and reciprocal chains need the third wave to hide
+// their latency; 2 per SIMD measured 8% slower, 4 no faster)
+constexpr int kInferWaves = GPIS_INFER_WAVES;
 constexpr int kNT = 2;          // 16-row tiles per wave and pass (weight-fragment reuse)
 
 // The persistent inference kernel: features -> standardise -> MLP -> CDF -> a sampled
@@ -440,9 +449,15 @@ __global__ __launch_bounds__(kInferWaves * 64) void infer_kernel(
             bf16x8 nb[kNT][4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
+                // the accumulators start at the bias of their four output features
+                const f32x4 b0 = *(const f32x4 *)(bias + l * kWidth + 16 * (2 * c) + 4 * g);
+                const f32x4 b1 = *(const f32x4 *)(bias + l * kWidth + 16 * (2 * c + 1) + 4 * g);
                 f32x4 acc[kNT][2];
 #pragma unroll
-                for (int nt = 0; nt < kNT; ++nt) acc[nt][0] = acc[nt][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+                for (int nt = 0; nt < kNT; ++nt) {
+                    acc[nt][0] = b0;
+                    acc[nt][1] = b1;
+                }
 #pragma unroll
                 for (int kc = 0; kc < 4; ++kc) {
                     if (kc < KC) {
@@ -458,11 +473,9 @@ __global__ __launch_bounds__(kInferWaves * 64) void infer_kernel(
 #pragma unroll
                 for (int h = 0; h < 2; ++h)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const float bb = bias[l * kWidth + 16 * (2 * c + h) + 4 * g + r];
+                    for (int r = 0; r < 4; ++r)
 #pragma unroll
-                        for (int nt = 0; nt < kNT; ++nt) nb[nt][c][h * 4 + r] = (bf16)gelu(acc[nt][h][r] + bb);
-                    }
+                        for (int nt = 0; nt < kNT; ++nt) nb[nt][c][h * 4 + r] = (bf16)gelu(acc[nt][h][r]);
             }
 #pragma unroll
             for (int nt = 0; nt < kNT; ++nt)
@@ -553,28 +566,27 @@ __global__ __launch_bounds__(kInferWaves * 64) void infer_kernel(
                             cum += e[mt][r];
                         }
                         const float nt0 = near_t[row], ft = far_t[row];
-                        float t, val = 0.f;
-                        uint32_t oc;
+                        float t;
                         if (cls >= kBins) {
-                            t = ft;
-                            oc = 0u;  // exit
+                            t = ft;  // exit
                         } else {
                             float fr = pc > 0.f ? (u - c0) / pc : 0.f;
                             fr = fminf(fmaxf(fr, 0.f), 1.f);
                             const float s = C.edges[cls] + fr * (C.edges[cls + 1] - C.edges[cls]);
                             t = nt0 + s * (ft - nt0);
-                            oc = 1u;  // crossing
                         }
+                        // the distance and the class; the endpoint (gradient, outcome, the
+                        // value at an exit) is completed by served_finish_kernel
                         out.t[row] = t;
-                        out.value[row] = val;
-                        out.gx[row] = out.gy[row] = out.gz[row] = 0.f;
-                        out.outcome[row] = oc;
+                        out.outcome[row] = kServedPending | (uint32_t)cls;
                         if (out.evals) out.evals[row] = 0u;
                     }
                 }
             }
-            // compaction of the fallback rows: one atomic per wave, ranks by ballot
+            // compaction of the fallback rows: one atomic per wave, ranks by ballot; their
+            // outcome word holds no pending bit until the plain method replies
             const bool mine = sample && fallback && row < n && g == 0;
+            if (mine) out.outcome[row] = 3u;  // GPIS_INVALID
             const unsigned long long mask = __ballot(mine);
             if (mask) {
                 const unsigned cnt = (unsigned)__popcll(mask);

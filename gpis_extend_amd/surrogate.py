@@ -47,23 +47,31 @@ class Calibration:
         return c
 
 
-def minimax_edges(samples: np.ndarray, n_bins: int = N_BINS, grid: int = 20001) -> np.ndarray:
-    """Bin edges over [0, 1] that minimise the largest gap between the empirical CDF of
-    `samples` and its piecewise-linear interpolation through the edges (bins of
-    uniform density): bisection on the tolerance, each candidate filled greedily
-    with the longest bins that keep the gap within it.  Quantile edges leave one
-    wide tail bin whose linearisation error alone exceeds 0.01 in KS."""
-    sc = np.sort(np.clip(np.asarray(samples, np.float64), 0.0, 1.0))
-    n = len(sc)
-    xs = np.unique(np.concatenate([np.linspace(0.0, 1.0, grid), np.quantile(sc, np.linspace(0, 1, grid))]))
-    Fr = np.searchsorted(sc, xs, side="right") / n
-    Fl = np.searchsorted(sc, xs, side="left") / n
+def minimax_edges(samples: np.ndarray, n_bins: int = N_BINS, grid: int = 20001, strata=None) -> np.ndarray:
+    """Bin edges over [0, 1] that minimise the largest gap between an empirical CDF of
+    `samples` and its piecewise-linear interpolation through the edges (bins of uniform
+    density): bisection on the tolerance, each candidate filled greedily with the longest
+    bins that keep the gap within it.  With `strata` (an integer label per sample) the gap
+    is the largest over the strata's own CDFs, so the bins are fine wherever any stratum's
+    distribution bends: a flight with a short length scale crosses within a few
+    hundredths of its bound, where the pooled CDF is nearly straight.  Quantile edges of
+    the pooled sample leave one wide tail bin whose linearisation error alone exceeds
+    0.01 in KS."""
+    x = np.clip(np.asarray(samples, np.float64), 0.0, 1.0)
+    lab = np.zeros(len(x), np.int64) if strata is None else np.asarray(strata)
+    groups = [np.sort(x[lab == k]) for k in np.unique(lab)]
+    pooled = np.sort(x)
+    pts = [np.linspace(0.0, 1.0, grid), np.quantile(pooled, np.linspace(0, 1, grid))]
+    if strata is not None:
+        pts.append(np.geomspace(1e-4, 1.0, grid // 2))
+    xs = np.unique(np.concatenate(pts))
+    Fr = np.stack([np.searchsorted(g, xs, side="right") / len(g) for g in groups])
+    Fl = np.stack([np.searchsorted(g, xs, side="left") / len(g) for g in groups])
 
     def gap(i, j):
-        seg = slice(i, j + 1)
         a, b = xs[i], xs[j]
-        model = Fr[i] + (Fl[j] - Fr[i]) * (xs[seg] - a) / max(b - a, 1e-300)
-        return max(np.max(np.abs(Fr[seg][:-1] - model[:-1])), np.max(np.abs(Fl[seg][1:] - model[1:])))
+        model = Fr[:, i:i + 1] + (Fl[:, j:j + 1] - Fr[:, i:i + 1]) * (xs[i:j + 1] - a) / max(b - a, 1e-300)
+        return max(np.max(np.abs(Fr[:, i:j] - model[:, :-1])), np.max(np.abs(Fl[:, i + 1:j + 1] - model[:, 1:])))
 
     def fill(eps):
         idx, i = [0], 0
@@ -82,7 +90,7 @@ def minimax_edges(samples: np.ndarray, n_bins: int = N_BINS, grid: int = 20001) 
         return idx
 
     lo, hi = 1e-6, 0.5
-    for _ in range(40):
+    for _ in range(32):
         mid = (lo * hi) ** 0.5
         if fill(mid) is None:
             lo = mid
@@ -100,9 +108,14 @@ def minimax_edges(samples: np.ndarray, n_bins: int = N_BINS, grid: int = 20001) 
     return e
 
 
-def calibrate(feats, s_cross, n_sample: int = 1 << 19) -> Calibration:
-    """Standardisation, hull, the atom class at s = 0 and minimax bin edges from training
-    features and the normalised distances of the crossings among them."""
+# features whose quantile strata shape the bin edges: value, directional derivative, length scale
+EDGE_STRATA_FEATURES = (0, 1, 2)
+
+
+def calibrate(feats, s, exited, n_sample: int = 1 << 19, strata_bins: int = 3) -> Calibration:
+    """Standardisation, hull, the atom class at s = 0 and the bin edges from the training
+    features and normalised distances: 30 bins over the continuous part of the crossings,
+    minimax over the quantile strata of (value, derivative, length scale)."""
     import torch
     f = feats.double()
     mean = f.mean(0).cpu().numpy()
@@ -110,15 +123,19 @@ def calibrate(feats, s_cross, n_sample: int = 1 << 19) -> Calibration:
     inv_std = np.where(std > 1e-12, 1.0 / np.maximum(std, 1e-30), 1.0)
     lo = feats.min(0).values.cpu().numpy().astype(np.float64)
     hi = feats.max(0).values.cpu().numpy().astype(np.float64)
-    s = s_cross
-    if s.numel() > n_sample:
-        g = torch.Generator(device=s.device)
+    cross = ~exited
+    sc, fc = s[cross], feats[cross]
+    if sc.numel() > n_sample:
+        g = torch.Generator(device=sc.device)
         g.manual_seed(SEED_TRAIN)
-        s = s[torch.randperm(s.numel(), device=s.device, generator=g)[:n_sample]]
-    sv = s.float().cpu().numpy()
-    # class 0: the atom at s = 0 (crossings returned at the ray origin); 30 minimax bins
-    # over the continuous part
-    edges = np.concatenate([[0.0], minimax_edges(sv[sv > 0], n_bins=N_BINS - 1)]).astype(np.float32)
+        pick = torch.randperm(sc.numel(), device=sc.device, generator=g)[:n_sample]
+        sc, fc = sc[pick], fc[pick]
+    sv = sc.float().cpu().numpy()
+    sid = strata_of(fc[:, list(EDGE_STRATA_FEATURES)].cpu().numpy(), strata_bins)
+    pos = sv > 0
+    # class 0: the atom at s = 0 (crossings returned at the ray origin); 30 bins after it
+    edges = np.concatenate([[0.0], minimax_edges(sv[pos], n_bins=N_BINS - 1, grid=4001, strata=sid[pos])])
+    edges = edges.astype(np.float32)
     for b in range(2, N_BINS + 1):  # strictly increasing in float32 after the atom
         if edges[b] <= edges[b - 1]:
             edges[b] = np.nextafter(edges[b - 1], np.float32(2.0))
@@ -329,6 +346,51 @@ def ks_two_sample(a: np.ndarray, b: np.ndarray) -> float:
     fa = np.searchsorted(a, grid, side="right") / len(a)
     fb = np.searchsorted(b, grid, side="right") / len(b)
     return float(np.max(np.abs(fa - fb)))
+
+
+def strata_of(keys: np.ndarray, bins: int = 4) -> np.ndarray:
+    """Stratum index of each row: the quantile bin (of `bins`) of every column of keys."""
+    keys = np.asarray(keys, np.float64)
+    sid = np.zeros(keys.shape[0], np.int64)
+    for j in range(keys.shape[1]):
+        qs = np.quantile(keys[:, j], np.linspace(0.0, 1.0, bins + 1)[1:-1])
+        sid = sid * bins + np.searchsorted(qs, keys[:, j], side="right")
+    return sid
+
+
+def stratified_ks(sid: np.ndarray, a: np.ndarray, b: np.ndarray, min_count: int = 2000) -> Dict[str, float]:
+    """Two-sample KS distance between a and b (paired rows) inside each stratum: the
+    count-weighted mean over strata, the largest, and the number of strata used."""
+    order = np.argsort(sid, kind="stable")
+    sids = sid[order]
+    cuts = np.flatnonzero(np.diff(sids)) + 1
+    tot, wsum, worst, used = 0.0, 0, 0.0, 0
+    for grp in np.split(order, cuts):
+        if len(grp) < min_count:
+            continue
+        k = ks_two_sample(a[grp], b[grp])
+        tot += k * len(grp)
+        wsum += len(grp)
+        worst = max(worst, k)
+        used += 1
+    return {"mean": tot / max(wsum, 1), "max": worst, "strata": used}
+
+
+def image_bias(samples_a: np.ndarray, samples_b: np.ndarray) -> Tuple[float, float]:
+    """Noise-corrected relative RMS bias between two images given as (pixels, spp) samples,
+    and the bias below which it cannot tell the images apart.  With per-pixel means m and
+    unbiased sample variances v, d2 = mean over pixels of (ma - mb)^2 - va/spp - vb/spp has
+    expectation mean((bias)^2) whatever the noise; returns sqrt(max(d2, 0)) / mean(mb) and
+    sqrt(2 * standard error of d2) / mean(mb)."""
+    a = np.asarray(samples_a, np.float64)
+    b = np.asarray(samples_b, np.float64)
+    ma, mb = a.mean(1), b.mean(1)
+    va, vb = a.var(1, ddof=1), b.var(1, ddof=1)
+    terms = (ma - mb) ** 2 - va / a.shape[1] - vb / b.shape[1]
+    d2 = float(terms.mean())
+    se = float(terms.std() / math.sqrt(len(terms)))
+    scale = max(abs(float(mb.mean())), 1e-12)
+    return math.sqrt(max(d2, 0.0)) / scale, math.sqrt(2.0 * se) / scale
 
 
 def pit_ks(probs: np.ndarray, edges: np.ndarray, s: np.ndarray, exited: np.ndarray, rng) -> float:

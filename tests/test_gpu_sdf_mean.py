@@ -45,6 +45,41 @@ def test_mean_and_gradient_bit_exact(func):
     assert not bad.any()
 
 
+def test_value_rounds_the_noise_product_plus_the_double_mean():
+    """sigma > 0: the value is (float)((double)(amplitude * noise) + mean) with the mean in
+    double, the gradient amplitude * noise_gradient + (float)mean_gradient unfused
+    (SparseConvolutionNoise.cpp:113-140).  The noise term comes from the same field over a
+    zero mean (a plane of normal 0), whose value and gradient are the float products
+    themselves; a fused multiply-add or a mean rounded to float first changes the bits."""
+    from gpis_extend_amd import GpisDevice, SparseConvolutionNoiseMedium, workloads as W
+    print("case: eval_points sigma 0.7, knob mean against a zero mean, 8192 points", flush=True)
+    tf = {"position": [0.05, -0.1, 0.02], "scale": 0.8, "rotation": [20.0, -35.0, 10.0]}
+    js_proc = W.with_procedural_mean(W.config0_json(), "knob", tf, scale=0.7, offset=0.01)
+    js_zero = W.config0_json()
+    js_zero["gaussian_process"]["mean"] = {"type": "plane", "normal": [0.0, 0.0, 0.0], "offset": 0.0}
+    for js in (js_proc, js_zero):
+        js["gaussian_process"]["covariance"]["sigma"] = 0.7
+    x = W.points_workload(8192, seed=W.SEED_POINTS) * np.float32(1.2)
+    xt = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    out = []
+    for js in (js_proc, js_zero):
+        med = SparseConvolutionNoiseMedium.from_json(js)
+        dev = GpisDevice(med)
+        v, gx, gy, gz = dev.eval_points(xt[:, 0].contiguous(), xt[:, 1].contiguous(), xt[:, 2].contiguous(),
+                                        med.seed)
+        out.append(torch.stack([v, gx, gy, gz], 1).cpu().numpy())
+        dev.close()
+    got, noise = out
+    m = mean(oracle_desc(SparseConvolutionNoiseMedium.from_json(js_proc)), x.astype(np.float64))
+    want = np.empty_like(got)
+    want[:, 0] = (noise[:, 0].astype(np.float64) + m[:, 0]).astype(np.float32)
+    want[:, 1:] = noise[:, 1:] + m[:, 1:].astype(np.float32)
+    assert np.abs(noise[:, 0]).max() > 0.1  # the noise term is present
+    bad = np.any(got.view(np.uint32) != want.view(np.uint32), axis=1)
+    print(f"{bad.sum()} of {len(x)} points differ")
+    assert not bad.any()
+
+
 def _flights(js, rays, domain=None):
     from gpis_extend_amd import GpisDevice, RayBatch, SparseConvolutionNoiseMedium
     med = SparseConvolutionNoiseMedium.from_json(js)

@@ -10,7 +10,10 @@ plain method it stands in for:
   frequencies): what the trained network gains over the method without it;
 * surrogate flights follow the plain method's distance distribution on fresh rays,
   and every ray that falls back (unconditioned, outside the training hull) gets
-  the plain method's reply bit for bit.
+  the plain method's reply bit for bit;
+* a served flight's endpoint (gradient, outcome by the rejection rules, the value at
+  an exit) follows the law of the conditioned process that oracle/served_oracle.py
+  restates (pinned on the CPU by tests/test_served_oracle.py).
 """
 import math
 
@@ -112,18 +115,18 @@ def test_adam_step_matches_oracle_update():
 
 @pytest.fixture(scope="module")
 def trained():
-    """A network trained briefly (256 Adam steps) on 2^20 plain-method flights of the
-    renewal workload."""
+    """A network trained on 2^22 plain-method flights of the renewal workload (8 epochs,
+    512 Adam steps of 65,536)."""
     from gpis_extend_amd import GpisDevice, SparseConvolutionNoiseMedium, workloads as W
     from gpis_extend_amd import surrogate as S
     med = SparseConvolutionNoiseMedium.from_json(W.config3_json())
     dev = GpisDevice(med)
     net = S.FlightSurrogate(seed=S.SEED_TRAIN)
-    feats, s, ex, oc = S.flight_dataset(dev, net, med, 1 << 20, seed=S.SEED_TRAIN)
-    cal = S.calibrate(feats, s[~ex])
+    feats, s, ex, oc = S.flight_dataset(dev, net, med, 1 << 22, seed=S.SEED_TRAIN)
+    cal = S.calibrate(feats, s, ex)
     net.set_calibration(cal)
     cls = S.classes_of(net, s, oc)
-    losses = net.train(feats, cls, epochs=4, batch=16384, lr=1e-3, seed=S.SEED_TRAIN)
+    losses = net.train(feats, cls, epochs=8, batch=65536, lr=1e-3, seed=S.SEED_TRAIN)
     counts = torch.bincount(cls[cls >= 0].long(), minlength=32).double()
     pm = counts / counts.sum()
     marginal_ce = float(-(pm[pm > 0] * pm[pm > 0].log()).sum())
@@ -141,20 +144,45 @@ def test_training_gains_over_marginal(trained):
     assert losses[-1] < mce - 0.2
 
 
-def test_surrogate_flights_follow_plain_distribution(trained):
+def test_surrogate_flights_follow_plain_distribution_per_stratum(trained):
+    """The conditional gate: the KS distance between surrogate and plain distances inside
+    each of the 27 quantile strata of (value, derivative, length scale), against the same
+    statistic between two plain renders with independent realizations (its noise floor).
+    An input-free sampler (the pooled law of the plain distances for every ray) passes the
+    pooled KS but fails this gate."""
     from gpis_extend_amd import RayBatch, workloads as W
     from gpis_extend_amd import surrogate as S
     dev, net, med = trained["dev"], trained["net"], trained["med"]
-    rays = W.renewal1d_rays(med, n=1 << 18, seed=S.SEED_INFER)
+    n = 1 << 20
+    print(f"case: conditional KS gate, renewal workload, {n} rays seed {S.SEED_INFER}", flush=True)
+    rays = W.renewal1d_rays(med, n=n, seed=S.SEED_INFER)
     rb = RayBatch.from_numpy(rays)
     plain = dev.trace_flights_1d(rb)
+    rays2 = dict(rays)
+    rays2["noise_seed"] = (rays["noise_seed"] ^ np.uint32(0x9E3779B9)).astype(np.uint32)
+    rays2["normal_seed"] = (rays["normal_seed"] ^ np.uint32(0x85EBCA6B)).astype(np.uint32)
+    rays2["jitter"] = np.random.default_rng(99).uniform(size=n).astype(np.float32)
+    rb2 = RayBatch.from_numpy(rays2)
+    plain2 = dev.trace_flights_1d(rb2)
     sur, nfb = net.flights(dev, rb)
+    fe = net.features(dev, rb)
     torch.cuda.synchronize()
-    sp, _, _ = S.normalised_distance(rb, plain)
-    ss, _, _ = S.normalised_distance(rb, sur)
-    ks = S.ks_two_sample(sp.cpu().numpy(), ss.cpu().numpy())
-    print(f"surrogate vs plain on {rb.n} fresh rays: KS {ks:.4f}, fallback {nfb}")
-    assert ks < 0.05
+    sp = S.normalised_distance(rb, plain)[0].cpu().numpy()
+    sp2 = S.normalised_distance(rb2, plain2)[0].cpu().numpy()
+    ss = S.normalised_distance(rb, sur)[0].cpu().numpy()
+    sid = S.strata_of(fe.cpu().numpy()[:, list(S.EDGE_STRATA_FEATURES)], 3)
+    got = S.stratified_ks(sid, sp, ss)
+    floor = S.stratified_ks(sid, sp, sp2)
+    free_s = sp[np.random.default_rng(3).permutation(n)]
+    free = S.stratified_ks(sid, sp, free_s)
+    pooled = S.ks_two_sample(sp, ss)
+    print(f"strata KS mean/max: surrogate {got['mean']:.4f}/{got['max']:.4f}, plain vs plain "
+          f"{floor['mean']:.4f}/{floor['max']:.4f}, input-free {free['mean']:.4f}/{free['max']:.4f}; "
+          f"pooled KS {pooled:.4f}; fallback {nfb}")
+    assert got["strata"] == 27
+    assert got["mean"] <= floor["mean"] + 0.01 and got["max"] <= floor["max"] + 0.02
+    assert S.ks_two_sample(sp, free_s) < 0.01  # the input-free sampler passes the pooled test
+    assert free["mean"] > floor["mean"] + 0.05  # ... and fails the conditional one
 
 
 def test_fallback_rays_get_the_plain_reply(trained):
@@ -179,14 +207,124 @@ def test_fallback_rays_get_the_plain_reply(trained):
         a = plain[k].cpu().numpy()[must]
         b = sur[k].cpu().numpy()[must]
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), k
-    # surrogate-served rays: a crossing inside the bound or an exit at far
+    # surrogate-served rays: a hit or an outward failure inside the bound, or an exit at far
     served = ~must
     oc = sur["outcome"].cpu().numpy() & 0xFF
     t = sur["t"].cpu().numpy()
     ev = sur["evals"].cpu().numpy()
     sv = served & (ev == 0)
     assert sv.sum() > 0.5 * served.sum()
-    assert np.all((oc[sv] == 1) | (oc[sv] == 0))
+    assert np.all((oc[sv] == 0) | (oc[sv] == 1) | (oc[sv] == 2))
     far = rays["far_t"]
     assert np.all(t[sv & (oc == 0)] == far[sv & (oc == 0)])
-    assert np.all((t[sv & (oc == 1)] >= 0) & (t[sv & (oc == 1)] <= far[sv & (oc == 1)]))
+    cross = sv & (oc != 0)
+    assert np.all((t[cross] >= 0) & (t[cross] <= far[cross]))
+
+
+def _served_law_inputs(rays, t, lc):
+    """Per ray: a, b of the observations, the lag of t, the mean and its slope at t (the
+    renewal workload's sphere mean of radius 0.5, sigma 1)."""
+    o = np.stack([rays["ox"], rays["oy"], rays["oz"]], 1).astype(np.float64)
+    d = np.stack([rays["dx"], rays["dy"], rays["dz"]], 1).astype(np.float64)
+    g = np.stack([rays["prev_gx"], rays["prev_gy"], rays["prev_gz"]], 1).astype(np.float64)
+    ro = np.linalg.norm(o, axis=1)
+    a = rays["prev_value"].astype(np.float64) - (ro - 0.5)
+    b = lc * np.einsum("ij,ij->i", g - o / ro[:, None], d)
+    q = o + t[:, None].astype(np.float64) * d
+    rq = np.linalg.norm(q, axis=1)
+    m = rq - 0.5
+    ms = np.einsum("ij,ij->i", q / rq[:, None], d)
+    return a, b, m, ms, d
+
+
+def test_served_endpoints_follow_their_law(trained):
+    """The endpoint of a served flight: outcomes follow the rejection rules (a hit's
+    gradient faces the ray, an outward failure's points along it); a crossing's
+    derivative along the ray follows the Rice law of the conditioned process given
+    f(t) = 0 on the sense's side (the PIT through oracle/served_oracle.py's CDF is
+    uniform); the sense follows the law of the first lattice point's value; the atom at
+    the origin returns the observed derivative; an exit's value lies on the sense's
+    side with the truncated law; the transverse gradient is the plain method's draw of
+    the same seed."""
+    from gpis_extend_amd import RayBatch, workloads as W
+    from gpis_extend_amd import surrogate as S
+    from oracle import served_oracle as so
+    from scipy import special
+    dev, net, med = trained["dev"], trained["net"], trained["med"]
+    n = 1 << 18
+    print(f"case: served endpoints, renewal workload, {n} rays seed 61", flush=True)
+    rays = W.renewal1d_rays(med, n=n, seed=61)
+    rb = RayBatch.from_numpy(rays)
+    plain = dev.trace_flights_1d(rb)
+    sur, nfb = net.flights(dev, rb)
+    torch.cuda.synchronize()
+    P = {k: v.cpu().numpy() for k, v in plain.items()}
+    R = {k: v.cpu().numpy() for k, v in sur.items()}
+    served = R["evals"] == 0
+    oc = R["outcome"] & 0xFF
+    reason = (R["outcome"] >> 8) & 0xFF
+    t = R["t"]
+    g = np.stack([R["gx"], R["gy"], R["gz"]], 1).astype(np.float64)
+    assert np.all(np.isfinite(g[served])) and np.all(np.isfinite(t[served]))
+    lc = rays["length_scale"].astype(np.float64) / np.sqrt(2.0)
+    a, b, m, ms, d = _served_law_inputs(rays, t, lc)
+    gd = np.einsum("ij,ij->i", g, d)
+    hit, fail, ex = served & (oc == 1), served & (oc == 2), served & (oc == 0)
+    assert np.all(gd[hit] <= 0) and np.all(gd[fail] > 0) and np.all(reason[fail] == 1)
+    # the atom: the observed derivative along the ray
+    atom = served & (oc != 0) & (t == 0)
+    gobs = (np.stack([rays["prev_gx"], rays["prev_gy"], rays["prev_gz"]], 1).astype(np.float64) * d).sum(1)
+    assert np.max(np.abs(gd[atom] - gobs[atom]), initial=0.0) < 1e-4
+    # crossings: the Rice law on the sense's side
+    cr = served & (oc != 0) & (t > 0)
+    e1, e2, c11, c12, c22 = so.cond_law(t[cr] / lc[cr], a[cr], b[cr], True)
+    mu, var = so.slope_law(e1, e2, c11, c12, c22, -m[cr])
+    muY, sY = ms[cr] + mu / lc[cr], np.sqrt(var) / lc[cr]
+    down = oc[cr] == 1
+    mt = np.where(down, -muY, muY) / sY
+    u = np.sort(so.rice_cdf(np.abs(gd[cr]) / sY, mt))
+    k = len(u)
+    ks = float(np.max(np.maximum(np.arange(1, k + 1) / k - u, u - np.arange(k) / k)))
+    print(f"served crossings {k}: Rice PIT KS {ks:.4f}; hits {hit.sum()} fails {fail.sum()} exits {ex.sum()}")
+    assert ks < 1.63 / math.sqrt(k) + 0.004
+    # the sense: failures among the crossings against the law of f(t0) < 0
+    step = np.minimum(np.float32(med.step_size), (rays["far_t"] - rays["near_t"]) / np.float32(med.min_step))
+    t0 = rays["near_t"] + step * rays["jitter"]
+    a0, b0, m0, _, _ = _served_law_inputs(rays, t0.astype(np.float32), lc)
+    e10, _, c110, _, _ = so.cond_law(t0 / lc, a0, b0, True)
+    p_neg = special.ndtr((-m0 - e10) / np.sqrt(np.maximum(c110, 1e-300)))
+    exp_fail = float(p_neg[cr].sum())
+    var_fail = float((p_neg[cr] * (1 - p_neg[cr])).sum())
+    print(f"failures among crossings {int(fail[cr].sum())}, expected {exp_fail:.1f}")
+    assert abs(fail[cr].sum() - exp_fail) < 5 * math.sqrt(var_fail) + 5
+    # exits: the value on the sense's side, distributed as the truncated conditional law
+    tf = rays["far_t"]
+    af, bf, mf, _, _ = _served_law_inputs(rays, tf, lc)
+    e1f, _, c11f, _, _ = so.cond_law(tf / lc, af, bf, True)
+    v = R["value"].astype(np.float64)
+    z = (v - mf - e1f) / np.sqrt(c11f)
+    alpha = (-mf - e1f) / np.sqrt(c11f)
+    pos = ex & (v > 0)
+    neg = ex & (v < 0)
+    assert pos.sum() + neg.sum() == ex.sum()
+    up = np.concatenate([so.truncated_normal_cdf_above(z[pos], alpha[pos]),
+                         so.truncated_normal_cdf_above(-z[neg], -alpha[neg])])
+    up = np.sort(up)
+    k = len(up)
+    ks = float(np.max(np.maximum(np.arange(1, k + 1) / k - up, up - np.arange(k) / k)))
+    print(f"served exits {k}: truncated value law PIT KS {ks:.4f}")
+    assert ks < 1.63 / math.sqrt(k) + 0.004
+    # the transverse draw is the plain method's: in the ray frame, off the ray direction
+    both = served & (P["t"] > 0) & (t > 0) & ((P["outcome"] & 0xFF) != 3)
+
+    def transverse(reply, tt):
+        q = np.stack([rays["ox"], rays["oy"], rays["oz"]], 1).astype(np.float64) + tt[:, None] * d
+        mg = q / np.linalg.norm(q, axis=1, keepdims=True)
+        w = (np.stack([reply["gx"], reply["gy"], reply["gz"]], 1).astype(np.float64) - mg) * lc[:, None]
+        return w - np.einsum("ij,ij->i", w, d)[:, None] * d
+    idx = np.nonzero(both)[0][:20000]
+    ws = transverse(R, t.astype(np.float64))[idx]
+    wp = transverse(P, P["t"].astype(np.float64))[idx]
+    err = np.abs(ws - wp).max()
+    print(f"transverse gradient, served vs plain of the same seed: max |diff| {err:.2e} over {len(idx)} rays")
+    assert err < 2e-4

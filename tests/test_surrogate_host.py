@@ -54,3 +54,45 @@ def test_atom_class_at_origin():
     sv, ex = M.cdf_sample(p, edges, rng.uniform(size=1000))
     assert np.all((sv == 0.0) | ((sv >= edges[5]) & (sv <= edges[6])))
     assert 0.4 < np.mean(sv == 0.0) < 0.6
+
+
+def test_strata_edges_bound_every_stratum_gap():
+    """Two strata of different scales (short and long length scales): edges minimax over
+    the strata keep each stratum's linearisation gap small, where the pooled edges leave
+    the narrow stratum's distribution inside a few wide bins."""
+    rng = np.random.default_rng(3)
+    narrow = np.clip(rng.gamma(4.0, 0.005, 40000), 0, 1)  # a peaked tenth of the sample
+    wide = np.clip(rng.exponential(0.2, 360000), 0, 1)
+    x = np.concatenate([narrow, wide])
+    lab = np.concatenate([np.zeros(len(narrow), int), np.ones(len(wide), int)])
+    e_pool = S.minimax_edges(x, grid=4001)
+    e_str = S.minimax_edges(x, grid=4001, strata=lab)
+    assert len(e_str) == S.N_BINS + 1 and np.all(np.diff(e_str) > 0)
+    g_pool = max(_linear_gap(narrow, e_pool), _linear_gap(wide, e_pool))
+    g_str = max(_linear_gap(narrow, e_str), _linear_gap(wide, e_str))
+    assert g_str < 0.6 * g_pool, (g_str, g_pool)
+    assert g_str < 5e-3
+
+
+def test_stratified_ks_and_image_bias():
+    rng = np.random.default_rng(4)
+    n = 200000
+    key = rng.uniform(size=(n, 1))
+    a = rng.exponential(1.0, n) * (0.5 + key[:, 0])
+    b = rng.exponential(1.0, n) * (0.5 + key[:, 0])
+    sid = S.strata_of(key, 4)
+    same = S.stratified_ks(sid, a, b)
+    # an input-free sampler (the pooled law for every row) fails the conditional test
+    free = S.stratified_ks(sid, a, b[rng.permutation(n)])
+    assert same["strata"] == 4 and same["mean"] < 0.01
+    assert free["mean"] > 5 * same["mean"] and free["mean"] > 0.05
+    # image bias: zero in expectation for two unbiased renders, the bias when one is shifted
+    px, spp = 4096, 16
+    mu = rng.uniform(0.2, 1.0, (px, 1))
+    A = mu + rng.normal(0, 0.3, (px, spp))
+    B = mu + rng.normal(0, 0.3, (px, spp))
+    rms, limit = S.image_bias(A, B)
+    assert rms <= limit < 0.05, (rms, limit)
+    rms, limit = S.image_bias(A * 1.1, B)
+    want = 0.1 * np.sqrt(np.mean(mu ** 2)) / np.mean(mu)
+    assert rms > limit and abs(rms - want) < 0.2 * want, (rms, want, limit)
