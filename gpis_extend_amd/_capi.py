@@ -36,6 +36,7 @@ EXPORTS = (
     "gpis_net_create", "gpis_net_destroy", "gpis_net_last_error", "gpis_net_set_params", "gpis_net_get_params",
     "gpis_net_set_calibration", "gpis_net_features_1d", "gpis_net_classes_1d", "gpis_net_gradient",
     "gpis_net_train_step", "gpis_net_forward", "gpis_surrogate_flights_1d", "gpis_surrogate_flights_1d_noalloc",
+    "gpis_net_set_averaging", "gpis_net_use_average",
 )
 
 NET_FEATURES, NET_CLASSES, NET_PARAMS = 8, 32, 54816
@@ -180,6 +181,10 @@ def lib():
     L.gpis_net_gradient.argtypes = [_vp, _vp, _vp, C.c_int64, _vp, _vp, _vp]
     L.gpis_net_train_step.restype = st
     L.gpis_net_train_step.argtypes = [_vp, _vp, _vp, C.c_int64, C.c_float, _vp, _vp]
+    L.gpis_net_set_averaging.restype = st
+    L.gpis_net_set_averaging.argtypes = [_vp, C.c_float]
+    L.gpis_net_use_average.restype = st
+    L.gpis_net_use_average.argtypes = [_vp]
     L.gpis_net_forward.restype = st
     L.gpis_net_forward.argtypes = [_vp, _vp, C.c_int64, _vp, _vp]
     L.gpis_surrogate_flights_1d.restype = st

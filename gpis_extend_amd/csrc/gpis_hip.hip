@@ -1650,6 +1650,217 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(
     if (threadIdx.x < ST_N && bstat[threadIdx.x]) atomicAdd(&stats[threadIdx.x], (unsigned long long)bstat[threadIdx.x]);
 }
 
+// ------------------------------------------------- the endpoint of a served flight --
+// The learned sampler returns a distance, not a realization.  resumeDistance still draws a
+// gradient at the endpoint and applies its rejection rules (GaussianProcessMedium.cpp:300-330),
+// and an exit carries the field value at the bound (the next flight's conditioning value,
+// SparseConvolutionNoiseMedium.cpp:339-346).  A served endpoint takes them from the law of
+// the flight's conditioned 1D SE process psi(u) (isotropic ray units u = k s, k =
+// |worldToLocal(d)|, correlation exp(-u^2/4)), f(s) = m(s) + sigma psi(k s), observed at the
+// origin as psi(0) = a and, for Renewal+, psi'(0) = b (conditioning1D,
+// SparseConvolutionNoise.cpp:576-656):
+//   * the crossing's sense is the sign of f at the first lattice point t0 = near + step *
+//     jitter, where a conditioned flight takes its sign reference (:304-306), drawn from its
+//     conditional law;
+//   * at a crossing (f(t) = 0) the derivative along the ray has the Rice density
+//     |y| N(y; mu, s^2) on the sense's side (y < 0: a down-crossing, a hit; y > 0: the
+//     outward crossing the rules reject);
+//   * at the t = 0 atom (the secant's cut-off returns the ray origin) the derivative is the
+//     conditioned one there (b under Renewal+, N(0, 1/2) under Renewal);
+//   * at an exit the value follows the conditional law of f(far) on the sense's side and
+//     the derivative its law given that value.
+// The transverse gradient is the plain method's own Box-Muller draw (finish1d), so the
+// served gradient differs from a plain one of the same ray only along the ray.  The law's
+// draws use the normal seed's PCG stream after the four draws of the plain method's pairs.
+
+// Conditional mean and covariance of (psi(x), psi'(x)) at lag x (isotropic units) given
+// psi(0) = a and, when `plus`, psi'(0) = b.
+struct Law1D { double e1, e2, c11, c12, c22; };
+__device__ __forceinline__ Law1D cond_law1d(double dl, double a, double b, bool plus) {
+    const double x = 0.5 * dl * dl;
+    const double rho = exp(-0.5 * x), r2 = rho * rho;
+    Law1D L;
+    if (plus) {
+        L.e1 = rho * (a + dl * b);
+        L.e2 = rho * (-0.5 * dl * a + (1.0 - x) * b);
+        // 1 - e^{-x} (1 + x) by its series where the difference cancels
+        L.c11 = x < 1e-3 ? x * x * (0.5 + x * (-1.0 / 3.0 + x * (1.0 / 8.0 + x * (-1.0 / 30.0 + x / 144.0))))
+                         : 1.0 - r2 * (1.0 + x);
+        L.c22 = 0.5 * (1.0 - r2 * (1.0 - x + x * x));
+        L.c12 = 0.25 * dl * dl * dl * r2;
+    } else {
+        L.e1 = rho * a;
+        L.e2 = -0.5 * dl * rho * a;
+        L.c11 = -expm1(-x);
+        L.c22 = 0.5 * (1.0 - x * r2);
+        L.c12 = 0.5 * dl * r2;
+    }
+    return L;
+}
+
+// Draws of the endpoint law: uniforms of a PCG stream and Box-Muller normals (FP32).
+struct LawDraws {
+    Pcg g;
+    __device__ __forceinline__ float uni() { return u01(g.next()); }
+    __device__ __forceinline__ float normal() {
+        const float u1 = uni(), u2 = uni();
+        return sqrtf(-2.0f * __logf(1.0f - u1)) * __cosf(6.28318530718f * u2);
+    }
+};
+
+// X >= 0 with density proportional to x exp(-x^2/2 + m x), the size-biased normal N(m, 1)
+// on [0, inf), by rejection: for m < 0 from a Gamma(2) proposal of rate 1/mode, accepted with
+// exp(-(x - mode)^2 / 2); for m >= 0 from a unit normal about m + 1/mode, accepted with
+// (x/mode) exp(1 - x/mode).  Both acceptances are at least about 0.4; one round draws both
+// proposals so the lanes of a wave stay on one path whatever their m.
+__device__ __forceinline__ float rice_draw(float m, LawDraws &D) {
+    m = fminf(fmaxf(m, -1e6f), 1e6f);
+    const float sq = sqrtf(m * m + 4.0f);
+    const float xm = m < 0.f ? 2.0f / (sq - m) : 0.5f * (m + sq);
+    const float inv_xm = 1.0f / xm;
+    const bool gam = m < 0.f;
+    for (int it = 0; it < 64; ++it) {
+        const float u1 = D.uni(), u2 = D.uni(), u3 = D.uni();
+        // Gamma(2): -log(u u') * mode; normal: Box-Muller of (u1, u2)
+        const float lg = __logf((1.0f - u1) * (1.0f - u2));
+        const float xn = m + inv_xm + sqrtf(-2.0f * __logf(1.0f - u1)) * __cosf(6.28318530718f * u2);
+        const float x = gam ? -lg * xm : xn;
+        const float r = x * inv_xm;
+        const float acc = gam ? __expf(-0.5f * (x - xm) * (x - xm)) : r * __expf(1.0f - r);
+        if (x > 0.f && u3 < acc) return x;
+    }
+    return xm;
+}
+
+// Z ~ N(0, 1) restricted to z > alpha (FP32): the inverse CDF for alpha < 3, Robert's
+// exponential rejection above.
+__device__ __forceinline__ float normal_above(float alpha, LawDraws &D) {
+    alpha = fminf(alpha, 1e6f);
+    if (alpha < 3.0f) {
+        const float q = 0.5f * erfcf(alpha * 0.70710678f);  // P(Z > alpha)
+        const float u = 1.0f - D.uni();                      // (0, 1]
+        return 1.41421356f * erfcinvf(2.0f * u * q);
+    }
+    for (int it = 0; it < 64; ++it) {
+        const float z = alpha - __logf(1.0f - D.uni()) / alpha;
+        const float dz = z - alpha;
+        if (D.uni() < __expf(-0.5f * dz * dz)) return z;
+    }
+    return alpha + 1.0f / alpha;
+}
+
+// The endpoint of served row r (its outcome word holds the pending bit and the class).
+template <bool PROC>
+__device__ __forceinline__ void served_finish_row(const FieldParams &P, const RayIn &in, const float *__restrict__ lscale,
+                                               unsigned r, const ReplyOut &out) {
+#pragma clang fp contract(off)
+    const uint32_t code = out.outcome[r];
+    if (!(code & mlp::kServedPending)) return;
+    const int cls = (int)(code & 0xFFu);
+    const float ox = in.ox[r], oy = in.oy[r], oz = in.oz[r];
+    const float dx = in.dx[r], dy = in.dy[r], dz = in.dz[r];
+    const float near_t = in.near_t[r], far_t = in.far_t[r];
+    const uint32_t normal_seed = in.normal_seed[r];
+    Ray1D R = {};
+    ray_scales(P.aniso, P.l_aniso, P.l_aniso_inv, lscale, r, R.la, R.ia);
+    float ix = R.ia[0] * dx, iy = R.ia[1] * dy, iz = R.ia[2] * dz;
+    const double k = sqrt((double)ix * ix + (double)iy * iy + (double)iz * iz);  // du/ds
+    normalize3(ix, iy, iz);
+    R.F = make_frame(ix, iy, iz);
+    R.cond = 1;
+    const double sig = P.amp_const;
+    const bool plus = P.ctxt == GPIS_CTX_RENEWAL_PLUS;
+    const bool corr = P.corr_xy && plus;
+    // the observations at the origin (conditioning1D's targets)
+    float mgx, mgy, mgz;
+    mean_grad_d<PROC>(P, ox, oy, oz, mgx, mgy, mgz);
+    const double a = ((double)in.pv[r] - mean_value_d<PROC>(P, ox, oy, oz)) / sig;
+    const float amp = P.amp_const;
+    const float tgx = (in.pgx[r] - mgx) / amp, tgy = (in.pgy[r] - mgy) / amp, tgz = (in.pgz[r] - mgz) / amp;
+    const float gwx = R.la[0] * tgx, gwy = R.la[1] * tgy, gwz = R.la[2] * tgz;
+    const double b = plus ? (double)dot3(R.F.nx, R.F.ny, R.F.nz, gwx, gwy, gwz) : 0.0;
+    float obs_x = 0.f, obs_y = 0.f;
+    if (plus) {
+        obs_x = dot3(R.F.tx, R.F.ty, R.F.tz, gwx, gwy, gwz);
+        obs_y = dot3(R.F.bx, R.F.by, R.F.bz, gwx, gwy, gwz);
+    }
+    LawDraws D;
+    D.g.set((uint64_t)normal_seed);
+    for (int i = 0; i < 4; ++i) (void)D.g.next();  // past the plain method's two pairs
+    // the sense: the sign of f at the first lattice point
+    float step = (far_t - near_t) / P.min_step_f;
+    if (P.step_size < step) step = P.step_size;
+    const float t0 = mul_add_exact(step, in.jitter[r], near_t);
+    float qx, qy, qz;
+    ray_point(ox, oy, oz, dx, dy, dz, (double)t0, qx, qy, qz);
+    const Law1D L0 = cond_law1d(k * (double)t0, a, b, plus);
+    const double f0 = mean_value_d<PROC>(P, qx, qy, qz) + sig * (L0.e1 + sqrt(fmax(L0.c11, 0.0)) * (double)D.normal());
+    const bool down = !(f0 < 0.0);  // a positive side looks for a down-crossing
+    float t_end, ndv, value = 0.f;
+    int hit = 1;
+    if (cls == 0) {
+        // the secant's cut-off: the ray origin and the conditioned derivative there
+        t_end = 0.f;
+        const Law1D L = cond_law1d(0.0, a, b, plus);
+        ndv = (float)(L.e2 + sqrt(fmax(L.c22, 0.0)) * (double)D.normal());
+    } else {
+        t_end = cls >= mlp::kBins ? far_t : out.t[r];
+        hit = cls < mlp::kBins;
+    }
+    grad_point(ox, oy, oz, dx, dy, dz, (double)t_end, qx, qy, qz);
+    if (cls != 0) {
+        const double m = mean_value_d<PROC>(P, qx, qy, qz);
+        float ex, ey, ez;
+        mean_grad_d<PROC>(P, qx, qy, qz, ex, ey, ez);
+        const double ms = (double)ex * dx + (double)ey * dy + (double)ez * dz;  // mean slope
+        const Law1D L = cond_law1d(k * (double)t_end, a, b, plus);
+        const bool ok = L.c11 > 1e-300;
+        const double q = ok ? L.c12 / L.c11 : 0.0;
+        const double vs = fmax(L.c22 - q * L.c12, 0.0);
+        const double sk = sig * k;
+        if (hit) {
+            // the Rice law of y = f'(t) at f(t) = 0, on the sense's side
+            const double mu = L.e2 + (ok ? q * (-m / sig - L.e1) : 0.0);
+            const double muy = ms + sk * mu, sy = sk * sqrt(vs);
+            double y;
+            if (sy > 1e-30 * (fabs(muy) + 1.0)) {
+                const double w = (double)rice_draw((float)((down ? -muy : muy) / sy), D) * sy;
+                y = down ? -w : w;
+            } else {
+                y = muy;
+            }
+            ndv = (float)((y - ms) / sk);
+        } else {
+            // the value at the bound on the sense's side, then the derivative given it
+            const double sd = sqrt(fmax(L.c11, 0.0));
+            double psi = L.e1;
+            if (sd > 0.0) {
+                const double alpha = (-m / sig - L.e1) / sd;  // psi > x0 keeps f > 0
+                psi = L.e1 + sd * (double)(down ? normal_above((float)alpha, D) : -normal_above((float)-alpha, D));
+            }
+            value = (float)(m + sig * psi);
+            const double mu = L.e2 + (ok ? q * (psi - L.e1) : 0.0);
+            ndv = (float)(mu + sqrt(vs) * (double)D.normal());
+        }
+    }
+    float gx, gy, gz;
+    const uint32_t oc = finish1d<PROC>(P, R, corr, plus, obs_x, obs_y, ox, oy, oz, qx, qy, qz, dx, dy, dz,
+                                       normal_seed, t_end, ndv, hit, gx, gy, gz);
+    out.t[r] = t_end;
+    out.value[r] = value;
+    out.gx[r] = gx;
+    out.gy[r] = gy;
+    out.gz[r] = gz;
+    out.outcome[r] = oc;
+}
+
+template <bool PROC>
+__global__ __launch_bounds__(256) void served_finish_kernel(const FieldParams *__restrict__ Pg, RayIn in,
+                                                            const float *__restrict__ lscale, unsigned n, ReplyOut out) {
+    const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) served_finish_row<PROC>(*Pg, in, lscale, r, out);
+}
+
 // Flights of a short index list (the learned sampler's fallback rays): a group of 16 lanes
 // (four groups per wave) traces one flight.  The march evaluates 16 consecutive lattice
 // points at once, one per lane; conditioning, f(nearT), the secant, f(farT) and the endpoint
@@ -1658,7 +1869,12 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(
 // reports the plain march's count), while a long flight takes about steps/16 evaluation
 // rounds: a launch over a few thousand rays no longer waits ~steps rounds for its longest
 // flight.  Ray indices come from idx[0 .. *n_dev).
-constexpr int kGroup = 16;
+#ifndef GPIS_FALLBACK_GROUP
+#define GPIS_FALLBACK_GROUP 16
+#endif
+constexpr int kGroup = GPIS_FALLBACK_GROUP;  // lanes per flight: 16, 32 or 64
+static_assert(kGroup == 16 || kGroup == 32 || kGroup == 64, "a group is a power-of-two part of a wave");
+constexpr unsigned long long kGroupMask = kGroup == 64 ? ~0ull : ((1ull << kGroup) - 1ull);
 template <int NI, bool PROC, bool FAST>
 __global__ __launch_bounds__(256, 2) void flight1d_group_kernel(const FieldParams *__restrict__ Pg, RayIn in,
                                                                 const float *__restrict__ lscale, ReplyOut out,
@@ -1805,11 +2021,11 @@ __global__ __launch_bounds__(256, 2) void flight1d_group_kernel(const FieldParam
             // a conditioned flight's first march point only sets the reference sign
             if (!first && kb == 0) sign0 = __shfl(sc, (int)gshift);
             const bool cand = valid && sc != sign0 && (first || kb + sub != 0);
-            const unsigned cm = (unsigned)((__ballot(cand) >> gshift) & 0xFFFFull);
-            const unsigned vm = (unsigned)((__ballot(valid) >> gshift) & 0xFFFFull);
+            const unsigned long long cm = (__ballot(cand) >> gshift) & kGroupMask;
+            const unsigned long long vm = (__ballot(valid) >> gshift) & kGroupMask;
             const float f_last = __shfl(fv, (int)(gshift + kGroup - 1));
             if (cm) {
-                const unsigned j = (unsigned)(__ffs((int)cm) - 1);
+                const unsigned j = (unsigned)(__ffsll((long long)cm) - 1);
                 const float fj = __shfl(fv, (int)(gshift + j));
                 const float fjm1 = __shfl(fv, (int)(gshift + (j > 0 ? j - 1 : 0)));
                 const float pfx = j > 0 ? fjm1 : pf;
@@ -1824,8 +2040,8 @@ __global__ __launch_bounds__(256, 2) void flight1d_group_kernel(const FieldParam
                 phase = PH_SECANT;
                 tq = tt;
                 unit = false;
-            } else if (vm != 0xFFFFu) {
-                evals_m = kb + (unsigned)__popc(vm);
+            } else if (vm != kGroupMask) {
+                evals_m = kb + (unsigned)__popcll(vm);
                 k = evals_m;
                 phase = PH_FAR;
                 tq = (double)far_t;
@@ -2531,212 +2747,6 @@ __global__ void classes1d_kernel(RayIn in, const float *__restrict__ t, const ui
     cls[r] = c;
 }
 
-// ------------------------------------------------- the endpoint of a served flight --
-// The learned sampler returns a distance, not a realization.  resumeDistance still draws a
-// gradient at the endpoint and applies its rejection rules (GaussianProcessMedium.cpp:300-330),
-// and an exit carries the field value at the bound (the next flight's conditioning value,
-// SparseConvolutionNoiseMedium.cpp:339-346).  A served endpoint takes them from the law of
-// the flight's conditioned 1D SE process psi(u) (isotropic ray units u = k s, k =
-// |worldToLocal(d)|, correlation exp(-u^2/4)), f(s) = m(s) + sigma psi(k s), observed at the
-// origin as psi(0) = a and, for Renewal+, psi'(0) = b (conditioning1D,
-// SparseConvolutionNoise.cpp:576-656):
-//   * the crossing's sense is the sign of f at the first lattice point t0 = near + step *
-//     jitter, where a conditioned flight takes its sign reference (:304-306), drawn from its
-//     conditional law;
-//   * at a crossing (f(t) = 0) the derivative along the ray has the Rice density
-//     |y| N(y; mu, s^2) on the sense's side (y < 0: a down-crossing, a hit; y > 0: the
-//     outward crossing the rules reject);
-//   * at the t = 0 atom (the secant's cut-off returns the ray origin) the derivative is the
-//     conditioned one there (b under Renewal+, N(0, 1/2) under Renewal);
-//   * at an exit the value follows the conditional law of f(far) on the sense's side and
-//     the derivative its law given that value.
-// The transverse gradient is the plain method's own Box-Muller draw (finish1d), so the
-// served gradient differs from a plain one of the same ray only along the ray.  The law's
-// draws use the normal seed's PCG stream after the four draws of the plain method's pairs.
-
-// Conditional mean and covariance of (psi(x), psi'(x)) at lag x (isotropic units) given
-// psi(0) = a and, when `plus`, psi'(0) = b.
-struct Law1D { double e1, e2, c11, c12, c22; };
-__device__ __forceinline__ Law1D cond_law1d(double dl, double a, double b, bool plus) {
-    const double x = 0.5 * dl * dl;
-    const double rho = exp(-0.5 * x), r2 = rho * rho;
-    Law1D L;
-    if (plus) {
-        L.e1 = rho * (a + dl * b);
-        L.e2 = rho * (-0.5 * dl * a + (1.0 - x) * b);
-        // 1 - e^{-x} (1 + x) by its series where the difference cancels
-        L.c11 = x < 1e-3 ? x * x * (0.5 + x * (-1.0 / 3.0 + x * (1.0 / 8.0 + x * (-1.0 / 30.0 + x / 144.0))))
-                         : 1.0 - r2 * (1.0 + x);
-        L.c22 = 0.5 * (1.0 - r2 * (1.0 - x + x * x));
-        L.c12 = 0.25 * dl * dl * dl * r2;
-    } else {
-        L.e1 = rho * a;
-        L.e2 = -0.5 * dl * rho * a;
-        L.c11 = -expm1(-x);
-        L.c22 = 0.5 * (1.0 - x * r2);
-        L.c12 = 0.5 * dl * r2;
-    }
-    return L;
-}
-
-// Draws of the endpoint law: uniforms of a PCG stream and Box-Muller normals (FP32).
-struct LawDraws {
-    Pcg g;
-    __device__ __forceinline__ float uni() { return u01(g.next()); }
-    __device__ __forceinline__ float normal() {
-        const float u1 = uni(), u2 = uni();
-        return sqrtf(-2.0f * __logf(1.0f - u1)) * __cosf(6.28318530718f * u2);
-    }
-};
-
-// X >= 0 with density proportional to x exp(-x^2/2 + m x), the size-biased normal N(m, 1)
-// on [0, inf), by rejection: for m < 0 from a Gamma(2) proposal of rate 1/mode, accepted with
-// exp(-(x - mode)^2 / 2); for m >= 0 from a unit normal about m + 1/mode, accepted with
-// (x/mode) exp(1 - x/mode).  Both acceptances are at least about 0.4; one round draws both
-// proposals so the lanes of a wave stay on one path whatever their m.
-__device__ __forceinline__ float rice_draw(float m, LawDraws &D) {
-    m = fminf(fmaxf(m, -1e6f), 1e6f);
-    const float sq = sqrtf(m * m + 4.0f);
-    const float xm = m < 0.f ? 2.0f / (sq - m) : 0.5f * (m + sq);
-    const float inv_xm = 1.0f / xm;
-    const bool gam = m < 0.f;
-    for (int it = 0; it < 64; ++it) {
-        const float u1 = D.uni(), u2 = D.uni(), u3 = D.uni();
-        // Gamma(2): -log(u u') * mode; normal: Box-Muller of (u1, u2)
-        const float lg = __logf((1.0f - u1) * (1.0f - u2));
-        const float xn = m + inv_xm + sqrtf(-2.0f * __logf(1.0f - u1)) * __cosf(6.28318530718f * u2);
-        const float x = gam ? -lg * xm : xn;
-        const float r = x * inv_xm;
-        const float acc = gam ? __expf(-0.5f * (x - xm) * (x - xm)) : r * __expf(1.0f - r);
-        if (x > 0.f && u3 < acc) return x;
-    }
-    return xm;
-}
-
-// Z ~ N(0, 1) restricted to z > alpha (FP32): the inverse CDF for alpha < 3, Robert's
-// exponential rejection above.
-__device__ __forceinline__ float normal_above(float alpha, LawDraws &D) {
-    alpha = fminf(alpha, 1e6f);
-    if (alpha < 3.0f) {
-        const float q = 0.5f * erfcf(alpha * 0.70710678f);  // P(Z > alpha)
-        const float u = 1.0f - D.uni();                      // (0, 1]
-        return 1.41421356f * erfcinvf(2.0f * u * q);
-    }
-    for (int it = 0; it < 64; ++it) {
-        const float z = alpha - __logf(1.0f - D.uni()) / alpha;
-        const float dz = z - alpha;
-        if (D.uni() < __expf(-0.5f * dz * dz)) return z;
-    }
-    return alpha + 1.0f / alpha;
-}
-
-template <bool PROC>
-__global__ __launch_bounds__(256) void served_finish_kernel(const FieldParams *__restrict__ Pg, RayIn in,
-                                                            const float *__restrict__ lscale, int n, ReplyOut out) {
-#pragma clang fp contract(off)
-    const FieldParams &P = *Pg;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const uint32_t code = out.outcome[r];
-    if (!(code & mlp::kServedPending)) return;
-    const int cls = (int)(code & 0xFFu);
-    const float ox = in.ox[r], oy = in.oy[r], oz = in.oz[r];
-    const float dx = in.dx[r], dy = in.dy[r], dz = in.dz[r];
-    const float near_t = in.near_t[r], far_t = in.far_t[r];
-    const uint32_t normal_seed = in.normal_seed[r];
-    Ray1D R = {};
-    ray_scales(P.aniso, P.l_aniso, P.l_aniso_inv, lscale, (unsigned)r, R.la, R.ia);
-    float ix = R.ia[0] * dx, iy = R.ia[1] * dy, iz = R.ia[2] * dz;
-    const double k = sqrt((double)ix * ix + (double)iy * iy + (double)iz * iz);  // du/ds
-    normalize3(ix, iy, iz);
-    R.F = make_frame(ix, iy, iz);
-    R.cond = 1;
-    const double sig = P.amp_const;
-    const bool plus = P.ctxt == GPIS_CTX_RENEWAL_PLUS;
-    const bool corr = P.corr_xy && plus;
-    // the observations at the origin (conditioning1D's targets)
-    float mgx, mgy, mgz;
-    mean_grad_d<PROC>(P, ox, oy, oz, mgx, mgy, mgz);
-    const double a = ((double)in.pv[r] - mean_value_d<PROC>(P, ox, oy, oz)) / sig;
-    const float amp = P.amp_const;
-    const float tgx = (in.pgx[r] - mgx) / amp, tgy = (in.pgy[r] - mgy) / amp, tgz = (in.pgz[r] - mgz) / amp;
-    const float gwx = R.la[0] * tgx, gwy = R.la[1] * tgy, gwz = R.la[2] * tgz;
-    const double b = plus ? (double)dot3(R.F.nx, R.F.ny, R.F.nz, gwx, gwy, gwz) : 0.0;
-    float obs_x = 0.f, obs_y = 0.f;
-    if (plus) {
-        obs_x = dot3(R.F.tx, R.F.ty, R.F.tz, gwx, gwy, gwz);
-        obs_y = dot3(R.F.bx, R.F.by, R.F.bz, gwx, gwy, gwz);
-    }
-    LawDraws D;
-    D.g.set((uint64_t)normal_seed);
-    for (int i = 0; i < 4; ++i) (void)D.g.next();  // past the plain method's two pairs
-    // the sense: the sign of f at the first lattice point
-    float step = (far_t - near_t) / P.min_step_f;
-    if (P.step_size < step) step = P.step_size;
-    const float t0 = mul_add_exact(step, in.jitter[r], near_t);
-    float qx, qy, qz;
-    ray_point(ox, oy, oz, dx, dy, dz, (double)t0, qx, qy, qz);
-    const Law1D L0 = cond_law1d(k * (double)t0, a, b, plus);
-    const double f0 = mean_value_d<PROC>(P, qx, qy, qz) + sig * (L0.e1 + sqrt(fmax(L0.c11, 0.0)) * (double)D.normal());
-    const bool down = !(f0 < 0.0);  // a positive side looks for a down-crossing
-    float t_end, ndv, value = 0.f;
-    int hit = 1;
-    if (cls == 0) {
-        // the secant's cut-off: the ray origin and the conditioned derivative there
-        t_end = 0.f;
-        const Law1D L = cond_law1d(0.0, a, b, plus);
-        ndv = (float)(L.e2 + sqrt(fmax(L.c22, 0.0)) * (double)D.normal());
-    } else {
-        t_end = cls >= mlp::kBins ? far_t : out.t[r];
-        hit = cls < mlp::kBins;
-    }
-    grad_point(ox, oy, oz, dx, dy, dz, (double)t_end, qx, qy, qz);
-    if (cls != 0) {
-        const double m = mean_value_d<PROC>(P, qx, qy, qz);
-        float ex, ey, ez;
-        mean_grad_d<PROC>(P, qx, qy, qz, ex, ey, ez);
-        const double ms = (double)ex * dx + (double)ey * dy + (double)ez * dz;  // mean slope
-        const Law1D L = cond_law1d(k * (double)t_end, a, b, plus);
-        const bool ok = L.c11 > 1e-300;
-        const double q = ok ? L.c12 / L.c11 : 0.0;
-        const double vs = fmax(L.c22 - q * L.c12, 0.0);
-        const double sk = sig * k;
-        if (hit) {
-            // the Rice law of y = f'(t) at f(t) = 0, on the sense's side
-            const double mu = L.e2 + (ok ? q * (-m / sig - L.e1) : 0.0);
-            const double muy = ms + sk * mu, sy = sk * sqrt(vs);
-            double y;
-            if (sy > 1e-30 * (fabs(muy) + 1.0)) {
-                const double w = (double)rice_draw((float)((down ? -muy : muy) / sy), D) * sy;
-                y = down ? -w : w;
-            } else {
-                y = muy;
-            }
-            ndv = (float)((y - ms) / sk);
-        } else {
-            // the value at the bound on the sense's side, then the derivative given it
-            const double sd = sqrt(fmax(L.c11, 0.0));
-            double psi = L.e1;
-            if (sd > 0.0) {
-                const double alpha = (-m / sig - L.e1) / sd;  // psi > x0 keeps f > 0
-                psi = L.e1 + sd * (double)(down ? normal_above((float)alpha, D) : -normal_above((float)-alpha, D));
-            }
-            value = (float)(m + sig * psi);
-            const double mu = L.e2 + (ok ? q * (psi - L.e1) : 0.0);
-            ndv = (float)(mu + sqrt(vs) * (double)D.normal());
-        }
-    }
-    float gx, gy, gz;
-    const uint32_t oc = finish1d<PROC>(P, R, corr, plus, obs_x, obs_y, ox, oy, oz, qx, qy, qz, dx, dy, dz,
-                                       normal_seed, t_end, ndv, hit, gx, gy, gz);
-    out.t[r] = t_end;
-    out.value[r] = value;
-    out.gx[r] = gx;
-    out.gy[r] = gy;
-    out.gz[r] = gz;
-    out.outcome[r] = oc;
-}
-
 __global__ void fill_u32_kernel(uint32_t *p, int n, uint32_t v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -2760,6 +2770,8 @@ struct gpis_net {
     int64_t fb_rows = 0;
     float *loss = nullptr;
     int64_t step = 0;
+    float *avg = nullptr;  // moving average of the parameters (gpis_net_set_averaging)
+    float avg_beta = 0.0f;
     // served endpoints run on a side stream beside the fallback flights (fork and join
     // through two events; the side stream has the default priority)
     hipStream_t side = nullptr;
@@ -2885,7 +2897,7 @@ gpis_status gpis_net_destroy(gpis_net *n) {
     if (!n) return GPIS_INVALID_ARGUMENT;
     (void)hipDeviceSynchronize();
     void *ptrs[] = {n->theta, n->m, n->v, n->grad, n->part, n->afwd, n->abwd, n->calib,
-                    n->xt, n->feats, n->fallback, n->n_fallback, n->loss};
+                    n->xt, n->feats, n->fallback, n->n_fallback, n->loss, n->avg};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (n->fork) (void)hipEventDestroy(n->fork);
@@ -2897,6 +2909,29 @@ gpis_status gpis_net_destroy(gpis_net *n) {
 
 const char *gpis_net_last_error(const gpis_net *n) { return n ? n->err.c_str() : "null network"; }
 
+gpis_status gpis_net_set_averaging(gpis_net *n, float beta) {
+    if (!n) return GPIS_INVALID_ARGUMENT;
+    if (!(beta >= 0.0f && beta < 1.0f)) return nfail(n, GPIS_INVALID_ARGUMENT, "beta must lie in [0, 1)");
+    if (beta > 0.0f) {
+        const size_t pb = sizeof(float) * mlp::kParams;
+        if (!n->avg) NET_TRY(n, hipMalloc((void **)&n->avg, pb));
+        NET_TRY(n, hipMemcpy(n->avg, n->theta, pb, hipMemcpyDeviceToDevice));  // the average starts here
+    }
+    n->avg_beta = beta;
+    return GPIS_OK;
+}
+
+gpis_status gpis_net_use_average(gpis_net *n) {
+    if (!n) return GPIS_INVALID_ARGUMENT;
+    if (!n->avg || !(n->avg_beta > 0.0f)) return nfail(n, GPIS_INVALID_ARGUMENT, "averaging is not enabled");
+    NET_TRY(n, hipMemcpy(n->theta, n->avg, sizeof(float) * mlp::kParams, hipMemcpyDeviceToDevice));
+    n->avg_beta = 0.0f;
+    gpis_status s = net_pack(n, 0);
+    if (s != GPIS_OK) return s;
+    NET_TRY(n, hipDeviceSynchronize());
+    return GPIS_OK;
+}
+
 gpis_status gpis_net_set_params(gpis_net *n, const float *params, int64_t count) {
     if (!n || !params) return nfail(n, GPIS_INVALID_ARGUMENT, "null argument");
     if (count != mlp::kParams) return nfail(n, GPIS_INVALID_ARGUMENT, "parameter count must be 54816");
@@ -2905,6 +2940,7 @@ gpis_status gpis_net_set_params(gpis_net *n, const float *params, int64_t count)
     NET_TRY(n, hipMemset(n->m, 0, pb));
     NET_TRY(n, hipMemset(n->v, 0, pb));
     n->step = 0;
+    n->avg_beta = 0.0f;  // new parameters end any averaging
     gpis_status s = net_pack(n, 0);
     if (s != GPIS_OK) return s;
     NET_TRY(n, hipDeviceSynchronize());
@@ -2977,7 +3013,8 @@ gpis_status gpis_net_gradient(gpis_net *n, const float *feats, const int32_t *cl
     gpis_status s = net_backprop(n, feats, classes, batch, loss_sum, st, &chunks);
     if (s != GPIS_OK) return s;
     hipLaunchKernelGGL(mlp::adam_kernel, dim3((mlp::kParams + 255) / 256), dim3(256), 0, st, n->part, chunks,
-                       1.0f / (float)batch, n->theta, n->m, n->v, grad, 0.0f, 1.0f, 1.0f, 0);
+                       1.0f / (float)batch, n->theta, n->m, n->v, grad, 0.0f, 1.0f, 1.0f, 0, (float *)nullptr,
+                       0.0f);
     NET_TRY(n, hipGetLastError());
     return GPIS_OK;
 }
@@ -2994,7 +3031,8 @@ gpis_status gpis_net_train_step(gpis_net *n, const float *feats, const int32_t *
     const float bc1 = (float)(1.0 / (1.0 - pow(0.9, (double)n->step)));
     const float bc2 = (float)(1.0 / (1.0 - pow(0.999, (double)n->step)));
     hipLaunchKernelGGL(mlp::adam_kernel, dim3((mlp::kParams + 255) / 256), dim3(256), 0, st, n->part, chunks,
-                       1.0f / (float)batch, n->theta, n->m, n->v, n->grad, lr, bc1, bc2, 1);
+                       1.0f / (float)batch, n->theta, n->m, n->v, n->grad, lr, bc1, bc2, 1,
+                       n->avg_beta > 0.0f ? n->avg : (float *)nullptr, n->avg_beta);
     NET_TRY(n, hipGetLastError());
     return net_pack(n, st);
 }
@@ -3036,25 +3074,27 @@ static gpis_status surrogate_medium_ok(gpis_ctx *c) {
     return GPIS_OK;
 }
 
-// After the inference kernel on `st`: the served endpoints on the side stream, the
-// fallback flights (the plain method, one group of lanes per flight) on `st`, joined.
+// After the inference kernel on `st`: the fallback flights (the plain method, a group of
+// lanes per flight) on `st`, and beside them the served rows' endpoints on the network's
+// side stream (fork and join through two events; the side stream has the default priority).
 static gpis_status launch_served_and_fallback(gpis_ctx *c, gpis_net *net, const gpis_ray_soa *rays, int64_t n,
                                               const ReplyOut &out, const uint32_t *fallback, const uint32_t *n_fb,
                                               hipStream_t st, unsigned *work) {
     HIP_TRY(c, hipEventRecord(net->fork, st));
+    gpis_status s = launch_flight1d_groups(c, rays, n, out, fallback, n_fb, st, work);
+    if (s != GPIS_OK) return s;
     HIP_TRY(c, hipStreamWaitEvent(net->side, net->fork, 0));
     const unsigned blocks = (unsigned)((n + 255) / 256);
     if (c->P.mean_kind == GPIS_MEAN_PROCEDURAL)
         hipLaunchKernelGGL(served_finish_kernel<true>, dim3(blocks), dim3(256), 0, net->side, c->P_dev, make_in(rays),
-                           rays->length_scale, (int)n, out);
+                           rays->length_scale, (unsigned)n, out);
     else
         hipLaunchKernelGGL(served_finish_kernel<false>, dim3(blocks), dim3(256), 0, net->side, c->P_dev,
-                           make_in(rays), rays->length_scale, (int)n, out);
+                           make_in(rays), rays->length_scale, (unsigned)n, out);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipEventRecord(net->join, net->side));
-    gpis_status s = launch_flight1d_groups(c, rays, n, out, fallback, n_fb, st, work);
     HIP_TRY(c, hipStreamWaitEvent(st, net->join, 0));
-    return s;
+    return GPIS_OK;
 }
 
 extern "C" {

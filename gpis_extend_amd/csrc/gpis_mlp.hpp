@@ -353,9 +353,11 @@ __global__ __launch_bounds__(256) void wgrad_kernel(const bf16 *__restrict__ xt,
 }
 
 // Sum of the partial gradients scaled by 1/B (optionally written out), then Adam.
+// avg (may be null): an exponential moving average of the parameters after each update,
+// avg = beta avg + (1 - beta) theta (Polyak averaging of the Adam iterates).
 __global__ void adam_kernel(const float *__restrict__ part, int chunks, float inv_b, float *__restrict__ theta,
                             float *__restrict__ m, float *__restrict__ v, float *__restrict__ grad_out, float lr,
-                            float bc1, float bc2, int update) {
+                            float bc1, float bc2, int update, float *__restrict__ avg, float avg_beta) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kParams) return;
     float gsum = 0.f;
@@ -367,7 +369,9 @@ __global__ void adam_kernel(const float *__restrict__ part, int chunks, float in
     const float vv = 0.999f * v[p] + 0.001f * gr * gr;
     m[p] = mm;
     v[p] = vv;
-    theta[p] -= lr * (mm * bc1) / (sqrtf(vv * bc2) + 1e-8f);
+    const float th = theta[p] - lr * (mm * bc1) / (sqrtf(vv * bc2) + 1e-8f);
+    theta[p] = th;
+    if (avg) avg[p] = avg_beta * avg[p] + (1.0f - avg_beta) * th;
 }
 
 // ---------------------------------------------------------------- inference --

@@ -23,6 +23,12 @@ N_FEAT, N_CLASS, N_BINS, N_PARAMS = A.NET_FEATURES, A.NET_CLASSES, A.NET_CLASSES
 FEATURE_NAMES = ("value", "dir_derivative", "length_scale", "variance", "bound_length", "mean_origin",
                  "mean_slope", "kernel_se")
 SEED_TRAIN, SEED_INFER = 11, 12
+# Polyak averaging of the served network: the moving average of the Adam iterates with this
+# beta (a time constant of 1,000 steps, four epochs of the 2^24-flight training set)
+AVERAGE_BETA = 0.999
+# rays whose predicted entropy exceeds this quantile of the training set's fall back to the
+# plain method
+ENTROPY_QUANTILE = 0.999
 
 
 @dataclass
@@ -230,9 +236,19 @@ class FlightSurrogate:
                                                lr, buf.data_ptr(), self._stream()), "gpis_net_train_step")
         return buf
 
+    def set_averaging(self, beta: float):
+        self._check(self.L.gpis_net_set_averaging(self.h, float(beta)), "gpis_net_set_averaging")
+
+    def use_average(self):
+        self._check(self.L.gpis_net_use_average(self.h), "gpis_net_use_average")
+
     def train(self, feats, classes, epochs: int = 20, batch: int = 65536, lr: float = 1e-3, seed: int = SEED_TRAIN,
-              log=None) -> List[float]:
-        """Shuffled mini-batch Adam over the whole set; returns the mean loss of each epoch."""
+              log=None, average: float = 0.0) -> List[float]:
+        """Shuffled mini-batch Adam over the whole set; returns the mean loss of each epoch.
+        `average` > 0: the network ends with the exponential moving average (that beta) of
+        its Adam iterates, which removes the step noise of the constant learning rate."""
+        if average > 0.0:
+            self.set_averaging(average)
         import torch
         n = (feats.shape[0] // batch) * batch
         gen = torch.Generator(device="cuda")
@@ -250,6 +266,8 @@ class FlightSurrogate:
             out.append(m)
             if log:
                 log(f"epoch {ep + 1}/{epochs}: mean cross-entropy {m:.4f}")
+        if average > 0.0:
+            self.use_average()
         return out
 
     # ---- inference ---------------------------------------------------------

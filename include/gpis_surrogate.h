@@ -74,16 +74,30 @@ gpis_status gpis_net_gradient(gpis_net *net, const float *feats, const int32_t *
  * weight gradients, update, BF16 repack. */
 gpis_status gpis_net_train_step(gpis_net *net, const float *feats, const int32_t *classes, int64_t batch,
                                 float lr, float *loss_sum, void *stream);
+/* Polyak averaging of the Adam iterates: with beta in (0, 1) every later gpis_net_train_step
+ * also updates avg = beta * avg + (1 - beta) * params, starting from the current
+ * parameters; beta 0 stops it.  gpis_net_use_average makes the average the network's
+ * parameters (and stops averaging).  The learning rate stays the caller's; the average
+ * removes the iterates' step noise from the served network.  gpis_net_set_params stops
+ * averaging. */
+gpis_status gpis_net_set_averaging(gpis_net *net, float beta);
+gpis_status gpis_net_use_average(gpis_net *net);
 /* Class probabilities (n x 32, device) of raw features (n x 8), standardised inside
  * with the calibration. */
 gpis_status gpis_net_forward(gpis_net *net, const float *feats, int64_t n, float *probs, void *stream);
 
-/* Surrogate flights for a 1D_sampling medium: each conditioned ray inside the
- * training hull and below the entropy bound gets t sampled from the network's CDF
- * with u = its jitter (outcome GPIS_HIT for a crossing, GPIS_EXIT with t = far;
- * value and gradient 0, evals 0); every other ray is traced by the plain method in
- * the same call.  n_fallback (host, may be NULL) receives how many rays fell back;
- * asking for it synchronises the stream. */
+/* Surrogate flights for a 1D_sampling medium (Renewal or Renewal+, stationary
+ * single-level SE, sigma > 0; others: GPIS_UNSUPPORTED): each conditioned ray inside the
+ * training hull and below the entropy bound gets t sampled from the network's CDF with
+ * u = its jitter, and a complete sampleDistance reply (evals 0): the endpoint gradient
+ * and resumeDistance's rules (GaussianProcessMedium.cpp:300-330: GPIS_HIT, or GPIS_FAIL
+ * with GPIS_FAIL_OUTWARD for an up-crossing, or GPIS_EXIT with t = far and the field
+ * value there), drawn from the law of the flight's conditioned 1D process (the
+ * derivative along the ray from its Rice law at f(t) = 0, the crossing's sense from the
+ * value at the first lattice point, the transverse gradient the plain method's own draw
+ * of the normal seed).  Every other ray is traced by the plain method in the same call.
+ * n_fallback (host, may be NULL) receives how many rays fell back; asking for it
+ * synchronises the stream. */
 gpis_status gpis_surrogate_flights_1d(gpis_ctx *ctx, gpis_net *net, const gpis_ray_soa *rays, int64_t n,
                                       const gpis_reply_soa *reply, uint32_t *n_fallback, void *stream);
 

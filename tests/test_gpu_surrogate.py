@@ -113,10 +113,32 @@ def test_adam_step_matches_oracle_update():
     net.close()
 
 
+def test_parameter_average_is_the_moving_average_of_the_iterates():
+    rng = np.random.default_rng(34)
+    theta = M.init_params(seed=11)
+    net = _net_with(theta)
+    beta = 0.9
+    net.set_averaging(beta)
+    avg = theta.astype(np.float64).copy()
+    for k in range(3):
+        x = torch.from_numpy(rng.normal(size=(256, 8)).astype(np.float32)).cuda()
+        y = torch.from_numpy(rng.integers(0, 32, size=256).astype(np.int32)).cuda()
+        net.train_step(x, y, lr=1e-3)
+        avg = beta * avg + (1 - beta) * net.get_params().astype(np.float64)
+    last = net.get_params()
+    net.use_average()
+    got = net.get_params()
+    print(f"average vs host EMA: max |d| {np.abs(got - avg).max():.2e}; moved from the last iterate "
+          f"{np.abs(got - last).max():.2e}")
+    assert np.abs(got - avg).max() < 1e-6
+    assert np.abs(got - last).max() > 1e-5
+    net.close()
+
+
 @pytest.fixture(scope="module")
 def trained():
     """A network trained on 2^22 plain-method flights of the renewal workload (8 epochs,
-    512 Adam steps of 65,536)."""
+    512 Adam steps of 65,536), served with the moving average of its iterates (beta 0.99)."""
     from gpis_extend_amd import GpisDevice, SparseConvolutionNoiseMedium, workloads as W
     from gpis_extend_amd import surrogate as S
     med = SparseConvolutionNoiseMedium.from_json(W.config3_json())
@@ -126,7 +148,7 @@ def trained():
     cal = S.calibrate(feats, s, ex)
     net.set_calibration(cal)
     cls = S.classes_of(net, s, oc)
-    losses = net.train(feats, cls, epochs=8, batch=65536, lr=1e-3, seed=S.SEED_TRAIN)
+    losses = net.train(feats, cls, epochs=8, batch=65536, lr=1e-3, seed=S.SEED_TRAIN, average=0.99)
     counts = torch.bincount(cls[cls >= 0].long(), minlength=32).double()
     pm = counts / counts.sum()
     marginal_ce = float(-(pm[pm > 0] * pm[pm > 0].log()).sum())
