@@ -21,7 +21,9 @@ SDF_FUNCTIONS = ("knob", "knob_inner", "knob_outer", "two_spheres", "plane", "ro
                  "three_slabs")
 CTX_GLOBAL, CTX_RENEWAL, CTX_RENEWAL_PLUS, CTX_NONE = 0, 1, 2, 3
 SCHEME_UNI, SCHEME_NEE, SCHEME_MIS = 0, 1, 2
-LS_CONST, LS_SINCOS = 0, 1
+LS_CONST, LS_SINCOS, LS_NOISE = 0, 1, 2
+VAR_CONST, VAR_NOISE = 0, 1
+NOISE_TYPES = ("bottom_top", "left_right", "front_back", "bottom_top_left_right", "sandstone", "rust")
 EXIT, HIT, FAIL, INVALID = 0, 1, 2, 3
 FAIL_NONE, FAIL_OUTWARD, FAIL_ZERO_GRAD, FAIL_NONFINITE, FAIL_BOUNCE_LIMIT = 0, 1, 2, 3, 4
 PHASE_MIRROR, PHASE_LAMBERT = 0, 1
@@ -56,6 +58,8 @@ class FieldDesc(C.Structure):
         ("isotropic_3d", C.c_int32),
         ("sdf_function", C.c_int32), ("sdf_inverse_transform", C.c_float * 12),
         ("sdf_min", C.c_float), ("sdf_scale", C.c_float), ("sdf_offset", C.c_float),
+        ("ns_var_field", C.c_int32), ("ns_var_noise", C.c_int32), ("ns_ls_noise", C.c_int32),
+        ("ns_var_noise_params", C.c_double * 8), ("ns_ls_noise_params", C.c_double * 8),
     ]
 
 

@@ -26,6 +26,7 @@ static inline int max(int a, int b) { return a > b ? a : b; }
 #endif
 
 #include "gpis_lists.hpp"
+#include "gpis_procnoise.hpp"
 #include "gpis_sdf.hpp"
 
 namespace gpis {
@@ -184,6 +185,9 @@ struct FieldParams {
     float l_max;         // max(l_aniso) (splattingKernelRadius)
     float kernel_scale;  // _kernelScale (3)
     int ls_field; float ls_params[4]; float ls_max; float ns_var;
+    // "noise" fields (gpis_procnoise.hpp): var_field 1 makes the amplitude var(p) * sigma;
+    // ls_field 2 takes the lateral scale from ls_noise
+    int var_field; NoiseField var_noise, ls_noise;
     int N; float Nf;
     int iso;                  // isotropic kernel: exponent = c * r2
     int seed_add_single;      // floor(log(kss)/log(2.5)) for the single-level grid

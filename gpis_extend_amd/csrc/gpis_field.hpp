@@ -470,9 +470,17 @@ __device__ __forceinline__ void cond_splat(const FieldParams &P, const RealState
 // Procedural length-scale field of the nonstationary covariance, evaluated in double at
 // the double-widened point and rounded once to float (the covariance's lateral scale
 // getKernelScale, GPFunctions.cpp:1734-1740, takes a Vec3d).
+// Kernels with the generic impulse count instantiated as NI = -1 carry the noise fields.
+template <int NI> struct HasNoise { static constexpr bool value = NI < 0; };
+
+// getKernelScale (GPFunctions.cpp:1734-1740): the largest float component of the ls field at
+// p (ls_field 0: constant; 1: the sin*cos field defined in code; 2: ProceduralNoiseVec, only
+// in the kernels instantiated with NOISE: the out-of-line fbm changes a whole kernel's
+// register allocation, so the other kernels never contain the call).
+template <bool NOISE>
 __device__ __forceinline__ float ls_field(const FieldParams &P, float x, float y, float z) {
 #pragma clang fp contract(off)
-    (void)y;
+    if (NOISE && P.ls_field == 2) return (float)noise_field(P.ls_noise, true, (double)x, (double)y, (double)z);
     if (P.ls_field == 1) {
         const double fr = P.ls_params[2];
         return (float)((double)P.ls_params[0] +
@@ -590,7 +598,7 @@ __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealStat
         } else {
             // brute-force nonstationary: gs = ls_max, ls = ls(p)/ls_max
             gs = P.ls_max;
-            ls = ls_field(P, px, py, pz) / P.ls_max;
+            ls = ls_field<HasNoise<NI>::value>(P, px, py, pz) / P.ls_max;
             L = make_level(P, gs, ls, (uint32_t)P.seed_add_single);
             L.R = P.kernel_scale * gs * P.l_max;
         }
@@ -603,7 +611,7 @@ __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealStat
         return r;
     }
     // multi-resolution: two adjacent levels of the base-2.5 ladder (kernelScaleLevelRatio)
-    const LevelRatio lr = level_ratio(ls_field(P, px, py, pz));
+    const LevelRatio lr = level_ratio(ls_field<HasNoise<NI>::value>(P, px, py, pz));
     const float slo = lr.slo, shi = lr.shi, wlo = lr.wlo, whi = lr.whi;
     const int ilo = lr.li + 32, ihi = lr.li + 33;  // in [0, 64]: the table holds L = -32 .. 32
     Level A = make_level(P, slo, 1.0f, (uint32_t)P.level_seed_add[ilo]);
@@ -620,18 +628,26 @@ __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealStat
     return r;
 }
 
-__device__ __forceinline__ float amplitude(const FieldParams &P) { return P.amp_const; }
+// sparseConvNoiseAmplitude(p): sigma, or getVariance(p) * sigma for a nonstationary
+// covariance (GPFunctions.cpp:1245-1247), whose "noise" variance field varies with p (NOISE
+// kernels only, as ls_field).
+template <bool NOISE>
+__device__ __forceinline__ float amplitude(const FieldParams &P, float x, float y, float z) {
+    if (NOISE && P.var_field == 1)
+        return (float)(noise_field(P.var_noise, false, (double)x, (double)y, (double)z) * (double)P.sigma);
+    return P.amp_const;
+}
 
 // f(p) = amplitude * noise + mean and its world gradient, rounded as evaluateValue and
 // evaluateGradient round them (SparseConvolutionNoise.cpp:113-140): the float product
 // amplitude * noise added to the double mean and rounded once; the gradient's float
 // product and float mean gradient added unfused.  With an analytic (float) mean the double
 // sum of two floats is exact, so the float addition rounds the same.
-template <int MEAN>
+template <int MEAN, bool NOISE = false>
 __device__ __forceinline__ void add_mean(const FieldParams &P, float px, float py, float pz, Eval4 &r) {
 #pragma clang fp contract(off)
     float mgx, mgy, mgz;
-    const float a = amplitude(P);
+    const float a = amplitude<NOISE>(P, px, py, pz);
     const float an = a * r.v;
     if (MEAN == MEAN_ANALYTIC) {
         r.v = an + analytic_mean_value(P, px, py, pz, mgx, mgy, mgz);
@@ -649,7 +665,7 @@ __device__ __forceinline__ Eval4 eval_field(const FieldParams &P, const RealStat
                                             float px, float py, float pz, bool cond, float dx = 0.f,
                                             float dy = 0.f, float dz = 1.f) {
     Eval4 r = eval_noise<NI, ISO, IRAY>(P, rs, px, py, pz, cond, dx, dy, dz);
-    add_mean<MEAN>(P, px, py, pz, r);
+    add_mean<MEAN, HasNoise<NI>::value>(P, px, py, pz, r);
     return r;
 }
 
@@ -677,7 +693,7 @@ __device__ __forceinline__ void eval_field2(const FieldParams &P, const RealStat
             Eval4 &E = w ? B : A;
             const float *p = w ? pb : pa;
             if (cond && rs.cond) cond_splat(P, rs, p[0], p[1], p[2], 1.0f, 1.0f, E);
-            add_mean<MEAN>(P, p[0], p[1], p[2], E);
+            add_mean<MEAN, HasNoise<NI>::value>(P, p[0], p[1], p[2], E);
         }
         return;
     }

@@ -185,6 +185,30 @@ static gpis_status build_params(gpis_ctx *c) {
     P.ls_max = d.cov_kind == GPIS_COV_SE ? 1.0f : d.ns_ls_max;
     P.ns_var = d.ns_var;
     P.amp_const = d.cov_kind == GPIS_COV_SE ? d.sigma : (float)((double)d.ns_var * d.sigma);
+    // ProceduralNoise fields: _scale = 1 / (end - start), _offset = -start * _scale
+    // (GPFunctions.hpp:726-729)
+    auto noise = [](int type, const double *q) {
+        NoiseField F;
+        F.type = type;
+        F.mn = q[0]; F.mx = q[1]; F.scale = 1.0 / (q[3] - q[2]); F.offset = -q[2] * F.scale;
+        F.mn2 = q[4]; F.mx2 = q[5]; F.scale2 = 1.0 / (q[7] - q[6]); F.offset2 = -q[6] * F.scale2;
+        return F;
+    };
+    if (d.cov_kind == GPIS_COV_PROC_NS && d.ns_var_field == GPIS_VAR_NOISE) {
+        if (d.ns_var_noise < GPIS_NOISE_BOTTOM_TOP || d.ns_var_noise > GPIS_NOISE_RUST)
+            return fail(c, GPIS_INVALID_ARGUMENT, "unknown variance noise type");
+        P.var_field = 1;
+        P.var_noise = noise(d.ns_var_noise, d.ns_var_noise_params);
+    } else if (d.ns_var_field != GPIS_VAR_CONST) {
+        return fail(c, GPIS_INVALID_ARGUMENT, "variance fields belong to proc_nonstationary covariances");
+    }
+    if (d.cov_kind == GPIS_COV_PROC_NS && d.ns_ls_field == GPIS_LS_NOISE) {
+        if (d.ns_ls_noise < GPIS_NOISE_BOTTOM_TOP || d.ns_ls_noise > GPIS_NOISE_RUST)
+            return fail(c, GPIS_INVALID_ARGUMENT, "unknown length-scale noise type");
+        P.ls_noise = noise(d.ns_ls_noise, d.ns_ls_noise_params);
+    }
+    if (d.ns_ls_field < GPIS_LS_CONST || d.ns_ls_field > GPIS_LS_NOISE)
+        return fail(c, GPIS_INVALID_ARGUMENT, "unknown length-scale field");
     P.N = (int)(unsigned)d.impulse_density;
     P.iso = d.aniso[0] == d.aniso[1] && d.aniso[1] == d.aniso[2];
     P.Nf = (float)P.N;
@@ -487,13 +511,13 @@ __device__ __forceinline__ float box_exit(float ox, float oy, float oz, float dx
 // in FP64.  With one observation point the value and gradient observations decouple
 // (the covariance gradient vanishes at zero lag), so the Schur complement of the
 // value+gradient observation covariance is the diagonal second-derivative block.
-template <int MEAN, bool IRAY>
+template <int MEAN, bool IRAY, bool NOISE = false>
 __device__ __noinline__ void condition_coefficients(const FieldParams &P, const Eval4 &e, float ox, float oy, float oz,
                                                     float pv, float pgx, float pgy, float pgz, float dx, float dy,
                                                     float dz, RealState &rs) {
     float mgx, mgy, mgz;
     double mean = mean_value<MEAN>(P, ox, oy, oz, mgx, mgy, mgz);
-    double amp = P.amp_const;
+    double amp = amplitude<NOISE>(P, ox, oy, oz);
     double cur_v = ((double)e.v - mean) / amp;
     double a = ((double)pv - mean) / amp - cur_v;
     double b0 = 0.0, b1 = 0.0, b2 = 0.0;
@@ -501,7 +525,7 @@ __device__ __noinline__ void condition_coefficients(const FieldParams &P, const 
     if (P.multires) {
         // kernelScaleLevelRatio at the origin: value /= wlo + whi, gradient /= wlo/slo^2 + whi/shi^2
         // (the sums in float, as the weights and scales are floats)
-        const LevelRatio k = level_ratio(ls_field(P, ox, oy, oz));
+        const LevelRatio k = level_ratio(ls_field<NOISE>(P, ox, oy, oz));
         wsum = k.wlo + k.whi;
         gsum = k.wlo / (k.slo * k.slo) + k.whi / (k.shi * k.shi);
     }
@@ -525,7 +549,7 @@ __device__ __noinline__ void condition_coefficients(const FieldParams &P, const 
             if (P.kind == GPIS_COV_PROC_NS && !P.multires) {
                 // the brute-force kernel splats at l(o): gradient_scale *= sqr(lateral scale)
                 // (SparseConvolutionNoise.cpp:563-565), the square formed in float
-                const float lat = ls_field(P, ox, oy, oz);
+                const float lat = ls_field<NOISE>(P, ox, oy, oz);
                 const double s2 = (double)(lat * lat);
                 b0 *= s2; b1 *= s2; b2 *= s2;
             }
@@ -779,7 +803,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(cons
             ++evals_flight;
             const float fv = e.v;
             if (COND && !second && phase == PH_COND) {
-                condition_coefficients<GRID ? MEAN_EXT : MEAN_ANALYTIC, IRAY>(P, e, ox, oy, oz, prev_v, prev_gx, prev_gy,
+                condition_coefficients<GRID ? MEAN_EXT : MEAN_ANALYTIC, IRAY, (NI < 0)>(P, e, ox, oy, oz, prev_v, prev_gx, prev_gy,
                                                                               prev_gz, dx, dy, dz, rs);
                 phase = PH_F0;
                 tq = (double)near_t;
@@ -2143,7 +2167,13 @@ static void launch_flights(gpis_ctx *c, unsigned blocks, unsigned threads, hipSt
                                c->P_dev, in, out, pout, pseeds, n, c->work, c->counters, max_bounces, bounce_seed, box, c->batch_k);
         return;
     }
-    if (c->P.N == 8 && c->P.iso && !cond) GPIS_FLIGHT_LAUNCH(8, true, false);
+    // "noise" variance or length-scale fields: the generic kernels that carry them (NI = -1)
+    const bool noise = c->P.var_field == 1 || c->P.ls_field == GPIS_LS_NOISE;
+    if (noise && c->P.iso && !cond) GPIS_FLIGHT_LAUNCH(-1, true, false);
+    else if (noise && c->P.iso) GPIS_FLIGHT_LAUNCH(-1, true, true);
+    else if (noise && !cond) GPIS_FLIGHT_LAUNCH(-1, false, false);
+    else if (noise) GPIS_FLIGHT_LAUNCH(-1, false, true);
+    else if (c->P.N == 8 && c->P.iso && !cond) GPIS_FLIGHT_LAUNCH(8, true, false);
     else if (c->P.N == 8 && c->P.iso) GPIS_FLIGHT_LAUNCH(8, true, true);
     else if (c->P.N == 8 && !cond) GPIS_FLIGHT_LAUNCH(8, false, false);
     else if (c->P.N == 8) GPIS_FLIGHT_LAUNCH(8, false, true);
@@ -2412,7 +2442,10 @@ gpis_status gpis_eval_points(gpis_ctx *c, const float *x, const float *y, const 
                                (hipStream_t)stream, c->P_dev, x, y, z, (long long)n, seed, v, gx, gy, gz);            \
     } while (0)
     const bool ext = c->P.mean_kind == GPIS_MEAN_GRID || c->P.mean_kind == GPIS_MEAN_PROCEDURAL;
-    if (c->P.N == 8 && c->P.iso) GPIS_EVAL_LAUNCH(8, true);
+    const bool noise = c->P.var_field == 1 || c->P.ls_field == GPIS_LS_NOISE;
+    if (noise && c->P.iso) GPIS_EVAL_LAUNCH(-1, true);
+    else if (noise) GPIS_EVAL_LAUNCH(-1, false);
+    else if (c->P.N == 8 && c->P.iso) GPIS_EVAL_LAUNCH(8, true);
     else if (c->P.N == 8) GPIS_EVAL_LAUNCH(8, false);
     else if (c->P.iso) GPIS_EVAL_LAUNCH(0, true);
     else GPIS_EVAL_LAUNCH(0, false);

@@ -105,6 +105,27 @@ class DenseMeanGrid:
         return g
 
 
+def _noise_field(js):
+    """A ProceduralNoise / ProceduralNoiseVec of type "noise" (GPFunctions.hpp:716-735): its
+    type index and (min, max, start, end, min2, max2, start2, end2) with the reference's
+    defaults."""
+    kind = js.get("noise", "bottom_top")
+    if kind not in A.NOISE_TYPES:
+        raise MediumConfigError(f"Invalid noise typ function: '{kind}'")
+    prm = tuple(float(js.get(k, d)) for k, d in (("min", 1.0), ("max", 500.0), ("start", 0.0), ("end", 1.0),
+                                                ("min2", 1.0), ("max2", 500.0), ("start2", 0.0), ("end2", 1.0)))
+    return A.NOISE_TYPES.index(kind), prm
+
+
+def _noise_max(kind, prm):
+    """ProceduralNoiseVec::maxVal() (GPFunctions.cpp:121-139), as a float."""
+    if kind <= 2:
+        return float(np.float32(max(prm[1], prm[0])))
+    if kind == 3:
+        return float(np.float32(max(prm[1], prm[0]) * max(prm[5], prm[4])))
+    return 1.0
+
+
 @dataclass
 class SparseConvolutionNoiseMedium:
     """Parameters of one sparse_conv_noise medium.  Defaults are the reference
@@ -132,6 +153,13 @@ class SparseConvolutionNoiseMedium:
     ns_ls_params: tuple = (1.0, 0.0, 0.0, 0.0)
     ns_ls_max: float = 1.0
     ns_var: float = 1.0
+    # "noise" fields (ProceduralNoise): type index of A.NOISE_TYPES and the JSON min, max,
+    # start, end, min2, max2, start2, end2
+    ns_var_field: int = A.VAR_CONST
+    ns_var_noise: int = 0
+    ns_var_noise_params: tuple = (1.0, 500.0, 0.0, 1.0, 1.0, 500.0, 0.0, 1.0)
+    ns_ls_noise: int = 0
+    ns_ls_noise_params: tuple = (1.0, 500.0, 0.0, 1.0, 1.0, 500.0, 0.0, 1.0)
     # gaussian_process.mean
     mean_kind: int = A.MEAN_SPHERE
     mean_center: tuple = (0.0, 0.0, 0.0)
@@ -247,13 +275,22 @@ class SparseConvolutionNoiseMedium:
             elif ls.get("type", "constant") == "constant":
                 v = float(ls.get("value", 1.0))
                 m.ns_ls_field, m.ns_ls_params, m.ns_ls_max = A.LS_CONST, (v, 0.0, 0.0, 0.0), v
+            elif ls.get("type") == "noise":
+                m.ns_ls_field = A.LS_NOISE
+                m.ns_ls_noise, m.ns_ls_noise_params = _noise_field(ls)
+                m.ns_ls_max = _noise_max(m.ns_ls_noise, m.ns_ls_noise_params)
             else:
                 raise MediumConfigError(f"length-scale field {ls.get('type')!r} is not served by the GPU flight path")
             var = cov.get("var", 1.0)
             if isinstance(var, dict):
-                if var.get("type", "constant") != "constant":
+                if var.get("type") == "noise":
+                    m.ns_var_field = A.VAR_NOISE
+                    m.ns_var_noise, m.ns_var_noise_params = _noise_field(var)
+                    var = 1.0
+                elif var.get("type", "constant") != "constant":
                     raise MediumConfigError(f"variance field {var.get('type')!r} is not served by the GPU flight path")
-                var = var.get("value", 1.0)
+                else:
+                    var = var.get("value", 1.0)
             m.ns_var = float(var)
             cov = cov.get("cov", {"type": "squared_exponential"})
             ct = cov.get("type")
@@ -282,6 +319,10 @@ class SparseConvolutionNoiseMedium:
         for i in range(4):
             d.ns_ls_params[i] = self.ns_ls_params[i]
         d.ns_ls_max, d.ns_var = self.ns_ls_max, self.ns_var
+        d.ns_var_field, d.ns_var_noise, d.ns_ls_noise = self.ns_var_field, self.ns_var_noise, self.ns_ls_noise
+        for i in range(8):
+            d.ns_var_noise_params[i] = self.ns_var_noise_params[i]
+            d.ns_ls_noise_params[i] = self.ns_ls_noise_params[i]
         d.mean_kind, d.mean_radius, d.plane_offset = self.mean_kind, self.mean_radius, self.plane_offset
         d.global_seed = self.seed
         d.impulse_density = self.impulse_density

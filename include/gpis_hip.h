@@ -46,7 +46,11 @@ enum { GPIS_CTX_GLOBAL = 0, GPIS_CTX_RENEWAL = 1, GPIS_CTX_RENEWAL_PLUS = 2, GPI
 /* SparseConv1DSamplingScheme (Medium.hpp:42-46). */
 enum { GPIS_SCHEME_UNI = 0, GPIS_SCHEME_NEE = 1, GPIS_SCHEME_MIS = 2 };
 /* Procedural length-scale fields of the nonstationary covariance. */
-enum { GPIS_LS_CONST = 0, GPIS_LS_SINCOS = 1 };
+enum { GPIS_LS_CONST = 0, GPIS_LS_SINCOS = 1, GPIS_LS_NOISE = 2 };
+enum { GPIS_VAR_CONST = 0, GPIS_VAR_NOISE = 1 };
+/* ProceduralNoise "noise" types (GPFunctions.hpp:655-690) */
+enum { GPIS_NOISE_BOTTOM_TOP = 0, GPIS_NOISE_LEFT_RIGHT = 1, GPIS_NOISE_FRONT_BACK = 2,
+       GPIS_NOISE_BOTTOM_TOP_LEFT_RIGHT = 3, GPIS_NOISE_SANDSTONE = 4, GPIS_NOISE_RUST = 5 };
 
 /* The sparse_conv_noise medium's parameters (SparseConvolutionNoiseMedium.cpp:102-118,
  * GaussianProcessMedium.cpp:98-127, GPFunctions.cpp:657-685). */
@@ -81,6 +85,14 @@ typedef struct gpis_field_desc {
     int32_t sdf_function;                      /* "func", GPIS_SDF_* */
     float sdf_inverse_transform[12];           /* _invConfigTransform rows 0-2, row-major */
     float sdf_min, sdf_scale, sdf_offset;      /* "min" (-FLT_MAX), "scale" (1), "offset" (0) */
+    /* proc_nonstationary fields of type "noise" (ProceduralNoise "var", ProceduralNoiseVec
+     * "ls"; GPFunctions.hpp:647-830, GPFunctions.cpp:53-135): ns_var_field GPIS_VAR_NOISE
+     * and/or ns_ls_field GPIS_LS_NOISE, the GPIS_NOISE_* type of each and its JSON "min",
+     * "max", "start", "end", "min2", "max2", "start2", "end2"; ns_ls_max is then the ls
+     * field's maxVal() */
+    int32_t ns_var_field;
+    int32_t ns_var_noise, ns_ls_noise;
+    double ns_var_noise_params[8], ns_ls_noise_params[8];
 } gpis_field_desc;
 
 /* A wavefront of flight requests in structure-of-arrays form

@@ -247,9 +247,154 @@ typedef struct {
 
 static int is_se(const orc_field *f) { return f->d.cov_kind == ORC_COV_SE; }
 
+/* ---- ProceduralNoise fields over SdfFunctions' simplex fbm ----------------------
+ * random3, simplex3d, fbm: SdfFunctions.cpp:84-183 (float arithmetic as written; random3's
+ * sin takes the float dot in double, then 4096 * sin rounds to float -- the reference
+ * build resolves the unqualified sin to the double one). */
+static const float ORC_F3 = 0.3333333f, ORC_G3 = 0.1666667f;
+static v3f orc_random3_v(v3f c) {
+    float dd = vdot(c, V3(17.0f, 59.4f, 15.0f));
+    float j = (float)(4096.0 * sin((double)dd));
+    double t = 512.0 * (double)j;
+    float rz = (float)(t - floor(t));
+    j = (float)((double)j * 0.125);
+    t = 512.0 * (double)j;
+    float rx = (float)(t - floor(t));
+    j = (float)((double)j * 0.125);
+    t = 512.0 * (double)j;
+    float ry = (float)(t - floor(t));
+    return V3(rx - 0.5f, ry - 0.5f, rz - 0.5f);
+}
+static float orc_simplex3d_v(v3f p) {
+    float dp = vdot(p, V3(ORC_F3, ORC_F3, ORC_F3));
+    v3f s = V3(floorf(p.x + dp), floorf(p.y + dp), floorf(p.z + dp));
+    float ds = vdot(s, V3(ORC_G3, ORC_G3, ORC_G3));
+    v3f x = V3((p.x - s.x) + ds, (p.y - s.y) + ds, (p.z - s.z) + ds);
+    /* e = step(0, x - x.yzx); i1 = e * (1 - e.zxy); i2 = 1 - e.zxy * (1 - e) */
+    v3f e = V3((x.x - x.y) < 0.0f ? 0.0f : 1.0f, (x.y - x.z) < 0.0f ? 0.0f : 1.0f, (x.z - x.x) < 0.0f ? 0.0f : 1.0f);
+    v3f ezxy = V3(e.z, e.x, e.y);
+    v3f i1 = V3(e.x * (1.0f - ezxy.x), e.y * (1.0f - ezxy.y), e.z * (1.0f - ezxy.z));
+    v3f i2 = V3(1.0f - ezxy.x * (1.0f - e.x), 1.0f - ezxy.y * (1.0f - e.y), 1.0f - ezxy.z * (1.0f - e.z));
+    const float g2 = 2.0f * ORC_G3, g3 = 3.0f * ORC_G3;
+    v3f x1 = V3((x.x - i1.x) + ORC_G3, (x.y - i1.y) + ORC_G3, (x.z - i1.z) + ORC_G3);
+    v3f x2 = V3((x.x - i2.x) + g2, (x.y - i2.y) + g2, (x.z - i2.z) + g2);
+    v3f x3 = V3((x.x - 1.0f) + g3, (x.y - 1.0f) + g3, (x.z - 1.0f) + g3);
+    float w[4] = {vdot(x, x), vdot(x1, x1), vdot(x2, x2), vdot(x3, x3)};
+    for (int i = 0; i < 4; ++i) { float a = 0.6f - w[i]; w[i] = a < 0.0f ? 0.0f : a; }
+    float d[4];
+    d[0] = vdot(orc_random3_v(s), x);
+    d[1] = vdot(orc_random3_v(V3(s.x + i1.x, s.y + i1.y, s.z + i1.z)), x1);
+    d[2] = vdot(orc_random3_v(V3(s.x + i2.x, s.y + i2.y, s.z + i2.z)), x2);
+    d[3] = vdot(orc_random3_v(V3(s.x + 1.0f, s.y + 1.0f, s.z + 1.0f)), x3);
+    for (int i = 0; i < 4; ++i) { w[i] *= w[i]; }
+    for (int i = 0; i < 4; ++i) { w[i] *= w[i]; }
+    for (int i = 0; i < 4; ++i) { d[i] *= w[i]; }
+    float r = d[0] * 52.0f;
+    r += d[1] * 52.0f;
+    r += d[2] * 52.0f;
+    r += d[3] * 52.0f;
+    return r;
+}
+static double orc_fbm_v(v3d uv, int octaves) {
+    const float gain = (float)0.65, lacunarity = (float)2.1042;
+    float frequency = 0.5f, amplitude = gain;
+    uv = D3(uv.x * 5.0, uv.y * 5.0, uv.z * 5.0);
+    v3f u = f_of(uv);
+    float total = orc_simplex3d_v(u);
+    for (int i = 0; i < octaves; ++i) {
+        total += orc_simplex3d_v(V3(u.x * frequency, u.y * frequency, u.z * frequency)) * amplitude;
+        frequency *= lacunarity;
+        amplitude *= gain;
+    }
+    total = (float)(((double)total + 2.0) / 4.0);
+    return (double)total;
+}
+static double dclamp(double v, double lo, double hi) { double a = v < lo ? lo : v; return a > hi ? hi : a; }
+static double dlerp(double a, double b, double t) { return a * (1.0 - t) + b * t; }
+/* sqrt(exp(lerp(log(min^2), log(max^2), clamp(c * scale + offset, 0, 1)))) */
+static double noise_ramp(double mn, double mx, double c, double scale, double offset) {
+    return sqrt(exp(dlerp(log(mn * mn), log(mx * mx), dclamp(c * scale + offset, 0.0, 1.0))));
+}
+/* ProceduralNoise::operator() (vec = 0) and the largest component of
+ * ProceduralNoiseVec::operator() (vec = 1), GPFunctions.cpp:53-119; q = min, max, start,
+ * end, min2, max2, start2, end2; _const = 1; scale = 1 / (end - start), offset = -start * scale. */
+static double noise_field(int type, const double *q, int vec, v3d p) {
+    const double cst = 1.0;
+    const double scale = 1.0 / (q[3] - q[2]), offset = -q[2] * scale;
+    const double scale2 = 1.0 / (q[7] - q[6]), offset2 = -q[6] * scale2;
+    const double mn = q[0] + cst, mx = q[1] + cst;
+    double v;
+    switch (type) {
+    case ORC_NOISE_BOTTOM_TOP: v = noise_ramp(mn, mx, p.y, scale, offset) - cst; break;
+    case ORC_NOISE_LEFT_RIGHT: v = noise_ramp(mn, mx, p.x, scale, offset) - cst; break;
+    case ORC_NOISE_FRONT_BACK: v = noise_ramp(mn, mx, p.z, scale, offset) - cst; break;
+    case ORC_NOISE_BOTTOM_TOP_LEFT_RIGHT: {
+        const double mn2 = q[4] + cst, mx2 = q[5] + cst;
+        float bt = (float)noise_ramp(mn, mx, p.y, scale, offset);
+        float lr = (float)noise_ramp(mn2, mx2, p.x, scale2, offset2);
+        v = (double)(bt * lr) - cst * cst;
+        break;
+    }
+    case ORC_NOISE_SANDSTONE: {
+        const int oct = vec ? 10 : 2;
+        v3d ps = D3(p.x * 0.3, p.y * 0.3, p.z * 0.3);
+        double a = orc_fbm_v(ps, oct);
+        double b = orc_fbm_v(D3(ps.x + a, ps.y + a, ps.z + a), oct);
+        double f = orc_fbm_v(D3(ps.x + b, ps.y + b, ps.z + b), oct);
+        if (!vec) {
+            double cx = sqrt(f * 1.9 * 1.2) - 0.35;
+            v = dlerp(q[0], q[1], dclamp(cx, 0., 1.));
+        } else {
+            /* clamp(col * 0.2, 0, 1) per component; the largest as floats */
+            float cx = (float)dclamp((sqrt(f * 1.9 * 1.2) - 0.35) * 0.2, 0., 1.);
+            float cy = (float)dclamp((sqrt(f * 0.7 * 1.2) - 0.35) * 0.2, 0., 1.);
+            float cz = (float)dclamp((sqrt(f * 0.25 * 1.2) - 0.35) * 0.2, 0., 1.);
+            return (double)fmaxf(fmaxf(cx, cy), cz);
+        }
+        break;
+    }
+    case ORC_NOISE_RUST: default: {
+        const int oct = vec ? 10 : 2;
+        v3d ps = D3(p.x * 2.0, p.y * 2.0, p.z * 2.0);
+        double inner = orc_fbm_v(D3(ps.x * .1, ps.y * .1, ps.z * .1), oct) * 0.4;
+        double a = orc_fbm_v(D3(ps.x + inner, ps.y + inner, ps.z + inner), oct);
+        double c = orc_fbm_v(D3(ps.x * 25., ps.y * 25., ps.z * 25.), oct) * 0.1;
+        double x = vec ? a + c : a - c;
+        x = dclamp((x - 0.4) / (0.6 - 0.4), 0.0, 1.0);
+        double f = x * x * (3.0 - 2.0 * x);
+        if (!vec) {
+            v = dlerp(q[0], q[1], dclamp(f, 0., 1.));
+        } else {
+            float cx = (float)(0.278 * (1.0 - f) + 1.0 * f);
+            float cy = (float)(0.212 * (1.0 - f) + 1.0 * f);
+            float cz = (float)(0.141 * (1.0 - f) + 1.0 * f);
+            return (double)fmaxf(fmaxf(cx, cy), cz);
+        }
+        break;
+    }
+    }
+    return vec ? (double)(float)v : v;
+}
+void orc_random3(const float *p, int64_t n, float *out) {
+    for (int64_t i = 0; i < n; ++i) {
+        v3f r = orc_random3_v(V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]));
+        out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z;
+    }
+}
+void orc_simplex3d(const float *p, int64_t n, float *out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = orc_simplex3d_v(V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]));
+}
+void orc_fbm(const double *p, int64_t n, int octaves, double *out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = orc_fbm_v(D3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), octaves);
+}
+void orc_noise_field(int32_t type, const double params8[8], int vec, const double *p, int64_t n, double *out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = noise_field(type, params8, vec, D3(p[3 * i], p[3 * i + 1], p[3 * i + 2]));
+}
+
 /* ---- procedural nonstationary fields (D3: defined in code) ---------- */
 static double ns_ls_value(const orc_field *f, v3d p) {
     const orc_f32 *q = f->d.ns_ls_params;
+    if (f->d.ns_ls_field == ORC_LS_NOISE) return noise_field(f->d.ns_ls_noise, f->d.ns_ls_noise_params, 1, p);
     switch (f->d.ns_ls_field) {
     case ORC_LS_SINCOS: {
         double fr = q[2];
@@ -262,8 +407,9 @@ static double ns_ls_value(const orc_field *f, v3d p) {
 
 /* ---- CovarianceFunction virtuals (GPFunctions.cpp:610-875, 1225-1345, 1600-1760) */
 static float cov_amplitude(const orc_field *f, v3f p) {
-    (void)p;
     if (is_se(f)) return f->d.sigma;                       /* :721 */
+    if (f->d.ns_var_field == ORC_VAR_NOISE)                 /* getVariance(p) * sigma, :1245-1247 */
+        return (float)(noise_field(f->d.ns_var_noise, f->d.ns_var_noise_params, 0, d_of(p)) * (double)f->d.sigma);
     return (float)((double)f->d.ns_var * f->d.sigma);       /* :1241-1243 */
 }
 static float cov_lateral(const orc_field *f, v3f p) {

@@ -57,7 +57,11 @@ enum { ORC_SDF_KNOB = 0, ORC_SDF_KNOB_INNER = 1, ORC_SDF_KNOB_OUTER = 2, ORC_SDF
        ORC_SDF_PLANE = 4, ORC_SDF_ROUNDED_U_GROOVE = 5, ORC_SDF_TWO_SLABS = 6, ORC_SDF_THREE_SLABS = 7 };
 enum { ORC_CTX_GLOBAL = 0, ORC_CTX_RENEWAL = 1, ORC_CTX_RENEWAL_PLUS = 2, ORC_CTX_NONE = 3 };
 enum { ORC_SCHEME_UNI = 0, ORC_SCHEME_NEE = 1, ORC_SCHEME_MIS = 2 };
-enum { ORC_LS_CONST = 0, ORC_LS_SINCOS = 1 };
+enum { ORC_LS_CONST = 0, ORC_LS_SINCOS = 1, ORC_LS_NOISE = 2 };
+enum { ORC_VAR_CONST = 0, ORC_VAR_NOISE = 1 };
+/* ProceduralNoise::NoiseType (GPFunctions.hpp:655-662) */
+enum { ORC_NOISE_BOTTOM_TOP = 0, ORC_NOISE_LEFT_RIGHT = 1, ORC_NOISE_FRONT_BACK = 2,
+       ORC_NOISE_BOTTOM_TOP_LEFT_RIGHT = 3, ORC_NOISE_SANDSTONE = 4, ORC_NOISE_RUST = 5 };
 
 typedef struct orc_medium_desc {
     /* covariance (SquaredExponentialCovariance JSON: sigma, lengthScale, localScale, aniso) */
@@ -104,6 +108,13 @@ typedef struct orc_medium_desc {
     int32_t sdf_function;
     orc_f32 sdf_inverse_transform[12];
     orc_f32 sdf_min, sdf_scale, sdf_offset;
+    /* ProceduralNonstationaryCovariance fields of type "noise" (ProceduralNoise "var",
+     * ProceduralNoiseVec "ls"; GPFunctions.hpp:647-830): ns_var_field ORC_VAR_NOISE and/or
+     * ns_ls_field ORC_LS_NOISE; the noise type and its JSON min, max, start, end, min2, max2,
+     * start2, end2 */
+    int32_t ns_var_field;
+    int32_t ns_var_noise, ns_ls_noise;
+    double ns_var_noise_params[8], ns_ls_noise_params[8];
 } orc_medium_desc;
 
 /* Dense-grid density at a world point (trilinear, clamped), and TabulatedMean::mean
@@ -244,6 +255,13 @@ void orc_conditioned_eval_ls(const orc_medium_desc *desc, const orc_ray *rays, c
                              const float *q, const float *t_segment, int64_t n, float *out4);
 void orc_nee_ls(const orc_medium_desc *desc, const orc_ray *rays, const float *ls, const float *normals,
                 const float *points, const float *t_segment, int64_t n, float *grads, float *pdfs);
+
+/* ProceduralNoise pieces (tests): SdfFunctions' random3, simplex3d, fbm; the scalar
+ * ("var") and vector ("ls", its largest component as getKernelScale takes it) fields. */
+void orc_random3(const float *p, int64_t n, float *out);
+void orc_simplex3d(const float *p, int64_t n, float *out);
+void orc_fbm(const double *p, int64_t n, int octaves, double *out);
+void orc_noise_field(int32_t type, const double params8[8], int vec, const double *p, int64_t n, double *out);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,9 @@ MEAN_SPHERE, MEAN_BOX, MEAN_PLANE, MEAN_GRID, MEAN_PROC = 0, 1, 2, 3, 4
  SDF_THREE_SLABS) = range(8)
 CTX_GLOBAL, CTX_RENEWAL, CTX_RENEWAL_PLUS, CTX_NONE = 0, 1, 2, 3
 SCHEME_UNI, SCHEME_NEE, SCHEME_MIS = 0, 1, 2
-LS_CONST, LS_SINCOS = 0, 1
+LS_CONST, LS_SINCOS, LS_NOISE = 0, 1, 2
+VAR_CONST, VAR_NOISE = 0, 1
+NOISE_TYPES = ("bottom_top", "left_right", "front_back", "bottom_top_left_right", "sandstone", "rust")
 EXIT, HIT, FAIL = 0, 1, 2
 FAIL_NONE, FAIL_OUTWARD, FAIL_ZERO_GRAD, FAIL_NONFINITE, FAIL_BOUNCE_LIMIT = 0, 1, 2, 3, 4
 
@@ -44,6 +46,8 @@ class MediumDesc(C.Structure):
         ("iso3d", C.c_int32),
         ("sdf_function", C.c_int32), ("sdf_inverse_transform", C.c_float * 12),
         ("sdf_min", C.c_float), ("sdf_scale", C.c_float), ("sdf_offset", C.c_float),
+        ("ns_var_field", C.c_int32), ("ns_var_noise", C.c_int32), ("ns_ls_noise", C.c_int32),
+        ("ns_var_noise_params", C.c_double * 8), ("ns_ls_noise_params", C.c_double * 8),
     ]
 
 
@@ -159,6 +163,7 @@ def ref():
         L.ref_standard_normals.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_double)]
         L.ref_tangent_frame.argtypes = [f32p, f32p]
         L.ref_pcg_words.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_uint32)]
+        L.ref_random3.restype = L.ref_simplex3d.restype = L.ref_fbm.restype = None
         _ref = L
     return _ref
 
@@ -242,6 +247,13 @@ def make_desc(**kw) -> MediumDesc:
     d.sdf_min = kw.get("sdf_min", -3.4028234663852886e38)
     d.sdf_scale = kw.get("sdf_scale", 1.0)
     d.sdf_offset = kw.get("sdf_offset", 0.0)
+    d.ns_var_field = kw.get("ns_var_field", VAR_CONST)
+    d.ns_var_noise = kw.get("ns_var_noise", 0)
+    d.ns_ls_noise = kw.get("ns_ls_noise", 0)
+    for i, a in enumerate(kw.get("ns_var_noise_params", (1.0, 500.0, 0.0, 1.0, 1.0, 500.0, 0.0, 1.0))):
+        d.ns_var_noise_params[i] = a
+    for i, a in enumerate(kw.get("ns_ls_noise_params", (1.0, 500.0, 0.0, 1.0, 1.0, 500.0, 0.0, 1.0))):
+        d.ns_ls_noise_params[i] = a
     g = kw.get("grid")
     if g is not None:
         set_grid(d, g.voxels, g.origin, g.world_to_index, g.clamp_min, g.clamp_max,
@@ -521,4 +533,48 @@ def conditioned_eval_ls64(desc: MediumDesc, rays: np.ndarray, ls, q, t_segment):
     out = np.zeros((n, 4), np.float64)
     lib64().orc_conditioned_eval_ls(C.byref(desc), rays.ctypes.data, None if lsa is None else lsa.ctypes.data,
                                     qq.ctypes.data, ts.ctypes.data, n, out.ctypes.data)
+    return out
+
+
+# ---- ProceduralNoise fields (oracle restatement, and the reference build's fbm) ----
+def _noise_io(p):
+    return np.ascontiguousarray(p, np.float64).reshape(-1, 3)
+
+
+def random3(p, which="oracle"):
+    L = lib() if which == "oracle" else ref()
+    q = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+    out = np.zeros_like(q)
+    fn = L.orc_random3 if which == "oracle" else L.ref_random3
+    fn(q.ctypes.data_as(C.POINTER(C.c_float)), C.c_int64(len(q)), out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def simplex3d(p, which="oracle"):
+    L = lib() if which == "oracle" else ref()
+    q = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+    out = np.zeros(len(q), np.float32)
+    fn = L.orc_simplex3d if which == "oracle" else L.ref_simplex3d
+    fn(q.ctypes.data_as(C.POINTER(C.c_float)), C.c_int64(len(q)), out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def fbm(p, octaves, which="oracle"):
+    L = lib() if which == "oracle" else ref()
+    q = _noise_io(p)
+    out = np.zeros(len(q), np.float64)
+    fn = L.orc_fbm if which == "oracle" else L.ref_fbm
+    fn(q.ctypes.data_as(C.POINTER(C.c_double)), C.c_int64(len(q)), C.c_int(octaves),
+       out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def noise_field(kind, params8, p, vec=False):
+    """ProceduralNoise (vec False) or the largest component of ProceduralNoiseVec (vec True)
+    of type `kind` (an index of NOISE_TYPES) at points p."""
+    q = _noise_io(p)
+    out = np.zeros(len(q), np.float64)
+    prm = (C.c_double * 8)(*[float(v) for v in params8])
+    lib().orc_noise_field(C.c_int32(kind), prm, C.c_int(1 if vec else 0), q.ctypes.data_as(C.POINTER(C.c_double)),
+                          C.c_int64(len(q)), out.ctypes.data_as(C.POINTER(C.c_double)))
     return out

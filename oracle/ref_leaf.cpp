@@ -13,6 +13,7 @@
 //   src/core/sampling/Gaussian.cpp:22-34,89-104  rand_normal_2, sample_standard_normal
 //   src/core/math/SdfFunctionsQueries.cpp        SdfFunctions::evaluateJet, eval (float API)
 //   src/core/math/Mat4f.{hpp,cpp}                rotYXZ, invert, transformVector
+//   src/core/math/SdfFunctions.cpp:84-183        random3, simplex3d, fbm (the ProceduralNoise fields)
 #include "math/Mat4f.hpp"
 #include "math/MathUtil.hpp"
 #include "math/SdfFunctions.hpp"
@@ -24,7 +25,26 @@
 
 using namespace Tungsten;
 
+namespace Tungsten {
+Vec3f random3(Vec3f c);
+float simplex3d(Vec3f p);
+}
+
 extern "C" {
+
+// random3, simplex3d and fbm of SdfFunctions.cpp at n points (xyz triples).
+void ref_random3(const float *p, int64_t n, float *out) {
+    for (int64_t i = 0; i < n; ++i) {
+        Vec3f r = random3(Vec3f(p[3 * i], p[3 * i + 1], p[3 * i + 2]));
+        out[3 * i] = r.x(); out[3 * i + 1] = r.y(); out[3 * i + 2] = r.z();
+    }
+}
+void ref_simplex3d(const float *p, int64_t n, float *out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = simplex3d(Vec3f(p[3 * i], p[3 * i + 1], p[3 * i + 2]));
+}
+void ref_fbm(const double *p, int64_t n, int octaves, double *out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = fbm(Vec3d(p[3 * i], p[3 * i + 1], p[3 * i + 2]), octaves);
+}
 
 uint32_t ref_xxhash32_1(uint32_t p) { return MathUtil::xxhash32(uint(p)); }
 uint32_t ref_xxhash32_2(uint32_t x, uint32_t y) { return MathUtil::xxhash32(Vec2u(x, y)); }

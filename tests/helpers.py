@@ -20,7 +20,9 @@ def oracle_desc(med):
         max_bounce=med.max_bounces, grid=med.mean_grid, grid_offset=med.mean_offset, grid_scale=med.mean_scale,
         grid_volume=med.mean_volume, iso3d=int(med.isotropic_3D_sampling and not med.one_d),
         sdf_function=med.sdf_function, sdf_inverse_transform=med.sdf_inverse_transform, sdf_min=med.sdf_min,
-        sdf_scale=med.sdf_scale, sdf_offset=med.sdf_offset)
+        sdf_scale=med.sdf_scale, sdf_offset=med.sdf_offset, ns_var_field=med.ns_var_field,
+        ns_var_noise=med.ns_var_noise, ns_var_noise_params=med.ns_var_noise_params, ns_ls_noise=med.ns_ls_noise,
+        ns_ls_noise_params=med.ns_ls_noise_params)
     return d
 
 
