@@ -60,6 +60,7 @@ class FieldDesc(C.Structure):
         ("sdf_min", C.c_float), ("sdf_scale", C.c_float), ("sdf_offset", C.c_float),
         ("ns_var_field", C.c_int32), ("ns_var_noise", C.c_int32), ("ns_ls_noise", C.c_int32),
         ("ns_var_noise_params", C.c_double * 8), ("ns_ls_noise_params", C.c_double * 8),
+        ("use_aniso_mtx", C.c_int32), ("aniso_mtx", C.c_float * 9),
     ]
 
 

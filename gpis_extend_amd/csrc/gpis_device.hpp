@@ -190,6 +190,15 @@ struct FieldParams {
     int var_field; NoiseField var_noise, ls_noise;
     int N; float Nf;
     int iso;                  // isotropic kernel: exponent = c * r2
+    // full anisotropy matrix (useAnisoMtx; stationary SE in world space, NI = -1 kernels):
+    // with M = cinv / 2 (cinv = W^T W, W the world-to-local matrix) the splat in grid units
+    // t is exp2(fc . (tx^2, ty^2, tz^2, tx ty, tx tz, ty tz)), fc = -log2 e R^2 (M00, M11,
+    // M22, 2 M01, 2 M02, 2 M12); fg = -2 R M takes the moments sum w e t to the world
+    // gradient; fk holds cinv / 4 (the covariance splat of the conditioning, 00 11 22 01 02
+    // 12) and fhinv the inverse of -2 cinv / 4 (sparseConvNoiseOneOverSecondDerivative)
+    int full;
+    float fc[6], fg[9], fk[6];
+    double fhinv[9];
     int seed_add_single;      // floor(log(kss)/log(2.5)) for the single-level grid
     int level_seed_add[65];   // the same for multi-resolution level L = -32 .. 32, indexed L + 32
     // single-level world-space constants (stationary SE)

@@ -467,10 +467,105 @@ __device__ __forceinline__ void cond_splat(const FieldParams &P, const RealState
     n.gz += (rs.a * d2 + bd * d2 - 2.f * m2 * rs.bz) * k;
 }
 
-// Procedural length-scale field of the nonstationary covariance, evaluated in double at
-// the double-widened point and rounded once to float (the covariance's lateral scale
-// getKernelScale, GPFunctions.cpp:1734-1740, takes a Vec3d).
-// Kernels with the generic impulse count instantiated as NI = -1 carry the noise fields.
+// One pair of the full-matrix splat (useAnisoMtx): the exponent is the quadratic form
+// t^T M t in grid units with the host's coefficients fc (splattingKernel3DVal with the
+// matrix getInvCovMtx, GPFunctions.cpp:784-821); the support test is the same |t|^2 < 1.
+__device__ __forceinline__ void pair_full(const float fc[6], float tx, float ty, float tz, float w, float &v,
+                                          float &ax, float &ay, float &az) {
+    const float r2 = support_r2(tx, ty, tz);
+    const float q = fc[0] * (tx * tx) + fc[1] * (ty * ty) + fc[2] * (tz * tz) + fc[3] * (tx * ty) +
+                    fc[4] * (tx * tz) + fc[5] * (ty * tz);
+    const float we = (r2 < 1.0f) ? w * hw_exp2(q) : 0.0f;
+    v += we;
+    ax += we * tx;
+    ay += we * ty;
+    az += we * tz;
+}
+
+// The 27-cell sum of the full-matrix splat at world point p (noise3D, SparseConvolutionNoise.cpp
+// :419-452): impulses from the packed table where it holds the cell, else from the generator;
+// returns (sum w e, sum w e t) in grid units.  Out of line: only the kernels instantiated with
+// the generic impulse count (NI = -1) contain the call.
+__device__ __noinline__ Eval4 noise3d_full(const FieldParams &P, float px, float py, float pz, uint32_t seed) {
+    const float gx = px / P.R, gy = py / P.R, gz = pz / P.R;
+    const float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+    const float rx = gx - fx, ry = gy - fy, rz = gz - fz;
+    const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    const bool tab = P.has_table && seed == P.tseed;
+    const int N = P.N;
+    float v = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
+    for (int dx = -1; dx <= 1; ++dx) {
+        for (int dy = -1; dy <= 1; ++dy) {
+            for (int dz = -1; dz <= 1; ++dz) {
+                if (axis_bound2(rx, dx + 1) + axis_bound2(ry, dy + 1) + axis_bound2(rz, dz + 1) >= kCullR2) continue;
+                const int cx = ix + dx, cy = iy + dy, cz = iz + dz;
+                const float lx = rx - (float)dx, ly = ry - (float)dy, lz = rz - (float)dz;
+                const unsigned ux = (unsigned)(cx - P.tlo[0]), uy = (unsigned)(cy - P.tlo[1]),
+                               uz = (unsigned)(cz - P.tlo[2]);
+                if (tab && ux < (unsigned)P.tdim[0] && uy < (unsigned)P.tdim[1] && uz < (unsigned)P.tdim[2]) {
+                    const size_t qi = ((size_t)(uz * (unsigned)P.tdim[1] + uy) * (unsigned)P.tdim[0] + ux) * (3 * N);
+                    GPIS_CHECK_INDEX(qi + 3 * N - 1, (size_t)P.table_n);
+                    const float *q = P.table + qi;
+                    for (int k = 0; k < N; ++k) {
+                        const float xs = q[3 * k];
+                        pair_full(P.fc, lx - fabsf(xs), ly - q[3 * k + 1], lz - q[3 * k + 2], copysignf(1.0f, xs), v,
+                                  ax, ay, az);
+                    }
+                } else {
+                    Pcg g;
+                    g.set((uint64_t)(uint32_t)(xxh4((uint32_t)cz, (uint32_t)cy, (uint32_t)cx, seed) + 1u));
+                    for (int k = 0; k < N; ++k) {
+                        const float iz_ = u01(g.next());
+                        const float iy_ = u01(g.next());
+                        const float ix_ = u01(g.next());
+                        const float w = u01(g.next()) < 0.5f ? -1.0f : 1.0f;
+                        pair_full(P.fc, lx - ix_, ly - iy_, lz - iz_, w, v, ax, ay, az);
+                    }
+                }
+            }
+        }
+    }
+    Eval4 r = {v, ax, ay, az};
+    return r;
+}
+
+// The normalised full-matrix noise: value and the world gradient -2 R M (sum w e t).
+__device__ __forceinline__ Eval4 noise_full_normalized(const FieldParams &P, float px, float py, float pz,
+                                                       uint32_t seed) {
+    const Eval4 n = noise3d_full(P, px, py, pz, seed);
+    const float *G = P.fg;
+    Eval4 r;
+    r.v = n.v * P.nf_inv;
+    r.gx = (G[0] * n.gx + G[1] * n.gy + G[2] * n.gz) * P.nf_inv;
+    r.gy = (G[3] * n.gx + G[4] * n.gy + G[5] * n.gz) * P.nf_inv;
+    r.gz = (G[6] * n.gx + G[7] * n.gy + G[8] * n.gz) * P.nf_inv;
+    return r;
+}
+
+// Conditioning splats of the full matrix: kernel exp(-a^T K a) with K = cinv / 4 (isCov),
+// first derivatives d = -2 K a and second derivatives d d^T - 2 K (splattingKernel3D /
+// splattingKernel3DGrad with splattingKernel3D2ndGrad, GPFunctions.cpp:555-580, 829-843).
+__device__ __forceinline__ void cond_splat_full(const FieldParams &P, const RealState &rs, float px, float py,
+                                                float pz, Eval4 &n) {
+    const float *K = P.fk;  // 00 11 22 01 02 12
+    const float ax = px - rs.ox, ay = py - rs.oy, az = pz - rs.oz;
+    const float r0 = ax * K[0] + ay * K[3] + az * K[4];
+    const float r1 = ax * K[3] + ay * K[1] + az * K[5];
+    const float r2 = ax * K[4] + ay * K[5] + az * K[2];
+    const float k = __expf(-(r0 * ax + r1 * ay + r2 * az));
+    const float d0 = -2.f * r0, d1 = -2.f * r1, d2 = -2.f * r2;
+    const float bd = rs.bx * d0 + rs.by * d1 + rs.bz * d2;
+    const float kb0 = K[0] * rs.bx + K[3] * rs.by + K[4] * rs.bz;
+    const float kb1 = K[3] * rs.bx + K[1] * rs.by + K[5] * rs.bz;
+    const float kb2 = K[4] * rs.bx + K[5] * rs.by + K[2] * rs.bz;
+    n.v += (rs.a + bd) * k;
+    n.gx += (rs.a * d0 + bd * d0 - 2.f * kb0) * k;
+    n.gy += (rs.a * d1 + bd * d1 - 2.f * kb1) * k;
+    n.gz += (rs.a * d2 + bd * d2 - 2.f * kb2) * k;
+}
+
+// Kernels with the generic impulse count instantiated as NI = -1 carry the "noise" fields of
+// the nonstationary covariance and the full anisotropy matrix.
 template <int NI> struct HasNoise { static constexpr bool value = NI < 0; };
 
 // getKernelScale (GPFunctions.cpp:1734-1740): the largest float component of the ls field at
@@ -588,6 +683,12 @@ __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealStat
         }
         return r;
     }
+    if (HasNoise<NI>::value && P.full) {
+        // full anisotropy matrix: stationary SE, single level, world space
+        Eval4 r = noise_full_normalized(P, px, py, pz, rs.seed);
+        if (cond && rs.cond) cond_splat_full(P, rs, px, py, pz, r);
+        return r;
+    }
     if (!P.multires) {
         Eval4 r;
         float ls = 1.0f, gs = 1.0f;
@@ -676,7 +777,7 @@ template <int NI, bool ISO, int MEAN = MEAN_ANY, bool IRAY = false>
 __device__ __forceinline__ void eval_field2(const FieldParams &P, const RealState &rs, const float pa[3],
                                             const float pb[3], bool two, bool cond, float dx, float dy, float dz,
                                             Eval4 &A, Eval4 &B) {
-    if (!IRAY && !P.multires && P.kind == 0) {
+    if (!IRAY && !P.multires && P.kind == 0 && !(HasNoise<NI>::value && P.full)) {
         Eval4 na, nb;
         noise3d_raw2<NI, ISO>(P, pa, pb, two, rs.seed, P.R, P.c2[0], P.c2[1], P.c2[2], na, nb);
         A.v = na.v * P.nf_inv;

@@ -137,6 +137,89 @@ static gpis_status build_sdf_mean(gpis_ctx *c) {
     return GPIS_OK;
 }
 
+// 3x3 float inverse by cofactors and one reciprocal of the determinant (the closed form of
+// Eigen's size-3 inverse); returns the determinant.
+static float inverse3f(const float m[3][3], float r[3][3]) {
+    const float c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+    const float c10 = m[0][2] * m[2][1] - m[0][1] * m[2][2];
+    const float c20 = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+    const float det = c00 * m[0][0] + c10 * m[1][0] + c20 * m[2][0];
+    const float inv = 1.0f / det;
+    r[0][0] = c00 * inv; r[0][1] = c10 * inv; r[0][2] = c20 * inv;
+    r[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) * inv;
+    r[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * inv;
+    r[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * inv;
+    r[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * inv;
+    r[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * inv;
+    r[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * inv;
+    return det;
+}
+
+// The full anisotropy matrix of the SE covariance (SquaredExponentialCovariance::fromJson with
+// useAnisoMtx, GPFunctions.cpp:675-681): L = l_conv * anisoMtx, W = L^-1, cinv = W^T W.
+// splattingKernelRadius takes the largest component of L's row sums (:714-717),
+// sparseConvNoiseVariance3D the factor 1 / sqrt(det cinv) (:757-758), getInvCovMtx cinv
+// (:800-806), sparseConvNoiseOneOverSecondDerivative the inverse of -2 cinv / 4 (:738-748).
+static gpis_status build_aniso_mtx(gpis_ctx *c, float l_conv) {
+    const gpis_field_desc &d = c->desc;
+    FieldParams &P = c->P;
+    if (d.cov_kind != GPIS_COV_SE)
+        return fail(c, GPIS_UNSUPPORTED, "anisoMtx is served for the stationary SE kernel");
+    if (d.one_d || d.isotropic_3d)
+        return fail(c, GPIS_UNSUPPORTED, "anisoMtx is served for 3D sampling in world space");
+    float A[3][3], W[3][3], ci[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A[i][j] = l_conv * d.aniso_mtx[3 * i + j];
+    const float detA = inverse3f(A, W);
+    bool finite = std::isfinite(detA) && detA != 0.0f;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            float s = W[0][i] * W[0][j];
+            s += W[1][i] * W[1][j];
+            s += W[2][i] * W[2][j];
+            ci[i][j] = s;
+            finite = finite && std::isfinite(W[i][j]);
+        }
+    const float cdet = ci[0][0] * (ci[1][1] * ci[2][2] - ci[1][2] * ci[2][1]) -
+                       ci[0][1] * (ci[1][0] * ci[2][2] - ci[1][2] * ci[2][0]) +
+                       ci[0][2] * (ci[1][0] * ci[2][1] - ci[1][1] * ci[2][0]);
+    if (!finite || !(cdet > 0.0f) || !std::isfinite(cdet))
+        return fail(c, GPIS_INVALID_ARGUMENT, "anisoMtx must be a finite invertible matrix");
+    float e[3];
+    for (int i = 0; i < 3; ++i) e[i] = (A[i][0] + A[i][1]) + A[i][2];
+    P.l_max = fmaxf(fmaxf(e[0], e[1]), e[2]);
+    if (!(P.l_max > 0.0f)) return fail(c, GPIS_INVALID_ARGUMENT, "anisoMtx gives a non-positive kernel radius");
+    P.R = P.kernel_scale * 1.0f * P.l_max;
+    P.invR = 1.0f / P.R;
+    const float R2 = P.R * P.R, k = -1.4426950408889634f;
+    const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
+    for (int q = 0; q < 6; ++q) {
+        const float m = ci[ii[q]][jj[q]] * 0.5f;
+        P.fc[q] = k * (R2 * (q < 3 ? m : 2.0f * m));
+        P.fk[q] = ci[ii[q]][jj[q]] * 0.5f * 0.5f;
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) P.fg[3 * i + j] = -2.f * P.R * (ci[i][j] * 0.5f);
+    const double var = (double)(P.Nf / (P.R * P.R * P.R)) * pow(M_PI, 1.5) * (1.0 / sqrt((double)cdet));
+    P.nf_inv = (float)(1.0 / sqrt((double)(float)var));
+    // inverse of H = -2 cinv / 4 in double from the float entries of H
+    double H[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) H[i][j] = (double)(-2.f * (ci[i][j] * 0.25f));
+    const double h00 = H[1][1] * H[2][2] - H[1][2] * H[2][1], h10 = H[0][2] * H[2][1] - H[0][1] * H[2][2],
+                 h20 = H[0][1] * H[1][2] - H[0][2] * H[1][1];
+    const double hdet = h00 * H[0][0] + h10 * H[1][0] + h20 * H[2][0];
+    const double hi[9] = {h00, h10, h20,
+                          H[1][2] * H[2][0] - H[1][0] * H[2][2], H[0][0] * H[2][2] - H[0][2] * H[2][0],
+                          H[0][2] * H[1][0] - H[0][0] * H[1][2],
+                          H[1][0] * H[2][1] - H[1][1] * H[2][0], H[0][1] * H[2][0] - H[0][0] * H[2][1],
+                          H[0][0] * H[1][1] - H[0][1] * H[1][0]};
+    for (int q = 0; q < 9; ++q) P.fhinv[q] = hi[q] / hdet;
+    P.iso = 0;
+    P.full = 1;
+    return GPIS_OK;
+}
+
 static gpis_status build_params(gpis_ctx *c) {
     const gpis_field_desc &d = c->desc;
     FieldParams &P = c->P;
@@ -226,6 +309,10 @@ static gpis_status build_params(gpis_ctx *c) {
     double var = (double)(P.Nf / (P.R * P.R * P.R)) * pow(M_PI, 1.5) *
                  (double)(P.l_aniso[0] * P.l_aniso[1] * P.l_aniso[2]);
     P.nf_inv = (float)(1.0 / sqrt((double)(float)var));
+    if (d.use_aniso_mtx) {
+        gpis_status s = build_aniso_mtx(c, l_conv);
+        if (s != GPIS_OK) return s;
+    }
     P.mean_kind = d.mean_kind;
     for (int i = 0; i < 3; ++i) {
         P.mc[i] = (float)d.mean_center[i];
@@ -541,6 +628,12 @@ __device__ __noinline__ void condition_coefficients(const FieldParams &P, const 
             b0 = -2.0 * ((double)F.tx * l0 + (double)F.ty * l1 + (double)F.tz * l2);
             b1 = -2.0 * ((double)F.bx * l0 + (double)F.by * l1 + (double)F.bz * l2);
             b2 = -2.0 * ((double)F.nx * l0 + (double)F.ny * l1 + (double)F.nz * l2);
+        } else if (NOISE && P.full) {
+            // the full matrix: the inverse of the covariance's second derivative -2 cinv / 4
+            const double *H = P.fhinv;
+            b0 = H[0] * del0 + H[1] * del1 + H[2] * del2;
+            b1 = H[3] * del0 + H[4] * del1 + H[5] * del2;
+            b2 = H[6] * del0 + H[7] * del1 + H[8] * del2;
         } else {
             // inverse second derivative of the covariance at zero lag: -2 / l_aniso_inv_i^2
             b0 = -2.0 / ((double)P.l_aniso_inv[0] * P.l_aniso_inv[0]) * del0 / gsum;
@@ -2167,8 +2260,9 @@ static void launch_flights(gpis_ctx *c, unsigned blocks, unsigned threads, hipSt
                                c->P_dev, in, out, pout, pseeds, n, c->work, c->counters, max_bounces, bounce_seed, box, c->batch_k);
         return;
     }
-    // "noise" variance or length-scale fields: the generic kernels that carry them (NI = -1)
-    const bool noise = c->P.var_field == 1 || c->P.ls_field == GPIS_LS_NOISE;
+    // "noise" variance or length-scale fields and the full anisotropy matrix: the generic
+    // kernels that carry them (NI = -1)
+    const bool noise = c->P.var_field == 1 || c->P.ls_field == GPIS_LS_NOISE || c->P.full;
     if (noise && c->P.iso && !cond) GPIS_FLIGHT_LAUNCH(-1, true, false);
     else if (noise && c->P.iso) GPIS_FLIGHT_LAUNCH(-1, true, true);
     else if (noise && !cond) GPIS_FLIGHT_LAUNCH(-1, false, false);
@@ -2442,7 +2536,7 @@ gpis_status gpis_eval_points(gpis_ctx *c, const float *x, const float *y, const 
                                (hipStream_t)stream, c->P_dev, x, y, z, (long long)n, seed, v, gx, gy, gz);            \
     } while (0)
     const bool ext = c->P.mean_kind == GPIS_MEAN_GRID || c->P.mean_kind == GPIS_MEAN_PROCEDURAL;
-    const bool noise = c->P.var_field == 1 || c->P.ls_field == GPIS_LS_NOISE;
+    const bool noise = c->P.var_field == 1 || c->P.ls_field == GPIS_LS_NOISE || c->P.full;
     if (noise && c->P.iso) GPIS_EVAL_LAUNCH(-1, true);
     else if (noise) GPIS_EVAL_LAUNCH(-1, false);
     else if (c->P.N == 8 && c->P.iso) GPIS_EVAL_LAUNCH(8, true);

@@ -148,6 +148,9 @@ class SparseConvolutionNoiseMedium:
     length_scale: float = 1.0
     local_scale: float = 3.0
     aniso: tuple = (1.0, 1.0, 1.0)
+    # "useAnisoMtx" and the row-major "anisoMtx" (GPFunctions.cpp:663-681)
+    use_aniso_mtx: bool = False
+    aniso_mtx: tuple = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
     ns_multires: bool = False
     ns_ls_field: int = A.LS_CONST
     ns_ls_params: tuple = (1.0, 0.0, 0.0, 0.0)
@@ -294,6 +297,9 @@ class SparseConvolutionNoiseMedium:
             m.ns_var = float(var)
             cov = cov.get("cov", {"type": "squared_exponential"})
             ct = cov.get("type")
+            if cov.get("useAnisoMtx", False):
+                raise MediumConfigError("useAnisoMtx is served for the stationary squared_exponential "
+                                        "covariance, not inside proc_nonstationary")
         if ct != "squared_exponential":
             raise MediumConfigError(f"covariance {ct!r} is not served by the GPU flight path")
         m.sigma = float(cov.get("sigma", 1.0))
@@ -302,7 +308,15 @@ class SparseConvolutionNoiseMedium:
         a = cov.get("aniso", (1.0, 1.0, 1.0))
         m.aniso = tuple(float(x) for x in (a if isinstance(a, (list, tuple)) else (a, a, a)))
         if cov.get("useAnisoMtx", False):
-            raise MediumConfigError("useAnisoMtx is not served by the GPU flight path")
+            mt = cov.get("anisoMtx")
+            # JsonPtr::get(Eigen::Matrix3f): a 9-element array, row-major
+            if not isinstance(mt, (list, tuple)) or len(mt) != 9:
+                raise MediumConfigError("useAnisoMtx needs \"anisoMtx\": an array of 9 numbers, row-major")
+            m.use_aniso_mtx = True
+            m.aniso_mtx = tuple(float(np.float32(x)) for x in mt)
+            if m.one_d or m.isotropic_3D_sampling:
+                raise MediumConfigError("useAnisoMtx is served for 3D sampling in world space "
+                                        "(not 1D_sampling or isotropic_3D_sampling)")
         return m
 
     def to_desc(self) -> A.FieldDesc:
@@ -337,6 +351,9 @@ class SparseConvolutionNoiseMedium:
         for i in range(12):
             d.sdf_inverse_transform[i] = self.sdf_inverse_transform[i]
         d.sdf_min, d.sdf_scale, d.sdf_offset = self.sdf_min, self.sdf_scale, self.sdf_offset
+        d.use_aniso_mtx = int(self.use_aniso_mtx)
+        for i in range(9):
+            d.aniso_mtx[i] = self.aniso_mtx[i]
         return d
 
     @property
@@ -344,7 +361,12 @@ class SparseConvolutionNoiseMedium:
         """World cell size of the 3D grid, splattingKernelRadius(false, 1) (GPFunctions.cpp:706-719)."""
         f = np.float32
         l_conv = f(self.length_scale) * np.sqrt(f(2.0)) / f(2)
-        la = [f(l_conv) * f(a) for a in self.aniso]
+        if self.use_aniso_mtx:
+            # the largest component of the row sums of l_conv * anisoMtx (:714-717)
+            L = [f(l_conv) * f(a) for a in self.aniso_mtx]
+            la = [(L[3 * i] + L[3 * i + 1]) + L[3 * i + 2] for i in range(3)]
+        else:
+            la = [f(l_conv) * f(a) for a in self.aniso]
         ls = f(1.0) if self.cov_kind == A.COV_SE or self.ns_multires else f(self.ns_ls_max)
         return float(f(self.local_scale) * ls * max(la))
 

@@ -226,6 +226,10 @@ struct orc_field {
     float l_conv;          /* _l_conv = _l*sqrt(2.f)/2 (GPFunctions.cpp:661) */
     v3f l_aniso, l_aniso_inv;
     int multires;          /* SparseConvolutionNoise.cpp:47 */
+    /* useAnisoMtx (GPFunctions.cpp:675-681): _local_to_world = l_conv * anisoMtx, its float
+     * inverse, _cov_mtx_inv = W^T W and its determinant */
+    int full;
+    float l2w[3][3], w2l[3][3], cinv[3][3], cinv_det;
     int activate;          /* :38 */
     int scheme1d;          /* :40-43 */
     float base;            /* :49 */
@@ -426,7 +430,15 @@ static float cov_world_sampling_scale(const orc_field *f) { return cov_ls_max(f)
 static float se_kernel_radius(const orc_field *f, int identity, float local_scale) { /* :706-719 */
     float s = f->d.kernel_scale;
     if (identity) return s;
-    float mf = fmaxf(fmaxf(f->l_aniso.x, f->l_aniso.y), f->l_aniso.z);
+    float mf;
+    if (!f->full) {
+        mf = fmaxf(fmaxf(f->l_aniso.x, f->l_aniso.y), f->l_aniso.z);
+    } else {
+        /* end_pt = col(0) + col(1) + col(2), its largest component */
+        float e[3];
+        for (int i = 0; i < 3; ++i) e[i] = (f->l2w[i][0] + f->l2w[i][1]) + f->l2w[i][2];
+        mf = fmaxf(fmaxf(e[0], e[1]), e[2]);
+    }
     return s * local_scale * mf;
 }
 static float cov_kernel_radius(const orc_field *f, int identity, float local_scale) {
@@ -437,7 +449,8 @@ static float se_var3d(const orc_field *f, float N, float R, int identity, float 
     double dens = N / (R * R * R);
     double det = 1.0;
     if (!identity) {
-        det = f->l_aniso.x * f->l_aniso.y * f->l_aniso.z;
+        if (!f->full) det = f->l_aniso.x * f->l_aniso.y * f->l_aniso.z;
+        else det = 1.0 / sqrt(f->cinv_det);
         det *= pow(gs, 3);
     }
     det *= pow(ls, 3);
@@ -489,9 +502,38 @@ static v3f inv_cov_diag(const orc_field *f, int is_cov, int is_iso, float gs, fl
     m = V3(m.x / l2, m.y / l2, m.z / l2);
     return vscale(0.5f, m);
 }
+/* getInvCovMtx with the full matrix (useAnisoMtx, not isotropic): _cov_mtx_inv / gs^2,
+ * times 0.5 for a covariance, / ls^2, times 0.5 (GPFunctions.cpp:784-812) */
+static m3f inv_cov_full(const orc_field *f, int is_cov, float gs, float ls) {
+    m3f m;
+    float g2 = sqrf(gs), l2 = sqrf(ls);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            float v = f->cinv[i][j] / g2;
+            if (is_cov) v *= 0.5f;
+            v /= l2;
+            m.m[i][j] = v * 0.5f;
+        }
+    return m;
+}
+/* ab^T M ab (dist2_ab, GPFunctions.hpp:77-80) and -2 ab . M.col(i) */
+static float quad_form(const m3f *m, v3f ab, v3f *dots) {
+    float r0 = ab.x * m->m[0][0] + ab.y * m->m[1][0] + ab.z * m->m[2][0];
+    float r1 = ab.x * m->m[0][1] + ab.y * m->m[1][1] + ab.z * m->m[2][1];
+    float r2 = ab.x * m->m[0][2] + ab.y * m->m[1][2] + ab.z * m->m[2][2];
+    if (dots) *dots = V3(-2.f * r0, -2.f * r1, -2.f * r2);
+    return (r0 * ab.x + r1 * ab.y) + r2 * ab.z;
+}
 /* CovarianceFunction::splattingKernel3D (GPFunctions.cpp:555-565, 814-827) */
 static v4f splat3d(const orc_field *f, v3f pa, v3f pb, int is_cov, int is_iso, float gs, v3f p_world) {
     float ls = cov_splat_scale(f, p_world);
+    if (f->full && !is_iso) {
+        m3f M = inv_cov_full(f, is_cov, gs, ls);
+        v3f dots;
+        float absq = quad_form(&M, vsub(pa, pb), &dots);
+        float v = (float)exp(-(double)absq);
+        return V4(v, dots.x * v, dots.y * v, dots.z * v);
+    }
     v3f m = inv_cov_diag(f, is_cov, is_iso, gs, ls);
     v3f ab = vsub(pa, pb);
     float absq = ab.x * m.x * ab.x + ab.y * m.y * ab.y + ab.z * m.z * ab.z;
@@ -501,6 +543,20 @@ static v4f splat3d(const orc_field *f, v3f pa, v3f pb, int is_cov, int is_iso, f
 /* CovarianceFunction::splattingKernel3DGrad (:567-580, 829-843) */
 static v4f splat3d_grad(const orc_field *f, v3f pa, v3f pb, v3f coeff, int is_cov, int is_iso, float gs, v3f p_world) {
     float ls = cov_splat_scale(f, p_world);
+    if (f->full && !is_iso) {
+        m3f M = inv_cov_full(f, is_cov, gs, ls);
+        v3f d;
+        float absq = quad_form(&M, vsub(pa, pb), &d);
+        float v = (float)exp(-(double)absq);
+        float dd[3] = {d.x, d.y, d.z};
+        v4f rows[3];
+        for (int i = 0; i < 3; ++i) {
+            float h[3];
+            for (int j = 0; j < 3; ++j) h[j] = (dd[i] * dd[j] - 2.f * M.m[i][j]) * v;
+            rows[i] = V4(dd[i] * v, h[0], h[1], h[2]);
+        }
+        return v4add(v4add(v4scale(coeff.x, rows[0]), v4scale(coeff.y, rows[1])), v4scale(coeff.z, rows[2]));
+    }
     v3f m = inv_cov_diag(f, is_cov, is_iso, gs, ls);
     v3f ab = vsub(pa, pb);
     float absq = ab.x * m.x * ab.x + ab.y * m.y * ab.y + ab.z * m.z * ab.z;
@@ -515,6 +571,11 @@ static v4f splat3d_grad(const orc_field *f, v3f pa, v3f pb, v3f coeff, int is_co
 /* sparseConvNoiseOneOverSecondDerivative (:723-749), no anisotropy field */
 static m3f one_over_second(const orc_field *f, int is_iso) {
     m3f M; memset(&M, 0, sizeof M);
+    if (f->full && !is_iso) {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) M.m[i][j] = -2.f * (f->cinv[i][j] * 0.25f);
+        return m3_inverse(M);
+    }
     v3f d = is_iso ? V3(1.f, 1.f, 1.f) : vmul(f->l_aniso_inv, f->l_aniso_inv);
     M.m[0][0] = d.x * 0.25f * -2.f; M.m[1][1] = d.y * 0.25f * -2.f; M.m[2][2] = d.z * 0.25f * -2.f;
     return m3_inverse(M);
@@ -1405,6 +1466,26 @@ orc_field *orc_field_create(const orc_medium_desc *desc) {
     if (!f->d.single_realization && f->d.one_d) f->scheme1d = f->d.scheme;
     f->multires = !is_se(f) && f->d.ns_multires;
     f->base = 2.5f;
+    if (f->d.use_aniso_mtx) {
+        m3f A;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) { A.m[i][j] = f->l_conv * f->d.aniso_mtx[3 * i + j]; f->l2w[i][j] = A.m[i][j]; }
+        m3f W = m3_inverse(A);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) f->w2l[i][j] = W.m[i][j];
+        /* _cov_mtx_inv = W^T W */
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                float s = W.m[0][i] * W.m[0][j];
+                s += W.m[1][i] * W.m[1][j];
+                s += W.m[2][i] * W.m[2][j];
+                f->cinv[i][j] = s;
+            }
+        const float (*c)[3] = f->cinv;
+        f->cinv_det = c[0][0] * (c[1][1] * c[2][2] - c[1][2] * c[2][1]) - c[0][1] * (c[1][0] * c[2][2] - c[1][2] * c[2][0]) +
+                      c[0][2] * (c[1][0] * c[2][1] - c[1][1] * c[2][0]);
+        f->full = 1;
+    }
     return f;
 }
 void orc_field_destroy(orc_field *f) { free(f); }

@@ -115,6 +115,9 @@ typedef struct orc_medium_desc {
     int32_t ns_var_field;
     int32_t ns_var_noise, ns_ls_noise;
     double ns_var_noise_params[8], ns_ls_noise_params[8];
+    /* "useAnisoMtx" and "anisoMtx" (row-major) of the SE covariance (GPFunctions.cpp:663-681) */
+    int32_t use_aniso_mtx;
+    orc_f32 aniso_mtx[9];
 } orc_medium_desc;
 
 /* Dense-grid density at a world point (trilinear, clamped), and TabulatedMean::mean

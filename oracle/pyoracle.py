@@ -48,6 +48,7 @@ class MediumDesc(C.Structure):
         ("sdf_min", C.c_float), ("sdf_scale", C.c_float), ("sdf_offset", C.c_float),
         ("ns_var_field", C.c_int32), ("ns_var_noise", C.c_int32), ("ns_ls_noise", C.c_int32),
         ("ns_var_noise_params", C.c_double * 8), ("ns_ls_noise_params", C.c_double * 8),
+        ("use_aniso_mtx", C.c_int32), ("aniso_mtx", C.c_float * 9),
     ]
 
 
@@ -254,6 +255,10 @@ def make_desc(**kw) -> MediumDesc:
         d.ns_var_noise_params[i] = a
     for i, a in enumerate(kw.get("ns_ls_noise_params", (1.0, 500.0, 0.0, 1.0, 1.0, 500.0, 0.0, 1.0))):
         d.ns_ls_noise_params[i] = a
+    m = kw.get("aniso_mtx")
+    d.use_aniso_mtx = int(m is not None)
+    for i, a in enumerate(np.asarray(m if m is not None else np.eye(3), np.float64).reshape(9)):
+        d.aniso_mtx[i] = float(a)
     g = kw.get("grid")
     if g is not None:
         set_grid(d, g.voxels, g.origin, g.world_to_index, g.clamp_min, g.clamp_max,

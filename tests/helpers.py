@@ -22,7 +22,8 @@ def oracle_desc(med):
         sdf_function=med.sdf_function, sdf_inverse_transform=med.sdf_inverse_transform, sdf_min=med.sdf_min,
         sdf_scale=med.sdf_scale, sdf_offset=med.sdf_offset, ns_var_field=med.ns_var_field,
         ns_var_noise=med.ns_var_noise, ns_var_noise_params=med.ns_var_noise_params, ns_ls_noise=med.ns_ls_noise,
-        ns_ls_noise_params=med.ns_ls_noise_params)
+        ns_ls_noise_params=med.ns_ls_noise_params,
+        aniso_mtx=np.asarray(med.aniso_mtx, np.float32).reshape(3, 3) if med.use_aniso_mtx else None)
     return d
 
 
