@@ -457,7 +457,7 @@ def run_points(args):
         dev.eval_points(x, y, z, med.seed)
 
     best = None
-    for sub in ((0, 3, 4, 6) if not args.no_autotune else (3,)):
+    for sub in ((0, 3, 4, 6, 8) if not args.no_autotune else (3,)):
         dev.set_lists(sub, 2)
         step()
         torch.cuda.synchronize()

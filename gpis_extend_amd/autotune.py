@@ -21,7 +21,7 @@ from typing import Callable, Dict, Tuple
 
 _CACHE: Dict[Tuple, Tuple[int, int, int, int]] = {}
 
-SUBCELLS = (3, 4, 6)
+SUBCELLS = (3, 4, 6, 8)
 WAVES_PER_CU = (8, 12, 16, 24)
 BLOCK_SIZES = (64, 128, 256)
 REFILL = (8, 16, 32)

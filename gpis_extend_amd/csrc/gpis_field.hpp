@@ -115,26 +115,102 @@ __device__ __forceinline__ void cell_generated(int cx, int cy, int cz, int n, ui
 }
 
 // One fixed-point impulse of the support lists: (X, Y, Z) in 2^-23 grid units and the
-// weight's float bits, read from `base` at byte offset `off`.
+// weight's float bits.
 struct Imp4 { int32_t x, y, z, w; };
-__device__ __forceinline__ Imp4 load_imp(const int32_t *base, uint32_t off) {
+
+// The two arrays of the support lists, read through buffer resources: a load's byte offset
+// is a 32-bit VGPR and the base lives in SGPRs, so no 64-bit address arithmetic runs per
+// load, and a load past an array's end returns zeros without touching memory (the lists'
+// loops read one group ahead).  The host check build reads the arrays through pointers, with
+// every index checked, and gives the same zeros past the end.
+struct ListSrc {
 #ifdef GPIS_HOST_CHECK
-    const int32_t *q = (const int32_t *)((const char *)base + off);
-    Imp4 r = {q[0], q[1], q[2], q[3]};
+    const uint32_t *ent; size_t ent_n;
+    const int32_t *fix; size_t fix_bytes;
 #else
-    const int4 q = *reinterpret_cast<const int4 *>((const char *)base + off);
-    Imp4 r = {q.x, q.y, q.z, q.w};
+    __amdgpu_buffer_rsrc_t ent, fix;
 #endif
-    return r;
-}
-__device__ __forceinline__ void load_group(const uint32_t *ent, uint32_t e[4]) {
+};
+#ifndef GPIS_HOST_CHECK
+typedef unsigned int gpis_u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kBufferWord3 = 0x00020000;  // raw buffer, 32-bit dwords (gfx9 family)
+#endif
+__device__ __forceinline__ ListSrc list_src(const FieldParams &P) {
+    ListSrc s;
 #ifdef GPIS_HOST_CHECK
-    for (int i = 0; i < 4; ++i) e[i] = ent[i];
+    s.ent = P.lent; s.ent_n = P.lent_n;
+    s.fix = P.lfix; s.fix_bytes = P.lfix_bytes;
 #else
-    const uint4 q = *reinterpret_cast<const uint4 *>(ent);
+    s.ent = __builtin_amdgcn_make_buffer_rsrc((void *)P.lent, (short)0, (int)(P.lent_n * 4u), kBufferWord3);
+    s.fix = __builtin_amdgcn_make_buffer_rsrc((void *)P.lfix, (short)0, (int)P.lfix_bytes, kBufferWord3);
+#endif
+    return s;
+}
+// The four entries of list group g (byte offsets into the fixed-point impulses).
+__device__ __forceinline__ void load_group(const ListSrc &s, uint32_t g, uint32_t e[4]) {
+#ifdef GPIS_HOST_CHECK
+    for (int i = 0; i < 4; ++i) {
+        const size_t k = 4 * (size_t)g + i;
+        e[i] = k < s.ent_n ? s.ent[k] : 0u;
+        if (k < s.ent_n) GPIS_CHECK_INDEX(k, s.ent_n);
+    }
+#else
+    const gpis_u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(s.ent, (int)(g * 16u), 0, 0);
     e[0] = q.x; e[1] = q.y; e[2] = q.z; e[3] = q.w;
 #endif
 }
+// The fixed-point impulse at byte offset `off`.
+__device__ __forceinline__ Imp4 load_imp(const ListSrc &s, uint32_t off) {
+#ifdef GPIS_HOST_CHECK
+    GPIS_CHECK_INDEX((size_t)off + 15, s.fix_bytes);
+    const int32_t *q = (const int32_t *)((const char *)s.fix + off);
+    Imp4 r = {q[0], q[1], q[2], q[3]};
+#else
+    const gpis_u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(s.fix, (int)off, 0, 0);
+    Imp4 r = {(int32_t)q.x, (int32_t)q.y, (int32_t)q.z, (int32_t)q.w};
+#endif
+    return r;
+}
+// "This lane's predicate" gathered over a wave as a lane mask, and this lane's bit of one.
+#ifdef GPIS_HOST_CHECK
+static inline uint64_t lane_mask(bool p) { return p ? 1u : 0u; }
+static inline bool own_bit(uint64_t m) { return (m & 1u) != 0; }
+#else
+__device__ __forceinline__ uint64_t lane_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ bool own_bit(uint64_t m) { return ((m >> __lane_id()) & 1u) != 0; }
+#endif
+
+// The loads of one list pass, two groups per loop iteration, software-pipelined: the group
+// being summed is in registers while the other group's impulses and the next group's entries
+// are in flight.  Groups past the list's `ng` read the final group of the entry array, which
+// holds only the far sentinel impulse, so an odd list's last iteration adds exact zeros.
+// Usage: for (j = 0; j < ng; j += 2) { step0(j); sum q0; step1(j); sum q1; }
+struct ListPipe {
+    const ListSrc &src;
+    uint32_t first, ng, pad;
+    uint32_t e0[4], e1[4];
+    Imp4 q0[4], q1[4];
+    __device__ __forceinline__ uint32_t group(uint32_t k) const { return k < ng ? first + k : pad; }
+    __device__ __forceinline__ ListPipe(const ListSrc &s, uint32_t first_, uint32_t ng_, uint32_t pad_)
+        : src(s), first(first_), ng(ng_), pad(pad_) {
+        load_group(src, group(0), e0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) q0[i] = load_imp(src, e0[i]);
+        load_group(src, group(1), e1);
+    }
+    // before summing q0 (group j): group j + 1's impulses and group j + 2's entries in flight
+    __device__ __forceinline__ void step0(uint32_t j) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) q1[i] = load_imp(src, e1[i]);
+        load_group(src, group(j + 2), e0);
+    }
+    // before summing q1 (group j + 1): group j + 2's impulses and group j + 3's entries
+    __device__ __forceinline__ void step1(uint32_t j) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) q0[i] = load_imp(src, e0[i]);
+        load_group(src, group(j + 3), e1);
+    }
+};
 __device__ __forceinline__ int32_t fix_coord(float g) {
 #ifdef GPIS_HOST_CHECK
     return (int32_t)lrintf(g * lists::kFixScale);
@@ -146,10 +222,10 @@ __device__ __forceinline__ int32_t fix_coord(float g) {
 // The splat of one pair given t = query - impulse in 2^-23 grid units and the weight's bits.
 template <bool ISO>
 __device__ __forceinline__ void pair_splat(float tx, float ty, float tz, int32_t wbits, float s0, float s1, float s2,
-                                           float &v, float &ax, float &ay, float &az, bool &near) {
+                                           float &v, float &ax, float &ay, float &az, uint64_t &near) {
     const float r2 = tx * tx + ty * ty + tz * tz;
     const bool in = r2 < 0x1p46f;
-    near = near || fabsf(r2 - 0x1p46f) < 0x1p46f * lists::kNearRel;
+    near |= lane_mask(fabsf(r2 - 0x1p46f) < 0x1p46f * lists::kNearRel);
     float e;
     if (ISO) e = hw_exp2(s0 * r2);
     else e = hw_exp2(s0 * (tx * tx) + s1 * (ty * ty) + s2 * (tz * tz));
@@ -168,7 +244,7 @@ __device__ __forceinline__ void pair_splat(float tx, float ty, float tz, int32_t
 // decision is left to the exact path).  s0..s2 are the exponent factors scaled by 2^-46.
 template <bool ISO>
 __device__ __forceinline__ void pair_fix(int32_t Gx, int32_t Gy, int32_t Gz, const Imp4 &q, float s0, float s1,
-                                         float s2, float &v, float &ax, float &ay, float &az, bool &near) {
+                                         float s2, float &v, float &ax, float &ay, float &az, uint64_t &near) {
     const float tx = (float)(int32_t)((uint32_t)Gx - (uint32_t)q.x);
     const float ty = (float)(int32_t)((uint32_t)Gy - (uint32_t)q.y);
     const float tz = (float)(int32_t)((uint32_t)Gz - (uint32_t)q.z);
@@ -190,27 +266,23 @@ __device__ __forceinline__ bool list_sum(const FieldParams &P, unsigned lc, floa
     const unsigned first = P.lrange[2 * sc], ng = P.lrange[2 * sc + 1];
     const int32_t Gx = fix_coord(gx), Gy = fix_coord(gy), Gz = fix_coord(gz);
     const float s0 = c0 * 0x1p-46f, s1 = c1 * 0x1p-46f, s2 = c2 * 0x1p-46f;
-    bool near = false;
+    uint64_t near = 0;
     v = ax = ay = az = 0.0f;
-    const uint32_t *ent = P.lent + 4 * (size_t)first;
+    const ListSrc src = list_src(P);
+    ListPipe L(src, first, ng, P.lent_n / 4u - 1u);
 #pragma unroll 1
-    for (unsigned j = 0; j < ng; ++j) {
-        GPIS_CHECK_INDEX(4 * ((size_t)first + j) + 3, (size_t)P.lent_n);
-        uint32_t e[4];
-        load_group(ent + 4 * j, e);
-        Imp4 q[4];
+    for (unsigned j = 0; j < ng; j += 2) {
+        L.step0(j);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            GPIS_CHECK_INDEX((size_t)e[i] + 15, (size_t)P.lfix_bytes);
-            q[i] = load_imp(P.lfix, e[i]);
-        }
+        for (int i = 0; i < 4; ++i) pair_fix<ISO>(Gx, Gy, Gz, L.q0[i], s0, s1, s2, v, ax, ay, az, near);
+        L.step1(j);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) pair_fix<ISO>(Gx, Gy, Gz, q[i], s0, s1, s2, v, ax, ay, az, near);
+        for (int i = 0; i < 4; ++i) pair_fix<ISO>(Gx, Gy, Gz, L.q1[i], s0, s1, s2, v, ax, ay, az, near);
     }
     ax *= 0x1p-23f;
     ay *= 0x1p-23f;
     az *= 0x1p-23f;
-    return near;
+    return own_bit(near);
 }
 
 // The table index of cell (ix, iy, iz) when it has support lists.
@@ -239,33 +311,33 @@ __device__ __forceinline__ void list_sum2(const FieldParams &P, unsigned lc, con
     const int32_t Bx = fix_coord(gb[0]), By = fix_coord(gb[1]), Bz = fix_coord(gb[2]);
     const float dbx = (float)(Bx - Ax), dby = (float)(By - Ay), dbz = (float)(Bz - Az);
     const float s0 = c0 * 0x1p-46f, s1 = c1 * 0x1p-46f, s2 = c2 * 0x1p-46f;
-    near_a = near_b = false;
+    uint64_t mask_a = 0, mask_b = 0;
     A.v = A.gx = A.gy = A.gz = 0.0f;
     B.v = B.gx = B.gy = B.gz = 0.0f;
-    const uint32_t *ent = P.lent + 4 * (size_t)first;
-#pragma unroll 1
-    for (unsigned j = 0; j < ng; ++j) {
-        GPIS_CHECK_INDEX(4 * ((size_t)first + j) + 3, (size_t)P.lent_n);
-        uint32_t e[4];
-        load_group(ent + 4 * j, e);
-        Imp4 q[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            GPIS_CHECK_INDEX((size_t)e[i] + 15, (size_t)P.lfix_bytes);
-            q[i] = load_imp(P.lfix, e[i]);
-        }
+    const ListSrc src = list_src(P);
+    ListPipe L(src, first, ng, P.lent_n / 4u - 1u);
+    auto sum4 = [&](const Imp4 *q) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const float tx = (float)(int32_t)((uint32_t)Ax - (uint32_t)q[i].x);
             const float ty = (float)(int32_t)((uint32_t)Ay - (uint32_t)q[i].y);
             const float tz = (float)(int32_t)((uint32_t)Az - (uint32_t)q[i].z);
-            pair_splat<ISO>(tx, ty, tz, q[i].w, s0, s1, s2, A.v, A.gx, A.gy, A.gz, near_a);
+            pair_splat<ISO>(tx, ty, tz, q[i].w, s0, s1, s2, A.v, A.gx, A.gy, A.gz, mask_a);
             // b - impulse = (a - impulse) + (b - a): integers below 2^24 in magnitude for
             // every impulse within reach, so the float sum is exact and equals the rounded
             // integer difference (far sentinel impulses round, and stay out of support)
-            pair_splat<ISO>(tx + dbx, ty + dby, tz + dbz, q[i].w, s0, s1, s2, B.v, B.gx, B.gy, B.gz, near_b);
+            pair_splat<ISO>(tx + dbx, ty + dby, tz + dbz, q[i].w, s0, s1, s2, B.v, B.gx, B.gy, B.gz, mask_b);
         }
+    };
+#pragma unroll 1
+    for (unsigned j = 0; j < ng; j += 2) {
+        L.step0(j);
+        sum4(L.q0);
+        L.step1(j);
+        sum4(L.q1);
     }
+    near_a = own_bit(mask_a);
+    near_b = own_bit(mask_b);
     A.gx *= 0x1p-23f; A.gy *= 0x1p-23f; A.gz *= 0x1p-23f;
     B.gx *= 0x1p-23f; B.gy *= 0x1p-23f; B.gz *= 0x1p-23f;
 }

@@ -37,7 +37,7 @@ struct gpis_ctx {
     uint32_t *lent = nullptr;
     uint32_t *lrange = nullptr;
     int lists_sub = 3;     // sub-cells per cell edge; 0: no lists
-    int lists_points = 2;  // queries one list pass serves (1, or 2 with lists of reach kPairReach)
+    int lists_points = 2;  // queries one list pass serves (1, or 2 with lists of lists::pair_reach)
     unsigned long long *counters = nullptr;  // device: flights evals steps hits exits fails invalid launches
     unsigned int *work = nullptr;            // device work counter of the persistent kernels
     int device = 0;
@@ -2399,7 +2399,7 @@ static gpis_status build_lists(gpis_ctx *c) {
     free_lists(c);
     const int S = c->lists_sub;
     if (S <= 0 || !c->table) return GPIS_OK;
-    const double reach = c->lists_points == 2 ? (double)lists::kPairReach : 1.0;
+    const double reach = c->lists_points == 2 ? lists::pair_reach((double)c->P.step_size, (double)c->P.R) : 1.0;
     const int *lo = c->table_lo, *dim = c->table_dim;
     for (int i = 0; i < 3; ++i)
         if (lo[i] < -lists::kMaxAbsCell || lo[i] + dim[i] - 1 > lists::kMaxAbsCell) return GPIS_OK;
@@ -2429,9 +2429,15 @@ static gpis_status build_lists(gpis_ctx *c) {
         range[2 * i + 1] = g[i];
         total += g[i];
     }
-    if (total == 0 || total * 4 >= (1ULL << 32)) { free_lists(c); return GPIS_OK; }
+    // the kernels read the entries through a buffer resource of 32-bit byte offsets, one
+    // group beyond a list's end
+    if (total == 0 || (total + 2) * 16 >= (1ULL << 32)) { free_lists(c); return GPIS_OK; }
     HIP_TRY(c, hipMemcpy(c->lrange, range.data(), range.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMalloc((void **)&c->lent, (size_t)total * 4 * sizeof(uint32_t)));
+    // the lists, then one group of the sentinel alone: the kernels read it for the groups
+    // past a list's end
+    HIP_TRY(c, hipMalloc((void **)&c->lent, (size_t)(total + 1) * 4 * sizeof(uint32_t)));
+    const uint32_t pad[4] = {(uint32_t)(imps * 16), (uint32_t)(imps * 16), (uint32_t)(imps * 16), (uint32_t)(imps * 16)};
+    HIP_TRY(c, hipMemcpy(c->lent + total * 4, pad, sizeof pad, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(list_fill_kernel, dim3((unsigned)((nsub + threads - 1) / threads)), dim3(threads), 0, 0,
                        c->table, lo[0], lo[1], lo[2], dim[0], dim[1], dim[2], N, S, reach, nsub, c->lrange, c->lent,
                        (uint32_t)(imps * 16));
@@ -2441,7 +2447,7 @@ static gpis_status build_lists(gpis_ctx *c) {
     P.lfix = c->lfix; P.lent = c->lent; P.lrange = c->lrange;
     P.lsub = S;
     P.lfix_bytes = (uint32_t)((imps + 1) * 16);
-    P.lent_n = (uint32_t)(total * 4);
+    P.lent_n = (uint32_t)((total + 1) * 4);
     P.lrange_n = (uint32_t)nsub;
     P.lmargin = lists::list_margin(reach);
     // a second query within (reach - 1) of the first is served by the same pass (with a
@@ -2630,7 +2636,7 @@ gpis_status gpis_trace_paths(gpis_ctx *c, const gpis_ray_soa *rays, const gpis_p
 
 static gpis_status gpis_set_lists_impl(gpis_ctx *c, int32_t sub, int32_t points) {
     if (!c) return GPIS_INVALID_ARGUMENT;
-    if (sub < 0 || sub > lists::kMaxSub) return fail(c, GPIS_INVALID_ARGUMENT, "sub-cells per cell edge must be in 0..6");
+    if (sub < 0 || sub > lists::kMaxSub) return fail(c, GPIS_INVALID_ARGUMENT, "sub-cells per cell edge must be in 0..8");
     if (points != 1 && points != 2) return fail(c, GPIS_INVALID_ARGUMENT, "points per list pass must be 1 or 2");
     c->lists_sub = sub;
     c->lists_points = points;

@@ -44,14 +44,23 @@ constexpr float kFixScale = 8388608.0f;      // 2^23
 constexpr int kMaxAbsCell = 120;             // |cell index| bound: query - impulse and query - sentinel fit int32
 constexpr int32_t kSentinel = 1 << 30;       // sentinel impulse coordinate (2^7 cells away)
 constexpr double kReach2 = 1.0001;           // list membership: squared distance margin (relative)
-constexpr float kPairReach = 1.125f;         // reach of lists that serve two queries per pass
+constexpr float kPairReach = 1.125f;         // largest reach of lists that serve two queries per pass
 constexpr double kBoxPad = 1e-5;             // sub-cell box padding (rounding of frac * S)
 // Pairs with | |to|^2 - 1 | below kNearRel are decided again on the exact path.  The list
 // path's |to|^2 differs from the reference's by the fused products (a few ulp, < 3e-7) and,
 // for queries with |p / R| < 1, by the rounding of the query to 2^-23 (|to| moves by at most
 // 2^-23 per axis: < 8e-7); 2e-6 covers both with a margin.
 constexpr float kNearRel = 2e-6f;
-constexpr int kMaxSub = 6;                   // sub-cells per cell edge: 1..kMaxSub
+constexpr int kMaxSub = 8;                   // sub-cells per cell edge: 1..kMaxSub
+
+// Reach of lists that serve two queries per pass: one grid unit plus the farthest second
+// query of a flight, the march's lattice step (step_size / R grid units for a unit
+// direction) with a margin, at most kPairReach.  A pair farther apart than the lists allow
+// is summed as two passes, so the reach only sets how many pairs share a pass.
+GPIS_LHD double pair_reach(double step_size, double R) {
+    const double e = step_size / R * 1.01 + 1e-4;
+    return 1.0 + (e < (double)kPairReach - 1.0 ? e : (double)kPairReach - 1.0);
+}
 
 // Linear index of the table cell (x, y, z) (absolute cell indices), x fastest.
 GPIS_LHD long long cell_linear(const int lo[3], const int dim[3], int x, int y, int z) {

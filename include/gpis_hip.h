@@ -294,10 +294,11 @@ gpis_status gpis_set_launch(gpis_ctx *ctx, int32_t waves_per_cu, int32_t block_s
 gpis_status gpis_set_refill(gpis_ctx *ctx, int32_t batch);
 
 /* Support lists of the impulse table (the autotuner's knobs: "sub-cells per cell edge",
- * 1..6, default 3, 0 = none; "points per list pass", 1 or 2, default 2).  For every cell of
+ * 1..8, default 3, 0 = none; "points per list pass", 1 or 2, default 2).  For every cell of
  * the table whose neighbourhood is in the table, each of its sub^3 sub-cells lists the
- * impulses within reach of the sub-cell (one grid unit, or 1.125 when a pass serves two
- * points), in cell3D's visiting order (SparseConvolutionNoise.cpp:419-452); a lane sums only
+ * impulses within reach of the sub-cell (one grid unit, or, when a pass serves two points,
+ * one grid unit plus the march's step_size in grid units and a 1% margin, at most 1.125),
+ * in cell3D's visiting order (SparseConvolutionNoise.cpp:419-452); a lane sums only
  * its query's list, and with points = 2 the 3D flight kernels evaluate the next point the
  * march already determines (next lattice point, next secant candidate, the gradient at
  * farT) in the same pass.  Results are the same as without lists (same support decisions,

@@ -105,7 +105,9 @@ static void build_lists(Table &T, int N, int S, double reach) {
         T.range[2 * sc + 1] = g;
         total += g;
     }
-    T.ent.assign(total * 4, 0xFFFFFFFFu);
+    // the lists, then one group of the sentinel alone (read for the groups past a list's end)
+    T.ent.assign((total + 1) * 4, 0xFFFFFFFFu);
+    for (int i = 0; i < 4; ++i) T.ent[total * 4 + i] = (uint32_t)(imps * 16);
     for (long long sc = 0; sc < nsub; ++sc) {
         if (!T.range[2 * sc + 1]) continue;
         const long long c = sc / (S * S * S);
@@ -223,9 +225,13 @@ int main(int argc, char **argv) {
     double max_rel = 0.0, mean_list = 0.0;
     long long list_cnt = 0, list_missing = 0, list_extra = 0;
     long long pair_mismatch = 0, pair_evals = 0;
-    for (int variant = 0; variant < 2 * lists::kMaxSub; ++variant) {
-        const int S = variant % lists::kMaxSub + 1;
-        const double reach = variant < lists::kMaxSub ? 1.0 : (double)lists::kPairReach;
+    // sub-cells per edge 1, 3, 6 and 8; reach one grid unit, the pair reach of this field's
+    // step (0.01: 1.095 grid units) and the largest pair reach
+    const int subs[4] = {1, 3, 6, 8};
+    const double reaches[3] = {1.0, lists::pair_reach(0.01, (double)R), (double)lists::kPairReach};
+    for (int variant = 0; variant < 12; ++variant) {
+        const int S = subs[variant % 4];
+        const double reach = reaches[variant / 4];
         const int mg = lists::list_margin(reach);
         build_lists(full, N, S, reach);
         const FieldParams Pl = params(&full, true, R, seed, N);
