@@ -203,8 +203,15 @@ def run_1d(args):
     batch = RayBatch.from_numpy(rays)
     reply = dev.alloc_reply(n)
     stream = torch.cuda.current_stream()
+    launch = None
     if args.waves_per_cu:
         dev.set_launch(args.waves_per_cu, 0)
+    elif not args.no_autotune:
+        from gpis_extend_amd import autotune
+        r, w, _, _ = autotune.tune(dev, "flights_1d", n,
+                                   lambda: dev.trace_flights_1d(batch, light=W.LIGHT_POS, reply=reply),
+                                   blocks=(256,))
+        launch = {"refill_batch": r, "waves_per_cu": w, "block": 256}
     for _ in range(args.warmup):
         dev.trace_flights_1d(batch, light=W.LIGHT_POS, reply=reply)
     torch.cuda.synchronize()
@@ -248,7 +255,7 @@ def run_1d(args):
                                "NEE toward a point light at (0, 2, 0)",
                    "rays": n, "flights_per_step": flights_per_step, "evals_per_step": evals_per_step,
                    "hits_per_step": c["hits"] / args.steps, "fails_per_step": c["fails"] / args.steps,
-                   "precision": "fast (FP32 splats and mean)",
+                   "precision": "fast (FP32 splats and mean)", "launch": launch,
                    "exact_rounding_mode": {"kernel_ms": exact_ms,
                                            "flights_per_s": flights_per_step / (exact_ms / 1e3)}},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

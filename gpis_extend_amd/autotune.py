@@ -55,10 +55,11 @@ def _apply(dev, choice, with_lists: bool):
 
 
 def tune(dev, mode: str, n: int, run: Callable[[], None], repeats: int = 1, verbose: bool = False,
-         lists: bool = False):
+         lists: bool = False, blocks=BLOCK_SIZES):
     """Pick (refill, waves_per_cu, block_size, sub-cells) for `run` (which launches once)
     and apply it to `dev`.  `lists`: the medium has an impulse table whose support lists
-    are tuned too.  Returns the chosen tuple."""
+    are tuned too; `blocks`: the block sizes the entry point takes (the 1D kernels run
+    256-thread blocks).  Returns the chosen tuple."""
     key = _key(dev, mode, n) + (lists,)
     if key in _CACHE:
         _apply(dev, _CACHE[key], lists)
@@ -77,7 +78,7 @@ def tune(dev, mode: str, n: int, run: Callable[[], None], repeats: int = 1, verb
                 best_sub, best_ms = sub, ms
         dev.set_lists(best_sub, 2)
     best, best_ms = None, float("inf")
-    for r, w, b in itertools.product(REFILL, WAVES_PER_CU, BLOCK_SIZES):
+    for r, w, b in itertools.product(REFILL, WAVES_PER_CU, blocks):
         dev.set_refill(r)
         dev.set_launch(w, b)
         ms = _time(run, repeats)

@@ -216,27 +216,46 @@ __device__ __forceinline__ void cells_for(Cells1D<NI> &C, int i, uint32_t seed, 
 // kernels compute it ((float)exp((double)x), GPFunctions.cpp:845-875).
 __device__ __forceinline__ float exp_dr(float x) { return (float)exp((double)x); }
 
-// exp(-x^2 / 2) in FP32 by the hardware exponential: exp2(-x^2 log2(e) / 2).
+// -log2(e) / 2: exp(-x^2 / 2) = exp2(kNegHalfLog2e x^2) on the hardware exponential.
 constexpr float kNegHalfLog2e = -0.72134752044448170f;
-__device__ __forceinline__ float gauss_f32(float x) { return hw_exp2(kNegHalfLog2e * (x * x)); }
 
-// noise1D (:454-485) at coordinate u with kernel radius R = 3: the raw sums of w*h and
-// w*dh/du.  FAST = false: bit-identical to the reference's arithmetic (per-cell partial
-// sums added cell by cell, splat products rounded as written, exponentials rounded from
-// double).  FAST = true: the same support decisions and summation order, the splat by
-// the FP32 hardware exponential (a few ulp).
+// The sums of noise1D (:454-485) over the three cells around the query, given the
+// impulses of cells i-1, i, i+1 (s[dx][k], t_k with the weight in its sign bit) and the
+// query's fraction fr of cell i: the raw sums of w*h and w*dh/du with kernel radius R = 3.
+// FAST = false: bit-identical to the reference's arithmetic (per-cell partial sums added
+// cell by cell, splat products rounded as written, exponentials rounded from double).
+// FAST = true: the same support decisions and summation order, the splat by the FP32
+// hardware exponential (a few ulp).
 template <int NI, bool FAST = false>
-__device__ __forceinline__ void noise1d_raw(Cells1D<NI> &C, bool &cache_valid, float u, uint32_t seed,
-                                            float &v, float &dv) {
+__device__ __forceinline__ void noise1d_sum(const float (&s)[3][NI > 0 ? NI : 1], float fr, float &v, float &dv) {
 #pragma clang fp contract(off)
     const float R = 3.0f;
-    float tg = u / R;
-    float fl = floorf(tg);
-    float fr = tg - fl;
-    int i = (int)fl;
-    cells_for<NI>(C, i, seed, cache_valid);
-    cache_valid = true;
     float sv = 0.f, sd = 0.f;
+    if (FAST) {
+        // the splat exp(-ab^2 / 2) with ab = R to, from the to^2 of the support test:
+        // exp2(-log2(e) R^2 / 2 * to^2); the derivative sum w e (-ab) as -R * sum w e to
+        const float kq = kNegHalfLog2e * (R * R);
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            const float t = fr - (float)(dx - 1);
+            float cv = 0.f, cd = 0.f;
+#pragma unroll
+            for (int k = 0; k < NI; ++k) {
+                const float ts = s[dx][k];
+                const float to = t - fabsf(ts);
+                const float q = to * to;
+                const float e = hw_exp2(kq * q);
+                const float we = q < 1.0f ? copysignf(e, ts) : 0.0f;
+                cv += we;
+                cd += we * to;
+            }
+            sv += cv;
+            sd += cd;
+        }
+        v = sv;
+        dv = -R * sd;
+        return;
+    }
 #pragma unroll
     for (int dx = 0; dx < 3; ++dx) {
         const float t = fr - (float)(dx - 1);
@@ -244,13 +263,13 @@ __device__ __forceinline__ void noise1d_raw(Cells1D<NI> &C, bool &cache_valid, f
         float cv = 0.f, cd = 0.f;
 #pragma unroll
         for (int k = 0; k < NI; ++k) {
-            const float ts = C.s[dx][k];
+            const float ts = s[dx][k];
             const float ti = fabsf(ts);
             const float to = t - ti;
             if (to * to < 1.0f) {
                 const float pc = R * ti;
                 const float ab = pq - pc;                // splattingKernel1D: pQuery - pCenter
-                const float e = FAST ? gauss_f32(ab) : (float)exp_splat_d((double)(-(ab * ab / 2.0f)));
+                const float e = (float)exp_splat_d((double)(-(ab * ab / 2.0f)));
                 const float g = -ab * e;                 // (-2 ab / 2) * e, exact in double then rounded
                 const float w = copysignf(1.0f, ts);
                 cv += w * e;
@@ -262,6 +281,157 @@ __device__ __forceinline__ void noise1d_raw(Cells1D<NI> &C, bool &cache_valid, f
     }
     v = sv;
     dv = sd;
+}
+
+// noise1D at coordinate u with the lane's register cache of three cells.
+template <int NI, bool FAST = false>
+__device__ __forceinline__ void noise1d_raw(Cells1D<NI> &C, bool &cache_valid, float u, uint32_t seed,
+                                            float &v, float &dv) {
+#pragma clang fp contract(off)
+    const float R = 3.0f;
+    float tg = u / R;
+    float fl = floorf(tg);
+    float fr = tg - fl;
+    int i = (int)fl;
+    cells_for<NI>(C, i, seed, cache_valid);
+    cache_valid = true;
+    noise1d_sum<NI, FAST>(C.s, fr, v, dv);
+}
+
+// ------------------------------------------------------------------------------------------
+// The cells of the 1D flight kernel in LDS, generated by the whole wave.
+//
+// A lane's three cells i-1, i, i+1 live in a ring of three LDS slots (cell c in slot c mod 3),
+// so advancing one cell replaces one slot and stepping back (a secant point) keeps two.  A
+// cell is NI impulses from cell1D's PCG stream (set_state(xxhash32(c, seed) + 1), then per
+// impulse t and the weight's draw).  Draw j of a stream is the output of the state after
+// j + 2 steps (set_state discards two draws), and PCG's state after m steps is a^m s + c_m
+// (c_m = a^(m-1) + ... + 1), so any impulse of any cell is two multiply-adds of the seeded
+// state: the missing cells of all lanes of the wave are cut into (cell, impulse) items, and
+// every active lane makes one item per round.  The values are those of the sequential
+// generator to the bit; a lane no longer runs a whole cell's chain while the wave waits.
+//
+// LDS layout per wave: [slot 3][impulse quad NI/4][lane 64] float4, read as two 16-byte
+// loads per cell with consecutive lanes at consecutive addresses (no bank conflicts).
+constexpr uint64_t kPcgMult = 6364136223846793005ULL;
+constexpr int kJumpSteps = 2 * 8 + 2;  // states 0 .. 2 NI + 1 of a cell's stream (NI = 8)
+struct PcgJump {
+    uint64_t a[kJumpSteps], c[kJumpSteps];
+};
+constexpr PcgJump make_pcg_jump() {
+    PcgJump J{};
+    uint64_t a = 1, c = 0;
+    for (int m = 0; m < kJumpSteps; ++m) {
+        J.a[m] = a;
+        J.c[m] = c;
+        c = c * kPcgMult + 1ULL;
+        a = a * kPcgMult;
+    }
+    return J;
+}
+__constant__ PcgJump kPcgJumpTable = make_pcg_jump();
+// PCG32's output of the state `old` (the XSH-RR permutation of Pcg::next).
+__device__ __forceinline__ uint32_t pcg_output(uint64_t old) {
+    const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+// The ring slot of cell c.
+__device__ __forceinline__ int ring_slot(int c) {
+    const int m = c % 3;
+    return m < 0 ? m + 3 : m;
+}
+
+template <int NI>
+struct Cells1DShared {
+    static constexpr int kQuads = NI / 4;
+    static constexpr int kMaxJobs = 3 * 64;
+    float4 *cells;       // this wave's [3][kQuads][64] float4
+    uint32_t *job_st;    // this wave's seeded states of the cells being made
+    uint32_t *job_dst;   // their owner lane | slot << 6
+    int base;            // the lane's middle cell
+    bool valid;
+};
+
+// Make the lane's slots hold cells i-1, i, i+1 (all active lanes of the wave call this
+// together; `need` lanes with nothing missing still take part in the rounds).
+template <int NI>
+__device__ __forceinline__ void cells_shared_for(Cells1DShared<NI> &C, int i, uint32_t seed) {
+    static_assert(2 * NI + 2 <= kJumpSteps, "jump table covers a cell's stream");
+    int miss[3], nj = 0;
+    const int d = i - C.base;
+    if (!C.valid || d > 2 || d < -2) {
+        miss[0] = i - 1; miss[1] = i; miss[2] = i + 1; nj = 3;
+    } else if (d == 1) {
+        miss[0] = i + 1; nj = 1;
+    } else if (d == 2) {
+        miss[0] = i; miss[1] = i + 1; nj = 2;
+    } else if (d == -1) {
+        miss[0] = i - 1; nj = 1;
+    } else if (d == -2) {
+        miss[0] = i - 1; miss[1] = i; nj = 2;
+    }
+    C.base = i;
+    C.valid = true;
+    const uint64_t b1 = __builtin_amdgcn_ballot_w64(nj >= 1);
+    if (b1 == 0) return;  // wave-uniform
+    const uint64_t b2 = __builtin_amdgcn_ballot_w64(nj >= 2), b3 = __builtin_amdgcn_ballot_w64(nj >= 3);
+    const uint64_t act = __builtin_amdgcn_ballot_w64(true);
+    const unsigned lane = __lane_id();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const unsigned first = (unsigned)(__popcll(b1 & lt) + __popcll(b2 & lt) + __popcll(b3 & lt));
+    const unsigned jobs = (unsigned)(__popcll(b1) + __popcll(b2) + __popcll(b3));
+    for (int r = 0; r < 3; ++r) {
+        if (r < nj) {
+            C.job_st[first + r] = xxh2((uint32_t)miss[r], seed) + 1u;  // cell1D (:466)
+            C.job_dst[first + r] = lane | ((unsigned)ring_slot(miss[r]) << 6);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // items (job j, impulse k) = (item / NI, item % NI), taken by the active lanes in turn
+    const unsigned na = (unsigned)__popcll(act), rank = (unsigned)__popcll(act & lt);
+    const unsigned items = jobs * NI;
+    for (unsigned it = rank; it < items; it += na) {
+        const unsigned j = it / NI, k = it % NI;
+        const uint64_t at = kPcgJumpTable.a[2 * k + 2], ct = kPcgJumpTable.c[2 * k + 2];
+        const uint64_t aw = kPcgJumpTable.a[2 * k + 3], cw = kPcgJumpTable.c[2 * k + 3];
+        const uint64_t st = (uint64_t)C.job_st[j];
+        const uint32_t dst = C.job_dst[j];
+        const float t = u01(pcg_output(at * st + ct));
+        const float w = u01(pcg_output(aw * st + cw)) < 0.5f ? -1.0f : 1.0f;
+        const unsigned owner = dst & 63u, slot = dst >> 6;
+        float *q = reinterpret_cast<float *>(&C.cells[(slot * Cells1DShared<NI>::kQuads + k / 4) * 64 + owner]);
+        q[k & 3] = copysignf(t, w);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// noise1D at coordinate u with the cells in LDS.
+template <int NI, bool FAST = false>
+__device__ __forceinline__ void noise1d_shared(Cells1DShared<NI> &C, float u, uint32_t seed, float &v, float &dv) {
+#pragma clang fp contract(off)
+    const float R = 3.0f;
+    float tg = u / R;
+    float fl = floorf(tg);
+    float fr = tg - fl;
+    int i = (int)fl;
+    cells_shared_for<NI>(C, i, seed);
+    const unsigned lane = __lane_id();
+    float s[3][NI];
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) {
+        const int slot = ring_slot(i - 1 + dx);
+#pragma unroll
+        for (int qd = 0; qd < NI / 4; ++qd) {
+            const float4 q = C.cells[(slot * (NI / 4) + qd) * 64 + lane];
+            s[dx][4 * qd] = q.x; s[dx][4 * qd + 1] = q.y; s[dx][4 * qd + 2] = q.z; s[dx][4 * qd + 3] = q.w;
+        }
+    }
+    noise1d_sum<NI, FAST>(s, fr, v, dv);
 }
 
 // Covariance splats of the 1D conditioning (covarianceKernel1D / 1DGrad with the

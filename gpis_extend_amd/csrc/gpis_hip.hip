@@ -1051,6 +1051,30 @@ __device__ __forceinline__ void eval1d(const FieldParams &P, const Ray1D &R, Cel
     else f = (float)((double)(P.amp_const * v) + mean_value_d<PROC>(P, qx, qy, qz));
 }
 
+// eval1d with the cells in LDS, generated by the wave (flight1d_kernel).
+template <int NI, bool PROC, bool FAST = false>
+__device__ __forceinline__ void eval1d_shared(const FieldParams &P, const Ray1D &R, Cells1DShared<NI> &C, float nf,
+                                              float qx, float qy, float qz, bool cond, float &f, float &nv,
+                                              float &ndv) {
+#pragma clang fp contract(off)
+    float u = ray_coord(R, qx, qy, qz);
+    float v, dv;
+    noise1d_shared<NI, FAST>(C, u, R.seed, v, dv);
+    if (FAST) {
+        const float inf = 1.0f / nf;
+        v *= inf;
+        dv *= inf;
+    } else {
+        v = v / nf;
+        dv = dv / nf;
+    }
+    if (cond && R.cond) cond_splat1d<FAST>(R, u, v, dv);
+    nv = v;
+    ndv = dv;
+    if (FAST) f = P.amp_const * v + mean_value_f<PROC>(P, qx, qy, qz);
+    else f = (float)((double)(P.amp_const * v) + mean_value_d<PROC>(P, qx, qy, qz));
+}
+
 // neeShared + neeGrad + neePDF (SparseConvolutionNoise.cpp:662-821) for the single-level
 // stationary kernel: the mirror normal `h`, the hit point p, the conditioned derivative
 // along the ray `dvp`; with `tr` the transverse density is the correlated law of
@@ -1310,9 +1334,18 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(
     // (flights); 3: the bounce is next (paths)
     int nee_stage = 0;
     Ray1D R = {};
-    Cells1D<NI> C;
-    C.base = 0;
-    bool cache_valid = false;
+    // the lane's three cells in LDS (a ring per lane), made by the whole wave
+    __shared__ float4 cells_lds[4 * 3 * (NI / 4) * 64];
+    __shared__ uint32_t jobs_lds[4 * 2 * Cells1DShared<NI>::kMaxJobs];
+    Cells1DShared<NI> C;
+    {
+        const unsigned w = threadIdx.x / 64u;
+        C.cells = cells_lds + w * (3 * (NI / 4) * 64);
+        C.job_st = jobs_lds + w * (2 * Cells1DShared<NI>::kMaxJobs);
+        C.job_dst = C.job_st + Cells1DShared<NI>::kMaxJobs;
+        C.base = 0;
+        C.valid = false;
+    }
     // path state (PATHS)
     int pb = 0;                 // flight index within the path (pixelSampleSegment.w - w0)
     bool shadow = false;        // the running flight is a light sample's shadow flight
@@ -1332,7 +1365,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(
         evals_flight = 0;
         nee_stage = 0;
         observed = false;
-        cache_valid = false;
+        C.valid = false;
         R.cond = 0;
         float ix = R.ia[0] * dx, iy = R.ia[1] * dy, iz = R.ia[2] * dz;
         normalize3(ix, iy, iz);
@@ -1655,7 +1688,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT1D_MIN_WAVES) void flight1d_kernel(
         if (phase == PH_NEED || phase == PH_DEAD || phase == PH_DONE) continue;
 
         float fv, nv, ndv;
-        eval1d<NI, PROC, FAST>(P, R, C, cache_valid, nf, qx, qy, qz, phase != PH_COND, fv, nv, ndv);
+        eval1d_shared<NI, PROC, FAST>(P, R, C, nf, qx, qy, qz, phase != PH_COND, fv, nv, ndv);
         ++evals_flight;
         double tq = 0.0;
         bool unit = false;

@@ -303,8 +303,14 @@ def test_served_endpoints_follow_their_law(trained):
     mu, var = so.slope_law(e1, e2, c11, c12, c22, -m[cr])
     muY, sY = ms[cr] + mu / lc[cr], np.sqrt(var) / lc[cr]
     down = oc[cr] == 1
-    mt = np.where(down, -muY, muY) / sY
-    u = np.sort(so.rice_cdf(np.abs(gd[cr]) / sY, mt))
+    # a crossing within ~1e-4 correlation lengths of the origin has a slope law of variance
+    # 0 in double (the law degenerates to the observed derivative): no PIT there, and such
+    # crossings must stay rare
+    deg = ~(sY > 0)
+    print(f"crossings with a degenerate slope law: {int(deg.sum())}, min t/l {float(np.min(t[cr] / lc[cr])):.2e}")
+    assert deg.sum() <= max(20, 1e-4 * len(deg))
+    mt = np.where(down, -muY, muY)[~deg] / sY[~deg]
+    u = np.sort(so.rice_cdf(np.abs(gd[cr][~deg]) / sY[~deg], mt))
     k = len(u)
     ks = float(np.max(np.maximum(np.arange(1, k + 1) / k - u, u - np.arange(k) / k)))
     print(f"served crossings {k}: Rice PIT KS {ks:.4f}; hits {hit.sum()} fails {fail.sum()} exits {ex.sum()}")
