@@ -373,25 +373,23 @@ __device__ __forceinline__ void cells_shared_for(Cells1DShared<NI> &C, int i, ui
     }
     C.base = i;
     C.valid = true;
-    const uint64_t b1 = __builtin_amdgcn_ballot_w64(nj >= 1);
+    const uint64_t b1 = wave_ballot(nj >= 1);
     if (b1 == 0) return;  // wave-uniform
-    const uint64_t b2 = __builtin_amdgcn_ballot_w64(nj >= 2), b3 = __builtin_amdgcn_ballot_w64(nj >= 3);
-    const uint64_t act = __builtin_amdgcn_ballot_w64(true);
-    const unsigned lane = __lane_id();
+    const uint64_t b2 = wave_ballot(nj >= 2), b3 = wave_ballot(nj >= 3);
+    const uint64_t act = wave_ballot(true);
+    const unsigned lane = lane64();
     const uint64_t lt = (1ull << lane) - 1ull;
-    const unsigned first = (unsigned)(__popcll(b1 & lt) + __popcll(b2 & lt) + __popcll(b3 & lt));
-    const unsigned jobs = (unsigned)(__popcll(b1) + __popcll(b2) + __popcll(b3));
+    const unsigned first = (unsigned)(popc64(b1 & lt) + popc64(b2 & lt) + popc64(b3 & lt));
+    const unsigned jobs = (unsigned)(popc64(b1) + popc64(b2) + popc64(b3));
     for (int r = 0; r < 3; ++r) {
         if (r < nj) {
             C.job_st[first + r] = xxh2((uint32_t)miss[r], seed) + 1u;  // cell1D (:466)
             C.job_dst[first + r] = lane | ((unsigned)ring_slot(miss[r]) << 6);
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync();
     // items (job j, impulse k) = (item / NI, item % NI), taken by the active lanes in turn
-    const unsigned na = (unsigned)__popcll(act), rank = (unsigned)__popcll(act & lt);
+    const unsigned na = (unsigned)popc64(act), rank = (unsigned)popc64(act & lt);
     const unsigned items = jobs * NI;
     for (unsigned it = rank; it < items; it += na) {
         const unsigned j = it / NI, k = it % NI;
@@ -405,9 +403,7 @@ __device__ __forceinline__ void cells_shared_for(Cells1DShared<NI> &C, int i, ui
         float *q = reinterpret_cast<float *>(&C.cells[(slot * Cells1DShared<NI>::kQuads + k / 4) * 64 + owner]);
         q[k & 3] = copysignf(t, w);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync();
 }
 
 // noise1D at coordinate u with the cells in LDS.
@@ -420,7 +416,7 @@ __device__ __forceinline__ void noise1d_shared(Cells1DShared<NI> &C, float u, ui
     float fr = tg - fl;
     int i = (int)fl;
     cells_shared_for<NI>(C, i, seed);
-    const unsigned lane = __lane_id();
+    const unsigned lane = lane64();
     float s[3][NI];
 #pragma unroll
     for (int dx = 0; dx < 3; ++dx) {

@@ -15,6 +15,7 @@
 #define __global__
 #define __forceinline__ inline
 #define __noinline__
+#define __constant__
 #define __expf expf
 #define __logf logf
 #define __powf powf
@@ -32,13 +33,27 @@ static inline int max(int a, int b) { return a > b ? a : b; }
 namespace gpis {
 
 // exp2 on the transcendental unit (v_exp_f32), and "some lane of the wave" of a predicate.
+// wave_ballot: the lanes of the wave whose predicate holds; lane64: this lane's index;
+// popc64: bits set; wave_sync: order this wave's LDS writes before its later reads.
 #ifdef GPIS_HOST_CHECK
 static inline float hw_exp2(float x) { return exp2f(x); }
 #define GPIS_CHECK_INDEX(i, n) gpis_check_index((size_t)(i), (size_t)(n), __FILE__, __LINE__)
 static inline bool wave_any(bool p) { return p; }
+static inline uint64_t wave_ballot(bool p) { return p ? 1u : 0u; }
+static inline unsigned lane64() { return 0u; }
+static inline int popc64(uint64_t m) { return __builtin_popcountll(m); }
+static inline void wave_sync() {}
 #else
 __device__ __forceinline__ float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ unsigned lane64() { return __lane_id(); }
+__device__ __forceinline__ int popc64(uint64_t m) { return __popcll(m); }
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 #define GPIS_CHECK_INDEX(i, n) ((void)0)
 #endif
 

@@ -172,13 +172,8 @@ __device__ __forceinline__ Imp4 load_imp(const ListSrc &s, uint32_t off) {
     return r;
 }
 // "This lane's predicate" gathered over a wave as a lane mask, and this lane's bit of one.
-#ifdef GPIS_HOST_CHECK
-static inline uint64_t lane_mask(bool p) { return p ? 1u : 0u; }
-static inline bool own_bit(uint64_t m) { return (m & 1u) != 0; }
-#else
-__device__ __forceinline__ uint64_t lane_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
-__device__ __forceinline__ bool own_bit(uint64_t m) { return ((m >> __lane_id()) & 1u) != 0; }
-#endif
+__device__ __forceinline__ uint64_t lane_mask(bool p) { return wave_ballot(p); }
+__device__ __forceinline__ bool own_bit(uint64_t m) { return ((m >> lane64()) & 1u) != 0; }
 
 // The loads of one list pass, two groups per loop iteration, software-pipelined: the group
 // being summed is in registers while the other group's impulses and the next group's entries

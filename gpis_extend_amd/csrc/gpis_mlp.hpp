@@ -101,7 +101,6 @@ __device__ __forceinline__ float gelu_grad(float z) {
     // d/dz z s(2u) = s + z s (1 - s) 2u',  2u' = 2 sqrt(2/pi) (1 + 3 * 0.044715 z^2)
     return s + z * s * (1.0f - s) * (1.5957691216057308f * fmaf(0.134145f, z * z, 1.0f));
 }
-__device__ __forceinline__ unsigned lane64() { return __lane_id(); }
 
 // ------------------------------------------------------------------ packing --
 // Forward fragment element (layer l, index e) and backward fragment element.

@@ -13,7 +13,9 @@
 //   3. every list holds each impulse within reach of its sub-cell exactly once, in the
 //      reference's order, and nothing out of reach of the whole sub-cell;
 //   4. the multi-resolution level index, level scales and weights of every float lateral
-//      scale in [2^-12, 2^12] are those of the float logf/powf formula.
+//      scale in [2^-12, 2^12] are those of the float logf/powf formula;
+//   5. the 1D kernel's cells in LDS, made by PCG jump-ahead, equal the sequential
+//      generator's after every kind of move along the ray.
 // Queries: uniform in the table's interior, near the origin planes (|p / R| < 1, where the
 // fixed-point query is rounded), on cell and sub-cell faces, at distance 1 +- 2^-22 grid
 // units from an impulse (the support radius), and up to four cells below the table's low
@@ -377,8 +379,47 @@ int main(int argc, char **argv) {
         }
     }
     printf("level_mismatch %lld weight_mismatch %lld of %lld\n", level_bad, weight_bad, level_n);
+
+    // 5. the 1D flight kernel's cells in LDS, made through PCG jump-ahead: after every move of
+    // a lane (steps of 0, +-1, +-2 cells, jumps, a new flight) its ring slots hold cells
+    // i-1, i, i+1 exactly as the sequential generator makes them
+    long long cells1d_bad = 0, cells1d_n = 0;
+    {
+        static float4 lds[3 * 2 * 64];
+        static uint32_t jobs[2 * Cells1DShared<8>::kMaxJobs];
+        Cells1DShared<8> C;
+        C.cells = lds; C.job_st = jobs; C.job_dst = jobs + Cells1DShared<8>::kMaxJobs;
+        C.base = 0; C.valid = false;
+        std::mt19937 r1(7);
+        uint32_t seed1 = 0x9E3779B9u;
+        int i = -40;
+        for (int it = 0; it < 20000; ++it) {
+            const int pick = (int)(r1() % 16);
+            if (pick < 5) i += 0;
+            else if (pick < 9) i += 1;
+            else if (pick < 11) i -= 1;
+            else if (pick < 12) i += 2;
+            else if (pick < 13) i -= 2;
+            else if (pick < 15) i += (int)(r1() % 97) - 48;
+            else { C.valid = false; seed1 = (uint32_t)r1(); }
+            cells_shared_for<8>(C, i, seed1);
+            for (int dx = 0; dx < 3; ++dx) {
+                float ref[8];
+                gen_cell1d<8>(ref, i - 1 + dx, seed1);
+                const int slot = ring_slot(i - 1 + dx);
+                for (int k = 0; k < 8; ++k) {
+                    const float4 q = lds[(slot * 2 + k / 4) * 64 + 0];
+                    const float got = (&q.x)[k & 3];
+                    if (memcmp(&got, &ref[k], 4) != 0) ++cells1d_bad;
+                    ++cells1d_n;
+                }
+            }
+        }
+    }
+    printf("cells1d_mismatch %lld of %lld\n", cells1d_bad, cells1d_n);
     const bool ok = level_bad == 0 && weight_bad == 0 && table_mismatch == 0 && support_mismatch == 0 && pair_mismatch == 0 && pair_evals > 0 && value_bad == 0 && list_missing == 0 &&
-                    list_extra == 0 && near * 100 < evals && near_radius * 10 > evals_radius * 9;
+                    list_extra == 0 && near * 100 < evals && near_radius * 10 > evals_radius * 9 && cells1d_bad == 0 &&
+                    cells1d_n > 0;
     printf("%s\n", ok ? "OK" : "FAIL");
     return ok ? 0 : 1;
 }

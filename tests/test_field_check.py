@@ -2,7 +2,8 @@
 fixed-point index (tests/field_check.cpp): generator, partial table and full table give
 bit-identical fields; support lists keep the exact path's support decisions and hold
 every impulse within reach of their sub-cell, in the reference's order; the
-multi-resolution level index, scales and weights are the float formula's."""
+multi-resolution level index, scales and weights are the float formula's; the 1D
+kernel's cells made by PCG jump-ahead equal the sequential generator's."""
 import os
 import shutil
 import subprocess
@@ -24,4 +25,5 @@ def test_field_tables_and_support_lists_on_the_host(tmp_path):
     assert lines[0] == "table_mismatch 0"
     assert "list_missing 0 list_extra 0" in out.stdout
     assert "level_mismatch 0 weight_mismatch 0" in out.stdout
+    assert "cells1d_mismatch 0 of" in out.stdout
     assert out.stdout.strip().endswith("OK")
