@@ -50,8 +50,8 @@ def flops_per_eval_1d(n_imp: int) -> float:
 MEASURED_TRAFFIC = {
     "flight_kernel<8,true>": {"bytes": (1.0303e7 * 2.0 + 1.0734e6) * 1024, "fetch_bytes": 1.0303e7 * 2.0 * 1024,
                               "write_bytes": 1.0734e6 * 1024, "source": "profiles/flight_kernel_c1_paths_2048.md"},
-    "flight1d_kernel<8,fast>": {"bytes": (2.40e5 * 2.0 + 6.09e5) * 1024, "fetch_bytes": 2.40e5 * 2.0 * 1024,
-                                "write_bytes": 6.09e5 * 1024, "source": "profiles/flight1d_kernel_c3_2m.md"},
+    "flight1d_kernel<8,fast>": {"bytes": (1.646e5 * 2.0 + 4.630e5) * 1024, "fetch_bytes": 1.646e5 * 2.0 * 1024,
+                                "write_bytes": 4.630e5 * 1024, "source": "profiles/flight1d_kernel_c3_2m.md"},
 }
 
 PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector peak (packed FMA issue)
@@ -204,7 +204,12 @@ def run_1d(args):
     reply = dev.alloc_reply(n)
     stream = torch.cuda.current_stream()
     launch = None
-    if args.waves_per_cu:
+    if args.launch:
+        v = [int(x) for x in args.launch.split(",")]
+        dev.set_refill(v[0])
+        dev.set_launch(v[1], 256)
+        launch = {"refill_batch": v[0], "waves_per_cu": v[1], "block": 256}
+    elif args.waves_per_cu:
         dev.set_launch(args.waves_per_cu, 0)
     elif not args.no_autotune:
         from gpis_extend_amd import autotune
@@ -750,7 +755,8 @@ def main():
                     help="skip the launch-shape sweep that runs before the warmup steps")
     ap.add_argument("--launch", default="",
                     help="R,W,B[,S]: refill batch, waves per CU, threads per block and support-list "
-                         "sub-cells for the camera-ray paths; replaces the autotuner's sweep")
+                         "sub-cells for the camera-ray paths (R,W for the 1D flights, whose blocks are "
+                         "256 threads); replaces the autotuner's sweep")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed region, copy this many MiB device-to-device once "
                          "(a known-size kernel for calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE)")

@@ -24,7 +24,7 @@ _CACHE: Dict[Tuple, Tuple[int, int, int, int]] = {}
 SUBCELLS = (3, 4, 6, 8)
 WAVES_PER_CU = (8, 12, 16, 24)
 BLOCK_SIZES = (64, 128, 256)
-REFILL = (8, 16, 32)
+REFILL = (8, 16, 32, 48)
 
 
 def _key(dev, mode: str, n: int):
