@@ -245,6 +245,7 @@ struct FieldParams {
     uint32_t lfix_bytes, lent_n, lrange_n;  // their sizes (bytes, entries, sub-cells)
     int lmargin;        // cells whose lists exist: those at least lmargin cells inside the table
     float lpair2;       // lists serve a second query within sqrt(lpair2) grid units (0: one query)
+    int pair_cells;     // two points of a flight in one grid cell share a pass over its block's cells
 };
 
 struct Eval4 { float v, gx, gy, gz; };

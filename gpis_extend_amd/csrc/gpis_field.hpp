@@ -452,9 +452,66 @@ __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, flo
     return r;
 }
 
+// Two queries in the same grid cell (ix, iy, iz) of one level, summed in one pass over their
+// 27-cell block: each cell's impulses are read from the table or generated (cell3D's stream)
+// once and splatted at both points.  Each point keeps its own cell cull and support test and
+// runs over the same cells and impulses in the same order as noise3d_raw, so each sum is that
+// point's single-query sum to the bit.  ra, rb: the points' in-cell coordinates.
+template <int NI, bool ISO>
+__device__ __forceinline__ void noise3d_cells2(const FieldParams &P, int ix, int iy, int iz, const float ra[3],
+                                               const float rb[3], uint32_t seed, float c0, float c1, float c2,
+                                               Eval4 &A, Eval4 &B) {
+    float av = 0.f, aax = 0.f, aay = 0.f, aaz = 0.f, bv = 0.f, bax = 0.f, bay = 0.f, baz = 0.f;
+    const int N = NI > 0 ? NI : P.N;
+    const bool tab = P.has_table && seed == P.tseed;
+    for (int dx = -1; dx <= 1; ++dx) {
+        for (int dy = -1; dy <= 1; ++dy) {
+            for (int dz = -1; dz <= 1; ++dz) {
+                const bool na = axis_bound2(ra[0], dx + 1) + axis_bound2(ra[1], dy + 1) + axis_bound2(ra[2], dz + 1) <
+                                kCullR2;
+                const bool nb = axis_bound2(rb[0], dx + 1) + axis_bound2(rb[1], dy + 1) + axis_bound2(rb[2], dz + 1) <
+                                kCullR2;
+                if (!na && !nb) continue;
+                const int cx = ix + dx, cy = iy + dy, cz = iz + dz;
+                const float lax = ra[0] - (float)dx, lay = ra[1] - (float)dy, laz = ra[2] - (float)dz;
+                const float lbx = rb[0] - (float)dx, lby = rb[1] - (float)dy, lbz = rb[2] - (float)dz;
+                const unsigned ux = (unsigned)(cx - P.tlo[0]), uy = (unsigned)(cy - P.tlo[1]),
+                               uz = (unsigned)(cz - P.tlo[2]);
+                if (tab && ux < (unsigned)P.tdim[0] && uy < (unsigned)P.tdim[1] && uz < (unsigned)P.tdim[2]) {
+                    const size_t qi = ((size_t)(uz * (unsigned)P.tdim[1] + uy) * (unsigned)P.tdim[0] + ux) * (3 * N);
+                    GPIS_CHECK_INDEX(qi + 3 * N - 1, (size_t)P.table_n);
+                    const float *q = P.table + qi;
+#pragma unroll
+                    for (int k = 0; k < N; ++k) {
+                        if (na) pair3p<ISO>(lax, lay, laz, q[3 * k], q[3 * k + 1], q[3 * k + 2], c0, c1, c2, av, aax, aay, aaz);
+                        if (nb) pair3p<ISO>(lbx, lby, lbz, q[3 * k], q[3 * k + 1], q[3 * k + 2], c0, c1, c2, bv, bax, bay, baz);
+                    }
+                } else {
+                    Pcg g;
+                    g.set((uint64_t)(uint32_t)(xxh4((uint32_t)cz, (uint32_t)cy, (uint32_t)cx, seed) + 1u));
+#pragma unroll
+                    for (int k = 0; k < N; ++k) {
+                        float4 im;
+                        im.z = u01(g.next());
+                        im.y = u01(g.next());
+                        im.x = u01(g.next());
+                        im.w = u01(g.next()) < 0.5f ? -1.0f : 1.0f;
+                        if (na) pair3<ISO>(lax, lay, laz, im, c0, c1, c2, av, aax, aay, aaz);
+                        if (nb) pair3<ISO>(lbx, lby, lbz, im, c0, c1, c2, bv, bax, bay, baz);
+                    }
+                }
+            }
+        }
+    }
+    A.v = av; A.gx = aax; A.gy = aay; A.gz = aaz;
+    B.v = bv; B.gx = bax; B.gy = bay; B.gz = baz;
+}
+
 // Two queries at once (a and, when `two`, b): when lists serve two queries and b lies within
-// their extra reach of a, one pass over a's list sums both; otherwise each is summed on its
-// own.  Queries with a pair at the support radius are summed again on the exact path.
+// their extra reach of a, one pass over a's list sums both; when b shares a's grid cell and
+// a has no list, one pass over the block's table or generated cells sums both; otherwise each
+// is summed on its own.  Queries with a pair at the support radius are summed again on the
+// exact path.
 template <int NI, bool ISO>
 __device__ __forceinline__ void noise3d_raw2(const FieldParams &P, const float pa[3], const float pb[3], bool two,
                                              uint32_t seed, float R, float c0, float c1, float c2, Eval4 &A,
@@ -473,11 +530,23 @@ __device__ __forceinline__ void noise3d_raw2(const FieldParams &P, const float p
     if (wave_any(pair)) {
         if (pair) list_sum2<ISO>(P, lc, ga, ra, gb, c0, c1, c2, A, B, near_a, near_b);
     }
+    // b in a's grid cell, a without a list: one pass over the cells of the block
+    unsigned lca = 0;
+    const bool listed = P.has_lists && P.has_table && seed == P.tseed &&
+                        list_cell(P, (int)fa[0], (int)fa[1], (int)fa[2], lca);
+    const bool cells2 = two && !pair && !listed && floorf(gb[0]) == fa[0] && floorf(gb[1]) == fa[1] &&
+                        floorf(gb[2]) == fa[2];
+    if (wave_any(cells2)) {
+        if (cells2) {
+            const float rb[3] = {gb[0] - fa[0], gb[1] - fa[1], gb[2] - fa[2]};
+            noise3d_cells2<NI, ISO>(P, (int)fa[0], (int)fa[1], (int)fa[2], ra, rb, seed, c0, c1, c2, A, B);
+        }
+    }
     // the rest one query at a time (one inlined copy of the single-query sum): queries off the
-    // pair path on their own lists or the table, near-radius queries on the exact path
+    // pair paths on their own lists or the table, near-radius queries on the exact path
 #pragma unroll 1
     for (int w = 0; w < 2; ++w) {
-        const bool need = w == 0 ? (!pair || near_a) : (two && (!pair || near_b));
+        const bool need = !cells2 && (w == 0 ? (!pair || near_a) : (two && (!pair || near_b)));
         if (!wave_any(need)) continue;
         if (need) {
             const Eval4 r = noise3d_raw<NI, ISO>(P, w ? pb[0] : pa[0], w ? pb[1] : pa[1], w ? pb[2] : pa[2], seed, R, c0,
@@ -717,6 +786,25 @@ __device__ __forceinline__ void from_iso_ray(const FieldParams &P, const Frame3 
     wx = P.l_aniso_inv[0] * x; wy = P.l_aniso_inv[1] * y; wz = P.l_aniso_inv[2] * z;
 }
 
+// One level's term of a multi-resolution evaluation (evaluateNoise3D :146-153): the level's raw
+// sums normalised, its conditioning splats added, weighted by w and added to acc (the first
+// level sets acc).
+__device__ __forceinline__ void ml_level(const FieldParams &P, const RealState &rs, float px, float py, float pz,
+                                         bool cond, float scale, float w, const Level &L, const Eval4 &n, bool first,
+                                         Eval4 &acc) {
+    Eval4 r = {n.v * L.nf_inv, n.gx * L.g0 * L.nf_inv, n.gy * L.g1 * L.nf_inv, n.gz * L.g2 * L.nf_inv};
+    if (cond && rs.cond) cond_splat(P, rs, px, py, pz, scale, 1.0f, r);
+    if (first) {
+        acc.v = w * r.v; acc.gx = w * r.gx; acc.gy = w * r.gy; acc.gz = w * r.gz;
+    } else {
+        acc.v = acc.v + w * r.v; acc.gx = acc.gx + w * r.gx; acc.gy = acc.gy + w * r.gy; acc.gz = acc.gz + w * r.gz;
+    }
+}
+// Level l (0: the lower, 1: the upper) of the ladder at lr.
+__device__ __forceinline__ Level ml_make(const FieldParams &P, const LevelRatio &lr, int l) {
+    return make_level(P, l ? lr.shi : lr.slo, 1.0f, (uint32_t)P.level_seed_add[lr.li + 32 + l]);
+}
+
 // Normalised noise value and world gradient (evaluateNoise3D :142-165): world space, or
 // with IRAY the isotropic ray space of the flight along (dx, dy, dz).
 template <int NI, bool ISO, bool IRAY = false>
@@ -780,20 +868,14 @@ __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealStat
     }
     // multi-resolution: two adjacent levels of the base-2.5 ladder (kernelScaleLevelRatio)
     const LevelRatio lr = level_ratio(ls_field<HasNoise<NI>::value>(P, px, py, pz));
-    const float slo = lr.slo, shi = lr.shi, wlo = lr.wlo, whi = lr.whi;
-    const int ilo = lr.li + 32, ihi = lr.li + 33;  // in [0, 64]: the table holds L = -32 .. 32
-    Level A = make_level(P, slo, 1.0f, (uint32_t)P.level_seed_add[ilo]);
-    Level B = make_level(P, shi, 1.0f, (uint32_t)P.level_seed_add[ihi]);
-    Eval4 na = noise3d_raw<NI, ISO>(P, px, py, pz, rs.seed + A.seed_add, A.R, A.c0, A.c1, A.c2);
-    Eval4 nb = noise3d_raw<NI, ISO>(P, px, py, pz, rs.seed + B.seed_add, B.R, B.c0, B.c1, B.c2);
-    Eval4 ra = {na.v * A.nf_inv, na.gx * A.g0 * A.nf_inv, na.gy * A.g1 * A.nf_inv, na.gz * A.g2 * A.nf_inv};
-    Eval4 rb = {nb.v * B.nf_inv, nb.gx * B.g0 * B.nf_inv, nb.gy * B.g1 * B.nf_inv, nb.gz * B.g2 * B.nf_inv};
-    if (cond && rs.cond) {
-        cond_splat(P, rs, px, py, pz, slo, 1.0f, ra);
-        cond_splat(P, rs, px, py, pz, shi, 1.0f, rb);
+    Eval4 acc;
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        const Level L = ml_make(P, lr, l);
+        const Eval4 n = noise3d_raw<NI, ISO>(P, px, py, pz, rs.seed + L.seed_add, L.R, L.c0, L.c1, L.c2);
+        ml_level(P, rs, px, py, pz, cond, l ? lr.shi : lr.slo, l ? lr.whi : lr.wlo, L, n, l == 0, acc);
     }
-    Eval4 r = {wlo * ra.v + whi * rb.v, wlo * ra.gx + whi * rb.gx, wlo * ra.gy + whi * rb.gy, wlo * ra.gz + whi * rb.gz};
-    return r;
+    return acc;
 }
 
 // sparseConvNoiseAmplitude(p): sigma, or getVariance(p) * sigma for a nonstationary

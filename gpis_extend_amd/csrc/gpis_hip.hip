@@ -313,6 +313,7 @@ static gpis_status build_params(gpis_ctx *c) {
         gpis_status s = build_aniso_mtx(c, l_conv);
         if (s != GPIS_OK) return s;
     }
+    P.pair_cells = c->lists_points == 2;
     P.mean_kind = d.mean_kind;
     for (int i = 0; i < 3; ++i) {
         P.mc[i] = (float)d.mean_center[i];
@@ -692,7 +693,11 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(cons
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, near_t = 0, far_t = 0, step = 0, t0 = 0;
     double tq_p = 0.0;  // the lane's next evaluation point: t along the ray (unit_p: unit direction)
     bool unit_p = false;
-    const bool pair_evals = P.lpair2 > 0.0f;
+    // a second evaluation point per pass where one pass can serve both: support lists, or two
+    // points in one grid cell of the table or the generated cells of a single-level SE field
+    // (the isotropic ray space, the full anisotropy matrix and the nonstationary fields
+    // evaluate each point on its own)
+    const bool pair_evals = !IRAY && (P.lpair2 > 0.0f || (P.pair_cells && P.kind == GPIS_COV_SE && !(NI < 0 && P.full)));
     unsigned lane_passes = 0, lane_paired = 0;  // evaluation passes of this lane, passes with two used
     float pf = 0.f, t_end = 0.f, val_end = 0.f;
     int first = 1, sign0 = 1, k = 0, hit = 0;
@@ -2673,6 +2678,7 @@ static gpis_status gpis_set_lists_impl(gpis_ctx *c, int32_t sub, int32_t points)
     if (points != 1 && points != 2) return fail(c, GPIS_INVALID_ARGUMENT, "points per list pass must be 1 or 2");
     c->lists_sub = sub;
     c->lists_points = points;
+    c->P.pair_cells = points == 2;
     return build_lists(c);
 }
 

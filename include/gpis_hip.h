@@ -301,7 +301,9 @@ gpis_status gpis_set_refill(gpis_ctx *ctx, int32_t batch);
  * in cell3D's visiting order (SparseConvolutionNoise.cpp:419-452); a lane sums only
  * its query's list, and with points = 2 the 3D flight kernels evaluate the next point the
  * march already determines (next lattice point, next secant candidate, the gradient at
- * farT) in the same pass.  Results are the same as without lists (same support decisions,
+ * farT) in the same pass -- over the list, or, without lists (per-flight realizations, the
+ * multi-resolution grid), over the generated or table cells of the block when both points
+ * lie in one grid cell.  Results are the same as without lists (same support decisions,
  * same order of the sums; replies and evaluation counts unchanged).  Rebuilds the lists of
  * an existing table (synchronous); later gpis_set_domain calls build them too.  No
  * reference counterpart (a data structure of the single-realization table). */
