@@ -233,6 +233,21 @@ __device__ __forceinline__ void pair_splat(float tx, float ty, float tz, int32_t
     az += we * tz;
 }
 
+// The isotropic splat of one pair given |t|^2 (r2) and t for the moments.
+__device__ __forceinline__ void pair_splat_r2(float r2, float tx, float ty, float tz, int32_t wbits, float s0,
+                                              float &v, float &ax, float &ay, float &az, uint64_t &near) {
+    const bool in = r2 < 0x1p46f;
+    near |= lane_mask(fabsf(r2 - 0x1p46f) < 0x1p46f * lists::kNearRel);
+    const float e = hw_exp2(s0 * r2);
+    union { int32_t i; float f; } w;
+    w.i = wbits;
+    const float we = in ? e * w.f : 0.0f;
+    v += we;
+    ax += we * tx;
+    ay += we * ty;
+    az += we * tz;
+}
+
 // One query-impulse pair of a support list, in 2^-23 grid units: t = query - impulse is
 // the exact integer difference rounded once to float; |t|^2 < 2^46 is the support test.
 // `near` collects pairs whose |t|^2 lies within a relative kNearRel of the radius (their
@@ -311,17 +326,30 @@ __device__ __forceinline__ void list_sum2(const FieldParams &P, unsigned lc, con
     B.v = B.gx = B.gy = B.gz = 0.0f;
     const ListSrc src = list_src(P);
     ListPipe L(src, first, ng, P.lent_n / 4u - 1u);
+    // isotropic kernels: b's |t|^2 from a's, |t + d|^2 = |t|^2 + |d|^2 + 2 t.d, and b's moments
+    // summed against a's t (sum w e (t + d) = sum w e t + d sum w e, the d term added once
+    // after the loop).  The rounding of this |t|^2 is far inside the near-radius band, so b's
+    // support decisions stay the exact path's.
+    const float d2x = 2.0f * dbx, d2y = 2.0f * dby, d2z = 2.0f * dbz;
+    const float dd = dbx * dbx + dby * dby + dbz * dbz;
     auto sum4 = [&](const Imp4 *q) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const float tx = (float)(int32_t)((uint32_t)Ax - (uint32_t)q[i].x);
             const float ty = (float)(int32_t)((uint32_t)Ay - (uint32_t)q[i].y);
             const float tz = (float)(int32_t)((uint32_t)Az - (uint32_t)q[i].z);
-            pair_splat<ISO>(tx, ty, tz, q[i].w, s0, s1, s2, A.v, A.gx, A.gy, A.gz, mask_a);
-            // b - impulse = (a - impulse) + (b - a): integers below 2^24 in magnitude for
-            // every impulse within reach, so the float sum is exact and equals the rounded
-            // integer difference (far sentinel impulses round, and stay out of support)
-            pair_splat<ISO>(tx + dbx, ty + dby, tz + dbz, q[i].w, s0, s1, s2, B.v, B.gx, B.gy, B.gz, mask_b);
+            if (ISO) {
+                const float r2a = tx * tx + ty * ty + tz * tz;
+                pair_splat_r2(r2a, tx, ty, tz, q[i].w, s0, A.v, A.gx, A.gy, A.gz, mask_a);
+                const float r2b = fmaf(tx, d2x, fmaf(ty, d2y, fmaf(tz, d2z, r2a + dd)));
+                pair_splat_r2(r2b, tx, ty, tz, q[i].w, s0, B.v, B.gx, B.gy, B.gz, mask_b);
+            } else {
+                pair_splat<ISO>(tx, ty, tz, q[i].w, s0, s1, s2, A.v, A.gx, A.gy, A.gz, mask_a);
+                // b - impulse = (a - impulse) + (b - a): integers below 2^24 in magnitude for
+                // every impulse within reach, so the float sum is exact and equals the rounded
+                // integer difference (far sentinel impulses round, and stay out of support)
+                pair_splat<ISO>(tx + dbx, ty + dby, tz + dbz, q[i].w, s0, s1, s2, B.v, B.gx, B.gy, B.gz, mask_b);
+            }
         }
     };
 #pragma unroll 1
@@ -333,6 +361,11 @@ __device__ __forceinline__ void list_sum2(const FieldParams &P, unsigned lc, con
     }
     near_a = own_bit(mask_a);
     near_b = own_bit(mask_b);
+    if (ISO) {
+        B.gx = B.gx + dbx * B.v;
+        B.gy = B.gy + dby * B.v;
+        B.gz = B.gz + dbz * B.v;
+    }
     A.gx *= 0x1p-23f; A.gy *= 0x1p-23f; A.gz *= 0x1p-23f;
     B.gx *= 0x1p-23f; B.gy *= 0x1p-23f; B.gz *= 0x1p-23f;
 }
