@@ -214,6 +214,7 @@ struct FieldParams {
     int full;
     float fc[6], fg[9], fk[6];
     double fhinv[9];
+    float w2l[9], l2w[9];  // W and L, row-major (the isotropic ray space's transforms)
     int seed_add_single;      // floor(log(kss)/log(2.5)) for the single-level grid
     int level_seed_add[65];   // the same for multi-resolution level L = -32 .. 32, indexed L + 32
     // single-level world-space constants (stationary SE)

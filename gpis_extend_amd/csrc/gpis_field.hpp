@@ -789,11 +789,26 @@ __device__ __forceinline__ LevelRatio level_ratio(float lat) {
     return k;
 }
 
+// worldToLocal v: diag(1 / l_aniso) v, or W v for the full anisotropy matrix (each component
+// left to right, as Eigen's 3x3 product).
+__device__ __forceinline__ void world_to_local(const FieldParams &P, float vx, float vy, float vz, float &ix, float &iy,
+                                               float &iz) {
+#pragma clang fp contract(off)
+    if (P.full) {
+        const float *W = P.w2l;
+        ix = dot3(W[0], W[1], W[2], vx, vy, vz);
+        iy = dot3(W[3], W[4], W[5], vx, vy, vz);
+        iz = dot3(W[6], W[7], W[8], vx, vy, vz);
+    } else {
+        ix = P.l_aniso_inv[0] * vx; iy = P.l_aniso_inv[1] * vy; iz = P.l_aniso_inv[2] * vz;
+    }
+}
 // The isotropic ray space of a flight along (dx, dy, dz): the frame of
 // normalize(worldToLocal d) (evaluateNoise3DIsotropicRayNormalized :349-352).
 __device__ __forceinline__ Frame3 iso_ray_frame(const FieldParams &P, float dx, float dy, float dz) {
 #pragma clang fp contract(off)
-    float ix = P.l_aniso_inv[0] * dx, iy = P.l_aniso_inv[1] * dy, iz = P.l_aniso_inv[2] * dz;
+    float ix, iy, iz;
+    world_to_local(P, dx, dy, dz, ix, iy, iz);
     normalize3(ix, iy, iz);
     return make_frame(ix, iy, iz);
 }
@@ -801,12 +816,14 @@ __device__ __forceinline__ Frame3 iso_ray_frame(const FieldParams &P, float dx, 
 __device__ __forceinline__ void to_iso_ray(const FieldParams &P, const Frame3 &F, float px, float py, float pz,
                                            float &rx, float &ry, float &rz) {
 #pragma clang fp contract(off)
-    const float ix = P.l_aniso_inv[0] * px, iy = P.l_aniso_inv[1] * py, iz = P.l_aniso_inv[2] * pz;
+    float ix, iy, iz;
+    world_to_local(P, px, py, pz, ix, iy, iz);
     rx = dot3(F.tx, F.ty, F.tz, ix, iy, iz);
     ry = dot3(F.bx, F.by, F.bz, ix, iy, iz);
     rz = dot3(F.nx, F.ny, F.nz, ix, iy, iz);
 }
-// A gradient from the isotropic ray space to world space: localToWorld^-T frame.toGlobal g.
+// A gradient from the isotropic ray space to world space: localToWorld^-T frame.toGlobal g
+// (worldToLocal^T g: the diagonal, or W^T for the full matrix).
 __device__ __forceinline__ void from_iso_ray(const FieldParams &P, const Frame3 &F, float gx, float gy, float gz,
                                              float &wx, float &wy, float &wz) {
 #pragma clang fp contract(off)
@@ -816,7 +833,14 @@ __device__ __forceinline__ void from_iso_ray(const FieldParams &P, const Frame3 
     y += F.ny * gz;
     float z = F.tz * gx + F.bz * gy;
     z += F.nz * gz;
-    wx = P.l_aniso_inv[0] * x; wy = P.l_aniso_inv[1] * y; wz = P.l_aniso_inv[2] * z;
+    if (P.full) {
+        const float *W = P.w2l;
+        wx = dot3(W[0], W[3], W[6], x, y, z);
+        wy = dot3(W[1], W[4], W[7], x, y, z);
+        wz = dot3(W[2], W[5], W[8], x, y, z);
+    } else {
+        wx = P.l_aniso_inv[0] * x; wy = P.l_aniso_inv[1] * y; wz = P.l_aniso_inv[2] * z;
+    }
 }
 
 // One level's term of a multi-resolution evaluation (evaluateNoise3D :146-153): the level's raw

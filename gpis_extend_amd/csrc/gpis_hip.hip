@@ -165,8 +165,7 @@ static gpis_status build_aniso_mtx(gpis_ctx *c, float l_conv) {
     FieldParams &P = c->P;
     if (d.cov_kind != GPIS_COV_SE)
         return fail(c, GPIS_UNSUPPORTED, "anisoMtx is served for the stationary SE kernel");
-    if (d.one_d || d.isotropic_3d)
-        return fail(c, GPIS_UNSUPPORTED, "anisoMtx is served for 3D sampling in world space");
+    if (d.one_d) return fail(c, GPIS_UNSUPPORTED, "anisoMtx is served for 3D sampling");
     float A[3][3], W[3][3], ci[3][3];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) A[i][j] = l_conv * d.aniso_mtx[3 * i + j];
@@ -215,6 +214,8 @@ static gpis_status build_aniso_mtx(gpis_ctx *c, float l_conv) {
                           H[1][0] * H[2][1] - H[1][1] * H[2][0], H[0][1] * H[2][0] - H[0][0] * H[2][1],
                           H[0][0] * H[1][1] - H[0][1] * H[1][0]};
     for (int q = 0; q < 9; ++q) P.fhinv[q] = hi[q] / hdet;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) { P.w2l[3 * i + j] = W[i][j]; P.l2w[3 * i + j] = A[i][j]; }
     P.iso = 0;
     P.full = 1;
     return GPIS_OK;
@@ -624,8 +625,14 @@ __device__ __noinline__ void condition_coefficients(const FieldParams &P, const 
         if (IRAY) {
             // the isotropic ray space (:539-552): -2 frame.toLocal(localToWorld^T delta)
             const Frame3 F = iso_ray_frame(P, dx, dy, dz);
-            const double l0 = (double)P.l_aniso[0] * del0, l1 = (double)P.l_aniso[1] * del1,
-                         l2 = (double)P.l_aniso[2] * del2;
+            double l0 = (double)P.l_aniso[0] * del0, l1 = (double)P.l_aniso[1] * del1,
+                   l2 = (double)P.l_aniso[2] * del2;
+            if (P.full) {
+                const float *L = P.l2w;
+                l0 = (double)L[0] * del0 + (double)L[3] * del1 + (double)L[6] * del2;
+                l1 = (double)L[1] * del0 + (double)L[4] * del1 + (double)L[7] * del2;
+                l2 = (double)L[2] * del0 + (double)L[5] * del1 + (double)L[8] * del2;
+            }
             b0 = -2.0 * ((double)F.tx * l0 + (double)F.ty * l1 + (double)F.tz * l2);
             b1 = -2.0 * ((double)F.bx * l0 + (double)F.by * l1 + (double)F.bz * l2);
             b2 = -2.0 * ((double)F.nx * l0 + (double)F.ny * l1 + (double)F.nz * l2);

@@ -314,9 +314,9 @@ class SparseConvolutionNoiseMedium:
                 raise MediumConfigError("useAnisoMtx needs \"anisoMtx\": an array of 9 numbers, row-major")
             m.use_aniso_mtx = True
             m.aniso_mtx = tuple(float(np.float32(x)) for x in mt)
-            if m.one_d or m.isotropic_3D_sampling:
-                raise MediumConfigError("useAnisoMtx is served for 3D sampling in world space "
-                                        "(not 1D_sampling or isotropic_3D_sampling)")
+            if m.one_d:
+                raise MediumConfigError("useAnisoMtx is served for 3D sampling (world space or the "
+                                        "isotropic ray space), not 1D_sampling")
         return m
 
     def to_desc(self) -> A.FieldDesc:

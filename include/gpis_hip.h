@@ -95,8 +95,8 @@ typedef struct gpis_field_desc {
     double ns_var_noise_params[8], ns_ls_noise_params[8];
     /* "useAnisoMtx" and "anisoMtx" of the SE covariance (GPFunctions.cpp:663-681): when
      * use_aniso_mtx is set, the local-to-world matrix is lengthScale / sqrt(2) * aniso_mtx
-     * (row-major) in place of the diagonal "aniso"; served in world space for the
-     * stationary SE kernel */
+     * (row-major) in place of the diagonal "aniso"; served for 3D sampling (world space
+     * and the isotropic ray space) with the stationary SE kernel */
     int32_t use_aniso_mtx;
     float aniso_mtx[9];
 } gpis_field_desc;

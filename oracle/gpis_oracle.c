@@ -473,17 +473,31 @@ static float cov_var1d(const orc_field *f, v3f p, float N, float R) {
 }
 /* transforms (:634-644; nonstationary :1267-1285) */
 static float ns_scale(const orc_field *f, float gs) { return (is_se(f) || f->multires) ? gs : cov_ls_max(f); }
+/* M v and M^T v for a row-major 3x3 float matrix, each component left to right */
+static v3f mat_mul(const float M[3][3], v3f v) {
+    return V3((M[0][0] * v.x + M[0][1] * v.y) + M[0][2] * v.z, (M[1][0] * v.x + M[1][1] * v.y) + M[1][2] * v.z,
+              (M[2][0] * v.x + M[2][1] * v.y) + M[2][2] * v.z);
+}
+static v3f mat_tmul(const float M[3][3], v3f v) {
+    return V3((M[0][0] * v.x + M[1][0] * v.y) + M[2][0] * v.z, (M[0][1] * v.x + M[1][1] * v.y) + M[2][1] * v.z,
+              (M[0][2] * v.x + M[1][2] * v.y) + M[2][2] * v.z);
+}
+/* transformPosDirWorldtoLocal: _world_to_local * v / s; transformGradLocaltoWorld:
+ * _world_to_local^T * g / s; transformGradWorldtoLocal: _local_to_world^T * g * s
+ * (GPFunctions.cpp:634-644); the diagonal matrices of "aniso" or the full "anisoMtx" */
 static v3f w2l(const orc_field *f, v3f v, float gs) {
     float s = ns_scale(f, gs);
+    if (f->full) return vdivs(mat_mul(f->w2l, v), s);
     return vdivs(vmul(f->l_aniso_inv, v), s);
 }
 static v3f grad_l2w(const orc_field *f, v3f g, float gs) {
     float s = ns_scale(f, gs);
+    if (f->full) return vdivs(mat_tmul(f->w2l, g), s);
     return vdivs(vmul(f->l_aniso_inv, g), s);
 }
 static v3f grad_w2l(const orc_field *f, v3f g, float gs) {
     float s = ns_scale(f, gs);
-    v3f r = vmul(f->l_aniso, g);
+    v3f r = f->full ? mat_tmul(f->l2w, g) : vmul(f->l_aniso, g);
     return V3(r.x * s, r.y * s, r.z * s);
 }
 

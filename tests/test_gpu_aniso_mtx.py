@@ -3,7 +3,7 @@ GPFunctions.cpp:663-681, 706-764, 784-843) on the HIP path against the oracle on
 seeded inputs: field values and gradients at fixed points with impulses from the generator
 and from the packed table, primary flights and flights conditioned under Renewal+ (the
 full-matrix covariance splats and the inverse of its second derivative) through the parity
-gate."""
+gate, in world space and in the isotropic ray space."""
 import math
 import os
 
@@ -114,3 +114,21 @@ def test_flights_full_matrix_renewal_plus(name):
     got, ref, orc = _flights(js, rays)
     parity_gate(got, ref, f"anisoMtx {name} Renewal+", min_agree=0.99, oracle=orc)
 
+
+
+@pytest.mark.parametrize("name", sorted(MATS))
+def test_ray_space_flights_full_matrix(name):
+    """The full matrix in the isotropic ray space (isotropic_3D_sampling): the frame of
+    normalize(W d), points at frame.toLocal(W p), gradients W^T frame.toGlobal g; primary
+    flights and Renewal+ flights from a surface (L^T delta into the ray space)."""
+    from gpis_extend_amd import SparseConvolutionNoiseMedium, workloads as W
+    print(f"case: isotropic ray space, anisoMtx {name}: 8,192 primary and 2,048 Renewal+ flights", flush=True)
+    js = _js(name, isotropic_3D_sampling=True)
+    rays = W.sphere_rays(SparseConvolutionNoiseMedium.from_json(js), n=8192, seed=44)
+    got, ref, orc = _flights(js, rays)
+    parity_gate(got, ref, f"ray space anisoMtx {name} primary", min_agree=0.999, oracle=orc)
+    js = _js(name, isotropic_3D_sampling=True, correlation_context="renewal+", single_realization=False)
+    rays = W.box_surface_rays(SparseConvolutionNoiseMedium.from_json(js), n=2048)
+    got, ref, orc = _flights(js, rays)
+    # surface-born flights start at f ~ 0: their first sign decisions are grazing by construction
+    parity_gate(got, ref, f"ray space anisoMtx {name} Renewal+", min_agree=0.99, oracle=orc)

@@ -172,11 +172,97 @@ def test_host_parses_and_refuses_aniso_mtx():
     # JsonPtr::get(Eigen::Matrix3f) takes an array of exactly 9 numbers
     with pytest.raises(MediumConfigError, match="anisoMtx"):
         SparseConvolutionNoiseMedium.from_json(js_with([1.0, 0.0, 0.0]))
-    with pytest.raises(MediumConfigError, match="world space"):
-        SparseConvolutionNoiseMedium.from_json(js_with(A.reshape(9).tolist(), isotropic_3D_sampling=True))
-    with pytest.raises(MediumConfigError, match="world space"):
+    iso = SparseConvolutionNoiseMedium.from_json(js_with(A.reshape(9).tolist(), isotropic_3D_sampling=True))
+    assert iso.use_aniso_mtx and iso.to_desc().isotropic_3d == 1
+    with pytest.raises(MediumConfigError, match="1D_sampling"):
         SparseConvolutionNoiseMedium.from_json(js_with(A.reshape(9).tolist(), **{"1D_sampling": True}))
     js = W.config2_json()
     js["gaussian_process"]["covariance"]["cov"].update({"useAnisoMtx": True, "anisoMtx": A.reshape(9).tolist()})
     with pytest.raises(MediumConfigError, match="proc_nonstationary"):
         SparseConvolutionNoiseMedium.from_json(js)
+
+
+# ---- the full matrix in the isotropic ray space (isotropic_3D_sampling) ----
+# The ray space of a flight along d is frame(normalize(W d)).toLocal(W p), W = L^-1: the frame
+# is a rotation, so for a fixed direction the field keeps the covariance of the world-space
+# field above, its gradient is W^T frame.toGlobal of the ray-space one, and conditioning takes
+# L^T delta into the ray space (GPFunctions.cpp:634-644, SparseConvolutionNoise.cpp:345-377,
+# 539-556).
+
+def _iso_field(A, **kw):
+    kw.setdefault("ctxt", po.CTX_GLOBAL)
+    kw.setdefault("single_realization", 1)
+    kw.setdefault("mean_kind", po.MEAN_PLANE)
+    kw.setdefault("plane_normal", (0.0, 0.0, 0.0))
+    d = po.make_desc(sigma=1.0, length_scale=L, impulse_density=8, global_seed=1, aniso_mtx=A, iso3d=1, **kw)
+    return po.Field(d)
+
+
+def test_ray_space_keeps_the_covariance_law():
+    A = MATS["sheared"].astype(np.float64)
+    Sinv = np.linalg.inv(L * L * A @ A.T)
+    F = _iso_field(MATS["sheared"])
+    d = np.array([0.3, -0.5, 0.8]); d /= np.linalg.norm(d)
+    rng = np.random.default_rng(31)
+    n = 20000
+    x = rng.uniform(-1, 1, (n, 3))
+    seeds = rng.integers(1, 2 ** 31, size=n)
+    lags = [0.7 * L * A[:, 0], np.array([0.0, 0.08, 0.0]), np.array([0.05, -0.05, 0.05])]
+    vals = np.zeros((len(lags) + 1, n))
+    for i in range(n):
+        pts = np.stack([x[i]] + [x[i] + lg for lg in lags]).astype(np.float32)
+        vals[:, i] = F.eval_points_dir(pts, int(seeds[i]), d)[:, 0]
+    assert abs(vals[0].var() - 1.0) < 6 * math.sqrt(2.0 / n) + 0.01
+    for k, lg in enumerate(lags, start=1):
+        rho = float(np.mean(vals[0] * vals[k]))
+        want = math.exp(-0.5 * float(lg @ Sinv @ lg))
+        se = math.sqrt((1 + want * want) / n)
+        assert abs(rho - want) < 6 * se + 0.02, (lg, rho, want)
+
+
+def test_ray_space_gradient_is_gradient_of_value():
+    F = _iso_field(MATS["rotated"])
+    rng = np.random.default_rng(32)
+    x = rng.uniform(-1, 1, (2000, 3))
+    d = np.array([0.6, 0.0, 0.8], np.float32)
+    base = F.eval_points_dir(x.astype(np.float32), 1, d, nthreads=8).astype(np.float64)
+    # ray-space coordinates are |W| ~ 14-28 times the world ones, so the float rounding of a
+    # query moves f more than in world space: a longer difference step averages it down
+    h = 1e-4
+    for ax in range(3):
+        e = np.zeros(3); e[ax] = h
+        xp, xm = (x + e).astype(np.float32), (x - e).astype(np.float32)
+        fp = F.eval_points_dir(xp, 1, d, nthreads=8)[:, 0].astype(np.float64)
+        fm = F.eval_points_dir(xm, 1, d, nthreads=8)[:, 0].astype(np.float64)
+        fd = (fp - fm) / (xp.astype(np.float64) - xm.astype(np.float64))[:, ax]
+        err = np.abs(fd - base[:, 1 + ax]) / max(1.0, float(np.median(np.abs(base[:, 1 + ax]))))
+        assert np.median(err) < 2e-3 and np.quantile(err, 0.9) < 1e-2
+
+
+@pytest.mark.parametrize("name", sorted(MATS))
+def test_ray_space_renewal_plus_reproduces_observation(name):
+    F = _iso_field(MATS[name], ctxt=po.CTX_RENEWAL_PLUS, single_realization=0, mean_kind=po.MEAN_SPHERE,
+                   mean_radius=0.5)
+    rng = np.random.default_rng(33)
+    for _ in range(20):
+        o = rng.uniform(-0.7, 0.7, 3).astype(np.float32)
+        dvec = rng.normal(size=3); dvec /= np.linalg.norm(dvec)
+        tv = float(rng.normal(0, 0.3))
+        tg = rng.normal(0, 3, 3)
+        out = F.conditioned_eval(_ray(o, dvec.astype(np.float32), tv, tg), o, t_segment=1.0)
+        assert abs(out[0] - tv) < 1e-4, out
+        assert np.max(np.abs(out[1:] - tg.astype(np.float32))) < 1e-3 * max(1, np.abs(tg).max()), (out, tg)
+
+
+def test_ray_space_diagonal_matrix_equals_diagonal_aniso():
+    a = (1.5, 0.6, 0.9)
+    Fm = _iso_field(np.diag(a).astype(np.float32))
+    d = po.make_desc(sigma=1.0, length_scale=L, impulse_density=8, global_seed=1, aniso=a, iso3d=1,
+                     ctxt=po.CTX_GLOBAL, single_realization=1, mean_kind=po.MEAN_PLANE, plane_normal=(0.0, 0.0, 0.0))
+    Fd = po.Field(d)
+    rng = np.random.default_rng(34)
+    x = rng.uniform(-1, 1, (4000, 3)).astype(np.float32)
+    dirn = np.array([0.48, -0.6, 0.64], np.float32)
+    a1, a2 = Fm.eval_points_dir(x, 1, dirn, nthreads=8), Fd.eval_points_dir(x, 1, dirn, nthreads=8)
+    close = np.all(np.abs(a1 - a2) <= 1e-4 * np.maximum(1, np.abs(a2)), axis=1)
+    assert close.mean() > 0.995, close.mean()
