@@ -428,7 +428,8 @@ def run_c2(args):
                    "launch": {"refill_batch": tuned[0], "waves_per_cu": tuned[1], "block": tuned[2]} if tuned else None},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "flight_kernel<0,true,COND,PATHS>", "kernel_ms": kernel_ms, "flops_per_eval": fpe},
+                     "kernel": f"flight_kernel<{int(med.impulse_density)},true,COND,PATHS>", "kernel_ms": kernel_ms,
+                     "flops_per_eval": fpe},
     }
     if not args.no_cpu_baseline:
         from tests.helpers import oracle_field, oracle_rays

@@ -32,7 +32,7 @@ PHASE_MIRROR, PHASE_LAMBERT = 0, 1
 EXPORTS = (
     "gpis_create", "gpis_destroy", "gpis_last_error", "gpis_set_domain", "gpis_impulse_table",
     "gpis_eval_points", "gpis_trace_flights", "gpis_trace_paths", "gpis_get_counters",
-    "gpis_reset_counters", "gpis_set_launch", "gpis_set_refill", "gpis_trace_flights_1d",
+    "gpis_reset_counters", "gpis_set_launch", "gpis_set_refill", "gpis_set_coop_cells", "gpis_trace_flights_1d",
     "gpis_set_mean_grid", "gpis_trace_paths_1d", "gpis_set_lists", "gpis_set_precision",
     # include/gpis_surrogate.h
     "gpis_net_create", "gpis_net_destroy", "gpis_net_last_error", "gpis_net_set_params", "gpis_net_get_params",
@@ -160,6 +160,8 @@ def lib():
     L.gpis_set_launch.argtypes = [_vp, C.c_int32, C.c_int32]
     L.gpis_set_refill.restype = st
     L.gpis_set_refill.argtypes = [_vp, C.c_int32]
+    L.gpis_set_coop_cells.restype = st
+    L.gpis_set_coop_cells.argtypes = [_vp, C.c_int32]
     L.gpis_set_lists.restype = st
     L.gpis_set_lists.argtypes = [_vp, C.c_int32, C.c_int32]
     L.gpis_set_precision.restype = st

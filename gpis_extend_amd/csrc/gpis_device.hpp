@@ -7,9 +7,10 @@
 #include <stdint.h>
 
 #ifdef GPIS_HOST_CHECK
-// Host build of the field code for the CPU checks under tests/ (one lane per wave): the
-// few device intrinsics the field evaluation uses, restated for one lane.
+// Host build of the field code for the CPU checks under tests/ (one lane per wave, or one
+// thread per lane): the few device intrinsics the field evaluation uses, restated.
 #include <math.h>
+#include <pthread.h>
 #define __host__
 #define __device__
 #define __global__
@@ -38,11 +39,32 @@ namespace gpis {
 #ifdef GPIS_HOST_CHECK
 static inline float hw_exp2(float x) { return exp2f(x); }
 #define GPIS_CHECK_INDEX(i, n) gpis_check_index((size_t)(i), (size_t)(n), __FILE__, __LINE__)
+// A wave of several lanes for the checks of the wave-cooperative code: one host thread per
+// lane, the lanes meeting at a barrier in every ballot and wave_sync (their code calls these
+// in the same sequence on every lane).  Without one (host_wave null) each thread is a wave
+// of one lane.
+struct HostWave {
+    pthread_barrier_t bar;
+    uint64_t vote;
+    unsigned first;  // a lane of the wave, which clears the vote
+};
+inline HostWave *host_wave = nullptr;
+inline thread_local unsigned host_lane = 0;
 static inline bool wave_any(bool p) { return p; }
-static inline uint64_t wave_ballot(bool p) { return p ? 1u : 0u; }
-static inline unsigned lane64() { return 0u; }
+static inline uint64_t wave_ballot(bool p) {
+    if (!host_wave) return p ? 1u : 0u;
+    pthread_barrier_wait(&host_wave->bar);
+    if (host_lane == host_wave->first) __atomic_store_n(&host_wave->vote, 0ull, __ATOMIC_SEQ_CST);
+    pthread_barrier_wait(&host_wave->bar);
+    if (p) __atomic_fetch_or(&host_wave->vote, 1ull << host_lane, __ATOMIC_SEQ_CST);
+    pthread_barrier_wait(&host_wave->bar);
+    return __atomic_load_n(&host_wave->vote, __ATOMIC_SEQ_CST);
+}
+static inline unsigned lane64() { return host_wave ? host_lane : 0u; }
 static inline int popc64(uint64_t m) { return __builtin_popcountll(m); }
-static inline void wave_sync() {}
+static inline void wave_sync() {
+    if (host_wave) pthread_barrier_wait(&host_wave->bar);
+}
 #else
 __device__ __forceinline__ float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
@@ -247,6 +269,7 @@ struct FieldParams {
     int lmargin;        // cells whose lists exist: those at least lmargin cells inside the table
     float lpair2;       // lists serve a second query within sqrt(lpair2) grid units (0: one query)
     int pair_cells;     // two points of a flight in one grid cell share a pass over its block's cells
+    int coop_cells;     // the flight kernels make generated cells wave-cooperatively (cells_coop)
 };
 
 struct Eval4 { float v, gx, gy, gz; };

@@ -24,6 +24,8 @@ namespace gpis {
 struct RealState {
     uint32_t seed;
     int cond;            // conditioning splats active (non-zero coefficients)
+    int coop;            // generated cells made by the wave together: 0 off, 1 on, 2 on and
+                         // this lane has no point of its own (it only makes cells for others)
     float ox, oy, oz;    // conditioning origin (coeff_3D.ray_origin)
     float a;             // coeff_3D.value_scale
     float bx, by, bz;    // coeff_3D.gradient_scale
@@ -111,6 +113,125 @@ __device__ __forceinline__ void cell_generated(int cx, int cy, int cz, int n, ui
         im.x = u01(g.next());
         im.w = u01(g.next()) < 0.5f ? -1.0f : 1.0f;
         pair3<ISO>(lx, ly, lz, im, c0, c1, c2, v, ax, ay, az);
+    }
+}
+
+// The generated cells of a wave's queries made by all of its lanes together.  Each lane's
+// cells within reach (the cull of noise3d_raw), in the cell order of the 3x3x3 loop, are its
+// jobs.  Jobs are packed column by column — the first job of every lane that has one, then
+// the second, ... — and a round takes as many whole columns as the wave has active lanes:
+// the active lanes make one job each (the impulses of one cell, from cell3D's stream) into
+// the wave's LDS, and then every lane splats its jobs of those columns in its order.  Each
+// lane's sum is therefore the sequential loop's to the bit.  A wave with many lanes holding
+// points runs the plain per-lane loop instead (its columns are full, so making the cells
+// apart costs only the LDS round trip); a wave with few (the end of a launch, where a few
+// long chains keep their waves alive) makes their cells in parallel, with the lanes that
+// have no point of their own helping.  LDS per wave: 64 job records, then the impulses
+// k-major (impulse k of slot s at 64 k + s), 64 cells of n impulses.
+struct CoopJob { int32_t x, y, z; uint32_t seed; };
+constexpr int kCoopMaxN = 12;     // impulses per cell at most: a block of 256 lanes then holds < 64 KB
+constexpr int kCoopMaxLive = 32;  // lanes with points above which the wave runs the per-lane loop
+// float4 slots of one wave's part of the launch's dynamic LDS
+__host__ __device__ constexpr int coop_wave_slots(int n) { return 64 + 64 * n; }
+#ifdef GPIS_HOST_CHECK
+inline float4 *host_coop_lds = nullptr;
+inline int host_coop_max_live = kCoopMaxLive;  // the checks run both choices
+static inline float4 *coop_wave_lds(int) { return host_coop_lds; }
+#define GPIS_COOP_MAX_LIVE host_coop_max_live
+#else
+extern __shared__ float4 gpis_coop_lds[];
+__device__ __forceinline__ float4 *coop_wave_lds(int n) {
+    return gpis_coop_lds + (threadIdx.x >> 6) * coop_wave_slots(n);
+}
+#define GPIS_COOP_MAX_LIVE kCoopMaxLive
+#endif
+
+template <int NI, bool ISO>
+__device__ __forceinline__ void cells_coop(float4 *lds, int ix, int iy, int iz, float rx, float ry, float rz,
+                                           int n, uint32_t seed, bool idle, float c0, float c1, float c2,
+                                           float &v, float &ax, float &ay, float &az) {
+    const int N = NI > 0 ? NI : n;
+    uint32_t m = 0;  // bit j: cell (j / 9 - 1, j / 3 % 3 - 1, j % 3 - 1) is within reach
+    if (!idle) {
+#pragma unroll
+        for (int j = 0; j < 27; ++j)
+            if (axis_bound2(rx, j / 9) + axis_bound2(ry, (j / 3) % 3) + axis_bound2(rz, j % 3) < kCullR2)
+                m |= 1u << j;
+    }
+    const int cnt = __builtin_popcount(m);
+    if (popc64(wave_ballot(cnt > 0)) > GPIS_COOP_MAX_LIVE) {
+        // the sequential loop, each lane on its own
+        uint32_t w = m;
+        while (w) {
+            const int j = __builtin_ctz(w);
+            w &= w - 1u;
+            const int dx = j / 9 - 1, dy = (j / 3) % 3 - 1, dz = j % 3 - 1;
+            cell_generated<NI, ISO>(ix + dx, iy + dy, iz + dz, N, seed, rx - (float)dx, ry - (float)dy,
+                                    rz - (float)dz, c0, c1, c2, v, ax, ay, az);
+        }
+        return;
+    }
+    const unsigned me = lane64();
+    const uint64_t below = me ? (~0ull >> (64u - me)) : 0ull;
+    const uint64_t act = wave_ballot(true);
+    const int A = popc64(act), rank = popc64(act & below);
+    CoopJob *jobs = reinterpret_cast<CoopJob *>(lds);
+    float4 *imps = lds + 64;
+    uint32_t wgen = m, wspl = m;  // this lane's next job to record, and to splat (lowest bit)
+    int i0 = 0;                   // the round's first column
+#pragma unroll 1
+    for (;;) {
+        // record the jobs of columns i0 .. i1 - 1, as many whole columns as fit in A slots
+        int total = 0, i1 = i0;
+#pragma unroll 1
+        for (; i1 < 27; ++i1) {
+            const uint64_t col = wave_ballot(cnt > i1);
+            const int ni = popc64(col);
+            if (ni == 0 || total + ni > A) break;
+            if (cnt > i1) {
+                const int j = __builtin_ctz(wgen);
+                wgen &= wgen - 1u;
+                const int slot = total + popc64(col & below);
+                GPIS_CHECK_INDEX(slot, A);
+                CoopJob d;
+                d.x = ix + j / 9 - 1; d.y = iy + (j / 3) % 3 - 1; d.z = iz + j % 3 - 1; d.seed = seed;
+                jobs[slot] = d;
+            }
+            total += ni;
+        }
+        if (total == 0) break;
+        wave_sync();
+        if (rank < total) {
+            const CoopJob d = jobs[rank];
+            Pcg g;
+            g.set((uint64_t)(uint32_t)(xxh4((uint32_t)d.z, (uint32_t)d.y, (uint32_t)d.x, d.seed) + 1u));
+            for (int k = 0; k < N; ++k) {
+                GPIS_CHECK_INDEX(64 * k + rank, 64 * N);
+                float4 im;
+                im.z = u01(g.next());
+                im.y = u01(g.next());
+                im.x = u01(g.next());
+                im.w = u01(g.next()) < 0.5f ? -1.0f : 1.0f;
+                imps[64 * k + rank] = im;
+            }
+        }
+        wave_sync();
+        int cb = 0;
+#pragma unroll 1
+        for (int i = i0; i < i1; ++i) {
+            const uint64_t col = wave_ballot(cnt > i);
+            if (cnt > i) {
+                const int j = __builtin_ctz(wspl);
+                wspl &= wspl - 1u;
+                const int slot = cb + popc64(col & below);
+                const float lx = rx - (float)(j / 9 - 1), ly = ry - (float)((j / 3) % 3 - 1),
+                            lz = rz - (float)(j % 3 - 1);
+                for (int k = 0; k < N; ++k) pair3<ISO>(lx, ly, lz, imps[64 * k + slot], c0, c1, c2, v, ax, ay, az);
+            }
+            cb += popc64(col);
+        }
+        wave_sync();  // the next round's records and cells overwrite these
+        i0 = i1;
     }
 }
 
@@ -377,14 +498,14 @@ __device__ __forceinline__ void list_sum2(const FieldParams &P, unsigned lc, con
 // than one per cell; otherwise each cell comes from the table or the generator.
 template <int NI, bool ISO>
 __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, float py, float pz, uint32_t seed, float R,
-                                             float c0, float c1, float c2, bool use_lists = true) {
+                                             float c0, float c1, float c2, bool use_lists = true, int coop = 0) {
     float gx = px / R, gy = py / R, gz = pz / R;
     float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
     float rx = gx - fx, ry = gy - fy, rz = gz - fz;
     int ix = (int)fx, iy = (int)fy, iz = (int)fz;
     float v = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
     const int N = NI > 0 ? NI : P.N;
-    bool tab = P.has_table && seed == P.tseed;
+    bool tab = P.has_table && seed == P.tseed && coop != 2;  // a helper lane (coop 2) has no point
     // the 3x3x3 block's low cell relative to the table; signed, so that a query below the
     // table's low corner fails the test instead of wrapping around to a huge offset
     const int sbx = ix - 1 - P.tlo[0], sby = iy - 1 - P.tlo[1], sbz = iz - 1 - P.tlo[2];
@@ -458,6 +579,11 @@ __device__ __forceinline__ Eval4 noise3d_raw(const FieldParams &P, float px, flo
             }
         }
 #endif
+        Eval4 r = {v, ax, ay, az};
+        return r;
+    }
+    if (coop && !tab) {
+        cells_coop<NI, ISO>(coop_wave_lds(N), ix, iy, iz, rx, ry, rz, N, seed, coop == 2, c0, c1, c2, v, ax, ay, az);
         Eval4 r = {v, ax, ay, az};
         return r;
     }
@@ -915,7 +1041,7 @@ __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealStat
             L = make_level(P, gs, ls, (uint32_t)P.seed_add_single);
             L.R = P.kernel_scale * gs * P.l_max;
         }
-        Eval4 n = noise3d_raw<NI, ISO>(P, px, py, pz, rs.seed + L.seed_add, L.R, L.c0, L.c1, L.c2);
+        Eval4 n = noise3d_raw<NI, ISO>(P, px, py, pz, rs.seed + L.seed_add, L.R, L.c0, L.c1, L.c2, true, rs.coop);
         r.v = n.v * L.nf_inv;
         r.gx = n.gx * L.g0 * L.nf_inv;
         r.gy = n.gy * L.g1 * L.nf_inv;
@@ -929,7 +1055,7 @@ __device__ __forceinline__ Eval4 eval_noise(const FieldParams &P, const RealStat
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         const Level L = ml_make(P, lr, l);
-        const Eval4 n = noise3d_raw<NI, ISO>(P, px, py, pz, rs.seed + L.seed_add, L.R, L.c0, L.c1, L.c2);
+        const Eval4 n = noise3d_raw<NI, ISO>(P, px, py, pz, rs.seed + L.seed_add, L.R, L.c0, L.c1, L.c2, true, rs.coop);
         ml_level(P, rs, px, py, pz, cond, l ? lr.shi : lr.slo, l ? lr.whi : lr.wlo, L, n, l == 0, acc);
     }
     return acc;

@@ -58,6 +58,15 @@ static void free_lists(gpis_ctx *c);
 static gpis_status build_lists(gpis_ctx *c);
 static gpis_status sync_params(gpis_ctx *c);
 
+// Media whose conditioned flights (Renewal, Renewal+) evaluate generated cells (per-flight
+// realizations, never the impulse table) through noise3d_raw one point at a time: the
+// nonstationary fields, single-level and multi-resolution.  Their flight kernels (the COND
+// instantiations) make the cells wave-cooperatively (cells_coop).
+static int coop_eligible(const FieldParams &P) {
+    return (P.multires || P.kind == GPIS_COV_PROC_NS) && P.activate && !P.single && !P.iso3d && !P.full &&
+           P.N >= 1 && P.N <= kCoopMaxN;
+}
+
 static gpis_status fail(gpis_ctx *c, gpis_status s, const char *msg) {
     if (c) c->err = msg;
     return s;
@@ -315,6 +324,7 @@ static gpis_status build_params(gpis_ctx *c) {
         if (s != GPIS_OK) return s;
     }
     P.pair_cells = c->lists_points == 2;
+    P.coop_cells = coop_eligible(P);
     P.mean_kind = d.mean_kind;
     for (int i = 0; i < 3; ++i) {
         P.mc[i] = (float)d.mean_center[i];
@@ -705,6 +715,7 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(cons
     // (the isotropic ray space, the full anisotropy matrix and the nonstationary fields
     // evaluate each point on its own)
     const bool pair_evals = !IRAY && (P.lpair2 > 0.0f || (P.pair_cells && P.kind == GPIS_COV_SE && !(NI < 0 && P.full)));
+    const bool coop = COND && !IRAY && P.coop_cells;  // the launch gave each wave its cells_coop LDS
     unsigned lane_passes = 0, lane_paired = 0;  // evaluation passes of this lane, passes with two used
     float pf = 0.f, t_end = 0.f, val_end = 0.f;
     int first = 1, sign0 = 1, k = 0, hit = 0;
@@ -868,7 +879,10 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(cons
             }
         }
         if (!__ballot(phase != PH_DEAD)) break;
-        if (phase == PH_NEED || phase == PH_DEAD || phase == PH_DONE) continue;
+        // a lane with no flight to evaluate skips the evaluation, or, where the wave makes
+        // its generated cells together, joins it to make cells for the other lanes
+        const bool idle = phase == PH_NEED || phase == PH_DEAD || phase == PH_DONE;
+        if (idle && !coop) continue;
 
         // ---- up to two field evaluations per live lane: the point its flight needs next
         // (tq_p along the ray, along the normalised direction when unit_p; the origin in the
@@ -894,11 +908,14 @@ __global__ __launch_bounds__(256, GPIS_FLIGHT_MIN_WAVES) void flight_kernel(cons
                 spec_t = (double)far_t; spec_phase = PH_GRAD; spec_unit = true;
             }
         }
-        const bool two = spec_phase != PH_DEAD;
+        const bool two = spec_phase != PH_DEAD && !idle;
         if (two) point_at(ox, oy, oz, dx, dy, dz, spec_t, spec_unit, pb[0], pb[1], pb[2]);
+        if (idle) pa[0] = pa[1] = pa[2] = 0.f;
+        rs.coop = coop ? (idle ? 2 : 1) : 0;
         Eval4 ea, eb;
         eval_field2<NI, ISO, GRID ? MEAN_EXT : MEAN_ANALYTIC, IRAY>(P, rs, pa, pb, two, !COND || phase != PH_COND, dx,
                                                                      dy, dz, ea, eb);
+        if (idle) continue;
         // the next query point: t along the ray (unit: along the normalised direction)
         double tq = 0.0;
         bool unit = false;
@@ -2282,16 +2299,18 @@ template <bool PATHS>
 static void launch_flights(gpis_ctx *c, unsigned blocks, unsigned threads, hipStream_t st, RayIn in, ReplyOut out,
                            PathOut pout, PathSeeds3D pseeds, unsigned n, int max_bounces, uint32_t bounce_seed,
                            float box) {
+    // the wave-cooperative cells' LDS: one part per wave of the block (cells_coop)
+    const size_t coop_lds = c->P.coop_cells ? (size_t)((threads + 63) / 64) * coop_wave_slots(c->P.N) * sizeof(float4) : 0;
 #define GPIS_FLIGHT_LAUNCH(NI, ISO, COND)                                                                          \
     do {                                                                                                           \
         if (grid)                                                                                                  \
-            hipLaunchKernelGGL((flight_kernel<NI, ISO, COND, PATHS, true>), dim3(blocks), dim3(threads), 0, st,    \
-                               c->P_dev, in, out, pout, pseeds, n, c->work, c->counters, max_bounces, bounce_seed, box,        \
-                               c->batch_k);                                                                        \
+            hipLaunchKernelGGL((flight_kernel<NI, ISO, COND, PATHS, true>), dim3(blocks), dim3(threads), coop_lds, \
+                               st, c->P_dev, in, out, pout, pseeds, n, c->work, c->counters, max_bounces,          \
+                               bounce_seed, box, c->batch_k);                                                      \
         else                                                                                                       \
-            hipLaunchKernelGGL((flight_kernel<NI, ISO, COND, PATHS, false>), dim3(blocks), dim3(threads), 0, st,   \
-                               c->P_dev, in, out, pout, pseeds, n, c->work, c->counters, max_bounces, bounce_seed, box,        \
-                               c->batch_k);                                                                        \
+            hipLaunchKernelGGL((flight_kernel<NI, ISO, COND, PATHS, false>), dim3(blocks), dim3(threads),          \
+                               coop_lds, st, c->P_dev, in, out, pout, pseeds, n, c->work, c->counters,             \
+                               max_bounces, bounce_seed, box, c->batch_k);                                         \
     } while (0)
     const bool cond = c->P.activate != 0;
     const bool grid = c->P.mean_kind == GPIS_MEAN_GRID || c->P.mean_kind == GPIS_MEAN_PROCEDURAL;
@@ -2702,6 +2721,13 @@ gpis_status gpis_set_precision(gpis_ctx *c, int32_t mode) {
         return fail(c, GPIS_INVALID_ARGUMENT, "precision mode must be GPIS_PRECISION_FAST or GPIS_PRECISION_EXACT");
     c->fast1d = mode == GPIS_PRECISION_FAST;
     return GPIS_OK;
+}
+
+gpis_status gpis_set_coop_cells(gpis_ctx *c, int32_t on) {
+    if (!c) return GPIS_INVALID_ARGUMENT;
+    if (on != 0 && on != 1) return fail(c, GPIS_INVALID_ARGUMENT, "coop_cells must be 0 or 1");
+    c->P.coop_cells = on ? coop_eligible(c->P) : 0;
+    return sync_params(c);
 }
 
 gpis_status gpis_set_refill(gpis_ctx *c, int32_t batch) {

@@ -474,8 +474,13 @@ class GpisDevice:
         """Lanes per wave served together by the finish/bounce/refill step (1..64)."""
         A.check(self.L.gpis_set_refill(self.h, batch), self.h, "gpis_set_refill")
 
+    def set_coop_cells(self, on: bool):
+        """Wave-cooperative generated cells in the 3D flight kernels (on by default where the
+        medium generates its cells per flight); results are the same either way."""
+        A.check(self.L.gpis_set_coop_cells(self.h, 1 if on else 0), self.h, "gpis_set_coop_cells")
+
     def set_lists(self, sub: int, points: int = 2):
-        """Support lists of the impulse table: sub-cells per cell edge (1..6, 0 for none) and
+        """Support lists of the impulse table: sub-cells per cell edge (1..8, 0 for none) and
         evaluation points per list pass (1 or 2)."""
         A.check(self.L.gpis_set_lists(self.h, sub, points), self.h, "gpis_set_lists")
 

@@ -292,6 +292,14 @@ gpis_status gpis_set_launch(gpis_ctx *ctx, int32_t waves_per_cu, int32_t block_s
  * rays submitted in spatially coherent groups of 64 (8x8 pixel tiles) march together.
  * No reference counterpart (a scheduling knob; results do not depend on it). */
 gpis_status gpis_set_refill(gpis_ctx *ctx, int32_t batch);
+/* Wave-cooperative generated cells in the 3D flight kernels (1 = on, the default where it
+ * applies; 0 = off): for media whose conditioned flights generate their cells (Renewal
+ * and Renewal+ over the nonstationary fields, single-level or multi-resolution, at most 12
+ * impulses per cell), the lanes of a wave make the cells within reach of its points
+ * together in LDS, and lanes without a point help; each point's sum is the same to the bit.
+ * Other media ignore it.  No reference counterpart (a scheduling knob; results do not
+ * depend on it). */
+gpis_status gpis_set_coop_cells(gpis_ctx *ctx, int32_t on);
 
 /* Support lists of the impulse table (the autotuner's knobs: "sub-cells per cell edge",
  * 1..8, default 3, 0 = none; "points per list pass", 1 or 2, default 2).  For every cell of

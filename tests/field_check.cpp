@@ -15,7 +15,10 @@
 //   4. the multi-resolution level index, level scales and weights of every float lateral
 //      scale in [2^-12, 2^12] are those of the float logf/powf formula;
 //   5. the 1D kernel's cells in LDS, made by PCG jump-ahead, equal the sequential
-//      generator's after every kind of move along the ray.
+//      generator's after every kind of move along the ray;
+//   6. the 3D cells made wave-cooperatively (one host thread per lane, 1..64 active lanes,
+//      idle lanes among them; the shared rounds, and the per-lane loop of a wave with many
+//      lanes holding points) give every lane the sequential loop's sum to the bit.
 // Queries: uniform in the table's interior, near the origin planes (|p / R| < 1, where the
 // fixed-point query is rounded), on cell and sub-cell faces, at distance 1 +- 2^-22 grid
 // units from an impulse (the support radius), and up to four cells below the table's low
@@ -25,7 +28,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <random>
+#include <thread>
 #include <vector>
 
 struct float4 { float x, y, z, w; };
@@ -417,9 +422,75 @@ int main(int argc, char **argv) {
         }
     }
     printf("cells1d_mismatch %lld of %lld\n", cells1d_bad, cells1d_n);
+    // 6. wave-cooperative generated cells (cells_coop): one thread per lane of a wave of 1..64
+    //    active lanes (a random subset of the 64 lane ids), some of them idle; every lane's
+    //    sum equals the sequential loop's to the bit, an idle lane's is zero
+    long long coop_bad = 0, coop_n = 0;
+    {
+        std::mt19937 rc(11);
+        std::vector<float4> lds(coop_wave_slots(kCoopMaxN));
+        for (int trial = 0; trial < 180; ++trial) {
+            const int pick = trial % 6;
+            const int nact = pick == 0 ? 64 : pick == 1 ? 1 : pick == 2 ? 2 : 1 + (int)(rc() % 64);
+            const int n_imp = trial % 2 ? 8 : 10;  // NI = 8, and the generic count with 10
+            const float idle_p = (float)(trial % 4) / 4.0f;
+            std::vector<unsigned> ids(64);
+            for (unsigned i = 0; i < 64; ++i) ids[i] = i;
+            std::shuffle(ids.begin(), ids.end(), rc);
+            ids.resize(nact);
+            std::sort(ids.begin(), ids.end());
+            struct LaneIn { float p[3], R; uint32_t seed; bool idle; Eval4 got, ref; };
+            std::vector<LaneIn> in(nact);
+            std::uniform_real_distribution<float> u(-1.0f, 1.0f), ur(0.03f, 0.2f), u01f(0.0f, 1.0f);
+            for (auto &L : in) {
+                for (float &c : L.p) c = u(rc);
+                L.R = ur(rc);
+                L.seed = (uint32_t)rc();
+                L.idle = u01f(rc) < idle_p;
+            }
+            FieldParams Pc = params(nullptr, false, 0.1f, 1u, n_imp);
+            host_coop_max_live = trial % 3 == 0 ? kCoopMaxLive : 64;  // the per-lane loop, or always shared
+            HostWave W;
+            pthread_barrier_init(&W.bar, nullptr, (unsigned)nact);
+            W.vote = 0;
+            W.first = ids[0];
+            host_wave = &W;
+            host_coop_lds = lds.data();
+            std::vector<std::thread> th;
+            for (int l = 0; l < nact; ++l) {
+                th.emplace_back([&, l] {
+                    host_lane = ids[l];
+                    LaneIn &L = in[l];
+                    const float c = -1.4426950408889634f * 2.25f;
+                    if (n_imp == 8)
+                        L.got = noise3d_raw<8, true>(Pc, L.p[0], L.p[1], L.p[2], L.seed, L.R, c, c, c, true, L.idle ? 2 : 1);
+                    else
+                        L.got = noise3d_raw<0, true>(Pc, L.p[0], L.p[1], L.p[2], L.seed, L.R, c, c, c, true, L.idle ? 2 : 1);
+                });
+            }
+            for (auto &t : th) t.join();
+            host_wave = nullptr;
+            host_coop_lds = nullptr;
+            pthread_barrier_destroy(&W.bar);
+            for (auto &L : in) {
+                const float c = -1.4426950408889634f * 2.25f;
+                const Eval4 zero = {0.f, 0.f, 0.f, 0.f};
+                if (L.idle) L.ref = zero;
+                else if (n_imp == 8) L.ref = noise3d_raw<8, true>(Pc, L.p[0], L.p[1], L.p[2], L.seed, L.R, c, c, c);
+                else L.ref = noise3d_raw<0, true>(Pc, L.p[0], L.p[1], L.p[2], L.seed, L.R, c, c, c);
+                if (!same_bits(L.got, L.ref)) {
+                    if (coop_bad < 4)
+                        printf("coop: trial %d lanes %d lane value %.9g ref %.9g\n", trial, nact, L.got.v, L.ref.v);
+                    ++coop_bad;
+                }
+                ++coop_n;
+            }
+        }
+    }
+    printf("coop_mismatch %lld of %lld\n", coop_bad, coop_n);
     const bool ok = level_bad == 0 && weight_bad == 0 && table_mismatch == 0 && support_mismatch == 0 && pair_mismatch == 0 && pair_evals > 0 && value_bad == 0 && list_missing == 0 &&
                     list_extra == 0 && near * 100 < evals && near_radius * 10 > evals_radius * 9 && cells1d_bad == 0 &&
-                    cells1d_n > 0;
+                    cells1d_n > 0 && coop_bad == 0 && coop_n > 0;
     printf("%s\n", ok ? "OK" : "FAIL");
     return ok ? 0 : 1;
 }

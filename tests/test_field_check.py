@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
 def test_field_tables_and_support_lists_on_the_host(tmp_path):
     exe = tmp_path / "field_check"
-    subprocess.run(["g++", "-O2", "-std=c++17", "-Wno-unknown-pragmas", "-ffp-contract=off",
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", "-Wno-unknown-pragmas", "-ffp-contract=off",
                     os.path.join(HERE, "field_check.cpp"), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe), "2000", "101"], capture_output=True, text=True)
     print(out.stdout)
@@ -26,4 +26,5 @@ def test_field_tables_and_support_lists_on_the_host(tmp_path):
     assert "list_missing 0 list_extra 0" in out.stdout
     assert "level_mismatch 0 weight_mismatch 0" in out.stdout
     assert "cells1d_mismatch 0 of" in out.stdout
+    assert "coop_mismatch 0 of" in out.stdout
     assert out.stdout.strip().endswith("OK")

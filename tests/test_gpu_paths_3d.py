@@ -172,3 +172,40 @@ def test_config2_chain_generation_by_generation():
     assert total > 16384
     # about 1e-3 of the surface-born flights cross within the first lattice step at |f| ~ 1e-4
     assert differ <= 2e-3 * total
+
+
+def _replies(med, rays, coop, paths):
+    from gpis_extend_amd import GpisDevice, RayBatch
+    dev = GpisDevice(med)
+    dev.set_coop_cells(coop)
+    rb = RayBatch.from_numpy(rays)
+    if paths:
+        rep = dev.trace_paths(rb, 15, BOUNCE_SEED, 1.0, pss=rays["pss"], scene_seed=int(rays["scene_seed"]))
+    else:
+        rep = dev.trace_flights(rb)
+    torch.cuda.synchronize()
+    out = {k: v.cpu().numpy() for k, v in rep.items() if not k.startswith("_")}
+    dev.close()
+    return out
+
+
+@pytest.mark.parametrize("multires", [True, False])
+def test_coop_cells_bit_identical(multires):
+    """Cells made by the whole wave (gpis_set_coop_cells, on by default for these media)
+    against each lane making its own: every reply of the C2 chains (multi-resolution) and of
+    brute-force Renewal+ flights is the same to the bit."""
+    from gpis_extend_amd import SparseConvolutionNoiseMedium, workloads as W
+    js = W.config2_json()
+    js["gaussian_process"]["covariance"]["multiResolutionGrid"] = multires
+    med = SparseConvolutionNoiseMedium.from_json(js)
+    rays = W.box_surface_rays(med, n=16384)
+    for paths in (True, False):
+        print(f"case: coop cells on/off, multires={multires}, {'chains' if paths else 'flights'}, 16,384 rays",
+              flush=True)
+        a = _replies(med, rays, True, paths)
+        b = _replies(med, rays, False, paths)
+        assert a.keys() == b.keys() and len(a) > 3
+        for k in a:
+            assert a[k].tobytes() == b[k].tobytes(), f"{k} differs ({(a[k] != b[k]).sum()} entries)"
+        n = a["flights"].sum() if paths else len(a["t"])
+        print(f"   {len(a)} arrays identical; {n} flights")
