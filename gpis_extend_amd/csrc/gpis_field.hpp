@@ -130,7 +130,11 @@ __device__ __forceinline__ void cell_generated(int cx, int cy, int cz, int n, ui
 // k-major (impulse k of slot s at 64 k + s), 64 cells of n impulses.
 struct CoopJob { int32_t x, y, z; uint32_t seed; };
 constexpr int kCoopMaxN = 12;     // impulses per cell at most: a block of 256 lanes then holds < 64 KB
-constexpr int kCoopMaxLive = 32;  // lanes with points above which the wave runs the per-lane loop
+#ifndef GPIS_COOP_MAX_LIVE_LANES
+#define GPIS_COOP_MAX_LIVE_LANES 32
+#endif
+// lanes with points above which the wave runs the per-lane loop
+constexpr int kCoopMaxLive = GPIS_COOP_MAX_LIVE_LANES;
 // float4 slots of one wave's part of the launch's dynamic LDS
 __host__ __device__ constexpr int coop_wave_slots(int n) { return 64 + 64 * n; }
 #ifdef GPIS_HOST_CHECK

@@ -324,7 +324,6 @@ static gpis_status build_params(gpis_ctx *c) {
         if (s != GPIS_OK) return s;
     }
     P.pair_cells = c->lists_points == 2;
-    P.coop_cells = coop_eligible(P);
     P.mean_kind = d.mean_kind;
     for (int i = 0; i < 3; ++i) {
         P.mc[i] = (float)d.mean_center[i];
@@ -2367,6 +2366,7 @@ gpis_status gpis_create(const gpis_field_desc *desc, gpis_ctx **out) {
         if (e == hipSuccess) e = hipMemcpy(c->sdf_dev, &c->sdf_host, sizeof(sdf::MeanParams), hipMemcpyHostToDevice);
         c->P.sdf = c->sdf_dev;
     }
+    c->P.coop_cells = coop_eligible(c->P);  // the medium's parameters are complete here
     if (e == hipSuccess) e = hipMalloc((void **)&c->P_dev, sizeof(FieldParams));
     if (e == hipSuccess) e = hipMemcpy(c->P_dev, &c->P, sizeof(FieldParams), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
